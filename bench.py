@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Throughput of the coupled D3Q19 LBM + IBM + Skalak membrane timestep on one
+MI355X (MLUPS), with the lattice kernel's achieved HBM bandwidth against the
+8 TB/s peak and the FP64 host oracle timed beside it.
+
+Default workload: 256 x 128 x 128 slab channel (half-way bounce-back walls in
+y, periodic x and z, body force 1e-6 along x), 1024 Evans-Fung cells with a
+seeded mesh mix of 50% 642-, 40% 2562- and 10% 10242-vertex meshes, radius
+7 lattice units on a jittered 16x8x8 grid, Skalak C=100, Gs=5e-4, FP32 with
+populations stored as deviations, whole steps replayed from a hipGraph.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|f64|f16]
+                    [--workload suspension|tgv] [--cpu-seconds S] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X
+Q = 19
+STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2}
+ARITH_BYTES = {"f64": 8, "f32": 4, "f16": 4}
+
+
+def suspension_workload(nc=1024, lx=256, ly=128, lz=128, radius=7.0, seed=2024):
+    from preprocess_src_amd import mesh
+    levels = mesh.mesh_mix(nc, [0.5, 0.4, 0.1], [3, 4, 5], seed)
+    centres, rot = mesh.place(nc, radius, lx, ly, lz, seed, mode="grid", grid=(16, 8, 8))
+    cells = mesh.assemble(levels, centres, rot, radius)
+    gs = 5e-4
+    params = dict(tau=1.0, p=0.5e-6, e=(1.0, 0.0, 0.0), b=gs / 2.0, m=1, c=100.0, runge_kutta=4,
+                  residual=1e-10, itrmax=1000, volume_correction=True)
+    ht = float(mesh.cell_volume(cells).sum() / (lx * ly * lz))
+    desc = (f"batched suspension {lx}x{ly}x{lz} channel, {nc} cells "
+            f"({int((levels == 3).sum())}x642 + {int((levels == 4).sum())}x2562 + {int((levels == 5).sum())}x10242 "
+            f"vertices), r={radius:g} lu, Ht={100 * ht:.1f}%")
+    return (lx, ly, lz), cells, params, desc, ht
+
+
+def tgv_workload(n=256):
+    return (n, n, n), None, dict(tau=0.6, periodic_y=True), f"Taylor-Green vortex {n}^3 fully periodic", 0.0
+
+
+def tgv_init(shape, u0=0.01):
+    lx, ly, lz = shape
+    k = 2 * np.pi / lx
+    xp = np.arange(lx + 4) - 1.5 - lx / 2
+    yp = np.arange(ly + 4) - 1.5 - ly / 2
+    X, Y = np.meshgrid(xp, yp, indexing="xy")
+    u2 = u0 * np.cos(k * X) * np.sin(k * Y)
+    v2 = -u0 * np.sin(k * X) * np.cos(k * Y)
+    r2 = 1.0 - 0.75 * u0 * u0 * (np.cos(2 * k * X) + np.cos(2 * k * Y))
+    rep = lambda a: np.broadcast_to(a, (lz + 4,) + a.shape).reshape(-1)
+    return rep(r2), rep(u2), rep(v2)
+
+
+def lattice_bytes_per_node(precision, coupled):
+    s = STORE_BYTES[precision]
+    t = ARITH_BYTES[precision]
+    return 2 * Q * s + (3 * t * 2 if coupled else 3 * t)  # populations + (F read, u write) or u write
+
+
+def cpu_baseline(shape, cells, params, seconds):
+    """FP64 host oracle on the same workload for a bounded number of steps."""
+    from oracle import pyoracle
+    lx, ly, lz = shape
+    kw = {k: v for k, v in params.items() if k not in ("store_macros",)}
+    t0 = time.time()
+    o = pyoracle.OracleSim(lx, ly, lz, cells=cells, **kw)
+    o.init_fluid()
+    if cells is not None:
+        o.volume_init()
+    setup = time.time() - t0
+    steps, t_run = 0, 0.0
+    while t_run < seconds or steps == 0:
+        t1 = time.time()
+        rc, _ = o.step(steps + 1)
+        t_run += time.time() - t1
+        steps += 1
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed ({rc})")
+    n = lx * ly * lz
+    return dict(value=n * steps / t_run / 1e6, unit="MLUPS", cores=int(pyoracle.lib().orc_num_threads()),
+                kind="port",
+                sample=f"{steps} full step(s) of the same workload in the FP64 OpenMP oracle "
+                       f"({t_run:.1f} s timed, {setup:.1f} s set-up excluded)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64", "f16"])
+    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv"])
+    ap.add_argument("--stage-steps", type=int, default=6)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path is single-GPU, "
+              "multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    from preprocess_src_amd.simulation import Simulation
+
+    if a.workload == "suspension":
+        shape, cells, params, desc, ht = suspension_workload()
+    else:
+        shape, cells, params, desc, ht = tgv_workload()
+    lx, ly, lz = shape
+    n = lx * ly * lz
+    t_setup = time.time()
+    sim = Simulation(lx, ly, lz, cells=cells, precision=a.precision, **params)
+    if a.workload == "tgv":
+        sim.set_equilibrium(*tgv_init(shape))
+    if cells is not None:
+        sim.volume_init()
+    print(f"[bench] set-up {time.time() - t_setup:.1f} s: {desc}, precision {a.precision}", file=sys.stderr,
+          flush=True)
+
+    graph = not a.no_graph
+    sim.run(0, a.warmup, graph=graph)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.run(a.warmup, a.steps, graph=graph)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    ms_per_step = 1e3 * dt / a.steps
+    mlups = n * a.steps / dt / 1e6
+    print(f"[bench] timed {a.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
+
+    # per-stage HIP-event timing on the library stream, right after the timed region
+    stages = sim.time_stages(a.warmup + a.steps, a.stage_steps)
+    t_lbm = stages["lbm_d3q19"][0] * 1e-3
+    coupled = cells is not None
+    bpn = lattice_bytes_per_node(a.precision, coupled)
+    lbm_bytes = n * bpn
+    achieved = lbm_bytes / t_lbm
+    nv = sim.nv
+    nt = sim.nt
+    step_bytes = n * (bpn + (12 if coupled else 0)) + (64 * nv + 28 * nt)
+    out = {
+        "metric": "MLUPS (coupled D3Q19+IBM+Skalak step)",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.precision,
+        "data": "synthetic (seeded cell placement, fluid at rest start)",
+        "config": {"workload": desc, "lattice": [lx, ly, lz], "cells": sim.nc, "vertices": nv, "triangles": nt,
+                   "hematocrit": round(ht, 4), "precision": a.precision, "graph": graph,
+                   "params": {k: (list(v) if isinstance(v, tuple) else v) for k, v in params.items()}},
+        "roofline": {"bound": "hbm", "kernel": "k_lbm (fused pull stream-collide)",
+                     "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "bytes_per_node": bpn, "avg_ms": round(stages["lbm_d3q19"][0], 5)},
+        "step_roofline": {"algorithmic_bytes": step_bytes, "frac": round(step_bytes / (ms_per_step * 1e-3) / HBM_PEAK, 4)},
+        "stages_ms": {k: round(v[0], 5) for k, v in stages.items()},
+    }
+    if not a.no_cpu:
+        print("[bench] cpu baseline (FP64 oracle)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(shape, cells, params, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    sim.close()
+
+
+if __name__ == "__main__":
+    main()

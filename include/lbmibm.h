@@ -1,0 +1,192 @@
+/*
+ * lbmibm.h -- C ABI of the MI355X-native coupled D3Q19-LBM / IBM / Skalak
+ * membrane timestep (liblbmibm.so, built from preprocess_src_amd/csrc).
+ *
+ * The five stage entry points replace the reference's host functions one for
+ * one, so a driver keeps main.cu's call sequence and M-gating
+ * (main.cu:80-106):
+ *
+ *   lbmibm_stress            <- void stress(process*, domain*, cell*, fem*, lattice*)   stress.cu:580
+ *   lbmibm_ibm_st1           <- void ibm_st1(domain*, lattice*, cell*)                 ibm.cu:299
+ *   lbmibm_lbm_d3q19         <- void lbm_d3q19(process*, domain*, lattice*)            lbm.cu:1463
+ *   lbmibm_ibm_st2           <- void ibm_st2(domain*, lattice*, cell*, fem*)           ibm.cu:642
+ *   lbmibm_volume_correction <- int  volume_correction(domain*, cell*, fem*)           volume_correction.cu:453
+ *   lbmibm_volume_init       <- void volume_init(domain*, cell*)                       volume_correction.cu:427
+ *
+ * Host arrays crossing the boundary use the reference layouts:
+ *   - padded lattice of n = (LX/dx+4)(LY/dx+4)(LZ/dx+4) nodes, index
+ *     i = ix + nx*iy + nx*ny*iz (init1.cu:145-167);
+ *   - populations AoS f[i*19+q] in FP64, D3Q19 order of lbm.cu:1-20, with the
+ *     reference's post-stream meaning of ltc->fn (lbm.cu:1575);
+ *   - macroscopic fields [n] (dn,un,vn,wn,dm,um,vm,wm), velocities in units
+ *     of Uc (lattice velocity = Cr*u, lbm.cu:1116);
+ *   - vertex positions SoA x,y,z [Nv] (cel->x) and AoS xd [3Nv] (fem->xd),
+ *     element vertex ids ele [3Nt] (cel->ele), global vertex ids.
+ * Cells may have different meshes: cell c owns vertices voff[c]..voff[c+1]-1
+ * and elements eoff[c]..eoff[c+1]-1.
+ *
+ * Every call returns an lbmibm_status; failures that the reference answers
+ * with exit(1) (stress.cu:364-366, volume_correction.cu:490-493) come back as
+ * status codes.  lbmibm_last_error() gives a message.
+ */
+#ifndef LBMIBM_H
+#define LBMIBM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lbmibm_status {
+    LBMIBM_OK = 0,
+    LBMIBM_ERR_ARG = 1,            /* bad argument or size                                  */
+    LBMIBM_ERR_HIP = 2,            /* HIP runtime error or no device                        */
+    LBMIBM_ERR_BICG_BREAKDOWN = 3, /* BiCGSTAB (r,r^)=0 with b!=0, flag 1 (matrix_solver.cu:334) */
+    LBMIBM_ERR_BICG_NOCONV = 4,    /* BiCGSTAB hit ITRMAX, flag 2 (matrix_solver.cu:374)     */
+    LBMIBM_ERR_VOLUME = 5,         /* volume correction hit 1000 passes (:487-493)          */
+    LBMIBM_ERR_TOPOLOGY = 6,       /* mesh topology check failed, error(3) (init2.cu:161)   */
+    LBMIBM_ERR_STATE = 7           /* call not valid in the current state                   */
+} lbmibm_status;
+
+typedef enum lbmibm_precision {
+    LBMIBM_F64 = 0, /* FP64 populations and arithmetic (the reference's precision)            */
+    LBMIBM_F32 = 1, /* FP32 populations stored as f - w*rho0, FP32 arithmetic; FP64 inner
+                       products and FP64 per-cell volume/area reductions                       */
+    LBMIBM_F16 = 2  /* populations stored in FP16 as (f - w*rho0)/w, FP32 arithmetic           */
+} lbmibm_precision;
+
+/* Runtime replacement of the def.h macros and cdo_init-derived constants. */
+typedef struct lbmibm_params {
+    int    lx, ly, lz;       /* fluid nodes per axis, LX/dx, LY/dx, LZ/dx                    */
+    int    periodic_y;       /* 0: half-way bounce-back walls at y=+-LY/2 (slab channel of
+                                lbm.cu:706-979); 1: fully periodic                           */
+    double dx;               /* lattice spacing  dx = Lc/KL   (init1.cu:138)                 */
+    double dt;               /* time step        dt = Cr dx/Uc (init1.cu:139)                */
+    double cr;               /* Courant number Cr                                            */
+    double df;               /* diffusion number Df; tau = (6Df+1)/2 (init1.cu:140)          */
+    double lambda1;          /* viscosity ratio LAMBDA1 (lbm.cu:1131)                        */
+    double p;                /* body force magnitude P (Guo force 2 rho P e)                 */
+    double ex, ey, ez;       /* body force direction e                                       */
+    int    force_modulated;  /* 1: e (1 + 0.5 sin(pi t/FREQ)) for iter > FLOWSTART, else 0
+                                (lbm.cu:1032-1035); 0: constant e (lbm.cu:518)               */
+    int    flowstart;        /* FLOWSTART                                                    */
+    double freq;             /* FREQ                                                         */
+    double b;                /* IBM coupling B (lbm.cu:1116-1118, 1135)                      */
+    int    m;                /* membrane sub-cycling M (main.cu:80-101)                      */
+    double c;                /* Skalak area dilation C (stress.cu:96)                        */
+    double rgs;              /* cel->rg (1.0 for RBCs, init1.cu:524)                         */
+    double br;               /* bending rigidity BR (stress.cu:538)                          */
+    int    itrmax;           /* BiCGSTAB ITRMAX                                              */
+    double residual;         /* BiCGSTAB RESIDUAL (relative)                                 */
+    int    runge_kutta;      /* 1, 2 or 4 (ibm.cu:430)                                       */
+    int    neohookean;       /* 0: SKALAK, 1: NEOHOOKEAN (stress.cu:95-99)                   */
+    int    volume_correction;/* 1: lbmibm_step runs volume correction (main.cu:101-106)      */
+    int    precision;        /* lbmibm_precision                                             */
+    int    store_macros;     /* 1: keep dn,un,vn,wn and the dm accumulation each step (the
+                                reference's outputs); 0: only the M-averaged velocity that
+                                the interpolation reads                                      */
+    int    fileoutput;       /* FILEOUTPUT period of the tp / qi accumulators (0: off)       */
+} lbmibm_params;
+
+/* Initial cell set: reference SoA positions and element vertex ids. */
+typedef struct lbmibm_cells {
+    int           nc;
+    const int    *voff;      /* [nc+1] */
+    const int    *eoff;      /* [nc+1] */
+    const double *x, *y, *z; /* [Nv]   */
+    const int    *ele;       /* [3Nt]  */
+} lbmibm_cells;
+
+typedef struct lbmibm_ctx lbmibm_ctx;
+
+/* Defaults for an lx*ly*lz slab channel: dx=dt=Cr=1, Df=0.1 (tau=0.8),
+ * LAMBDA1=1, no body force, B=0, M=1, C=100, rg=1, BR=0, ITRMAX=1000,
+ * RESIDUAL=1e-10, RK4, Skalak, volume correction on, FP32, store_macros=0. */
+void          lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz);
+
+/* cdo_init + flu_init + cel_init + fem_init + mem_init (init1.cu:13-1092,
+ * init2.cu:258-400): allocates every device buffer once, builds the FEM
+ * topology from the cells (may be NULL) and sets the fluid to rest (rho=1,
+ * u=0).  volume_init is separate, as in main.cu:60. */
+lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, lbmibm_ctx **out);
+void          lbmibm_destroy(lbmibm_ctx *ctx);
+const char   *lbmibm_last_error(const lbmibm_ctx *ctx);
+/* Sizes: padded lattice nodes n, vertices, elements, cells. */
+void          lbmibm_sizes(const lbmibm_ctx *ctx, long long *n_padded, int *nv, int *nt, int *nc);
+
+/* ---- stage entry points (reference host functions) ------------------- */
+lbmibm_status lbmibm_lbm_d3q19(lbmibm_ctx *ctx, int iter);                 /* lbm.cu:1463          */
+lbmibm_status lbmibm_ibm_st1(lbmibm_ctx *ctx);                             /* ibm.cu:299           */
+lbmibm_status lbmibm_ibm_st2(lbmibm_ctx *ctx);                             /* ibm.cu:642           */
+lbmibm_status lbmibm_stress(lbmibm_ctx *ctx, int iter);                    /* stress.cu:580        */
+lbmibm_status lbmibm_volume_init(lbmibm_ctx *ctx);                         /* volume_correction.cu:427 */
+lbmibm_status lbmibm_volume_correction(lbmibm_ctx *ctx, int *iterations);  /* volume_correction.cu:453;
+                                                                              *iterations = iter-1 */
+/* One main.cu loop body (main.cu:75-106) with the M gating. */
+lbmibm_status lbmibm_step(lbmibm_ctx *ctx, int iter);
+/* Steps iter0+1 .. iter0+nsteps without host synchronisation; whole
+ * M-cycles are replayed from a captured hipGraph when use_graph != 0.
+ * Device-side solver/volume flags are checked once at the end. */
+lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph);
+lbmibm_status lbmibm_synchronize(lbmibm_ctx *ctx);
+/* Timers (no reference counterpart; the reference's timer is commented out,
+ * main.cu:67-70): runs steps iter0+1 .. iter0+nsteps eagerly with HIP events
+ * on the context's stream around every stage and returns the average
+ * milliseconds per executed call in stage_ms[6] and the call counts in
+ * stage_calls[6], in the order stress, ibm_st1, lbm_d3q19, ibm_st2,
+ * volume_correction, whole step. */
+lbmibm_status lbmibm_time_stages(lbmibm_ctx *ctx, int iter0, int nsteps, double *stage_ms, int *stage_calls);
+
+/* ---- data exchange in the reference layouts --------------------------- */
+/* flu_init: f = f_eq(rho, u) on every node; rho,u,v,w padded [n] or NULL
+ * (rho=1, u=0).  Also resets macros and the lattice force. init1.cu:281-449 */
+lbmibm_status lbmibm_set_equilibrium(lbmibm_ctx *ctx, const double *rho, const double *u,
+                                     const double *v, const double *w);
+lbmibm_status lbmibm_upload_populations(lbmibm_ctx *ctx, const double *fn);   /* ltc->fnH -> device */
+lbmibm_status lbmibm_download_populations(lbmibm_ctx *ctx, double *fn);       /* COMP nodes filled  */
+/* Any pointer may be NULL.  dn,un,vn,wn,dm need store_macros=1.  COMP and
+ * BUFF nodes are filled (BUFF = periodic image, lbm.cu:1316-1432). */
+lbmibm_status lbmibm_download_macros(lbmibm_ctx *ctx, double *dn, double *un, double *vn, double *wn,
+                                     double *dm, double *um, double *vm, double *wm);
+lbmibm_status lbmibm_upload_macros(lbmibm_ctx *ctx, const double *um, const double *vm, const double *wm);
+lbmibm_status lbmibm_download_lattice_force(lbmibm_ctx *ctx, double *fx, double *fy, double *fz);
+lbmibm_status lbmibm_upload_lattice_force(lbmibm_ctx *ctx, const double *fx, const double *fy,
+                                          const double *fz);
+/* Cell state; any pointer may be NULL.  x,y,z,u,v,w,fx,fy,fz [Nv]; xd [3Nv]. */
+lbmibm_status lbmibm_download_cells(lbmibm_ctx *ctx, double *x, double *y, double *z, double *u,
+                                    double *v, double *w, double *fx, double *fy, double *fz, double *xd);
+lbmibm_status lbmibm_upload_cells(lbmibm_ctx *ctx, const double *x, const double *y, const double *z,
+                                  const double *xd, const double *fx, const double *fy, const double *fz);
+/* Membrane internals; any pointer may be NULL.  q [3Nv] (fem->q without the
+ * per-cell power-of-two padding), t [4Nt], rg [Nt], tp [2Nt], qi [3Nv]. */
+lbmibm_status lbmibm_download_membrane(lbmibm_ctx *ctx, double *q, double *t, double *rg, double *tp,
+                                       double *qi, int *bicg_iters);
+/* Per-cell volume0, volume, area [nc] and centroid gx [3nc] (FP64). */
+lbmibm_status lbmibm_download_volume(lbmibm_ctx *ctx, double *volume0, double *volume, double *area,
+                                     double *gx);
+/* Per-cell force scale (per-cell Gs / reference Gs), [nc]; NULL resets to 1. */
+lbmibm_status lbmibm_set_cell_force_scale(lbmibm_ctx *ctx, const double *scale);
+
+/* ---- host-side preprocessing (no GPU needed) ------------------------- */
+/* Icosphere of the given subdivision level on the unit sphere, outward
+ * triangles: 10*4^level+2 vertices, 20*4^level triangles.  Call with NULL
+ * arrays to get the counts.  Replaces ./cell/icosa_div files (init1.cu:490). */
+lbmibm_status lbmibm_mesh_icosphere(int level, int *nv, int *nt, double *xyz, int *tri);
+/* Evans-Fung biconcave map of unit-sphere points (axis z), in place; replaces
+ * ./cell/rbc_div files (init1.cu:506). */
+lbmibm_status lbmibm_mesh_biconcave(int nv, double *xyz);
+/* Seeded placement of nc cells of radius r (bounding sphere) in the domain:
+ * mode 0 = random sequential addition (centre distance >= 2r+1, wall
+ * distance >= r+1), mode 1 = jittered grid of gx*gy*gz slots (nc must fit).
+ * Writes centres [3nc] and rotation matrices [9nc] (Shoemake-uniform). */
+lbmibm_status lbmibm_place_cells(int nc, double r, double lxd, double lyd, double lzd, int periodic_y,
+                                 unsigned long long seed, int mode, int gx, int gy, int gz,
+                                 double *centres, double *rot);
+/* fem_init / cel_init connectivity (init1.cu:626-853, init2.cu:12-400). */
+lbmibm_status lbmibm_build_topology(int nc, const int *voff, const int *eoff, const double *xr,
+                                    const int *ele, int *nln, int *ln, int *lo, int *nle, int *le,
+                                    double *nref, int *ec, double *c0, double *s0);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

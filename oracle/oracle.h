@@ -1,0 +1,172 @@
+/*
+ * FP64 host ORACLE for the coupled D3Q19-LBM / IBM / Skalak timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a from-scratch CPU restatement of the
+ * reference algorithm, used as the checker by tests/, by
+ * __graft_entry__.smoke() and as the cpu_baseline leg of bench.py.  Nothing
+ * in the product (preprocess_src_amd/) links, loads or calls it.
+ *
+ * It deliberately keeps the reference's own data model so that it is an
+ * independent statement of the algorithm, not a copy of the GPU design:
+ *   - padded lattice with two halo layers per side, nx = LX/dx + 4
+ *     (init1.cu:145-147), node index i = ix + nx*iy + nx*ny*iz,
+ *   - AoS populations f[i*Q+q] in FP64, push streaming with half-way
+ *     bounce-back into the node's own opposite slot (lbm.cu:1273-1310),
+ *   - bc mask COMP/BUFF/WALL (init1.cu:164-178),
+ *   - per-cell ELL mass matrix solved by BiCGSTAB (stress.cu:146-272,
+ *     matrix_solver.cu:279-377),
+ *   - per-cell volume/area sums (volume_correction.cu:246-281).
+ * The one generalisation is that cells may have different meshes: cell c owns
+ * vertices voff[c]..voff[c+1]-1 and elements eoff[c]..eoff[c+1]-1 (the
+ * reference assumes one mesh per species, jid = ic*vertex + j, init1.cu:642).
+ *
+ * Parity pinning (what checks each function beyond this restatement) is listed
+ * per function below and in the test files.
+ */
+#ifndef LBMIBM_ORACLE_H
+#define LBMIBM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_COMP = 0, ORC_BUFF = 1, ORC_WALL = 2 };
+
+typedef struct orc_params {
+    int    nx, ny, nz;      /* padded lattice, LX/dx + 4 per axis (init1.cu:145-147)      */
+    int    periodic_y;      /* 0: WALL outside |y|<LY/2 (slab channel), 1: fully periodic  */
+    double dx, dt, cr;      /* lattice spacing, time step, Courant number Cr              */
+    double df, lambda1;     /* omega = 1/(3(1+(lambda1-1)vf)Df + 0.5)  (lbm.cu:1131)       */
+    double p;               /* body force magnitude P (Guo force 2 rho P e, lbm.cu:590)  */
+    double ex, ey, ez;      /* body force direction e                                      */
+    int    force_modulated; /* 1: e*(1+0.5 sin(pi t/FREQ)) after FLOWSTART (lbm.cu:1032)   */
+    int    flowstart;
+    double freq;
+    double b;               /* IBM force coupling B (lbm.cu:1116)                          */
+    int    m;               /* membrane sub-cycling M (main.cu:80-101)                     */
+    double lx, ly, lz;      /* domain extents LX, LY, LZ                                   */
+    double c;               /* Skalak area dilation C (stress.cu:96)                       */
+    double rgs;             /* cel->rg, 1.0 for RBCs (init1.cu:524)                        */
+    double br;              /* bending rigidity (stress.cu:538)                            */
+    int    itrmax;          /* BiCGSTAB ITRMAX (matrix_solver.cu:330)                      */
+    double residual;        /* BiCGSTAB RESIDUAL (matrix_solver.cu:370)                    */
+    int    runge_kutta;     /* 1, 2 or 4 (ibm.cu:430)                                      */
+    int    neohookean;      /* 0: SKALAK, 1: NEOHOOKEAN (stress.cu:95-99)                  */
+    int    fileoutput;      /* FILEOUTPUT period of the tp/qi accumulators                 */
+    int    volume_correction; /* 1: run volume_correction in orc_step (main.cu:101)        */
+} orc_params;
+
+/* Pointer set equivalent to the reference's lattice/cell/fem structs
+ * (Appendix of def.h reconstructed from use).  Lattice arrays are padded
+ * [n = nx*ny*nz]; membrane arrays use global vertex / element ids. */
+typedef struct orc_state {
+    /* lattice */
+    double *fn, *fm;                       /* [n*19]  (swapped after each lbm step)     */
+    double *dn, *un, *vn, *wn;             /* [n]                                        */
+    double *dm, *um, *vm, *wm;             /* [n]                                        */
+    double *fx, *fy, *fz;                  /* [n]  lattice IBM force                     */
+    double *vf;                            /* [n] or NULL (NULL = vf=0 everywhere)       */
+    int    *bc;                            /* [n]                                        */
+    /* cells */
+    int     nc;
+    const int *voff, *eoff;                /* [nc+1]                                     */
+    double *x, *y, *z;                     /* [Nv] SoA positions (cel->x)                */
+    double *u, *v, *w;                     /* [Nv]                                       */
+    double *cfx, *cfy, *cfz;               /* [Nv] nodal membrane force (cel->fx)        */
+    double *xd;                            /* [3Nv] AoS positions used by FEM (fem->xd)  */
+    const int *ele;                        /* [3Nt] global vertex ids                    */
+    const int *ec;                         /* [3Nt] edge-adjacent elements               */
+    const double *c0, *s0;                 /* [3Nt] bending reference angle              */
+    const double *nref;                    /* [4Nt] reference contravariant metric       */
+    double *t, *rg, *tp;                   /* [4Nt], [Nt], [2Nt] (tp may be NULL)        */
+    const int *nln, *ln, *lo, *nle, *le;   /* [Nv], [7Nv], [7Nv], [Nv], [6Nv]            */
+    double *q, *qi;                        /* [3Nv] load; qi may be NULL                 */
+    const double *gs_scale;                /* [nc] per-cell force scale or NULL (=1)     */
+    double *volume0, *volume, *area, *gx;  /* [nc], [nc], [nc], [3nc]                    */
+    int    *bicg_iters;                    /* [nc] iterations of the last solve or NULL  */
+} orc_state;
+
+/* ---- lattice ---------------------------------------------------------- */
+/* cdo_init mask, slab (walls outside |y|<LY/2, BUFF halo in x,z) or fully
+ * periodic (BUFF halo in x,y,z).  init1.cu:133-188 */
+void orc_bc_init(const orc_params *p, int *bc);
+/* flu_init: f = w rho (1 + 3cU + 4.5(cU)^2 - 1.5U^2), U = Cr u on every node;
+ * rho/u/v/w are padded [n] arrays (NULL rho -> 1, NULL velocity -> 0);
+ * fm = 0, macros = initial values, lattice force = 0.  init1.cu:281-449 */
+void orc_flu_init(const orc_params *p, orc_state *s, const double *rho,
+                  const double *u, const double *v, const double *w);
+/* One lbm_d3q19 call: fused collide + push + BB, BUFF macro copy, M-average,
+ * then the fn/fm pointer swap.  lbm.cu:706-1312, 1316-1460, 1463-1579 */
+void orc_lbm_step(const orc_params *p, orc_state *s, int iter);
+
+/* ---- immersed boundary ----------------------------------------------- */
+/* ibm_st1: zero lattice force, spread nodal forces with the 4-point cosine
+ * delta (weight phi^3/dx^3), fold BUFF contributions onto their periodic
+ * COMP images.  ibm.cu:27-131, 214-369 */
+void orc_ibm_st1(const orc_params *p, orc_state *s);
+/* ibm_st2: interpolate (um,vm,wm) with weight phi^3, RK1/2/4 on the frozen
+ * field, advect by M*dt, wrap x and z; writes x,y,z,u,v,w and xd.
+ * ibm.cu:372-694 */
+void orc_ibm_st2(const orc_params *p, orc_state *s);
+/* Single-point helpers used by the delta-function tests. */
+double orc_delta_weight(double r_over_dx);                 /* (1+cos(pi r/2))/4     */
+
+/* ---- membrane ---------------------------------------------------------- */
+/* stress(): calc_tension, per-cell ELL assembly + BiCGSTAB, q2f,
+ * fourpoint_bend.  Returns the first non-zero BiCGSTAB flag (1 breakdown,
+ * 2 not converged) or 0.  stress.cu:29-659 */
+int  orc_stress(const orc_params *p, orc_state *s, int iter);
+/* calc_tension only (stress.cu:29-121). */
+void orc_calc_tension(const orc_params *p, orc_state *s, int iter);
+/* Assemble the Jacobi-scaled ELL system of cell c (stress.cu:146-272):
+ * ptr[mn], index[7*mn], value[7*mn], b[mn] with mn = expand2pow2(3V).
+ * Requires orc_calc_tension first.  Returns mn. */
+int  orc_assemble_cell(const orc_params *p, const orc_state *s, int c,
+                       int *ptr, int *index, double *value, double *b);
+/* bicgstab(): x starts at the given value (the reference passes x=0).
+ * Returns 0, 1 (breakdown) or 2 (ITRMAX).  iters_out may be NULL.
+ * matrix_solver.cu:279-377 */
+int  orc_bicgstab(const int *ptr, const int *index, const double *value,
+                  double *x, const double *b, int n, int itrmax,
+                  double residual, int *iters_out);
+int  orc_expand2pow2(int n);                                 /* etc.cu:425-446 */
+
+/* ---- volume ------------------------------------------------------------- */
+/* Per-cell volume and area by the divergence theorem on SoA x,y,z
+ * (volume_correction.cu:138-200, 246-281). */
+void orc_calc_volume(const orc_params *p, const orc_state *s, double *volume,
+                     double *area);
+void orc_volume_init(const orc_params *p, orc_state *s);   /* :427-450 */
+/* Returns iter-1 as the reference does, or -1 when 1000 passes were hit. */
+int  orc_volume_correction(const orc_params *p, orc_state *s); /* :453-524 */
+
+/* ---- topology (init1.cu:626-853, init2.cu:12-400) -------------------- */
+/* Builds nln/ln (cyclically sorted ring), lo (ring+self ascending),
+ * nle/le, reference contravariant metric nref from the reference positions
+ * xr (AoS [3Nv]), ec (edge neighbours) and the flat bending reference
+ * c0=1, s0=0.  Returns 0 or 3 (the reference's error(3)). */
+int  orc_fem_topology(int nc, const int *voff, const int *eoff,
+                      const double *xr, const int *ele,
+                      int *nln, int *ln, int *lo, int *nle, int *le,
+                      double *nref, int *ec, double *c0, double *s0);
+
+/* ---- one main.cu iteration body with the M gating (main.cu:73-110) ---- */
+/* Returns 0, or the stress flag (1/2) or 3 when volume correction failed.
+ * vol_iters may be NULL. */
+int  orc_step(const orc_params *p, orc_state *s, int iter, int *vol_iters);
+
+/* Discrete Skalak energy sum_e A0_e (1/4)(I1^2 + 2 I1 - 2 I2 + C I2^2)
+ * (normalised by Gs), used only to pin the tension/RHS by finite
+ * differences.  Uses xd and nref. */
+double orc_skalak_energy(const orc_params *p, const orc_state *s);
+/* Bending energy sum over hinges of (2/sqrt3) br (1 - cos(theta - theta0))
+ * with the sign convention of fourpoint_bend, used to pin its forces. */
+double orc_bending_energy(const orc_params *p, const orc_state *s);
+
+/* OpenMP threads the oracle runs with (omp_get_max_threads). */
+int  orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,789 @@
+// C ABI of liblbmibm: context set-up (the reference's cdo_init / flu_init /
+// fem_init / mem_init, init1.cu:13-1092, init2.cu:258-400), the five stage
+// entry points that replace the reference host functions, the main.cu loop
+// body, graph-captured multi-step runs, and host<->device exchange in the
+// reference's padded / AoS layouts.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace lbmibm;
+
+namespace {
+
+constexpr int CXh[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int CYh[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 0, 0, 0, 0};
+constexpr int CZh[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
+constexpr double WDh[Q] = {3, 18, 18, 18, 18, 18, 18, 36, 36, 36, 36, 36, 36, 36, 36, 36, 36, 36, 36};
+inline int opph(int q) { return q == 0 ? 0 : q + ((q % 2) << 1) - 1; }
+
+struct StepParams {
+    int force_modulated, flowstart;
+    double dt, freq;
+};
+
+__global__ void k_set_step(DevStep *s, int iter, StepParams p) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int it = iter >= 0 ? iter : s->iter + 1;
+    double amp = 1.0;
+    if (p.force_modulated) amp = it > p.flowstart ? 1.0 + 0.5 * sin(M_PI * (p.dt * (double)it) / p.freq) : 0.0;
+    s->iter = it;
+    s->amp = amp;
+}
+
+__global__ void k_reset_status(DevStatus *s) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        s->bicg_flag = 0;
+        s->vol_passes = 0;
+    }
+}
+
+template <typename T> __global__ void k_convert_from_double(const double *__restrict__ a, T *__restrict__ b, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (T)a[i];
+}
+template <typename T> __global__ void k_convert_to_double(const T *__restrict__ a, double *__restrict__ b, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+lbmibm_status dev_alloc(lbmibm_ctx *ctx, void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return LBMIBM_OK;
+    LBMIBM_HIP(ctx, hipMalloc(p, bytes));
+    return LBMIBM_OK;
+}
+
+// Host double array -> device buffer of the arithmetic type.
+lbmibm_status put_T(lbmibm_ctx *ctx, void *dst, const double *src, size_t n) {
+    if (n == 0) return LBMIBM_OK;
+    if (ctx->szT == 8) {
+        LBMIBM_HIP(ctx, hipMemcpy(dst, src, n * 8, hipMemcpyHostToDevice));
+        return LBMIBM_OK;
+    }
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; i++) tmp[i] = (float)src[i];
+    LBMIBM_HIP(ctx, hipMemcpy(dst, tmp.data(), n * 4, hipMemcpyHostToDevice));
+    return LBMIBM_OK;
+}
+
+lbmibm_status get_T(lbmibm_ctx *ctx, double *dst, const void *src, size_t n) {
+    if (n == 0) return LBMIBM_OK;
+    LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->szT == 8) {
+        LBMIBM_HIP(ctx, hipMemcpy(dst, src, n * 8, hipMemcpyDeviceToHost));
+        return LBMIBM_OK;
+    }
+    std::vector<float> tmp(n);
+    LBMIBM_HIP(ctx, hipMemcpy(tmp.data(), src, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) dst[i] = (double)tmp[i];
+    return LBMIBM_OK;
+}
+
+inline long long padded_index(const lbmibm_ctx *ctx, int ix, int iy, int iz) {
+    return (long long)(ix + 2) + (long long)ctx->nxp * ((iy + 2) + (long long)ctx->nyp * (iz + 2));
+}
+
+// Source node of direction q at fluid node (ix,iy,iz) for the pull, or -1
+// when it lies beyond a wall.
+inline long long pull_source(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q) {
+    int sx = ix - CXh[q], sy = iy - CYh[q], sz = iz - CZh[q];
+    sx = (sx + ctx->nx) % ctx->nx;
+    sz = (sz + ctx->nz) % ctx->nz;
+    if (ctx->p.periodic_y) sy = (sy + ctx->ny) % ctx->ny;
+    else if (sy < 0 || sy >= ctx->ny) return -1;
+    return (long long)sx + (long long)ctx->nx * (sy + (long long)ctx->ny * sz);
+}
+
+inline long long push_target(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q) {
+    int tx = ix + CXh[q], ty = iy + CYh[q], tz = iz + CZh[q];
+    tx = (tx + ctx->nx) % ctx->nx;
+    tz = (tz + ctx->nz) % ctx->nz;
+    if (ctx->p.periodic_y) ty = (ty + ctx->ny) % ctx->ny;
+    else if (ty < 0 || ty >= ctx->ny) return -1;
+    return (long long)tx + (long long)ctx->nx * (ty + (long long)ctx->ny * tz);
+}
+
+// Periodic image in the padded frame (lbm.cu:1350-1353, 1410-1411), or -1
+// for a wall node.
+inline long long padded_image(const lbmibm_ctx *ctx, int px, int py, int pz, bool &is_comp) {
+    auto w = [](int a, int n) { return a < 2 ? a + (n - 4) : (a > n - 3 ? a - (n - 4) : a); };
+    const bool iny = py >= 2 && py <= ctx->nyp - 3;
+    is_comp = px >= 2 && px <= ctx->nxp - 3 && iny && pz >= 2 && pz <= ctx->nzp - 3;
+    if (!iny && !ctx->p.periodic_y) return -1;
+    const int ix = w(px, ctx->nxp) - 2, iy = w(py, ctx->nyp) - 2, iz = w(pz, ctx->nzp) - 2;
+    return (long long)ix + (long long)ctx->nx * (iy + (long long)ctx->ny * iz);
+}
+
+lbmibm_status sync_check(lbmibm_ctx *ctx) {
+    LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LBMIBM_OK;
+}
+
+lbmibm_status read_status(lbmibm_ctx *ctx, DevStatus &s) {
+    LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LBMIBM_HIP(ctx, hipMemcpy(&s, ctx->status, sizeof(DevStatus), hipMemcpyDeviceToHost));
+    return LBMIBM_OK;
+}
+
+lbmibm_status check_flags(lbmibm_ctx *ctx, int *vol_iters) {
+    DevStatus s;
+    lbmibm_status r = read_status(ctx, s);
+    if (r != LBMIBM_OK) return r;
+    if (s.bicg_flag == 1) { ctx->err = "BiCGSTAB breakdown: (r, r^) = 0"; return LBMIBM_ERR_BICG_BREAKDOWN; }
+    if (s.bicg_flag == 2) { ctx->err = "BiCGSTAB did not converge within ITRMAX"; return LBMIBM_ERR_BICG_NOCONV; }
+    if (s.vol_passes >= 1000) { ctx->err = "volume correction failed (1000 passes)"; return LBMIBM_ERR_VOLUME; }
+    if (vol_iters) *vol_iters = s.vol_passes > 0 ? s.vol_passes - 1 : 0;
+    return LBMIBM_OK;
+}
+
+lbmibm_status reset_status(lbmibm_ctx *ctx) {
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, ctx->stream, ctx->status);
+    LBMIBM_LAUNCHED(ctx, "k_reset_status");
+    return LBMIBM_OK;
+}
+
+// One main.cu loop body on the stream, no host synchronisation.
+lbmibm_status enqueue_step(lbmibm_ctx *ctx, int iter, bool device_advance) {
+    lbmibm_status s = set_step(ctx, device_advance ? -1 : iter, false);
+    const int M = ctx->p.m;
+    if (s == LBMIBM_OK && (iter - 1) % M == 0 && ctx->nc > 0) {
+        s = membrane_forces(ctx);
+        if (s == LBMIBM_OK) s = ibm_spread(ctx);
+    }
+    if (s == LBMIBM_OK) s = lattice_step(ctx);
+    if (s == LBMIBM_OK && iter % M == 0 && ctx->nc > 0) {
+        s = ibm_interp(ctx);
+        if (s == LBMIBM_OK && ctx->p.volume_correction) s = volume_run(ctx, 1);
+    }
+    return s;
+}
+
+void free_all(lbmibm_ctx *ctx) {
+    void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F, ctx->U, ctx->Ui, ctx->rho, ctx->rhom, ctx->voff, ctx->eoff,
+                    ctx->vcell, ctx->ele, ctx->ec, ctx->nln, ctx->ln, ctx->lo8, ctx->nle, ctx->le, ctx->xs,
+                    ctx->xd, ctx->uvw, ctx->cf, ctx->c0, ctx->s0, ctx->nref, ctx->tt, ctx->rg, ctx->tp, ctx->qv,
+                    ctx->qi, ctx->coef, ctx->bv, ctx->rv, ctx->rh, ctx->pv, ctx->vv, ctx->tv, ctx->gscale,
+                    ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->bicg_iters, ctx->vol_passes,
+                    ctx->step, ctx->status};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+}
+
+}  // namespace
+
+namespace lbmibm {
+lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool) {
+    StepParams sp{ctx->p.force_modulated, ctx->p.flowstart, ctx->p.dt, ctx->p.freq};
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, ctx->stream, ctx->step, iter, sp);
+    LBMIBM_LAUNCHED(ctx, "k_set_step");
+    return LBMIBM_OK;
+}
+}  // namespace lbmibm
+
+extern "C" {
+
+void lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz) {
+    std::memset(p, 0, sizeof(*p));
+    p->lx = lx; p->ly = ly; p->lz = lz;
+    p->periodic_y = 0;
+    p->dx = 1.0; p->dt = 1.0; p->cr = 1.0;
+    p->df = 0.1;  // tau = 0.8
+    p->lambda1 = 1.0;
+    p->p = 0.0; p->ex = 1.0; p->ey = 0.0; p->ez = 0.0;
+    p->force_modulated = 0; p->flowstart = 0; p->freq = 1.0;
+    p->b = 0.0; p->m = 1;
+    p->c = 100.0; p->rgs = 1.0; p->br = 0.0;
+    p->itrmax = 1000; p->residual = 1.0e-10; p->runge_kutta = 4; p->neohookean = 0;
+    p->volume_correction = 1;
+    p->precision = LBMIBM_F32;
+    p->store_macros = 0;
+    p->fileoutput = 0;
+}
+
+const char *lbmibm_last_error(const lbmibm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void lbmibm_sizes(const lbmibm_ctx *ctx, long long *n_padded, int *nv, int *nt, int *nc) {
+    if (!ctx) return;
+    if (n_padded) *n_padded = ctx->np;
+    if (nv) *nv = ctx->nv;
+    if (nt) *nt = ctx->nt;
+    if (nc) *nc = ctx->nc;
+}
+
+lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, lbmibm_ctx **out) {
+    if (!p || !out) return LBMIBM_ERR_ARG;
+    *out = nullptr;
+    if (p->lx < 4 || p->ly < 4 || p->lz < 4 || p->m < 1 || p->dx <= 0.0 || p->cr <= 0.0 ||
+        (p->runge_kutta != 1 && p->runge_kutta != 2 && p->runge_kutta != 4) || p->precision < 0 ||
+        p->precision > 2 || p->itrmax < 1 || (p->force_modulated && p->freq == 0.0))
+        return LBMIBM_ERR_ARG;
+    if (p->lambda1 != 1.0) return LBMIBM_ERR_ARG;  // needs the VOF indicator (front tracking)
+    const long long N = (long long)p->lx * p->ly * p->lz;
+    if (N >= (1LL << 32)) return LBMIBM_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LBMIBM_ERR_HIP;
+
+    lbmibm_ctx *ctx = new (std::nothrow) lbmibm_ctx();
+    if (!ctx) return LBMIBM_ERR_ARG;
+    ctx->p = *p;
+    ctx->nx = p->lx; ctx->ny = p->ly; ctx->nz = p->lz;
+    ctx->N = N;
+    ctx->nxp = p->lx + 4; ctx->nyp = p->ly + 4; ctx->nzp = p->lz + 4;
+    ctx->np = (long long)ctx->nxp * ctx->nyp * ctx->nzp;
+    ctx->prec = p->precision;
+    ctx->szS = p->precision == LBMIBM_F64 ? 8 : (p->precision == LBMIBM_F32 ? 4 : 2);
+    ctx->szT = p->precision == LBMIBM_F64 ? 8 : 4;
+    auto fail = [&](lbmibm_status s) {
+        free_all(ctx);
+        delete ctx;
+        return s;
+    };
+#define CK(x)                                    \
+    do {                                         \
+        lbmibm_status s_ = (x);                  \
+        if (s_ != LBMIBM_OK) return fail(s_);    \
+    } while (0)
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+
+    const size_t szS = ctx->szS, szT = ctx->szT;
+    CK(dev_alloc(ctx, &ctx->f[0], (size_t)Q * N * szS));
+    CK(dev_alloc(ctx, &ctx->f[1], (size_t)Q * N * szS));
+    CK(dev_alloc(ctx, &ctx->F, 3 * (size_t)N * szT));
+    CK(dev_alloc(ctx, &ctx->U, 3 * (size_t)N * szT));
+    if (p->store_macros) {
+        CK(dev_alloc(ctx, &ctx->Ui, 3 * (size_t)N * szT));
+        CK(dev_alloc(ctx, &ctx->rho, (size_t)N * szT));
+        CK(dev_alloc(ctx, &ctx->rhom, (size_t)N * szT));
+    }
+    CK(dev_alloc(ctx, (void **)&ctx->step, sizeof(DevStep)));
+    CK(dev_alloc(ctx, (void **)&ctx->status, sizeof(DevStatus)));
+
+    if (cells && cells->nc > 0) {
+        const int nc = cells->nc;
+        if (!cells->voff || !cells->eoff || !cells->x || !cells->y || !cells->z || !cells->ele) return fail(LBMIBM_ERR_ARG);
+        const int nv = cells->voff[nc], nt = cells->eoff[nc];
+        if (cells->voff[0] != 0 || cells->eoff[0] != 0 || nv <= 0 || nt <= 0) return fail(LBMIBM_ERR_ARG);
+        ctx->nc = nc; ctx->nv = nv; ctx->nt = nt;
+        std::vector<double> xr(3 * (size_t)nv);
+        for (int j = 0; j < nv; j++) { xr[3 * j] = cells->x[j]; xr[3 * j + 1] = cells->y[j]; xr[3 * j + 2] = cells->z[j]; }
+        std::vector<int> nln(nv), ln(7 * (size_t)nv), lo(7 * (size_t)nv), nle(nv), le(6 * (size_t)nv), ec(3 * (size_t)nt);
+        std::vector<double> nref(4 * (size_t)nt), c0(3 * (size_t)nt), s0(3 * (size_t)nt);
+        lbmibm_status ts = lbmibm_build_topology(nc, cells->voff, cells->eoff, xr.data(), cells->ele, nln.data(),
+                                                 ln.data(), lo.data(), nle.data(), le.data(), nref.data(),
+                                                 ec.data(), c0.data(), s0.data());
+        if (ts != LBMIBM_OK) {
+            ctx->err = "mesh topology check failed";
+            lbmibm_status e = ts;
+            free_all(ctx);
+            delete ctx;
+            return e;
+        }
+        std::vector<int> lo8(8 * (size_t)nv), vcell(nv);
+        for (int j = 0; j < nv; j++)
+            for (int k = 0; k < 8; k++) lo8[8 * (size_t)j + k] = k <= nln[j] && k < 7 ? lo[7 * (size_t)j + k] : j;
+        for (int c = 0; c < nc; c++) {
+            ctx->max_cell_v = std::max(ctx->max_cell_v, cells->voff[c + 1] - cells->voff[c]);
+            for (int j = cells->voff[c]; j < cells->voff[c + 1]; j++) vcell[j] = c;
+        }
+        auto up_i = [&](int **dst, const int *src, size_t n) -> lbmibm_status {
+            lbmibm_status s = dev_alloc(ctx, (void **)dst, n * sizeof(int));
+            if (s != LBMIBM_OK) return s;
+            LBMIBM_HIP(ctx, hipMemcpy(*dst, src, n * sizeof(int), hipMemcpyHostToDevice));
+            return LBMIBM_OK;
+        };
+        CK(up_i(&ctx->voff, cells->voff, nc + 1));
+        CK(up_i(&ctx->eoff, cells->eoff, nc + 1));
+        CK(up_i(&ctx->vcell, vcell.data(), nv));
+        CK(up_i(&ctx->ele, cells->ele, 3 * (size_t)nt));
+        CK(up_i(&ctx->ec, ec.data(), 3 * (size_t)nt));
+        CK(up_i(&ctx->nln, nln.data(), nv));
+        CK(up_i(&ctx->ln, ln.data(), 7 * (size_t)nv));
+        CK(up_i(&ctx->lo8, lo8.data(), 8 * (size_t)nv));
+        CK(up_i(&ctx->nle, nle.data(), nv));
+        CK(up_i(&ctx->le, le.data(), 6 * (size_t)nv));
+        auto up_T = [&](void **dst, const double *src, size_t n) -> lbmibm_status {
+            lbmibm_status s = dev_alloc(ctx, dst, n * szT);
+            if (s != LBMIBM_OK) return s;
+            return src ? put_T(ctx, *dst, src, n) : lattice_fill(ctx, *dst, n, 0.0, false);
+        };
+        std::vector<double> xs(3 * (size_t)nv);
+        std::copy(cells->x, cells->x + nv, xs.begin());
+        std::copy(cells->y, cells->y + nv, xs.begin() + nv);
+        std::copy(cells->z, cells->z + nv, xs.begin() + 2 * (size_t)nv);
+        // geometry in FP64 in every precision mode
+        auto up_D = [&](void **dst, const double *src, size_t n) -> lbmibm_status {
+            lbmibm_status s = dev_alloc(ctx, dst, n * sizeof(double));
+            if (s != LBMIBM_OK) return s;
+            LBMIBM_HIP(ctx, hipMemcpy(*dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+            return LBMIBM_OK;
+        };
+        CK(up_D(&ctx->xs, xs.data(), 3 * (size_t)nv));
+        CK(up_D(&ctx->xd, xr.data(), 3 * (size_t)nv));
+        CK(up_D(&ctx->c0, c0.data(), 3 * (size_t)nt));
+        CK(up_D(&ctx->s0, s0.data(), 3 * (size_t)nt));
+        CK(up_D(&ctx->nref, nref.data(), 4 * (size_t)nt));
+        CK(up_T(&ctx->uvw, nullptr, 3 * (size_t)nv));
+        CK(up_T(&ctx->cf, nullptr, 3 * (size_t)nv));
+        CK(up_T(&ctx->tt, nullptr, 4 * (size_t)nt));
+        CK(up_T(&ctx->rg, nullptr, (size_t)nt));
+        CK(up_T(&ctx->tp, nullptr, 2 * (size_t)nt));
+        for (void **v : {&ctx->qv, &ctx->qi, &ctx->bv, &ctx->rv, &ctx->rh, &ctx->pv, &ctx->vv, &ctx->tv})
+            CK(up_T(v, nullptr, 3 * (size_t)nv));
+        CK(up_T(&ctx->coef, nullptr, 8 * (size_t)nv));
+        std::vector<double> ones(nc, 1.0);
+        CK(up_T(&ctx->gscale, ones.data(), nc));
+        CK(dev_alloc(ctx, (void **)&ctx->normal, 3 * (size_t)nt * sizeof(double)));
+        CK(dev_alloc(ctx, (void **)&ctx->vol0, nc * sizeof(double)));
+        CK(dev_alloc(ctx, (void **)&ctx->vol, nc * sizeof(double)));
+        CK(dev_alloc(ctx, (void **)&ctx->area, nc * sizeof(double)));
+        CK(dev_alloc(ctx, (void **)&ctx->gx, 3 * nc * sizeof(double)));
+        CK(dev_alloc(ctx, (void **)&ctx->bicg_iters, nc * sizeof(int)));
+        CK(dev_alloc(ctx, (void **)&ctx->vol_passes, nc * sizeof(int)));
+        if (hipMemset(ctx->bicg_iters, 0, nc * sizeof(int)) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+        if (hipMemset(ctx->vol_passes, 0, nc * sizeof(int)) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+        if (hipMemset(ctx->gx, 0, 3 * nc * sizeof(double)) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+    }
+    // fluid at rest: f = w (deviation 0), macros rho = 1, u = 0 (init1.cu:413-449)
+    CK(lattice_fill(ctx, ctx->f[0], (size_t)Q * N, 0.0, true));
+    CK(lattice_fill(ctx, ctx->f[1], (size_t)Q * N, 0.0, true));
+    CK(lattice_fill(ctx, ctx->F, 3 * (size_t)N, 0.0, false));
+    CK(lattice_fill(ctx, ctx->U, 3 * (size_t)N, 0.0, false));
+    if (p->store_macros) {
+        CK(lattice_fill(ctx, ctx->Ui, 3 * (size_t)N, 0.0, false));
+        CK(lattice_fill(ctx, ctx->rho, (size_t)N, 1.0, false));
+        CK(lattice_fill(ctx, ctx->rhom, (size_t)N, 1.0, false));
+    }
+    CK(set_step(ctx, 0, false));
+    CK(reset_status(ctx));
+    CK(sync_check(ctx));
+#undef CK
+    *out = ctx;
+    return LBMIBM_OK;
+}
+
+void lbmibm_destroy(lbmibm_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+}
+
+// ---- stage entry points -------------------------------------------------
+lbmibm_status lbmibm_lbm_d3q19(lbmibm_ctx *ctx, int iter) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    lbmibm_status s = set_step(ctx, iter, false);
+    if (s == LBMIBM_OK) s = lattice_step(ctx);
+    if (s == LBMIBM_OK) s = sync_check(ctx);  // the reference syncs (lbm.cu:1573)
+    return s;
+}
+
+lbmibm_status lbmibm_ibm_st1(lbmibm_ctx *ctx) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    lbmibm_status s = ibm_spread(ctx);
+    return s == LBMIBM_OK ? sync_check(ctx) : s;
+}
+
+lbmibm_status lbmibm_ibm_st2(lbmibm_ctx *ctx) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    lbmibm_status s = ibm_interp(ctx);
+    return s == LBMIBM_OK ? sync_check(ctx) : s;
+}
+
+lbmibm_status lbmibm_stress(lbmibm_ctx *ctx, int iter) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    lbmibm_status s = set_step(ctx, iter, false);
+    if (s == LBMIBM_OK) s = reset_status(ctx);
+    if (s == LBMIBM_OK) s = membrane_forces(ctx);
+    return s == LBMIBM_OK ? check_flags(ctx, nullptr) : s;
+}
+
+lbmibm_status lbmibm_volume_init(lbmibm_ctx *ctx) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    lbmibm_status s = volume_run(ctx, 0);
+    return s == LBMIBM_OK ? sync_check(ctx) : s;
+}
+
+lbmibm_status lbmibm_volume_correction(lbmibm_ctx *ctx, int *iterations) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    lbmibm_status s = reset_status(ctx);
+    if (s == LBMIBM_OK) s = volume_run(ctx, 1);
+    int k = 0;
+    if (s == LBMIBM_OK) s = check_flags(ctx, &k);
+    if (iterations) *iterations = k;
+    return s;
+}
+
+lbmibm_status lbmibm_step(lbmibm_ctx *ctx, int iter) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    lbmibm_status s = reset_status(ctx);
+    if (s == LBMIBM_OK) s = enqueue_step(ctx, iter, false);
+    return s == LBMIBM_OK ? check_flags(ctx, nullptr) : s;
+}
+
+lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) {
+    if (!ctx || nsteps < 0) return LBMIBM_ERR_ARG;
+    lbmibm_status s = reset_status(ctx);
+    if (s != LBMIBM_OK) return s;
+    const int M = ctx->p.m;
+    // a replayable block is whole M-cycles with an even number of lattice
+    // steps, so that the population buffers are back in place
+    const int L = (M % 2 == 0) ? M : 2 * M;
+    int it = iter0;
+    const int end = iter0 + nsteps;
+    s = set_step(ctx, iter0, false);
+    // eager steps until the next step starts an M-cycle
+    while (s == LBMIBM_OK && it < end && (use_graph == 0 || (it % M) != 0)) {
+        s = enqueue_step(ctx, it + 1, true);
+        it++;
+    }
+    if (s == LBMIBM_OK && use_graph && end - it >= L) {
+        if (ctx->graph_ready && (ctx->graph_cur != ctx->cur || ctx->graph_force_zero != ctx->force_zero)) {
+            (void)hipGraphExecDestroy(ctx->graph);
+            ctx->graph = nullptr;
+            ctx->graph_ready = false;
+        }
+        if (!ctx->graph_ready) {
+            ctx->graph_cur = ctx->cur;
+            ctx->graph_force_zero = ctx->force_zero;
+            hipGraph_t g = nullptr;
+            LBMIBM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            lbmibm_status cs = LBMIBM_OK;
+            for (int k = 0; k < L && cs == LBMIBM_OK; k++) cs = enqueue_step(ctx, it + k + 1, true);
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (cs != LBMIBM_OK) return cs;
+            if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+            e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphInstantiate");
+            ctx->graph_ready = true;
+        }
+        while (end - it >= L) {
+            LBMIBM_HIP(ctx, hipGraphLaunch(ctx->graph, ctx->stream));
+            it += L;
+        }
+    }
+    while (s == LBMIBM_OK && it < end) {
+        s = enqueue_step(ctx, it + 1, true);
+        it++;
+    }
+    return s == LBMIBM_OK ? check_flags(ctx, nullptr) : s;
+}
+
+lbmibm_status lbmibm_time_stages(lbmibm_ctx *ctx, int iter0, int nsteps, double *stage_ms, int *stage_calls) {
+    if (!ctx || nsteps < 1 || !stage_ms) return LBMIBM_ERR_ARG;
+    hipEvent_t ev[6];
+    for (auto &e : ev) LBMIBM_HIP(ctx, hipEventCreate(&e));
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    int calls[6] = {0, 0, 0, 0, 0, 0};
+    lbmibm_status s = reset_status(ctx);
+    const int M = ctx->p.m;
+    for (int k = 0; k < nsteps && s == LBMIBM_OK; k++) {
+        const int it = iter0 + k + 1;
+        const bool mem = (it - 1) % M == 0 && ctx->nc > 0, post = it % M == 0 && ctx->nc > 0;
+        s = set_step(ctx, it, false);
+        if (s != LBMIBM_OK) break;
+        (void)hipEventRecord(ev[0], ctx->stream);
+        if (mem) s = membrane_forces(ctx);
+        (void)hipEventRecord(ev[1], ctx->stream);
+        if (s == LBMIBM_OK && mem) s = ibm_spread(ctx);
+        (void)hipEventRecord(ev[2], ctx->stream);
+        if (s == LBMIBM_OK) s = lattice_step(ctx);
+        (void)hipEventRecord(ev[3], ctx->stream);
+        if (s == LBMIBM_OK && post) s = ibm_interp(ctx);
+        (void)hipEventRecord(ev[4], ctx->stream);
+        if (s == LBMIBM_OK && post && ctx->p.volume_correction) s = volume_run(ctx, 1);
+        (void)hipEventRecord(ev[5], ctx->stream);
+        if (hipEventSynchronize(ev[5]) != hipSuccess) { s = LBMIBM_ERR_HIP; break; }
+        const bool ran[5] = {mem, mem, true, post, post && ctx->p.volume_correction != 0};
+        for (int j = 0; j < 5; j++) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[j], ev[j + 1]);
+            if (ran[j]) { acc[j] += ms; calls[j]++; }
+        }
+        float tot = 0.f;
+        (void)hipEventElapsedTime(&tot, ev[0], ev[5]);
+        acc[5] += tot;
+        calls[5]++;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    for (int j = 0; j < 6; j++) {
+        stage_ms[j] = calls[j] ? acc[j] / calls[j] : 0.0;
+        if (stage_calls) stage_calls[j] = calls[j];
+    }
+    return s == LBMIBM_OK ? check_flags(ctx, nullptr) : s;
+}
+
+lbmibm_status lbmibm_synchronize(lbmibm_ctx *ctx) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    return sync_check(ctx);
+}
+
+// ---- exchange in reference layouts -------------------------------------
+lbmibm_status lbmibm_upload_populations(lbmibm_ctx *ctx, const double *fn) {
+    if (!ctx || !fn) return LBMIBM_ERR_ARG;
+    // f*_q(x) = fn_q(x + c_q), or fn_opp(q)(x) when x + c_q is a wall: the
+    // inverse of the push-with-bounce-back of lbm.cu:1273-1310.
+    const long long N = ctx->N;
+    std::vector<double> fs((size_t)Q * N);
+    for (int iz = 0; iz < ctx->nz; iz++)
+        for (int iy = 0; iy < ctx->ny; iy++)
+            for (int ix = 0; ix < ctx->nx; ix++) {
+                const long long i = ix + (long long)ctx->nx * (iy + (long long)ctx->ny * iz);
+                const long long pi = padded_index(ctx, ix, iy, iz);
+                for (int q = 0; q < Q; q++) {
+                    const long long t = push_target(ctx, ix, iy, iz, q);
+                    double v;
+                    if (t < 0) v = fn[pi * Q + opph(q)];
+                    else {
+                        const int tx = (int)(t % ctx->nx), ty = (int)((t / ctx->nx) % ctx->ny), tz = (int)(t / ((long long)ctx->nx * ctx->ny));
+                        v = fn[padded_index(ctx, tx, ty, tz) * Q + q];
+                    }
+                    fs[(size_t)q * N + i] = v;
+                }
+            }
+    ctx->graph_ready = false;
+    return lattice_set_state_from_fstar(ctx, fs.data());
+}
+
+lbmibm_status lbmibm_download_populations(lbmibm_ctx *ctx, double *fn) {
+    if (!ctx || !fn) return LBMIBM_ERR_ARG;
+    const long long N = ctx->N;
+    std::vector<double> fs((size_t)Q * N);
+    lbmibm_status s = lattice_get_fstar(ctx, fs.data());
+    if (s != LBMIBM_OK) return s;
+    for (int iz = 0; iz < ctx->nz; iz++)
+        for (int iy = 0; iy < ctx->ny; iy++)
+            for (int ix = 0; ix < ctx->nx; ix++) {
+                const long long i = ix + (long long)ctx->nx * (iy + (long long)ctx->ny * iz);
+                const long long pi = padded_index(ctx, ix, iy, iz);
+                for (int q = 0; q < Q; q++) {
+                    const long long src = pull_source(ctx, ix, iy, iz, q);
+                    fn[pi * Q + q] = src < 0 ? fs[(size_t)opph(q) * N + i] : fs[(size_t)q * N + src];
+                }
+            }
+    return LBMIBM_OK;
+}
+
+lbmibm_status lbmibm_set_equilibrium(lbmibm_ctx *ctx, const double *rho, const double *u, const double *v,
+                                     const double *w) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    // flu_init (init1.cu:413-449) on the padded grid, then uploaded with the
+    // reference's post-stream meaning.
+    const long long np = ctx->np;
+    const double Cr = ctx->p.cr, Cr2 = Cr * Cr;
+    std::vector<double> fn((size_t)np * Q);
+    for (long long i = 0; i < np; i++) {
+        const double td = rho ? rho[i] : 1.0, tu = u ? u[i] : 0.0, tv = v ? v[i] : 0.0, tw = w ? w[i] : 0.0;
+        const double uvw = -1.5 * (tu * tu + tv * tv + tw * tw) * Cr2;
+        for (int q = 0; q < Q; q++) {
+            const double cu = CXh[q] * tu + CYh[q] * tv + CZh[q] * tw;
+            fn[i * Q + q] = td / WDh[q] * (1.0 + 3.0 * cu * Cr + 4.5 * cu * cu * Cr2 + uvw);
+        }
+    }
+    lbmibm_status s = lbmibm_upload_populations(ctx, fn.data());
+    if (s != LBMIBM_OK) return s;
+    // macros take their initial values on the fluid nodes
+    const long long N = ctx->N;
+    std::vector<double> uu(3 * (size_t)N), rr((size_t)N);
+    for (int iz = 0; iz < ctx->nz; iz++)
+        for (int iy = 0; iy < ctx->ny; iy++)
+            for (int ix = 0; ix < ctx->nx; ix++) {
+                const long long i = ix + (long long)ctx->nx * (iy + (long long)ctx->ny * iz);
+                const long long pi = padded_index(ctx, ix, iy, iz);
+                uu[i] = u ? u[pi] : 0.0;
+                uu[N + i] = v ? v[pi] : 0.0;
+                uu[2 * N + i] = w ? w[pi] : 0.0;
+                rr[i] = rho ? rho[pi] : 1.0;
+            }
+    s = put_T(ctx, ctx->U, uu.data(), 3 * (size_t)N);
+    if (s == LBMIBM_OK && ctx->p.store_macros) {
+        s = put_T(ctx, ctx->Ui, uu.data(), 3 * (size_t)N);
+        if (s == LBMIBM_OK) s = put_T(ctx, ctx->rho, rr.data(), (size_t)N);
+        if (s == LBMIBM_OK) s = put_T(ctx, ctx->rhom, rr.data(), (size_t)N);
+    }
+    if (s == LBMIBM_OK) s = lattice_fill(ctx, ctx->F, 3 * (size_t)N, 0.0, false);
+    ctx->force_zero = true;
+    return s == LBMIBM_OK ? sync_check(ctx) : s;
+}
+
+lbmibm_status lbmibm_download_macros(lbmibm_ctx *ctx, double *dn, double *un, double *vn, double *wn, double *dm,
+                                     double *um, double *vm, double *wm) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    if ((dn || un || vn || wn || dm) && !ctx->p.store_macros) {
+        ctx->err = "dn/un/vn/wn/dm need store_macros=1";
+        return LBMIBM_ERR_STATE;
+    }
+    const long long N = ctx->N;
+    std::vector<double> U(3 * (size_t)N), Ui, R, RM;
+    lbmibm_status s = get_T(ctx, U.data(), ctx->U, 3 * (size_t)N);
+    if (s == LBMIBM_OK && ctx->p.store_macros) {
+        Ui.resize(3 * (size_t)N); R.resize(N); RM.resize(N);
+        s = get_T(ctx, Ui.data(), ctx->Ui, 3 * (size_t)N);
+        if (s == LBMIBM_OK) s = get_T(ctx, R.data(), ctx->rho, N);
+        if (s == LBMIBM_OK) s = get_T(ctx, RM.data(), ctx->rhom, N);
+    }
+    if (s != LBMIBM_OK) return s;
+    for (int pz = 0; pz < ctx->nzp; pz++)
+        for (int py = 0; py < ctx->nyp; py++)
+            for (int px = 0; px < ctx->nxp; px++) {
+                bool comp;
+                const long long i = padded_image(ctx, px, py, pz, comp);
+                if (i < 0) continue;
+                const long long pi = px + (long long)ctx->nxp * (py + (long long)ctx->nyp * pz);
+                if (um) um[pi] = U[i];
+                if (vm) vm[pi] = U[N + i];
+                if (wm) wm[pi] = U[2 * N + i];
+                if (dn) dn[pi] = R[i];
+                if (un) un[pi] = Ui[i];
+                if (vn) vn[pi] = Ui[N + i];
+                if (wn) wn[pi] = Ui[2 * N + i];
+                if (dm) dm[pi] = RM[i];
+            }
+    return LBMIBM_OK;
+}
+
+lbmibm_status lbmibm_upload_macros(lbmibm_ctx *ctx, const double *um, const double *vm, const double *wm) {
+    if (!ctx || !um || !vm || !wm) return LBMIBM_ERR_ARG;
+    const long long N = ctx->N;
+    std::vector<double> U(3 * (size_t)N);
+    for (int iz = 0; iz < ctx->nz; iz++)
+        for (int iy = 0; iy < ctx->ny; iy++)
+            for (int ix = 0; ix < ctx->nx; ix++) {
+                const long long i = ix + (long long)ctx->nx * (iy + (long long)ctx->ny * iz);
+                const long long pi = padded_index(ctx, ix, iy, iz);
+                U[i] = um[pi]; U[N + i] = vm[pi]; U[2 * N + i] = wm[pi];
+            }
+    lbmibm_status s = put_T(ctx, ctx->U, U.data(), 3 * (size_t)N);
+    return s == LBMIBM_OK ? sync_check(ctx) : s;
+}
+
+lbmibm_status lbmibm_download_lattice_force(lbmibm_ctx *ctx, double *fx, double *fy, double *fz) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    const long long N = ctx->N;
+    std::vector<double> F(3 * (size_t)N);
+    lbmibm_status s = get_T(ctx, F.data(), ctx->F, 3 * (size_t)N);
+    if (s != LBMIBM_OK) return s;
+    for (int iz = 0; iz < ctx->nz; iz++)
+        for (int iy = 0; iy < ctx->ny; iy++)
+            for (int ix = 0; ix < ctx->nx; ix++) {
+                const long long i = ix + (long long)ctx->nx * (iy + (long long)ctx->ny * iz);
+                const long long pi = padded_index(ctx, ix, iy, iz);
+                if (fx) fx[pi] = F[i];
+                if (fy) fy[pi] = F[N + i];
+                if (fz) fz[pi] = F[2 * N + i];
+            }
+    return LBMIBM_OK;
+}
+
+lbmibm_status lbmibm_upload_lattice_force(lbmibm_ctx *ctx, const double *fx, const double *fy, const double *fz) {
+    if (!ctx || !fx || !fy || !fz) return LBMIBM_ERR_ARG;
+    const long long N = ctx->N;
+    std::vector<double> F(3 * (size_t)N);
+    bool zero = true;
+    for (int iz = 0; iz < ctx->nz; iz++)
+        for (int iy = 0; iy < ctx->ny; iy++)
+            for (int ix = 0; ix < ctx->nx; ix++) {
+                const long long i = ix + (long long)ctx->nx * (iy + (long long)ctx->ny * iz);
+                const long long pi = padded_index(ctx, ix, iy, iz);
+                F[i] = fx[pi]; F[N + i] = fy[pi]; F[2 * N + i] = fz[pi];
+                zero = zero && F[i] == 0.0 && F[N + i] == 0.0 && F[2 * N + i] == 0.0;
+            }
+    lbmibm_status s = put_T(ctx, ctx->F, F.data(), 3 * (size_t)N);
+    ctx->force_zero = zero;
+    ctx->graph_ready = false;
+    return s == LBMIBM_OK ? sync_check(ctx) : s;
+}
+
+lbmibm_status lbmibm_download_cells(lbmibm_ctx *ctx, double *x, double *y, double *z, double *u, double *v,
+                                    double *w, double *fx, double *fy, double *fz, double *xd) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    const size_t nv = ctx->nv;
+    if (nv == 0) return LBMIBM_OK;
+    std::vector<double> a(3 * nv);
+    struct { void *src; bool dbl; double *d[3]; } sets[3] = {
+        {ctx->xs, true, {x, y, z}}, {ctx->uvw, false, {u, v, w}}, {ctx->cf, false, {fx, fy, fz}}};
+    LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &st : sets) {
+        if (!st.d[0] && !st.d[1] && !st.d[2]) continue;
+        if (st.dbl) {
+            LBMIBM_HIP(ctx, hipMemcpy(a.data(), st.src, 3 * nv * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            lbmibm_status s = get_T(ctx, a.data(), st.src, 3 * nv);
+            if (s != LBMIBM_OK) return s;
+        }
+        for (int d = 0; d < 3; d++)
+            if (st.d[d]) std::copy(a.begin() + d * nv, a.begin() + (d + 1) * nv, st.d[d]);
+    }
+    if (xd) LBMIBM_HIP(ctx, hipMemcpy(xd, ctx->xd, 3 * nv * sizeof(double), hipMemcpyDeviceToHost));
+    return LBMIBM_OK;
+}
+
+lbmibm_status lbmibm_upload_cells(lbmibm_ctx *ctx, const double *x, const double *y, const double *z,
+                                  const double *xd, const double *fx, const double *fy, const double *fz) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    const size_t nv = ctx->nv;
+    if (nv == 0) return LBMIBM_OK;
+    std::vector<double> a(3 * nv);
+    lbmibm_status s = LBMIBM_OK;
+    if (x && y && z) {
+        std::copy(x, x + nv, a.begin());
+        std::copy(y, y + nv, a.begin() + nv);
+        std::copy(z, z + nv, a.begin() + 2 * nv);
+        LBMIBM_HIP(ctx, hipMemcpy(ctx->xs, a.data(), 3 * nv * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (xd) LBMIBM_HIP(ctx, hipMemcpy(ctx->xd, xd, 3 * nv * sizeof(double), hipMemcpyHostToDevice));
+    if (s == LBMIBM_OK && fx && fy && fz) {
+        std::copy(fx, fx + nv, a.begin());
+        std::copy(fy, fy + nv, a.begin() + nv);
+        std::copy(fz, fz + nv, a.begin() + 2 * nv);
+        s = put_T(ctx, ctx->cf, a.data(), 3 * nv);
+    }
+    return s == LBMIBM_OK ? sync_check(ctx) : s;
+}
+
+lbmibm_status lbmibm_download_membrane(lbmibm_ctx *ctx, double *q, double *t, double *rg, double *tp, double *qi,
+                                       int *bicg_iters) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    if (ctx->nc == 0) return LBMIBM_OK;
+    lbmibm_status s = LBMIBM_OK;
+    if (q) s = get_T(ctx, q, ctx->qv, 3 * (size_t)ctx->nv);
+    if (s == LBMIBM_OK && t) s = get_T(ctx, t, ctx->tt, 4 * (size_t)ctx->nt);
+    if (s == LBMIBM_OK && rg) s = get_T(ctx, rg, ctx->rg, (size_t)ctx->nt);
+    if (s == LBMIBM_OK && tp) s = get_T(ctx, tp, ctx->tp, 2 * (size_t)ctx->nt);
+    if (s == LBMIBM_OK && qi) s = get_T(ctx, qi, ctx->qi, 3 * (size_t)ctx->nv);
+    if (s == LBMIBM_OK && bicg_iters) {
+        LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        LBMIBM_HIP(ctx, hipMemcpy(bicg_iters, ctx->bicg_iters, ctx->nc * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    return s;
+}
+
+lbmibm_status lbmibm_download_volume(lbmibm_ctx *ctx, double *volume0, double *volume, double *area, double *gx) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    if (ctx->nc == 0) return LBMIBM_OK;
+    LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t nc = ctx->nc;
+    if (volume0) LBMIBM_HIP(ctx, hipMemcpy(volume0, ctx->vol0, nc * 8, hipMemcpyDeviceToHost));
+    if (volume) LBMIBM_HIP(ctx, hipMemcpy(volume, ctx->vol, nc * 8, hipMemcpyDeviceToHost));
+    if (area) LBMIBM_HIP(ctx, hipMemcpy(area, ctx->area, nc * 8, hipMemcpyDeviceToHost));
+    if (gx) LBMIBM_HIP(ctx, hipMemcpy(gx, ctx->gx, 3 * nc * 8, hipMemcpyDeviceToHost));
+    return LBMIBM_OK;
+}
+
+lbmibm_status lbmibm_set_cell_force_scale(lbmibm_ctx *ctx, const double *scale) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    if (ctx->nc == 0) return LBMIBM_OK;
+    std::vector<double> s(ctx->nc, 1.0);
+    if (scale) std::copy(scale, scale + ctx->nc, s.begin());
+    lbmibm_status r = put_T(ctx, ctx->gscale, s.data(), ctx->nc);
+    return r == LBMIBM_OK ? sync_check(ctx) : r;
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+// Internal declarations shared by the HIP translation units of liblbmibm.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "lbmibm.h"
+
+namespace lbmibm {
+
+constexpr int Q = 19;
+constexpr int WG = 256;  // work-group size of the streaming kernels (4 wave64)
+
+// Per-step values that kernels read from device memory, so that a captured
+// graph replays unchanged: the iteration counter and the body-force
+// amplitude e(t) of lbm.cu:1032-1035.
+struct DevStep {
+    int iter;
+    int pad;
+    double amp;
+};
+
+// Device status words, checked by the host after a synchronisation.
+struct DevStatus {
+    int bicg_flag;    // max BiCGSTAB flag over cells (1 breakdown, 2 ITRMAX)
+    int vol_passes;   // max volume-correction passes over cells
+};
+
+}  // namespace lbmibm
+
+struct lbmibm_ctx {
+    lbmibm_params p;
+    int nx, ny, nz;            // fluid nodes (unpadded)
+    long long N;               // nx*ny*nz
+    int nxp, nyp, nzp;         // reference padded sizes (+4)
+    long long np;
+    int nc = 0, nv = 0, nt = 0;
+    int prec;                  // lbmibm_precision
+    size_t szS, szT;           // storage / arithmetic type sizes
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // lattice (device)
+    void *f[2] = {nullptr, nullptr};  // SoA [19][N] post-collision populations, type S
+    int cur = 0;                      // f[cur] holds the current state
+    void *F = nullptr;                // [3][N] lattice IBM force (T)
+    void *U = nullptr;                // [3][N] M-averaged velocity um,vm,wm (T)
+    void *Ui = nullptr;               // [3][N] instantaneous velocity (store_macros)
+    void *rho = nullptr;              // [N] instantaneous density (store_macros)
+    void *rhom = nullptr;             // [N] accumulated density dm (store_macros)
+    bool force_zero = true;           // lattice force known to be identically zero
+
+    // cells (device): positions, metric and bending reference in FP64; the
+    // other membrane arrays in the arithmetic type T
+    int *voff = nullptr, *eoff = nullptr, *vcell = nullptr;
+    int *ele = nullptr, *ec = nullptr;
+    int *nln = nullptr, *ln = nullptr, *lo8 = nullptr, *nle = nullptr, *le = nullptr;
+    void *xs = nullptr;    // [3][Nv] SoA positions (cel->x,y,z), FP64
+    void *xd = nullptr;    // [3Nv] AoS positions (fem->xd), FP64
+    void *uvw = nullptr;   // [3][Nv] interpolated velocity (cel->u,v,w)
+    void *cf = nullptr;    // [3][Nv] nodal force (cel->fx,fy,fz)
+    void *c0 = nullptr, *s0 = nullptr, *nref = nullptr;  // FP64
+    void *tt = nullptr, *rg = nullptr, *tp = nullptr;
+    void *qv = nullptr, *qi = nullptr;                 // [3Nv]
+    void *coef = nullptr;                              // [Nv][8] Jacobi-scaled mass-matrix rows
+    void *bv = nullptr, *rv = nullptr, *rh = nullptr, *pv = nullptr, *vv = nullptr, *tv = nullptr;
+    void *gscale = nullptr;                            // [nc] (T)
+    double *normal = nullptr;                          // [3Nt] FP64 element normals
+    double *vol0 = nullptr, *vol = nullptr, *area = nullptr, *gx = nullptr;  // [nc], [3nc]
+    int *bicg_iters = nullptr, *vol_passes = nullptr;  // [nc]
+    int max_cell_v = 0;
+
+    lbmibm::DevStep *step = nullptr;
+    lbmibm::DevStatus *status = nullptr;
+
+    // whole-M-cycle graph
+    hipGraphExec_t graph = nullptr;
+    bool graph_ready = false;
+    int graph_cur = 0;              // population buffer current when captured
+    bool graph_force_zero = true;   // lattice-force variant when captured
+};
+
+namespace lbmibm {
+
+// Launchers implemented per translation unit; each dispatches on ctx->prec.
+lbmibm_status lattice_step(lbmibm_ctx *ctx);                 // lattice.hip
+lbmibm_status lattice_set_state_from_fstar(lbmibm_ctx *ctx, const double *fstar_soa);  // lattice.hip
+lbmibm_status lattice_get_fstar(lbmibm_ctx *ctx, double *fstar_soa);  // lattice.hip
+lbmibm_status lattice_fill(lbmibm_ctx *ctx, void *ptr, size_t count, double value, bool storage_type);
+lbmibm_status ibm_spread(lbmibm_ctx *ctx);                   // ibm.hip
+lbmibm_status ibm_interp(lbmibm_ctx *ctx);                   // ibm.hip
+lbmibm_status membrane_forces(lbmibm_ctx *ctx);              // membrane.hip
+lbmibm_status volume_run(lbmibm_ctx *ctx, int mode);         // volume.hip (0 init, 1 correct)
+lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool advance);  // api.hip
+
+inline lbmibm_status hip_fail(lbmibm_ctx *ctx, hipError_t e, const char *what) {
+    ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+    return LBMIBM_ERR_HIP;
+}
+
+#define LBMIBM_HIP(ctx, call)                                           \
+    do {                                                                \
+        hipError_t e_ = (call);                                         \
+        if (e_ != hipSuccess) return ::lbmibm::hip_fail(ctx, e_, #call); \
+    } while (0)
+
+#define LBMIBM_LAUNCHED(ctx, name)                                      \
+    do {                                                                \
+        hipError_t e_ = hipGetLastError();                              \
+        if (e_ != hipSuccess) return ::lbmibm::hip_fail(ctx, e_, name); \
+    } while (0)
+
+// Work-group remapping so that each of the 8 XCDs (which receive work groups
+// round-robin) walks one contiguous eighth of the index space: neighbouring
+// rows read by the stencil then hit that XCD's own L2.
+__device__ __forceinline__ int xcd_block(int nblocks) {
+    const int per = (nblocks + 7) >> 3;
+    const int b = blockIdx.x;
+    return (b & 7) * per + (b >> 3);
+}
+inline int xcd_grid(int nblocks) { return ((nblocks + 7) >> 3) << 3; }
+
+template <typename T> __device__ __forceinline__ T dev_min_image(T d, T L) {
+    if (d > L * T(0.5)) d -= L;
+    else if (d < -L * T(0.5)) d += L;
+    return d;
+}
+
+// FP64 sum over a 256-thread work group in a fixed order (deterministic).
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+}  // namespace lbmibm

@@ -1,0 +1,270 @@
+// D3Q19 stream-collide for gfx950.
+//
+// Reference: lbm_CapsuleSHEARFLOW / lbm_CapsuleCHANNELFLOW / lbm_CHANNELFLOW
+// (lbm.cu:481-1312) plus lbm_*_Boundary and lbm_AVERAGE (lbm.cu:1316-1460),
+// host launcher lbm_d3q19 (lbm.cu:1463-1579).
+//
+// Layout: structure-of-arrays f[q][N] over the unpadded fluid lattice
+// (i = ix + nx*(iy + ny*iz)); the reference's two halo layers and its bc mask
+// are replaced by index wrapping (periodic x, z and optionally y) and the
+// analytic wall rows iy = -1 and iy = ny (half-way bounce-back at
+// y = +-LY/2).  The buffers hold POST-collision populations: one launch
+// pulls f_q(x) = f*_q(x - c_q) (or f*_opp(q)(x) across a wall), forms the
+// moments, collides and stores f*(x) at x.  Every population is read once
+// and written once, both fully coalesced; the reference's post-stream array
+// is the pull-gather of this state (lattice_get_fstar + host gather).
+//
+// Storage: deviations g = f - w_q rho0 (rho0 = 1), in FP64, FP32, or FP16
+// as g/w_q; arithmetic in T (double or float).
+#include "common.hpp"
+
+namespace lbmibm {
+namespace {
+
+__host__ __device__ constexpr int cx(int q) {
+    return q == 1 || q == 7 || q == 10 || q == 15 || q == 17 ? 1 : (q == 2 || q == 8 || q == 9 || q == 16 || q == 18 ? -1 : 0);
+}
+__host__ __device__ constexpr int cy(int q) {
+    return q == 3 || q == 7 || q == 9 || q == 11 || q == 14 ? 1 : (q == 4 || q == 8 || q == 10 || q == 12 || q == 13 ? -1 : 0);
+}
+__host__ __device__ constexpr int cz(int q) {
+    return q == 5 || q == 11 || q == 13 || q == 15 || q == 18 ? 1 : (q == 6 || q == 12 || q == 14 || q == 16 || q == 17 ? -1 : 0);
+}
+__host__ __device__ constexpr int opp(int q) { return q == 0 ? 0 : q + ((q % 2) << 1) - 1; }
+template <typename T> __host__ __device__ constexpr T wq(int q) {
+    return q == 0 ? T(1) / T(3) : (q <= 6 ? T(1) / T(18) : T(1) / T(36));
+}
+template <typename T> __host__ __device__ constexpr T inv_wq(int q) { return q == 0 ? T(3) : (q <= 6 ? T(18) : T(36)); }
+
+template <typename S, typename T> struct Pop {
+    __device__ static T ld(const S *p, int) { return (T)(*p); }
+    __device__ static S st(T g, int) { return (S)g; }
+};
+template <typename T> struct Pop<__half, T> {
+    __device__ static T ld(const __half *p, int q) { return (T)__half2float(*p) * wq<T>(q); }
+    __device__ static __half st(T g, int q) { return __float2half((float)(g * inv_wq<T>(q))); }
+};
+
+struct LbmDims {
+    unsigned N;
+    int nx, ny, nz;
+    int nblocks;
+};
+
+template <typename S, typename T, bool HASF, bool MACROS, bool WALLS>
+__global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__restrict__ fout, LbmDims d,
+                                            const T *__restrict__ F, T *__restrict__ U, T *__restrict__ Ui,
+                                            T *__restrict__ rho_out, T *__restrict__ rhom, T om, T inv_cr,
+                                            T pex, T pey, T pez, T B, const DevStep *__restrict__ st, int m) {
+    const int lb = xcd_block(d.nblocks);
+    if (lb >= d.nblocks) return;
+    const unsigned i = (unsigned)lb * WG + threadIdx.x;
+    if (i >= d.N) return;
+    const unsigned nx = d.nx, ny = d.ny, nz = d.nz;
+    const size_t N = d.N;
+    const unsigned ix = i % nx, r = i / nx, iy = r % ny, iz = r / ny;
+    const unsigned xm = ix == 0 ? nx - 1 : ix - 1, xp = ix == nx - 1 ? 0 : ix + 1;
+    const unsigned zm = iz == 0 ? nz - 1 : iz - 1, zp = iz == nz - 1 ? 0 : iz + 1;
+    bool wall_lo = false, wall_hi = false;
+    unsigned ym, yp;
+    if (WALLS) {
+        wall_lo = iy == 0;       // sources at iy-1 are wall nodes
+        wall_hi = iy == ny - 1;  // sources at iy+1 are wall nodes
+        ym = iy - 1;
+        yp = iy + 1;
+    } else {
+        ym = iy == 0 ? ny - 1 : iy - 1;
+        yp = iy == ny - 1 ? 0 : iy + 1;
+    }
+
+    T g[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const unsigned sx = cx(q) == 1 ? xm : (cx(q) == -1 ? xp : ix);
+        const unsigned sy = cy(q) == 1 ? ym : (cy(q) == -1 ? yp : iy);
+        const unsigned sz = cz(q) == 1 ? zm : (cz(q) == -1 ? zp : iz);
+        const bool bb = WALLS && ((cy(q) == 1 && wall_lo) || (cy(q) == -1 && wall_hi));
+        const size_t src = bb ? (size_t)opp(q) * N + i : (size_t)q * N + (sx + nx * (sy + ny * sz));
+        g[q] = Pop<S, T>::ld(fin + src, bb ? opp(q) : q);
+    }
+
+    T drho = T(0), jx = T(0), jy = T(0), jz = T(0);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        drho += g[q];
+        if (cx(q)) jx += cx(q) * g[q];
+        if (cy(q)) jy += cy(q) * g[q];
+        if (cz(q)) jz += cz(q) * g[q];
+    }
+    const T rho = T(1) + drho;
+    const T amp = (T)st->amp;
+    T ax = pex * amp, ay = pey * amp, az = pez * amp;
+    if (HASF) {
+        ax += B * F[i];
+        ay += B * F[N + i];
+        az += B * F[2 * N + i];
+    }
+    const T irho = T(1) / rho;
+    // U = (j + G/2)/rho with the Guo force G = 2 rho (P e + B F) (lbm.cu:1116-1118)
+    const T Ux = jx * irho + ax, Uy = jy * irho + ay, Uz = jz * irho + az;
+    const T uu = Ux * Ux + Uy * Uy + Uz * Uz;
+    const T Ua = Ux * ax + Uy * ay + Uz * az;
+    const T kf = (T(1) - om * T(0.5)) * T(2) * rho;
+
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const T cu = (cx(q) ? cx(q) * Ux : T(0)) + (cy(q) ? cy(q) * Uy : T(0)) + (cz(q) ? cz(q) * Uz : T(0));
+        const T ca = (cx(q) ? cx(q) * ax : T(0)) + (cy(q) ? cy(q) * ay : T(0)) + (cz(q) ? cz(q) * az : T(0));
+        const T geq = wq<T>(q) * (drho + rho * (T(3) * cu + T(4.5) * cu * cu - T(1.5) * uu));
+        const T src = wq<T>(q) * kf * (T(3) * ca * (T(1) + T(3) * cu) - T(3) * Ua);
+        const T gs = g[q] + om * (geq - g[q]) + src;
+        fout[(size_t)q * N + i] = Pop<S, T>::st(gs, q);
+    }
+
+    const int iter = st->iter;
+    const bool reset = ((iter - 1) % m) == 0, last = (iter % m) == 0;
+    const T ux = Ux * inv_cr, uy = Uy * inv_cr, uz = Uz * inv_cr;
+    if (m == 1) {
+        U[i] = ux;
+        U[N + i] = uy;
+        U[2 * N + i] = uz;
+    } else {
+        // running sum reset at (iter-1)%M==0 and divided by M at iter%M==0
+        // (lbm.cu:1097-1102, 1120-1127, 1455-1457)
+        T a0 = reset ? ux : U[i] + ux, a1 = reset ? uy : U[N + i] + uy, a2 = reset ? uz : U[2 * N + i] + uz;
+        if (last) { a0 /= (T)m; a1 /= (T)m; a2 /= (T)m; }
+        U[i] = a0;
+        U[N + i] = a1;
+        U[2 * N + i] = a2;
+    }
+    if (MACROS) {
+        rho_out[i] = rho;
+        Ui[i] = ux;
+        Ui[N + i] = uy;
+        Ui[2 * N + i] = uz;
+        rhom[i] = reset ? rho : rhom[i] + rho;  // dm is never divided (lbm.cu:1455-1457)
+    }
+}
+
+template <typename S, typename T>
+__global__ void k_from_fstar(const double *__restrict__ fs, S *__restrict__ f, unsigned N) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[(size_t)q * N + i] = Pop<S, T>::st((T)(fs[(size_t)q * N + i] - wq<double>(q)), q);
+}
+
+template <typename S, typename T>
+__global__ void k_to_fstar(const S *__restrict__ f, double *__restrict__ fs, unsigned N) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+#pragma unroll
+    for (int q = 0; q < Q; q++) fs[(size_t)q * N + i] = (double)Pop<S, T>::ld(f + (size_t)q * N + i, q) + wq<double>(q);
+}
+
+template <typename V> __global__ void k_fill(V *__restrict__ p, size_t n, V v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename S, typename T> lbmibm_status step_impl(lbmibm_ctx *ctx) {
+    const lbmibm_params &p = ctx->p;
+    LbmDims d;
+    d.N = (unsigned)ctx->N;
+    d.nx = ctx->nx; d.ny = ctx->ny; d.nz = ctx->nz;
+    d.nblocks = (int)((ctx->N + WG - 1) / WG);
+    const int grid = xcd_grid(d.nblocks);
+    const double om = 1.0 / (3.0 * p.df + 0.5);  // lbm.cu:1131 with vf = 0
+    const S *fin = (const S *)ctx->f[ctx->cur];
+    S *fout = (S *)ctx->f[ctx->cur ^ 1];
+    const bool hasf = !ctx->force_zero, macros = p.store_macros != 0, walls = p.periodic_y == 0;
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, ctx->stream, fin, fout, d, (const T *)ctx->F,
+                           (T *)ctx->U, (T *)ctx->Ui, (T *)ctx->rho, (T *)ctx->rhom, (T)om, (T)(1.0 / p.cr),
+                           (T)(p.p * p.ex), (T)(p.p * p.ey), (T)(p.p * p.ez), (T)p.b, ctx->step, p.m);
+    };
+#define LBM_CASE(HF, MC, WL) \
+    if (hasf == HF && macros == MC && walls == WL) launch(k_lbm<S, T, HF, MC, WL>);
+    LBM_CASE(true, true, true) LBM_CASE(true, true, false) LBM_CASE(true, false, true) LBM_CASE(true, false, false)
+    LBM_CASE(false, true, true) LBM_CASE(false, true, false) LBM_CASE(false, false, true) LBM_CASE(false, false, false)
+#undef LBM_CASE
+    LBMIBM_LAUNCHED(ctx, "k_lbm");
+    ctx->cur ^= 1;  // pointer swap of lbm.cu:1575
+    return LBMIBM_OK;
+}
+
+template <typename S, typename T> lbmibm_status from_fstar_impl(lbmibm_ctx *ctx, const double *dev_fs) {
+    const unsigned N = (unsigned)ctx->N;
+    hipLaunchKernelGGL((k_from_fstar<S, T>), dim3((N + 255) / 256), dim3(256), 0, ctx->stream, dev_fs,
+                       (S *)ctx->f[ctx->cur], N);
+    LBMIBM_LAUNCHED(ctx, "k_from_fstar");
+    return LBMIBM_OK;
+}
+
+template <typename S, typename T> lbmibm_status to_fstar_impl(lbmibm_ctx *ctx, double *dev_fs) {
+    const unsigned N = (unsigned)ctx->N;
+    hipLaunchKernelGGL((k_to_fstar<S, T>), dim3((N + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const S *)ctx->f[ctx->cur], dev_fs, N);
+    LBMIBM_LAUNCHED(ctx, "k_to_fstar");
+    return LBMIBM_OK;
+}
+
+}  // namespace
+
+lbmibm_status lattice_step(lbmibm_ctx *ctx) {
+    switch (ctx->prec) {
+        case LBMIBM_F64: return step_impl<double, double>(ctx);
+        case LBMIBM_F32: return step_impl<float, float>(ctx);
+        case LBMIBM_F16: return step_impl<__half, float>(ctx);
+    }
+    return LBMIBM_ERR_ARG;
+}
+
+lbmibm_status lattice_set_state_from_fstar(lbmibm_ctx *ctx, const double *host_fs) {
+    double *dev = nullptr;
+    const size_t bytes = sizeof(double) * Q * (size_t)ctx->N;
+    LBMIBM_HIP(ctx, hipMalloc(&dev, bytes));
+    lbmibm_status s = LBMIBM_OK;
+    if (hipMemcpy(dev, host_fs, bytes, hipMemcpyHostToDevice) != hipSuccess) s = LBMIBM_ERR_HIP;
+    if (s == LBMIBM_OK) {
+        switch (ctx->prec) {
+            case LBMIBM_F64: s = from_fstar_impl<double, double>(ctx, dev); break;
+            case LBMIBM_F32: s = from_fstar_impl<float, float>(ctx, dev); break;
+            case LBMIBM_F16: s = from_fstar_impl<__half, float>(ctx, dev); break;
+        }
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess && s == LBMIBM_OK) s = LBMIBM_ERR_HIP;
+    (void)hipFree(dev);
+    if (s != LBMIBM_OK && ctx->err.empty()) ctx->err = "population upload failed";
+    return s;
+}
+
+lbmibm_status lattice_get_fstar(lbmibm_ctx *ctx, double *host_fs) {
+    double *dev = nullptr;
+    const size_t bytes = sizeof(double) * Q * (size_t)ctx->N;
+    LBMIBM_HIP(ctx, hipMalloc(&dev, bytes));
+    lbmibm_status s = LBMIBM_OK;
+    switch (ctx->prec) {
+        case LBMIBM_F64: s = to_fstar_impl<double, double>(ctx, dev); break;
+        case LBMIBM_F32: s = to_fstar_impl<float, float>(ctx, dev); break;
+        case LBMIBM_F16: s = to_fstar_impl<__half, float>(ctx, dev); break;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess && s == LBMIBM_OK) s = LBMIBM_ERR_HIP;
+    if (s == LBMIBM_OK && hipMemcpy(host_fs, dev, bytes, hipMemcpyDeviceToHost) != hipSuccess) s = LBMIBM_ERR_HIP;
+    (void)hipFree(dev);
+    if (s != LBMIBM_OK && ctx->err.empty()) ctx->err = "population download failed";
+    return s;
+}
+
+lbmibm_status lattice_fill(lbmibm_ctx *ctx, void *ptr, size_t count, double value, bool storage_type) {
+    if (!ptr || count == 0) return LBMIBM_OK;
+    const dim3 grid((unsigned)((count + 255) / 256));
+    const size_t sz = storage_type ? ctx->szS : ctx->szT;
+    if (sz == 8) hipLaunchKernelGGL(k_fill<double>, grid, dim3(256), 0, ctx->stream, (double *)ptr, count, value);
+    else if (sz == 4) hipLaunchKernelGGL(k_fill<float>, grid, dim3(256), 0, ctx->stream, (float *)ptr, count, (float)value);
+    else hipLaunchKernelGGL(k_fill<__half>, grid, dim3(256), 0, ctx->stream, (__half *)ptr, count, __float2half((float)value));
+    LBMIBM_LAUNCHED(ctx, "k_fill");
+    return LBMIBM_OK;
+}
+
+}  // namespace lbmibm

@@ -1,0 +1,447 @@
+// Skalak membrane forces on gfx950: stress() = calc_tension + calc_load
+// (per-cell consistent-mass system, BiCGSTAB) + q2f + fourpoint_bend
+// (stress.cu:29-659, matrix_solver.cu:225-377, dot_product.cu:192-223).
+//
+// The reference assembles, per cell and one cell at a time, an ELL matrix of
+// 3V rows and runs a host-driven BiCGSTAB with a blocking device-to-host copy
+// per inner product.  Here:
+//   - k_tension: one thread per element (all cells in one launch);
+//   - k_assemble: one thread per vertex writes the 7 Jacobi-scaled row
+//     coefficients once (x, y and z rows share them, stress.cu:164-171) and
+//     the three scaled right-hand sides;
+//   - k_bicgstab: one work group per cell runs the reference's BiCGSTAB
+//     iteration to convergence on the device (FP64 inner products reduced in
+//     a fixed order, device-side convergence and breakdown flags), so all
+//     cells of any mesh size go in one launch and nothing returns to the host;
+//   - k_q2f and k_bend as the reference.
+// Vertex positions, the reference metric and all geometric arithmetic
+// (edges, metric invariants, tension, matrix entries, ring areas, hinge
+// angles) are FP64 in every precision mode: the strains are small
+// differences of positions, and with C = 100 the area term turns FP32
+// rounding of |x| ~ 10-100 positions into percent-level force errors.  The
+// stored tension, matrix rows, right-hand sides, BiCGSTAB vectors and nodal
+// forces use the arithmetic type T; inner products are FP64.
+#include "common.hpp"
+
+namespace lbmibm {
+namespace {
+
+struct PeriodicBox {
+    double lx, ly, lz;
+    int py;
+};
+
+template <typename T> __device__ __forceinline__ void min_img3(T v[3], const PeriodicBox &b) {
+    v[0] = dev_min_image<T>(v[0], (T)b.lx);
+    v[2] = dev_min_image<T>(v[2], (T)b.lz);
+    if (b.py) v[1] = dev_min_image<T>(v[1], (T)b.ly);
+}
+
+struct TensionArgs {
+    double c, rgs, wtp;  // wtp = M / FILEOUTPUT
+    int neo, fileoutput, m;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tension(const double *__restrict__ xd, const int *__restrict__ ele,
+                                                 const double *__restrict__ nref, T *__restrict__ tt,
+                                                 T *__restrict__ rg, T *__restrict__ tp, int ne,
+                                                 PeriodicBox box, TensionArgs a, const DevStep *__restrict__ st) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ne) return;
+    const int ea = ele[3 * j], eb = ele[3 * j + 1], ec = ele[3 * j + 2];
+    double v1[3], v2[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        v1[d] = xd[3 * eb + d] - xd[3 * ea + d];
+        v2[d] = xd[3 * ec + d] - xd[3 * ea + d];
+    }
+    min_img3<double>(v1, box);
+    min_img3<double>(v2, box);
+    const double n0 = nref[4 * j], n1 = nref[4 * j + 1], n2 = nref[4 * j + 2], n3 = nref[4 * j + 3];
+    const double l1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+    const double l2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+    const double i12 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+    const double dv = (l1 * l1) * (l2 * l2) - i12 * i12;
+    const double dn = n0 * n3 - n1 * n2;
+    rg[j] = (T)sqrt(dv);
+    const double g0 = l1 * l1, g1 = i12, g3 = l2 * l2;
+    const double nd0 = g3 / dv, nd1 = -g1 / dv, nd3 = g0 / dv;
+    const double i1 = n0 * g0 + n1 * g1 + n2 * g1 + n3 * g3 - 2.0;
+    const double i2 = dv * dn - 1.0;
+    const double js = sqrt(i2 + 1.0);
+    const double nn[4] = {n0, n1, n2, n3}, nd[4] = {nd0, nd1, nd1, nd3};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double t;
+        if (!a.neo) t = a.rgs * (i1 + 1.0) / js * nn[k] + (a.c * i2 - 1.0) * js * nd[k];
+        else t = a.rgs * (nn[k] / js - nd[k] * js / (i2 + 1.0) / (i2 + 1.0));
+        tt[4 * j + k] = (T)t;
+    }
+    if (tp) {  // principal-tension accumulator for output (stress.cu:101-118)
+        const int iter = st->iter;
+        double t0 = 0.0, t1 = 0.0;
+        if ((iter - 1) % a.fileoutput != 0) { t0 = (double)tp[2 * j]; t1 = (double)tp[2 * j + 1]; }
+        const double judge = fabs((i1 + 2.0) * (i1 + 2.0) - 4.0 * (i2 + 1.0));
+        const double root = sqrt(judge);
+        const double l12 = 0.5 * (i1 + 2.0 + root), l22 = 0.5 * (i1 + 2.0 - root);
+        const double l11 = sqrt(l12), l21 = sqrt(l22);
+        tp[2 * j] = (T)(t0 + a.wtp * l11 / l21 * (l12 - 1.0 + a.c * l22 * i2));
+        tp[2 * j + 1] = (T)(t1 + a.wtp * l21 / l11 * (l22 - 1.0 + a.c * l12 * i2));
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ xd, const int *__restrict__ ele,
+                                                  const int *__restrict__ nln, const int *__restrict__ lo8,
+                                                  const int *__restrict__ le, const T *__restrict__ rg,
+                                                  const T *__restrict__ tt, T *__restrict__ coef,
+                                                  T *__restrict__ bv, int nv, PeriodicBox box) {
+    const int jj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jj >= nv) return;
+    const int n = nln[jj];
+    int lo[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) lo[k] = lo8[8 * jj + k];
+    double tv[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double tb[3] = {0.0, 0.0, 0.0};
+    int order = 0;
+    for (int i = 0; i < n; i++) {
+        const int e = le[6 * jj + i];
+        const double gab = (double)rg[e];
+        const int ea = ele[3 * e], eb = ele[3 * e + 1], ec = ele[3 * e + 2];
+        int oa = 0, ob = 0, oc = 0;
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            if (k <= n) {
+                if (ea == lo[k]) oa = k;
+                if (eb == lo[k]) ob = k;
+                if (ec == lo[k]) oc = k;
+                if (jj == lo[k]) order = k;
+            }
+        }
+        // consistent mass matrix rg/12 on the diagonal, rg/24 off it (stress.cu:237-242)
+        const double wa = ea == jj ? gab / 12.0 : gab / 24.0;
+        const double wb = eb == jj ? gab / 12.0 : gab / 24.0;
+        const double wc = ec == jj ? gab / 12.0 : gab / 24.0;
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            if (k == oa) tv[k] += wa;
+            if (k == ob) tv[k] += wb;
+            if (k == oc) tv[k] += wc;
+        }
+        const int en = ea == jj ? 0 : (eb == jj ? 1 : 2);
+        // dN/dxi of the linear triangle: a(-1,-1), b(1,0), c(0,1)
+        const double f0 = en == 0 ? -1.0 : (en == 1 ? 1.0 : 0.0);
+        const double f1 = en == 0 ? -1.0 : (en == 1 ? 0.0 : 1.0);
+        const double t0 = tt[4 * e], t1 = tt[4 * e + 1], t2 = tt[4 * e + 2], t3 = tt[4 * e + 3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double vd0 = xd[3 * eb + d] - xd[3 * ea + d];
+            double vd1 = xd[3 * ec + d] - xd[3 * ea + d];
+            if (d == 0) { vd0 = dev_min_image<double>(vd0, box.lx); vd1 = dev_min_image<double>(vd1, box.lx); }
+            else if (d == 2) { vd0 = dev_min_image<double>(vd0, box.lz); vd1 = dev_min_image<double>(vd1, box.lz); }
+            else if (box.py) { vd0 = dev_min_image<double>(vd0, box.ly); vd1 = dev_min_image<double>(vd1, box.ly); }
+            // sum over (alpha,beta) of t^{ab} (vd_a dN_b + vd_b dN_a)/4 (stress.cu:256-260)
+            double s = tb[d];
+            s += gab * t0 * (vd0 * f0 + vd0 * f0) / 4.0;
+            s += gab * t1 * (vd0 * f1 + vd1 * f0) / 4.0;
+            s += gab * t2 * (vd1 * f0 + vd0 * f1) / 4.0;
+            s += gab * t3 * (vd1 * f1 + vd1 * f1) / 4.0;
+            tb[d] = s;
+        }
+    }
+    double jac = tv[0];
+#pragma unroll
+    for (int k = 1; k < 7; k++)
+        if (k == order) jac = tv[k];
+#pragma unroll
+    for (int k = 0; k < 7; k++) coef[8 * (size_t)jj + k] = (T)(tv[k] / jac);
+    coef[8 * (size_t)jj + 7] = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
+}
+
+// y = A x for the vertices of one cell; returns nothing, writes y[3v+d].
+template <typename T>
+__device__ __forceinline__ void row_apply(const T *__restrict__ coef, const int *__restrict__ lo8,
+                                          const int *__restrict__ nln, const T *__restrict__ x, int v, T y[3]) {
+    const int n = nln[v];
+    y[0] = y[1] = y[2] = T(0);
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        if (k <= n) {
+            const T c = coef[8 * (size_t)v + k];
+            const int col = lo8[8 * v + k];
+            y[0] += c * x[3 * (size_t)col];
+            y[1] += c * x[3 * (size_t)col + 1];
+            y[2] += c * x[3 * (size_t)col + 2];
+        }
+    }
+}
+
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) { sh[w] = a; sh[4 + w] = b; }
+    __syncthreads();
+    a = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    b = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+}
+
+// One work group per cell; matrix_solver.cu:279-377 in the same update order,
+// with the next iteration's (r, r^) fused into the residual reduction.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bicgstab(const int *__restrict__ voff, const T *__restrict__ coef,
+                                                  const int *__restrict__ lo8, const int *__restrict__ nln,
+                                                  const T *__restrict__ bv, T *__restrict__ x, T *__restrict__ r,
+                                                  T *__restrict__ rh, T *__restrict__ p, T *__restrict__ v,
+                                                  T *__restrict__ t, int itrmax, double residual,
+                                                  int *__restrict__ iters, DevStatus *__restrict__ status) {
+    __shared__ double sh[8];
+    const int c = blockIdx.x;
+    const int v0 = voff[c], v1 = voff[c + 1];
+    const int tid = threadIdx.x;
+    double pb = 0.0;
+    for (int vv = v0 + tid; vv < v1; vv += blockDim.x)
+        for (int d = 0; d < 3; d++) {
+            const size_t k = 3 * (size_t)vv + d;
+            const T b = bv[k];
+            x[k] = T(0);
+            r[k] = b;
+            rh[k] = b;
+            p[k] = b;
+            pb += (double)b * (double)b;
+        }
+    const double bb = block_sum(pb, sh);  // also orders the writes above
+    if (bb == 0.0) {  // exactly zero load: x = 0 solves the system
+        if (tid == 0) iters[c] = 0;
+        return;
+    }
+    double rr = bb, rr_old = 0.0, alfa = 1.0, omeg = 1.0;
+    int it = 0, flag = 0;
+    bool conv = false;
+    for (it = 0; it < itrmax; it++) {
+        if (rr == 0.0) { flag = 1; break; }
+        if (it > 0) {
+            const double beta = (rr / rr_old) * (alfa / omeg);
+            for (int vv = v0 + tid; vv < v1; vv += blockDim.x)
+                for (int d = 0; d < 3; d++) {
+                    const size_t k = 3 * (size_t)vv + d;
+                    p[k] = (T)((double)r[k] + beta * ((double)p[k] - omeg * (double)v[k]));
+                }
+            __syncthreads();
+        }
+        // v = A p, alfa = (r, r^)/(v, r^)
+        double pv = 0.0;
+        for (int vv = v0 + tid; vv < v1; vv += blockDim.x) {
+            T y[3];
+            row_apply<T>(coef, lo8, nln, p, vv, y);
+            for (int d = 0; d < 3; d++) {
+                const size_t k = 3 * (size_t)vv + d;
+                v[k] = y[d];
+                pv += (double)y[d] * (double)rh[k];
+            }
+        }
+        alfa = rr / block_sum(pv, sh);
+        for (int vv = v0 + tid; vv < v1; vv += blockDim.x)
+            for (int d = 0; d < 3; d++) {
+                const size_t k = 3 * (size_t)vv + d;
+                r[k] = (T)((double)r[k] - alfa * (double)v[k]);
+            }
+        __syncthreads();
+        // t = A r, omeg = (r, t)/(t, t)
+        double prt = 0.0, ptt = 0.0;
+        for (int vv = v0 + tid; vv < v1; vv += blockDim.x) {
+            T y[3];
+            row_apply<T>(coef, lo8, nln, r, vv, y);
+            for (int d = 0; d < 3; d++) {
+                const size_t k = 3 * (size_t)vv + d;
+                t[k] = y[d];
+                prt += (double)r[k] * (double)y[d];
+                ptt += (double)y[d] * (double)y[d];
+            }
+        }
+        block_sum2(prt, ptt, sh);
+        omeg = prt / ptt;
+        // x += alfa p + omeg r ; r -= omeg t ; err = sqrt((r,r)/(b,b))
+        double prr = 0.0, prh = 0.0;
+        for (int vv = v0 + tid; vv < v1; vv += blockDim.x)
+            for (int d = 0; d < 3; d++) {
+                const size_t k = 3 * (size_t)vv + d;
+                const double ri = (double)r[k];
+                x[k] = (T)((double)x[k] + alfa * (double)p[k] + omeg * ri);
+                const T rn = (T)(ri - omeg * (double)t[k]);
+                r[k] = rn;
+                prr += (double)rn * (double)rn;
+                prh += (double)rn * (double)rh[k];
+            }
+        block_sum2(prr, prh, sh);
+        rr_old = rr;
+        rr = prh;
+        if (sqrt(prr / bb) < residual) { conv = true; break; }
+    }
+    if (tid == 0) {
+        iters[c] = conv ? it + 1 : it;
+        if (!conv && flag == 0) flag = 2;
+        if (flag) atomicMax(&status->bicg_flag, flag);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_q2f(const double *__restrict__ xd, const T *__restrict__ q,
+                                             const int *__restrict__ nln, const int *__restrict__ ln,
+                                             const int *__restrict__ vcell, const T *__restrict__ gscale,
+                                             T *__restrict__ cf, int nv, PeriodicBox box) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nv) return;
+    const double x1 = xd[3 * j], y1 = xd[3 * j + 1], z1 = xd[3 * j + 2];
+    const int n = nln[j];
+    double ds = 0.0;
+    for (int k = 0; k < n; k++) {
+        const int j2 = ln[7 * j + k], j3 = ln[7 * j + (k + 1) % n];
+        double a[3] = {xd[3 * j2] - x1, xd[3 * j2 + 1] - y1, xd[3 * j2 + 2] - z1};
+        double b[3] = {xd[3 * j3] - x1, xd[3 * j3 + 1] - y1, xd[3 * j3 + 2] - z1};
+        min_img3<double>(a, box);
+        min_img3<double>(b, box);
+        const double cxv = -(a[1] * b[2] - a[2] * b[1]);
+        const double cyv = -(a[2] * b[0] - a[0] * b[2]);
+        const double czv = -(a[0] * b[1] - a[1] * b[0]);
+        ds += sqrt(cxv * cxv + cyv * cyv + czv * czv) / 2.0 / 3.0;
+    }
+    const double s = (double)gscale[vcell[j]];
+    cf[j] = (T)(-1.0 * (double)q[3 * j] * ds * s);
+    cf[nv + j] = (T)(-1.0 * (double)q[3 * j + 1] * ds * s);
+    cf[2 * nv + j] = (T)(-1.0 * (double)q[3 * j + 2] * ds * s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bend(const double *__restrict__ xs, const int *__restrict__ ele,
+                                              const int *__restrict__ ec, const double *__restrict__ c0,
+                                              const double *__restrict__ s0, T *__restrict__ cf, int nv, int ne,
+                                              double br, PeriodicBox box) {
+    const int e1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e1 >= ne) return;
+    const int i1 = ele[3 * e1], i2 = ele[3 * e1 + 1], i3 = ele[3 * e1 + 2];
+    for (int k = 0; k < 3; k++) {
+        const int e2 = ec[3 * e1 + k];
+        const int j1 = ele[3 * e2], j2 = ele[3 * e2 + 1], j3 = ele[3 * e2 + 2];
+        int k1, k2, k3, k4;
+        // hinge identification (stress.cu:497-505)
+        if      ((i1==j1 && i2==j2)||(i1==j2 && i2==j1)) { k1 = i3; k2 = i1; k3 = i2; k4 = j3; }
+        else if ((i1==j2 && i2==j3)||(i1==j3 && i2==j2)) { k1 = i3; k2 = i1; k3 = i2; k4 = j1; }
+        else if ((i1==j3 && i2==j1)||(i1==j1 && i2==j3)) { k1 = i3; k2 = i1; k3 = i2; k4 = j2; }
+        else if ((i2==j1 && i3==j2)||(i2==j2 && i3==j1)) { k1 = i1; k2 = i2; k3 = i3; k4 = j3; }
+        else if ((i2==j2 && i3==j3)||(i2==j3 && i3==j2)) { k1 = i1; k2 = i2; k3 = i3; k4 = j1; }
+        else if ((i2==j3 && i3==j1)||(i2==j1 && i3==j3)) { k1 = i1; k2 = i2; k3 = i3; k4 = j2; }
+        else if ((i3==j1 && i1==j2)||(i3==j2 && i1==j1)) { k1 = i2; k2 = i3; k3 = i1; k4 = j3; }
+        else if ((i3==j2 && i1==j3)||(i3==j3 && i1==j2)) { k1 = i2; k2 = i3; k3 = i1; k4 = j1; }
+        else if ((i3==j3 && i1==j1)||(i3==j1 && i1==j3)) { k1 = i2; k2 = i3; k3 = i1; k4 = j2; }
+        else continue;
+        double a1[3], a2[3], a3[3], a4[3];
+        const int ks[4] = {k1, k2, k3, k4};
+        double *as[4] = {a1, a2, a3, a4};
+        for (int m = 0; m < 4; m++)
+            for (int d = 0; d < 3; d++) as[m][d] = xs[(size_t)d * nv + ks[m]];
+        // wrap the hinge vertices next to a1 (stress.cu:513-518)
+        const double Ls[3] = {box.lx, box.ly, box.lz};
+        for (int m = 1; m < 4; m++)
+            for (int d = 0; d < 3; d++) {
+                if (d == 1 && !box.py) continue;
+                if (as[m][d] - a1[d] > Ls[d] * 0.5) as[m][d] -= Ls[d];
+                else if (as[m][d] - a1[d] < -Ls[d] * 0.5) as[m][d] += Ls[d];
+            }
+        double a21[3], a31[3], a34[3], a24[3], a23[3], t1[3], t2[3];
+        for (int d = 0; d < 3; d++) {
+            a21[d] = a2[d] - a1[d]; a31[d] = a3[d] - a1[d]; a34[d] = a3[d] - a4[d];
+            a24[d] = a2[d] - a4[d]; a23[d] = a2[d] - a3[d];
+            t1[d] = (a1[d] + a2[d] + a3[d]) / 3.0; t2[d] = (a4[d] + a2[d] + a3[d]) / 3.0;
+        }
+        const double xi[3] = {a21[1] * a31[2] - a21[2] * a31[1], a21[2] * a31[0] - a21[0] * a31[2],
+                         a21[0] * a31[1] - a21[1] * a31[0]};
+        const double ze[3] = {a34[1] * a24[2] - a34[2] * a24[1], a34[2] * a24[0] - a34[0] * a24[2],
+                         a34[0] * a24[1] - a34[1] * a24[0]};
+        const double lxi = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+        const double lze = sqrt(ze[0] * ze[0] + ze[1] * ze[1] + ze[2] * ze[2]);
+        const double cost = (xi[0] * ze[0] + xi[1] * ze[1] + xi[2] * ze[2]) / lxi / lze;
+        const double cost2 = 1.0 - cost * cost;
+        if (cost2 < 1.0e-6) continue;
+        const double cost0 = c0[3 * e1 + k], sint0 = s0[3 * e1 + k];
+        const double side = (xi[0] - ze[0]) * (t1[0] - t2[0]) + (xi[1] - ze[1]) * (t1[1] - t2[1]) +
+                       (xi[2] - ze[2]) * (t1[2] - t2[2]);
+        const double kb = 2.0 / sqrt(3.0) * br;
+        double sint, beta;
+        if (side >= 0.0) { sint = sqrt(cost2); beta = kb * (sint * cost0 - cost * sint0) / sqrt(cost2); }
+        else { sint = -sqrt(cost2); beta = -kb * (sint * cost0 - cost * sint0) / sqrt(cost2); }
+        const double b11 = -beta * cost / (xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+        const double b12 = beta / lxi / lze;
+        const double b22 = -beta * cost / (ze[0] * ze[0] + ze[1] * ze[1] + ze[2] * ze[2]);
+        double f[3];
+        f[0] = b11 * (xi[1] * -a23[2] - xi[2] * -a23[1]) + b12 * (ze[1] * -a23[2] - ze[2] * -a23[1]);
+        f[1] = b11 * (xi[2] * -a23[0] - xi[0] * -a23[2]) + b12 * (ze[2] * -a23[0] - ze[0] * -a23[2]);
+        f[2] = b11 * (xi[0] * -a23[1] - xi[1] * -a23[0]) + b12 * (ze[0] * -a23[1] - ze[1] * -a23[0]);
+        for (int d = 0; d < 3; d++) unsafeAtomicAdd(cf + (size_t)d * nv + k1, (T)f[d]);
+        f[0] = b11 * (xi[1] * -a31[2] - xi[2] * -a31[1]) + b22 * (ze[1] * a34[2] - ze[2] * a34[1])
+             + b12 * ((xi[1] * a34[2] - xi[2] * a34[1]) + (ze[1] * -a31[2] - ze[2] * -a31[1]));
+        f[1] = b11 * (xi[2] * -a31[0] - xi[0] * -a31[2]) + b22 * (ze[2] * a34[0] - ze[0] * a34[2])
+             + b12 * ((xi[2] * a34[0] - xi[0] * a34[2]) + (ze[2] * -a31[0] - ze[0] * -a31[2]));
+        f[2] = b11 * (xi[0] * -a31[1] - xi[1] * -a31[0]) + b22 * (ze[0] * a34[1] - ze[1] * a34[0])
+             + b12 * ((xi[0] * a34[1] - xi[1] * a34[0]) + (ze[0] * -a31[1] - ze[1] * -a31[0]));
+        for (int d = 0; d < 3; d++) unsafeAtomicAdd(cf + (size_t)d * nv + k2, (T)(0.5 * f[d]));
+        f[0] = b11 * (xi[1] * a21[2] - xi[2] * a21[1]) + b22 * (ze[1] * -a24[2] - ze[2] * -a24[1])
+             + b12 * ((xi[1] * -a24[2] - xi[2] * -a24[1]) + (ze[1] * a21[2] - ze[2] * a21[1]));
+        f[1] = b11 * (xi[2] * a21[0] - xi[0] * a21[2]) + b22 * (ze[2] * -a24[0] - ze[0] * -a24[2])
+             + b12 * ((xi[2] * -a24[0] - xi[0] * -a24[2]) + (ze[2] * a21[0] - ze[0] * a21[2]));
+        f[2] = b11 * (xi[0] * a21[1] - xi[1] * a21[0]) + b22 * (ze[0] * -a24[1] - ze[1] * -a24[0])
+             + b12 * ((xi[0] * -a24[1] - xi[1] * -a24[0]) + (ze[0] * a21[1] - ze[1] * a21[0]));
+        for (int d = 0; d < 3; d++) unsafeAtomicAdd(cf + (size_t)d * nv + k3, (T)(0.5 * f[d]));
+    }
+}
+
+template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
+    const lbmibm_params &p = ctx->p;
+    PeriodicBox box{ctx->nx * p.dx, ctx->ny * p.dx, ctx->nz * p.dx, p.periodic_y};
+    TensionArgs ta;
+    ta.c = p.c; ta.rgs = p.rgs; ta.neo = p.neohookean; ta.m = p.m;
+    ta.fileoutput = p.fileoutput > 0 ? p.fileoutput : 1;
+    ta.wtp = (double)p.m / (double)ta.fileoutput;
+    const int ne = ctx->nt, nv = ctx->nv;
+    hipLaunchKernelGGL(k_tension<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
+                       ctx->ele, (const double *)ctx->nref, (T *)ctx->tt, (T *)ctx->rg,
+                       p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ne, box, ta, ctx->step);
+    LBMIBM_LAUNCHED(ctx, "k_tension");
+    hipLaunchKernelGGL(k_assemble<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
+                       ctx->ele, ctx->nln, ctx->lo8, ctx->le, (const T *)ctx->rg, (const T *)ctx->tt,
+                       (T *)ctx->coef, (T *)ctx->bv, nv, box);
+    LBMIBM_LAUNCHED(ctx, "k_assemble");
+    hipLaunchKernelGGL(k_bicgstab<T>, dim3(ctx->nc), dim3(256), 0, ctx->stream, ctx->voff, (const T *)ctx->coef,
+                       ctx->lo8, ctx->nln, (const T *)ctx->bv, (T *)ctx->qv, (T *)ctx->rv, (T *)ctx->rh,
+                       (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, p.itrmax, p.residual, ctx->bicg_iters,
+                       ctx->status);
+    LBMIBM_LAUNCHED(ctx, "k_bicgstab");
+    hipLaunchKernelGGL(k_q2f<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
+                       (const T *)ctx->qv, ctx->nln, ctx->ln, ctx->vcell, (const T *)ctx->gscale, (T *)ctx->cf,
+                       nv, box);
+    LBMIBM_LAUNCHED(ctx, "k_q2f");
+    if (p.br != 0.0) {
+        hipLaunchKernelGGL(k_bend<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
+                           ctx->ele, ctx->ec, (const double *)ctx->c0, (const double *)ctx->s0, (T *)ctx->cf, nv, ne, p.br,
+                           box);
+        LBMIBM_LAUNCHED(ctx, "k_bend");
+    }
+    return LBMIBM_OK;
+}
+
+}  // namespace
+
+lbmibm_status membrane_forces(lbmibm_ctx *ctx) {
+    if (ctx->nc == 0) return LBMIBM_OK;
+    return ctx->prec == LBMIBM_F64 ? forces_impl<double>(ctx) : forces_impl<float>(ctx);
+}
+
+}  // namespace lbmibm

@@ -1,0 +1,196 @@
+"""Host-side mirror of the reference's driver for the coupled timestep.
+
+``Simulation`` owns one ``lbmibm_ctx`` and exposes the reference's host
+functions under their own names (``stress``, ``ibm_st1``, ``lbm_d3q19``,
+``ibm_st2``, ``volume_init``, ``volume_correction``; stress.cu:580,
+ibm.cu:299, lbm.cu:1463, ibm.cu:642, volume_correction.cu:427/453) so a
+Python driver reads like main.cu:73-110.  Arrays crossing this boundary use
+the reference layouts (padded lattice, AoS populations, SoA/AoS vertices).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native as N
+
+Q = 19
+PRECISIONS = {"f64": N.F64, "f32": N.F32, "f16": N.F16}
+
+
+def make_params(lx, ly, lz, *, precision="f32", tau=None, df=None, dx=1.0, cr=1.0, uc=1.0, p=0.0,
+                e=(1.0, 0.0, 0.0), force_modulated=False, flowstart=0, freq=1.0, b=0.0, m=1, c=100.0,
+                rgs=1.0, br=0.0, itrmax=1000, residual=1e-10, runge_kutta=4, neohookean=False,
+                volume_correction=True, store_macros=False, fileoutput=0, periodic_y=False, lambda1=1.0):
+    P = N.Params()
+    N.lib().lbmibm_default_params(C.byref(P), int(lx), int(ly), int(lz))
+    if df is None:
+        df = (2.0 * (0.8 if tau is None else tau) - 1.0) / 6.0  # tau = (6Df+1)/2 (init1.cu:140)
+    P.periodic_y = int(periodic_y)
+    P.dx, P.cr = float(dx), float(cr)
+    P.dt = cr * dx / uc  # init1.cu:139
+    P.df, P.lambda1, P.p = float(df), float(lambda1), float(p)
+    P.ex, P.ey, P.ez = map(float, e)
+    P.force_modulated, P.flowstart, P.freq = int(force_modulated), int(flowstart), float(freq)
+    P.b, P.m, P.c, P.rgs, P.br = float(b), int(m), float(c), float(rgs), float(br)
+    P.itrmax, P.residual, P.runge_kutta = int(itrmax), float(residual), int(runge_kutta)
+    P.neohookean, P.volume_correction = int(neohookean), int(volume_correction)
+    P.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    P.store_macros, P.fileoutput = int(store_macros), int(fileoutput)
+    return P
+
+
+class Simulation:
+    def __init__(self, lx, ly, lz, cells=None, **kw):
+        self.L = N.lib()
+        self.P = make_params(lx, ly, lz, **kw)
+        self.lx, self.ly, self.lz = int(lx), int(ly), int(lz)
+        self.nxp, self.nyp, self.nzp = self.lx + 4, self.ly + 4, self.lz + 4
+        self.np_ = self.nxp * self.nyp * self.nzp
+        self._keep = []
+        cp = None
+        if cells is not None and len(cells["voff"]) > 1:
+            arrs = dict(voff=np.ascontiguousarray(cells["voff"], np.int32),
+                        eoff=np.ascontiguousarray(cells["eoff"], np.int32),
+                        x=np.ascontiguousarray(cells["x"], np.float64), y=np.ascontiguousarray(cells["y"], np.float64),
+                        z=np.ascontiguousarray(cells["z"], np.float64), ele=np.ascontiguousarray(cells["ele"], np.int32))
+            self._keep.append(arrs)
+            cp = N.Cells(len(arrs["voff"]) - 1, N.ip(arrs["voff"]), N.ip(arrs["eoff"]), N.dp(arrs["x"]),
+                         N.dp(arrs["y"]), N.dp(arrs["z"]), N.ip(arrs["ele"]))
+        h = C.c_void_p()
+        code = self.L.lbmibm_create(C.byref(self.P), C.byref(cp) if cp is not None else None, C.byref(h))
+        if code != 0:
+            raise N.LbmibmError(code, "lbmibm_create failed")
+        self.ctx = h
+        n = C.c_longlong(0)
+        nv, nt, nc = C.c_int(0), C.c_int(0), C.c_int(0)
+        self.L.lbmibm_sizes(self.ctx, C.byref(n), C.byref(nv), C.byref(nt), C.byref(nc))
+        self.nv, self.nt, self.nc = nv.value, nt.value, nc.value
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.lbmibm_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, code):
+        N.check(code, self.ctx)
+
+    # ---- the reference's host functions --------------------------------
+    def lbm_d3q19(self, it):
+        self._ck(self.L.lbmibm_lbm_d3q19(self.ctx, int(it)))
+
+    def ibm_st1(self):
+        self._ck(self.L.lbmibm_ibm_st1(self.ctx))
+
+    def ibm_st2(self):
+        self._ck(self.L.lbmibm_ibm_st2(self.ctx))
+
+    def stress(self, it):
+        self._ck(self.L.lbmibm_stress(self.ctx, int(it)))
+
+    def volume_init(self):
+        self._ck(self.L.lbmibm_volume_init(self.ctx))
+
+    def volume_correction(self):
+        k = C.c_int(0)
+        self._ck(self.L.lbmibm_volume_correction(self.ctx, C.byref(k)))
+        return k.value
+
+    def step(self, it):
+        self._ck(self.L.lbmibm_step(self.ctx, int(it)))
+
+    def run(self, iter0, nsteps, graph=True):
+        self._ck(self.L.lbmibm_run(self.ctx, int(iter0), int(nsteps), int(bool(graph))))
+
+    STAGES = ("stress", "ibm_st1", "lbm_d3q19", "ibm_st2", "volume_correction", "step")
+
+    def time_stages(self, iter0, nsteps):
+        """Average ms per call of each stage, timed with HIP events on the library stream."""
+        ms = np.zeros(6)
+        calls = np.zeros(6, np.int32)
+        self._ck(self.L.lbmibm_time_stages(self.ctx, int(iter0), int(nsteps), N.dp(ms), N.ip(calls)))
+        return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(self.STAGES)}
+
+    def synchronize(self):
+        self._ck(self.L.lbmibm_synchronize(self.ctx))
+
+    # ---- reference-layout exchange --------------------------------------
+    def set_equilibrium(self, rho=None, u=None, v=None, w=None):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)
+        arrs = [conv(a) for a in (rho, u, v, w)]
+        for a in arrs:
+            assert a is None or a.size == self.np_
+        self._ck(self.L.lbmibm_set_equilibrium(self.ctx, *[N.dp(a) for a in arrs]))
+
+    def populations(self):
+        fn = np.zeros(self.np_ * Q)
+        self._ck(self.L.lbmibm_download_populations(self.ctx, N.dp(fn)))
+        return fn
+
+    def set_populations(self, fn):
+        fn = np.ascontiguousarray(fn, np.float64).reshape(-1)
+        assert fn.size == self.np_ * Q
+        self._ck(self.L.lbmibm_upload_populations(self.ctx, N.dp(fn)))
+
+    def macros(self, instantaneous=False):
+        names = ["dn", "un", "vn", "wn", "dm", "um", "vm", "wm"]
+        out = {k: np.zeros(self.np_) for k in names}
+        want = names if instantaneous else ["um", "vm", "wm"]
+        self._ck(self.L.lbmibm_download_macros(self.ctx, *[N.dp(out[k]) if k in want else None for k in names]))
+        return {k: out[k] for k in want}
+
+    def set_macros(self, um, vm, wm):
+        a = [np.ascontiguousarray(v, np.float64).reshape(-1) for v in (um, vm, wm)]
+        self._ck(self.L.lbmibm_upload_macros(self.ctx, *[N.dp(v) for v in a]))
+
+    def lattice_force(self):
+        out = [np.zeros(self.np_) for _ in range(3)]
+        self._ck(self.L.lbmibm_download_lattice_force(self.ctx, *[N.dp(v) for v in out]))
+        return out
+
+    def set_lattice_force(self, fx, fy, fz):
+        a = [np.ascontiguousarray(v, np.float64).reshape(-1) for v in (fx, fy, fz)]
+        self._ck(self.L.lbmibm_upload_lattice_force(self.ctx, *[N.dp(v) for v in a]))
+
+    def cells_state(self):
+        nv = self.nv
+        names = ["x", "y", "z", "u", "v", "w", "fx", "fy", "fz"]
+        out = {k: np.zeros(nv) for k in names}
+        out["xd"] = np.zeros(3 * nv)
+        self._ck(self.L.lbmibm_download_cells(self.ctx, *[N.dp(out[k]) for k in names], N.dp(out["xd"])))
+        return out
+
+    def set_cells_state(self, x=None, y=None, z=None, xd=None, fx=None, fy=None, fz=None):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)
+        arrs = [conv(a) for a in (x, y, z, xd, fx, fy, fz)]
+        self._ck(self.L.lbmibm_upload_cells(self.ctx, *[N.dp(a) for a in arrs]))
+
+    def membrane(self):
+        out = dict(q=np.zeros(3 * self.nv), t=np.zeros(4 * self.nt), rg=np.zeros(self.nt),
+                   tp=np.zeros(2 * self.nt), qi=np.zeros(3 * self.nv), bicg_iters=np.zeros(self.nc, np.int32))
+        self._ck(self.L.lbmibm_download_membrane(self.ctx, N.dp(out["q"]), N.dp(out["t"]), N.dp(out["rg"]),
+                                                 N.dp(out["tp"]), N.dp(out["qi"]), N.ip(out["bicg_iters"])))
+        return out
+
+    def volumes(self):
+        out = dict(volume0=np.zeros(self.nc), volume=np.zeros(self.nc), area=np.zeros(self.nc),
+                   gx=np.zeros(3 * self.nc))
+        self._ck(self.L.lbmibm_download_volume(self.ctx, *[N.dp(out[k]) for k in ("volume0", "volume", "area", "gx")]))
+        return out
+
+    def set_cell_force_scale(self, scale=None):
+        s = None if scale is None else np.ascontiguousarray(scale, np.float64)
+        self._ck(self.L.lbmibm_set_cell_force_scale(self.ctx, N.dp(s)))
+
+    def comp_mask(self):
+        """COMP nodes of the padded reference lattice (init1.cu:164-178)."""
+        m = np.zeros((self.nzp, self.nyp, self.nxp), bool)
+        m[2:-2, 2:-2, 2:-2] = True
+        return m.reshape(-1)
