@@ -169,7 +169,9 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->xd, ctx->uvw, ctx->cf, ctx->c0, ctx->s0, ctx->nref, ctx->tt, ctx->rg, ctx->tp, ctx->qv,
                     ctx->qi, ctx->coef, ctx->bv, ctx->rv, ctx->rh, ctx->pv, ctx->vv, ctx->tv, ctx->gscale,
                     ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->bicg_iters, ctx->vol_passes,
-                    ctx->step, ctx->status};
+                    ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
+                    ctx->tile_off, ctx->tile_cur, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->small_cells,
+                    ctx->big_cells};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -336,9 +338,40 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         CK(up_T(&ctx->tp, nullptr, 2 * (size_t)nt));
         for (void **v : {&ctx->qv, &ctx->qi, &ctx->bv, &ctx->rv, &ctx->rh, &ctx->pv, &ctx->vv, &ctx->tv})
             CK(up_T(v, nullptr, 3 * (size_t)nv));
-        CK(up_T(&ctx->coef, nullptr, 8 * (size_t)nv));
+        CK(up_T(&ctx->coef, nullptr, 6 * (size_t)nv));
+        CK(dev_alloc(ctx, (void **)&ctx->nbr, 6 * (size_t)nv * sizeof(unsigned short)));
+        // cells solved on chip and the rest
+        std::vector<int> small, big;
+        const int lim = bicg_onchip_limit(ctx->prec);
+        for (int c = 0; c < nc; c++) {
+            const int V = cells->voff[c + 1] - cells->voff[c];
+            if (V > 65535) return fail(LBMIBM_ERR_ARG);  // 16-bit local column ids
+            if (V <= lim) { small.push_back(c); ctx->max_small_v = std::max(ctx->max_small_v, V); }
+            else big.push_back(c);
+        }
+        ctx->n_small = (int)small.size();
+        ctx->n_big = (int)big.size();
+        if (!small.empty()) CK(up_i(&ctx->small_cells, small.data(), small.size()));
+        if (!big.empty()) CK(up_i(&ctx->big_cells, big.data(), big.size()));
+        CK(membrane_setup(ctx));
         std::vector<double> ones(nc, 1.0);
         CK(up_T(&ctx->gscale, ones.data(), nc));
+        // lattice tiles and vertex bins of the immersed-boundary kernels
+        ctx->ntx = (ctx->nx + TILE_X - 1) / TILE_X;
+        ctx->nty = (ctx->ny + TILE_Y - 1) / TILE_Y;
+        ctx->ntz = (ctx->nz + TILE_Z - 1) / TILE_Z;
+        ctx->ntiles = ctx->ntx * ctx->nty * ctx->ntz;
+        ctx->nblk_bin = (nv + 255) / 256;
+        if (ctx->nx > 30000 || ctx->ny > 29990 || ctx->nz > 30000) return fail(LBMIBM_ERR_ARG);  // short4 bins
+        CK(dev_alloc(ctx, (void **)&ctx->node0, (size_t)nv * sizeof(short4)));
+        CK(dev_alloc(ctx, (void **)&ctx->home, (size_t)nv * sizeof(int)));
+        CK(dev_alloc(ctx, (void **)&ctx->home_list, (size_t)nv * sizeof(int)));
+        CK(dev_alloc(ctx, (void **)&ctx->tile_count, (size_t)ctx->ntiles * sizeof(int)));
+        CK(dev_alloc(ctx, (void **)&ctx->tile_off, (size_t)ctx->ntiles * sizeof(int)));
+        CK(dev_alloc(ctx, (void **)&ctx->tile_cur, (size_t)ctx->ntiles * sizeof(int)));
+        CK(dev_alloc(ctx, (void **)&ctx->blk_fmax, (size_t)ctx->nblk_bin * sizeof(float)));
+        CK(dev_alloc(ctx, (void **)&ctx->absmax, sizeof(unsigned)));
+        CK(ibm_setup(ctx));
         CK(dev_alloc(ctx, (void **)&ctx->normal, 3 * (size_t)nt * sizeof(double)));
         CK(dev_alloc(ctx, (void **)&ctx->vol0, nc * sizeof(double)));
         CK(dev_alloc(ctx, (void **)&ctx->vol, nc * sizeof(double)));
@@ -386,8 +419,9 @@ lbmibm_status lbmibm_lbm_d3q19(lbmibm_ctx *ctx, int iter) {
 
 lbmibm_status lbmibm_ibm_st1(lbmibm_ctx *ctx) {
     if (!ctx) return LBMIBM_ERR_ARG;
-    lbmibm_status s = ibm_spread(ctx);
-    return s == LBMIBM_OK ? sync_check(ctx) : s;
+    lbmibm_status s = reset_status(ctx);
+    if (s == LBMIBM_OK) s = ibm_spread(ctx);
+    return s == LBMIBM_OK ? check_flags(ctx, nullptr) : s;
 }
 
 lbmibm_status lbmibm_ibm_st2(lbmibm_ctx *ctx) {
@@ -737,6 +771,7 @@ lbmibm_status lbmibm_upload_cells(lbmibm_ctx *ctx, const double *x, const double
         std::copy(y, y + nv, a.begin() + nv);
         std::copy(z, z + nv, a.begin() + 2 * nv);
         LBMIBM_HIP(ctx, hipMemcpy(ctx->xs, a.data(), 3 * nv * sizeof(double), hipMemcpyHostToDevice));
+        ctx->bins_valid = false;
     }
     if (xd) LBMIBM_HIP(ctx, hipMemcpy(ctx->xd, xd, 3 * nv * sizeof(double), hipMemcpyHostToDevice));
     if (s == LBMIBM_OK && fx && fy && fz) {

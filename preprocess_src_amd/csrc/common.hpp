@@ -27,7 +27,11 @@ struct DevStep {
 struct DevStatus {
     int bicg_flag;    // max BiCGSTAB flag over cells (1 breakdown, 2 ITRMAX)
     int vol_passes;   // max volume-correction passes over cells
+    int pad0, pad1;
 };
+
+// Lattice tiles of the immersed-boundary kernels.
+constexpr int TILE_X = 16, TILE_Y = 8, TILE_Z = 8;
 
 }  // namespace lbmibm
 
@@ -65,13 +69,26 @@ struct lbmibm_ctx {
     void *c0 = nullptr, *s0 = nullptr, *nref = nullptr;  // FP64
     void *tt = nullptr, *rg = nullptr, *tp = nullptr;
     void *qv = nullptr, *qi = nullptr;                 // [3Nv]
-    void *coef = nullptr;                              // [Nv][8] Jacobi-scaled mass-matrix rows
+    void *coef = nullptr;                              // [6][Nv] off-diagonal Jacobi-scaled coefficients
+    unsigned short *nbr = nullptr;                     // [6][Nv] cell-local neighbour ids (self for unused)
+    int *small_cells = nullptr, *big_cells = nullptr;  // cell ids solved on chip / from global memory
+    int n_small = 0, n_big = 0, max_small_v = 0;
     void *bv = nullptr, *rv = nullptr, *rh = nullptr, *pv = nullptr, *vv = nullptr, *tv = nullptr;
     void *gscale = nullptr;                            // [nc] (T)
     double *normal = nullptr;                          // [3Nt] FP64 element normals
     double *vol0 = nullptr, *vol = nullptr, *area = nullptr, *gx = nullptr;  // [nc], [3nc]
     int *bicg_iters = nullptr, *vol_passes = nullptr;  // [nc]
     int max_cell_v = 0;
+
+    // immersed boundary: vertices binned by the lattice tile of their first
+    // stencil node (rebuilt whenever the positions change)
+    int ntx = 0, nty = 0, ntz = 0, ntiles = 0, nblk_bin = 0;
+    short4 *node0 = nullptr;                 // [Nv] first stencil node (x,z wrapped, y+3)
+    int *home = nullptr, *home_list = nullptr;  // [Nv]
+    int *tile_count = nullptr, *tile_off = nullptr, *tile_cur = nullptr;  // [ntiles]
+    float *blk_fmax = nullptr;               // [nblk_bin] max |f| per binning block
+    unsigned *absmax = nullptr;
+    bool bins_valid = false;                 // bins match the current positions
 
     lbmibm::DevStep *step = nullptr;
     lbmibm::DevStatus *status = nullptr;
@@ -92,7 +109,11 @@ lbmibm_status lattice_get_fstar(lbmibm_ctx *ctx, double *fstar_soa);  // lattice
 lbmibm_status lattice_fill(lbmibm_ctx *ctx, void *ptr, size_t count, double value, bool storage_type);
 lbmibm_status ibm_spread(lbmibm_ctx *ctx);                   // ibm.hip
 lbmibm_status ibm_interp(lbmibm_ctx *ctx);                   // ibm.hip
+lbmibm_status ibm_bin(lbmibm_ctx *ctx);                      // ibm.hip
+lbmibm_status ibm_setup(lbmibm_ctx *ctx);                    // ibm.hip
 lbmibm_status membrane_forces(lbmibm_ctx *ctx);              // membrane.hip
+int bicg_onchip_limit(int prec);                             // membrane.hip
+lbmibm_status membrane_setup(lbmibm_ctx *ctx);               // membrane.hip
 lbmibm_status volume_run(lbmibm_ctx *ctx, int mode);         // volume.hip (0 init, 1 correct)
 lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool advance);  // api.hip
 

@@ -10,10 +10,23 @@
 // ix_p*dx) and then mapped to the unpadded lattice by ix = ix_p - 2 with
 // periodic wrapping, which is what the halo fold achieves; nodes beyond the
 // walls are dropped (spread) or contribute zero velocity (interpolation).
-// The delta factorises, so each vertex evaluates 4 cosines per axis (12)
-// instead of 3 per stencil node (192).  Vertex positions, stencil selection
-// and weights are FP64 in every precision mode; the lattice fields (u, F)
-// and the nodal forces are in the arithmetic type T.
+// Vertex positions and stencil selection are FP64 in every precision mode;
+// weights, lattice fields (u, F) and nodal forces use the arithmetic type T.
+//
+// Both directions work on lattice tiles of 16x8x8 nodes.  After each change
+// of the positions the vertices are binned by the tile of their first stencil
+// node (counting sort with wave-aggregated counters).
+//   - Spreading: one work group per tile takes the vertices of its own bin
+//     and of its 7 lower neighbours, keeps those whose stencil meets the tile
+//     (per-wave queue), and accumulates their contributions into the tile in
+//     LDS, one lane per vertex.  Sums are 64-bit fixed point with a
+//     power-of-two scale fixed by max |f| of the call, so the result does not
+//     depend on the order of the additions (bitwise reproducible) and needs no
+//     global atomics.  The tile then writes all of its nodes, zeros included,
+//     which replaces the zeroing pass of ibm_st1a.
+//   - Interpolation: one work group per tile stages the velocity of the tile
+//     plus a 3/4-node halo in LDS once and runs the four frozen-field RK
+//     stages of its vertices from LDS, one lane per vertex.
 #include "common.hpp"
 
 namespace lbmibm {
@@ -26,100 +39,407 @@ struct IbmGeom {
     double dx;
 };
 
+struct TileGeom {
+    int ntx, nty, ntz, ntiles;
+};
+
 __device__ __forceinline__ bool wrap_index(int &i, int n) {
     if (i < 0) i += n;
     else if (i >= n) i -= n;
     return (unsigned)i < (unsigned)n;
 }
 
-template <typename T>
-__device__ __forceinline__ int stencil(T xtr, T h, int &first, T w[4], bool unit_spacing) {
-    // weights 1 + cos(pi |x_node - xtr| / (2h)) of the 4 nodes from floor(xtr/h)-1
-    const T s = unit_spacing ? T(1) : h;
-    first = (int)floor(xtr / s) - 1;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const T r = fabs((T)(first + k) * s - xtr);
-        w[k] = T(1) + cos(T(M_PI) * r / (T(2) * s));
+__device__ __forceinline__ int pmod(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+__device__ __forceinline__ void sincos_t(float a, float *s, float *c) { sincosf(a, s, c); }
+__device__ __forceinline__ void sincos_t(double a, double *s, double *c) { sincos(a, s, c); }
+
+// Weights 1 + cos(pi |x_node - xtr| / (2h)) of the 4 nodes from
+// floor(xtr/h)-1 (ibm.cu:103-121).  With t = frac(xtr/h) the distances are
+// 1+t, t, 1-t, 2-t, so the four cosines are -sin, cos, sin, -cos of pi t/2:
+// one sincos of an argument in [0, pi/2) per axis, evaluated in W.
+template <typename W>
+__device__ __forceinline__ void stencil(double xtr, double h, int &first, W w[4], bool unit_spacing) {
+    const double q = unit_spacing ? xtr : xtr / h;
+    const double fl = floor(q);
+    first = (int)fl - 1;
+    const W t = (W)(q - fl);
+    W sn, cs;
+    sincos_t(W(M_PI / 2) * t, &sn, &cs);
+    w[0] = W(1) - sn;
+    w[1] = W(1) + cs;
+    w[2] = W(1) + sn;
+    w[3] = W(1) - cs;
+}
+
+// Unpadded first stencil nodes (unwrapped) and weights at a position.
+template <typename W>
+__device__ __forceinline__ void stencil3(const IbmGeom &g, double x, double y, double z, bool unit, int &x1, int &y1,
+                                         int &z1, W wx[4], W wy[4], W wz[4]) {
+    if (!unit) {
+        stencil<W>(x + (double)(g.nxp - 1) * g.dx / 2.0, g.dx, x1, wx, false);
+        stencil<W>(y + (double)(g.nyp - 1) * g.dx / 2.0, g.dx, y1, wy, false);
+        stencil<W>(z + (double)(g.nzp - 1) * g.dx / 2.0, g.dx, z1, wz, false);
+    } else {  // RK2 / Euler branches use unit spacing (ibm.cu:524-551, 581-608)
+        stencil<W>(x + (double)(g.nxp - 1) / 2.0, g.dx, x1, wx, true);
+        stencil<W>(y + (double)(g.nyp - 1) / 2.0, g.dx, y1, wy, true);
+        stencil<W>(z + (double)(g.nzp - 1) / 2.0, g.dx, z1, wz, true);
     }
-    return first;
+    x1 -= 2;
+    y1 -= 2;
+    z1 -= 2;
+}
+
+// ---- binning ---------------------------------------------------------------
+
+// Slot for a lane in counters[key], lanes with equal keys aggregated into one
+// atomic per wave.  All lanes of the wave must call it.
+__device__ __forceinline__ int wave_aggregated_add(int *counters, int key, bool active) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(active);
+    int result = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int lkey = __shfl(key, leader);
+        const unsigned long long grp = __ballot(active && key == lkey) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counters[lkey], __popcll(grp));
+        base = __shfl(base, leader);
+        if ((grp >> lane) & 1ull) result = base + __popcll(grp & ((1ull << lane) - 1ull));
+        pending &= ~grp;
+    }
+    return result;
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_spread(const double *__restrict__ xs, const T *__restrict__ cf,
-                                                T *__restrict__ F, int nv, IbmGeom g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nv) return;
-    const double dx = g.dx;
-    const double X = xs[j], Y = xs[nv + j], Z = xs[2 * (size_t)nv + j];
-    const T fx = cf[j], fy = cf[nv + j], fz = cf[2 * nv + j];
+__global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs, const T *__restrict__ cf, int nv,
+                                                   IbmGeom g, TileGeom tg, short4 *__restrict__ node0,
+                                                   int *__restrict__ home, int *__restrict__ tile_count,
+                                                   float *__restrict__ blk_fmax) {
+    __shared__ float shm[4];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = v < nv;
+    int t = 0;
+    float fm = 0.0f;
+    if (on) {
+        int x1, y1, z1;
+        T wx[4], wy[4], wz[4];
+        stencil3<T>(g, xs[v], xs[nv + v], xs[2 * (size_t)nv + v], false, x1, y1, z1, wx, wy, wz);
+        int xw = x1, zw = z1, yh = y1;
+        wrap_index(xw, g.nx);
+        wrap_index(zw, g.nz);
+        if (g.walls) yh = min(max(yh, 0), g.ny - 1);
+        else wrap_index(yh, g.ny);
+        // invalid positions (NaN or far outside) land in a valid bin; their
+        // stencils meet no tile and are dropped later
+        xw = min(max(xw, 0), g.nx - 1);
+        zw = min(max(zw, 0), g.nz - 1);
+        yh = min(max(yh, 0), g.ny - 1);
+        t = xw / TILE_X + tg.ntx * (yh / TILE_Y + tg.nty * (zw / TILE_Z));
+        node0[v] = make_short4((short)xw, (short)min(max(y1 + 3, -30000), 30000), (short)zw, 0);
+        home[v] = t;
+        fm = fmaxf(fabsf((float)cf[v]), fmaxf(fabsf((float)cf[nv + v]), fabsf((float)cf[2 * (size_t)nv + v])));
+        fm *= 1.0001f;  // margin for the float rounding of |f|
+    }
+    (void)wave_aggregated_add(tile_count, t, on);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) fm = fmaxf(fm, __shfl_xor(fm, o));
+    if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = fm;
+    __syncthreads();
+    if (threadIdx.x == 0) blk_fmax[blockIdx.x] = fmaxf(fmaxf(shm[0], shm[1]), fmaxf(shm[2], shm[3]));
+}
+
+__global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile_count, int *__restrict__ tile_off,
+                                                    int *__restrict__ tile_cur, int ntiles,
+                                                    const float *__restrict__ blk_fmax, int nblk,
+                                                    unsigned *__restrict__ absmax) {
+    // exclusive scan over tiles in one work group, 1024 entries per round,
+    // and the maximum of the per-block force maxima
+    __shared__ int sh[1024];
+    __shared__ float shm[1024];
+    __shared__ int carry;
+    float fm = 0.0f;
+    for (int i = threadIdx.x; i < nblk; i += 1024) fm = fmaxf(fm, blk_fmax[i]);
+    shm[threadIdx.x] = fm;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) shm[threadIdx.x] = fmaxf(shm[threadIdx.x], shm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *absmax = __float_as_uint(shm[0]);
+    for (int base = 0; base < ntiles; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < ntiles ? tile_count[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < ntiles) {
+            tile_off[i] = carry + sh[threadIdx.x] - v;
+            tile_cur[i] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += sh[1023];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ home, int nv,
+                                                     const int *__restrict__ tile_off, int *__restrict__ tile_cur,
+                                                     int *__restrict__ home_list) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = v < nv;
+    const int t = on ? home[v] : 0;
+    const int r = wave_aggregated_add(tile_cur, t, on);
+    if (on) home_list[tile_off[t] + r] = v;
+}
+
+__global__ void k_zero_int(int *__restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---- spreading ---------------------------------------------------------------
+
+// LDS accumulator layout of a 16x8x8 tile with row strides 20 and 176: the
+// 64 (kx,ky,kz) offsets of one stencil fall on 32 distinct 8-byte bank slots.
+constexpr int ACC_SX = 20, ACC_SXY = 176, ACC_N = ACC_SXY * TILE_Z;
+
+// Mask of the 4 stencil nodes first..first+3 (periodic when wrap) that fall
+// in [o, o+len) of an axis of n nodes; bit k for node first+k.
+__device__ __forceinline__ unsigned axis_mask(int first, int o, int len, int n, bool wrap) {
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int a = first + k;
+        if (wrap) {
+            if (a < 0) a += n;
+            else if (a >= n) a -= n;
+        }
+        if (a >= o && a < o + len && a >= 0 && a < n) m |= 1u << k;
+    }
+    return m;
+}
+
+// Tiles along one axis whose nodes can hold the first stencil node of a
+// vertex reaching [o, o+len): the tile itself and the tiles of nodes o-3..o-1
+// (periodic when wrap).  Returns the number of distinct tiles (1..4).
+__device__ __forceinline__ int source_tiles(int t, int o, int n, int tile, bool wrap, int out[4]) {
+    int cnt = 1;
+    out[0] = t;
+    for (int d = 1; d <= 3; d++) {
+        int a = o - d;
+        if (a < 0) {
+            if (!wrap) break;
+            a = ((a % n) + n) % n;
+        }
+        const int s = a / tile;
+        bool seen = false;
+        for (int i = 0; i < cnt; i++) seen |= out[i] == s;
+        if (!seen) out[cnt++] = s;
+    }
+    return cnt;
+}
+
+template <typename T>
+__device__ __forceinline__ void spread_vertex(const double *__restrict__ xs, const T *__restrict__ cf, int nv, int v,
+                                              const IbmGeom &g, int ox, int oy, int oz, int lx, int ly, int lz,
+                                              double scale, T w64, unsigned long long *acc) {
     int x1, y1, z1;
-    double wx[4], wy[4], wz[4];
-    stencil<double>(X + (double)(g.nxp - 1) * dx / 2.0, dx, x1, wx, false);
-    stencil<double>(Y + (double)(g.nyp - 1) * dx / 2.0, dx, y1, wy, false);
-    stencil<double>(Z + (double)(g.nzp - 1) * dx / 2.0, dx, z1, wz, false);
-    const double inv = 1.0 / (64.0 * dx * dx * dx);
-    const size_t N = (size_t)g.nx * g.ny * g.nz;
-#pragma unroll
+    T wx[4], wy[4], wz[4];
+    stencil3<T>(g, xs[v], xs[nv + v], xs[2 * (size_t)nv + v], false, x1, y1, z1, wx, wy, wz);
+    const unsigned mx = axis_mask(x1, ox, lx, g.nx, true);
+    const unsigned my = axis_mask(y1, oy, ly, g.ny, !g.walls);
+    const unsigned mz = axis_mask(z1, oz, lz, g.nz, true);
+    if (!mx || !my || !mz) return;
+    const T f0 = cf[v], f1 = cf[nv + v], f2 = cf[2 * (size_t)nv + v];
     for (int kz = 0; kz < 4; kz++) {
-        int iz = z1 + kz - 2;
-        if (!wrap_index(iz, g.nz)) continue;
-#pragma unroll
+        if (!((mz >> kz) & 1u)) continue;
+        int c = z1 + kz;
+        wrap_index(c, g.nz);
         for (int ky = 0; ky < 4; ky++) {
-            int iy = y1 + ky - 2;
-            if (g.walls) {
-                if (iy < 0 || iy >= g.ny) continue;
-            } else if (!wrap_index(iy, g.ny)) continue;
-            const double wyz = wy[ky] * wz[kz];
-#pragma unroll
+            if (!((my >> ky) & 1u)) continue;
+            int b = y1 + ky;
+            if (!g.walls) wrap_index(b, g.ny);
+            const T wyz = wy[ky] * wz[kz] * w64;
             for (int kx = 0; kx < 4; kx++) {
-                int ix = x1 + kx - 2;
-                if (!wrap_index(ix, g.nx)) continue;
-                const T dlt = (T)(wx[kx] * wyz * inv);
-                const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
-                unsafeAtomicAdd(F + n, dlt * fx);
-                unsafeAtomicAdd(F + N + n, dlt * fy);
-                unsafeAtomicAdd(F + 2 * N + n, dlt * fz);
+                if (!((mx >> kx) & 1u)) continue;
+                int a = x1 + kx;
+                wrap_index(a, g.nx);
+                const int li = (a - ox) + ACC_SX * (b - oy) + ACC_SXY * (c - oz);
+                const T w = wx[kx] * wyz;
+                // the power-of-two scale is applied in FP64 (exact) so that no
+                // FP32 product can overflow
+                atomicAdd(&acc[li], (unsigned long long)llrint((double)(w * f0) * scale));
+                atomicAdd(&acc[ACC_N + li], (unsigned long long)llrint((double)(w * f1) * scale));
+                atomicAdd(&acc[2 * ACC_N + li], (unsigned long long)llrint((double)(w * f2) * scale));
             }
         }
     }
 }
 
 template <typename T>
-__device__ __forceinline__ void interp(const T *__restrict__ U, size_t N, const IbmGeom &g, double x, double y,
-                                       double z, bool rk4, T &ur, T &vr, T &wr) {
-    const double dx = g.dx;
-    int x1, y1, z1;
-    double wx[4], wy[4], wz[4];
-    if (rk4) {
-        stencil<double>(x + (double)(g.nxp - 1) * dx / 2.0, dx, x1, wx, false);
-        stencil<double>(y + (double)(g.nyp - 1) * dx / 2.0, dx, y1, wy, false);
-        stencil<double>(z + (double)(g.nzp - 1) * dx / 2.0, dx, z1, wz, false);
-    } else {  // RK2 / Euler branches use unit spacing (ibm.cu:524-551, 581-608)
-        stencil<double>(x + (double)(g.nxp - 1) / 2.0, dx, x1, wx, true);
-        stencil<double>(y + (double)(g.nyp - 1) / 2.0, dx, y1, wy, true);
-        stencil<double>(z + (double)(g.nzp - 1) / 2.0, dx, z1, wz, true);
+__global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__ xs, const T *__restrict__ cf,
+                                                      const short4 *__restrict__ node0,
+                                                      const int *__restrict__ tile_count,
+                                                      const int *__restrict__ tile_off,
+                                                      const int *__restrict__ home_list, T *__restrict__ F, int nv,
+                                                      IbmGeom g, TileGeom tg, const unsigned *__restrict__ absmax) {
+    __shared__ unsigned long long acc[3 * ACC_N];
+    __shared__ int queue[4][128];
+    const int tile = xcd_block(tg.ntiles);
+    if (tile >= tg.ntiles) return;
+    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+    const int ox = tx * TILE_X, oy = ty * TILE_Y, oz = tz * TILE_Z;
+    const int lx = min(TILE_X, g.nx - ox), ly = min(TILE_Y, g.ny - oy), lz = min(TILE_Z, g.nz - oz);
+    for (int k = threadIdx.x; k < 3 * ACC_N; k += blockDim.x) acc[k] = 0ull;
+    // source bins: tiles holding first stencil nodes that reach this tile
+    int sxs[4], sys[4], szs[4];
+    const int nsx = source_tiles(tx, ox, g.nx, TILE_X, true, sxs);
+    const int nsy = source_tiles(ty, oy, g.ny, TILE_Y, !g.walls, sys);
+    const int nsz = source_tiles(tz, oz, g.nz, TILE_Z, true, szs);
+    // fixed-point scale: at most `total` vertices reach one node, each with a
+    // weight at most 1/(8 dx^3), so |sum| <= total |f|max / (8 dx^3), which
+    // the power of two keeps below 2^62 / scale
+    long long total = 0;
+    for (int iz = 0; iz < nsz; iz++)
+        for (int iy = 0; iy < nsy; iy++)
+            for (int ix = 0; ix < nsx; ix++) total += tile_count[sxs[ix] + tg.ntx * (sys[iy] + tg.nty * szs[iz])];
+    const double fmax = (double)__uint_as_float(*absmax);
+    double scale = 1.0;
+    if (fmax > 0.0 && total > 0) {
+        const double bound = (double)total * fmax / (8.0 * g.dx * g.dx * g.dx);
+        scale = ldexp(1.0, 61 - ilogb(bound));
     }
+    const T w64 = (T)(1.0 / (64.0 * g.dx * g.dx * g.dx));
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int *q = queue[wave];
+    int qn = 0;
+    for (int s = 0; s < nsx * nsy * nsz; s++) {
+        const int bin = sxs[s % nsx] + tg.ntx * (sys[(s / nsx) % nsy] + tg.nty * szs[s / (nsx * nsy)]);
+        const int cnt = tile_count[bin], off = tile_off[bin];
+        for (int base = wave * 64; base < cnt; base += 256) {
+            const int i = base + lane;
+            int v = -1;
+            bool touch = false;
+            if (i < cnt) {
+                v = home_list[off + i];
+                const short4 n0 = node0[v];
+                touch = axis_mask(n0.x, ox, lx, g.nx, true) && axis_mask(n0.y - 3, oy, ly, g.ny, !g.walls) &&
+                        axis_mask(n0.z, oz, lz, g.nz, true);
+            }
+            const unsigned long long m = __ballot(touch);
+            if (touch) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = v;
+            qn += __popcll(m);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (qn >= 64) {
+                spread_vertex<T>(xs, cf, nv, q[lane], g, ox, oy, oz, lx, ly, lz, scale, w64, acc);
+                const int rest = qn - 64;
+                const int mv = lane < rest ? q[64 + lane] : 0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (lane < rest) q[lane] = mv;
+                qn = rest;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+    }
+    if (lane < qn) spread_vertex<T>(xs, cf, nv, q[lane], g, ox, oy, oz, lx, ly, lz, scale, w64, acc);
+    __syncthreads();
+    const double inv_scale = 1.0 / scale;
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    for (int k = threadIdx.x; k < TILE_X * TILE_Y * TILE_Z; k += blockDim.x) {
+        const int la = k % TILE_X, lb = (k / TILE_X) % TILE_Y, lc = k / (TILE_X * TILE_Y);
+        if (la >= lx || lb >= ly || lc >= lz) continue;
+        const size_t nidx = (size_t)(ox + la) + (size_t)g.nx * ((size_t)(oy + lb) + (size_t)g.ny * (oz + lc));
+        const int li = la + ACC_SX * lb + ACC_SXY * lc;
+#pragma unroll
+        for (int d = 0; d < 3; d++) F[d * N + nidx] = (T)((double)(long long)acc[d * ACC_N + li] * inv_scale);
+    }
+}
+
+// ---- interpolation + advection ------------------------------------------------
+
+// Velocity box staged per tile: the tile plus 3 nodes below and 4 above on
+// every axis (first stencil node of the tile's vertices, 4-node stencil and a
+// one-node move of the RK stages).
+constexpr int RX = TILE_X + 7, RY = TILE_Y + 7, RZ = TILE_Z + 7, RN = RX * RY * RZ;
+
+template <typename T>
+__device__ __forceinline__ void interp_global(const T *__restrict__ U, size_t N, const IbmGeom &g, double x,
+                                              double y, double z, bool unit, T &ur, T &vr, T &wr) {
+    int x1, y1, z1;
+    T wx[4], wy[4], wz[4];
+    stencil3<T>(g, x, y, z, unit, x1, y1, z1, wx, wy, wz);
     ur = vr = wr = T(0);
 #pragma unroll
     for (int kz = 0; kz < 4; kz++) {
-        int iz = z1 + kz - 2;
+        int iz = z1 + kz;
         if (!wrap_index(iz, g.nz)) continue;
 #pragma unroll
         for (int ky = 0; ky < 4; ky++) {
-            int iy = y1 + ky - 2;
+            int iy = y1 + ky;
             if (g.walls) {
                 if (iy < 0 || iy >= g.ny) continue;
             } else if (!wrap_index(iy, g.ny)) continue;
 #pragma unroll
             for (int kx = 0; kx < 4; kx++) {
-                int ix = x1 + kx - 2;
+                int ix = x1 + kx;
                 if (!wrap_index(ix, g.nx)) continue;
-                const T dlt = (T)(wx[kx] * wy[ky] * wz[kz] / 64.0);
+                const T dlt = wx[kx] * wy[ky] * wz[kz] / T(64);
                 const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
                 ur += dlt * U[n];
                 vr += dlt * U[N + n];
                 wr += dlt * U[2 * N + n];
+            }
+        }
+    }
+}
+
+// Interpolation from the staged box when the stencil lies inside it.
+template <typename T>
+__device__ __forceinline__ void interp_tile(const T *__restrict__ U, size_t N, const T *__restrict__ Us,
+                                            const IbmGeom &g, int bx, int by, int bz, double x, double y, double z,
+                                            bool unit, T &ur, T &vr, T &wr) {
+    int x1, y1, z1;
+    T wx[4], wy[4], wz[4];
+    stencil3<T>(g, x, y, z, unit, x1, y1, z1, wx, wy, wz);
+    int rx = x1 - bx, rz = z1 - bz, ry = y1 - by;
+    if (rx < 0) rx += g.nx; else if (rx >= g.nx) rx -= g.nx;
+    if (rz < 0) rz += g.nz; else if (rz >= g.nz) rz -= g.nz;
+    if (!g.walls) { if (ry < 0) ry += g.ny; else if (ry >= g.ny) ry -= g.ny; }
+    if (rx < 0 || rx + 3 >= RX || ry < 0 || ry + 3 >= RY || rz < 0 || rz + 3 >= RZ) {
+        interp_global<T>(U, N, g, x, y, z, unit, ur, vr, wr);
+        return;
+    }
+    ur = vr = wr = T(0);
+#pragma unroll
+    for (int kz = 0; kz < 4; kz++) {
+#pragma unroll
+        for (int ky = 0; ky < 4; ky++) {
+            const T wyz = wy[ky] * wz[kz];
+            const int row = rx + RX * ((ry + ky) + RY * (rz + kz));
+#pragma unroll
+            for (int kx = 0; kx < 4; kx++) {
+                const T dlt = wx[kx] * wyz / T(64);
+                ur += dlt * Us[row + kx];
+                vr += dlt * Us[RN + row + kx];
+                wr += dlt * Us[2 * RN + row + kx];
             }
         }
     }
@@ -133,50 +453,79 @@ struct AdvectArgs {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_interp_advect(const T *__restrict__ U, double *__restrict__ xs,
-                                                       T *__restrict__ uvw, double *__restrict__ xd, int nv,
-                                                       IbmGeom g, AdvectArgs a) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nv) return;
+__global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, double *__restrict__ xs,
+                                                      T *__restrict__ uvw, double *__restrict__ xd,
+                                                      const int *__restrict__ tile_count,
+                                                      const int *__restrict__ tile_off,
+                                                      const int *__restrict__ home_list, int nv, IbmGeom g,
+                                                      TileGeom tg, AdvectArgs a) {
+    extern __shared__ double dyn[];
+    T *Us = (T *)dyn;  // [3][RZ][RY][RX]
+    const int tile = xcd_block(tg.ntiles);
+    if (tile >= tg.ntiles) return;
+    const int cnt = tile_count[tile], off = tile_off[tile];
+    if (cnt == 0) return;  // whole work group leaves together
+    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+    const int bx = tx * TILE_X - 3, by = ty * TILE_Y - 3, bz = tz * TILE_Z - 3;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
-    const double xn = xs[j], yn = xs[nv + j], zn = xs[2 * (size_t)nv + j];
+    for (int k = threadIdx.x; k < RN; k += blockDim.x) {
+        const int rx = k % RX, ry = (k / RX) % RY, rz = k / (RX * RY);
+        // the box may span an axis more than once on small lattices
+        const int ix = pmod(bx + rx, g.nx), iz = pmod(bz + rz, g.nz);
+        int iy = by + ry;
+        bool ok = true;
+        if (g.walls) ok = iy >= 0 && iy < g.ny;
+        else iy = pmod(iy, g.ny);
+        T u = T(0), v = T(0), w = T(0);
+        if (ok) {
+            const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
+            u = U[n]; v = U[N + n]; w = U[2 * N + n];
+        }
+        Us[k] = u; Us[RN + k] = v; Us[2 * RN + k] = w;
+    }
+    __syncthreads();
     const double M = a.M, dt = a.dt;
-    T um, vm, wm;
-    if (a.rk == 4) {
-        T u1, v1, w1, u2, v2, w2, u3, v3, w3, u4, v4, w4;
-        interp<T>(U, N, g, xn, yn, zn, true, u1, v1, w1);
-        interp<T>(U, N, g, xn + (double)u1 * M * dt / 2.0, yn + (double)v1 * M * dt / 2.0,
-                  zn + (double)w1 * M * dt / 2.0, true, u2, v2, w2);
-        interp<T>(U, N, g, xn + (double)u2 * M * dt / 2.0, yn + (double)v2 * M * dt / 2.0,
-                  zn + (double)w2 * M * dt / 2.0, true, u3, v3, w3);
-        interp<T>(U, N, g, xn + (double)u3 * M * dt, yn + (double)v3 * M * dt, zn + (double)w3 * M * dt, true, u4,
-                  v4, w4);
-        um = (u1 + T(2) * u2 + T(2) * u3 + u4) / T(6);
-        vm = (v1 + T(2) * v2 + T(2) * v3 + v4) / T(6);
-        wm = (w1 + T(2) * w2 + T(2) * w3 + w4) / T(6);
-    } else if (a.rk == 2) {
-        T u1, v1, w1, u2, v2, w2;
-        interp<T>(U, N, g, xn, yn, zn, false, u1, v1, w1);
-        interp<T>(U, N, g, xn + (double)u1 * M, yn + (double)v1 * M, zn + (double)w1 * M, false, u2, v2, w2);
-        um = (u1 + u2) / T(2);
-        vm = (v1 + v2) / T(2);
-        wm = (w1 + w2) / T(2);
-    } else {
-        interp<T>(U, N, g, xn, yn, zn, false, um, vm, wm);
+    const bool unit = a.rk != 4;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const int j = home_list[off + i];
+        const double xn = xs[j], yn = xs[nv + j], zn = xs[2 * (size_t)nv + j];
+        T um, vm, wm;
+        if (a.rk == 4) {
+            T u1, v1, w1, u2, v2, w2, u3, v3, w3, u4, v4, w4;
+            interp_tile<T>(U, N, Us, g, bx, by, bz, xn, yn, zn, false, u1, v1, w1);
+            interp_tile<T>(U, N, Us, g, bx, by, bz, xn + (double)u1 * M * dt / 2.0, yn + (double)v1 * M * dt / 2.0,
+                           zn + (double)w1 * M * dt / 2.0, false, u2, v2, w2);
+            interp_tile<T>(U, N, Us, g, bx, by, bz, xn + (double)u2 * M * dt / 2.0, yn + (double)v2 * M * dt / 2.0,
+                           zn + (double)w2 * M * dt / 2.0, false, u3, v3, w3);
+            interp_tile<T>(U, N, Us, g, bx, by, bz, xn + (double)u3 * M * dt, yn + (double)v3 * M * dt,
+                           zn + (double)w3 * M * dt, false, u4, v4, w4);
+            um = (u1 + T(2) * u2 + T(2) * u3 + u4) / T(6);
+            vm = (v1 + T(2) * v2 + T(2) * v3 + v4) / T(6);
+            wm = (w1 + T(2) * w2 + T(2) * w3 + w4) / T(6);
+        } else if (a.rk == 2) {
+            T u1, v1, w1, u2, v2, w2;
+            interp_tile<T>(U, N, Us, g, bx, by, bz, xn, yn, zn, unit, u1, v1, w1);
+            interp_tile<T>(U, N, Us, g, bx, by, bz, xn + (double)u1 * M, yn + (double)v1 * M, zn + (double)w1 * M,
+                           unit, u2, v2, w2);
+            um = (u1 + u2) / T(2);
+            vm = (v1 + v2) / T(2);
+            wm = (w1 + w2) / T(2);
+        } else {
+            interp_tile<T>(U, N, Us, g, bx, by, bz, xn, yn, zn, unit, um, vm, wm);
+        }
+        double xr = xn + (double)um * M * dt, yr = yn + (double)vm * M * dt, zr = zn + (double)wm * M * dt;
+        if (xr < -a.lx / 2.0) xr += a.lx;
+        if (xr > a.lx / 2.0) xr -= a.lx;
+        if (zr < -a.lz / 2.0) zr += a.lz;
+        if (zr > a.lz / 2.0) zr -= a.lz;
+        if (a.periodic_y) {
+            if (yr < -a.ly / 2.0) yr += a.ly;
+            if (yr > a.ly / 2.0) yr -= a.ly;
+        }
+        uvw[j] = um; uvw[nv + j] = vm; uvw[2 * (size_t)nv + j] = wm;
+        xs[j] = xr; xs[nv + j] = yr; xs[2 * (size_t)nv + j] = zr;
+        xd[3 * (size_t)j] = xr; xd[3 * (size_t)j + 1] = yr; xd[3 * (size_t)j + 2] = zr;
     }
-    double xr = xn + (double)um * M * dt, yr = yn + (double)vm * M * dt, zr = zn + (double)wm * M * dt;
-    const double lx = a.lx, ly = a.ly, lz = a.lz;
-    if (xr < -lx / 2.0) xr += lx;
-    if (xr > lx / 2.0) xr -= lx;
-    if (zr < -lz / 2.0) zr += lz;
-    if (zr > lz / 2.0) zr -= lz;
-    if (a.periodic_y) {
-        if (yr < -ly / 2.0) yr += ly;
-        if (yr > ly / 2.0) yr -= ly;
-    }
-    uvw[j] = um; uvw[nv + j] = vm; uvw[2 * nv + j] = wm;
-    xs[j] = xr; xs[nv + j] = yr; xs[2 * (size_t)nv + j] = zr;
-    xd[3 * j] = xr; xd[3 * j + 1] = yr; xd[3 * j + 2] = zr;
 }
 
 IbmGeom geom_of(const lbmibm_ctx *ctx) {
@@ -188,30 +537,76 @@ IbmGeom geom_of(const lbmibm_ctx *ctx) {
     return g;
 }
 
+template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
+    const IbmGeom g = geom_of(ctx);
+    TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
+    const int nv = ctx->nv;
+    hipLaunchKernelGGL(k_zero_int, dim3((ctx->ntiles + 255) / 256), dim3(256), 0, ctx->stream, ctx->tile_count,
+                       ctx->ntiles);
+    LBMIBM_LAUNCHED(ctx, "k_zero_int");
+    hipLaunchKernelGGL(k_bin_count<T>, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
+                       (const T *)ctx->cf, nv, g, tg, ctx->node0, ctx->home, ctx->tile_count, ctx->blk_fmax);
+    LBMIBM_LAUNCHED(ctx, "k_bin_count");
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->tile_count, ctx->tile_off,
+                       ctx->tile_cur, ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax);
+    LBMIBM_LAUNCHED(ctx, "k_tile_scan");
+    hipLaunchKernelGGL(k_bin_scatter, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, ctx->home, nv, ctx->tile_off,
+                       ctx->tile_cur, ctx->home_list);
+    LBMIBM_LAUNCHED(ctx, "k_bin_scatter");
+    ctx->bins_valid = true;
+    return LBMIBM_OK;
+}
+
 template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
-    lbmibm_status s = lattice_fill(ctx, ctx->F, 3 * (size_t)ctx->N, 0.0, false);
+    if (ctx->nv == 0) return lattice_fill(ctx, ctx->F, 3 * (size_t)ctx->N, 0.0, false);
+    // the nodal forces change between calls: the binning pass also takes max |f|
+    lbmibm_status s = bin_impl<T>(ctx);
     if (s != LBMIBM_OK) return s;
-    if (ctx->nv == 0) return LBMIBM_OK;
-    hipLaunchKernelGGL(k_spread<T>, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
-                       (const T *)ctx->cf, (T *)ctx->F, ctx->nv, geom_of(ctx));
-    LBMIBM_LAUNCHED(ctx, "k_spread");
+    TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
+    hipLaunchKernelGGL(k_spread_tiles<T>, dim3(xcd_grid(ctx->ntiles)), dim3(256), 0, ctx->stream,
+                       (const double *)ctx->xs, (const T *)ctx->cf, ctx->node0, ctx->tile_count, ctx->tile_off,
+                       ctx->home_list, (T *)ctx->F, ctx->nv, geom_of(ctx), tg, ctx->absmax);
+    LBMIBM_LAUNCHED(ctx, "k_spread_tiles");
     return LBMIBM_OK;
 }
 
 template <typename T> lbmibm_status interp_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return LBMIBM_OK;
+    if (!ctx->bins_valid) {
+        lbmibm_status s = bin_impl<T>(ctx);
+        if (s != LBMIBM_OK) return s;
+    }
     AdvectArgs a;
     a.dt = ctx->p.dt; a.M = ctx->p.m;
     a.lx = ctx->nx * ctx->p.dx; a.ly = ctx->ny * ctx->p.dx; a.lz = ctx->nz * ctx->p.dx;
     a.rk = ctx->p.runge_kutta;
     a.periodic_y = ctx->p.periodic_y;
-    hipLaunchKernelGGL(k_interp_advect<T>, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream,
-                       (const T *)ctx->U, (double *)ctx->xs, (T *)ctx->uvw, (double *)ctx->xd, ctx->nv, geom_of(ctx), a);
-    LBMIBM_LAUNCHED(ctx, "k_interp_advect");
+    TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
+    const size_t lds = 3 * (size_t)RN * sizeof(T);
+    hipLaunchKernelGGL(k_interp_tiles<T>, dim3(xcd_grid(ctx->ntiles)), dim3(256), lds, ctx->stream,
+                       (const T *)ctx->U, (double *)ctx->xs, (T *)ctx->uvw, (double *)ctx->xd, ctx->tile_count,
+                       ctx->tile_off, ctx->home_list, ctx->nv, geom_of(ctx), tg, a);
+    LBMIBM_LAUNCHED(ctx, "k_interp_tiles");
+    ctx->bins_valid = false;  // the positions moved
     return LBMIBM_OK;
 }
 
 }  // namespace
+
+lbmibm_status ibm_setup(lbmibm_ctx *ctx) {
+    const int lds64 = (int)(3 * (size_t)RN * sizeof(double));
+    const int lds32 = (int)(3 * (size_t)RN * sizeof(float));
+    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_interp_tiles<double>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds64));
+    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_interp_tiles<float>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds32));
+    return LBMIBM_OK;
+}
+
+lbmibm_status ibm_bin(lbmibm_ctx *ctx) {
+    if (ctx->nv == 0) return LBMIBM_OK;
+    return ctx->prec == LBMIBM_F64 ? bin_impl<double>(ctx) : bin_impl<float>(ctx);
+}
 
 lbmibm_status ibm_spread(lbmibm_ctx *ctx) {
     lbmibm_status s = ctx->prec == LBMIBM_F64 ? spread_impl<double>(ctx) : spread_impl<float>(ctx);

@@ -9,10 +9,11 @@
 //   - k_assemble: one thread per vertex writes the 7 Jacobi-scaled row
 //     coefficients once (x, y and z rows share them, stress.cu:164-171) and
 //     the three scaled right-hand sides;
-//   - k_bicgstab: one work group per cell runs the reference's BiCGSTAB
-//     iteration to convergence on the device (FP64 inner products reduced in
-//     a fixed order, device-side convergence and breakdown flags), so all
-//     cells of any mesh size go in one launch and nothing returns to the host;
+//   - k_bicg_onchip / k_bicg_global: one 1024-thread work group per cell runs
+//     the reference's BiCGSTAB iteration to convergence on the device (FP64
+//     inner products reduced in a fixed order, device-side convergence and
+//     breakdown flags); cells of up to 3072 vertices keep every vector and
+//     their matrix rows in registers and LDS, larger ones use global memory;
 //   - k_q2f and k_bend as the reference.
 // Vertex positions, the reference metric and all geometric arithmetic
 // (edges, metric invariants, tension, matrix entries, ring areas, hinge
@@ -95,8 +96,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ xd, const int *__restrict__ ele,
                                                   const int *__restrict__ nln, const int *__restrict__ lo8,
                                                   const int *__restrict__ le, const T *__restrict__ rg,
-                                                  const T *__restrict__ tt, T *__restrict__ coef,
-                                                  T *__restrict__ bv, int nv, PeriodicBox box) {
+                                                  const T *__restrict__ tt, const int *__restrict__ vcell,
+                                                  const int *__restrict__ voff, T *__restrict__ coef,
+                                                  unsigned short *__restrict__ nbr, T *__restrict__ bv, int nv,
+                                                  PeriodicBox box) {
     const int jj = blockIdx.x * blockDim.x + threadIdx.x;
     if (jj >= nv) return;
     const int n = nln[jj];
@@ -155,71 +158,239 @@ __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ xd,
 #pragma unroll
     for (int k = 1; k < 7; k++)
         if (k == order) jac = tv[k];
+    // Jacobi-scaled row (stress.cu:263-268): the diagonal is exactly 1, the
+    // off-diagonal entries are stored with cell-local column ids
+    const int base = voff[vcell[jj]];
+    int m = 0;
 #pragma unroll
-    for (int k = 0; k < 7; k++) coef[8 * (size_t)jj + k] = (T)(tv[k] / jac);
-    coef[8 * (size_t)jj + 7] = T(0);
+    for (int k = 0; k < 7; k++) {
+        if (k <= n && k != order) {
+            coef[(size_t)m * nv + jj] = (T)(tv[k] / jac);
+            nbr[(size_t)m * nv + jj] = (unsigned short)(lo[k] - base);
+            m++;
+        }
+    }
+    for (; m < 6; m++) {
+        coef[(size_t)m * nv + jj] = T(0);
+        nbr[(size_t)m * nv + jj] = (unsigned short)(jj - base);
+    }
 #pragma unroll
     for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
 }
 
-// y = A x for the vertices of one cell; returns nothing, writes y[3v+d].
-template <typename T>
-__device__ __forceinline__ void row_apply(const T *__restrict__ coef, const int *__restrict__ lo8,
-                                          const int *__restrict__ nln, const T *__restrict__ x, int v, T y[3]) {
-    const int n = nln[v];
-    y[0] = y[1] = y[2] = T(0);
+// FP64 sums over a 1024-thread work group in a fixed order (deterministic).
+template <int NV>
+__device__ __forceinline__ void wg_sum(double (&v)[NV], double *sh) {
 #pragma unroll
-    for (int k = 0; k < 7; k++) {
-        if (k <= n) {
-            const T c = coef[8 * (size_t)v + k];
-            const int col = lo8[8 * v + k];
-            y[0] += c * x[3 * (size_t)col];
-            y[1] += c * x[3 * (size_t)col + 1];
-            y[2] += c * x[3 * (size_t)col + 2];
+    for (int i = 0; i < NV; i++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o);
+    }
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        for (int i = 0; i < NV; i++) sh[i * 16 + w] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double s = 0.0;
+        for (int k = 0; k < nw; k++) s += sh[i * 16 + k];
+        v[i] = s;
+    }
+}
+
+// On-chip BiCGSTAB for one cell of at most 1024*KMAX vertices: every thread
+// keeps the rows of its vertices (x, r, p, v, t and the 6 off-diagonal
+// coefficients) in registers; p and r are mirrored in LDS for the neighbour
+// gathers of the matrix-vector products.  matrix_solver.cu:279-377 in the
+// same update order; (r, r^) of the next iteration is fused with the
+// residual reduction; r^ = b because x starts at 0 (stress.cu:139).
+template <typename T, int KMAX>
+__global__ __launch_bounds__(1024) void k_bicg_onchip(const int *__restrict__ cells, const int *__restrict__ voff,
+                                                      const T *__restrict__ coef,
+                                                      const unsigned short *__restrict__ nbr,
+                                                      const T *__restrict__ bv, T *__restrict__ xout, int nv,
+                                                      int itrmax, double residual, int *__restrict__ iters,
+                                                      DevStatus *__restrict__ status) {
+    extern __shared__ double dyn[];
+    __shared__ double sh[2 * 16];
+    const int c = cells[blockIdx.x];
+    const int v0 = voff[c], V = voff[c + 1] - v0;
+    T *Ps = (T *)dyn;              // [V][3]
+    T *Rs = Ps + 3 * (size_t)V;    // [V][3]
+    T *Bs = Rs + 3 * (size_t)V;    // [V][3] right-hand side = r^
+    const int tid = threadIdx.x;
+    T x[KMAX][3], r[KMAX][3], p[KMAX][3], v[KMAX][3], t[KMAX][3], cf[KMAX][6];
+    unsigned nb[KMAX][3];  // two 16-bit column ids per register
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        const int j = tid + k * 1024;
+        const bool on = j < V;
+        const size_t g = (size_t)v0 + (on ? j : 0);
+#pragma unroll
+        for (int m = 0; m < 6; m++) cf[k][m] = on ? coef[(size_t)m * nv + g] : T(0);
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+            nb[k][m] = on ? ((unsigned)nbr[(size_t)(2 * m) * nv + g] | ((unsigned)nbr[(size_t)(2 * m + 1) * nv + g] << 16))
+                          : 0u;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const T bb = on ? bv[3 * g + d] : T(0);
+            x[k][d] = T(0); r[k][d] = bb; p[k][d] = bb; v[k][d] = T(0); t[k][d] = T(0);
+            acc[0] += (double)bb * (double)bb;
+            if (on) { Ps[3 * j + d] = bb; Bs[3 * j + d] = bb; }
         }
     }
-}
-
-__device__ __forceinline__ void block_sum2(double &a, double &b, double *sh) {
+    wg_sum<1>(*(double(*)[1])acc, sh);  // orders the LDS writes above
+    const double bb = acc[0];
+    if (bb == 0.0) {  // exactly zero load: x = 0 solves the system
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        a += __shfl_xor(a, o);
-        b += __shfl_xor(b, o);
+        for (int k = 0; k < KMAX; k++) {
+            const int j = tid + k * 1024;
+            if (j < V)
+                for (int d = 0; d < 3; d++) xout[3 * ((size_t)v0 + j) + d] = T(0);
+        }
+        if (tid == 0) iters[c] = 0;
+        return;
     }
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) { sh[w] = a; sh[4 + w] = b; }
-    __syncthreads();
-    a = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-    b = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+    double rr = bb, rr_old = 0.0, alfa = 1.0, omeg = 1.0;
+    int it = 0, flag = 0;
+    bool conv = false;
+    for (it = 0; it < itrmax; it++) {
+        if (rr == 0.0) { flag = 1; break; }
+        if (it > 0) {
+            const T beta = (T)((rr / rr_old) * (alfa / omeg)), om = (T)omeg;
+#pragma unroll
+            for (int k = 0; k < KMAX; k++) {
+                const int j = tid + k * 1024;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    p[k][d] = r[k][d] + beta * (p[k][d] - om * v[k][d]);
+                    if (j < V) Ps[3 * j + d] = p[k][d];
+                }
+            }
+            __syncthreads();
+        }
+        // v = A p ; alfa = (r, r^) / (v, r^)
+        acc[0] = 0.0;
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            const int j = tid + k * 1024;
+            if (j < V) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    T y = p[k][d];
+#pragma unroll
+                    for (int m = 0; m < 6; m++) y += cf[k][m] * Ps[3 * ((nb[k][m >> 1] >> (16 * (m & 1))) & 0xffffu) + d];
+                    v[k][d] = y;
+                    acc[0] += (double)y * (double)Bs[3 * j + d];
+                }
+            }
+        }
+        wg_sum<1>(*(double(*)[1])acc, sh);
+        alfa = rr / acc[0];
+        const T al = (T)alfa;
+        // r = r - alfa v
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            const int j = tid + k * 1024;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                r[k][d] = r[k][d] - al * v[k][d];
+                if (j < V) Rs[3 * j + d] = r[k][d];
+            }
+        }
+        __syncthreads();
+        // t = A r ; omeg = (r, t) / (t, t)
+        acc[0] = 0.0; acc[1] = 0.0;
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            const int j = tid + k * 1024;
+            if (j < V) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    T y = r[k][d];
+#pragma unroll
+                    for (int m = 0; m < 6; m++) y += cf[k][m] * Rs[3 * ((nb[k][m >> 1] >> (16 * (m & 1))) & 0xffffu) + d];
+                    t[k][d] = y;
+                    acc[0] += (double)r[k][d] * (double)y;
+                    acc[1] += (double)y * (double)y;
+                }
+            }
+        }
+        wg_sum<2>(acc, sh);
+        omeg = acc[0] / acc[1];
+        const T om = (T)omeg;
+        // x = x + alfa p + omeg r ; r = r - omeg t
+        acc[0] = 0.0; acc[1] = 0.0;
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            const int j = tid + k * 1024;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const T ri = r[k][d];
+                x[k][d] = x[k][d] + al * p[k][d] + om * ri;
+                const T rn = ri - om * t[k][d];
+                r[k][d] = rn;
+                if (j < V) {
+                    acc[0] += (double)rn * (double)rn;
+                    acc[1] += (double)rn * (double)Bs[3 * j + d];
+                }
+            }
+        }
+        wg_sum<2>(acc, sh);
+        rr_old = rr;
+        rr = acc[1];
+        if (sqrt(acc[0] / bb) < residual) { conv = true; break; }
+    }
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        const int j = tid + k * 1024;
+        if (j < V)
+            for (int d = 0; d < 3; d++) xout[3 * ((size_t)v0 + j) + d] = x[k][d];
+    }
+    if (tid == 0) {
+        iters[c] = conv ? it + 1 : it;
+        if (!conv && flag == 0) flag = 2;
+        if (flag) atomicMax(&status->bicg_flag, flag);
+    }
 }
 
-// One work group per cell; matrix_solver.cu:279-377 in the same update order,
-// with the next iteration's (r, r^) fused into the residual reduction.
+// Same iteration for cells of any size with the vectors in global memory.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bicgstab(const int *__restrict__ voff, const T *__restrict__ coef,
-                                                  const int *__restrict__ lo8, const int *__restrict__ nln,
-                                                  const T *__restrict__ bv, T *__restrict__ x, T *__restrict__ r,
-                                                  T *__restrict__ rh, T *__restrict__ p, T *__restrict__ v,
-                                                  T *__restrict__ t, int itrmax, double residual,
-                                                  int *__restrict__ iters, DevStatus *__restrict__ status) {
-    __shared__ double sh[8];
-    const int c = blockIdx.x;
+__global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ cells, const int *__restrict__ voff,
+                                                      const T *__restrict__ coef,
+                                                      const unsigned short *__restrict__ nbr,
+                                                      const T *__restrict__ bv, T *__restrict__ x,
+                                                      T *__restrict__ r, T *__restrict__ p, T *__restrict__ v,
+                                                      T *__restrict__ t, int nv, int itrmax, double residual,
+                                                      int *__restrict__ iters, DevStatus *__restrict__ status) {
+    __shared__ double sh[2 * 16];
+    const int c = cells[blockIdx.x];
     const int v0 = voff[c], v1 = voff[c + 1];
-    const int tid = threadIdx.x;
-    double pb = 0.0;
-    for (int vv = v0 + tid; vv < v1; vv += blockDim.x)
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto apply = [&](const T *__restrict__ in, int vv, T y[3]) {
+        const T *own = in + 3 * (size_t)vv;
+        y[0] = own[0]; y[1] = own[1]; y[2] = own[2];
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            const T cm = coef[(size_t)m * nv + vv];
+            const T *o = in + 3 * ((size_t)v0 + nbr[(size_t)m * nv + vv]);
+            y[0] += cm * o[0]; y[1] += cm * o[1]; y[2] += cm * o[2];
+        }
+    };
+    double acc[2] = {0.0, 0.0};
+    for (int vv = v0 + tid; vv < v1; vv += nt)
         for (int d = 0; d < 3; d++) {
             const size_t k = 3 * (size_t)vv + d;
             const T b = bv[k];
-            x[k] = T(0);
-            r[k] = b;
-            rh[k] = b;
-            p[k] = b;
-            pb += (double)b * (double)b;
+            x[k] = T(0); r[k] = b; p[k] = b;
+            acc[0] += (double)b * (double)b;
         }
-    const double bb = block_sum(pb, sh);  // also orders the writes above
-    if (bb == 0.0) {  // exactly zero load: x = 0 solves the system
+    wg_sum<1>(*(double(*)[1])acc, sh);
+    const double bb = acc[0];
+    if (bb == 0.0) {
         if (tid == 0) iters[c] = 0;
         return;
     }
@@ -230,61 +401,59 @@ __global__ __launch_bounds__(256) void k_bicgstab(const int *__restrict__ voff, 
         if (rr == 0.0) { flag = 1; break; }
         if (it > 0) {
             const double beta = (rr / rr_old) * (alfa / omeg);
-            for (int vv = v0 + tid; vv < v1; vv += blockDim.x)
+            for (int vv = v0 + tid; vv < v1; vv += nt)
                 for (int d = 0; d < 3; d++) {
                     const size_t k = 3 * (size_t)vv + d;
                     p[k] = (T)((double)r[k] + beta * ((double)p[k] - omeg * (double)v[k]));
                 }
             __syncthreads();
         }
-        // v = A p, alfa = (r, r^)/(v, r^)
-        double pv = 0.0;
-        for (int vv = v0 + tid; vv < v1; vv += blockDim.x) {
+        acc[0] = 0.0;
+        for (int vv = v0 + tid; vv < v1; vv += nt) {
             T y[3];
-            row_apply<T>(coef, lo8, nln, p, vv, y);
+            apply(p, vv, y);
             for (int d = 0; d < 3; d++) {
                 const size_t k = 3 * (size_t)vv + d;
                 v[k] = y[d];
-                pv += (double)y[d] * (double)rh[k];
+                acc[0] += (double)y[d] * (double)bv[k];
             }
         }
-        alfa = rr / block_sum(pv, sh);
-        for (int vv = v0 + tid; vv < v1; vv += blockDim.x)
+        wg_sum<1>(*(double(*)[1])acc, sh);
+        alfa = rr / acc[0];
+        for (int vv = v0 + tid; vv < v1; vv += nt)
             for (int d = 0; d < 3; d++) {
                 const size_t k = 3 * (size_t)vv + d;
                 r[k] = (T)((double)r[k] - alfa * (double)v[k]);
             }
         __syncthreads();
-        // t = A r, omeg = (r, t)/(t, t)
-        double prt = 0.0, ptt = 0.0;
-        for (int vv = v0 + tid; vv < v1; vv += blockDim.x) {
+        acc[0] = 0.0; acc[1] = 0.0;
+        for (int vv = v0 + tid; vv < v1; vv += nt) {
             T y[3];
-            row_apply<T>(coef, lo8, nln, r, vv, y);
+            apply(r, vv, y);
             for (int d = 0; d < 3; d++) {
                 const size_t k = 3 * (size_t)vv + d;
                 t[k] = y[d];
-                prt += (double)r[k] * (double)y[d];
-                ptt += (double)y[d] * (double)y[d];
+                acc[0] += (double)r[k] * (double)y[d];
+                acc[1] += (double)y[d] * (double)y[d];
             }
         }
-        block_sum2(prt, ptt, sh);
-        omeg = prt / ptt;
-        // x += alfa p + omeg r ; r -= omeg t ; err = sqrt((r,r)/(b,b))
-        double prr = 0.0, prh = 0.0;
-        for (int vv = v0 + tid; vv < v1; vv += blockDim.x)
+        wg_sum<2>(acc, sh);
+        omeg = acc[0] / acc[1];
+        acc[0] = 0.0; acc[1] = 0.0;
+        for (int vv = v0 + tid; vv < v1; vv += nt)
             for (int d = 0; d < 3; d++) {
                 const size_t k = 3 * (size_t)vv + d;
                 const double ri = (double)r[k];
                 x[k] = (T)((double)x[k] + alfa * (double)p[k] + omeg * ri);
                 const T rn = (T)(ri - omeg * (double)t[k]);
                 r[k] = rn;
-                prr += (double)rn * (double)rn;
-                prh += (double)rn * (double)rh[k];
+                acc[0] += (double)rn * (double)rn;
+                acc[1] += (double)rn * (double)bv[k];
             }
-        block_sum2(prr, prh, sh);
+        wg_sum<2>(acc, sh);
         rr_old = rr;
-        rr = prh;
-        if (sqrt(prr / bb) < residual) { conv = true; break; }
+        rr = acc[1];
+        if (sqrt(acc[0] / bb) < residual) { conv = true; break; }
     }
     if (tid == 0) {
         iters[c] = conv ? it + 1 : it;
@@ -404,6 +573,7 @@ __global__ __launch_bounds__(256) void k_bend(const double *__restrict__ xs, con
 }
 
 template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
+    const int bicg_onchip_max_v = ctx->max_small_v;
     const lbmibm_params &p = ctx->p;
     PeriodicBox box{ctx->nx * p.dx, ctx->ny * p.dx, ctx->nz * p.dx, p.periodic_y};
     TensionArgs ta;
@@ -416,14 +586,23 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
                        p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ne, box, ta, ctx->step);
     LBMIBM_LAUNCHED(ctx, "k_tension");
     hipLaunchKernelGGL(k_assemble<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
-                       ctx->ele, ctx->nln, ctx->lo8, ctx->le, (const T *)ctx->rg, (const T *)ctx->tt,
-                       (T *)ctx->coef, (T *)ctx->bv, nv, box);
+                       ctx->ele, ctx->nln, ctx->lo8, ctx->le, (const T *)ctx->rg, (const T *)ctx->tt, ctx->vcell,
+                       ctx->voff, (T *)ctx->coef, ctx->nbr, (T *)ctx->bv, nv, box);
     LBMIBM_LAUNCHED(ctx, "k_assemble");
-    hipLaunchKernelGGL(k_bicgstab<T>, dim3(ctx->nc), dim3(256), 0, ctx->stream, ctx->voff, (const T *)ctx->coef,
-                       ctx->lo8, ctx->nln, (const T *)ctx->bv, (T *)ctx->qv, (T *)ctx->rv, (T *)ctx->rh,
-                       (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, p.itrmax, p.residual, ctx->bicg_iters,
-                       ctx->status);
-    LBMIBM_LAUNCHED(ctx, "k_bicgstab");
+    if (ctx->n_small > 0 && sizeof(T) == 4) {
+        const size_t lds = 3 * 3 * (size_t)bicg_onchip_max_v * sizeof(T);
+        hipLaunchKernelGGL((k_bicg_onchip<float, 3>), dim3(ctx->n_small), dim3(1024), lds, ctx->stream, ctx->small_cells,
+                           ctx->voff, (const float *)ctx->coef, ctx->nbr, (const float *)ctx->bv, (float *)ctx->qv,
+                           nv, p.itrmax, p.residual, ctx->bicg_iters, ctx->status);
+        LBMIBM_LAUNCHED(ctx, "k_bicg_onchip");
+    }
+    if (ctx->n_big > 0) {
+        hipLaunchKernelGGL(k_bicg_global<T>, dim3(ctx->n_big), dim3(1024), 0, ctx->stream, ctx->big_cells, ctx->voff,
+                           (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->qv, (T *)ctx->rv,
+                           (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, nv, p.itrmax, p.residual, ctx->bicg_iters,
+                           ctx->status);
+        LBMIBM_LAUNCHED(ctx, "k_bicg_global");
+    }
     hipLaunchKernelGGL(k_q2f<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
                        (const T *)ctx->qv, ctx->nln, ctx->ln, ctx->vcell, (const T *)ctx->gscale, (T *)ctx->cf,
                        nv, box);
@@ -438,6 +617,21 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
 }
 
 }  // namespace
+
+int bicg_onchip_limit(int prec) {
+    // cells solved on chip: 3 vertices per thread of a 1024-thread work group,
+    // in FP32 only (the FP64 rows do not fit the register budget)
+    return prec == LBMIBM_F64 ? 0 : 3 * 1024;
+}
+
+lbmibm_status membrane_setup(lbmibm_ctx *ctx) {
+    // allow the on-chip solver the LDS its largest cell needs (p, r and b)
+    if (ctx->n_small == 0) return LBMIBM_OK;
+    const int lds = (int)(3 * 3 * (size_t)ctx->max_small_v * ctx->szT);
+    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 3>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    return LBMIBM_OK;
+}
 
 lbmibm_status membrane_forces(lbmibm_ctx *ctx) {
     if (ctx->nc == 0) return LBMIBM_OK;

@@ -145,6 +145,7 @@ lbmibm_status run_impl(lbmibm_ctx *ctx, int mode) {
 
 lbmibm_status volume_run(lbmibm_ctx *ctx, int mode) {
     if (ctx->nc == 0) return LBMIBM_OK;
+    if (mode == 1) ctx->bins_valid = false;  // the correction moves the vertices
     return run_impl(ctx, mode);
 }
 

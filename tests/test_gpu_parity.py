@@ -52,17 +52,18 @@ def test_lbm_parity(prec, walls, hasf, m):
     assert rel(mac["dm"][comp], o.dm[comp]) < TOL[prec]
 
 
-def _cells_across_boundaries(lx, ly, lz, radius=3.0):
+def _cells_across_boundaries(lx, ly, lz, radius=3.0, periodic_y=False):
     from preprocess_src_amd import mesh
-    centres = np.array([[lx / 2 - 1.0, 0.3, -lz / 2 + 1.3], [0.7, -ly / 2 + radius + 1.2, 0.4]])
+    y1 = -ly / 2 + 0.4 if periodic_y else -ly / 2 + radius + 1.2
+    centres = np.array([[lx / 2 - 1.0, 0.3, -lz / 2 + 1.3], [0.7, y1, 0.4]])
     _, rot = mesh.place(2, radius, lx, ly, lz, seed=5)
     return mesh.assemble([2, 2], centres, rot, radius)
 
 
-def _wrapped(cells, lx, lz):
+def _wrapped(cells, lx, ly, lz, periodic_y=False):
     """Vertex positions wrapped into the box as ibm_st2a leaves them (ibm.cu:629-632)."""
     out = {k: cells[k].copy() for k in ("x", "y", "z")}
-    for k, L in (("x", lx), ("z", lz)):
+    for k, L in (("x", lx), ("z", lz)) + ((("y", ly),) if periodic_y else ()):
         a = out[k]
         a[a > L / 2] -= L
         a[a < -L / 2] += L
@@ -76,13 +77,18 @@ def _set_positions(o, g, pos):
     g.set_cells_state(x=pos["x"], y=pos["y"], z=pos["z"], xd=pos["xd"])
 
 
+# lattices with full tiles, with 1-node partial tiles and fully periodic
+SHAPES = [(16, 14, 12, False), (33, 17, 9, False), (18, 9, 10, True)]
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_ibm_spread_parity(prec):
-    lx, ly, lz = 16, 14, 12
-    cells = _cells_across_boundaries(lx, ly, lz)
-    o = oracle_of(lx, ly, lz, cells=cells)
-    g = gpu_of(lx, ly, lz, prec, cells=cells)
-    _set_positions(o, g, _wrapped(cells, lx, lz))
+@pytest.mark.parametrize("lx,ly,lz,py", SHAPES)
+def test_ibm_spread_parity(prec, lx, ly, lz, py):
+    print(f"[spread parity] prec={prec} shape={lx}x{ly}x{lz} periodic_y={py}", flush=True)
+    cells = _cells_across_boundaries(lx, ly, lz, periodic_y=py)
+    o = oracle_of(lx, ly, lz, cells=cells, periodic_y=py)
+    g = gpu_of(lx, ly, lz, prec, cells=cells, periodic_y=py)
+    _set_positions(o, g, _wrapped(cells, lx, ly, lz, py))
     rng = np.random.default_rng(7)
     fx, fy, fz = (1e-3 * rng.standard_normal(o.cells["x"].size) for _ in range(3))
     o.cells["cfx"][:], o.cells["cfy"][:], o.cells["cfz"][:] = fx, fy, fz
@@ -124,14 +130,16 @@ def _random_velocity_field(o, seed, amp=1e-2):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("rk,m,dx", [(4, 1, 1.0), (4, 2, 0.5), (2, 1, 1.0), (1, 3, 1.0)])
-def test_ibm_interp_parity(prec, rk, m, dx):
-    lx, ly, lz = 16, 14, 12
-    cells = _cells_across_boundaries(lx * dx, ly * dx, lz * dx, radius=3.0 * dx)
-    kw = dict(runge_kutta=rk, m=m, dx=dx, cr=0.7)
+@pytest.mark.parametrize("rk,m,dx,shape", [(4, 1, 1.0, SHAPES[0]), (4, 2, 0.5, SHAPES[0]), (2, 1, 1.0, SHAPES[0]),
+                                           (1, 3, 1.0, SHAPES[0]), (4, 1, 1.0, SHAPES[1]), (4, 1, 1.0, SHAPES[2])])
+def test_ibm_interp_parity(prec, rk, m, dx, shape):
+    lx, ly, lz, py = shape
+    print(f"[interp parity] prec={prec} rk={rk} m={m} dx={dx} shape={shape}", flush=True)
+    cells = _cells_across_boundaries(lx * dx, ly * dx, lz * dx, radius=3.0 * dx, periodic_y=py)
+    kw = dict(runge_kutta=rk, m=m, dx=dx, cr=0.7, periodic_y=py)
     o = oracle_of(lx, ly, lz, cells=cells, **kw)
     g = gpu_of(lx, ly, lz, prec, cells=cells, **kw)
-    _set_positions(o, g, _wrapped(cells, lx * dx, lz * dx))
+    _set_positions(o, g, _wrapped(cells, lx * dx, ly * dx, lz * dx, py))
     um, vm, wm = _random_velocity_field(o, seed=9)
     g.set_macros(um, vm, wm)
     o.ibm_st2()
@@ -211,7 +219,7 @@ def test_volume_parity(prec):
             if gc != 1:
                 d = np.where(d > L / 2, d - L, np.where(d < -L / 2, d + L, d))
             a[sl] = ref + 1.0033 * d
-    pos = _wrapped(dict(x=x, y=y, z=z), lx, lz)
+    pos = _wrapped(dict(x=x, y=y, z=z), lx, ly, lz)
     for k in "xyz":
         o.cells[k][:] = pos[k]
     g.set_cells_state(x=pos["x"], y=pos["y"], z=pos["z"])
@@ -288,3 +296,19 @@ def test_bicgstab_not_converged_is_reported():
     with pytest.raises(LbmibmError) as e:
         g.stress(1)
     assert e.value.code == 4
+
+
+def test_spread_is_bitwise_reproducible():
+    """Fixed-point LDS accumulation: repeated spreads give identical bits."""
+    lx, ly, lz = 24, 20, 24
+    cells = small_cells(lx, ly, lz, nc=3, radius=4.0, levels=[3, 2, 3], seed=51)
+    g = gpu_of(lx, ly, lz, "f32", cells=cells)
+    rng = np.random.default_rng(52)
+    n = cells["x"].size
+    g.set_cells_state(fx=rng.standard_normal(n), fy=rng.standard_normal(n), fz=rng.standard_normal(n))
+    g.ibm_st1()
+    F1 = [a.copy() for a in g.lattice_force()]
+    g.ibm_st1()
+    F2 = g.lattice_force()
+    for a, b in zip(F1, F2):
+        assert np.array_equal(a, b)
