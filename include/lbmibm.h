@@ -181,7 +181,12 @@ lbmibm_status lbmibm_mesh_biconcave(int nv, double *xyz);
 lbmibm_status lbmibm_place_cells(int nc, double r, double lxd, double lyd, double lzd, int periodic_y,
                                  unsigned long long seed, int mode, int gx, int gy, int gz,
                                  double *centres, double *rot);
-/* fem_init / cel_init connectivity (init1.cu:626-853, init2.cu:12-400). */
+/* fem_init / cel_init connectivity (init1.cu:626-853, init2.cu:12-400).
+ * LBMIBM_ERR_TOPOLOGY where the reference calls error(3): a vertex with more
+ * than 6 neighbours or 6 incident elements, an edge count that breaks
+ * Euler's formula, an unmatched edge, a degenerate reference triangle, or a
+ * cell of fewer than 86 vertices (per-cell system below 256 rows,
+ * init2.cu:286-287). */
 lbmibm_status lbmibm_build_topology(int nc, const int *voff, const int *eoff, const double *xr,
                                     const int *ele, int *nln, int *ln, int *lo, int *nle, int *le,
                                     double *nref, int *ec, double *c0, double *s0);

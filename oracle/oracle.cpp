@@ -834,6 +834,8 @@ int orc_fem_topology(int nc, const int *voff, const int *eoff, const double *xr,
                      int *nln, int *ln, int *lo, int *nle, int *le, double *nref, int *ec, double *c0,
                      double *s0) {
     const int nv = voff[nc], ne = eoff[nc];
+    for (int c = 0; c < nc; c++)  // fem_init rejects mn < 256 (init2.cu:286-287)
+        if (orc_expand2pow2(3 * (voff[c + 1] - voff[c])) < 256) return 3;
     for (int j = 0; j < nv; j++) {
         nln[j] = 0; nle[j] = 0;
         for (int k = 0; k < 7; k++) { ln[7 * j + k] = 0; lo[7 * j + k] = 0; }

@@ -185,6 +185,11 @@ lbmibm_status lbmibm_build_topology(int nc, const int *voff, const int *eoff, co
     for (int c = 0; c < nc; c++) {
         const int v0 = voff[c], v1 = voff[c + 1], e0 = eoff[c], e1 = eoff[c + 1];
         if (v1 <= v0 || e1 <= e0) return LBMIBM_ERR_TOPOLOGY;
+        // the per-cell system must have at least 256 rows after rounding 3V up
+        // to a power of two (init2.cu:286-287): meshes of 86+ vertices
+        int mn = 1;
+        while (mn < 3 * (v1 - v0)) mn *= 2;
+        if (mn < 256) return LBMIBM_ERR_TOPOLOGY;
         // Neighbour lists in first-seen order: for each element in order the
         // directed pairs (a,b),(a,c),(b,c),(b,a),(c,a),(c,b) (init2.cu:332-337);
         // incident elements in element order (init2.cu:339-341).
