@@ -168,8 +168,10 @@ lbmibm_status lbmibm_set_cell_force_scale(lbmibm_ctx *ctx, const double *scale);
 
 /* ---- host-side preprocessing (no GPU needed) ------------------------- */
 /* Icosphere of the given subdivision level on the unit sphere, outward
- * triangles: 10*4^level+2 vertices, 20*4^level triangles.  Call with NULL
- * arrays to get the counts.  Replaces ./cell/icosa_div files (init1.cu:490). */
+ * triangles: 10*4^level+2 vertices, 20*4^level triangles, vertices numbered
+ * along a Morton curve and triangles sorted by their lowest vertex id (each
+ * rotated to start there) for memory locality.  Call with NULL arrays to get
+ * the counts.  Replaces ./cell/icosa_div files (init1.cu:490). */
 lbmibm_status lbmibm_mesh_icosphere(int level, int *nv, int *nt, double *xyz, int *tri);
 /* Evans-Fung biconcave map of unit-sphere points (axis z), in place; replaces
  * ./cell/rbc_div files (init1.cu:506). */

@@ -1,0 +1,108 @@
+/*
+ * lbmibm_refcompat.hpp -- header-only adaptor that gives a reference-style
+ * driver (main.cu) the five stage functions it calls, by name, on top of the
+ * C ABI of lbmibm.h.
+ *
+ * The reference declares these in its def.h (not shipped) with C++ linkage
+ * and struct-pointer arguments:
+ *   void lbm_d3q19(process*, domain*, lattice*)                 lbm.cu:1463
+ *   void ibm_st1(domain*, lattice*, cell*)                       ibm.cu:299
+ *   void ibm_st2(domain*, lattice*, cell*, fem*)                 ibm.cu:642
+ *   void stress(process*, domain*, cell*, fem*, lattice*)        stress.cu:580
+ *   int  volume_correction(domain*, cell*, fem*)                 volume_correction.cu:453
+ *   void volume_init(domain*, cell*)                             volume_correction.cu:427
+ * The adaptor's functions are templates over those struct types, so they
+ * compile against whatever def.h defines; they read only prc->iter.  The
+ * device state lives in one lbmibm_ctx created by lbmibm_ref::attach() after
+ * the reference's own host initialisation (cdo_init, flu_init, cel_init,
+ * fem_init), which replaces mem_init.  lbmibm_ref::pull_lattice() and
+ * pull_cells() fill the reference's host mirrors (fnH, dnH..wmH, cel->x..,
+ * fem->x) for output_data / restart_file.
+ *
+ * Errors: the reference exits the process on a failed solve or volume
+ * correction (stress.cu:364-366, volume_correction.cu:490-493); the adaptor
+ * does the same after printing lbmibm_last_error(), so main.cu's behaviour is
+ * unchanged.
+ */
+#ifndef LBMIBM_REFCOMPAT_HPP
+#define LBMIBM_REFCOMPAT_HPP
+
+#include <cstdio>
+#include <cstdlib>
+
+#include "lbmibm.h"
+
+namespace lbmibm_ref {
+
+inline lbmibm_ctx *&context() {
+    static lbmibm_ctx *ctx = nullptr;
+    return ctx;
+}
+
+inline void check(lbmibm_status s, const char *where) {
+    if (s == LBMIBM_OK) return;
+    std::fprintf(stderr, "%s: %s\n", where, lbmibm_last_error(context()));
+    std::exit(1);
+}
+
+/* Creates the device context.  The caller fills p from its def.h constants
+ * (LX/LY/LZ over dx, Cr, Df, P, B, M, C, GS, BR, ITRMAX, RESIDUAL,
+ * RUNGE_KUTTA) and cells from cel->x/y/z and cel->ele (one mesh per cell
+ * species or any mix, as voff/eoff say). */
+inline void attach(const lbmibm_params &p, const lbmibm_cells &cells) {
+    check(lbmibm_create(&p, cells.nc > 0 ? &cells : nullptr, &context()), "lbmibm_create");
+}
+
+inline void detach() {
+    lbmibm_destroy(context());
+    context() = nullptr;
+}
+
+/* Host mirrors in the reference layouts: fn [n*19] post-stream populations,
+ * macros [n]; any pointer may be null. */
+inline void pull_lattice(double *fn, double *dn, double *un, double *vn, double *wn, double *dm, double *um,
+                         double *vm, double *wm) {
+    if (fn) check(lbmibm_download_populations(context(), fn), "lbmibm_download_populations");
+    check(lbmibm_download_macros(context(), dn, un, vn, wn, dm, um, vm, wm), "lbmibm_download_macros");
+}
+
+inline void pull_cells(double *x, double *y, double *z, double *u, double *v, double *w, double *fx, double *fy,
+                       double *fz, double *xd) {
+    check(lbmibm_download_cells(context(), x, y, z, u, v, w, fx, fy, fz, xd), "lbmibm_download_cells");
+}
+
+}  // namespace lbmibm_ref
+
+template <class Process, class Domain, class Lattice>
+inline void lbm_d3q19(Process *prc, Domain *, Lattice *) {
+    lbmibm_ref::check(lbmibm_lbm_d3q19(lbmibm_ref::context(), prc->iter), "lbm_d3q19");
+}
+
+template <class Domain, class Lattice, class Cell>
+inline void ibm_st1(Domain *, Lattice *, Cell *) {
+    lbmibm_ref::check(lbmibm_ibm_st1(lbmibm_ref::context()), "ibm_st1");
+}
+
+template <class Domain, class Lattice, class Cell, class Fem>
+inline void ibm_st2(Domain *, Lattice *, Cell *, Fem *) {
+    lbmibm_ref::check(lbmibm_ibm_st2(lbmibm_ref::context()), "ibm_st2");
+}
+
+template <class Process, class Domain, class Cell, class Fem, class Lattice>
+inline void stress(Process *prc, Domain *, Cell *, Fem *, Lattice *) {
+    lbmibm_ref::check(lbmibm_stress(lbmibm_ref::context(), prc->iter), "stress");
+}
+
+template <class Domain, class Cell>
+inline void volume_init(Domain *, Cell *) {
+    lbmibm_ref::check(lbmibm_volume_init(lbmibm_ref::context()), "volume_init");
+}
+
+template <class Domain, class Cell, class Fem>
+inline int volume_correction(Domain *, Cell *, Fem *) {
+    int iterations = 0;
+    lbmibm_ref::check(lbmibm_volume_correction(lbmibm_ref::context(), &iterations), "volume_correction");
+    return iterations;
+}
+
+#endif

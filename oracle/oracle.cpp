@@ -1015,7 +1015,10 @@ double orc_bending_energy(const orc_params *p, const orc_state *s) {
             const double cost = (xi[0] * ze[0] + xi[1] * ze[1] + xi[2] * ze[2]) / lxi / lze;
             const double side = (xi[0] - ze[0]) * (h.t1[0] - h.t2[0]) + (xi[1] - ze[1]) * (h.t1[1] - h.t2[1])
                               + (xi[2] - ze[2]) * (h.t1[2] - h.t2[2]);
-            double sint = std::sqrt(std::max(0.0, 1.0 - cost * cost));
+            // hinges that fourpoint_bend skips (sin^2 < 1e-6, stress.cu:536)
+            // carry no force: leave them out of the energy as well
+            if (1.0 - cost * cost < 1.0e-6) continue;
+            double sint = std::sqrt(1.0 - cost * cost);
             if (side < 0.0) sint = -sint;
             const double cost0 = s->c0[3 * e1 + k], sint0 = s->s0[3 * e1 + k];
             // each hinge is visited from both elements: half the energy each

@@ -160,7 +160,8 @@ int  orc_step(const orc_params *p, orc_state *s, int iter, int *vol_iters);
  * differences.  Uses xd and nref. */
 double orc_skalak_energy(const orc_params *p, const orc_state *s);
 /* Bending energy sum over hinges of (2/sqrt3) br (1 - cos(theta - theta0))
- * with the sign convention of fourpoint_bend, used to pin its forces. */
+ * with the sign convention of fourpoint_bend, used to pin its forces; like
+ * the force, it leaves out the nearly flat hinges (sin^2 < 1e-6). */
 double orc_bending_energy(const orc_params *p, const orc_state *s);
 
 /* OpenMP threads the oracle runs with (omp_get_max_threads). */

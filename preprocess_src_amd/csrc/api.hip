@@ -170,8 +170,8 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->qi, ctx->coef, ctx->bv, ctx->rv, ctx->rh, ctx->pv, ctx->vv, ctx->tv, ctx->gscale,
                     ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->bicg_iters, ctx->vol_passes,
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
-                    ctx->tile_off, ctx->tile_cur, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->small_cells,
-                    ctx->big_cells};
+                    ctx->tile_off, ctx->tile_cur, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
+                    ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -339,20 +339,27 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         for (void **v : {&ctx->qv, &ctx->qi, &ctx->bv, &ctx->rv, &ctx->rh, &ctx->pv, &ctx->vv, &ctx->tv})
             CK(up_T(v, nullptr, 3 * (size_t)nv));
         CK(up_T(&ctx->coef, nullptr, 6 * (size_t)nv));
+        CK(up_T(&ctx->dsv, nullptr, (size_t)nv));
         CK(dev_alloc(ctx, (void **)&ctx->nbr, 6 * (size_t)nv * sizeof(unsigned short)));
         // cells solved on chip and the rest
-        std::vector<int> small, big;
-        const int lim = bicg_onchip_limit(ctx->prec);
+        std::vector<int> cls[3];
         for (int c = 0; c < nc; c++) {
             const int V = cells->voff[c + 1] - cells->voff[c];
             if (V > 65535) return fail(LBMIBM_ERR_ARG);  // 16-bit local column ids
-            if (V <= lim) { small.push_back(c); ctx->max_small_v = std::max(ctx->max_small_v, V); }
-            else big.push_back(c);
+            const int k = bicg_class(ctx->prec, V);
+            cls[k].push_back(c);
+            ctx->bicg_maxv[k] = std::max(ctx->bicg_maxv[k], V);
         }
-        ctx->n_small = (int)small.size();
-        ctx->n_big = (int)big.size();
-        if (!small.empty()) CK(up_i(&ctx->small_cells, small.data(), small.size()));
-        if (!big.empty()) CK(up_i(&ctx->big_cells, big.data(), big.size()));
+        for (int k = 0; k < 3; k++) {
+            ctx->bicg_n[k] = (int)cls[k].size();
+            if (!cls[k].empty()) CK(up_i(&ctx->bicg_cells[k], cls[k].data(), cls[k].size()));
+        }
+        std::vector<int> vcls[2];
+        for (int c = 0; c < nc; c++) vcls[cells->voff[c + 1] - cells->voff[c] > VOL_SMALL].push_back(c);
+        for (int k = 0; k < 2; k++) {
+            ctx->vol_n[k] = (int)vcls[k].size();
+            if (!vcls[k].empty()) CK(up_i(&ctx->vol_cells[k], vcls[k].data(), vcls[k].size()));
+        }
         CK(membrane_setup(ctx));
         std::vector<double> ones(nc, 1.0);
         CK(up_T(&ctx->gscale, ones.data(), nc));

@@ -31,6 +31,7 @@ struct DevStatus {
 };
 
 // Lattice tiles of the immersed-boundary kernels.
+constexpr int VOL_SMALL = 3072;
 constexpr int TILE_X = 16, TILE_Y = 8, TILE_Z = 8;
 
 }  // namespace lbmibm
@@ -71,10 +72,16 @@ struct lbmibm_ctx {
     void *qv = nullptr, *qi = nullptr;                 // [3Nv]
     void *coef = nullptr;                              // [6][Nv] off-diagonal Jacobi-scaled coefficients
     unsigned short *nbr = nullptr;                     // [6][Nv] cell-local neighbour ids (self for unused)
-    int *small_cells = nullptr, *big_cells = nullptr;  // cell ids solved on chip / from global memory
-    int n_small = 0, n_big = 0, max_small_v = 0;
+    // BiCGSTAB cell classes: 0 on chip with one vertex per thread, 1 on chip
+    // with three vertices per thread, 2 vectors in global memory
+    int *bicg_cells[3] = {nullptr, nullptr, nullptr};
+    int bicg_n[3] = {0, 0, 0}, bicg_maxv[3] = {0, 0, 0};
+    // volume-correction work-group classes: up to VOL_SMALL vertices, larger
+    int *vol_cells[2] = {nullptr, nullptr};
+    int vol_n[2] = {0, 0};
     void *bv = nullptr, *rv = nullptr, *rh = nullptr, *pv = nullptr, *vv = nullptr, *tv = nullptr;
     void *gscale = nullptr;                            // [nc] (T)
+    void *dsv = nullptr;                               // [Nv] ring area x force scale (T)
     double *normal = nullptr;                          // [3Nt] FP64 element normals
     double *vol0 = nullptr, *vol = nullptr, *area = nullptr, *gx = nullptr;  // [nc], [3nc]
     int *bicg_iters = nullptr, *vol_passes = nullptr;  // [nc]
@@ -112,7 +119,7 @@ lbmibm_status ibm_interp(lbmibm_ctx *ctx);                   // ibm.hip
 lbmibm_status ibm_bin(lbmibm_ctx *ctx);                      // ibm.hip
 lbmibm_status ibm_setup(lbmibm_ctx *ctx);                    // ibm.hip
 lbmibm_status membrane_forces(lbmibm_ctx *ctx);              // membrane.hip
-int bicg_onchip_limit(int prec);                             // membrane.hip
+int bicg_class(int prec, int nvert);                         // membrane.hip
 lbmibm_status membrane_setup(lbmibm_ctx *ctx);               // membrane.hip
 lbmibm_status volume_run(lbmibm_ctx *ctx, int mode);         // volume.hip (0 init, 1 correct)
 lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool advance);  // api.hip
@@ -150,15 +157,58 @@ template <typename T> __device__ __forceinline__ T dev_min_image(T d, T L) {
     return d;
 }
 
-// FP64 sum over a 256-thread work group in a fixed order (deterministic).
-__device__ __forceinline__ double block_sum(double v, double *sh) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[w] = v;
-    __syncthreads();
-    return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+// ---- FP64 reductions ------------------------------------------------------------
+//
+// Wave sums by DPP row shifts and row broadcasts (a fixed pattern, so the
+// result does not depend on timing), then one value per wave through LDS and
+// a sequential sum of the wave totals by every thread.  Buffers alternate
+// between consecutive reductions, so each needs a single barrier: a buffer
+// is rewritten only after the barrier of the reduction in between.
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    int lo = (int)(unsigned)(u & 0xffffffffull), hi = (int)(unsigned)(u >> 32);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
+
+// Sum over the 64 lanes, returned to every lane.
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_move<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_move<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_move<0x118, 0xf>(v);  // row_shr:8  -> lane 15 of each row holds the row sum
+    v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3 -> lane 63 holds the sum
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffull), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+struct WgReduce {
+    double *sh;  // [2][4][16]
+    int parity;
+    template <int NV> __device__ __forceinline__ void sum(double (&v)[NV]) {
+        double *buf = sh + parity * 64;
+        parity ^= 1;
+        const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+        for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+        if ((threadIdx.x & 63) == 0)
+#pragma unroll
+            for (int i = 0; i < NV; i++) buf[i * 16 + w] = v[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            double s = 0.0;
+            for (int k = 0; k < nw; k++) s += buf[i * 16 + k];
+            v[i] = s;
+        }
+    }
+};
+
 
 }  // namespace lbmibm

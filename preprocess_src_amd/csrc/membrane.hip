@@ -14,7 +14,9 @@
 //     inner products reduced in a fixed order, device-side convergence and
 //     breakdown flags); cells of up to 3072 vertices keep every vector and
 //     their matrix rows in registers and LDS, larger ones use global memory;
-//   - k_q2f and k_bend as the reference.
+//   - q2f is fused: k_assemble also forms the ring area ds of each vertex
+//     and the solver epilogue writes f = -q ds with the load q;
+//   - k_bend as the reference.
 // Vertex positions, the reference metric and all geometric arithmetic
 // (edges, metric invariants, tension, matrix entries, ring areas, hinge
 // angles) are FP64 in every precision mode: the strains are small
@@ -97,8 +99,9 @@ __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ xd,
                                                   const int *__restrict__ nln, const int *__restrict__ lo8,
                                                   const int *__restrict__ le, const T *__restrict__ rg,
                                                   const T *__restrict__ tt, const int *__restrict__ vcell,
-                                                  const int *__restrict__ voff, T *__restrict__ coef,
-                                                  unsigned short *__restrict__ nbr, T *__restrict__ bv, int nv,
+                                                  const int *__restrict__ voff, const T *__restrict__ gscale,
+                                                  T *__restrict__ coef, unsigned short *__restrict__ nbr,
+                                                  T *__restrict__ bv, T *__restrict__ dsv, int nv,
                                                   PeriodicBox box) {
     const int jj = blockIdx.x * blockDim.x + threadIdx.x;
     if (jj >= nv) return;
@@ -108,10 +111,12 @@ __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ xd,
     for (int k = 0; k < 8; k++) lo[k] = lo8[8 * jj + k];
     double tv[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double tb[3] = {0.0, 0.0, 0.0};
+    double ds = 0.0;  // q2f ring area (stress.cu:415-438): a third of the incident triangles
     int order = 0;
     for (int i = 0; i < n; i++) {
         const int e = le[6 * jj + i];
         const double gab = (double)rg[e];
+        ds += gab / 2.0 / 3.0;  // rg = |e1 x e2| = twice the triangle area
         const int ea = ele[3 * e], eb = ele[3 * e + 1], ec = ele[3 * e + 2];
         int oa = 0, ob = 0, oc = 0;
 #pragma unroll
@@ -176,56 +181,82 @@ __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ xd,
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
+    dsv[jj] = (T)(ds * (double)gscale[vcell[jj]]);
 }
 
-// FP64 sums over a 1024-thread work group in a fixed order (deterministic).
-template <int NV>
-__device__ __forceinline__ void wg_sum(double (&v)[NV], double *sh) {
+// Solver epilogue per vertex: the load q (fem->q, substitute_load
+// stress.cu:275-303), its output accumulator qi, and the nodal force of q2f
+// f = -q ds (stress.cu:440-442) with ds the ring area times the cell's force
+// scale.
+template <typename T> struct SolveOut {
+    T *q, *cf, *qi;
+    const T *ds;
+    const DevStep *st;
+    int nv, fo;   // fo = FILEOUTPUT period of qi (0: no accumulator)
+    double wq;    // M / FILEOUTPUT
+    __device__ __forceinline__ void put(size_t g, const T x[3]) const {
+        const double d = (double)ds[g];
 #pragma unroll
-    for (int i = 0; i < NV; i++) {
+        for (int k = 0; k < 3; k++) {
+            q[3 * g + k] = x[k];
+            cf[(size_t)k * nv + g] = (T)(-1.0 * (double)x[k] * d);
+        }
+        if (fo > 0) {
+            const bool reset = (st->iter - 1) % fo == 0;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o);
+            for (int k = 0; k < 3; k++) {
+                const double prev = reset ? 0.0 : (double)qi[3 * g + k];
+                qi[3 * g + k] = (T)(prev + wq * (double)x[k]);
+            }
+        }
     }
-    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0)
-        for (int i = 0; i < NV; i++) sh[i * 16 + w] = v[i];
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NV; i++) {
-        double s = 0.0;
-        for (int k = 0; k < nw; k++) s += sh[i * 16 + k];
-        v[i] = s;
-    }
+};
+
+// Three components of vertex j from an LDS vector stored as [V][4].
+__device__ __forceinline__ void lds_load3(const float *a, int j, float o[3]) {
+    const float4 q = *reinterpret_cast<const float4 *>(a + 4 * j);
+    o[0] = q.x; o[1] = q.y; o[2] = q.z;
+}
+__device__ __forceinline__ void lds_load3(const double *a, int j, double o[3]) {
+    const double2 q = *reinterpret_cast<const double2 *>(a + 4 * j);
+    o[0] = q.x; o[1] = q.y; o[2] = a[4 * j + 2];
+}
+template <typename T> __device__ __forceinline__ void lds_store3(T *a, int j, const T v[3]) {
+    a[4 * j] = v[0]; a[4 * j + 1] = v[1]; a[4 * j + 2] = v[2];
 }
 
-// On-chip BiCGSTAB for one cell of at most 1024*KMAX vertices: every thread
+// Work-group shapes of the two on-chip classes (vertices per thread, threads).
+constexpr int BICG0_K = 4, BICG0_NT = 256;
+constexpr int BICG1_K = 6, BICG1_NT = 512;
+
+// On-chip BiCGSTAB for one cell of at most NT*KMAX vertices: every thread
 // keeps the rows of its vertices (x, r, p, v, t and the 6 off-diagonal
-// coefficients) in registers; p and r are mirrored in LDS for the neighbour
-// gathers of the matrix-vector products.  matrix_solver.cu:279-377 in the
-// same update order; (r, r^) of the next iteration is fused with the
-// residual reduction; r^ = b because x starts at 0 (stress.cu:139).
-template <typename T, int KMAX>
-__global__ __launch_bounds__(1024) void k_bicg_onchip(const int *__restrict__ cells, const int *__restrict__ voff,
+// coefficients) in registers; p, r and b are mirrored in LDS ([V][4]) for
+// the neighbour gathers of the matrix-vector products.  matrix_solver.cu:
+// 279-377 in the same update order; (r, r^) of the next iteration is fused
+// with the residual reduction; r^ = b because x starts at 0 (stress.cu:139).
+template <typename T, int KMAX, int NT>
+__global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cells, const int *__restrict__ voff,
                                                       const T *__restrict__ coef,
                                                       const unsigned short *__restrict__ nbr,
-                                                      const T *__restrict__ bv, T *__restrict__ xout, int nv,
+                                                      const T *__restrict__ bv, SolveOut<T> out, int nv,
                                                       int itrmax, double residual, int *__restrict__ iters,
                                                       DevStatus *__restrict__ status) {
     extern __shared__ double dyn[];
-    __shared__ double sh[2 * 16];
+    __shared__ double sh[128];
+    WgReduce red{sh, 0};
     const int c = cells[blockIdx.x];
     const int v0 = voff[c], V = voff[c + 1] - v0;
-    T *Ps = (T *)dyn;              // [V][3]
-    T *Rs = Ps + 3 * (size_t)V;    // [V][3]
-    T *Bs = Rs + 3 * (size_t)V;    // [V][3] right-hand side = r^
+    T *Ps = (T *)dyn;              // [V][4]
+    T *Rs = Ps + 4 * (size_t)V;    // [V][4]
+    T *Bs = Rs + 4 * (size_t)V;    // [V][4] right-hand side = r^
     const int tid = threadIdx.x;
     T x[KMAX][3], r[KMAX][3], p[KMAX][3], v[KMAX][3], t[KMAX][3], cf[KMAX][6];
     unsigned nb[KMAX][3];  // two 16-bit column ids per register
     double acc[2] = {0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
-        const int j = tid + k * 1024;
+        const int j = tid + k * NT;
         const bool on = j < V;
         const size_t g = (size_t)v0 + (on ? j : 0);
 #pragma unroll
@@ -239,21 +270,30 @@ __global__ __launch_bounds__(1024) void k_bicg_onchip(const int *__restrict__ ce
             const T bb = on ? bv[3 * g + d] : T(0);
             x[k][d] = T(0); r[k][d] = bb; p[k][d] = bb; v[k][d] = T(0); t[k][d] = T(0);
             acc[0] += (double)bb * (double)bb;
-            if (on) { Ps[3 * j + d] = bb; Bs[3 * j + d] = bb; }
         }
+        if (on) { lds_store3(Ps, j, p[k]); lds_store3(Bs, j, p[k]); }
     }
-    wg_sum<1>(*(double(*)[1])acc, sh);  // orders the LDS writes above
+    red.sum<1>(*(double(*)[1])acc);  // its barrier also orders the LDS writes above
     const double bb = acc[0];
     if (bb == 0.0) {  // exactly zero load: x = 0 solves the system
+        const T zero[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            const int j = tid + k * 1024;
-            if (j < V)
-                for (int d = 0; d < 3; d++) xout[3 * ((size_t)v0 + j) + d] = T(0);
+            const int j = tid + k * NT;
+            if (j < V) out.put((size_t)v0 + j, zero);
         }
         if (tid == 0) iters[c] = 0;
         return;
     }
+    auto spmv = [&](const T *src, int k, const T own[3], T y[3]) {
+        y[0] = own[0]; y[1] = own[1]; y[2] = own[2];
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            T o[3];
+            lds_load3(src, (int)((nb[k][m >> 1] >> (16 * (m & 1))) & 0xffffu), o);
+            y[0] += cf[k][m] * o[0]; y[1] += cf[k][m] * o[1]; y[2] += cf[k][m] * o[2];
+        }
+    };
     double rr = bb, rr_old = 0.0, alfa = 1.0, omeg = 1.0;
     int it = 0, flag = 0;
     bool conv = false;
@@ -263,12 +303,10 @@ __global__ __launch_bounds__(1024) void k_bicg_onchip(const int *__restrict__ ce
             const T beta = (T)((rr / rr_old) * (alfa / omeg)), om = (T)omeg;
 #pragma unroll
             for (int k = 0; k < KMAX; k++) {
-                const int j = tid + k * 1024;
+                const int j = tid + k * NT;
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    p[k][d] = r[k][d] + beta * (p[k][d] - om * v[k][d]);
-                    if (j < V) Ps[3 * j + d] = p[k][d];
-                }
+                for (int d = 0; d < 3; d++) p[k][d] = r[k][d] + beta * (p[k][d] - om * v[k][d]);
+                if (j < V) lds_store3(Ps, j, p[k]);
             }
             __syncthreads();
         }
@@ -276,57 +314,51 @@ __global__ __launch_bounds__(1024) void k_bicg_onchip(const int *__restrict__ ce
         acc[0] = 0.0;
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            const int j = tid + k * 1024;
+            const int j = tid + k * NT;
             if (j < V) {
+                spmv(Ps, k, p[k], v[k]);
+                T b3[3];
+                lds_load3(Bs, j, b3);
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    T y = p[k][d];
-#pragma unroll
-                    for (int m = 0; m < 6; m++) y += cf[k][m] * Ps[3 * ((nb[k][m >> 1] >> (16 * (m & 1))) & 0xffffu) + d];
-                    v[k][d] = y;
-                    acc[0] += (double)y * (double)Bs[3 * j + d];
-                }
+                for (int d = 0; d < 3; d++) acc[0] += (double)v[k][d] * (double)b3[d];
             }
         }
-        wg_sum<1>(*(double(*)[1])acc, sh);
+        red.sum<1>(*(double(*)[1])acc);
         alfa = rr / acc[0];
         const T al = (T)alfa;
         // r = r - alfa v
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            const int j = tid + k * 1024;
+            const int j = tid + k * NT;
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                r[k][d] = r[k][d] - al * v[k][d];
-                if (j < V) Rs[3 * j + d] = r[k][d];
-            }
+            for (int d = 0; d < 3; d++) r[k][d] = r[k][d] - al * v[k][d];
+            if (j < V) lds_store3(Rs, j, r[k]);
         }
         __syncthreads();
         // t = A r ; omeg = (r, t) / (t, t)
         acc[0] = 0.0; acc[1] = 0.0;
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            const int j = tid + k * 1024;
+            const int j = tid + k * NT;
             if (j < V) {
+                spmv(Rs, k, r[k], t[k]);
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    T y = r[k][d];
-#pragma unroll
-                    for (int m = 0; m < 6; m++) y += cf[k][m] * Rs[3 * ((nb[k][m >> 1] >> (16 * (m & 1))) & 0xffffu) + d];
-                    t[k][d] = y;
-                    acc[0] += (double)r[k][d] * (double)y;
-                    acc[1] += (double)y * (double)y;
+                    acc[0] += (double)r[k][d] * (double)t[k][d];
+                    acc[1] += (double)t[k][d] * (double)t[k][d];
                 }
             }
         }
-        wg_sum<2>(acc, sh);
+        red.sum<2>(acc);
         omeg = acc[0] / acc[1];
         const T om = (T)omeg;
         // x = x + alfa p + omeg r ; r = r - omeg t
         acc[0] = 0.0; acc[1] = 0.0;
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            const int j = tid + k * 1024;
+            const int j = tid + k * NT;
+            T b3[3] = {T(0), T(0), T(0)};
+            if (j < V) lds_load3(Bs, j, b3);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const T ri = r[k][d];
@@ -335,20 +367,19 @@ __global__ __launch_bounds__(1024) void k_bicg_onchip(const int *__restrict__ ce
                 r[k][d] = rn;
                 if (j < V) {
                     acc[0] += (double)rn * (double)rn;
-                    acc[1] += (double)rn * (double)Bs[3 * j + d];
+                    acc[1] += (double)rn * (double)b3[d];
                 }
             }
         }
-        wg_sum<2>(acc, sh);
+        red.sum<2>(acc);
         rr_old = rr;
         rr = acc[1];
         if (sqrt(acc[0] / bb) < residual) { conv = true; break; }
     }
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
-        const int j = tid + k * 1024;
-        if (j < V)
-            for (int d = 0; d < 3; d++) xout[3 * ((size_t)v0 + j) + d] = x[k][d];
+        const int j = tid + k * NT;
+        if (j < V) out.put((size_t)v0 + j, x[k]);
     }
     if (tid == 0) {
         iters[c] = conv ? it + 1 : it;
@@ -358,27 +389,45 @@ __global__ __launch_bounds__(1024) void k_bicg_onchip(const int *__restrict__ ce
 }
 
 // Same iteration for cells of any size with the vectors in global memory.
+// Each thread walks its vertices in batches of GB so that the loads of a
+// batch are in flight together.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ cells, const int *__restrict__ voff,
                                                       const T *__restrict__ coef,
                                                       const unsigned short *__restrict__ nbr,
                                                       const T *__restrict__ bv, T *__restrict__ x,
                                                       T *__restrict__ r, T *__restrict__ p, T *__restrict__ v,
-                                                      T *__restrict__ t, int nv, int itrmax, double residual,
-                                                      int *__restrict__ iters, DevStatus *__restrict__ status) {
-    __shared__ double sh[2 * 16];
+                                                      T *__restrict__ t, SolveOut<T> out, int nv, int itrmax,
+                                                      double residual, int *__restrict__ iters,
+                                                      DevStatus *__restrict__ status) {
+    __shared__ double sh[128];
+    WgReduce red{sh, 0};
     const int c = cells[blockIdx.x];
     const int v0 = voff[c], v1 = voff[c + 1];
     const int tid = threadIdx.x, nt = blockDim.x;
-    auto apply = [&](const T *__restrict__ in, int vv, T y[3]) {
-        const T *own = in + 3 * (size_t)vv;
-        y[0] = own[0]; y[1] = own[1]; y[2] = own[2];
+    constexpr int GB = sizeof(T) == 4 ? 4 : 2;
+    // y = A in for the vertices base + i*nt, i < GB (those below v1)
+    auto apply = [&](const T *__restrict__ in, int base, T y[GB][3]) {
+        int col[GB][6];
+        T cm[GB][6];
 #pragma unroll
-        for (int m = 0; m < 6; m++) {
-            const T cm = coef[(size_t)m * nv + vv];
-            const T *o = in + 3 * ((size_t)v0 + nbr[(size_t)m * nv + vv]);
-            y[0] += cm * o[0]; y[1] += cm * o[1]; y[2] += cm * o[2];
+        for (int i = 0; i < GB; i++) {
+            const int vv = min(base + i * nt, v1 - 1);
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+                cm[i][m] = coef[(size_t)m * nv + vv];
+                col[i][m] = v0 + nbr[(size_t)m * nv + vv];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) y[i][d] = in[3 * (size_t)vv + d];
         }
+#pragma unroll
+        for (int i = 0; i < GB; i++)
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+                const T *o = in + 3 * (size_t)col[i][m];
+                y[i][0] += cm[i][m] * o[0]; y[i][1] += cm[i][m] * o[1]; y[i][2] += cm[i][m] * o[2];
+            }
     };
     double acc[2] = {0.0, 0.0};
     for (int vv = v0 + tid; vv < v1; vv += nt)
@@ -388,9 +437,13 @@ __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ ce
             x[k] = T(0); r[k] = b; p[k] = b;
             acc[0] += (double)b * (double)b;
         }
-    wg_sum<1>(*(double(*)[1])acc, sh);
+    red.sum<1>(*(double(*)[1])acc);
     const double bb = acc[0];
     if (bb == 0.0) {
+        for (int vv = v0 + tid; vv < v1; vv += nt) {
+            const T zero[3] = {T(0), T(0), T(0)};
+            out.put(vv, zero);
+        }
         if (tid == 0) iters[c] = 0;
         return;
     }
@@ -409,16 +462,21 @@ __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ ce
             __syncthreads();
         }
         acc[0] = 0.0;
-        for (int vv = v0 + tid; vv < v1; vv += nt) {
-            T y[3];
-            apply(p, vv, y);
-            for (int d = 0; d < 3; d++) {
-                const size_t k = 3 * (size_t)vv + d;
-                v[k] = y[d];
-                acc[0] += (double)y[d] * (double)bv[k];
+        for (int base = v0 + tid; base < v1; base += GB * nt) {
+            T y[GB][3];
+            apply(p, base, y);
+#pragma unroll
+            for (int i = 0; i < GB; i++) {
+                const int vv = base + i * nt;
+                if (vv < v1)
+                    for (int d = 0; d < 3; d++) {
+                        const size_t k = 3 * (size_t)vv + d;
+                        v[k] = y[i][d];
+                        acc[0] += (double)y[i][d] * (double)bv[k];
+                    }
             }
         }
-        wg_sum<1>(*(double(*)[1])acc, sh);
+        red.sum<1>(*(double(*)[1])acc);
         alfa = rr / acc[0];
         for (int vv = v0 + tid; vv < v1; vv += nt)
             for (int d = 0; d < 3; d++) {
@@ -427,17 +485,22 @@ __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ ce
             }
         __syncthreads();
         acc[0] = 0.0; acc[1] = 0.0;
-        for (int vv = v0 + tid; vv < v1; vv += nt) {
-            T y[3];
-            apply(r, vv, y);
-            for (int d = 0; d < 3; d++) {
-                const size_t k = 3 * (size_t)vv + d;
-                t[k] = y[d];
-                acc[0] += (double)r[k] * (double)y[d];
-                acc[1] += (double)y[d] * (double)y[d];
+        for (int base = v0 + tid; base < v1; base += GB * nt) {
+            T y[GB][3];
+            apply(r, base, y);
+#pragma unroll
+            for (int i = 0; i < GB; i++) {
+                const int vv = base + i * nt;
+                if (vv < v1)
+                    for (int d = 0; d < 3; d++) {
+                        const size_t k = 3 * (size_t)vv + d;
+                        t[k] = y[i][d];
+                        acc[0] += (double)r[k] * (double)y[i][d];
+                        acc[1] += (double)y[i][d] * (double)y[i][d];
+                    }
             }
         }
-        wg_sum<2>(acc, sh);
+        red.sum<2>(acc);
         omeg = acc[0] / acc[1];
         acc[0] = 0.0; acc[1] = 0.0;
         for (int vv = v0 + tid; vv < v1; vv += nt)
@@ -450,43 +513,21 @@ __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ ce
                 acc[0] += (double)rn * (double)rn;
                 acc[1] += (double)rn * (double)bv[k];
             }
-        wg_sum<2>(acc, sh);
+        red.sum<2>(acc);
         rr_old = rr;
         rr = acc[1];
         if (sqrt(acc[0] / bb) < residual) { conv = true; break; }
+    }
+    __syncthreads();  // every thread's last x update is in memory
+    for (int vv = v0 + tid; vv < v1; vv += nt) {
+        const T xv[3] = {x[3 * (size_t)vv], x[3 * (size_t)vv + 1], x[3 * (size_t)vv + 2]};
+        out.put(vv, xv);
     }
     if (tid == 0) {
         iters[c] = conv ? it + 1 : it;
         if (!conv && flag == 0) flag = 2;
         if (flag) atomicMax(&status->bicg_flag, flag);
     }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_q2f(const double *__restrict__ xd, const T *__restrict__ q,
-                                             const int *__restrict__ nln, const int *__restrict__ ln,
-                                             const int *__restrict__ vcell, const T *__restrict__ gscale,
-                                             T *__restrict__ cf, int nv, PeriodicBox box) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nv) return;
-    const double x1 = xd[3 * j], y1 = xd[3 * j + 1], z1 = xd[3 * j + 2];
-    const int n = nln[j];
-    double ds = 0.0;
-    for (int k = 0; k < n; k++) {
-        const int j2 = ln[7 * j + k], j3 = ln[7 * j + (k + 1) % n];
-        double a[3] = {xd[3 * j2] - x1, xd[3 * j2 + 1] - y1, xd[3 * j2 + 2] - z1};
-        double b[3] = {xd[3 * j3] - x1, xd[3 * j3 + 1] - y1, xd[3 * j3 + 2] - z1};
-        min_img3<double>(a, box);
-        min_img3<double>(b, box);
-        const double cxv = -(a[1] * b[2] - a[2] * b[1]);
-        const double cyv = -(a[2] * b[0] - a[0] * b[2]);
-        const double czv = -(a[0] * b[1] - a[1] * b[0]);
-        ds += sqrt(cxv * cxv + cyv * cyv + czv * czv) / 2.0 / 3.0;
-    }
-    const double s = (double)gscale[vcell[j]];
-    cf[j] = (T)(-1.0 * (double)q[3 * j] * ds * s);
-    cf[nv + j] = (T)(-1.0 * (double)q[3 * j + 1] * ds * s);
-    cf[2 * nv + j] = (T)(-1.0 * (double)q[3 * j + 2] * ds * s);
 }
 
 template <typename T>
@@ -573,7 +614,6 @@ __global__ __launch_bounds__(256) void k_bend(const double *__restrict__ xs, con
 }
 
 template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
-    const int bicg_onchip_max_v = ctx->max_small_v;
     const lbmibm_params &p = ctx->p;
     PeriodicBox box{ctx->nx * p.dx, ctx->ny * p.dx, ctx->nz * p.dx, p.periodic_y};
     TensionArgs ta;
@@ -587,26 +627,42 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
     LBMIBM_LAUNCHED(ctx, "k_tension");
     hipLaunchKernelGGL(k_assemble<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
                        ctx->ele, ctx->nln, ctx->lo8, ctx->le, (const T *)ctx->rg, (const T *)ctx->tt, ctx->vcell,
-                       ctx->voff, (T *)ctx->coef, ctx->nbr, (T *)ctx->bv, nv, box);
+                       ctx->voff, (const T *)ctx->gscale, (T *)ctx->coef, ctx->nbr, (T *)ctx->bv, (T *)ctx->dsv, nv,
+                       box);
     LBMIBM_LAUNCHED(ctx, "k_assemble");
-    if (ctx->n_small > 0 && sizeof(T) == 4) {
-        const size_t lds = 3 * 3 * (size_t)bicg_onchip_max_v * sizeof(T);
-        hipLaunchKernelGGL((k_bicg_onchip<float, 3>), dim3(ctx->n_small), dim3(1024), lds, ctx->stream, ctx->small_cells,
-                           ctx->voff, (const float *)ctx->coef, ctx->nbr, (const float *)ctx->bv, (float *)ctx->qv,
-                           nv, p.itrmax, p.residual, ctx->bicg_iters, ctx->status);
-        LBMIBM_LAUNCHED(ctx, "k_bicg_onchip");
+    SolveOut<T> out;
+    out.q = (T *)ctx->qv;
+    out.cf = (T *)ctx->cf;
+    out.qi = (T *)ctx->qi;
+    out.ds = (const T *)ctx->dsv;
+    out.st = ctx->step;
+    out.nv = nv;
+    out.fo = p.fileoutput > 0 ? p.fileoutput : 0;
+    out.wq = p.fileoutput > 0 ? (double)p.m / (double)p.fileoutput : 0.0;
+    if (ctx->bicg_n[0] > 0) {
+        const size_t lds = 3 * 4 * (size_t)ctx->bicg_maxv[0] * sizeof(T);
+        hipLaunchKernelGGL((k_bicg_onchip<T, BICG0_K, BICG0_NT>), dim3(ctx->bicg_n[0]), dim3(BICG0_NT), lds,
+                           ctx->stream, ctx->bicg_cells[0], ctx->voff, (const T *)ctx->coef, ctx->nbr,
+                           (const T *)ctx->bv, out, nv, p.itrmax, p.residual, ctx->bicg_iters, ctx->status);
+        LBMIBM_LAUNCHED(ctx, "k_bicg_onchip (class 0)");
     }
-    if (ctx->n_big > 0) {
-        hipLaunchKernelGGL(k_bicg_global<T>, dim3(ctx->n_big), dim3(1024), 0, ctx->stream, ctx->big_cells, ctx->voff,
-                           (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->qv, (T *)ctx->rv,
-                           (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, nv, p.itrmax, p.residual, ctx->bicg_iters,
-                           ctx->status);
+    if constexpr (sizeof(T) == 4) {  // class 1 is FP32 only (bicg_class)
+        if (ctx->bicg_n[1] > 0) {
+            const size_t lds = 3 * 4 * (size_t)ctx->bicg_maxv[1] * sizeof(float);
+            hipLaunchKernelGGL((k_bicg_onchip<float, BICG1_K, BICG1_NT>), dim3(ctx->bicg_n[1]), dim3(BICG1_NT), lds,
+                               ctx->stream, ctx->bicg_cells[1], ctx->voff, (const float *)ctx->coef, ctx->nbr,
+                               (const float *)ctx->bv, out, nv, p.itrmax, p.residual, ctx->bicg_iters, ctx->status);
+            LBMIBM_LAUNCHED(ctx, "k_bicg_onchip (class 1)");
+        }
+    }
+    if (ctx->bicg_n[2] > 0) {
+        // x in its own buffer; the epilogue writes q
+        hipLaunchKernelGGL(k_bicg_global<T>, dim3(ctx->bicg_n[2]), dim3(1024), 0, ctx->stream, ctx->bicg_cells[2],
+                           ctx->voff, (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->rh,
+                           (T *)ctx->rv, (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, out, nv, p.itrmax, p.residual,
+                           ctx->bicg_iters, ctx->status);
         LBMIBM_LAUNCHED(ctx, "k_bicg_global");
     }
-    hipLaunchKernelGGL(k_q2f<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
-                       (const T *)ctx->qv, ctx->nln, ctx->ln, ctx->vcell, (const T *)ctx->gscale, (T *)ctx->cf,
-                       nv, box);
-    LBMIBM_LAUNCHED(ctx, "k_q2f");
     if (p.br != 0.0) {
         hipLaunchKernelGGL(k_bend<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
                            ctx->ele, ctx->ec, (const double *)ctx->c0, (const double *)ctx->s0, (T *)ctx->cf, nv, ne, p.br,
@@ -618,18 +674,32 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
 
 }  // namespace
 
-int bicg_onchip_limit(int prec) {
-    // cells solved on chip: 3 vertices per thread of a 1024-thread work group,
-    // in FP32 only (the FP64 rows do not fit the register budget)
-    return prec == LBMIBM_F64 ? 0 : 3 * 1024;
+int bicg_class(int prec, int nvert) {
+    // on chip: up to 1024 vertices with 4 per thread of a 256-thread work
+    // group (any precision), up to 3072 with 6 per thread of a 512-thread
+    // work group (FP32: the FP64 rows do not fit the register budget); else
+    // vectors in global memory
+    if (nvert <= BICG0_K * BICG0_NT) return 0;
+    if (nvert <= BICG1_K * BICG1_NT && prec != LBMIBM_F64) return 1;
+    return 2;
 }
 
 lbmibm_status membrane_setup(lbmibm_ctx *ctx) {
-    // allow the on-chip solver the LDS its largest cell needs (p, r and b)
-    if (ctx->n_small == 0) return LBMIBM_OK;
-    const int lds = (int)(3 * 3 * (size_t)ctx->max_small_v * ctx->szT);
-    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 3>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    // allow the on-chip solvers the LDS their largest cells need (p, r, b as [V][4])
+    if (ctx->bicg_n[0] > 0) {
+        const int lds = (int)(3 * 4 * (size_t)ctx->bicg_maxv[0] * ctx->szT);
+        if (ctx->prec == LBMIBM_F64)
+            LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<double, BICG0_K, BICG0_NT>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        else
+            LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, BICG0_K, BICG0_NT>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    }
+    if (ctx->bicg_n[1] > 0) {
+        const int lds = (int)(3 * 4 * (size_t)ctx->bicg_maxv[1] * sizeof(float));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, BICG1_K, BICG1_NT>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    }
     return LBMIBM_OK;
 }
 

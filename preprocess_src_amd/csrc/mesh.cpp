@@ -67,6 +67,43 @@ void subdivide(std::vector<std::array<double, 3>> &v, std::vector<std::array<int
     f.swap(nf);
 }
 
+// Morton code of a point of [-1,1]^3 quantised to 10 bits per axis.
+uint32_t morton3(const std::array<double, 3> &p) {
+    uint32_t key = 0;
+    uint32_t q[3];
+    for (int d = 0; d < 3; d++) {
+        const double u = std::min(std::max(0.5 * (p[d] + 1.0), 0.0), 1.0);
+        q[d] = std::min(1023u, (uint32_t)(u * 1024.0));
+    }
+    for (int b = 9; b >= 0; b--)
+        for (int d = 0; d < 3; d++) key = (key << 1) | ((q[d] >> b) & 1u);
+    return key;
+}
+
+// Renumber vertices along a Morton curve of their positions and sort the
+// triangles by their lowest vertex id, keeping each triangle's cyclic order:
+// vertices and elements that are close on the surface get close ids, so the
+// per-vertex and per-element kernels read neighbouring data from the same
+// cache lines.
+void locality_order(std::vector<std::array<double, 3>> &v, std::vector<std::array<int, 3>> &f) {
+    const int nv = (int)v.size();
+    std::vector<int> order(nv), rank(nv);
+    std::vector<uint32_t> key(nv);
+    for (int i = 0; i < nv; i++) { order[i] = i; key[i] = morton3(v[i]); }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    std::vector<std::array<double, 3>> nvx(nv);
+    for (int i = 0; i < nv; i++) { rank[order[i]] = i; nvx[i] = v[order[i]]; }
+    v.swap(nvx);
+    for (auto &t : f) {
+        for (auto &c : t) c = rank[c];
+        // rotate so that the lowest id comes first (same orientation)
+        while (t[0] != std::min(t[0], std::min(t[1], t[2]))) t = {t[1], t[2], t[0]};
+    }
+    std::stable_sort(f.begin(), f.end(), [](const std::array<int, 3> &a, const std::array<int, 3> &b) {
+        return a[0] != b[0] ? a[0] < b[0] : (a[1] != b[1] ? a[1] < b[1] : a[2] < b[2]);
+    });
+}
+
 }  // namespace
 
 extern "C" {
@@ -82,6 +119,7 @@ lbmibm_status lbmibm_mesh_icosphere(int level, int *nv, int *nt, double *xyz, in
     icosahedron(v, f);
     for (int l = 0; l < level; l++) subdivide(v, f);
     if ((int)v.size() != nvv || (int)f.size() != ntt) return LBMIBM_ERR_TOPOLOGY;
+    locality_order(v, f);
     if (xyz)
         for (int i = 0; i < nvv; i++)
             for (int d = 0; d < 3; d++) xyz[3 * i + d] = v[i][d];

@@ -21,7 +21,7 @@ struct VolArgs {
 };
 
 __device__ void cell_sums(const double *__restrict__ xs, int nv, const int *__restrict__ ele, int e0, int e1,
-                          double *__restrict__ normal, double *sh, double &V, double &A) {
+                          double *__restrict__ normal, WgReduce &red, double &V, double &A) {
     // CalcEleNormal_GPU (volume_correction.cu:138-200) + CalcSum_Thrust
     double pv = 0.0, pa = 0.0;
     for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
@@ -45,36 +45,42 @@ __device__ void cell_sums(const double *__restrict__ xs, int nv, const int *__re
         pv += 1.0 / 3.0 * (r0 * n0 + r1 * n1 + r2 * n2) * ds;
         pa += ds;
     }
-    V = block_sum(pv, sh);
-    A = block_sum(pa, sh);
+    double va[2] = {pv, pa};
+    red.sum<2>(va);
+    V = va[0];
+    A = va[1];
 }
 
-__device__ void cell_center(const double *__restrict__ xs, int nv, int v0, int v1, double *sh, const VolArgs &a,
+__device__ void cell_center(const double *__restrict__ xs, int nv, int v0, int v1, WgReduce &red, const VolArgs &a,
                             double g[3]) {
     // CalcCenter (volume_correction.cu:310-352)
     double s[3] = {0.0, 0.0, 0.0};
     for (int j = v0 + threadIdx.x; j < v1; j += blockDim.x)
         for (int d = 0; d < 3; d++) s[d] += (double)xs[(size_t)d * nv + j];
-    for (int d = 0; d < 3; d++) g[d] = block_sum(s[d], sh) / (double)(v1 - v0);
+    red.sum<3>(s);
+    for (int d = 0; d < 3; d++) g[d] = s[d] / (double)(v1 - v0);
     if (g[0] < -a.lx / 2.0) g[0] += a.lx; else if (g[0] > a.lx / 2.0) g[0] -= a.lx;
     if (g[2] < -a.lz / 2.0) g[2] += a.lz; else if (g[2] > a.lz / 2.0) g[2] -= a.lz;
     if (a.py) { if (g[1] < -a.ly / 2.0) g[1] += a.ly; else if (g[1] > a.ly / 2.0) g[1] -= a.ly; }
 }
 
-__global__ __launch_bounds__(256) void k_volume(int mode, double *__restrict__ xs, int nv, const int *__restrict__ voff,
+template <int NT>
+__global__ __launch_bounds__(NT) void k_volume(const int *__restrict__ cells, int mode, double *__restrict__ xs, int nv,
+                                               const int *__restrict__ voff,
                                                 const int *__restrict__ eoff, const int *__restrict__ ele,
                                                 const int *__restrict__ nle, const int *__restrict__ le,
                                                 double *__restrict__ normal, double *__restrict__ vol0,
                                                 double *__restrict__ vol, double *__restrict__ area,
                                                 double *__restrict__ gx, int *__restrict__ passes,
                                                 DevStatus *__restrict__ status, VolArgs a) {
-    __shared__ double sh[4];
-    const int c = blockIdx.x;
+    __shared__ double sh[128];
+    WgReduce red{sh, 0};
+    const int c = cells[blockIdx.x];
     const int v0 = voff[c], v1 = voff[c + 1], e0 = eoff[c], e1 = eoff[c + 1];
     double *X = xs, *Y = xs + nv, *Z = xs + 2 * (size_t)nv;
     double V = 0.0, A = 0.0;
     if (mode == 0) {  // volume_init (:427-450)
-        cell_sums(xs, nv, ele, e0, e1, nullptr, sh, V, A);
+        cell_sums(xs, nv, ele, e0, e1, nullptr, red, V, A);
         if (threadIdx.x == 0) { vol0[c] = V; vol[c] = V; area[c] = A; }
     } else {
         // CapsulePeriodicBefore around the previous centroid (:28-67)
@@ -93,7 +99,7 @@ __global__ __launch_bounds__(256) void k_volume(int mode, double *__restrict__ x
         const double V0 = vol0[c];
         int pass = 1;
         for (; pass <= 1000; pass++) {
-            cell_sums(xs, nv, ele, e0, e1, normal, sh, V, A);  // barriers order normals and positions
+            cell_sums(xs, nv, ele, e0, e1, normal, red, V, A);  // its barrier orders normals and positions
             const double dV = fabs(V - V0) / V0 * 100.0;
             if (dV < 1.0e-3) break;  // EPS (:15, :388-389)
             const double lambda = 18.0 * (V - V0) / A;
@@ -121,7 +127,7 @@ __global__ __launch_bounds__(256) void k_volume(int mode, double *__restrict__ x
         }
     }
     double g[3];
-    cell_center(xs, nv, v0, v1, sh, a, g);
+    cell_center(xs, nv, v0, v1, red, a, g);
     if (threadIdx.x == 0) { gx[3 * c] = g[0]; gx[3 * c + 1] = g[1]; gx[3 * c + 2] = g[2]; }
     if (mode == 1) {  // CapsulePeriodicAfter (:70-94)
         for (int j = v0 + threadIdx.x; j < v1; j += blockDim.x) {
@@ -134,10 +140,19 @@ __global__ __launch_bounds__(256) void k_volume(int mode, double *__restrict__ x
 
 lbmibm_status run_impl(lbmibm_ctx *ctx, int mode) {
     VolArgs a{ctx->nx * ctx->p.dx, ctx->ny * ctx->p.dx, ctx->nz * ctx->p.dx, ctx->p.periodic_y};
-    hipLaunchKernelGGL(k_volume, dim3(ctx->nc), dim3(256), 0, ctx->stream, mode, (double *)ctx->xs, ctx->nv,
-                       ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le, ctx->normal, ctx->vol0, ctx->vol,
-                       ctx->area, ctx->gx, ctx->vol_passes, ctx->status, a);
-    LBMIBM_LAUNCHED(ctx, "k_volume");
+    // cells of up to VOL_SMALL vertices with 256 threads, larger ones with 1024
+    if (ctx->vol_n[0] > 0) {
+        hipLaunchKernelGGL(k_volume<256>, dim3(ctx->vol_n[0]), dim3(256), 0, ctx->stream, ctx->vol_cells[0], mode,
+                           (double *)ctx->xs, ctx->nv, ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le,
+                           ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->vol_passes, ctx->status, a);
+        LBMIBM_LAUNCHED(ctx, "k_volume<256>");
+    }
+    if (ctx->vol_n[1] > 0) {
+        hipLaunchKernelGGL(k_volume<1024>, dim3(ctx->vol_n[1]), dim3(1024), 0, ctx->stream, ctx->vol_cells[1], mode,
+                           (double *)ctx->xs, ctx->nv, ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le,
+                           ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->vol_passes, ctx->status, a);
+        LBMIBM_LAUNCHED(ctx, "k_volume<1024>");
+    }
     return LBMIBM_OK;
 }
 

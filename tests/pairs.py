@@ -25,7 +25,8 @@ def oracle_of(lx, ly, lz, cells=None, **kw):
                   b=p["b"], m=p["m"], c=p["c"], br=p["br"], itrmax=p["itrmax"], residual=p["residual"],
                   runge_kutta=p["runge_kutta"], volume_correction=p["volume_correction"],
                   periodic_y=p["periodic_y"], force_modulated=p.get("force_modulated", False),
-                  flowstart=p.get("flowstart", 0), freq=p.get("freq", 1.0), cells=cells)
+                  flowstart=p.get("flowstart", 0), freq=p.get("freq", 1.0),
+                  fileoutput=p.get("fileoutput", 0) or 1000, cells=cells)
     o.init_fluid()
     return o
 

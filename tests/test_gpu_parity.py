@@ -152,28 +152,35 @@ def test_ibm_interp_parity(prec, rk, m, dx, shape):
     assert np.max(np.abs(s["xd"] - o.cells["xd"])) < TOL[prec] * max(lx, ly, lz) * dx
 
 
+# mesh levels 2-5 (162 / 642 / 2562 / 10242 vertices) cover the three solver
+# classes: on chip 4 x 256, on chip 6 x 512 (FP32), vectors in global memory
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("levels", [[2, 2], [2, 3]])
+@pytest.mark.parametrize("levels", [[2, 2], [2, 3], [4], [5]])
 def test_stress_parity(prec, levels):
     lx, ly, lz = 24, 20, 24
+    print(f"[stress parity] prec={prec} levels={levels}", flush=True)
     cells = small_cells(lx, ly, lz, nc=len(levels), radius=4.0, levels=levels, seed=21)
-    kw = dict(c=100.0, br=0.0)
+    kw = dict(c=100.0, br=0.0, fileoutput=2)
     o = oracle_of(lx, ly, lz, cells=cells, **kw)
     g = gpu_of(lx, ly, lz, prec, cells=cells, **kw)
-    d = perturb_cells(cells, seed=22, amp=0.05)
-    xd = np.stack([d["x"], d["y"], d["z"]], 1).reshape(-1).copy()
-    o.cells["xd"][:] = xd
-    g.set_cells_state(xd=xd)
-    assert o.stress(1) == 0
-    g.stress(1)
-    mb = g.membrane()
-    assert rel(mb["rg"], o.cells["rg"]) < TOL[prec]
-    assert rel(mb["t"], o.cells["t"]) < TOL[prec] * 10
-    assert rel(mb["q"], o.cells["q"]) < TOL[prec] * 10
-    s = g.cells_state()
-    for a, b in (("fx", "cfx"), ("fy", "cfy"), ("fz", "cfz")):
-        assert rel(s[a], o.cells[b]) < TOL[prec] * 10, a
-    assert np.all(mb["bicg_iters"] == o.cells["bicg_iters"]) or prec != "f64"
+    for it in (1, 2, 3):
+        d = perturb_cells(cells, seed=22 + it, amp=0.05)
+        xd = np.stack([d["x"], d["y"], d["z"]], 1).reshape(-1).copy()
+        o.cells["xd"][:] = xd
+        g.set_cells_state(xd=xd)
+        assert o.stress(it) == 0
+        g.stress(it)
+        mb = g.membrane()
+        assert rel(mb["rg"], o.cells["rg"]) < TOL[prec]
+        assert rel(mb["t"], o.cells["t"]) < TOL[prec] * 10
+        assert rel(mb["q"], o.cells["q"]) < TOL[prec] * 10
+        # FILEOUTPUT accumulators (stress.cu:101-118, 299-300), reset at (iter-1) % 2 == 0
+        assert rel(mb["qi"], o.cells["qi"]) < TOL[prec] * 10
+        assert rel(mb["tp"], o.cells["tp"]) < TOL[prec] * 10
+        s = g.cells_state()
+        for a, b in (("fx", "cfx"), ("fy", "cfy"), ("fz", "cfz")):
+            assert rel(s[a], o.cells[b]) < TOL[prec] * 10, a
+        assert np.all(mb["bicg_iters"] == o.cells["bicg_iters"]) or prec != "f64"
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
