@@ -171,7 +171,8 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->bicg_iters, ctx->vol_passes,
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
                     ctx->tile_off, ctx->tile_cur, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
-                    ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv};
+                    ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv,
+                    ctx->tile_cand, ctx->order_spread, ctx->order_interp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -376,6 +377,9 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         CK(dev_alloc(ctx, (void **)&ctx->tile_count, (size_t)ctx->ntiles * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->tile_off, (size_t)ctx->ntiles * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->tile_cur, (size_t)ctx->ntiles * sizeof(int)));
+        CK(dev_alloc(ctx, (void **)&ctx->tile_cand, (size_t)ctx->ntiles * sizeof(int)));
+        CK(dev_alloc(ctx, (void **)&ctx->order_spread, (size_t)ctx->ntiles * sizeof(int)));
+        CK(dev_alloc(ctx, (void **)&ctx->order_interp, (size_t)ctx->ntiles * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->blk_fmax, (size_t)ctx->nblk_bin * sizeof(float)));
         CK(dev_alloc(ctx, (void **)&ctx->absmax, sizeof(unsigned)));
         CK(ibm_setup(ctx));

@@ -93,6 +93,8 @@ struct lbmibm_ctx {
     short4 *node0 = nullptr;                 // [Nv] first stencil node (x,z wrapped, y+3)
     int *home = nullptr, *home_list = nullptr;  // [Nv]
     int *tile_count = nullptr, *tile_off = nullptr, *tile_cur = nullptr;  // [ntiles]
+    int *tile_cand = nullptr;                          // [ntiles] spreading candidates
+    int *order_spread = nullptr, *order_interp = nullptr;  // [ntiles] heavy-first tile orders
     float *blk_fmax = nullptr;               // [nblk_bin] max |f| per binning block
     unsigned *absmax = nullptr;
     bool bins_valid = false;                 // bins match the current positions

@@ -116,6 +116,42 @@ __device__ __forceinline__ void stencil3(const IbmGeom &g, double x, double y, d
     z1 -= 2;
 }
 
+// Mask of the 4 stencil nodes first..first+3 (periodic when wrap) that fall
+// in [o, o+len) of an axis of n nodes; bit k for node first+k.
+__device__ __forceinline__ unsigned axis_mask(int first, int o, int len, int n, bool wrap) {
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int a = first + k;
+        if (wrap) {
+            if (a < 0) a += n;
+            else if (a >= n) a -= n;
+        }
+        if (a >= o && a < o + len && a >= 0 && a < n) m |= 1u << k;
+    }
+    return m;
+}
+
+// Tiles along one axis whose nodes can hold the first stencil node of a
+// vertex reaching [o, o+len): the tile itself and the tiles of nodes o-3..o-1
+// (periodic when wrap).  Returns the number of distinct tiles (1..4).
+__device__ __forceinline__ int source_tiles(int t, int o, int n, int tile, bool wrap, int out[4]) {
+    int cnt = 1;
+    out[0] = t;
+    for (int d = 1; d <= 3; d++) {
+        int a = o - d;
+        if (a < 0) {
+            if (!wrap) break;
+            a = ((a % n) + n) % n;
+        }
+        const int s = a / tile;
+        bool seen = false;
+        for (int i = 0; i < cnt; i++) seen |= out[i] == s;
+        if (!seen) out[cnt++] = s;
+    }
+    return cnt;
+}
+
 // ---- binning ---------------------------------------------------------------
 
 // Slot for a lane in counters[key], lanes with equal keys aggregated into one
@@ -175,12 +211,46 @@ __global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs
     if (threadIdx.x == 0) blk_fmax[blockIdx.x] = fmaxf(fmaxf(shm[0], shm[1]), fmaxf(shm[2], shm[3]));
 }
 
+// Heavy-first order of the tiles: a counting sort on a 256-level key of the
+// per-tile load, largest first.  The tile kernels take their tiles in this
+// order, so the dispatcher starts the longest work groups first and fills the
+// tail with light ones (longest-processing-time-first scheduling); the order
+// within a key level is arbitrary and does not affect any result.
+__device__ void heavy_first_order(const int *load, int n, int *hist, int *order) {
+    __shared__ int maxload;
+    if (threadIdx.x == 0) maxload = 0;
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    int m = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = max(m, load[i]);
+    atomicMax(&maxload, m);
+    __syncthreads();
+    int shift = 0;
+    while ((maxload >> shift) > 255) shift++;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&hist[255 - (load[i] >> shift)], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive scan of 256 bins
+        int acc = 0;
+        for (int k = 0; k < 256; k++) {
+            const int c = hist[k];
+            hist[k] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) order[atomicAdd(&hist[255 - (load[i] >> shift)], 1)] = i;
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile_count, int *__restrict__ tile_off,
                                                     int *__restrict__ tile_cur, int ntiles,
                                                     const float *__restrict__ blk_fmax, int nblk,
-                                                    unsigned *__restrict__ absmax) {
+                                                    unsigned *__restrict__ absmax, IbmGeom g, TileGeom tg,
+                                                    int *__restrict__ tile_cand, int *__restrict__ order_spread,
+                                                    int *__restrict__ order_interp) {
     // exclusive scan over tiles in one work group, 1024 entries per round,
-    // and the maximum of the per-block force maxima
+    // the maximum of the per-block force maxima, the spreading candidates of
+    // every tile and the two heavy-first tile orders
     __shared__ int sh[1024];
     __shared__ float shm[1024];
     __shared__ int carry;
@@ -213,6 +283,22 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile
         if (threadIdx.x == 1023) carry += sh[1023];
         __syncthreads();
     }
+    if (!order_spread) return;
+    for (int t = threadIdx.x; t < ntiles; t += 1024) {
+        const int tx = t % tg.ntx, ty = (t / tg.ntx) % tg.nty, tz = t / (tg.ntx * tg.nty);
+        int sxs[4], sys[4], szs[4];
+        const int nsx = source_tiles(tx, tx * TILE_X, g.nx, TILE_X, true, sxs);
+        const int nsy = source_tiles(ty, ty * TILE_Y, g.ny, TILE_Y, !g.walls, sys);
+        const int nsz = source_tiles(tz, tz * TILE_Z, g.nz, TILE_Z, true, szs);
+        int c = 0;
+        for (int iz = 0; iz < nsz; iz++)
+            for (int iy = 0; iy < nsy; iy++)
+                for (int ix = 0; ix < nsx; ix++) c += tile_count[sxs[ix] + tg.ntx * (sys[iy] + tg.nty * szs[iz])];
+        tile_cand[t] = c;
+    }
+    __syncthreads();
+    heavy_first_order(tile_cand, ntiles, sh, order_spread);
+    heavy_first_order(tile_count, ntiles, sh, order_interp);
 }
 
 __global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ home, int nv,
@@ -236,51 +322,41 @@ __global__ void k_zero_int(int *__restrict__ p, int n) {
 // 64 (kx,ky,kz) offsets of one stencil fall on 32 distinct 8-byte bank slots.
 constexpr int ACC_SX = 20, ACC_SXY = 176, ACC_N = ACC_SXY * TILE_Z;
 
-// Mask of the 4 stencil nodes first..first+3 (periodic when wrap) that fall
-// in [o, o+len) of an axis of n nodes; bit k for node first+k.
-__device__ __forceinline__ unsigned axis_mask(int first, int o, int len, int n, bool wrap) {
-    unsigned m = 0;
+// Accumulation into the tile in 64-bit fixed point with a power-of-two scale,
+// so the integer sums do not depend on the order of the additions.
+//   FP32: the scale keeps |sum| below 2^50; the product w*(f*scale) is formed
+//   by one FP64 FMA with 1.5*2^52 added, which leaves round(w*f*scale) in the
+//   low mantissa bits: subtracting the constant's bit pattern gives the
+//   two's-complement integer (valid for |value| < 2^51).
+//   FP64: the scale keeps |sum| below 2^62 and llrint converts.
+template <typename T> struct FixedPoint;
+template <> struct FixedPoint<float> {
+    static constexpr int kBits = 50;
+    double f[3];
+    __device__ __forceinline__ FixedPoint(float f0, float f1, float f2, double scale) {
+        f[0] = (double)f0 * scale; f[1] = (double)f1 * scale; f[2] = (double)f2 * scale;
+    }
+    __device__ __forceinline__ void add(unsigned long long *acc, int li, float w) const {
+        const double wd = (double)w;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        int a = first + k;
-        if (wrap) {
-            if (a < 0) a += n;
-            else if (a >= n) a -= n;
+        for (int d = 0; d < 3; d++) {
+            const double t = fma(wd, f[d], 6755399441055744.0);  // + 1.5 * 2^52
+            atomicAdd(&acc[d * ACC_N + li],
+                      (unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull);
         }
-        if (a >= o && a < o + len && a >= 0 && a < n) m |= 1u << k;
     }
-    return m;
-}
-
-// Tiles along one axis whose nodes can hold the first stencil node of a
-// vertex reaching [o, o+len): the tile itself and the tiles of nodes o-3..o-1
-// (periodic when wrap).  Returns the number of distinct tiles (1..4).
-__device__ __forceinline__ int source_tiles(int t, int o, int n, int tile, bool wrap, int out[4]) {
-    int cnt = 1;
-    out[0] = t;
-    for (int d = 1; d <= 3; d++) {
-        int a = o - d;
-        if (a < 0) {
-            if (!wrap) break;
-            a = ((a % n) + n) % n;
-        }
-        const int s = a / tile;
-        bool seen = false;
-        for (int i = 0; i < cnt; i++) seen |= out[i] == s;
-        if (!seen) out[cnt++] = s;
+};
+template <> struct FixedPoint<double> {
+    static constexpr int kBits = 62;
+    double f[3], scale;
+    __device__ __forceinline__ FixedPoint(double f0, double f1, double f2, double s) : scale(s) {
+        f[0] = f0; f[1] = f1; f[2] = f2;
     }
-    return cnt;
-}
-
-// Accumulation into the tile in 64-bit fixed point: the power-of-two scale is
-// applied in FP64 (exact), so no FP32 product can overflow, and the integer
-// sums do not depend on the order of the additions.
-__device__ __forceinline__ void acc_add(unsigned long long *a, float v, double scale) {
-    atomicAdd(a, (unsigned long long)llrint((double)v * scale));
-}
-__device__ __forceinline__ void acc_add(unsigned long long *a, double v, double scale) {
-    atomicAdd(a, (unsigned long long)llrint(v * scale));
-}
+    __device__ __forceinline__ void add(unsigned long long *acc, int li, double w) const {
+#pragma unroll
+        for (int d = 0; d < 3; d++) atomicAdd(&acc[d * ACC_N + li], (unsigned long long)llrint(w * f[d] * scale));
+    }
+};
 
 template <typename T>
 __device__ __forceinline__ void spread_vertex(const double *__restrict__ xs, const T *__restrict__ cf, int nv, int v,
@@ -293,25 +369,31 @@ __device__ __forceinline__ void spread_vertex(const double *__restrict__ xs, con
     const unsigned my = axis_mask(y1, oy, ly, g.ny, !g.walls);
     const unsigned mz = axis_mask(z1, oz, lz, g.nz, true);
     if (!mx || !my || !mz) return;
-    const T f0 = cf[v], f1 = cf[nv + v], f2 = cf[2 * (size_t)nv + v];
+    const FixedPoint<T> fp(cf[v], cf[nv + v], cf[2 * (size_t)nv + v], scale);
+    // tile-relative offsets of the stencil nodes on each axis
+    int ax[4], by[4], cz[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int a = x1 + k, b = y1 + k, c = z1 + k;
+        wrap_index(a, g.nx);
+        wrap_index(c, g.nz);
+        if (!g.walls) wrap_index(b, g.ny);
+        ax[k] = a - ox;
+        by[k] = ACC_SX * (b - oy);
+        cz[k] = ACC_SXY * (c - oz);
+    }
+#pragma unroll
     for (int kz = 0; kz < 4; kz++) {
         if (!((mz >> kz) & 1u)) continue;
-        int c = z1 + kz;
-        wrap_index(c, g.nz);
+#pragma unroll
         for (int ky = 0; ky < 4; ky++) {
             if (!((my >> ky) & 1u)) continue;
-            int b = y1 + ky;
-            if (!g.walls) wrap_index(b, g.ny);
             const T wyz = wy[ky] * wz[kz] * w64;
+            const int row = by[ky] + cz[kz];
+#pragma unroll
             for (int kx = 0; kx < 4; kx++) {
                 if (!((mx >> kx) & 1u)) continue;
-                int a = x1 + kx;
-                wrap_index(a, g.nx);
-                const int li = (a - ox) + ACC_SX * (b - oy) + ACC_SXY * (c - oz);
-                const T w = wx[kx] * wyz;
-                acc_add(&acc[li], w * f0, scale);
-                acc_add(&acc[ACC_N + li], w * f1, scale);
-                acc_add(&acc[2 * ACC_N + li], w * f2, scale);
+                fp.add(acc, row + ax[kx], wx[kx] * wyz);
             }
         }
     }
@@ -323,11 +405,11 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
                                                       const int *__restrict__ tile_count,
                                                       const int *__restrict__ tile_off,
                                                       const int *__restrict__ home_list, T *__restrict__ F, int nv,
-                                                      IbmGeom g, TileGeom tg, const unsigned *__restrict__ absmax) {
+                                                      IbmGeom g, TileGeom tg, const unsigned *__restrict__ absmax,
+                                                      const int *__restrict__ order) {
     __shared__ unsigned long long acc[3 * ACC_N];
     __shared__ int queue[4][128];
-    const int tile = xcd_block(tg.ntiles);
-    if (tile >= tg.ntiles) return;
+    const int tile = order[blockIdx.x];
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
     const int ox = tx * TILE_X, oy = ty * TILE_Y, oz = tz * TILE_Z;
     const int lx = min(TILE_X, g.nx - ox), ly = min(TILE_Y, g.ny - oy), lz = min(TILE_Z, g.nz - oz);
@@ -339,7 +421,7 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
     const int nsz = source_tiles(tz, oz, g.nz, TILE_Z, true, szs);
     // fixed-point scale: at most `total` vertices reach one node, each with a
     // weight at most 1/(8 dx^3), so |sum| <= total |f|max / (8 dx^3), which
-    // the power of two keeps below 2^62 / scale
+    // the power of two keeps below 2^kBits / scale
     long long total = 0;
     for (int iz = 0; iz < nsz; iz++)
         for (int iy = 0; iy < nsy; iy++)
@@ -348,7 +430,7 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
     double scale = 1.0;
     if (fmax > 0.0 && total > 0) {
         const double bound = (double)total * fmax / (8.0 * g.dx * g.dx * g.dx);
-        scale = ldexp(1.0, 61 - ilogb(bound));
+        scale = ldexp(1.0, FixedPoint<T>::kBits - 1 - ilogb(bound));
     }
     const T w64 = (T)(1.0 / (64.0 * g.dx * g.dx * g.dx));
     __syncthreads();
@@ -359,8 +441,13 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
     for (int s = 0; s < nsx * nsy * nsz; s++) {
         const int bin = sxs[s % nsx] + tg.ntx * (sys[(s / nsx) % nsy] + tg.nty * szs[s / (nsx * nsy)]);
         const int cnt = tile_count[bin], off = tile_off[bin];
-        for (int base = wave * 64; base < cnt; base += 256) {
-            const int i = base + lane;
+        // the bin read as 64 columns of R rows, lane l taking column l: the
+        // lanes of a wave get vertices far apart in the bin, whose stencils
+        // rarely share LDS addresses (64-bit LDS atomics to one address
+        // serialise, and neighbouring vertices in a bin are close in space)
+        const int R = (cnt + 63) >> 6;
+        for (int r = wave; r < R; r += 4) {
+            const int i = lane * R + r;
             int v = -1;
             bool touch = false;
             if (i < cnt) {
@@ -411,28 +498,30 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
 // one-node move of the RK stages).
 constexpr int RX = TILE_X + 7, RY = TILE_Y + 7, RZ = TILE_Z + 7, RN = RX * RY * RZ;
 
+// Out-of-line and not unrolled: it is the rare path, and inlining it into
+// every RK stage would multiply the kernel's code size.
 template <typename T>
-__device__ __forceinline__ void interp_global(const T *__restrict__ U, size_t N, const IbmGeom &g, double x,
-                                              double y, double z, bool unit, T &ur, T &vr, T &wr) {
+__device__ __noinline__ void interp_global(const T *__restrict__ U, size_t N, const IbmGeom &g, double x,
+                                           double y, double z, bool unit, T &ur, T &vr, T &wr) {
     int x1, y1, z1;
     T wx[4], wy[4], wz[4];
     stencil3<T>(g, x, y, z, unit, x1, y1, z1, wx, wy, wz);
     ur = vr = wr = T(0);
-#pragma unroll
+#pragma unroll 1
     for (int kz = 0; kz < 4; kz++) {
         int iz = z1 + kz;
         if (!wrap_index(iz, g.nz)) continue;
-#pragma unroll
+#pragma unroll 1
         for (int ky = 0; ky < 4; ky++) {
             int iy = y1 + ky;
             if (g.walls) {
                 if (iy < 0 || iy >= g.ny) continue;
             } else if (!wrap_index(iy, g.ny)) continue;
-#pragma unroll
+#pragma unroll 1
             for (int kx = 0; kx < 4; kx++) {
                 int ix = x1 + kx;
                 if (!wrap_index(ix, g.nx)) continue;
-                const T dlt = wx[kx] * wy[ky] * wz[kz] / T(64);
+                const T dlt = wx[kx] * wy[ky] * wz[kz] * T(1.0 / 64.0);
                 const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
                 ur += dlt * U[n];
                 vr += dlt * U[N + n];
@@ -461,13 +550,14 @@ __device__ __forceinline__ void interp_tile(const T *__restrict__ U, size_t N, c
     ur = vr = wr = T(0);
 #pragma unroll
     for (int kz = 0; kz < 4; kz++) {
+        const T wz64 = wz[kz] * T(1.0 / 64.0);
 #pragma unroll
         for (int ky = 0; ky < 4; ky++) {
-            const T wyz = wy[ky] * wz[kz];
+            const T wyz = wy[ky] * wz64;
             const int row = rx + RX * ((ry + ky) + RY * (rz + kz));
 #pragma unroll
             for (int kx = 0; kx < 4; kx++) {
-                const T dlt = wx[kx] * wyz / T(64);
+                const T dlt = wx[kx] * wyz;
                 ur += dlt * Us[row + kx];
                 vr += dlt * Us[RN + row + kx];
                 wr += dlt * Us[2 * RN + row + kx];
@@ -489,11 +579,10 @@ __global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, d
                                                       const int *__restrict__ tile_count,
                                                       const int *__restrict__ tile_off,
                                                       const int *__restrict__ home_list, int nv, IbmGeom g,
-                                                      TileGeom tg, AdvectArgs a) {
+                                                      TileGeom tg, AdvectArgs a, const int *__restrict__ order) {
     extern __shared__ double dyn[];
     T *Us = (T *)dyn;  // [3][RZ][RY][RX]
-    const int tile = xcd_block(tg.ntiles);
-    if (tile >= tg.ntiles) return;
+    const int tile = order[blockIdx.x];
     const int cnt = tile_count[tile], off = tile_off[tile];
     if (cnt == 0) return;  // whole work group leaves together
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
@@ -579,7 +668,8 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
                        (const T *)ctx->cf, nv, g, tg, ctx->node0, ctx->home, ctx->tile_count, ctx->blk_fmax);
     LBMIBM_LAUNCHED(ctx, "k_bin_count");
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->tile_count, ctx->tile_off,
-                       ctx->tile_cur, ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax);
+                       ctx->tile_cur, ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax, g, tg, ctx->tile_cand,
+                       ctx->order_spread, ctx->order_interp);
     LBMIBM_LAUNCHED(ctx, "k_tile_scan");
     hipLaunchKernelGGL(k_bin_scatter, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, ctx->home, nv, ctx->tile_off,
                        ctx->tile_cur, ctx->home_list);
@@ -594,9 +684,9 @@ template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
     lbmibm_status s = bin_impl<T>(ctx);
     if (s != LBMIBM_OK) return s;
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
-    hipLaunchKernelGGL(k_spread_tiles<T>, dim3(xcd_grid(ctx->ntiles)), dim3(256), 0, ctx->stream,
-                       (const double *)ctx->xs, (const T *)ctx->cf, ctx->node0, ctx->tile_count, ctx->tile_off,
-                       ctx->home_list, (T *)ctx->F, ctx->nv, geom_of(ctx), tg, ctx->absmax);
+    hipLaunchKernelGGL(k_spread_tiles<T>, dim3(ctx->ntiles), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
+                       (const T *)ctx->cf, ctx->node0, ctx->tile_count, ctx->tile_off, ctx->home_list, (T *)ctx->F,
+                       ctx->nv, geom_of(ctx), tg, ctx->absmax, ctx->order_spread);
     LBMIBM_LAUNCHED(ctx, "k_spread_tiles");
     return LBMIBM_OK;
 }
@@ -614,9 +704,9 @@ template <typename T> lbmibm_status interp_impl(lbmibm_ctx *ctx) {
     a.periodic_y = ctx->p.periodic_y;
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const size_t lds = 3 * (size_t)RN * sizeof(T);
-    hipLaunchKernelGGL(k_interp_tiles<T>, dim3(xcd_grid(ctx->ntiles)), dim3(256), lds, ctx->stream,
-                       (const T *)ctx->U, (double *)ctx->xs, (T *)ctx->uvw, (double *)ctx->xd, ctx->tile_count,
-                       ctx->tile_off, ctx->home_list, ctx->nv, geom_of(ctx), tg, a);
+    hipLaunchKernelGGL(k_interp_tiles<T>, dim3(ctx->ntiles), dim3(256), lds, ctx->stream, (const T *)ctx->U,
+                       (double *)ctx->xs, (T *)ctx->uvw, (double *)ctx->xd, ctx->tile_count, ctx->tile_off,
+                       ctx->home_list, ctx->nv, geom_of(ctx), tg, a, ctx->order_interp);
     LBMIBM_LAUNCHED(ctx, "k_interp_tiles");
     ctx->bins_valid = false;  // the positions moved
     return LBMIBM_OK;
