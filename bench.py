@@ -143,6 +143,11 @@ def main():
 
     # per-stage HIP-event timing on the library stream, right after the timed region
     stages = sim.time_stages(a.warmup + a.steps, a.stage_steps)
+    diag = {}
+    if cells is not None:
+        its = sim.membrane()["bicg_iters"]
+        nvc = np.diff(np.asarray(cells["voff"]))
+        diag["bicg_iters_mean_by_vertices"] = {int(k): round(float(its[nvc == k].mean()), 2) for k in np.unique(nvc)}
     t_lbm = stages["lbm_d3q19"][0] * 1e-3
     coupled = cells is not None
     bpn = lattice_bytes_per_node(a.precision, coupled)
@@ -173,6 +178,7 @@ def main():
                      "bytes_per_node": bpn, "avg_ms": round(stages["lbm_d3q19"][0], 5)},
         "step_roofline": {"algorithmic_bytes": step_bytes, "frac": round(step_bytes / (ms_per_step * 1e-3) / HBM_PEAK, 4)},
         "stages_ms": {k: round(v[0], 5) for k, v in stages.items()},
+        "diagnostics": diag,
     }
     if not a.no_cpu:
         print("[bench] cpu baseline (FP64 oracle)", file=sys.stderr, flush=True)

@@ -176,6 +176,9 @@ void free_all(lbmibm_ctx *ctx) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
@@ -254,6 +257,9 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         if (s_ != LBMIBM_OK) return fail(s_);    \
     } while (0)
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+    if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+    if (hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
 
     const size_t szS = ctx->szS, szT = ctx->szT;
     CK(dev_alloc(ctx, &ctx->f[0], (size_t)Q * N * szS));

@@ -46,6 +46,10 @@ struct lbmibm_ctx {
     int prec;                  // lbmibm_precision
     size_t szS, szT;           // storage / arithmetic type sizes
     hipStream_t stream = nullptr;
+    // second stream (default priority) for the large-cell solves, forked from
+    // and joined back into `stream` with events; captured graphs keep the fork
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
 
     // lattice (device)

@@ -454,6 +454,7 @@ __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ ce
         if (rr == 0.0) { flag = 1; break; }
         if (it > 0) {
             const double beta = (rr / rr_old) * (alfa / omeg);
+#pragma unroll 4
             for (int vv = v0 + tid; vv < v1; vv += nt)
                 for (int d = 0; d < 3; d++) {
                     const size_t k = 3 * (size_t)vv + d;
@@ -478,6 +479,7 @@ __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ ce
         }
         red.sum<1>(*(double(*)[1])acc);
         alfa = rr / acc[0];
+#pragma unroll 4
         for (int vv = v0 + tid; vv < v1; vv += nt)
             for (int d = 0; d < 3; d++) {
                 const size_t k = 3 * (size_t)vv + d;
@@ -503,6 +505,7 @@ __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ ce
         red.sum<2>(acc);
         omeg = acc[0] / acc[1];
         acc[0] = 0.0; acc[1] = 0.0;
+#pragma unroll 4
         for (int vv = v0 + tid; vv < v1; vv += nt)
             for (int d = 0; d < 3; d++) {
                 const size_t k = 3 * (size_t)vv + d;
@@ -639,6 +642,18 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
     out.nv = nv;
     out.fo = p.fileoutput > 0 ? p.fileoutput : 0;
     out.wq = p.fileoutput > 0 ? (double)p.m / (double)p.fileoutput : 0.0;
+    if (ctx->bicg_n[2] > 0) {
+        // the large cells (one work group each, vectors in global memory) run
+        // on the side stream, concurrently with the on-chip classes; x in its
+        // own buffer, the epilogue writes q
+        LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+        LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        hipLaunchKernelGGL(k_bicg_global<T>, dim3(ctx->bicg_n[2]), dim3(1024), 0, ctx->side, ctx->bicg_cells[2],
+                           ctx->voff, (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->rh,
+                           (T *)ctx->rv, (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, out, nv, p.itrmax, p.residual,
+                           ctx->bicg_iters, ctx->status);
+        LBMIBM_LAUNCHED(ctx, "k_bicg_global");
+    }
     if (ctx->bicg_n[0] > 0) {
         const size_t lds = 3 * 4 * (size_t)ctx->bicg_maxv[0] * sizeof(T);
         hipLaunchKernelGGL((k_bicg_onchip<T, BICG0_K, BICG0_NT>), dim3(ctx->bicg_n[0]), dim3(BICG0_NT), lds,
@@ -656,12 +671,8 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
         }
     }
     if (ctx->bicg_n[2] > 0) {
-        // x in its own buffer; the epilogue writes q
-        hipLaunchKernelGGL(k_bicg_global<T>, dim3(ctx->bicg_n[2]), dim3(1024), 0, ctx->stream, ctx->bicg_cells[2],
-                           ctx->voff, (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->rh,
-                           (T *)ctx->rv, (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, out, nv, p.itrmax, p.residual,
-                           ctx->bicg_iters, ctx->status);
-        LBMIBM_LAUNCHED(ctx, "k_bicg_global");
+        LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_join, ctx->side));
+        LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     }
     if (p.br != 0.0) {
         hipLaunchKernelGGL(k_bend<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
