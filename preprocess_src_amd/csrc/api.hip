@@ -172,7 +172,7 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
                     ctx->tile_off, ctx->tile_cur, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
                     ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv,
-                    ctx->tile_cand, ctx->order_spread, ctx->order_interp};
+                    ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->lec, ctx->epair};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -347,7 +347,48 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
             CK(up_T(v, nullptr, 3 * (size_t)nv));
         CK(up_T(&ctx->coef, nullptr, 6 * (size_t)nv));
         CK(up_T(&ctx->dsv, nullptr, (size_t)nv));
-        CK(dev_alloc(ctx, (void **)&ctx->nbr, 6 * (size_t)nv * sizeof(unsigned short)));
+        // static system structure: cell-local column ids of the 6 off-diagonal
+        // entries (lo order without the vertex itself, init2.cu:169-203), the
+        // corner of each vertex in its incident elements and, per column, the
+        // two incident elements that share the edge
+        {
+            std::vector<unsigned short> nbr16(6 * (size_t)nv);
+            std::vector<int> lec(6 * (size_t)nv, 0);
+            std::vector<unsigned char> epair(6 * (size_t)nv, 0xff);
+            for (int j = 0; j < nv; j++) {
+                const int n = nln[j], base = cells->voff[vcell[j]];
+                for (int sl = 0; sl < n; sl++) {
+                    const int e = le[6 * (size_t)j + sl];
+                    const int *ev = cells->ele + 3 * (size_t)e;
+                    lec[6 * (size_t)j + sl] = 3 * e + (ev[0] == j ? 0 : (ev[1] == j ? 1 : 2));
+                }
+                int m = 0;
+                for (int k = 0; k <= n && k < 7; k++) {
+                    const int l = lo[7 * (size_t)j + k];
+                    if (l == j) continue;
+                    int sa = -1, sb = -1;
+                    for (int sl = 0; sl < n; sl++) {
+                        const int *ev = cells->ele + 3 * (size_t)le[6 * (size_t)j + sl];
+                        if (ev[0] == l || ev[1] == l || ev[2] == l) {
+                            if (sa < 0) sa = sl;
+                            else if (sb < 0) sb = sl;
+                        }
+                    }
+                    if (m >= 6 || sa < 0 || sb < 0) return fail(LBMIBM_ERR_TOPOLOGY);
+                    nbr16[(size_t)m * nv + j] = (unsigned short)(l - base);
+                    epair[6 * (size_t)j + m] = (unsigned char)(sa | (sb << 4));
+                    m++;
+                }
+                for (; m < 6; m++) nbr16[(size_t)m * nv + j] = (unsigned short)(j - base);
+            }
+            CK(dev_alloc(ctx, (void **)&ctx->nbr, 6 * (size_t)nv * sizeof(unsigned short)));
+            LBMIBM_HIP(ctx, hipMemcpy(ctx->nbr, nbr16.data(), nbr16.size() * sizeof(unsigned short),
+                                      hipMemcpyHostToDevice));
+            CK(up_i(&ctx->lec, lec.data(), lec.size()));
+            CK(dev_alloc(ctx, (void **)&ctx->epair, epair.size()));
+            LBMIBM_HIP(ctx, hipMemcpy(ctx->epair, epair.data(), epair.size(), hipMemcpyHostToDevice));
+            CK(dev_alloc(ctx, &ctx->corner, 3 * (size_t)nt * 4 * szT));
+        }
         // cells solved on chip and the rest
         std::vector<int> cls[3];
         for (int c = 0; c < nc; c++) {

@@ -86,6 +86,9 @@ struct lbmibm_ctx {
     void *bv = nullptr, *rv = nullptr, *rh = nullptr, *pv = nullptr, *vv = nullptr, *tv = nullptr;
     void *gscale = nullptr;                            // [nc] (T)
     void *dsv = nullptr;                               // [Nv] ring area x force scale (T)
+    void *corner = nullptr;                            // [3Nt] per-corner rhs terms and rg (T4)
+    int *lec = nullptr;                                // [6Nv] corner of vertex j in its s-th element
+    unsigned char *epair = nullptr;                    // [6Nv] slots sharing the edge to column m
     double *normal = nullptr;                          // [3Nt] FP64 element normals
     double *vol0 = nullptr, *vol = nullptr, *area = nullptr, *gx = nullptr;  // [nc], [3nc]
     int *bicg_iters = nullptr, *vol_passes = nullptr;  // [nc]

@@ -5,16 +5,18 @@
 // The reference assembles, per cell and one cell at a time, an ELL matrix of
 // 3V rows and runs a host-driven BiCGSTAB with a blocking device-to-host copy
 // per inner product.  Here:
-//   - k_tension: one thread per element (all cells in one launch);
-//   - k_assemble: one thread per vertex writes the 7 Jacobi-scaled row
-//     coefficients once (x, y and z rows share them, stress.cu:164-171) and
-//     the three scaled right-hand sides;
+//   - k_elem: one thread per element (all cells in one launch) computes the
+//     tension and each corner's right-hand-side terms and area;
+//   - k_vert: one thread per vertex gathers its corners and writes the 6
+//     Jacobi-scaled off-diagonal coefficients once (x, y and z rows share
+//     them, stress.cu:164-171), the three scaled right-hand sides and the
+//     ring area; the column ids are static and built at set-up;
 //   - k_bicg_onchip / k_bicg_global: one 1024-thread work group per cell runs
 //     the reference's BiCGSTAB iteration to convergence on the device (FP64
 //     inner products reduced in a fixed order, device-side convergence and
 //     breakdown flags); cells of up to 3072 vertices keep every vector and
 //     their matrix rows in registers and LDS, larger ones use global memory;
-//   - q2f is fused: k_assemble also forms the ring area ds of each vertex
+//   - q2f is fused: k_vert also forms the ring area ds of each vertex
 //     and the solver epilogue writes f = -q ds with the load q;
 //   - k_bend as the reference.
 // Vertex positions, the reference metric and all geometric arithmetic
@@ -45,11 +47,22 @@ struct TensionArgs {
     int neo, fileoutput, m;
 };
 
+// Per-element part of stress(): calc_tension (stress.cu:29-121) and, for each
+// of the element's three corners, its terms of the consistent-mass system of
+// calc_load (stress.cu:182-272): the right-hand side
+//   sum_{alpha,beta} rg t^{alpha beta} (e_alpha dN_beta + e_beta dN_alpha) / 4
+// per component and the element's rg, from which the per-vertex pass builds
+// the mass-matrix row (rg/12 diagonal, rg/24 per shared edge).
+template <typename T> struct Corner;
+template <> struct Corner<float> { using type = float4; };
+template <> struct Corner<double> { using type = double4; };
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_tension(const double *__restrict__ xd, const int *__restrict__ ele,
-                                                 const double *__restrict__ nref, T *__restrict__ tt,
-                                                 T *__restrict__ rg, T *__restrict__ tp, int ne,
-                                                 PeriodicBox box, TensionArgs a, const DevStep *__restrict__ st) {
+__global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, const int *__restrict__ ele,
+                                              const double *__restrict__ nref, T *__restrict__ tt,
+                                              T *__restrict__ rg, T *__restrict__ tp,
+                                              typename Corner<T>::type *__restrict__ corner, int ne,
+                                              PeriodicBox box, TensionArgs a, const DevStep *__restrict__ st) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ne) return;
     const int ea = ele[3 * j], eb = ele[3 * j + 1], ec = ele[3 * j + 2];
@@ -74,12 +87,35 @@ __global__ __launch_bounds__(256) void k_tension(const double *__restrict__ xd, 
     const double i2 = dv * dn - 1.0;
     const double js = sqrt(i2 + 1.0);
     const double nn[4] = {n0, n1, n2, n3}, nd[4] = {nd0, nd1, nd1, nd3};
+    double tk[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         double t;
         if (!a.neo) t = a.rgs * (i1 + 1.0) / js * nn[k] + (a.c * i2 - 1.0) * js * nd[k];
         else t = a.rgs * (nn[k] / js - nd[k] * js / (i2 + 1.0) / (i2 + 1.0));
         tt[4 * j + k] = (T)t;
+        tk[k] = (double)(T)t;  // the stored tension, as the reference's calc_load reads it
+    }
+    // corners a, b, c: dN/dxi of the linear triangle (-1,-1), (1,0), (0,1)
+    const double gab = (double)(T)sqrt(dv);
+    const double F[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+#pragma unroll
+    for (int en = 0; en < 3; en++) {
+        double tb[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double vd[2] = {v1[d], v2[d]};
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int al = k >> 1, be = k & 1;
+                sum += gab * tk[k] * (vd[al] * F[en][be] + vd[be] * F[en][al]) / 4.0;
+            }
+            tb[d] = sum;
+        }
+        typename Corner<T>::type cv;
+        cv.x = (T)tb[0]; cv.y = (T)tb[1]; cv.z = (T)tb[2]; cv.w = (T)gab;
+        corner[3 * (size_t)j + en] = cv;
     }
     if (tp) {  // principal-tension accumulator for output (stress.cu:101-118)
         const int iter = st->iter;
@@ -94,90 +130,49 @@ __global__ __launch_bounds__(256) void k_tension(const double *__restrict__ xd, 
     }
 }
 
+// Per-vertex part of calc_load: the Jacobi-scaled row (stress.cu:263-268) and
+// right-hand side of each vertex from the corner terms of its incident
+// elements, summed in the reference's order (le list), plus the q2f ring area
+// (stress.cu:415-438: a third of the incident triangle areas, rg/6 each)
+// times the cell's force scale.  lec[6j+s] is the corner of vertex j in its
+// s-th incident element; epair[6j+m] holds the two slots s_A < s_B whose
+// elements share the edge to the m-th off-diagonal column (4 bits each,
+// 0xff for an unused column).
 template <typename T>
-__global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ xd, const int *__restrict__ ele,
-                                                  const int *__restrict__ nln, const int *__restrict__ lo8,
-                                                  const int *__restrict__ le, const T *__restrict__ rg,
-                                                  const T *__restrict__ tt, const int *__restrict__ vcell,
-                                                  const int *__restrict__ voff, const T *__restrict__ gscale,
-                                                  T *__restrict__ coef, unsigned short *__restrict__ nbr,
-                                                  T *__restrict__ bv, T *__restrict__ dsv, int nv,
-                                                  PeriodicBox box) {
+__global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const int *__restrict__ lec,
+                                              const unsigned char *__restrict__ epair,
+                                              const typename Corner<T>::type *__restrict__ corner,
+                                              const int *__restrict__ vcell, const T *__restrict__ gscale,
+                                              T *__restrict__ coef, T *__restrict__ bv, T *__restrict__ dsv,
+                                              int nv) {
     const int jj = blockIdx.x * blockDim.x + threadIdx.x;
     if (jj >= nv) return;
     const int n = nln[jj];
-    int lo[8];
+    double rgs[6], tb[3] = {0.0, 0.0, 0.0}, jac = 0.0, ds = 0.0;
 #pragma unroll
-    for (int k = 0; k < 8; k++) lo[k] = lo8[8 * jj + k];
-    double tv[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    double tb[3] = {0.0, 0.0, 0.0};
-    double ds = 0.0;  // q2f ring area (stress.cu:415-438): a third of the incident triangles
-    int order = 0;
-    for (int i = 0; i < n; i++) {
-        const int e = le[6 * jj + i];
-        const double gab = (double)rg[e];
-        ds += gab / 2.0 / 3.0;  // rg = |e1 x e2| = twice the triangle area
-        const int ea = ele[3 * e], eb = ele[3 * e + 1], ec = ele[3 * e + 2];
-        int oa = 0, ob = 0, oc = 0;
-#pragma unroll
-        for (int k = 0; k < 7; k++) {
-            if (k <= n) {
-                if (ea == lo[k]) oa = k;
-                if (eb == lo[k]) ob = k;
-                if (ec == lo[k]) oc = k;
-                if (jj == lo[k]) order = k;
-            }
-        }
-        // consistent mass matrix rg/12 on the diagonal, rg/24 off it (stress.cu:237-242)
-        const double wa = ea == jj ? gab / 12.0 : gab / 24.0;
-        const double wb = eb == jj ? gab / 12.0 : gab / 24.0;
-        const double wc = ec == jj ? gab / 12.0 : gab / 24.0;
-#pragma unroll
-        for (int k = 0; k < 7; k++) {
-            if (k == oa) tv[k] += wa;
-            if (k == ob) tv[k] += wb;
-            if (k == oc) tv[k] += wc;
-        }
-        const int en = ea == jj ? 0 : (eb == jj ? 1 : 2);
-        // dN/dxi of the linear triangle: a(-1,-1), b(1,0), c(0,1)
-        const double f0 = en == 0 ? -1.0 : (en == 1 ? 1.0 : 0.0);
-        const double f1 = en == 0 ? -1.0 : (en == 1 ? 0.0 : 1.0);
-        const double t0 = tt[4 * e], t1 = tt[4 * e + 1], t2 = tt[4 * e + 2], t3 = tt[4 * e + 3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            double vd0 = xd[3 * eb + d] - xd[3 * ea + d];
-            double vd1 = xd[3 * ec + d] - xd[3 * ea + d];
-            if (d == 0) { vd0 = dev_min_image<double>(vd0, box.lx); vd1 = dev_min_image<double>(vd1, box.lx); }
-            else if (d == 2) { vd0 = dev_min_image<double>(vd0, box.lz); vd1 = dev_min_image<double>(vd1, box.lz); }
-            else if (box.py) { vd0 = dev_min_image<double>(vd0, box.ly); vd1 = dev_min_image<double>(vd1, box.ly); }
-            // sum over (alpha,beta) of t^{ab} (vd_a dN_b + vd_b dN_a)/4 (stress.cu:256-260)
-            double s = tb[d];
-            s += gab * t0 * (vd0 * f0 + vd0 * f0) / 4.0;
-            s += gab * t1 * (vd0 * f1 + vd1 * f0) / 4.0;
-            s += gab * t2 * (vd1 * f0 + vd0 * f1) / 4.0;
-            s += gab * t3 * (vd1 * f1 + vd1 * f1) / 4.0;
-            tb[d] = s;
+    for (int s = 0; s < 6; s++) {
+        rgs[s] = 0.0;
+        if (s < n) {
+            const typename Corner<T>::type cv = corner[lec[6 * (size_t)jj + s]];
+            const double gab = (double)cv.w;
+            rgs[s] = gab;
+            jac += gab / 12.0;
+            ds += gab / 2.0 / 3.0;
+            tb[0] += (double)cv.x;
+            tb[1] += (double)cv.y;
+            tb[2] += (double)cv.z;
         }
     }
-    double jac = tv[0];
 #pragma unroll
-    for (int k = 1; k < 7; k++)
-        if (k == order) jac = tv[k];
-    // Jacobi-scaled row (stress.cu:263-268): the diagonal is exactly 1, the
-    // off-diagonal entries are stored with cell-local column ids
-    const int base = voff[vcell[jj]];
-    int m = 0;
+    for (int m = 0; m < 6; m++) {
+        const unsigned ep = epair[6 * (size_t)jj + m];
+        double ra = 0.0, rb = 0.0;
 #pragma unroll
-    for (int k = 0; k < 7; k++) {
-        if (k <= n && k != order) {
-            coef[(size_t)m * nv + jj] = (T)(tv[k] / jac);
-            nbr[(size_t)m * nv + jj] = (unsigned short)(lo[k] - base);
-            m++;
+        for (int s = 0; s < 6; s++) {
+            if ((int)(ep & 15u) == s) ra = rgs[s];
+            if ((int)(ep >> 4) == s) rb = rgs[s];
         }
-    }
-    for (; m < 6; m++) {
-        coef[(size_t)m * nv + jj] = T(0);
-        nbr[(size_t)m * nv + jj] = (unsigned short)(jj - base);
+        coef[(size_t)m * nv + jj] = ep == 0xffu ? T(0) : (T)((ra / 24.0 + rb / 24.0) / jac);
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
@@ -624,15 +619,15 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
     ta.fileoutput = p.fileoutput > 0 ? p.fileoutput : 1;
     ta.wtp = (double)p.m / (double)ta.fileoutput;
     const int ne = ctx->nt, nv = ctx->nv;
-    hipLaunchKernelGGL(k_tension<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
+    hipLaunchKernelGGL(k_elem<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
                        ctx->ele, (const double *)ctx->nref, (T *)ctx->tt, (T *)ctx->rg,
-                       p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ne, box, ta, ctx->step);
-    LBMIBM_LAUNCHED(ctx, "k_tension");
-    hipLaunchKernelGGL(k_assemble<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
-                       ctx->ele, ctx->nln, ctx->lo8, ctx->le, (const T *)ctx->rg, (const T *)ctx->tt, ctx->vcell,
-                       ctx->voff, (const T *)ctx->gscale, (T *)ctx->coef, ctx->nbr, (T *)ctx->bv, (T *)ctx->dsv, nv,
-                       box);
-    LBMIBM_LAUNCHED(ctx, "k_assemble");
+                       p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, (typename Corner<T>::type *)ctx->corner, ne,
+                       box, ta, ctx->step);
+    LBMIBM_LAUNCHED(ctx, "k_elem");
+    hipLaunchKernelGGL(k_vert<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln, ctx->lec,
+                       ctx->epair, (const typename Corner<T>::type *)ctx->corner, ctx->vcell,
+                       (const T *)ctx->gscale, (T *)ctx->coef, (T *)ctx->bv, (T *)ctx->dsv, nv);
+    LBMIBM_LAUNCHED(ctx, "k_vert");
     SolveOut<T> out;
     out.q = (T *)ctx->qv;
     out.cf = (T *)ctx->cf;
