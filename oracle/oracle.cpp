@@ -854,7 +854,7 @@ int orc_fem_topology(int nc, const int *voff, const int *eoff, const double *xr,
             if (nb == ln[7 * na + i]) return 1;
         return 0;
     };
-    for (int c = 0; c < nc; c++) {
+    auto topology_cell = [&](int c) -> int {
         const int V = voff[c + 1] - voff[c], T = eoff[c + 1] - eoff[c];
         for (int eid = eoff[c]; eid < eoff[c + 1]; eid++) {  // fem_init (init2.cu:317-341)
             const int ea = ele[3 * eid], eb = ele[3 * eid + 1], ecc = ele[3 * eid + 2];
@@ -936,7 +936,14 @@ int orc_fem_topology(int nc, const int *voff, const int *eoff, const double *xr,
                 s0[3 * eid + k] = 0.0;
             }
         }
-    }
+            return 0;
+    };
+    // cells are independent (disjoint vertex and element ranges): one cell per
+    // OpenMP task, the first error code of any cell is returned
+    int err = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : err)
+    for (int c = 0; c < nc; c++) err = std::max(err, topology_cell(c));
+    if (err) return err;
     // set_contravariant (init2.cu:206-255)
     for (int eid = 0; eid < ne; eid++) {
         const int ea = ele[3 * eid], eb = ele[3 * eid + 1], ecc = ele[3 * eid + 2];
