@@ -520,10 +520,12 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
 
 // ---- interpolation + advection ------------------------------------------------
 
-// Velocity box staged per tile: the tile plus 3 nodes below and 4 above on
-// every axis (first stencil node of the tile's vertices, 4-node stencil and a
-// one-node move of the RK stages).
-constexpr int RX = TILE_X + 7, RY = TILE_Y + 7, RZ = TILE_Z + 7, RN = RX * RY * RZ;
+// Velocity box staged per tile: the tile plus 1 node below and 4 above on
+// every axis.  The first stencil node of a home vertex lies in the tile, an
+// RK stage moves it by less than one node in practice (|u| M dt << dx), and
+// the stencil spans 4 nodes; stages that leave the box read global memory.
+constexpr int RX = TILE_X + 5, RY = TILE_Y + 5, RZ = TILE_Z + 5, RN = RX * RY * RZ;
+constexpr int RLO = 1;  // nodes of the box below the tile
 
 // Out-of-line and not unrolled: it is the rare path, and inlining it into
 // every RK stage would multiply the kernel's code size.
@@ -613,7 +615,7 @@ __global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, d
     const int cnt = tile_count[tile], off = tile_off[tile];
     if (cnt == 0) return;  // whole work group leaves together
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
-    const int bx = tx * TILE_X - 3, by = ty * TILE_Y - 3, bz = tz * TILE_Z - 3;
+    const int bx = tx * TILE_X - RLO, by = ty * TILE_Y - RLO, bz = tz * TILE_Z - RLO;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
     for (int k = threadIdx.x; k < RN; k += blockDim.x) {
         const int rx = k % RX, ry = (k / RX) % RY, rz = k / (RX * RY);

@@ -51,12 +51,22 @@ __device__ void cell_sums(const double *__restrict__ xs, int nv, const int *__re
     A = va[1];
 }
 
-__device__ void cell_center(const double *__restrict__ xs, int nv, int v0, int v1, WgReduce &red, const VolArgs &a,
-                            double g[3]) {
-    // CalcCenter (volume_correction.cu:310-352)
+// CalcCenter (volume_correction.cu:310-352), and with `rewrap` also
+// CapsulePeriodicAfter (:70-94) in the same pass: the centroid sums use the
+// unwrapped positions, which are then wrapped back into the box.
+__device__ void cell_center(double *__restrict__ xs, int nv, int v0, int v1, WgReduce &red, const VolArgs &a,
+                            bool rewrap, double g[3]) {
+    double *X = xs, *Y = xs + nv, *Z = xs + 2 * (size_t)nv;
     double s[3] = {0.0, 0.0, 0.0};
-    for (int j = v0 + threadIdx.x; j < v1; j += blockDim.x)
-        for (int d = 0; d < 3; d++) s[d] += (double)xs[(size_t)d * nv + j];
+    for (int j = v0 + threadIdx.x; j < v1; j += blockDim.x) {
+        double x = X[j], y = Y[j], z = Z[j];
+        s[0] += x; s[1] += y; s[2] += z;
+        if (rewrap) {
+            if (x < -a.lx / 2.0) X[j] = x + a.lx; else if (x > a.lx / 2.0) X[j] = x - a.lx;
+            if (z < -a.lz / 2.0) Z[j] = z + a.lz; else if (z > a.lz / 2.0) Z[j] = z - a.lz;
+            if (a.py) { if (y < -a.ly / 2.0) Y[j] = y + a.ly; else if (y > a.ly / 2.0) Y[j] = y - a.ly; }
+        }
+    }
     red.sum<3>(s);
     for (int d = 0; d < 3; d++) g[d] = s[d] / (double)(v1 - v0);
     if (g[0] < -a.lx / 2.0) g[0] += a.lx; else if (g[0] > a.lx / 2.0) g[0] -= a.lx;
@@ -127,15 +137,8 @@ __global__ __launch_bounds__(NT) void k_volume(const int *__restrict__ cells, in
         }
     }
     double g[3];
-    cell_center(xs, nv, v0, v1, red, a, g);
+    cell_center(xs, nv, v0, v1, red, a, mode == 1, g);
     if (threadIdx.x == 0) { gx[3 * c] = g[0]; gx[3 * c + 1] = g[1]; gx[3 * c + 2] = g[2]; }
-    if (mode == 1) {  // CapsulePeriodicAfter (:70-94)
-        for (int j = v0 + threadIdx.x; j < v1; j += blockDim.x) {
-            if (X[j] < -a.lx / 2.0) X[j] += a.lx; else if (X[j] > a.lx / 2.0) X[j] -= a.lx;
-            if (Z[j] < -a.lz / 2.0) Z[j] += a.lz; else if (Z[j] > a.lz / 2.0) Z[j] -= a.lz;
-            if (a.py) { if (Y[j] < -a.ly / 2.0) Y[j] += a.ly; else if (Y[j] > a.ly / 2.0) Y[j] -= a.ly; }
-        }
-    }
 }
 
 lbmibm_status run_impl(lbmibm_ctx *ctx, int mode) {
