@@ -1,6 +1,9 @@
 """Writes tests/golden/channel_cell_f64.npz: inputs and FP64 oracle outputs of a
 small coupled channel (16x14x12 lattice, slab walls, body force, one 162-vertex
-Evans-Fung cell of radius 3 across the x boundary, sheared start, 6 steps).
+Evans-Fung cell of radius 3 next to the periodic x boundary, sheared start,
+6 steps).  Initial positions must lie inside the box, as the reference's
+cell placement guarantees: spreading of positions beyond the halo layers is
+not defined by the reference.
 
 The reference ships no golden vectors; this fixture freezes the oracle's
 outputs so that later changes to the oracle or to the HIP path are checked
@@ -24,7 +27,7 @@ STEPS = 6
 def make_cells():
     from preprocess_src_amd import mesh
     lx, ly, lz = SHAPE
-    centres = np.array([[lx / 2 - 1.5, 0.4, 0.3]])
+    centres = np.array([[lx / 2 - 3.6, 0.4, 0.3]])
     _, rot = mesh.place(1, 3.0, lx, ly, lz, seed=77)
     return mesh.assemble([2], centres, rot, 3.0)
 
