@@ -425,8 +425,13 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         CK(dev_alloc(ctx, (void **)&ctx->tile_off, (size_t)ctx->ntiles * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->tile_cur, (size_t)ctx->ntiles * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->tile_cand, (size_t)ctx->ntiles * sizeof(int)));
-        CK(dev_alloc(ctx, (void **)&ctx->order_spread, (size_t)ctx->ntiles * sizeof(int)));
-        CK(dev_alloc(ctx, (void **)&ctx->order_interp, (size_t)ctx->ntiles * sizeof(int)));
+        {
+            // tile orders start as the identity (any permutation is valid)
+            std::vector<int> ident(ctx->ntiles);
+            for (int t = 0; t < ctx->ntiles; t++) ident[t] = t;
+            CK(up_i(&ctx->order_spread, ident.data(), ident.size()));
+            CK(up_i(&ctx->order_interp, ident.data(), ident.size()));
+        }
         CK(dev_alloc(ctx, (void **)&ctx->blk_fmax, (size_t)ctx->nblk_bin * sizeof(float)));
         CK(dev_alloc(ctx, (void **)&ctx->absmax, sizeof(unsigned)));
         CK(ibm_setup(ctx));

@@ -227,47 +227,27 @@ __device__ void heavy_first_order(const int *load, int n, int *hist, int *order)
     __syncthreads();
     int shift = 0;
     while ((maxload >> shift) > 255) shift++;
-    // many tiles share a key (light tiles above all): wave-aggregated counters
-    const int rounds = (n + (int)blockDim.x - 1) / (int)blockDim.x;
-    for (int rd = 0; rd < rounds; rd++) {
-        const int i = rd * blockDim.x + threadIdx.x;
-        (void)wave_aggregated_add(hist, i < n ? 255 - (load[i] >> shift) : 0, i < n);
-    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&hist[255 - (load[i] >> shift)], 1);
     __syncthreads();
-    if (threadIdx.x < 64) {  // exclusive scan of the 256 bins by one wave, 4 per lane
-        const int l = threadIdx.x;
-        const int h0 = hist[4 * l], h1 = hist[4 * l + 1], h2 = hist[4 * l + 2], h3 = hist[4 * l + 3];
-        const int own = h0 + h1 + h2 + h3;
-        int incl = own;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(incl, o);
-            if (l >= o) incl += y;
+    if (threadIdx.x == 0) {  // exclusive scan of 256 bins
+        int acc = 0;
+        for (int k = 0; k < 256; k++) {
+            const int c = hist[k];
+            hist[k] = acc;
+            acc += c;
         }
-        const int ex = incl - own;
-        hist[4 * l] = ex;
-        hist[4 * l + 1] = ex + h0;
-        hist[4 * l + 2] = ex + h0 + h1;
-        hist[4 * l + 3] = ex + h0 + h1 + h2;
     }
     __syncthreads();
-    for (int rd = 0; rd < rounds; rd++) {
-        const int i = rd * blockDim.x + threadIdx.x;
-        const int slot = wave_aggregated_add(hist, i < n ? 255 - (load[i] >> shift) : 0, i < n);
-        if (i < n) order[slot] = i;
-    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) order[atomicAdd(&hist[255 - (load[i] >> shift)], 1)] = i;
     __syncthreads();
 }
 
 __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile_count, int *__restrict__ tile_off,
                                                     int *__restrict__ tile_cur, int ntiles,
                                                     const float *__restrict__ blk_fmax, int nblk,
-                                                    unsigned *__restrict__ absmax, IbmGeom g, TileGeom tg,
-                                                    int *__restrict__ tile_cand, int *__restrict__ order_spread,
-                                                    int *__restrict__ order_interp) {
+                                                    unsigned *__restrict__ absmax) {
     // exclusive scan over tiles in one work group, 1024 entries per round,
-    // the maximum of the per-block force maxima, the spreading candidates of
-    // every tile and the two heavy-first tile orders
+    // and the maximum of the per-block force maxima
     __shared__ int sh[1024];
     __shared__ float shm[1024];
     __shared__ int carry;
@@ -300,7 +280,22 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile
         if (threadIdx.x == 1023) carry += sh[1023];
         __syncthreads();
     }
-    if (!order_spread) return;
+}
+
+// The heavy-first tile orders of the spreading (by candidate count: own bin
+// and the bins below on each axis) and of the interpolation (by own count).
+// They only steer the dispatch order, and tile loads change slowly with the
+// cells, so they are refreshed every TILE_ORDER_PERIOD steps (the step counter
+// is read on the device, so captured graphs refresh them too); any
+// permutation is a valid order, the initial one is the identity.
+constexpr int TILE_ORDER_PERIOD = 16;
+
+__global__ __launch_bounds__(1024) void k_tile_order(const int *__restrict__ tile_count, int ntiles, IbmGeom g,
+                                                     TileGeom tg, int *__restrict__ tile_cand,
+                                                     int *__restrict__ order_spread, int *__restrict__ order_interp,
+                                                     const DevStep *__restrict__ st) {
+    __shared__ int sh[256];
+    if ((st->iter - 1) % TILE_ORDER_PERIOD != 0) return;
     for (int t = threadIdx.x; t < ntiles; t += 1024) {
         const int tx = t % tg.ntx, ty = (t / tg.ntx) % tg.nty, tz = t / (tg.ntx * tg.nty);
         int sxs[4], sys[4], szs[4];
@@ -346,98 +341,74 @@ constexpr int ACC_SX = 20, ACC_SXY = 176, ACC_N = ACC_SXY * TILE_Z;
 //   low mantissa bits: subtracting the constant's bit pattern gives the
 //   two's-complement integer (valid for |value| < 2^51).
 //   FP64: the scale keeps |sum| below 2^62 and llrint converts.
-template <typename T> constexpr int fixed_bits() { return sizeof(T) == 4 ? 50 : 62; }
-
-// Per-(vertex, tile) record in LDS, written by the lane that found the vertex
-// and read by all 64 lanes of its wave, lane l = kx + 4 ky + 16 kz taking
-// stencil node (kx, ky, kz).
-template <typename T> struct SpreadRec {
-    T wx[4];                 // x weights
-    T wyz[16];               // wy[ky] * wz[kz] / (64 dx^3), index ky + 4 kz
-    double fs[3];            // force times the fixed-point scale (FP32) or force (FP64)
-    unsigned mlo, mhi;       // lanes whose node lies in the tile
-    unsigned ox4, oy4, oz4;  // tile-relative node offsets, one byte per k
+template <typename T> struct FixedPoint;
+template <> struct FixedPoint<float> {
+    static constexpr int kBits = 50;
+    double f[3];
+    __device__ __forceinline__ FixedPoint(float f0, float f1, float f2, double scale) {
+        f[0] = (double)f0 * scale; f[1] = (double)f1 * scale; f[2] = (double)f2 * scale;
+    }
+    __device__ __forceinline__ void add(unsigned long long *acc, int li, float w) const {
+        const double wd = (double)w;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double t = fma(wd, f[d], 6755399441055744.0);  // + 1.5 * 2^52
+            atomicAdd(&acc[d * ACC_N + li],
+                      (unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull);
+        }
+    }
+};
+template <> struct FixedPoint<double> {
+    static constexpr int kBits = 62;
+    double f[3], scale;
+    __device__ __forceinline__ FixedPoint(double f0, double f1, double f2, double s) : scale(s) {
+        f[0] = f0; f[1] = f1; f[2] = f2;
+    }
+    __device__ __forceinline__ void add(unsigned long long *acc, int li, double w) const {
+#pragma unroll
+        for (int d = 0; d < 3; d++) atomicAdd(&acc[d * ACC_N + li], (unsigned long long)llrint(w * f[d] * scale));
+    }
 };
 
-// 64-bit lane mask of the stencil nodes inside the tile from the per-axis
-// 4-bit masks: bit kx + 4 ky + 16 kz.
-__device__ __forceinline__ unsigned long long stencil_lane_mask(unsigned mx, unsigned my, unsigned mz) {
-    unsigned long long m = 0;
-#pragma unroll
-    for (int kz = 0; kz < 4; kz++)
-#pragma unroll
-        for (int ky = 0; ky < 4; ky++)
-            if (((my >> ky) & 1u) && ((mz >> kz) & 1u)) m |= (unsigned long long)mx << (4 * ky + 16 * kz);
-    return m;
-}
-
 template <typename T>
-__device__ __forceinline__ void make_record(const double *__restrict__ xs, const T *__restrict__ cf, int nv, int v,
-                                            const IbmGeom &g, int ox, int oy, int oz, int lx, int ly, int lz,
-                                            double scale, T w64, SpreadRec<T> &r) {
+__device__ __forceinline__ void spread_vertex(const double *__restrict__ xs, const T *__restrict__ cf, int nv, int v,
+                                              const IbmGeom &g, int ox, int oy, int oz, int lx, int ly, int lz,
+                                              double scale, T w64, unsigned long long *acc) {
     int x1, y1, z1;
     T wx[4], wy[4], wz[4];
     stencil3<T>(g, xs[v], xs[nv + v], xs[2 * (size_t)nv + v], false, x1, y1, z1, wx, wy, wz);
     const unsigned mx = axis_mask(x1, ox, lx, g.nx, true);
     const unsigned my = axis_mask(y1, oy, ly, g.ny, !g.walls);
     const unsigned mz = axis_mask(z1, oz, lz, g.nz, true);
-    const unsigned long long m = stencil_lane_mask(mx, my, mz);
-    unsigned o4[3] = {0u, 0u, 0u};
+    if (!mx || !my || !mz) return;
+    const FixedPoint<T> fp(cf[v], cf[nv + v], cf[2 * (size_t)nv + v], scale);
+    // tile-relative offsets of the stencil nodes on each axis
+    int ax[4], by[4], cz[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         int a = x1 + k, b = y1 + k, c = z1 + k;
         wrap_index(a, g.nx);
         wrap_index(c, g.nz);
         if (!g.walls) wrap_index(b, g.ny);
-        o4[0] |= ((unsigned)(a - ox) & 0xffu) << (8 * k);
-        o4[1] |= ((unsigned)(b - oy) & 0xffu) << (8 * k);
-        o4[2] |= ((unsigned)(c - oz) & 0xffu) << (8 * k);
+        ax[k] = a - ox;
+        by[k] = ACC_SX * (b - oy);
+        cz[k] = ACC_SXY * (c - oz);
     }
-#pragma unroll
-    for (int k = 0; k < 4; k++) r.wx[k] = wx[k];
 #pragma unroll
     for (int kz = 0; kz < 4; kz++) {
-        const T wz64 = wz[kz] * w64;
+        if (!((mz >> kz) & 1u)) continue;
 #pragma unroll
-        for (int ky = 0; ky < 4; ky++) r.wyz[ky + 4 * kz] = wy[ky] * wz64;
+        for (int ky = 0; ky < 4; ky++) {
+            if (!((my >> ky) & 1u)) continue;
+            const T wyz = wy[ky] * wz[kz] * w64;
+            const int row = by[ky] + cz[kz];
+#pragma unroll
+            for (int kx = 0; kx < 4; kx++) {
+                if (!((mx >> kx) & 1u)) continue;
+                fp.add(acc, row + ax[kx], wx[kx] * wyz);
+            }
+        }
     }
-    const double sf = sizeof(T) == 4 ? scale : 1.0;
-    r.fs[0] = (double)cf[v] * sf;
-    r.fs[1] = (double)cf[nv + v] * sf;
-    r.fs[2] = (double)cf[2 * (size_t)nv + v] * sf;
-    r.mlo = (unsigned)m;
-    r.mhi = (unsigned)(m >> 32);
-    r.ox4 = o4[0]; r.oy4 = o4[1]; r.oz4 = o4[2];
-}
-
-// Lane-per-node accumulation of one record: the 64 nodes of a stencil fall on
-// distinct 8-byte LDS slots up to a 2-way overlap (strides 20 and 176), so the
-// atomics of a wave do not conflict.
-__device__ __forceinline__ void acc_record(const SpreadRec<float> &r, int lane, double, unsigned long long *acc) {
-    const unsigned long long m = ((unsigned long long)r.mhi << 32) | r.mlo;
-    if (!((m >> lane) & 1ull)) return;
-    const int kx = lane & 3, kyz = lane >> 2, ky = kyz & 3, kz = lane >> 4;
-    const int li = (int)((r.ox4 >> (8 * kx)) & 0xffu) + ACC_SX * (int)((r.oy4 >> (8 * ky)) & 0xffu) +
-                   ACC_SXY * (int)((r.oz4 >> (8 * kz)) & 0xffu);
-    const double w = (double)(r.wx[kx] * r.wyz[kyz]);
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        // round(w f scale) as a two's-complement integer: 1.5 * 2^52 is added
-        // in the FMA, which leaves the integer in the low mantissa bits
-        const double t = fma(w, r.fs[d], 6755399441055744.0);
-        atomicAdd(&acc[d * ACC_N + li], (unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull);
-    }
-}
-__device__ __forceinline__ void acc_record(const SpreadRec<double> &r, int lane, double scale,
-                                           unsigned long long *acc) {
-    const unsigned long long m = ((unsigned long long)r.mhi << 32) | r.mlo;
-    if (!((m >> lane) & 1ull)) return;
-    const int kx = lane & 3, kyz = lane >> 2, ky = kyz & 3, kz = lane >> 4;
-    const int li = (int)((r.ox4 >> (8 * kx)) & 0xffu) + ACC_SX * (int)((r.oy4 >> (8 * ky)) & 0xffu) +
-                   ACC_SXY * (int)((r.oz4 >> (8 * kz)) & 0xffu);
-    const double w = r.wx[kx] * r.wyz[kyz];
-#pragma unroll
-    for (int d = 0; d < 3; d++) atomicAdd(&acc[d * ACC_N + li], (unsigned long long)llrint(w * r.fs[d] * scale));
 }
 
 template <typename T>
@@ -449,7 +420,7 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
                                                       IbmGeom g, TileGeom tg, const unsigned *__restrict__ absmax,
                                                       const int *__restrict__ order) {
     __shared__ unsigned long long acc[3 * ACC_N];
-    __shared__ SpreadRec<T> recs[4][64];
+    __shared__ int queue[4][128];
     const int tile = order[blockIdx.x];
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
     const int ox = tx * TILE_X, oy = ty * TILE_Y, oz = tz * TILE_Z;
@@ -462,7 +433,7 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
     const int nsz = source_tiles(tz, oz, g.nz, TILE_Z, true, szs);
     // fixed-point scale: at most `total` vertices reach one node, each with a
     // weight at most 1/(8 dx^3), so |sum| <= total |f|max / (8 dx^3), which
-    // the power of two keeps below 2^fixed_bits / scale
+    // the power of two keeps below 2^kBits / scale
     long long total = 0;
     for (int iz = 0; iz < nsz; iz++)
         for (int iy = 0; iy < nsy; iy++)
@@ -471,16 +442,21 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
     double scale = 1.0;
     if (fmax > 0.0 && total > 0) {
         const double bound = (double)total * fmax / (8.0 * g.dx * g.dx * g.dx);
-        scale = ldexp(1.0, fixed_bits<T>() - 1 - ilogb(bound));
+        scale = ldexp(1.0, FixedPoint<T>::kBits - 1 - ilogb(bound));
     }
     const T w64 = (T)(1.0 / (64.0 * g.dx * g.dx * g.dx));
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    SpreadRec<T> *q = recs[wave];
+    int *q = queue[wave];
+    int qn = 0;
     for (int s = 0; s < nsx * nsy * nsz; s++) {
         const int bin = sxs[s % nsx] + tg.ntx * (sys[(s / nsx) % nsy] + tg.nty * szs[s / (nsx * nsy)]);
         const int cnt = tile_count[bin], off = tile_off[bin];
+        // the bin read as 64 columns of R rows, lane l taking column l: the
+        // lanes of a wave get vertices far apart in the bin, whose stencils
+        // rarely share LDS addresses (64-bit LDS atomics to one address
+        // serialise, and neighbouring vertices in a bin are close in space)
         const int R = (cnt + 63) >> 6;
         for (int r = wave; r < R; r += 4) {
             const int i = lane * R + r;
@@ -493,18 +469,27 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
                         axis_mask(n0.z, oz, lz, g.nz, true);
             }
             const unsigned long long m = __ballot(touch);
-            if (touch) make_record<T>(xs, cf, nv, v, g, ox, oy, oz, lx, ly, lz, scale, w64,
-                                      q[__popcll(m & ((1ull << lane) - 1ull))]);
-            const int n = __popcll(m);
+            if (touch) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = v;
+            qn += __popcll(m);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            for (int k = 0; k < n; k++) acc_record(q[k], lane, scale, acc);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (qn >= 64) {
+                spread_vertex<T>(xs, cf, nv, q[lane], g, ox, oy, oz, lx, ly, lz, scale, w64, acc);
+                const int rest = qn - 64;
+                const int mv = lane < rest ? q[64 + lane] : 0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (lane < rest) q[lane] = mv;
+                qn = rest;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
         }
     }
+    if (lane < qn) spread_vertex<T>(xs, cf, nv, q[lane], g, ox, oy, oz, lx, ly, lz, scale, w64, acc);
     __syncthreads();
     const double inv_scale = 1.0 / scale;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
@@ -697,9 +682,11 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
                        (const T *)ctx->cf, nv, g, tg, ctx->node0, ctx->home, ctx->tile_count, ctx->blk_fmax);
     LBMIBM_LAUNCHED(ctx, "k_bin_count");
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->tile_count, ctx->tile_off,
-                       ctx->tile_cur, ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax, g, tg, ctx->tile_cand,
-                       ctx->order_spread, ctx->order_interp);
+                       ctx->tile_cur, ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax);
     LBMIBM_LAUNCHED(ctx, "k_tile_scan");
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, ctx->stream, ctx->tile_count, ctx->ntiles, g, tg,
+                       ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->step);
+    LBMIBM_LAUNCHED(ctx, "k_tile_order");
     hipLaunchKernelGGL(k_bin_scatter, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, ctx->home, nv, ctx->tile_off,
                        ctx->tile_cur, ctx->home_list);
     LBMIBM_LAUNCHED(ctx, "k_bin_scatter");

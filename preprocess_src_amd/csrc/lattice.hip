@@ -36,21 +36,13 @@ template <typename T> __host__ __device__ constexpr T wq(int q) {
 }
 template <typename T> __host__ __device__ constexpr T inv_wq(int q) { return q == 0 ? T(3) : (q <= 6 ? T(18) : T(36)); }
 
-// Population storage.  Stores are non-temporal: every population is written
-// once per step and the lattice is several times the size of the caches, so
-// keeping the written lines would only evict other data.  Loads stay normal:
-// a pulled row is shifted by one node against the wave, so neighbouring waves
-// share its edge lines through L2.
 template <typename S, typename T> struct Pop {
     __device__ static T ld(const S *p, int) { return (T)(*p); }
-    __device__ static void st(S *p, T g, int) { __builtin_nontemporal_store((S)g, p); }
+    __device__ static S st(T g, int) { return (S)g; }
 };
 template <typename T> struct Pop<__half, T> {
     __device__ static T ld(const __half *p, int q) { return (T)__half2float(*p) * wq<T>(q); }
-    __device__ static void st(__half *p, T g, int q) {
-        const __half h = __float2half((float)(g * inv_wq<T>(q)));
-        __builtin_nontemporal_store(__half_as_ushort(h), reinterpret_cast<unsigned short *>(p));
-    }
+    __device__ static __half st(T g, int q) { return __float2half((float)(g * inv_wq<T>(q))); }
 };
 
 struct LbmDims {
@@ -126,7 +118,7 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         const T geq = wq<T>(q) * (drho + rho * (T(3) * cu + T(4.5) * cu * cu - T(1.5) * uu));
         const T src = wq<T>(q) * kf * (T(3) * ca * (T(1) + T(3) * cu) - T(3) * Ua);
         const T gs = g[q] + om * (geq - g[q]) + src;
-        Pop<S, T>::st(fout + (size_t)q * N + i, gs, q);
+        fout[(size_t)q * N + i] = Pop<S, T>::st(gs, q);
     }
 
     const int iter = st->iter;
@@ -159,7 +151,7 @@ __global__ void k_from_fstar(const double *__restrict__ fs, S *__restrict__ f, u
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
 #pragma unroll
-    for (int q = 0; q < Q; q++) Pop<S, T>::st(f + (size_t)q * N + i, (T)(fs[(size_t)q * N + i] - wq<double>(q)), q);
+    for (int q = 0; q < Q; q++) f[(size_t)q * N + i] = Pop<S, T>::st((T)(fs[(size_t)q * N + i] - wq<double>(q)), q);
 }
 
 template <typename S, typename T>
