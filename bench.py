@@ -148,6 +148,7 @@ def main():
         its = sim.membrane()["bicg_iters"]
         nvc = np.diff(np.asarray(cells["voff"]))
         diag["bicg_iters_mean_by_vertices"] = {int(k): round(float(its[nvc == k].mean()), 2) for k in np.unique(nvc)}
+        diag["autotune"] = [l for l in sim.autotune_report().splitlines() if l]
     t_lbm = stages["lbm_d3q19"][0] * 1e-3
     coupled = cells is not None
     bpn = lattice_bytes_per_node(a.precision, coupled)

@@ -110,6 +110,12 @@ void          lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz);
 lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, lbmibm_ctx **out);
 void          lbmibm_destroy(lbmibm_ctx *ctx);
 const char   *lbmibm_last_error(const lbmibm_ctx *ctx);
+/* No reference counterpart.  lbmibm_create times the candidate work-group
+ * shapes of the on-chip membrane solvers on the cells' own matrices and keeps
+ * the fastest (in the environment, LBMIBM_AUTOTUNE=0 skips it and uses the
+ * first shapes, LBMIBM_BICG_SHAPES="a,b" fixes the variants of the two
+ * classes); this returns the timings and choices as text, owned by ctx. */
+const char   *lbmibm_autotune_report(const lbmibm_ctx *ctx);
 /* Sizes: padded lattice nodes n, vertices, elements, cells. */
 void          lbmibm_sizes(const lbmibm_ctx *ctx, long long *n_padded, int *nv, int *nt, int *nc);
 

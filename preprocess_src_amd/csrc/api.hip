@@ -215,6 +215,8 @@ void lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz) {
 
 const char *lbmibm_last_error(const lbmibm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+const char *lbmibm_autotune_report(const lbmibm_ctx *ctx) { return ctx ? ctx->tune_report.c_str() : ""; }
+
 void lbmibm_sizes(const lbmibm_ctx *ctx, long long *n_padded, int *nv, int *nt, int *nc) {
     if (!ctx) return;
     if (n_padded) *n_padded = ctx->np;
@@ -457,6 +459,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         CK(lattice_fill(ctx, ctx->rhom, (size_t)N, 1.0, false));
     }
     CK(set_step(ctx, 0, false));
+    CK(membrane_autotune(ctx));
     CK(reset_status(ctx));
     CK(sync_check(ctx));
 #undef CK

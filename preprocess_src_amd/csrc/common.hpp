@@ -80,6 +80,8 @@ struct lbmibm_ctx {
     // with three vertices per thread, 2 vectors in global memory
     int *bicg_cells[3] = {nullptr, nullptr, nullptr};
     int bicg_n[3] = {0, 0, 0}, bicg_maxv[3] = {0, 0, 0};
+    int bicg_var[2] = {0, 0};  // shape variant of the on-chip classes (autotuned)
+    std::string tune_report;
     // volume-correction work-group classes: up to VOL_SMALL vertices, larger
     int *vol_cells[2] = {nullptr, nullptr};
     int vol_n[2] = {0, 0};
@@ -130,6 +132,7 @@ lbmibm_status ibm_setup(lbmibm_ctx *ctx);                    // ibm.hip
 lbmibm_status membrane_forces(lbmibm_ctx *ctx);              // membrane.hip
 int bicg_class(int prec, int nvert);                         // membrane.hip
 lbmibm_status membrane_setup(lbmibm_ctx *ctx);               // membrane.hip
+lbmibm_status membrane_autotune(lbmibm_ctx *ctx);            // membrane.hip
 lbmibm_status volume_run(lbmibm_ctx *ctx, int mode);         // volume.hip (0 init, 1 correct)
 lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool advance);  // api.hip
 

@@ -220,9 +220,13 @@ template <typename T> __device__ __forceinline__ void lds_store3(T *a, int j, co
     a[4 * j] = v[0]; a[4 * j + 1] = v[1]; a[4 * j + 2] = v[2];
 }
 
-// Work-group shapes of the two on-chip classes (vertices per thread, threads).
-constexpr int BICG0_K = 4, BICG0_NT = 256;
-constexpr int BICG1_K = 6, BICG1_NT = 512;
+// Work-group shapes (vertices per thread K, threads NT) of the two on-chip
+// classes.  Each class has candidate shapes of the same capacity K*NT; the
+// one used is picked by timing them on the cells at set-up (autotune).
+constexpr int BICG0_CAP = 1024, BICG1_CAP = 3072;
+constexpr int BICG_NVAR = 3;
+constexpr int BICG0_SHAPES[BICG_NVAR][2] = {{4, 256}, {2, 512}, {1, 1024}};
+constexpr int BICG1_SHAPES[BICG_NVAR][2] = {{6, 512}, {3, 1024}, {4, 768}};
 
 // On-chip BiCGSTAB for one cell of at most NT*KMAX vertices: every thread
 // keeps the rows of its vertices (x, r, p, v, t and the 6 off-diagonal
@@ -611,13 +615,65 @@ __global__ __launch_bounds__(256) void k_bend(const double *__restrict__ xs, con
     }
 }
 
-template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
+// One on-chip class with the given shape variant.
+template <typename T>
+lbmibm_status launch_onchip(lbmibm_ctx *ctx, int cls, int var, const SolveOut<T> &out, hipStream_t stream) {
     const lbmibm_params &p = ctx->p;
-    PeriodicBox box{ctx->nx * p.dx, ctx->ny * p.dx, ctx->nz * p.dx, p.periodic_y};
-    TensionArgs ta;
-    ta.c = p.c; ta.rgs = p.rgs; ta.neo = p.neohookean; ta.m = p.m;
-    ta.fileoutput = p.fileoutput > 0 ? p.fileoutput : 1;
-    ta.wtp = (double)p.m / (double)ta.fileoutput;
+    const size_t lds = 3 * 4 * (size_t)ctx->bicg_maxv[cls] * sizeof(T);
+    const int n = ctx->bicg_n[cls];
+#define LBMIBM_ONCHIP(TT, K, NT)                                                                                \
+    hipLaunchKernelGGL((k_bicg_onchip<TT, K, NT>), dim3(n), dim3(NT), lds, stream, ctx->bicg_cells[cls],          \
+                       ctx->voff, (const TT *)ctx->coef, ctx->nbr, (const TT *)ctx->bv,                           \
+                       *reinterpret_cast<const SolveOut<TT> *>(&out), ctx->nv, p.itrmax, p.residual,             \
+                       ctx->bicg_iters, ctx->status)
+    if (cls == 0) {
+        switch (var) {
+            case 0: LBMIBM_ONCHIP(T, BICG0_SHAPES[0][0], BICG0_SHAPES[0][1]); break;
+            case 1: LBMIBM_ONCHIP(T, BICG0_SHAPES[1][0], BICG0_SHAPES[1][1]); break;
+            default: LBMIBM_ONCHIP(T, BICG0_SHAPES[2][0], BICG0_SHAPES[2][1]); break;
+        }
+    } else {
+        if constexpr (sizeof(T) == 4) {  // class 1 is FP32 only (bicg_class)
+            switch (var) {
+                case 0: LBMIBM_ONCHIP(float, BICG1_SHAPES[0][0], BICG1_SHAPES[0][1]); break;
+                case 1: LBMIBM_ONCHIP(float, BICG1_SHAPES[1][0], BICG1_SHAPES[1][1]); break;
+                default: LBMIBM_ONCHIP(float, BICG1_SHAPES[2][0], BICG1_SHAPES[2][1]); break;
+            }
+        } else {
+            return LBMIBM_ERR_STATE;
+        }
+    }
+#undef LBMIBM_ONCHIP
+    LBMIBM_LAUNCHED(ctx, "k_bicg_onchip");
+    return LBMIBM_OK;
+}
+
+template <typename T> SolveOut<T> solve_out(lbmibm_ctx *ctx) {
+    SolveOut<T> out;
+    out.q = (T *)ctx->qv;
+    out.cf = (T *)ctx->cf;
+    out.qi = (T *)ctx->qi;
+    out.ds = (const T *)ctx->dsv;
+    out.st = ctx->step;
+    out.nv = ctx->nv;
+    out.fo = ctx->p.fileoutput > 0 ? ctx->p.fileoutput : 0;
+    out.wq = ctx->p.fileoutput > 0 ? (double)ctx->p.m / (double)ctx->p.fileoutput : 0.0;
+    return out;
+}
+
+// Pseudo-random right-hand side in [-1, 1) for the autotune solves.
+template <typename T> __global__ void k_synth_rhs(T *__restrict__ b, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    b[i] = (T)((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
+}
+
+template <typename T> lbmibm_status assemble_impl(lbmibm_ctx *ctx, const PeriodicBox &box, const TensionArgs &ta) {
+    const lbmibm_params &p = ctx->p;
     const int ne = ctx->nt, nv = ctx->nv;
     hipLaunchKernelGGL(k_elem<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
                        ctx->ele, (const double *)ctx->nref, (T *)ctx->tt, (T *)ctx->rg,
@@ -628,15 +684,72 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
                        ctx->epair, (const typename Corner<T>::type *)ctx->corner, ctx->vcell,
                        (const T *)ctx->gscale, (T *)ctx->coef, (T *)ctx->bv, (T *)ctx->dsv, nv);
     LBMIBM_LAUNCHED(ctx, "k_vert");
-    SolveOut<T> out;
-    out.q = (T *)ctx->qv;
-    out.cf = (T *)ctx->cf;
-    out.qi = (T *)ctx->qi;
-    out.ds = (const T *)ctx->dsv;
-    out.st = ctx->step;
-    out.nv = nv;
-    out.fo = p.fileoutput > 0 ? p.fileoutput : 0;
-    out.wq = p.fileoutput > 0 ? (double)p.m / (double)p.fileoutput : 0.0;
+    return LBMIBM_OK;
+}
+
+PeriodicBox box_of(const lbmibm_ctx *ctx) {
+    return PeriodicBox{ctx->nx * ctx->p.dx, ctx->ny * ctx->p.dx, ctx->nz * ctx->p.dx, ctx->p.periodic_y};
+}
+
+TensionArgs tension_args(const lbmibm_ctx *ctx) {
+    const lbmibm_params &p = ctx->p;
+    TensionArgs ta;
+    ta.c = p.c; ta.rgs = p.rgs; ta.neo = p.neohookean; ta.m = p.m;
+    ta.fileoutput = p.fileoutput > 0 ? p.fileoutput : 1;
+    ta.wtp = (double)p.m / (double)ta.fileoutput;
+    return ta;
+}
+
+// Times every shape variant of the on-chip classes on the cells' own mass
+// matrices (reference state) with a pseudo-random right-hand side and keeps
+// the fastest; the solves' outputs are discarded.
+template <typename T> lbmibm_status autotune_impl(lbmibm_ctx *ctx) {
+    lbmibm_status s = assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx));
+    if (s != LBMIBM_OK) return s;
+    const size_t n3 = 3 * (size_t)ctx->nv;
+    hipLaunchKernelGGL(k_synth_rhs<T>, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, ctx->stream, (T *)ctx->bv,
+                       n3);
+    LBMIBM_LAUNCHED(ctx, "k_synth_rhs");
+    const SolveOut<T> out = solve_out<T>(ctx);
+    hipEvent_t e0, e1;
+    LBMIBM_HIP(ctx, hipEventCreate(&e0));
+    LBMIBM_HIP(ctx, hipEventCreate(&e1));
+    char line[160];
+    for (int cls = 0; cls < 2; cls++) {
+        if (ctx->bicg_n[cls] == 0 || (cls == 1 && sizeof(T) != 4)) continue;
+        float best = 0.0f;
+        for (int v = 0; v < BICG_NVAR; v++) {
+            s = launch_onchip<T>(ctx, cls, v, out, ctx->stream);  // warm-up
+            if (s != LBMIBM_OK) break;
+            LBMIBM_HIP(ctx, hipEventRecord(e0, ctx->stream));
+            for (int r = 0; r < 2 && s == LBMIBM_OK; r++) s = launch_onchip<T>(ctx, cls, v, out, ctx->stream);
+            LBMIBM_HIP(ctx, hipEventRecord(e1, ctx->stream));
+            LBMIBM_HIP(ctx, hipEventSynchronize(e1));
+            float ms = 0.0f;
+            LBMIBM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+            ms *= 0.5f;
+            const int *shape = cls == 0 ? BICG0_SHAPES[v] : BICG1_SHAPES[v];
+            snprintf(line, sizeof line, "bicg class %d shape %dx%d: %.4f ms\n", cls, shape[0], shape[1], ms);
+            ctx->tune_report += line;
+            if (v == 0 || ms < best) { best = ms; ctx->bicg_var[cls] = v; }
+        }
+        if (s != LBMIBM_OK) break;
+        const int *shape = cls == 0 ? BICG0_SHAPES[ctx->bicg_var[cls]] : BICG1_SHAPES[ctx->bicg_var[cls]];
+        snprintf(line, sizeof line, "bicg class %d: using %dx%d\n", cls, shape[0], shape[1]);
+        ctx->tune_report += line;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return s;
+}
+
+template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
+    const lbmibm_params &p = ctx->p;
+    const PeriodicBox box = box_of(ctx);
+    const int ne = ctx->nt, nv = ctx->nv;
+    lbmibm_status st = assemble_impl<T>(ctx, box, tension_args(ctx));
+    if (st != LBMIBM_OK) return st;
+    const SolveOut<T> out = solve_out<T>(ctx);
     if (ctx->bicg_n[2] > 0) {
         // the large cells (one work group each, vectors in global memory) run
         // on the side stream, concurrently with the on-chip classes; x in its
@@ -649,22 +762,11 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
                            ctx->bicg_iters, ctx->status);
         LBMIBM_LAUNCHED(ctx, "k_bicg_global");
     }
-    if (ctx->bicg_n[0] > 0) {
-        const size_t lds = 3 * 4 * (size_t)ctx->bicg_maxv[0] * sizeof(T);
-        hipLaunchKernelGGL((k_bicg_onchip<T, BICG0_K, BICG0_NT>), dim3(ctx->bicg_n[0]), dim3(BICG0_NT), lds,
-                           ctx->stream, ctx->bicg_cells[0], ctx->voff, (const T *)ctx->coef, ctx->nbr,
-                           (const T *)ctx->bv, out, nv, p.itrmax, p.residual, ctx->bicg_iters, ctx->status);
-        LBMIBM_LAUNCHED(ctx, "k_bicg_onchip (class 0)");
-    }
-    if constexpr (sizeof(T) == 4) {  // class 1 is FP32 only (bicg_class)
-        if (ctx->bicg_n[1] > 0) {
-            const size_t lds = 3 * 4 * (size_t)ctx->bicg_maxv[1] * sizeof(float);
-            hipLaunchKernelGGL((k_bicg_onchip<float, BICG1_K, BICG1_NT>), dim3(ctx->bicg_n[1]), dim3(BICG1_NT), lds,
-                               ctx->stream, ctx->bicg_cells[1], ctx->voff, (const float *)ctx->coef, ctx->nbr,
-                               (const float *)ctx->bv, out, nv, p.itrmax, p.residual, ctx->bicg_iters, ctx->status);
-            LBMIBM_LAUNCHED(ctx, "k_bicg_onchip (class 1)");
+    for (int cls = 0; cls < 2; cls++)
+        if (ctx->bicg_n[cls] > 0) {
+            lbmibm_status s = launch_onchip<T>(ctx, cls, ctx->bicg_var[cls], out, ctx->stream);
+            if (s != LBMIBM_OK) return s;
         }
-    }
     if (ctx->bicg_n[2] > 0) {
         LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_join, ctx->side));
         LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
@@ -681,32 +783,67 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
 }  // namespace
 
 int bicg_class(int prec, int nvert) {
-    // on chip: up to 1024 vertices with 4 per thread of a 256-thread work
-    // group (any precision), up to 3072 with 6 per thread of a 512-thread
-    // work group (FP32: the FP64 rows do not fit the register budget); else
-    // vectors in global memory
-    if (nvert <= BICG0_K * BICG0_NT) return 0;
-    if (nvert <= BICG1_K * BICG1_NT && prec != LBMIBM_F64) return 1;
+    // on chip: up to 1024 vertices (any precision) and up to 3072 (FP32: the
+    // FP64 rows do not fit the register budget); else vectors in global memory
+    if (nvert <= BICG0_CAP) return 0;
+    if (nvert <= BICG1_CAP && prec != LBMIBM_F64) return 1;
     return 2;
 }
 
 lbmibm_status membrane_setup(lbmibm_ctx *ctx) {
-    // allow the on-chip solvers the LDS their largest cells need (p, r, b as [V][4])
+    // allow every on-chip shape the LDS its class's largest cell needs (p, r, b as [V][4])
+    static_assert(BICG0_SHAPES[0][0] * BICG0_SHAPES[0][1] == BICG0_CAP && BICG0_SHAPES[1][0] * BICG0_SHAPES[1][1] ==
+                      BICG0_CAP && BICG0_SHAPES[2][0] * BICG0_SHAPES[2][1] == BICG0_CAP, "class 0 shapes");
+    static_assert(BICG1_SHAPES[0][0] * BICG1_SHAPES[0][1] == BICG1_CAP && BICG1_SHAPES[1][0] * BICG1_SHAPES[1][1] ==
+                      BICG1_CAP && BICG1_SHAPES[2][0] * BICG1_SHAPES[2][1] == BICG1_CAP, "class 1 shapes");
+    const hipFuncAttribute A = hipFuncAttributeMaxDynamicSharedMemorySize;
     if (ctx->bicg_n[0] > 0) {
         const int lds = (int)(3 * 4 * (size_t)ctx->bicg_maxv[0] * ctx->szT);
-        if (ctx->prec == LBMIBM_F64)
-            LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<double, BICG0_K, BICG0_NT>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-        else
-            LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, BICG0_K, BICG0_NT>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        if (ctx->prec == LBMIBM_F64) {
+            LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<double, 4, 256>, A, lds));
+            LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<double, 2, 512>, A, lds));
+            LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<double, 1, 1024>, A, lds));
+        } else {
+            LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 4, 256>, A, lds));
+            LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 2, 512>, A, lds));
+            LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 1, 1024>, A, lds));
+        }
     }
     if (ctx->bicg_n[1] > 0) {
         const int lds = (int)(3 * 4 * (size_t)ctx->bicg_maxv[1] * sizeof(float));
-        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, BICG1_K, BICG1_NT>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 6, 512>, A, lds));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 3, 1024>, A, lds));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 4, 768>, A, lds));
     }
     return LBMIBM_OK;
+}
+
+lbmibm_status membrane_autotune(lbmibm_ctx *ctx) {
+    ctx->bicg_var[0] = ctx->bicg_var[1] = 0;
+    ctx->tune_report.clear();
+    if (ctx->nc == 0) return LBMIBM_OK;
+    const char *env = getenv("LBMIBM_AUTOTUNE");
+    if (env && env[0] == '0') {
+        ctx->tune_report = "autotune off (LBMIBM_AUTOTUNE=0): first shapes\n";
+        return LBMIBM_OK;
+    }
+    // LBMIBM_BICG_SHAPES="a,b" fixes the shape variants of classes 0 and 1
+    const char *fixed = getenv("LBMIBM_BICG_SHAPES");
+    if (fixed && fixed[0]) {
+        int a = 0, b = 0;
+        if (sscanf(fixed, "%d,%d", &a, &b) != 2 || a < 0 || a >= BICG_NVAR || b < 0 || b >= BICG_NVAR)
+            return LBMIBM_ERR_ARG;
+        ctx->bicg_var[0] = a;
+        ctx->bicg_var[1] = b;
+        ctx->tune_report = "shapes fixed by LBMIBM_BICG_SHAPES\n";
+        return LBMIBM_OK;
+    }
+    lbmibm_status s = ctx->prec == LBMIBM_F64 ? autotune_impl<double>(ctx) : autotune_impl<float>(ctx);
+    if (s != LBMIBM_OK) return s;
+    // discard what the timing solves left in the output accumulators
+    s = lattice_fill(ctx, ctx->qi, 3 * (size_t)ctx->nv, 0.0, false);
+    if (s == LBMIBM_OK) s = lattice_fill(ctx, ctx->tp, 2 * (size_t)ctx->nt, 0.0, false);
+    return s;
 }
 
 lbmibm_status membrane_forces(lbmibm_ctx *ctx) {

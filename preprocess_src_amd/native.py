@@ -56,6 +56,7 @@ class Cells(C.Structure):
 
 # name -> (restype, argtypes); every declaration of include/lbmibm.h
 SIGNATURES = {
+    "lbmibm_autotune_report": (C.c_char_p, [C.c_void_p]),
     "lbmibm_default_params": (None, [C.POINTER(Params), C.c_int, C.c_int, C.c_int]),
     "lbmibm_create": (C.c_int, [C.POINTER(Params), C.POINTER(Cells), C.POINTER(C.c_void_p)]),
     "lbmibm_destroy": (None, [C.c_void_p]),

@@ -111,6 +111,10 @@ class Simulation:
 
     STAGES = ("stress", "ibm_st1", "lbm_d3q19", "ibm_st2", "volume_correction", "step")
 
+    def autotune_report(self):
+        """Timings and choices of the solver-shape autotuning done at creation."""
+        return self.L.lbmibm_autotune_report(self.ctx).decode()
+
     def time_stages(self, iter0, nsteps):
         """Average ms per call of each stage, timed with HIP events on the library stream."""
         ms = np.zeros(6)

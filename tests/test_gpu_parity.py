@@ -319,3 +319,34 @@ def test_spread_is_bitwise_reproducible():
     F2 = g.lattice_force()
     for a, b in zip(F1, F2):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_every_solver_shape_matches_the_oracle(variant, monkeypatch):
+    """Each candidate work-group shape of the on-chip solver classes (the
+    autotuner picks one) gives the oracle's loads and forces."""
+    monkeypatch.setenv("LBMIBM_BICG_SHAPES", f"{variant},{variant}")
+    lx, ly, lz = 24, 20, 24
+    cells = small_cells(lx, ly, lz, nc=3, radius=4.0, levels=[3, 4, 2], seed=61)
+    kw = dict(c=100.0, br=0.0)
+    o = oracle_of(lx, ly, lz, cells=cells, **kw)
+    g = gpu_of(lx, ly, lz, "f32", cells=cells, **kw)
+    assert "LBMIBM_BICG_SHAPES" in g.autotune_report()
+    d = perturb_cells(cells, seed=62, amp=0.05)
+    xd = np.stack([d["x"], d["y"], d["z"]], 1).reshape(-1).copy()
+    o.cells["xd"][:] = xd
+    g.set_cells_state(xd=xd)
+    assert o.stress(1) == 0
+    g.stress(1)
+    assert rel(g.membrane()["q"], o.cells["q"]) < 1e-4
+    s = g.cells_state()
+    for a, b in (("fx", "cfx"), ("fy", "cfy"), ("fz", "cfz")):
+        assert rel(s[a], o.cells[b]) < 1e-4, a
+
+
+def test_autotune_reports_a_choice_per_class():
+    lx, ly, lz = 24, 20, 24
+    cells = small_cells(lx, ly, lz, nc=2, radius=4.0, levels=[3, 4], seed=63)
+    g = gpu_of(lx, ly, lz, "f32", cells=cells)
+    rep = g.autotune_report()
+    assert "class 0: using" in rep and "class 1: using" in rep, rep
