@@ -69,6 +69,20 @@ def lattice_bytes_per_node(precision, coupled):
     return 2 * Q * s + (3 * t * 2 if coupled else 3 * t)  # populations + (F read, u write) or u write
 
 
+def lbm_traffic_from_profile(precision):
+    """HBM bytes per k_lbm launch measured by rocprofv3 FETCH_SIZE (x2.0, the
+    copy-kernel calibration) and WRITE_SIZE on this workload, from the summary
+    committed in profiles/ (FP32 only; None when absent)."""
+    path = os.path.join(ROOT, "profiles", "suspension_256x128x128_f32_hbm_traffic.txt")
+    if precision != "f32" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("k_lbm"):
+            parts = line.split()
+            return round((float(parts[-3]) + float(parts[-1])) * 1e6)  # read MB (x2) + write MB
+    return None
+
+
 def cpu_baseline(shape, cells, params, seconds):
     """FP64 host oracle on the same workload for a bounded number of steps."""
     from oracle import pyoracle
@@ -175,7 +189,8 @@ def main():
                    "params": {k: (list(v) if isinstance(v, tuple) else v) for k, v in params.items()}},
         "roofline": {"bound": "hbm", "kernel": "k_lbm (fused pull stream-collide)",
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": lbm_traffic_from_profile(a.precision) if a.workload == "suspension" else None,
                      "bytes_per_node": bpn, "avg_ms": round(stages["lbm_d3q19"][0], 5)},
         "step_roofline": {"algorithmic_bytes": step_bytes, "frac": round(step_bytes / (ms_per_step * 1e-3) / HBM_PEAK, 4)},
         "stages_ms": {k: round(v[0], 5) for k, v in stages.items()},

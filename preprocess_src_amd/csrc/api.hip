@@ -172,7 +172,7 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
                     ctx->tile_off, ctx->tile_cur, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
                     ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv,
-                    ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->lec, ctx->epair};
+                    ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair, ctx->srec};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -355,14 +355,14 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         // two incident elements that share the edge
         {
             std::vector<unsigned short> nbr16(6 * (size_t)nv);
-            std::vector<int> lec(6 * (size_t)nv, 0);
+            std::vector<int> cslot(3 * (size_t)nt, 0);
             std::vector<unsigned char> epair(6 * (size_t)nv, 0xff);
             for (int j = 0; j < nv; j++) {
                 const int n = nln[j], base = cells->voff[vcell[j]];
                 for (int sl = 0; sl < n; sl++) {
                     const int e = le[6 * (size_t)j + sl];
                     const int *ev = cells->ele + 3 * (size_t)e;
-                    lec[6 * (size_t)j + sl] = 3 * e + (ev[0] == j ? 0 : (ev[1] == j ? 1 : 2));
+                    cslot[3 * (size_t)e + (ev[0] == j ? 0 : (ev[1] == j ? 1 : 2))] = 6 * j + sl;
                 }
                 int m = 0;
                 for (int k = 0; k <= n && k < 7; k++) {
@@ -386,10 +386,10 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
             CK(dev_alloc(ctx, (void **)&ctx->nbr, 6 * (size_t)nv * sizeof(unsigned short)));
             LBMIBM_HIP(ctx, hipMemcpy(ctx->nbr, nbr16.data(), nbr16.size() * sizeof(unsigned short),
                                       hipMemcpyHostToDevice));
-            CK(up_i(&ctx->lec, lec.data(), lec.size()));
+            CK(up_i(&ctx->cslot, cslot.data(), cslot.size()));
             CK(dev_alloc(ctx, (void **)&ctx->epair, epair.size()));
             LBMIBM_HIP(ctx, hipMemcpy(ctx->epair, epair.data(), epair.size(), hipMemcpyHostToDevice));
-            CK(dev_alloc(ctx, &ctx->corner, 3 * (size_t)nt * 4 * szT));
+            CK(dev_alloc(ctx, &ctx->corner, 6 * (size_t)nv * 4 * szT));
         }
         // cells solved on chip and the rest
         std::vector<int> cls[3];
@@ -421,6 +421,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         ctx->nblk_bin = (nv + 255) / 256;
         if (ctx->nx > 30000 || ctx->ny > 29990 || ctx->nz > 30000) return fail(LBMIBM_ERR_ARG);  // short4 bins
         CK(dev_alloc(ctx, (void **)&ctx->node0, (size_t)nv * sizeof(short4)));
+        CK(dev_alloc(ctx, &ctx->srec, (size_t)nv * 6 * szT));
         CK(dev_alloc(ctx, (void **)&ctx->home, (size_t)nv * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->home_list, (size_t)nv * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->tile_count, (size_t)ctx->ntiles * sizeof(int)));

@@ -88,8 +88,8 @@ struct lbmibm_ctx {
     void *bv = nullptr, *rv = nullptr, *rh = nullptr, *pv = nullptr, *vv = nullptr, *tv = nullptr;
     void *gscale = nullptr;                            // [nc] (T)
     void *dsv = nullptr;                               // [Nv] ring area x force scale (T)
-    void *corner = nullptr;                            // [3Nt] per-corner rhs terms and rg (T4)
-    int *lec = nullptr;                                // [6Nv] corner of vertex j in its s-th element
+    void *corner = nullptr;                            // [6Nv] per-(vertex, slot) rhs terms and rg (T4)
+    int *cslot = nullptr;                              // [3Nt] slot 6j+s of each element corner
     unsigned char *epair = nullptr;                    // [6Nv] slots sharing the edge to column m
     double *normal = nullptr;                          // [3Nt] FP64 element normals
     double *vol0 = nullptr, *vol = nullptr, *area = nullptr, *gx = nullptr;  // [nc], [3nc]
@@ -99,7 +99,8 @@ struct lbmibm_ctx {
     // immersed boundary: vertices binned by the lattice tile of their first
     // stencil node (rebuilt whenever the positions change)
     int ntx = 0, nty = 0, ntz = 0, ntiles = 0, nblk_bin = 0;
-    short4 *node0 = nullptr;                 // [Nv] first stencil node (x,z wrapped, y+3)
+    short4 *node0 = nullptr;                 // [Nv] first stencil node (x,z wrapped, y+3), bin order
+    void *srec = nullptr;                    // [Nv] stencil fractions and force (6 T), bin order
     int *home = nullptr, *home_list = nullptr;  // [Nv]
     int *tile_count = nullptr, *tile_off = nullptr, *tile_cur = nullptr;  // [ntiles]
     int *tile_cand = nullptr;                          // [ntiles] spreading candidates

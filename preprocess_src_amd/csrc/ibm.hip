@@ -98,6 +98,25 @@ __device__ __forceinline__ void stencil(double xtr, double h, int &first, W w[4]
     w[3] = W(1) - cs;
 }
 
+// First stencil node (padded frame, as stencil()) and fraction t = frac(xtr/h).
+template <typename W>
+__device__ __forceinline__ void stencil_first(double xtr, double h, int &first, W &t) {
+    const double q = xtr / h;
+    const double fl = floor(q);
+    first = (int)fl - 1;
+    t = (W)(q - fl);
+}
+
+// Weights of the 4 nodes from the fraction t (as stencil()).
+template <typename W> __device__ __forceinline__ void stencil_weights(W t, W w[4]) {
+    W sn, cs;
+    sincos_quarter(t, &sn, &cs);
+    w[0] = W(1) - sn;
+    w[1] = W(1) + cs;
+    w[2] = W(1) + sn;
+    w[3] = W(1) - cs;
+}
+
 // Unpadded first stencil nodes (unwrapped) and weights at a position.
 template <typename W>
 __device__ __forceinline__ void stencil3(const IbmGeom &g, double x, double y, double z, bool unit, int &x1, int &y1,
@@ -175,9 +194,8 @@ __device__ __forceinline__ int wave_aggregated_add(int *counters, int key, bool 
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs, const T *__restrict__ cf, int nv,
-                                                   IbmGeom g, TileGeom tg, short4 *__restrict__ node0,
-                                                   int *__restrict__ home, int *__restrict__ tile_count,
-                                                   float *__restrict__ blk_fmax) {
+                                                   IbmGeom g, TileGeom tg, int *__restrict__ home,
+                                                   int *__restrict__ tile_count, float *__restrict__ blk_fmax) {
     __shared__ float shm[4];
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = v < nv;
@@ -198,7 +216,6 @@ __global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs
         zw = min(max(zw, 0), g.nz - 1);
         yh = min(max(yh, 0), g.ny - 1);
         t = xw / TILE_X + tg.ntx * (yh / TILE_Y + tg.nty * (zw / TILE_Z));
-        node0[v] = make_short4((short)xw, (short)min(max(y1 + 3, -30000), 30000), (short)zw, 0);
         home[v] = t;
         fm = fmaxf(fabsf((float)cf[v]), fmaxf(fabsf((float)cf[nv + v]), fabsf((float)cf[2 * (size_t)nv + v])));
         fm *= 1.0001f;  // margin for the float rounding of |f|
@@ -313,14 +330,40 @@ __global__ __launch_bounds__(1024) void k_tile_order(const int *__restrict__ til
     heavy_first_order(tile_count, ntiles, sh, order_interp);
 }
 
-__global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ home, int nv,
+// Spreading data of a vertex, stored in bin order next to its first stencil
+// node (node0: x and z wrapped, y + 3): the stencil fractions and the force.
+template <typename T> struct SpreadRec {
+    T t[3];
+    T f[3];
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_scatter(const double *__restrict__ xs, const T *__restrict__ cf,
+                                                     const int *__restrict__ home, int nv, IbmGeom g,
                                                      const int *__restrict__ tile_off, int *__restrict__ tile_cur,
-                                                     int *__restrict__ home_list) {
+                                                     int *__restrict__ home_list, short4 *__restrict__ node0,
+                                                     SpreadRec<T> *__restrict__ rec) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = v < nv;
     const int t = on ? home[v] : 0;
     const int r = wave_aggregated_add(tile_cur, t, on);
-    if (on) home_list[tile_off[t] + r] = v;
+    if (!on) return;
+    const int pos = tile_off[t] + r;
+    home_list[pos] = v;
+    int x1, y1, z1;
+    SpreadRec<T> sr;
+    stencil_first<T>(xs[v] + (double)(g.nxp - 1) * g.dx / 2.0, g.dx, x1, sr.t[0]);
+    stencil_first<T>(xs[nv + v] + (double)(g.nyp - 1) * g.dx / 2.0, g.dx, y1, sr.t[1]);
+    stencil_first<T>(xs[2 * (size_t)nv + v] + (double)(g.nzp - 1) * g.dx / 2.0, g.dx, z1, sr.t[2]);
+    x1 -= 2; y1 -= 2; z1 -= 2;  // unpadded
+    int xw = x1, zw = z1;
+    wrap_index(xw, g.nx);
+    wrap_index(zw, g.nz);
+    xw = min(max(xw, -30000), 30000);
+    zw = min(max(zw, -30000), 30000);
+    node0[pos] = make_short4((short)xw, (short)min(max(y1 + 3, -30000), 30000), (short)zw, 0);
+    sr.f[0] = cf[v]; sr.f[1] = cf[nv + v]; sr.f[2] = cf[2 * (size_t)nv + v];
+    rec[pos] = sr;
 }
 
 __global__ void k_zero_int(int *__restrict__ p, int n) {
@@ -371,17 +414,19 @@ template <> struct FixedPoint<double> {
 };
 
 template <typename T>
-__device__ __forceinline__ void spread_vertex(const double *__restrict__ xs, const T *__restrict__ cf, int nv, int v,
-                                              const IbmGeom &g, int ox, int oy, int oz, int lx, int ly, int lz,
-                                              double scale, T w64, unsigned long long *acc) {
-    int x1, y1, z1;
-    T wx[4], wy[4], wz[4];
-    stencil3<T>(g, xs[v], xs[nv + v], xs[2 * (size_t)nv + v], false, x1, y1, z1, wx, wy, wz);
+__device__ __forceinline__ void spread_vertex(const SpreadRec<T> &sr, short4 n0, const IbmGeom &g, int ox, int oy,
+                                              int oz, int lx, int ly, int lz, double scale, T w64,
+                                              unsigned long long *acc) {
+    const int x1 = n0.x, y1 = n0.y - 3, z1 = n0.z;
     const unsigned mx = axis_mask(x1, ox, lx, g.nx, true);
     const unsigned my = axis_mask(y1, oy, ly, g.ny, !g.walls);
     const unsigned mz = axis_mask(z1, oz, lz, g.nz, true);
     if (!mx || !my || !mz) return;
-    const FixedPoint<T> fp(cf[v], cf[nv + v], cf[2 * (size_t)nv + v], scale);
+    T wx[4], wy[4], wz[4];
+    stencil_weights<T>(sr.t[0], wx);
+    stencil_weights<T>(sr.t[1], wy);
+    stencil_weights<T>(sr.t[2], wz);
+    const FixedPoint<T> fp(sr.f[0], sr.f[1], sr.f[2], scale);
     // tile-relative offsets of the stencil nodes on each axis
     int ax[4], by[4], cz[4];
 #pragma unroll
@@ -412,11 +457,10 @@ __device__ __forceinline__ void spread_vertex(const double *__restrict__ xs, con
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__ xs, const T *__restrict__ cf,
+__global__ __launch_bounds__(256) void k_spread_tiles(const SpreadRec<T> *__restrict__ rec,
                                                       const short4 *__restrict__ node0,
                                                       const int *__restrict__ tile_count,
-                                                      const int *__restrict__ tile_off,
-                                                      const int *__restrict__ home_list, T *__restrict__ F, int nv,
+                                                      const int *__restrict__ tile_off, T *__restrict__ F,
                                                       IbmGeom g, TileGeom tg, const unsigned *__restrict__ absmax,
                                                       const int *__restrict__ order) {
     __shared__ unsigned long long acc[3 * ACC_N];
@@ -460,10 +504,10 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
         const int R = (cnt + 63) >> 6;
         for (int r = wave; r < R; r += 4) {
             const int i = lane * R + r;
-            int v = -1;
+            int v = -1;  // position in bin order
             bool touch = false;
             if (i < cnt) {
-                v = home_list[off + i];
+                v = off + i;
                 const short4 n0 = node0[v];
                 touch = axis_mask(n0.x, ox, lx, g.nx, true) && axis_mask(n0.y - 3, oy, ly, g.ny, !g.walls) &&
                         axis_mask(n0.z, oz, lz, g.nz, true);
@@ -475,7 +519,7 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (qn >= 64) {
-                spread_vertex<T>(xs, cf, nv, q[lane], g, ox, oy, oz, lx, ly, lz, scale, w64, acc);
+                spread_vertex<T>(rec[q[lane]], node0[q[lane]], g, ox, oy, oz, lx, ly, lz, scale, w64, acc);
                 const int rest = qn - 64;
                 const int mv = lane < rest ? q[64 + lane] : 0;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -489,7 +533,7 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const double *__restrict__
             }
         }
     }
-    if (lane < qn) spread_vertex<T>(xs, cf, nv, q[lane], g, ox, oy, oz, lx, ly, lz, scale, w64, acc);
+    if (lane < qn) spread_vertex<T>(rec[q[lane]], node0[q[lane]], g, ox, oy, oz, lx, ly, lz, scale, w64, acc);
     __syncthreads();
     const double inv_scale = 1.0 / scale;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
@@ -679,7 +723,7 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
                        ctx->ntiles);
     LBMIBM_LAUNCHED(ctx, "k_zero_int");
     hipLaunchKernelGGL(k_bin_count<T>, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
-                       (const T *)ctx->cf, nv, g, tg, ctx->node0, ctx->home, ctx->tile_count, ctx->blk_fmax);
+                       (const T *)ctx->cf, nv, g, tg, ctx->home, ctx->tile_count, ctx->blk_fmax);
     LBMIBM_LAUNCHED(ctx, "k_bin_count");
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->tile_count, ctx->tile_off,
                        ctx->tile_cur, ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax);
@@ -687,8 +731,9 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, ctx->stream, ctx->tile_count, ctx->ntiles, g, tg,
                        ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->step);
     LBMIBM_LAUNCHED(ctx, "k_tile_order");
-    hipLaunchKernelGGL(k_bin_scatter, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, ctx->home, nv, ctx->tile_off,
-                       ctx->tile_cur, ctx->home_list);
+    hipLaunchKernelGGL(k_bin_scatter<T>, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
+                       (const T *)ctx->cf, ctx->home, nv, g, ctx->tile_off, ctx->tile_cur, ctx->home_list,
+                       ctx->node0, (SpreadRec<T> *)ctx->srec);
     LBMIBM_LAUNCHED(ctx, "k_bin_scatter");
     ctx->bins_valid = true;
     return LBMIBM_OK;
@@ -700,9 +745,9 @@ template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
     lbmibm_status s = bin_impl<T>(ctx);
     if (s != LBMIBM_OK) return s;
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
-    hipLaunchKernelGGL(k_spread_tiles<T>, dim3(ctx->ntiles), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
-                       (const T *)ctx->cf, ctx->node0, ctx->tile_count, ctx->tile_off, ctx->home_list, (T *)ctx->F,
-                       ctx->nv, geom_of(ctx), tg, ctx->absmax, ctx->order_spread);
+    hipLaunchKernelGGL(k_spread_tiles<T>, dim3(ctx->ntiles), dim3(256), 0, ctx->stream,
+                       (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off, (T *)ctx->F,
+                       geom_of(ctx), tg, ctx->absmax, ctx->order_spread);
     LBMIBM_LAUNCHED(ctx, "k_spread_tiles");
     return LBMIBM_OK;
 }

@@ -57,10 +57,13 @@ template <typename T> struct Corner;
 template <> struct Corner<float> { using type = float4; };
 template <> struct Corner<double> { using type = double4; };
 
+// The corner records go to the slot of the corner's vertex: cslot[3e+en] =
+// 6 j + s for vertex j = ele[3e+en] with e its s-th incident element, so the
+// per-vertex pass reads each vertex's records contiguously.
 template <typename T>
 __global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, const int *__restrict__ ele,
                                               const double *__restrict__ nref, T *__restrict__ tt,
-                                              T *__restrict__ rg, T *__restrict__ tp,
+                                              T *__restrict__ rg, T *__restrict__ tp, const int *__restrict__ cslot,
                                               typename Corner<T>::type *__restrict__ corner, int ne,
                                               PeriodicBox box, TensionArgs a, const DevStep *__restrict__ st) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -115,7 +118,7 @@ __global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, con
         }
         typename Corner<T>::type cv;
         cv.x = (T)tb[0]; cv.y = (T)tb[1]; cv.z = (T)tb[2]; cv.w = (T)gab;
-        corner[3 * (size_t)j + en] = cv;
+        corner[cslot[3 * (size_t)j + en]] = cv;
     }
     if (tp) {  // principal-tension accumulator for output (stress.cu:101-118)
         const int iter = st->iter;
@@ -134,13 +137,12 @@ __global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, con
 // right-hand side of each vertex from the corner terms of its incident
 // elements, summed in the reference's order (le list), plus the q2f ring area
 // (stress.cu:415-438: a third of the incident triangle areas, rg/6 each)
-// times the cell's force scale.  lec[6j+s] is the corner of vertex j in its
-// s-th incident element; epair[6j+m] holds the two slots s_A < s_B whose
-// elements share the edge to the m-th off-diagonal column (4 bits each,
+// times the cell's force scale.  corner[6j+s] holds the record of vertex j
+// in its s-th incident element; epair[6j+m] holds the two slots s_A < s_B
+// whose elements share the edge to the m-th off-diagonal column (4 bits each,
 // 0xff for an unused column).
 template <typename T>
-__global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const int *__restrict__ lec,
-                                              const unsigned char *__restrict__ epair,
+__global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const unsigned char *__restrict__ epair,
                                               const typename Corner<T>::type *__restrict__ corner,
                                               const int *__restrict__ vcell, const T *__restrict__ gscale,
                                               T *__restrict__ coef, T *__restrict__ bv, T *__restrict__ dsv,
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
     for (int s = 0; s < 6; s++) {
         rgs[s] = 0.0;
         if (s < n) {
-            const typename Corner<T>::type cv = corner[lec[6 * (size_t)jj + s]];
+            const typename Corner<T>::type cv = corner[6 * (size_t)jj + s];
             const double gab = (double)cv.w;
             rgs[s] = gab;
             jac += gab / 12.0;
@@ -677,11 +679,11 @@ template <typename T> lbmibm_status assemble_impl(lbmibm_ctx *ctx, const Periodi
     const int ne = ctx->nt, nv = ctx->nv;
     hipLaunchKernelGGL(k_elem<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
                        ctx->ele, (const double *)ctx->nref, (T *)ctx->tt, (T *)ctx->rg,
-                       p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, (typename Corner<T>::type *)ctx->corner, ne,
-                       box, ta, ctx->step);
+                       p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ctx->cslot,
+                       (typename Corner<T>::type *)ctx->corner, ne, box, ta, ctx->step);
     LBMIBM_LAUNCHED(ctx, "k_elem");
-    hipLaunchKernelGGL(k_vert<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln, ctx->lec,
-                       ctx->epair, (const typename Corner<T>::type *)ctx->corner, ctx->vcell,
+    hipLaunchKernelGGL(k_vert<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln, ctx->epair,
+                       (const typename Corner<T>::type *)ctx->corner, ctx->vcell,
                        (const T *)ctx->gscale, (T *)ctx->coef, (T *)ctx->bv, (T *)ctx->dsv, nv);
     LBMIBM_LAUNCHED(ctx, "k_vert");
     return LBMIBM_OK;
