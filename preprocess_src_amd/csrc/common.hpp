@@ -80,7 +80,7 @@ struct lbmibm_ctx {
     // with three vertices per thread, 2 vectors in global memory
     int *bicg_cells[3] = {nullptr, nullptr, nullptr};
     int bicg_n[3] = {0, 0, 0}, bicg_maxv[3] = {0, 0, 0};
-    int bicg_var[2] = {0, 0};  // shape variant of the on-chip classes (autotuned)
+    int bicg_var[3] = {0, 0, 0};  // shape variant of the on-chip classes, form of class 2 (autotuned)
     std::string tune_report;
     // volume-correction work-group classes: up to VOL_SMALL vertices, larger
     int *vol_cells[2] = {nullptr, nullptr};

@@ -389,6 +389,179 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
     }
 }
 
+// Large cells, second form: p lives only in LDS (SoA [3][V]), so the first
+// matrix-vector product gathers from LDS and p never travels to memory; r, v,
+// t, x in global memory.  Every phase walks the thread's vertices in batches
+// of GB with all loads of a batch issued before use.  For cells whose p fits
+// LDS (V <= BIG_LDS_V); the autotuner times it against k_bicg_global.
+constexpr int BIG_LDS_V = 13568;  // 3 * 4 B * V within 160 KB, FP32
+
+template <typename T, int GB>
+__global__ __launch_bounds__(1024) void k_bicg_big(const int *__restrict__ cells, const int *__restrict__ voff,
+                                                   const T *__restrict__ coef,
+                                                   const unsigned short *__restrict__ nbr,
+                                                   const T *__restrict__ bv, T *__restrict__ x, T *__restrict__ r,
+                                                   T *__restrict__ v, T *__restrict__ t, SolveOut<T> out, int nv,
+                                                   int itrmax, double residual, int *__restrict__ iters,
+                                                   DevStatus *__restrict__ status) {
+    extern __shared__ double dyn[];
+    __shared__ double sh[128];
+    WgReduce red{sh, 0};
+    const int c = cells[blockIdx.x];
+    const int v0 = voff[c], V = voff[c + 1] - v0;
+    T *P = (T *)dyn;  // [3][V]
+    const int tid = threadIdx.x;
+    constexpr int NT = 1024;
+    // y = A in, with in read by get(local vertex, component)
+    auto spmv_batch = [&](int base, auto get, T y[GB][3]) {
+        T cm[GB][6];
+        int col[GB][6];
+#pragma unroll
+        for (int i = 0; i < GB; i++) {
+            const int j = min(base + i * NT, V - 1);
+            const size_t g = (size_t)v0 + j;
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+                cm[i][m] = coef[(size_t)m * nv + g];
+                col[i][m] = nbr[(size_t)m * nv + g];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) y[i][d] = get(j, d);
+        }
+#pragma unroll
+        for (int i = 0; i < GB; i++)
+#pragma unroll
+            for (int m = 0; m < 6; m++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) y[i][d] += cm[i][m] * get(col[i][m], d);
+    };
+    auto getP = [&](int j, int d) { return P[(size_t)d * V + j]; };
+    auto getR = [&](int j, int d) { return r[3 * ((size_t)v0 + j) + d]; };
+    double acc[2] = {0.0, 0.0};
+    for (int j = tid; j < V; j += NT) {
+        const size_t g = (size_t)v0 + j;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const T b = bv[3 * g + d];
+            x[3 * g + d] = T(0); r[3 * g + d] = b; P[(size_t)d * V + j] = b;
+            acc[0] += (double)b * (double)b;
+        }
+    }
+    red.sum<1>(*(double(*)[1])acc);  // its barrier also orders the writes above
+    const double bb = acc[0];
+    if (bb == 0.0) {
+        const T zero[3] = {T(0), T(0), T(0)};
+        for (int j = tid; j < V; j += NT) out.put((size_t)v0 + j, zero);
+        if (tid == 0) iters[c] = 0;
+        return;
+    }
+    double rr = bb, rr_old = 0.0, alfa = 1.0, omeg = 1.0;
+    int it = 0, flag = 0;
+    bool conv = false;
+    for (it = 0; it < itrmax; it++) {
+        if (rr == 0.0) { flag = 1; break; }
+        if (it > 0) {  // p = r + beta (p - omeg v)
+            const double beta = (rr / rr_old) * (alfa / omeg);
+            for (int base = tid; base < V; base += GB * NT) {
+                T rv_[GB][3], vv_[GB][3];
+#pragma unroll
+                for (int i = 0; i < GB; i++) {
+                    const size_t g = (size_t)v0 + min(base + i * NT, V - 1);
+#pragma unroll
+                    for (int d = 0; d < 3; d++) { rv_[i][d] = r[3 * g + d]; vv_[i][d] = v[3 * g + d]; }
+                }
+#pragma unroll
+                for (int i = 0; i < GB; i++) {
+                    const int j = base + i * NT;
+                    if (j < V)
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            T &pp = P[(size_t)d * V + j];
+                            pp = (T)((double)rv_[i][d] + beta * ((double)pp - omeg * (double)vv_[i][d]));
+                        }
+                }
+            }
+            __syncthreads();
+        }
+        // v = A p ; alfa = (r, r^) / (v, r^)
+        acc[0] = 0.0;
+        for (int base = tid; base < V; base += GB * NT) {
+            T y[GB][3];
+            spmv_batch(base, getP, y);
+#pragma unroll
+            for (int i = 0; i < GB; i++) {
+                const int j = base + i * NT;
+                if (j < V) {
+                    const size_t g = (size_t)v0 + j;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        v[3 * g + d] = y[i][d];
+                        acc[0] += (double)y[i][d] * (double)bv[3 * g + d];
+                    }
+                }
+            }
+        }
+        red.sum<1>(*(double(*)[1])acc);
+        alfa = rr / acc[0];
+        for (int j = tid; j < V; j += NT) {
+            const size_t g = (size_t)v0 + j;
+#pragma unroll
+            for (int d = 0; d < 3; d++) r[3 * g + d] = (T)((double)r[3 * g + d] - alfa * (double)v[3 * g + d]);
+        }
+        __syncthreads();
+        // t = A r ; omeg = (r, t) / (t, t)
+        acc[0] = 0.0; acc[1] = 0.0;
+        for (int base = tid; base < V; base += GB * NT) {
+            T y[GB][3];
+            spmv_batch(base, getR, y);
+#pragma unroll
+            for (int i = 0; i < GB; i++) {
+                const int j = base + i * NT;
+                if (j < V) {
+                    const size_t g = (size_t)v0 + j;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        t[3 * g + d] = y[i][d];
+                        acc[0] += (double)r[3 * g + d] * (double)y[i][d];
+                        acc[1] += (double)y[i][d] * (double)y[i][d];
+                    }
+                }
+            }
+        }
+        red.sum<2>(acc);
+        omeg = acc[0] / acc[1];
+        // x = x + alfa p + omeg r ; r = r - omeg t
+        acc[0] = 0.0; acc[1] = 0.0;
+        for (int j = tid; j < V; j += NT) {
+            const size_t g = (size_t)v0 + j;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double ri = (double)r[3 * g + d];
+                x[3 * g + d] = (T)((double)x[3 * g + d] + alfa * (double)P[(size_t)d * V + j] + omeg * ri);
+                const T rn = (T)(ri - omeg * (double)t[3 * g + d]);
+                r[3 * g + d] = rn;
+                acc[0] += (double)rn * (double)rn;
+                acc[1] += (double)rn * (double)bv[3 * g + d];
+            }
+        }
+        red.sum<2>(acc);
+        rr_old = rr;
+        rr = acc[1];
+        if (sqrt(acc[0] / bb) < residual) { conv = true; break; }
+    }
+    __syncthreads();
+    for (int j = tid; j < V; j += NT) {
+        const size_t g = (size_t)v0 + j;
+        const T xv[3] = {x[3 * g], x[3 * g + 1], x[3 * g + 2]};
+        out.put(g, xv);
+    }
+    if (tid == 0) {
+        iters[c] = conv ? it + 1 : it;
+        if (!conv && flag == 0) flag = 2;
+        if (flag) atomicMax(&status->bicg_flag, flag);
+    }
+}
+
 // Same iteration for cells of any size with the vectors in global memory.
 // Each thread walks its vertices in batches of GB so that the loads of a
 // batch are in flight together.
@@ -650,6 +823,30 @@ lbmibm_status launch_onchip(lbmibm_ctx *ctx, int cls, int var, const SolveOut<T>
     return LBMIBM_OK;
 }
 
+// The large-cell class with the given form: 0 = vectors in global memory,
+// 1 = p in LDS (only when the largest cell fits).
+template <typename T>
+lbmibm_status launch_big(lbmibm_ctx *ctx, int var, const SolveOut<T> &out, hipStream_t stream) {
+    const lbmibm_params &p = ctx->p;
+    if constexpr (sizeof(T) == 4) {
+        if (var == 1 && ctx->bicg_maxv[2] <= BIG_LDS_V) {
+            const size_t lds = 3 * (size_t)ctx->bicg_maxv[2] * sizeof(T);
+            hipLaunchKernelGGL((k_bicg_big<T, 4>), dim3(ctx->bicg_n[2]), dim3(1024), lds, stream, ctx->bicg_cells[2],
+                               ctx->voff, (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->rh,
+                               (T *)ctx->rv, (T *)ctx->vv, (T *)ctx->tv, out, ctx->nv, p.itrmax, p.residual,
+                               ctx->bicg_iters, ctx->status);
+            LBMIBM_LAUNCHED(ctx, "k_bicg_big");
+            return LBMIBM_OK;
+        }
+    }
+    hipLaunchKernelGGL(k_bicg_global<T>, dim3(ctx->bicg_n[2]), dim3(1024), 0, stream, ctx->bicg_cells[2], ctx->voff,
+                       (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->rh, (T *)ctx->rv,
+                       (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, out, ctx->nv, p.itrmax, p.residual,
+                       ctx->bicg_iters, ctx->status);
+    LBMIBM_LAUNCHED(ctx, "k_bicg_global");
+    return LBMIBM_OK;
+}
+
 template <typename T> SolveOut<T> solve_out(lbmibm_ctx *ctx) {
     SolveOut<T> out;
     out.q = (T *)ctx->qv;
@@ -740,6 +937,24 @@ template <typename T> lbmibm_status autotune_impl(lbmibm_ctx *ctx) {
         snprintf(line, sizeof line, "bicg class %d: using %dx%d\n", cls, shape[0], shape[1]);
         ctx->tune_report += line;
     }
+    if (s == LBMIBM_OK && ctx->bicg_n[2] > 0 && sizeof(T) == 4 && ctx->bicg_maxv[2] <= BIG_LDS_V) {
+        float best = 0.0f;
+        for (int v = 0; v < 2 && s == LBMIBM_OK; v++) {
+            s = launch_big<T>(ctx, v, out, ctx->stream);  // warm-up
+            if (s != LBMIBM_OK) break;
+            LBMIBM_HIP(ctx, hipEventRecord(e0, ctx->stream));
+            s = launch_big<T>(ctx, v, out, ctx->stream);
+            LBMIBM_HIP(ctx, hipEventRecord(e1, ctx->stream));
+            LBMIBM_HIP(ctx, hipEventSynchronize(e1));
+            float ms = 0.0f;
+            LBMIBM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+            snprintf(line, sizeof line, "bicg class 2 form %s: %.4f ms\n", v == 0 ? "global" : "p-in-LDS", ms);
+            ctx->tune_report += line;
+            if (v == 0 || ms < best) { best = ms; ctx->bicg_var[2] = v; }
+        }
+        snprintf(line, sizeof line, "bicg class 2: using %s\n", ctx->bicg_var[2] == 0 ? "global" : "p-in-LDS");
+        ctx->tune_report += line;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return s;
@@ -758,11 +973,8 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
         // own buffer, the epilogue writes q
         LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
         LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-        hipLaunchKernelGGL(k_bicg_global<T>, dim3(ctx->bicg_n[2]), dim3(1024), 0, ctx->side, ctx->bicg_cells[2],
-                           ctx->voff, (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->rh,
-                           (T *)ctx->rv, (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, out, nv, p.itrmax, p.residual,
-                           ctx->bicg_iters, ctx->status);
-        LBMIBM_LAUNCHED(ctx, "k_bicg_global");
+        lbmibm_status sb = launch_big<T>(ctx, ctx->bicg_var[2], out, ctx->side);
+        if (sb != LBMIBM_OK) return sb;
     }
     for (int cls = 0; cls < 2; cls++)
         if (ctx->bicg_n[cls] > 0) {
@@ -811,6 +1023,10 @@ lbmibm_status membrane_setup(lbmibm_ctx *ctx) {
             LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 1, 1024>, A, lds));
         }
     }
+    if (ctx->bicg_n[2] > 0 && ctx->prec != LBMIBM_F64 && ctx->bicg_maxv[2] <= BIG_LDS_V) {
+        const int lds = (int)(3 * (size_t)ctx->bicg_maxv[2] * sizeof(float));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_big<float, 4>, A, lds));
+    }
     if (ctx->bicg_n[1] > 0) {
         const int lds = (int)(3 * 4 * (size_t)ctx->bicg_maxv[1] * sizeof(float));
         LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 6, 512>, A, lds));
@@ -821,7 +1037,7 @@ lbmibm_status membrane_setup(lbmibm_ctx *ctx) {
 }
 
 lbmibm_status membrane_autotune(lbmibm_ctx *ctx) {
-    ctx->bicg_var[0] = ctx->bicg_var[1] = 0;
+    ctx->bicg_var[0] = ctx->bicg_var[1] = ctx->bicg_var[2] = 0;
     ctx->tune_report.clear();
     if (ctx->nc == 0) return LBMIBM_OK;
     const char *env = getenv("LBMIBM_AUTOTUNE");
@@ -829,14 +1045,17 @@ lbmibm_status membrane_autotune(lbmibm_ctx *ctx) {
         ctx->tune_report = "autotune off (LBMIBM_AUTOTUNE=0): first shapes\n";
         return LBMIBM_OK;
     }
-    // LBMIBM_BICG_SHAPES="a,b" fixes the shape variants of classes 0 and 1
+    // LBMIBM_BICG_SHAPES="a,b[,c]" fixes the shape variants of classes 0 and 1
+    // and the form of class 2
     const char *fixed = getenv("LBMIBM_BICG_SHAPES");
     if (fixed && fixed[0]) {
-        int a = 0, b = 0;
-        if (sscanf(fixed, "%d,%d", &a, &b) != 2 || a < 0 || a >= BICG_NVAR || b < 0 || b >= BICG_NVAR)
+        int a = 0, b = 0, cc = 0;
+        const int nread = sscanf(fixed, "%d,%d,%d", &a, &b, &cc);
+        if (nread < 2 || a < 0 || a >= BICG_NVAR || b < 0 || b >= BICG_NVAR || cc < 0 || cc > 1)
             return LBMIBM_ERR_ARG;
         ctx->bicg_var[0] = a;
         ctx->bicg_var[1] = b;
+        ctx->bicg_var[2] = cc;
         ctx->tune_report = "shapes fixed by LBMIBM_BICG_SHAPES\n";
         return LBMIBM_OK;
     }

@@ -323,11 +323,12 @@ def test_spread_is_bitwise_reproducible():
 
 @pytest.mark.parametrize("variant", [0, 1, 2])
 def test_every_solver_shape_matches_the_oracle(variant, monkeypatch):
-    """Each candidate work-group shape of the on-chip solver classes (the
-    autotuner picks one) gives the oracle's loads and forces."""
-    monkeypatch.setenv("LBMIBM_BICG_SHAPES", f"{variant},{variant}")
-    lx, ly, lz = 24, 20, 24
-    cells = small_cells(lx, ly, lz, nc=3, radius=4.0, levels=[3, 4, 2], seed=61)
+    """Each candidate work-group shape of the on-chip solver classes and each
+    form of the large-cell solver (the autotuner picks one of each) gives the
+    oracle's loads and forces."""
+    monkeypatch.setenv("LBMIBM_BICG_SHAPES", f"{variant},{variant},{variant % 2}")
+    lx, ly, lz = 28, 24, 28
+    cells = small_cells(lx, ly, lz, nc=4, radius=4.0, levels=[3, 4, 2, 5], seed=61)
     kw = dict(c=100.0, br=0.0)
     o = oracle_of(lx, ly, lz, cells=cells, **kw)
     g = gpu_of(lx, ly, lz, "f32", cells=cells, **kw)
@@ -350,3 +351,10 @@ def test_autotune_reports_a_choice_per_class():
     g = gpu_of(lx, ly, lz, "f32", cells=cells)
     rep = g.autotune_report()
     assert "class 0: using" in rep and "class 1: using" in rep, rep
+
+
+def test_autotune_covers_the_large_cell_forms():
+    lx, ly, lz = 24, 20, 24
+    cells = small_cells(lx, ly, lz, nc=1, radius=4.0, levels=[5], seed=64)
+    g = gpu_of(lx, ly, lz, "f32", cells=cells)
+    assert "class 2: using" in g.autotune_report()
