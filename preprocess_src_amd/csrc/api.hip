@@ -172,7 +172,7 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
                     ctx->tile_off, ctx->tile_cur, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
                     ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv,
-                    ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair, ctx->srec};
+                    ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair, ctx->srec, ctx->onchip_cells};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -403,6 +403,18 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         for (int k = 0; k < 3; k++) {
             ctx->bicg_n[k] = (int)cls[k].size();
             if (!cls[k].empty()) CK(up_i(&ctx->bicg_cells[k], cls[k].data(), cls[k].size()));
+        }
+        {
+            // on-chip classes together (assembled after the large cells)
+            std::vector<int> onchip(cls[0]);
+            onchip.insert(onchip.end(), cls[1].begin(), cls[1].end());
+            ctx->n_onchip = (int)onchip.size();
+            if (!onchip.empty()) CK(up_i(&ctx->onchip_cells, onchip.data(), onchip.size()));
+            for (int c = 0; c < nc; c++) {
+                const int big = bicg_class(ctx->prec, cells->voff[c + 1] - cells->voff[c]) == 2;
+                ctx->cls_maxe[big] = std::max(ctx->cls_maxe[big], cells->eoff[c + 1] - cells->eoff[c]);
+                ctx->cls_maxv[big] = std::max(ctx->cls_maxv[big], cells->voff[c + 1] - cells->voff[c]);
+            }
         }
         std::vector<int> vcls[2];
         for (int c = 0; c < nc; c++) vcls[cells->voff[c + 1] - cells->voff[c] > VOL_SMALL].push_back(c);

@@ -81,6 +81,9 @@ struct lbmibm_ctx {
     int *bicg_cells[3] = {nullptr, nullptr, nullptr};
     int bicg_n[3] = {0, 0, 0}, bicg_maxv[3] = {0, 0, 0};
     int bicg_var[3] = {0, 0, 0};  // shape variant of the on-chip classes, form of class 2 (autotuned)
+    int *onchip_cells = nullptr;  // cells of classes 0 and 1
+    int n_onchip = 0;
+    int cls_maxe[2] = {0, 0}, cls_maxv[2] = {0, 0};  // largest element / vertex count: on-chip, large
     std::string tune_report;
     // volume-correction work-group classes: up to VOL_SMALL vertices, larger
     int *vol_cells[2] = {nullptr, nullptr};

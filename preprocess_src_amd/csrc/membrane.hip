@@ -65,9 +65,17 @@ __global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, con
                                               const double *__restrict__ nref, T *__restrict__ tt,
                                               T *__restrict__ rg, T *__restrict__ tp, const int *__restrict__ cslot,
                                               typename Corner<T>::type *__restrict__ corner, int ne,
-                                              PeriodicBox box, TensionArgs a, const DevStep *__restrict__ st) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ne) return;
+                                              PeriodicBox box, TensionArgs a, const DevStep *__restrict__ st,
+                                              const int *__restrict__ cells, const int *__restrict__ eoff) {
+    // all elements (cells == nullptr, 1-D grid) or the elements of cell
+    // cells[blockIdx.y] (2-D grid)
+    int j = blockIdx.x * blockDim.x + threadIdx.x, jend = ne;
+    if (cells) {
+        const int c = cells[blockIdx.y];
+        j += eoff[c];
+        jend = eoff[c + 1];
+    }
+    if (j >= jend) return;
     const int ea = ele[3 * j], eb = ele[3 * j + 1], ec = ele[3 * j + 2];
     double v1[3], v2[3];
 #pragma unroll
@@ -146,9 +154,15 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
                                               const typename Corner<T>::type *__restrict__ corner,
                                               const int *__restrict__ vcell, const T *__restrict__ gscale,
                                               T *__restrict__ coef, T *__restrict__ bv, T *__restrict__ dsv,
-                                              int nv) {
-    const int jj = blockIdx.x * blockDim.x + threadIdx.x;
-    if (jj >= nv) return;
+                                              int nv, const int *__restrict__ cells, const int *__restrict__ voff) {
+    // all vertices or those of cell cells[blockIdx.y], as k_elem
+    int jj = blockIdx.x * blockDim.x + threadIdx.x, jend = nv;
+    if (cells) {
+        const int c = cells[blockIdx.y];
+        jj += voff[c];
+        jend = voff[c + 1];
+    }
+    if (jj >= jend) return;
     const int n = nln[jj];
     double rgs[6], tb[3] = {0.0, 0.0, 0.0}, jac = 0.0, ds = 0.0;
 #pragma unroll
@@ -871,17 +885,23 @@ template <typename T> __global__ void k_synth_rhs(T *__restrict__ b, size_t n) {
     b[i] = (T)((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
 }
 
-template <typename T> lbmibm_status assemble_impl(lbmibm_ctx *ctx, const PeriodicBox &box, const TensionArgs &ta) {
+// Tension, corner terms and rows of all cells (cells == nullptr) or of the
+// listed cells (n of them, at most max_e elements and max_v vertices each).
+template <typename T>
+lbmibm_status assemble_impl(lbmibm_ctx *ctx, const PeriodicBox &box, const TensionArgs &ta,
+                            const int *cells = nullptr, int n = 0, int max_e = 0, int max_v = 0) {
     const lbmibm_params &p = ctx->p;
     const int ne = ctx->nt, nv = ctx->nv;
-    hipLaunchKernelGGL(k_elem<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
-                       ctx->ele, (const double *)ctx->nref, (T *)ctx->tt, (T *)ctx->rg,
+    const dim3 ge = cells ? dim3((max_e + 255) / 256, n) : dim3((ne + 255) / 256);
+    const dim3 gv = cells ? dim3((max_v + 255) / 256, n) : dim3((nv + 255) / 256);
+    hipLaunchKernelGGL(k_elem<T>, ge, dim3(256), 0, ctx->stream, (const double *)ctx->xd, ctx->ele,
+                       (const double *)ctx->nref, (T *)ctx->tt, (T *)ctx->rg,
                        p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ctx->cslot,
-                       (typename Corner<T>::type *)ctx->corner, ne, box, ta, ctx->step);
+                       (typename Corner<T>::type *)ctx->corner, ne, box, ta, ctx->step, cells, ctx->eoff);
     LBMIBM_LAUNCHED(ctx, "k_elem");
-    hipLaunchKernelGGL(k_vert<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln, ctx->epair,
-                       (const typename Corner<T>::type *)ctx->corner, ctx->vcell,
-                       (const T *)ctx->gscale, (T *)ctx->coef, (T *)ctx->bv, (T *)ctx->dsv, nv);
+    hipLaunchKernelGGL(k_vert<T>, gv, dim3(256), 0, ctx->stream, ctx->nln, ctx->epair,
+                       (const typename Corner<T>::type *)ctx->corner, ctx->vcell, (const T *)ctx->gscale,
+                       (T *)ctx->coef, (T *)ctx->bv, (T *)ctx->dsv, nv, cells, ctx->voff);
     LBMIBM_LAUNCHED(ctx, "k_vert");
     return LBMIBM_OK;
 }
@@ -963,8 +983,14 @@ template <typename T> lbmibm_status autotune_impl(lbmibm_ctx *ctx) {
 template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
     const lbmibm_params &p = ctx->p;
     const PeriodicBox box = box_of(ctx);
+    const TensionArgs ta = tension_args(ctx);
     const int ne = ctx->nt, nv = ctx->nv;
-    lbmibm_status st = assemble_impl<T>(ctx, box, tension_args(ctx));
+    // with large cells, their systems are assembled first so that their
+    // solve (the longest) starts while the others are assembled and solved
+    const bool split = ctx->bicg_n[2] > 0 && ctx->bicg_n[2] <= 65535 && ctx->n_onchip <= 65535;  // grid y
+    lbmibm_status st = split ? assemble_impl<T>(ctx, box, ta, ctx->bicg_cells[2], ctx->bicg_n[2], ctx->cls_maxe[1],
+                                                ctx->bicg_maxv[2])
+                             : assemble_impl<T>(ctx, box, ta);
     if (st != LBMIBM_OK) return st;
     const SolveOut<T> out = solve_out<T>(ctx);
     if (ctx->bicg_n[2] > 0) {
@@ -975,6 +1001,10 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
         LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
         lbmibm_status sb = launch_big<T>(ctx, ctx->bicg_var[2], out, ctx->side);
         if (sb != LBMIBM_OK) return sb;
+        if (split && ctx->n_onchip > 0) {
+            sb = assemble_impl<T>(ctx, box, ta, ctx->onchip_cells, ctx->n_onchip, ctx->cls_maxe[0], ctx->cls_maxv[0]);
+            if (sb != LBMIBM_OK) return sb;
+        }
     }
     for (int cls = 0; cls < 2; cls++)
         if (ctx->bicg_n[cls] > 0) {
