@@ -576,6 +576,196 @@ __global__ __launch_bounds__(1024) void k_bicg_big(const int *__restrict__ cells
     }
 }
 
+// Large cells, third form (FP32): every vertex belongs to one thread for the
+// whole solve (j = tid + k*1024, k < K), so r (later t, which reuses its
+// registers) and v stay in registers.  One LDS vector [V][3] holds p for
+// v = A p and then s for t = A s, so both products gather from LDS.  p and x
+// are the only vectors in memory, each touched by its owner thread only, so
+// they need no barriers.  Per vertex and iteration that is 156 B of memory
+// traffic (two matrix reads, b twice, p three times, x twice) against about
+// 270 B for the p-in-LDS form.  Same arithmetic and update order as the
+// other forms (matrix_solver.cu:279-377).
+// A copy of v that the compiler cannot see through: addresses derived from it
+// are recomputed where they are used instead of being hoisted out of the
+// iteration loop and held in registers for the whole solve.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+template <int K>
+__global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells, const int *__restrict__ voff,
+                                                   const float *__restrict__ coef,
+                                                   const unsigned short *__restrict__ nbr,
+                                                   const float *__restrict__ bv, float *__restrict__ x,
+                                                   float *__restrict__ pg, SolveOut<float> out, int nv, int itrmax,
+                                                   double residual, int *__restrict__ iters,
+                                                   DevStatus *__restrict__ status) {
+    extern __shared__ double dyn[];
+    __shared__ double sh[128];
+    WgReduce red{sh, 0};
+    constexpr int NT = 1024;
+    const int c = cells[blockIdx.x];
+    const int v0 = voff[c], V = voff[c + 1] - v0;
+    float *L = (float *)dyn;  // [V][3]
+    float rt[K][3], vr[K][3];  // r (then t), v
+    // y = A L at local vertex j (global index g); 32-bit offsets from the
+    // uniform bases
+    auto spmv = [&](int j, unsigned g, float y[3]) {
+        float cm[6];
+        int col[6];
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            cm[m] = coef[(unsigned)m * (unsigned)nv + g];
+            col[m] = nbr[(unsigned)m * (unsigned)nv + g];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) y[d] = L[3 * j + d];
+#pragma unroll
+        for (int m = 0; m < 6; m++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) y[d] += cm[m] * L[3 * col[m] + d];
+    };
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = threadIdx.x + k * NT;
+#pragma unroll
+        for (int d = 0; d < 3; d++) { rt[k][d] = 0.0f; vr[k][d] = 0.0f; }
+        if (j < V) {
+            const unsigned g = (unsigned)(v0 + j);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const float b = bv[3 * g + d];
+                x[3 * g + d] = 0.0f;
+                rt[k][d] = b;
+                acc[0] += (double)b * (double)b;
+            }
+        }
+    }
+    red.sum<1>(*(double(*)[1])acc);
+    const double bb = acc[0];
+    if (bb == 0.0) {
+        const float zero[3] = {0.0f, 0.0f, 0.0f};
+        for (int j = threadIdx.x; j < V; j += NT) out.put((size_t)v0 + j, zero);
+        if (threadIdx.x == 0) iters[c] = 0;
+        return;
+    }
+    double rr = bb, rr_old = 0.0, alfa = 1.0, omeg = 1.0;
+    int it = 0, flag = 0;
+    bool conv = false;
+    for (it = 0; it < itrmax; it++) {
+        if (rr == 0.0) { flag = 1; break; }
+        // p = r (first pass) or r + beta (p - omeg v), into LDS and memory
+        const double beta = it > 0 ? (rr / rr_old) * (alfa / omeg) : 0.0;
+        {
+            const int t0 = opaque(threadIdx.x), b0 = opaque(v0);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int j = t0 + k * NT;
+                if (j < V) {
+                    const unsigned g = (unsigned)(b0 + j);
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        float pn = rt[k][d];
+                        if (it > 0)
+                            pn = (float)((double)rt[k][d] + beta * ((double)pg[3 * g + d] - omeg * (double)vr[k][d]));
+                        pg[3 * g + d] = pn;
+                        L[3 * j + d] = pn;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // v = A p ; alfa = (r, r^) / (v, r^)
+        acc[0] = 0.0;
+        {
+            const int t0 = opaque(threadIdx.x), b0 = opaque(v0);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int j = t0 + k * NT;
+                if (j < V) {
+                    const unsigned g = (unsigned)(b0 + j);
+                    spmv(j, g, vr[k]);
+#pragma unroll
+                    for (int d = 0; d < 3; d++) acc[0] += (double)vr[k][d] * (double)bv[3 * g + d];
+                }
+            }
+        }
+        red.sum<1>(*(double(*)[1])acc);  // its barrier ends the reads of p
+        alfa = rr / acc[0];
+        // s = r - alfa v into LDS
+        {
+            const int t0 = opaque(threadIdx.x);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int j = t0 + k * NT;
+                if (j < V)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) L[3 * j + d] = (float)((double)rt[k][d] - alfa * (double)vr[k][d]);
+            }
+        }
+        __syncthreads();
+        // t = A s ; omeg = (s, t) / (t, t)
+        acc[0] = 0.0; acc[1] = 0.0;
+        {
+            const int t0 = opaque(threadIdx.x), b0 = opaque(v0);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int j = t0 + k * NT;
+                if (j < V) {
+                    spmv(j, (unsigned)(b0 + j), rt[k]);
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        acc[0] += (double)L[3 * j + d] * (double)rt[k][d];
+                        acc[1] += (double)rt[k][d] * (double)rt[k][d];
+                    }
+                }
+            }
+        }
+        red.sum<2>(acc);
+        omeg = acc[0] / acc[1];
+        // x = x + alfa p + omeg s ; r = s - omeg t
+        acc[0] = 0.0; acc[1] = 0.0;
+        {
+            const int t0 = opaque(threadIdx.x), b0 = opaque(v0);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int j = t0 + k * NT;
+                if (j < V) {
+                    const unsigned g = (unsigned)(b0 + j);
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const double si = (double)L[3 * j + d];
+                        x[3 * g + d] = (float)((double)x[3 * g + d] + alfa * (double)pg[3 * g + d] + omeg * si);
+                        const float rn = (float)(si - omeg * (double)rt[k][d]);
+                        rt[k][d] = rn;
+                        acc[0] += (double)rn * (double)rn;
+                        acc[1] += (double)rn * (double)bv[3 * g + d];
+                    }
+                }
+            }
+        }
+        red.sum<2>(acc);  // its barrier ends the reads of s before p is written
+        rr_old = rr;
+        rr = acc[1];
+        if (sqrt(acc[0] / bb) < residual) { conv = true; break; }
+    }
+    for (int j = threadIdx.x; j < V; j += NT) {
+        const size_t g = (size_t)v0 + j;
+        const float xv[3] = {x[3 * g], x[3 * g + 1], x[3 * g + 2]};
+        out.put(g, xv);
+    }
+    if (threadIdx.x == 0) {
+        iters[c] = conv ? it + 1 : it;
+        if (!conv && flag == 0) flag = 2;
+        if (flag) atomicMax(&status->bicg_flag, flag);
+    }
+}
+
+// Vertices per thread of the register-resident form (V <= K * 1024).
+constexpr int BICG_RES_K = 11;
+
 // Same iteration for cells of any size with the vectors in global memory.
 // Each thread walks its vertices in batches of GB so that the loads of a
 // batch are in flight together.
@@ -837,13 +1027,32 @@ lbmibm_status launch_onchip(lbmibm_ctx *ctx, int cls, int var, const SolveOut<T>
     return LBMIBM_OK;
 }
 
-// The large-cell class with the given form: 0 = vectors in global memory,
-// 1 = p in LDS (only when the largest cell fits).
+// Forms of the large-cell class: 0 = vectors in global memory, 1 = p in LDS,
+// 2 = register-resident (FP32; 1 and 2 only when the largest cell fits).
+constexpr int BICG_NFORM = 3;
+constexpr const char *BICG_FORM_NAMES[BICG_NFORM] = {"global", "p-in-LDS", "resident"};
+
+bool big_form_fits(const lbmibm_ctx *ctx, int form) {
+    if (form == 0) return true;
+    if (ctx->prec == LBMIBM_F64) return false;
+    if (form == 1) return ctx->bicg_maxv[2] <= BIG_LDS_V;
+    return ctx->bicg_maxv[2] <= BICG_RES_K * 1024;
+}
+
 template <typename T>
 lbmibm_status launch_big(lbmibm_ctx *ctx, int var, const SolveOut<T> &out, hipStream_t stream) {
     const lbmibm_params &p = ctx->p;
     if constexpr (sizeof(T) == 4) {
-        if (var == 1 && ctx->bicg_maxv[2] <= BIG_LDS_V) {
+        if (var == 2 && big_form_fits(ctx, 2)) {
+            const size_t lds = 3 * (size_t)ctx->bicg_maxv[2] * sizeof(float);
+            hipLaunchKernelGGL(k_bicg_res<BICG_RES_K>, dim3(ctx->bicg_n[2]), dim3(1024), lds, stream,
+                               ctx->bicg_cells[2], ctx->voff, (const float *)ctx->coef, ctx->nbr,
+                               (const float *)ctx->bv, (float *)ctx->rh, (float *)ctx->pv, out, ctx->nv, p.itrmax,
+                               p.residual, ctx->bicg_iters, ctx->status);
+            LBMIBM_LAUNCHED(ctx, "k_bicg_res");
+            return LBMIBM_OK;
+        }
+        if (var == 1 && big_form_fits(ctx, 1)) {
             const size_t lds = 3 * (size_t)ctx->bicg_maxv[2] * sizeof(T);
             hipLaunchKernelGGL((k_bicg_big<T, 4>), dim3(ctx->bicg_n[2]), dim3(1024), lds, stream, ctx->bicg_cells[2],
                                ctx->voff, (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->rh,
@@ -957,9 +1166,10 @@ template <typename T> lbmibm_status autotune_impl(lbmibm_ctx *ctx) {
         snprintf(line, sizeof line, "bicg class %d: using %dx%d\n", cls, shape[0], shape[1]);
         ctx->tune_report += line;
     }
-    if (s == LBMIBM_OK && ctx->bicg_n[2] > 0 && sizeof(T) == 4 && ctx->bicg_maxv[2] <= BIG_LDS_V) {
+    if (s == LBMIBM_OK && ctx->bicg_n[2] > 0 && sizeof(T) == 4) {
         float best = 0.0f;
-        for (int v = 0; v < 2 && s == LBMIBM_OK; v++) {
+        for (int v = 0; v < BICG_NFORM && s == LBMIBM_OK; v++) {
+            if (!big_form_fits(ctx, v)) continue;
             s = launch_big<T>(ctx, v, out, ctx->stream);  // warm-up
             if (s != LBMIBM_OK) break;
             LBMIBM_HIP(ctx, hipEventRecord(e0, ctx->stream));
@@ -968,11 +1178,11 @@ template <typename T> lbmibm_status autotune_impl(lbmibm_ctx *ctx) {
             LBMIBM_HIP(ctx, hipEventSynchronize(e1));
             float ms = 0.0f;
             LBMIBM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-            snprintf(line, sizeof line, "bicg class 2 form %s: %.4f ms\n", v == 0 ? "global" : "p-in-LDS", ms);
+            snprintf(line, sizeof line, "bicg class 2 form %s: %.4f ms\n", BICG_FORM_NAMES[v], ms);
             ctx->tune_report += line;
             if (v == 0 || ms < best) { best = ms; ctx->bicg_var[2] = v; }
         }
-        snprintf(line, sizeof line, "bicg class 2: using %s\n", ctx->bicg_var[2] == 0 ? "global" : "p-in-LDS");
+        snprintf(line, sizeof line, "bicg class 2: using %s\n", BICG_FORM_NAMES[ctx->bicg_var[2]]);
         ctx->tune_report += line;
     }
     (void)hipEventDestroy(e0);
@@ -1053,9 +1263,13 @@ lbmibm_status membrane_setup(lbmibm_ctx *ctx) {
             LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 1, 1024>, A, lds));
         }
     }
-    if (ctx->bicg_n[2] > 0 && ctx->prec != LBMIBM_F64 && ctx->bicg_maxv[2] <= BIG_LDS_V) {
+    if (ctx->bicg_n[2] > 0 && big_form_fits(ctx, 1)) {
         const int lds = (int)(3 * (size_t)ctx->bicg_maxv[2] * sizeof(float));
         LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_big<float, 4>, A, lds));
+    }
+    if (ctx->bicg_n[2] > 0 && big_form_fits(ctx, 2)) {
+        const int lds = (int)(3 * (size_t)ctx->bicg_maxv[2] * sizeof(float));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K>, A, lds));
     }
     if (ctx->bicg_n[1] > 0) {
         const int lds = (int)(3 * 4 * (size_t)ctx->bicg_maxv[1] * sizeof(float));
@@ -1081,7 +1295,7 @@ lbmibm_status membrane_autotune(lbmibm_ctx *ctx) {
     if (fixed && fixed[0]) {
         int a = 0, b = 0, cc = 0;
         const int nread = sscanf(fixed, "%d,%d,%d", &a, &b, &cc);
-        if (nread < 2 || a < 0 || a >= BICG_NVAR || b < 0 || b >= BICG_NVAR || cc < 0 || cc > 1)
+        if (nread < 2 || a < 0 || a >= BICG_NVAR || b < 0 || b >= BICG_NVAR || cc < 0 || cc >= BICG_NFORM)
             return LBMIBM_ERR_ARG;
         ctx->bicg_var[0] = a;
         ctx->bicg_var[1] = b;
