@@ -10,7 +10,11 @@ seeded mesh mix of 50% 642-, 40% 2562- and 10% 10242-vertex meshes, radius
 populations stored as deviations, whole steps replayed from a hipGraph.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|f64|f16]
-                    [--workload suspension|tgv] [--cpu-seconds S] [--no-cpu]
+                    [--workload suspension|tgv|adjoint] [--cpu-seconds S] [--no-cpu]
+
+--workload adjoint times the checkpointed wall-shear gradient on a 128x64x64
+channel with 32 RBCs (500 steps, last 200 averaged, checkpoints every 50)
+against the forward run and checks it against a central difference.
 """
 from __future__ import annotations
 
@@ -44,6 +48,89 @@ def suspension_workload(nc=1024, lx=256, ly=128, lz=128, radius=7.0, seed=2024):
             f"({int((levels == 3).sum())}x642 + {int((levels == 4).sum())}x2562 + {int((levels == 5).sum())}x10242 "
             f"vertices), r={radius:g} lu, Ht={100 * ht:.1f}%")
     return (lx, ly, lz), cells, params, desc, ht
+
+
+def adjoint_workload(nc=32, lx=128, ly=64, lz=64, radius=4.0, seed=2025):
+    """128x64x64 channel with 32 biconcave RBCs of 2562 vertices, as the
+    128x64x64 suspension, for the wall-shear gradient."""
+    from preprocess_src_amd import mesh
+    levels = np.full(nc, 4)
+    centres, rot = mesh.place(nc, radius, lx, ly, lz, seed)
+    cells = mesh.assemble(levels, centres, rot, radius)
+    gs = 5e-4
+    params = dict(tau=1.0, p=0.5e-6, e=(1.0, 0.0, 0.0), b=gs / 2.0, m=1, c=100.0, runge_kutta=4,
+                  residual=1e-10, itrmax=1000, volume_correction=False)
+    ht = float(mesh.cell_volume(cells).sum() / (lx * ly * lz))
+    desc = (f"adjoint: {lx}x{ly}x{lz} channel, {nc} RBCs x 2562 vertices, r={radius:g} lu; J = mean wall shear "
+            f"over the last 200 of 500 steps; gradients w.r.t. the per-node body-force field and per-cell Gs")
+    return (lx, ly, lz), cells, params, desc, ht
+
+
+def adjoint_bench(a):
+    """Cost of one checkpointed gradient of J relative to the forward run, and
+    its agreement with a central finite difference along a random direction."""
+    from preprocess_src_amd.simulation import Simulation
+    shape, cells, params, desc, ht = adjoint_workload()
+    lx, ly, lz = shape
+    nsteps, window, K = 500, 200, 50
+    sim = Simulation(lx, ly, lz, cells=cells, precision=a.precision, **params)
+    x0 = np.concatenate([cells["x"], cells["y"], cells["z"]])
+
+    def reset():
+        sim.set_equilibrium()
+        x, y, z = x0.reshape(3, -1)
+        sim.set_cells_state(x=x, y=y, z=z, xd=np.stack([x, y, z], 1).reshape(-1).copy())
+
+    print(f"[bench] {desc}", file=sys.stderr, flush=True)
+    reset()
+    sim.wall_shear(0, 20, 10)  # warm-up
+    reset()
+    t0 = time.perf_counter()
+    J = sim.wall_shear(0, nsteps, window)
+    t_fwd = time.perf_counter() - t0
+    print(f"[bench] forward {nsteps} steps: {t_fwd:.3f} s", file=sys.stderr, flush=True)
+    reset()
+    t0 = time.perf_counter()
+    J2, gH, gs = sim.adjoint_wall_shear(0, nsteps, window, K)
+    t_adj = time.perf_counter() - t0
+    print(f"[bench] forward + adjoint: {t_adj:.3f} s", file=sys.stderr, flush=True)
+    # central difference along a random body-force direction (relative size 10% of P)
+    rng = np.random.default_rng(7)
+    m = sim.comp_mask()
+    d = [0.1 * params["p"] * rng.standard_normal(sim.np_) * m for _ in range(3)]
+    fd = []
+    for sgn in (1.0, -1.0):
+        reset()
+        sim.set_body_force_field(*[sgn * v for v in d])
+        fd.append(sim.wall_shear(0, nsteps, window))
+    sim.set_body_force_field()
+    dd = (fd[0] - fd[1]) / 2.0
+    ad = float(sum(g @ v for g, v in zip(gH, d)))
+    n = lx * ly * lz
+    out = {
+        "metric": "adjoint/forward cost (wall-shear gradient, checkpointed)",
+        "value": round((t_adj - t_fwd) / t_fwd, 3),
+        "unit": "x forward",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": 20,
+        "ms_per_step": round(1e3 * t_fwd / nsteps, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.precision,
+        "data": "synthetic (seeded cell placement, fluid at rest start)",
+        "config": {"workload": desc, "lattice": [lx, ly, lz], "cells": sim.nc, "vertices": sim.nv,
+                   "window": window, "checkpoint_every": K, "hematocrit": round(ht, 4)},
+        "forward_s": round(t_fwd, 4), "forward_plus_adjoint_s": round(t_adj, 4),
+        "forward_mlups": round(n * nsteps / t_fwd / 1e6, 1),
+        "J": J, "J_in_adjoint_call": J2,
+        "fd_check": {"directional_adjoint": ad, "central_difference": dd,
+                     "rel_err": abs(ad - dd) / max(abs(dd), 1e-300)},
+        "grad_force_scale_range": [float(np.min(gs)), float(np.max(gs))],
+    }
+    print(json.dumps(out), flush=True)
+    sim.close()
 
 
 def tgv_workload(n=256):
@@ -115,7 +202,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64", "f16"])
-    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv"])
+    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv", "adjoint"])
     ap.add_argument("--stage-steps", type=int, default=6)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -128,6 +215,9 @@ def main():
 
     from preprocess_src_amd.simulation import Simulation
 
+    if a.workload == "adjoint":
+        adjoint_bench(a)
+        return
     if a.workload == "suspension":
         shape, cells, params, desc, ht = suspension_workload()
     else:

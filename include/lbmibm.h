@@ -175,6 +175,58 @@ lbmibm_status lbmibm_download_volume(lbmibm_ctx *ctx, double *volume0, double *v
 /* Per-cell force scale (per-cell Gs / reference Gs), [nc]; NULL resets to 1. */
 lbmibm_status lbmibm_set_cell_force_scale(lbmibm_ctx *ctx, const double *scale);
 
+/* ---- reverse mode (no reference counterpart) --------------------------
+ * Adjoint (vector-Jacobian product) entry points for the main.cu loop body
+ * with M = 1, RUNGE_KUTTA = 4, BR = 0, no volume correction and no FILEOUTPUT
+ * accumulators, in FP64 or FP32 (not FP16 storage).  The context holds the
+ * adjoint state below; each stage VJP is taken at the current forward state,
+ * i.e. it differentiates the stage as it would run next, and leaves the
+ * forward state unchanged (except that lbmibm_adj_stress recomputes the
+ * membrane force at the current positions). */
+enum {
+    LBMIBM_ADJ_POPULATIONS = 0,     /* [n*19] AoS, the reference's post-stream layout of ltc->fn  */
+    LBMIBM_ADJ_VELOCITY = 1,        /* [3n] padded blocks x|y|z: the velocity field u (um..wm)     */
+    LBMIBM_ADJ_LATTICE_FORCE = 2,   /* [3n]: the lattice IBM force (ltc->fx..)                     */
+    LBMIBM_ADJ_POSITIONS = 3,       /* [3Nv] SoA blocks x|y|z: vertex positions (cel->x.., fem->xd) */
+    LBMIBM_ADJ_NODAL_FORCE = 4,     /* [3Nv]: nodal forces (cel->fx..)                             */
+    LBMIBM_ADJ_GRAD_BODY_FORCE = 5, /* [3n]: accumulated gradient w.r.t. the body-force field      */
+    LBMIBM_ADJ_GRAD_FORCE_SCALE = 6 /* [nc]: accumulated gradient w.r.t. the per-cell force scale  */
+};
+/* Per-node body-force field g, added to P e in the Guo force 2 rho (P e + B F
+ * + g); three padded [n] arrays (COMP nodes read), or all NULL to remove it. */
+lbmibm_status lbmibm_set_body_force_field(lbmibm_ctx *ctx, const double *gx, const double *gy, const double *gz);
+/* Zeroes every adjoint buffer and gradient (allocated on first use). */
+lbmibm_status lbmibm_adjoint_reset(lbmibm_ctx *ctx);
+/* Exchange of one adjoint buffer (which = LBMIBM_ADJ_*) in the layouts above;
+ * padded entries that are not COMP nodes read back as 0 and are ignored. */
+lbmibm_status lbmibm_adjoint_get(lbmibm_ctx *ctx, int which, double *out);
+lbmibm_status lbmibm_adjoint_set(lbmibm_ctx *ctx, int which, const double *in);
+/* Collide-stream: POPULATIONS and VELOCITY (adjoints of the step's outputs) ->
+ * POPULATIONS (of its input) and LATTICE_FORCE; adds to GRAD_BODY_FORCE and
+ * clears VELOCITY. */
+lbmibm_status lbmibm_adj_lbm_d3q19(lbmibm_ctx *ctx, int iter);
+/* Spreading: LATTICE_FORCE -> NODAL_FORCE; adds to POSITIONS. */
+lbmibm_status lbmibm_adj_ibm_st1(lbmibm_ctx *ctx);
+/* Interpolation + RK4 advection: POSITIONS (after) -> POSITIONS (before);
+ * adds to VELOCITY. */
+lbmibm_status lbmibm_adj_ibm_st2(lbmibm_ctx *ctx);
+/* Skalak membrane force (tension, consistent-mass solve, q2f): NODAL_FORCE ->
+ * adds to POSITIONS and GRAD_FORCE_SCALE (one transposed solve per cell). */
+lbmibm_status lbmibm_adj_stress(lbmibm_ctx *ctx, int iter);
+/* Wall-shear objective J = mean over the last `window` of steps iter0+1 ..
+ * iter0+nsteps of tau_w = 2 Df (dx/dt) (e . u) at the first fluid nodes of
+ * both y walls, averaged over the walls (e = unit body-force direction).
+ * Runs the steps. */
+lbmibm_status lbmibm_wall_shear(lbmibm_ctx *ctx, int iter0, int nsteps, int window, double *J);
+/* J as lbmibm_wall_shear and its gradient with respect to the body-force
+ * field (padded [n] arrays, any may be NULL) and the per-cell force scales
+ * ([nc] or NULL), by checkpoints every `checkpoint_every` steps and per-segment
+ * recomputation (device memory ~ checkpoint_every steps of populations,
+ * fields and positions).  The populations and positions end at step
+ * iter0+nsteps, as after lbmibm_run. */
+lbmibm_status lbmibm_adjoint_wall_shear(lbmibm_ctx *ctx, int iter0, int nsteps, int window, int checkpoint_every,
+                                        double *J, double *gx, double *gy, double *gz, double *dscale);
+
 /* ---- host-side preprocessing (no GPU needed) ------------------------- */
 /* Icosphere of the given subdivision level on the unit sphere, outward
  * triangles: 10*4^level+2 vertices, 20*4^level triangles, vertices numbered

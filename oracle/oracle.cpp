@@ -152,13 +152,15 @@ void orc_lbm_step(const orc_params *p, orc_state *s, int iter) {
         if (reset) { s->dm[i] = 0.0; s->um[i] = 0.0; s->vm[i] = 0.0; s->wm[i] = 0.0; }
         const double tfx = s->fx[i], tfy = s->fy[i], tfz = s->fz[i];
         const double tvf = s->vf ? s->vf[i] : 0.0;
+        // optional per-node body-force field, same convention as P e
+        const double gbx = s->gbx ? s->gbx[i] : 0.0, gby = s->gby ? s->gby[i] : 0.0, gbz = s->gbz ? s->gbz[i] : 0.0;
 
         double td = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
         for (int q = 0; q < Q; q++) td += f[q];
         for (int q = 0; q < Q; q++) { jx += CX[q] * f[q]; jy += CY[q] * f[q]; jz += CZ[q] * f[q]; }
-        const double tu = (jx + td * p->p * ui + td * p->b * tfx) / (td * Cr);
-        const double tv = (jy + td * p->p * vi + td * p->b * tfy) / (td * Cr);
-        const double tw = (jz + td * p->p * wi + td * p->b * tfz) / (td * Cr);
+        const double tu = (jx + td * p->p * ui + td * p->b * tfx + td * gbx) / (td * Cr);
+        const double tv = (jy + td * p->p * vi + td * p->b * tfy + td * gby) / (td * Cr);
+        const double tw = (jz + td * p->p * wi + td * p->b * tfz + td * gbz) / (td * Cr);
 
         s->dn[i] = td; s->un[i] = tu; s->vn[i] = tv; s->wn[i] = tw;
         s->dm[i] += td; s->um[i] += tu; s->vm[i] += tv; s->wm[i] += tw;
@@ -168,6 +170,7 @@ void orc_lbm_step(const orc_params *p, orc_state *s, int iter) {
         const double om1 = 1.0 - om;
         const double om2 = (1.0 - om / 2.0) * 2.0 * td * p->p;
         const double om3 = (1.0 - om / 2.0) * 2.0 * td * p->b;
+        const double om4 = (1.0 - om / 2.0) * 2.0 * td;
         const double uvw = -1.5 * (tu * tu + tv * tv + tw * tw) * Cr2;
         const double Ux = tu * Cr, Uy = tv * Cr, Uz = tw * Cr;
         for (int q = 0; q < Q; q++) {
@@ -177,7 +180,8 @@ void orc_lbm_step(const orc_params *p, orc_state *s, int iter) {
             const double gz = 3.0 * (CZ[q] - Uz) + 9.0 * cu * CZ[q];
             const double fs = om1 * f[q] + om * td / WDIV[q] * (1.0 + 3.0 * cu + 4.5 * cu * cu + uvw)
                             + om2 / WDIV[q] * (gx * ui + gy * vi + gz * wi)
-                            + om3 / WDIV[q] * (gx * tfx + gy * tfy + gz * tfz);
+                            + om3 / WDIV[q] * (gx * tfx + gy * tfy + gz * tfz)
+                            + om4 / WDIV[q] * (gx * gbx + gy * gby + gz * gbz);
             // Push with half-way bounce-back (lbm.cu:1273-1310).
             const int t = g.idx(ix + CX[q], iy + CY[q], iz + CZ[q]);
             int j;

@@ -84,6 +84,9 @@ typedef struct orc_state {
     const double *gs_scale;                /* [nc] per-cell force scale or NULL (=1)     */
     double *volume0, *volume, *area, *gx;  /* [nc], [nc], [nc], [3nc]                    */
     int    *bicg_iters;                    /* [nc] iterations of the last solve or NULL  */
+    /* per-node body-force field [n] added to P e (Guo force 2 rho (P e + g)),
+     * NULL = none; not in the reference, used by the gradient checks */
+    const double *gbx, *gby, *gbz;
 } orc_state;
 
 /* ---- lattice ---------------------------------------------------------- */

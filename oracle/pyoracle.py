@@ -53,6 +53,7 @@ class OrcState(C.Structure):
         ("q", c_double_p), ("qi", c_double_p), ("gs_scale", c_double_p),
         ("volume0", c_double_p), ("volume", c_double_p), ("area", c_double_p), ("gx", c_double_p),
         ("bicg_iters", c_int_p),
+        ("gbx", c_double_p), ("gby", c_double_p), ("gbz", c_double_p),
     ]
 
 
@@ -292,6 +293,35 @@ class OracleSim:
         k = C.c_int(0)
         rc = lib().orc_step(C.byref(self.P), C.byref(self.S), it, C.byref(k))
         return rc, k.value
+
+    def set_body_force_field(self, gx=None, gy=None, gz=None):
+        """Per-node body-force field (padded [n]) added to P e; None removes it."""
+        if gx is None:
+            self._gb = None
+            self.S.gbx = self.S.gby = self.S.gbz = None
+            return
+        self._gb = [np.ascontiguousarray(a, np.float64).reshape(-1).copy() for a in (gx, gy, gz)]
+        self.S.gbx, self.S.gby, self.S.gbz = (dp(a) for a in self._gb)
+
+    def wall_shear(self, iter0, nsteps, window):
+        """Mean over the last `window` steps of tau_w = 2 Df (dx/dt) (e . u) at
+        the first fluid nodes of both y walls (main.cu loop with M = 1)."""
+        P = self.P
+        e = np.array([P.ex, P.ey, P.ez], float)
+        e = e / np.linalg.norm(e) if np.linalg.norm(e) > 0 else np.array([1.0, 0.0, 0.0])
+        shape = (self.nz, self.ny, self.nx)
+        coef = 2.0 * P.df * P.dx / P.dt / (2.0 * (self.nx - 4) * (self.nz - 4)) / window
+        J = 0.0
+        for k in range(nsteps):
+            rc, _ = self.step(iter0 + k + 1)
+            if rc != 0:
+                raise RuntimeError(f"oracle step failed ({rc})")
+            if k >= nsteps - window:
+                for iy in (2, self.ny - 3):
+                    u = sum(e[d] * getattr(self, f)[...].reshape(shape)[2:-2, iy, 2:-2].sum()
+                            for d, f in enumerate(("um", "vm", "wm")))
+                    J += coef * u
+        return J
 
 
 def bicgstab(ptr, index, value, b, x0=None, itrmax=1000, residual=1e-12):

@@ -51,6 +51,10 @@ template <typename T> __global__ void k_convert_to_double(const T *__restrict__ 
     if (i < n) b[i] = (double)a[i];
 }
 
+}  // namespace
+
+namespace lbmibm {
+
 lbmibm_status dev_alloc(lbmibm_ctx *ctx, void **p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) return LBMIBM_OK;
@@ -84,13 +88,13 @@ lbmibm_status get_T(lbmibm_ctx *ctx, double *dst, const void *src, size_t n) {
     return LBMIBM_OK;
 }
 
-inline long long padded_index(const lbmibm_ctx *ctx, int ix, int iy, int iz) {
+long long padded_index(const lbmibm_ctx *ctx, int ix, int iy, int iz) {
     return (long long)(ix + 2) + (long long)ctx->nxp * ((iy + 2) + (long long)ctx->nyp * (iz + 2));
 }
 
 // Source node of direction q at fluid node (ix,iy,iz) for the pull, or -1
 // when it lies beyond a wall.
-inline long long pull_source(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q) {
+long long pull_source(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q) {
     int sx = ix - CXh[q], sy = iy - CYh[q], sz = iz - CZh[q];
     sx = (sx + ctx->nx) % ctx->nx;
     sz = (sz + ctx->nz) % ctx->nz;
@@ -99,7 +103,7 @@ inline long long pull_source(const lbmibm_ctx *ctx, int ix, int iy, int iz, int 
     return (long long)sx + (long long)ctx->nx * (sy + (long long)ctx->ny * sz);
 }
 
-inline long long push_target(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q) {
+long long push_target(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q) {
     int tx = ix + CXh[q], ty = iy + CYh[q], tz = iz + CZh[q];
     tx = (tx + ctx->nx) % ctx->nx;
     tz = (tz + ctx->nz) % ctx->nz;
@@ -163,6 +167,10 @@ lbmibm_status enqueue_step(lbmibm_ctx *ctx, int iter, bool device_advance) {
     return s;
 }
 
+}  // namespace lbmibm
+
+namespace {
+
 void free_all(lbmibm_ctx *ctx) {
     void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F, ctx->U, ctx->Ui, ctx->rho, ctx->rhom, ctx->voff, ctx->eoff,
                     ctx->vcell, ctx->ele, ctx->ec, ctx->nln, ctx->ln, ctx->lo8, ctx->nle, ctx->le, ctx->xs,
@@ -172,7 +180,10 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
                     ctx->tile_off, ctx->tile_cur, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
                     ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv,
-                    ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair, ctx->srec, ctx->onchip_cells};
+                    ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair,
+                    ctx->srec, ctx->onchip_cells, ctx->Hb, ctx->adj_f[0], ctx->adj_f[1], ctx->adj_u, ctx->adj_F,
+                    ctx->adj_x, ctx->adj_cf, ctx->grad_H, ctx->grad_gs, ctx->adj_lam, ctx->adj_dsv,
+                    ctx->adj_corner, ctx->adj_scratch};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);

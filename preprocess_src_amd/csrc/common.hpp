@@ -61,6 +61,24 @@ struct lbmibm_ctx {
     void *rho = nullptr;              // [N] instantaneous density (store_macros)
     void *rhom = nullptr;             // [N] accumulated density dm (store_macros)
     bool force_zero = true;           // lattice force known to be identically zero
+    void *Hb = nullptr;               // [3][N] per-node body-force field (T), or none
+
+    // adjoint state (allocated on first use): adjoints of the post-collision
+    // populations, the velocity field, the lattice force, the vertex
+    // positions and nodal forces; accumulated gradients with respect to the
+    // body-force field and the per-cell force scales
+    void *adj_f[2] = {nullptr, nullptr};  // [19][N] (T); adj_f[adj_cur] is current
+    int adj_cur = 0;
+    void *adj_u = nullptr;            // [3][N] (T)
+    void *adj_F = nullptr;            // [3][N] (T)
+    double *adj_x = nullptr;          // [3][Nv]
+    void *adj_cf = nullptr;           // [3][Nv] (T)
+    double *grad_H = nullptr;         // [3][N]
+    double *grad_gs = nullptr;        // [nc]
+    void *adj_lam = nullptr;          // [3Nv] (T) adjoint load of the membrane solve
+    double *adj_dsv = nullptr;        // [Nv] adjoint of ds x force scale
+    double *adj_corner = nullptr;     // [6Nv][4] per-(vertex, slot) position adjoints
+    void *adj_scratch = nullptr;      // [3][Nv] (T) discarded solver epilogue output
 
     // cells (device): positions, metric and bending reference in FP64; the
     // other membrane arrays in the arithmetic type T
@@ -129,16 +147,32 @@ lbmibm_status lattice_step(lbmibm_ctx *ctx);                 // lattice.hip
 lbmibm_status lattice_set_state_from_fstar(lbmibm_ctx *ctx, const double *fstar_soa);  // lattice.hip
 lbmibm_status lattice_get_fstar(lbmibm_ctx *ctx, double *fstar_soa);  // lattice.hip
 lbmibm_status lattice_fill(lbmibm_ctx *ctx, void *ptr, size_t count, double value, bool storage_type);
+lbmibm_status lattice_adjoint(lbmibm_ctx *ctx, const void *ga_out, void *ga_in);  // lattice.hip
 lbmibm_status ibm_spread(lbmibm_ctx *ctx);                   // ibm.hip
 lbmibm_status ibm_interp(lbmibm_ctx *ctx);                   // ibm.hip
 lbmibm_status ibm_bin(lbmibm_ctx *ctx);                      // ibm.hip
 lbmibm_status ibm_setup(lbmibm_ctx *ctx);                    // ibm.hip
+lbmibm_status ibm_spread_adjoint(lbmibm_ctx *ctx);           // ibm.hip
+lbmibm_status ibm_interp_adjoint(lbmibm_ctx *ctx);           // ibm.hip
 lbmibm_status membrane_forces(lbmibm_ctx *ctx);              // membrane.hip
+// have_q: q (qv) already holds the load at the current positions
+lbmibm_status membrane_forces_adjoint(lbmibm_ctx *ctx, bool have_q);  // membrane.hip
 int bicg_class(int prec, int nvert);                         // membrane.hip
 lbmibm_status membrane_setup(lbmibm_ctx *ctx);               // membrane.hip
 lbmibm_status membrane_autotune(lbmibm_ctx *ctx);            // membrane.hip
 lbmibm_status volume_run(lbmibm_ctx *ctx, int mode);         // volume.hip (0 init, 1 correct)
 lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool advance);  // api.hip
+// api.hip helpers shared with adjoint.hip
+lbmibm_status dev_alloc(lbmibm_ctx *ctx, void **p, size_t bytes);
+lbmibm_status put_T(lbmibm_ctx *ctx, void *dst, const double *src, size_t n);  // host double -> device T
+lbmibm_status get_T(lbmibm_ctx *ctx, double *dst, const void *src, size_t n);  // device T -> host double
+long long padded_index(const lbmibm_ctx *ctx, int ix, int iy, int iz);
+long long pull_source(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q);  // -1 beyond a wall
+long long push_target(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q);  // -1 beyond a wall
+lbmibm_status sync_check(lbmibm_ctx *ctx);
+lbmibm_status check_flags(lbmibm_ctx *ctx, int *vol_iters);
+lbmibm_status reset_status(lbmibm_ctx *ctx);
+lbmibm_status enqueue_step(lbmibm_ctx *ctx, int iter, bool device_advance);
 
 inline lbmibm_status hip_fail(lbmibm_ctx *ctx, hipError_t e, const char *what) {
     ctx->err = std::string(what) + ": " + hipGetErrorString(e);

@@ -706,6 +706,170 @@ __global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, d
     }
 }
 
+// ---- adjoints (vector-Jacobian products) ------------------------------------
+//
+// The delta weights are C^1 in the position: with t = frac(xtr/h) the four
+// weights are (1 - sin, 1 + cos, 1 + sin, 1 - cos)(pi t/2), and their
+// derivatives (pi/2h) (-cos, -sin, cos, sin)(pi t/2) agree at t = 0 and 1
+// with those of the neighbouring stencil, so the stencil switch adds no term.
+// Both adjoints walk the 4x4x4 stencil of each vertex in global memory (one
+// thread per vertex): the spread adjoint is a gather, the advection adjoint
+// gathers for the position terms and adds the velocity adjoint to the nodes
+// with atomics.
+
+template <typename W>
+__device__ __forceinline__ void stencil_d(double xtr, double h, int &first, W w[4], W dw[4]) {
+    const double q = xtr / h;
+    const double fl = floor(q);
+    first = (int)fl - 1;
+    const W t = (W)(q - fl);
+    W sn, cs;
+    sincos_quarter(t, &sn, &cs);
+    w[0] = W(1) - sn;
+    w[1] = W(1) + cs;
+    w[2] = W(1) + sn;
+    w[3] = W(1) - cs;
+    const W k = (W)(M_PI / (2.0 * h));
+    dw[0] = -k * cs;
+    dw[1] = -k * sn;
+    dw[2] = k * cs;
+    dw[3] = k * sn;
+}
+
+// Stencil of a position as stencil3 (spacing dx) with weight derivatives.
+template <typename W> struct StencilD {
+    int x1, y1, z1;
+    W wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+    __device__ __forceinline__ StencilD(const IbmGeom &g, double x, double y, double z) {
+        stencil_d<W>(x + (double)(g.nxp - 1) * g.dx / 2.0, g.dx, x1, wx, dx);
+        stencil_d<W>(y + (double)(g.nyp - 1) * g.dx / 2.0, g.dx, y1, wy, dy);
+        stencil_d<W>(z + (double)(g.nzp - 1) * g.dx / 2.0, g.dx, z1, wz, dz);
+        x1 -= 2;
+        y1 -= 2;
+        z1 -= 2;
+    }
+    // visits every stencil node inside the lattice: f(node index, weight
+    // product, its gradient with respect to the position)
+    template <typename Fn> __device__ __forceinline__ void each(const IbmGeom &g, Fn f) const {
+        for (int kz = 0; kz < 4; kz++) {
+            int iz = z1 + kz;
+            if (!wrap_index(iz, g.nz)) continue;
+            for (int ky = 0; ky < 4; ky++) {
+                int iy = y1 + ky;
+                if (g.walls) {
+                    if (iy < 0 || iy >= g.ny) continue;
+                } else if (!wrap_index(iy, g.ny)) continue;
+#pragma unroll
+                for (int kx = 0; kx < 4; kx++) {
+                    int ix = x1 + kx;
+                    if (!wrap_index(ix, g.nx)) continue;
+                    const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
+                    const W w = wx[kx] * wy[ky] * wz[kz];
+                    const W gx = dx[kx] * wy[ky] * wz[kz], gy = wx[kx] * dy[ky] * wz[kz], gz = wx[kx] * wy[ky] * dz[kz];
+                    f(n, w, gx, gy, gz);
+                }
+            }
+        }
+    }
+};
+
+// Adjoint of ibm_st1 at the current positions and nodal forces: given Fa (the
+// adjoint of the lattice force), cfa = sum D Fa is the adjoint of the nodal
+// force and xa += sum (Fa . f) grad D that of the position, D = phi^3/dx^3.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spread_adj(const double *__restrict__ xs, const T *__restrict__ cf,
+                                                    const T *__restrict__ Fa, T *__restrict__ cfa,
+                                                    double *__restrict__ xa, int nv, IbmGeom g) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nv) return;
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    const StencilD<T> st(g, xs[j], xs[nv + j], xs[2 * (size_t)nv + j]);
+    const T f0 = cf[j], f1 = cf[nv + j], f2 = cf[2 * (size_t)nv + j];
+    const T w64 = (T)(1.0 / (64.0 * g.dx * g.dx * g.dx));
+    T b0 = T(0), b1 = T(0), b2 = T(0), x0 = T(0), x1 = T(0), x2 = T(0);
+    st.each(g, [&](size_t n, T w, T gx, T gy, T gz) {
+        const T a0 = Fa[n], a1 = Fa[N + n], a2 = Fa[2 * N + n];
+        b0 += w * a0; b1 += w * a1; b2 += w * a2;
+        const T s = a0 * f0 + a1 * f1 + a2 * f2;
+        x0 += s * gx; x1 += s * gy; x2 += s * gz;
+    });
+    cfa[j] = b0 * w64; cfa[nv + j] = b1 * w64; cfa[2 * (size_t)nv + j] = b2 * w64;
+    xa[j] += (double)(x0 * w64);
+    xa[nv + j] += (double)(x1 * w64);
+    xa[2 * (size_t)nv + j] += (double)(x2 * w64);
+}
+
+// Adjoint of ibm_st2 with RK4 at the current positions and velocity field:
+// xa holds the adjoint of the advected positions on entry and that of the
+// current positions on exit; the adjoint of the velocity field is added to
+// ua.  The stage positions are recomputed; stage s contributes
+// ua += phi^3(X_s) kb_s and X_s-bar = sum (u . kb_s) grad phi^3, which feeds
+// the stage before it (ibm.cu:430-517: X2 = X + h/2 k1, X3 = X + h/2 k2,
+// X4 = X + h k3, X' = X + h (k1 + 2 k2 + 2 k3 + k4)/6, h = M dt).
+template <typename T>
+__global__ __launch_bounds__(256) void k_advect_adj(const double *__restrict__ xs, const T *__restrict__ U,
+                                                    double *__restrict__ xa, T *__restrict__ ua, int nv,
+                                                    IbmGeom g, double h) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nv) return;
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    const double X[3] = {xs[j], xs[nv + j], xs[2 * (size_t)nv + j]};
+    auto interp = [&](const double p[3], double k[3]) {
+        const StencilD<T> st(g, p[0], p[1], p[2]);
+        T u0 = T(0), u1 = T(0), u2 = T(0);
+        st.each(g, [&](size_t n, T w, T, T, T) {
+            u0 += w * U[n]; u1 += w * U[N + n]; u2 += w * U[2 * N + n];
+        });
+        k[0] = (double)(u0 * T(1.0 / 64.0));
+        k[1] = (double)(u1 * T(1.0 / 64.0));
+        k[2] = (double)(u2 * T(1.0 / 64.0));
+    };
+    double P[4][3], k[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) P[0][d] = X[d];
+    interp(P[0], k);
+#pragma unroll
+    for (int d = 0; d < 3; d++) P[1][d] = X[d] + k[d] * h / 2.0;
+    interp(P[1], k);
+#pragma unroll
+    for (int d = 0; d < 3; d++) P[2][d] = X[d] + k[d] * h / 2.0;
+    interp(P[2], k);
+#pragma unroll
+    for (int d = 0; d < 3; d++) P[3][d] = X[d] + k[d] * h;
+    const double xb[3] = {xa[j], xa[nv + j], xa[2 * (size_t)nv + j]};
+    double acc[3] = {xb[0], xb[1], xb[2]};
+    double kb[4][3];
+    const double cw[4] = {h / 6.0, h / 3.0, h / 3.0, h / 6.0};
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) kb[s][d] = cw[s] * xb[d];
+    const double back[4] = {0.0, h / 2.0, h / 2.0, h};  // X_s = X + back[s] k_{s-1}
+    for (int s = 3; s >= 0; s--) {
+        const StencilD<T> st(g, P[s][0], P[s][1], P[s][2]);
+        const T k0 = (T)kb[s][0], k1 = (T)kb[s][1], k2 = (T)kb[s][2];
+        const T q64 = T(1.0 / 64.0);
+        T g0 = T(0), g1 = T(0), g2 = T(0);
+        st.each(g, [&](size_t n, T w, T gx, T gy, T gz) {
+            const T wq = w * q64;
+            atomicAdd(&ua[n], wq * k0);
+            atomicAdd(&ua[N + n], wq * k1);
+            atomicAdd(&ua[2 * N + n], wq * k2);
+            const T sdot = U[n] * k0 + U[N + n] * k1 + U[2 * N + n] * k2;
+            g0 += sdot * gx; g1 += sdot * gy; g2 += sdot * gz;
+        });
+        const double xs_[3] = {(double)(g0 * q64), (double)(g1 * q64), (double)(g2 * q64)};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            acc[d] += xs_[d];
+            if (s > 0) kb[s - 1][d] += back[s] * xs_[d];
+        }
+    }
+    xa[j] = acc[0];
+    xa[nv + j] = acc[1];
+    xa[2 * (size_t)nv + j] = acc[2];
+}
+
 IbmGeom geom_of(const lbmibm_ctx *ctx) {
     IbmGeom g;
     g.nx = ctx->nx; g.ny = ctx->ny; g.nz = ctx->nz;
@@ -773,7 +937,37 @@ template <typename T> lbmibm_status interp_impl(lbmibm_ctx *ctx) {
     return LBMIBM_OK;
 }
 
+template <typename T> lbmibm_status spread_adj_impl(lbmibm_ctx *ctx) {
+    if (ctx->nv == 0) return LBMIBM_OK;
+    hipLaunchKernelGGL(k_spread_adj<T>, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const double *)ctx->xs, (const T *)ctx->cf, (const T *)ctx->adj_F, (T *)ctx->adj_cf,
+                       ctx->adj_x, ctx->nv, geom_of(ctx));
+    LBMIBM_LAUNCHED(ctx, "k_spread_adj");
+    return LBMIBM_OK;
+}
+
+template <typename T> lbmibm_status advect_adj_impl(lbmibm_ctx *ctx) {
+    if (ctx->nv == 0) return LBMIBM_OK;
+    hipLaunchKernelGGL(k_advect_adj<T>, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const double *)ctx->xs, (const T *)ctx->U, ctx->adj_x, (T *)ctx->adj_u, ctx->nv,
+                       geom_of(ctx), ctx->p.m * ctx->p.dt);
+    LBMIBM_LAUNCHED(ctx, "k_advect_adj");
+    return LBMIBM_OK;
+}
+
 }  // namespace
+
+lbmibm_status ibm_spread_adjoint(lbmibm_ctx *ctx) {
+    return ctx->prec == LBMIBM_F64 ? spread_adj_impl<double>(ctx) : spread_adj_impl<float>(ctx);
+}
+
+lbmibm_status ibm_interp_adjoint(lbmibm_ctx *ctx) {
+    if (ctx->p.runge_kutta != 4) {
+        ctx->err = "adjoint: the advection adjoint supports RUNGE_KUTTA = 4";
+        return LBMIBM_ERR_STATE;
+    }
+    return ctx->prec == LBMIBM_F64 ? advect_adj_impl<double>(ctx) : advect_adj_impl<float>(ctx);
+}
 
 lbmibm_status ibm_setup(lbmibm_ctx *ctx) {
     const int lds64 = (int)(3 * (size_t)RN * sizeof(double));

@@ -53,9 +53,10 @@ struct LbmDims {
 
 template <typename S, typename T, bool HASF, bool MACROS, bool WALLS>
 __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__restrict__ fout, LbmDims d,
-                                            const T *__restrict__ F, T *__restrict__ U, T *__restrict__ Ui,
-                                            T *__restrict__ rho_out, T *__restrict__ rhom, T om, T inv_cr,
-                                            T pex, T pey, T pez, T B, const DevStep *__restrict__ st, int m) {
+                                            const T *__restrict__ F, const T *__restrict__ H, T *__restrict__ U,
+                                            T *__restrict__ Ui, T *__restrict__ rho_out, T *__restrict__ rhom, T om,
+                                            T inv_cr, T pex, T pey, T pez, T B, const DevStep *__restrict__ st,
+                                            int m) {
     const int lb = xcd_block(d.nblocks);
     if (lb >= d.nblocks) return;
     const unsigned i = (unsigned)lb * WG + threadIdx.x;
@@ -104,6 +105,11 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         ay += B * F[N + i];
         az += B * F[2 * N + i];
     }
+    if (H) {  // per-node body-force field, same convention as P e
+        ax += H[i];
+        ay += H[N + i];
+        az += H[2 * N + i];
+    }
     const T irho = T(1) / rho;
     // U = (j + G/2)/rho with the Guo force G = 2 rho (P e + B F) (lbm.cu:1116-1118)
     const T Ux = jx * irho + ax, Uy = jy * irho + ay, Uz = jz * irho + az;
@@ -146,6 +152,127 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
     }
 }
 
+// Vector-Jacobian product of one k_lbm step with M = 1, taken at the state
+// the step would run from (pre-step populations fin, lattice force F, body
+// force field H).  Given the adjoint of the post-collision populations ga_out
+// and of the velocity output ua (u = U/Cr), it forms the adjoint of the pulled
+// pre-collision populations at every node and writes each to the location
+// the forward pulled it from, so ga_in (the adjoint of fin) is written once
+// per entry, without atomics.  The adjoint of the acceleration a = P e amp +
+// B F + H gives Fa = B abar and is added to Ha.  Derivatives of the BGK
+// equilibrium and the Guo source as written in k_lbm (drho, j, U = j/rho + a,
+// kf = (2 - om) rho).
+template <typename S, typename T, bool WALLS>
+__global__ __launch_bounds__(WG) void k_lbm_adj(const S *__restrict__ fin, const T *__restrict__ ga_out,
+                                                const T *__restrict__ ua, T *__restrict__ ga_in, LbmDims d,
+                                                const T *__restrict__ F, const T *__restrict__ H,
+                                                T *__restrict__ Fa, double *__restrict__ Ha, T om, T inv_cr,
+                                                T pex, T pey, T pez, T B, const DevStep *__restrict__ st) {
+    const int lb = xcd_block(d.nblocks);
+    if (lb >= d.nblocks) return;
+    const unsigned i = (unsigned)lb * WG + threadIdx.x;
+    if (i >= d.N) return;
+    const unsigned nx = d.nx, ny = d.ny, nz = d.nz;
+    const size_t N = d.N;
+    const unsigned ix = i % nx, r = i / nx, iy = r % ny, iz = r / ny;
+    const unsigned xm = ix == 0 ? nx - 1 : ix - 1, xp = ix == nx - 1 ? 0 : ix + 1;
+    const unsigned zm = iz == 0 ? nz - 1 : iz - 1, zp = iz == nz - 1 ? 0 : iz + 1;
+    bool wall_lo = false, wall_hi = false;
+    unsigned ym, yp;
+    if (WALLS) {
+        wall_lo = iy == 0;
+        wall_hi = iy == ny - 1;
+        ym = iy - 1;
+        yp = iy + 1;
+    } else {
+        ym = iy == 0 ? ny - 1 : iy - 1;
+        yp = iy == ny - 1 ? 0 : iy + 1;
+    }
+    T g[Q];
+    size_t src[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const unsigned sx = cx(q) == 1 ? xm : (cx(q) == -1 ? xp : ix);
+        const unsigned sy = cy(q) == 1 ? ym : (cy(q) == -1 ? yp : iy);
+        const unsigned sz = cz(q) == 1 ? zm : (cz(q) == -1 ? zp : iz);
+        const bool bb = WALLS && ((cy(q) == 1 && wall_lo) || (cy(q) == -1 && wall_hi));
+        src[q] = bb ? (size_t)opp(q) * N + i : (size_t)q * N + (sx + nx * (sy + ny * sz));
+        g[q] = Pop<S, T>::ld(fin + src[q], bb ? opp(q) : q);
+    }
+    T drho = T(0), jx = T(0), jy = T(0), jz = T(0);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        drho += g[q];
+        if (cx(q)) jx += cx(q) * g[q];
+        if (cy(q)) jy += cy(q) * g[q];
+        if (cz(q)) jz += cz(q) * g[q];
+    }
+    const T rho = T(1) + drho;
+    const T amp = (T)st->amp;
+    T ax = pex * amp, ay = pey * amp, az = pez * amp;
+    if (F) {
+        ax += B * F[i];
+        ay += B * F[N + i];
+        az += B * F[2 * N + i];
+    }
+    if (H) {
+        ax += H[i];
+        ay += H[N + i];
+        az += H[2 * N + i];
+    }
+    const T irho = T(1) / rho;
+    const T Ux = jx * irho + ax, Uy = jy * irho + ay, Uz = jz * irho + az;
+    const T uu = Ux * Ux + Uy * Uy + Uz * Uz;
+    const T Ua = Ux * ax + Uy * ay + Uz * az;
+    const T kf = (T(2) - om) * rho;
+    // adjoints: of drho, U (from the velocity output first), a
+    T drb = T(0);
+    T Ubx = ua[i] * inv_cr, Uby = ua[N + i] * inv_cr, Ubz = ua[2 * N + i] * inv_cr;
+    T abx = T(0), aby = T(0), abz = T(0);
+    T gb[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const T go = ga_out[(size_t)q * N + i];
+        gb[q] = (T(1) - om) * go;
+        const T cu = (cx(q) ? cx(q) * Ux : T(0)) + (cy(q) ? cy(q) * Uy : T(0)) + (cz(q) ? cz(q) * Uz : T(0));
+        const T ca = (cx(q) ? cx(q) * ax : T(0)) + (cy(q) ? cy(q) * ay : T(0)) + (cz(q) ? cz(q) * az : T(0));
+        const T w = wq<T>(q);
+        const T eb = om * go * w;  // adjoint of geq / w
+        const T sb = go * w;       // adjoint of src / w
+        const T Sq = T(3) * ca * (T(1) + T(3) * cu) - T(3) * Ua;
+        drb += eb * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * uu) + sb * (T(2) - om) * Sq;
+        // geq: rho (3 c + 9 cu c - 3 U); src: kf (9 ca c - 3 a) for U, kf (3 c (1 + 3 cu) - 3 U) for a
+        const T kc = eb * rho * (T(3) + T(9) * cu) + sb * kf * T(9) * ca;
+        const T ka = sb * kf * T(3) * (T(1) + T(3) * cu);
+        if (cx(q)) { Ubx += cx(q) * kc; abx += cx(q) * ka; }
+        if (cy(q)) { Uby += cy(q) * kc; aby += cy(q) * ka; }
+        if (cz(q)) { Ubz += cz(q) * kc; abz += cz(q) * ka; }
+        const T ku = eb * rho * T(3), kv = sb * kf * T(3);
+        Ubx -= ku * Ux + kv * ax; Uby -= ku * Uy + kv * ay; Ubz -= ku * Uz + kv * az;
+        abx -= kv * Ux; aby -= kv * Uy; abz -= kv * Uz;
+    }
+    // U = j / rho + a
+    abx += Ubx; aby += Uby; abz += Ubz;
+    const T jbx = Ubx * irho, jby = Uby * irho, jbz = Ubz * irho;
+    drb -= (jbx * jx + jby * jy + jbz * jz) * irho;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        T v = gb[q] + drb;
+        if (cx(q)) v += cx(q) * jbx;
+        if (cy(q)) v += cy(q) * jby;
+        if (cz(q)) v += cz(q) * jbz;
+        ga_in[src[q]] = v;
+    }
+    Fa[i] = B * abx;
+    Fa[N + i] = B * aby;
+    Fa[2 * N + i] = B * abz;
+    if (Ha) {
+        Ha[i] += (double)abx;
+        Ha[N + i] += (double)aby;
+        Ha[2 * N + i] += (double)abz;
+    }
+}
+
 template <typename S, typename T>
 __global__ void k_from_fstar(const double *__restrict__ fs, S *__restrict__ f, unsigned N) {
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,7 +307,7 @@ template <typename S, typename T> lbmibm_status step_impl(lbmibm_ctx *ctx) {
     const bool hasf = !ctx->force_zero, macros = p.store_macros != 0, walls = p.periodic_y == 0;
     auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, ctx->stream, fin, fout, d, (const T *)ctx->F,
-                           (T *)ctx->U, (T *)ctx->Ui, (T *)ctx->rho, (T *)ctx->rhom, (T)om, (T)(1.0 / p.cr),
+                           (const T *)ctx->Hb, (T *)ctx->U, (T *)ctx->Ui, (T *)ctx->rho, (T *)ctx->rhom, (T)om, (T)(1.0 / p.cr),
                            (T)(p.p * p.ex), (T)(p.p * p.ey), (T)(p.p * p.ez), (T)p.b, ctx->step, p.m);
     };
 #define LBM_CASE(HF, MC, WL) \
@@ -190,6 +317,29 @@ template <typename S, typename T> lbmibm_status step_impl(lbmibm_ctx *ctx) {
 #undef LBM_CASE
     LBMIBM_LAUNCHED(ctx, "k_lbm");
     ctx->cur ^= 1;  // pointer swap of lbm.cu:1575
+    return LBMIBM_OK;
+}
+
+// Adjoint of the lattice step from the current state into ga_in (separate
+// from ga_out); the forward state is left unchanged.
+template <typename S, typename T> lbmibm_status adj_step_impl(lbmibm_ctx *ctx, const void *ga_out, void *ga_in) {
+    const lbmibm_params &p = ctx->p;
+    LbmDims d;
+    d.N = (unsigned)ctx->N;
+    d.nx = ctx->nx; d.ny = ctx->ny; d.nz = ctx->nz;
+    d.nblocks = (int)((ctx->N + WG - 1) / WG);
+    const int grid = xcd_grid(d.nblocks);
+    const double om = 1.0 / (3.0 * p.df + 0.5);
+    const S *fin = (const S *)ctx->f[ctx->cur];
+    const T *F = ctx->force_zero ? nullptr : (const T *)ctx->F;
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, ctx->stream, fin, (const T *)ga_out, (const T *)ctx->adj_u,
+                           (T *)ga_in, d, F, (const T *)ctx->Hb, (T *)ctx->adj_F, ctx->grad_H, (T)om,
+                           (T)(1.0 / p.cr), (T)(p.p * p.ex), (T)(p.p * p.ey), (T)(p.p * p.ez), (T)p.b, ctx->step);
+    };
+    if (p.periodic_y == 0) launch(k_lbm_adj<S, T, true>);
+    else launch(k_lbm_adj<S, T, false>);
+    LBMIBM_LAUNCHED(ctx, "k_lbm_adj");
     return LBMIBM_OK;
 }
 
@@ -218,6 +368,14 @@ lbmibm_status lattice_step(lbmibm_ctx *ctx) {
         case LBMIBM_F16: return step_impl<__half, float>(ctx);
     }
     return LBMIBM_ERR_ARG;
+}
+
+lbmibm_status lattice_adjoint(lbmibm_ctx *ctx, const void *ga_out, void *ga_in) {
+    switch (ctx->prec) {
+        case LBMIBM_F64: return adj_step_impl<double, double>(ctx, ga_out, ga_in);
+        case LBMIBM_F32: return adj_step_impl<float, float>(ctx, ga_out, ga_in);
+        default: ctx->err = "adjoint: FP16 population storage has no adjoint"; return LBMIBM_ERR_STATE;
+    }
 }
 
 lbmibm_status lattice_set_state_from_fstar(lbmibm_ctx *ctx, const double *host_fs) {

@@ -592,15 +592,19 @@ __device__ __forceinline__ int opaque(int v) {
     asm volatile("" : "+v"(v));
     return v;
 }
+__device__ __forceinline__ double wide(float v) {
+    asm volatile("" : "+v"(v));
+    return (double)v;
+}
 
-template <int K>
+template <int K, int KB, bool VREG>
 __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells, const int *__restrict__ voff,
                                                    const float *__restrict__ coef,
                                                    const unsigned short *__restrict__ nbr,
                                                    const float *__restrict__ bv, float *__restrict__ x,
-                                                   float *__restrict__ pg, SolveOut<float> out, int nv, int itrmax,
-                                                   double residual, int *__restrict__ iters,
-                                                   DevStatus *__restrict__ status) {
+                                                   float *__restrict__ pg, float *__restrict__ vg,
+                                                   SolveOut<float> out, int nv, int itrmax, double residual,
+                                                   int *__restrict__ iters, DevStatus *__restrict__ status) {
     extern __shared__ double dyn[];
     __shared__ double sh[128];
     WgReduce red{sh, 0};
@@ -608,10 +612,21 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
     const int c = cells[blockIdx.x];
     const int v0 = voff[c], V = voff[c + 1] - v0;
     float *L = (float *)dyn;  // [V][3]
-    float rt[K][3], vr[K][3];  // r (then t), v
-    // y = A L at local vertex j (global index g); 32-bit offsets from the
-    // uniform bases
-    auto spmv = [&](int j, unsigned g, float y[3]) {
+    // r (then t) and, with VREG, v per slot; zero for the slots j >= V
+    float rt[K][3], vr[VREG ? K : 1][3];
+    // Slot k of this thread is vertex j = tid + k*1024.  Loads use the index
+    // clamped to V-1 so that they carry no branch; stores and sums are
+    // predicated on j < V.  A scheduling fence after every KB slots bounds
+    // how far the compiler hoists loads (all K slots at once would spill).
+    // Without VREG, v lives in memory (vg, owner thread only): 36 B more
+    // traffic per vertex and iteration, but registers for more loads in
+    // flight.  wide() converts without letting the compiler keep the FP64
+    // copies of the slots alive from one phase to the next.
+    auto fence = [](int k) {
+        if ((k + 1) % KB == 0) __builtin_amdgcn_sched_barrier(0);
+    };
+    // y = A L at vertex jc (global index g); 32-bit offsets from the uniform bases
+    auto spmv = [&](int jc, unsigned g, float y[3]) {
         float cm[6];
         int col[6];
 #pragma unroll
@@ -620,7 +635,7 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
             col[m] = nbr[(unsigned)m * (unsigned)nv + g];
         }
 #pragma unroll
-        for (int d = 0; d < 3; d++) y[d] = L[3 * j + d];
+        for (int d = 0; d < 3; d++) y[d] = L[3 * jc + d];
 #pragma unroll
         for (int m = 0; m < 6; m++)
 #pragma unroll
@@ -630,17 +645,15 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = threadIdx.x + k * NT;
+        const bool ok = j < V;
+        const unsigned g = (unsigned)(v0 + min(j, V - 1));
 #pragma unroll
-        for (int d = 0; d < 3; d++) { rt[k][d] = 0.0f; vr[k][d] = 0.0f; }
-        if (j < V) {
-            const unsigned g = (unsigned)(v0 + j);
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const float b = bv[3 * g + d];
-                x[3 * g + d] = 0.0f;
-                rt[k][d] = b;
-                acc[0] += (double)b * (double)b;
-            }
+        for (int d = 0; d < 3; d++) {
+            const float b = ok ? bv[3 * g + d] : 0.0f;
+            if (ok) x[3 * g + d] = 0.0f;
+            rt[k][d] = b;
+            if (VREG) vr[k][d] = 0.0f;
+            acc[0] += (double)b * (double)b;
         }
     }
     red.sum<1>(*(double(*)[1])acc);
@@ -657,23 +670,27 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
     for (it = 0; it < itrmax; it++) {
         if (rr == 0.0) { flag = 1; break; }
         // p = r (first pass) or r + beta (p - omeg v), into LDS and memory
-        const double beta = it > 0 ? (rr / rr_old) * (alfa / omeg) : 0.0;
+        const bool first = it == 0;
+        const double beta = first ? 0.0 : (rr / rr_old) * (alfa / omeg);
         {
             const int t0 = opaque(threadIdx.x), b0 = opaque(v0);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int j = t0 + k * NT;
-                if (j < V) {
-                    const unsigned g = (unsigned)(b0 + j);
+                const bool ok = j < V;
+                const int jc = min(j, V - 1);
+                const unsigned g = (unsigned)(b0 + jc);
 #pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        float pn = rt[k][d];
-                        if (it > 0)
-                            pn = (float)((double)rt[k][d] + beta * ((double)pg[3 * g + d] - omeg * (double)vr[k][d]));
+                for (int d = 0; d < 3; d++) {
+                    const float po = pg[3 * g + d];
+                    const float vo = VREG ? vr[k][d] : vg[3 * g + d];
+                    const float pn = first ? rt[k][d] : (float)(wide(rt[k][d]) + beta * ((double)po - omeg * wide(vo)));
+                    if (ok) {
                         pg[3 * g + d] = pn;
                         L[3 * j + d] = pn;
                     }
                 }
+                fence(k);
             }
         }
         __syncthreads();
@@ -684,25 +701,38 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int j = t0 + k * NT;
-                if (j < V) {
-                    const unsigned g = (unsigned)(b0 + j);
-                    spmv(j, g, vr[k]);
+                const bool ok = j < V;
+                const int jc = min(j, V - 1);
+                const unsigned g = (unsigned)(b0 + jc);
+                float y[3];
+                spmv(jc, g, y);
 #pragma unroll
-                    for (int d = 0; d < 3; d++) acc[0] += (double)vr[k][d] * (double)bv[3 * g + d];
+                for (int d = 0; d < 3; d++) {
+                    const float vn = ok ? y[d] : 0.0f;
+                    if (VREG) vr[k][d] = vn;
+                    else if (ok) vg[3 * g + d] = vn;
+                    acc[0] += wide(vn) * (double)bv[3 * g + d];
                 }
+                fence(k);
             }
         }
         red.sum<1>(*(double(*)[1])acc);  // its barrier ends the reads of p
         alfa = rr / acc[0];
         // s = r - alfa v into LDS
         {
-            const int t0 = opaque(threadIdx.x);
+            const int t0 = opaque(threadIdx.x), b0 = opaque(v0);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int j = t0 + k * NT;
-                if (j < V)
+                const int jc = min(j, V - 1);
+                const unsigned g = (unsigned)(b0 + jc);
 #pragma unroll
-                    for (int d = 0; d < 3; d++) L[3 * j + d] = (float)((double)rt[k][d] - alfa * (double)vr[k][d]);
+                for (int d = 0; d < 3; d++) {
+                    const float vo = VREG ? vr[k][d] : vg[3 * g + d];
+                    const float sv = (float)(wide(rt[k][d]) - alfa * wide(vo));
+                    if (j < V) L[3 * j + d] = sv;
+                }
+                fence(k);
             }
         }
         __syncthreads();
@@ -713,14 +743,17 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int j = t0 + k * NT;
-                if (j < V) {
-                    spmv(j, (unsigned)(b0 + j), rt[k]);
+                const bool ok = j < V;
+                const int jc = min(j, V - 1);
+                float y[3];
+                spmv(jc, (unsigned)(b0 + jc), y);
 #pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        acc[0] += (double)L[3 * j + d] * (double)rt[k][d];
-                        acc[1] += (double)rt[k][d] * (double)rt[k][d];
-                    }
+                for (int d = 0; d < 3; d++) {
+                    rt[k][d] = ok ? y[d] : 0.0f;
+                    acc[0] += (double)L[3 * jc + d] * wide(rt[k][d]);
+                    acc[1] += wide(rt[k][d]) * wide(rt[k][d]);
                 }
+                fence(k);
             }
         }
         red.sum<2>(acc);
@@ -732,18 +765,20 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int j = t0 + k * NT;
-                if (j < V) {
-                    const unsigned g = (unsigned)(b0 + j);
+                const bool ok = j < V;
+                const int jc = min(j, V - 1);
+                const unsigned g = (unsigned)(b0 + jc);
 #pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        const double si = (double)L[3 * j + d];
-                        x[3 * g + d] = (float)((double)x[3 * g + d] + alfa * (double)pg[3 * g + d] + omeg * si);
-                        const float rn = (float)(si - omeg * (double)rt[k][d]);
-                        rt[k][d] = rn;
-                        acc[0] += (double)rn * (double)rn;
-                        acc[1] += (double)rn * (double)bv[3 * g + d];
-                    }
+                for (int d = 0; d < 3; d++) {
+                    const double si = (double)L[3 * jc + d];
+                    const float xn = (float)((double)x[3 * g + d] + alfa * (double)pg[3 * g + d] + omeg * si);
+                    if (ok) x[3 * g + d] = xn;
+                    const float rn = ok ? (float)(si - omeg * wide(rt[k][d])) : 0.0f;
+                    rt[k][d] = rn;
+                    acc[0] += (double)rn * (double)rn;
+                    acc[1] += (double)rn * (double)bv[3 * g + d];
                 }
+                fence(k);
             }
         }
         red.sum<2>(acc);  // its barrier ends the reads of s before p is written
@@ -1029,8 +1064,9 @@ lbmibm_status launch_onchip(lbmibm_ctx *ctx, int cls, int var, const SolveOut<T>
 
 // Forms of the large-cell class: 0 = vectors in global memory, 1 = p in LDS,
 // 2 = register-resident (FP32; 1 and 2 only when the largest cell fits).
-constexpr int BICG_NFORM = 3;
-constexpr const char *BICG_FORM_NAMES[BICG_NFORM] = {"global", "p-in-LDS", "resident"};
+constexpr int BICG_NFORM = 6;
+constexpr const char *BICG_FORM_NAMES[BICG_NFORM] = {"global", "p-in-LDS", "resident/2", "resident/4",
+                                                     "resident-vmem/2", "resident-vmem/4"};
 
 bool big_form_fits(const lbmibm_ctx *ctx, int form) {
     if (form == 0) return true;
@@ -1043,12 +1079,20 @@ template <typename T>
 lbmibm_status launch_big(lbmibm_ctx *ctx, int var, const SolveOut<T> &out, hipStream_t stream) {
     const lbmibm_params &p = ctx->p;
     if constexpr (sizeof(T) == 4) {
-        if (var == 2 && big_form_fits(ctx, 2)) {
+        if (var >= 2 && big_form_fits(ctx, var)) {
             const size_t lds = 3 * (size_t)ctx->bicg_maxv[2] * sizeof(float);
-            hipLaunchKernelGGL(k_bicg_res<BICG_RES_K>, dim3(ctx->bicg_n[2]), dim3(1024), lds, stream,
-                               ctx->bicg_cells[2], ctx->voff, (const float *)ctx->coef, ctx->nbr,
-                               (const float *)ctx->bv, (float *)ctx->rh, (float *)ctx->pv, out, ctx->nv, p.itrmax,
-                               p.residual, ctx->bicg_iters, ctx->status);
+#define LBMIBM_RES(KB, VREG)                                                                                      \
+    hipLaunchKernelGGL((k_bicg_res<BICG_RES_K, KB, VREG>), dim3(ctx->bicg_n[2]), dim3(1024), lds, stream,        \
+                       ctx->bicg_cells[2], ctx->voff, (const float *)ctx->coef, ctx->nbr, (const float *)ctx->bv, \
+                       (float *)ctx->rh, (float *)ctx->pv, (float *)ctx->vv, out, ctx->nv, p.itrmax, p.residual,  \
+                       ctx->bicg_iters, ctx->status)
+            switch (var) {
+                case 2: LBMIBM_RES(2, true); break;
+                case 3: LBMIBM_RES(4, true); break;
+                case 4: LBMIBM_RES(2, false); break;
+                default: LBMIBM_RES(4, false); break;
+            }
+#undef LBMIBM_RES
             LBMIBM_LAUNCHED(ctx, "k_bicg_res");
             return LBMIBM_OK;
         }
@@ -1234,7 +1278,252 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
     return LBMIBM_OK;
 }
 
+// ---- adjoint of the membrane force ---------------------------------------------
+//
+// Forward per cell: q = M(X)^-1 b(X) (the Jacobi scaling does not change the
+// solution), cf = -q dsv with dsv = ds(X) s_c (ring area times the cell's
+// force scale).  Given cfa, the adjoint of cf:
+//   qbar = -dsv cfa,  dsvbar = -cfa . q,  s_c-bar += dsvbar ds,
+//   lambda = M^-1 qbar (M is symmetric: the same solver, right-hand side
+//   qbar / jac), bbar = lambda, Mbar = -lambda q^T,
+//   rg_e-bar = -(1/24)(sum_j lambda_j.q_j + (sum_j lambda_j).(sum_k q_k))
+//              + s_c sum_j dsvbar_j / 6                (M and ds from rg_e),
+//   Xbar += sum_e rg_e-bar grad rg_e + J_b^T lambda.
+// b is the gradient of the discrete Skalak (or neo-Hookean) energy, so its
+// Jacobian is symmetric and J_b^T lambda is the directional derivative of the
+// element right-hand sides along lambda, taken here in forward mode with dual
+// numbers through the same element formula as k_elem.
+
+struct Dual {
+    double v, d;
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+    const double r = a.v / b.v;
+    return {r, (a.d - r * b.d) / b.v};
+}
+__device__ __forceinline__ Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator/(double a, Dual b) {
+    const double r = a / b.v;
+    return {r, -r * b.d / b.v};
+}
+__device__ __forceinline__ Dual dsqrt(Dual a) {
+    const double r = sqrt(a.v);
+    return {r, a.d / (2.0 * r)};
+}
+__device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
+
+// Right-hand-side terms tb[corner][component] of one element (k_elem's
+// formula, without the rounding of the stored tension to T).
+template <typename R>
+__device__ __forceinline__ void elem_rhs(const R v1[3], const R v2[3], const double nn[4], const TensionArgs &a,
+                                         R tb[3][3]) {
+    const R g0 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+    const R g3 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
+    const R g1 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+    const R dv = g0 * g3 - g1 * g1;
+    const double dn = nn[0] * nn[3] - nn[1] * nn[2];
+    const R nd0 = g3 / dv, nd1 = -g1 / dv, nd3 = g0 / dv;
+    const R i1 = g0 * nn[0] + g1 * nn[1] + g1 * nn[2] + g3 * nn[3] - 2.0;
+    const R i2 = dv * dn - 1.0;
+    const R js = dsqrt(i2 + 1.0);
+    const R nd[4] = {nd0, nd1, nd1, nd3};
+    R tk[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (!a.neo) tk[k] = a.rgs * nn[k] * ((i1 + 1.0) / js) + (a.c * i2 - 1.0) * js * nd[k];
+        else tk[k] = a.rgs * (nn[k] / js - nd[k] * js / (i2 + 1.0) / (i2 + 1.0));
+    }
+    const R gab = dsqrt(dv);
+    const double F[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+#pragma unroll
+    for (int en = 0; en < 3; en++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const R vd[2] = {v1[d], v2[d]};
+            R sum = gab * 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int al = k >> 1, be = k & 1;
+                sum = sum + gab * tk[k] * (vd[al] * F[en][be] + vd[be] * F[en][al]) / 4.0;
+            }
+            tb[en][d] = sum;
+        }
+}
+
+// Per vertex: the adjoint right-hand side qbar/jac into bv, dsvbar, and the
+// per-cell force-scale gradient.
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_load(const int *__restrict__ nln,
+                                                  const typename Corner<T>::type *__restrict__ corner,
+                                                  const int *__restrict__ vcell, const T *__restrict__ dsv,
+                                                  const T *__restrict__ q, const T *__restrict__ cfa,
+                                                  T *__restrict__ bv, double *__restrict__ dsva,
+                                                  double *__restrict__ gsa, int nv) {
+    const int jj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jj >= nv) return;
+    const int n = nln[jj];
+    double jac = 0.0, ds = 0.0;
+    for (int s = 0; s < n; s++) {
+        const double gab = (double)corner[6 * (size_t)jj + s].w;
+        jac += gab / 12.0;
+        ds += gab / 2.0 / 3.0;
+    }
+    const double d = (double)dsv[jj];
+    double sb = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double a = (double)cfa[(size_t)k * nv + jj];
+        bv[3 * (size_t)jj + k] = (T)(-d * a / jac);
+        sb -= a * (double)q[3 * (size_t)jj + k];
+    }
+    dsva[jj] = sb;
+    if (gsa) atomicAdd(&gsa[vcell[jj]], sb * ds);
+}
+
+// Per element: its position adjoints (rg terms and J_b^T lambda) into the
+// slots of its corners.
+template <typename T>
+__global__ __launch_bounds__(256) void k_elem_adj(const double *__restrict__ xd, const int *__restrict__ ele,
+                                                  const double *__restrict__ nref, const int *__restrict__ cslot,
+                                                  const int *__restrict__ vcell, const T *__restrict__ gscale,
+                                                  const T *__restrict__ lam, const T *__restrict__ q,
+                                                  const double *__restrict__ dsva, double *__restrict__ cadj,
+                                                  int ne, PeriodicBox box, TensionArgs a) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ne) return;
+    const int vid[3] = {ele[3 * j], ele[3 * j + 1], ele[3 * j + 2]};
+    double v1[3], v2[3], L[3][3], Qv[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        v1[d] = xd[3 * vid[1] + d] - xd[3 * vid[0] + d];
+        v2[d] = xd[3 * vid[2] + d] - xd[3 * vid[0] + d];
+    }
+    min_img3<double>(v1, box);
+    min_img3<double>(v2, box);
+#pragma unroll
+    for (int en = 0; en < 3; en++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            L[en][d] = (double)lam[3 * (size_t)vid[en] + d];
+            Qv[en][d] = (double)q[3 * (size_t)vid[en] + d];
+        }
+    const double nn[4] = {nref[4 * j], nref[4 * j + 1], nref[4 * j + 2], nref[4 * j + 3]};
+    // rg_e-bar from the mass matrix and the ring areas
+    double lq = 0.0, sl[3] = {0.0, 0.0, 0.0}, sq[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int en = 0; en < 3; en++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            lq += L[en][d] * Qv[en][d];
+            sl[d] += L[en][d];
+            sq[d] += Qv[en][d];
+        }
+    const double gsc = (double)gscale[vcell[vid[0]]];
+    const double rgb = -(lq + sl[0] * sq[0] + sl[1] * sq[1] + sl[2] * sq[2]) / 24.0 +
+                       gsc * (dsva[vid[0]] + dsva[vid[1]] + dsva[vid[2]]) / 6.0;
+    // grad rg, rg = |v1 x v2|
+    const double g0 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+    const double g3 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
+    const double g1 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+    const double rg = sqrt(g0 * g3 - g1 * g1);
+    double d1[3], d2[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        d1[d] = (g3 * v1[d] - g1 * v2[d]) / rg;
+        d2[d] = (g0 * v2[d] - g1 * v1[d]) / rg;
+    }
+    // J_b^T lambda: tangent of the element terms along lambda
+    Dual w1[3], w2[3], tb[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        w1[d] = {v1[d], L[1][d] - L[0][d]};
+        w2[d] = {v2[d], L[2][d] - L[0][d]};
+    }
+    elem_rhs<Dual>(w1, w2, nn, a, tb);
+#pragma unroll
+    for (int en = 0; en < 3; en++) {
+        double xa[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double grg = en == 0 ? -(d1[d] + d2[d]) : (en == 1 ? d1[d] : d2[d]);
+            xa[d] = rgb * grg + tb[en][d].d;
+        }
+        double *o = cadj + 4 * (size_t)cslot[3 * (size_t)j + en];
+        o[0] = xa[0]; o[1] = xa[1]; o[2] = xa[2]; o[3] = 0.0;
+    }
+}
+
+// Per vertex: sum of its corner adjoints into the position adjoint.
+__global__ __launch_bounds__(256) void k_vert_adj(const int *__restrict__ nln, const double *__restrict__ cadj,
+                                                  double *__restrict__ xa, int nv) {
+    const int jj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jj >= nv) return;
+    const int n = nln[jj];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int s = 0; s < n; s++) {
+        const double *c = cadj + 4 * (6 * (size_t)jj + s);
+        s0 += c[0]; s1 += c[1]; s2 += c[2];
+    }
+    xa[jj] += s0;
+    xa[nv + jj] += s1;
+    xa[2 * (size_t)nv + jj] += s2;
+}
+
+template <typename T> lbmibm_status forces_adj_impl(lbmibm_ctx *ctx, bool have_q) {
+    const lbmibm_params &p = ctx->p;
+    const int ne = ctx->nt, nv = ctx->nv;
+    // forward at the current positions: rows, ring areas, corner records, and
+    // q unless the caller restored it (qv) from a recording of this step
+    lbmibm_status s = have_q ? assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx)) : forces_impl<T>(ctx);
+    if (s != LBMIBM_OK) return s;
+    hipLaunchKernelGGL(k_adj_load<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln,
+                       (const typename Corner<T>::type *)ctx->corner, ctx->vcell, (const T *)ctx->dsv,
+                       (const T *)ctx->qv, (const T *)ctx->adj_cf, (T *)ctx->bv, ctx->adj_dsv, ctx->grad_gs, nv);
+    LBMIBM_LAUNCHED(ctx, "k_adj_load");
+    SolveOut<T> out = solve_out<T>(ctx);
+    out.q = (T *)ctx->adj_lam;
+    out.cf = (T *)ctx->adj_scratch;
+    out.qi = nullptr;
+    out.fo = 0;
+    for (int cls = 0; cls < 2; cls++)
+        if (ctx->bicg_n[cls] > 0) {
+            s = launch_onchip<T>(ctx, cls, ctx->bicg_var[cls], out, ctx->stream);
+            if (s != LBMIBM_OK) return s;
+        }
+    if (ctx->bicg_n[2] > 0) {
+        s = launch_big<T>(ctx, ctx->bicg_var[2], out, ctx->stream);
+        if (s != LBMIBM_OK) return s;
+    }
+    hipLaunchKernelGGL(k_elem_adj<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
+                       ctx->ele, (const double *)ctx->nref, ctx->cslot, ctx->vcell, (const T *)ctx->gscale,
+                       (const T *)ctx->adj_lam, (const T *)ctx->qv, ctx->adj_dsv, ctx->adj_corner, ne, box_of(ctx),
+                       tension_args(ctx));
+    LBMIBM_LAUNCHED(ctx, "k_elem_adj");
+    hipLaunchKernelGGL(k_vert_adj, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln, ctx->adj_corner,
+                       ctx->adj_x, nv);
+    LBMIBM_LAUNCHED(ctx, "k_vert_adj");
+    (void)p;
+    return LBMIBM_OK;
+}
+
 }  // namespace
+
+lbmibm_status membrane_forces_adjoint(lbmibm_ctx *ctx, bool have_q) {
+    if (ctx->nc == 0) return LBMIBM_OK;
+    if (ctx->p.br != 0.0) {
+        ctx->err = "adjoint: bending (BR != 0) has no adjoint";
+        return LBMIBM_ERR_STATE;
+    }
+    return ctx->prec == LBMIBM_F64 ? forces_adj_impl<double>(ctx, have_q) : forces_adj_impl<float>(ctx, have_q);
+}
 
 int bicg_class(int prec, int nvert) {
     // on chip: up to 1024 vertices (any precision) and up to 3072 (FP32: the
@@ -1269,7 +1558,10 @@ lbmibm_status membrane_setup(lbmibm_ctx *ctx) {
     }
     if (ctx->bicg_n[2] > 0 && big_form_fits(ctx, 2)) {
         const int lds = (int)(3 * (size_t)ctx->bicg_maxv[2] * sizeof(float));
-        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K>, A, lds));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K, 2, true>, A, lds));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K, 4, true>, A, lds));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K, 2, false>, A, lds));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K, 4, false>, A, lds));
     }
     if (ctx->bicg_n[1] > 0) {
         const int lds = (int)(3 * 4 * (size_t)ctx->bicg_maxv[1] * sizeof(float));

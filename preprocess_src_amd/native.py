@@ -18,6 +18,9 @@ c_double_p = C.POINTER(C.c_double)
 c_int_p = C.POINTER(C.c_int)
 
 F64, F32, F16 = 0, 1, 2
+# adjoint buffers (LBMIBM_ADJ_*)
+ADJ_POPULATIONS, ADJ_VELOCITY, ADJ_LATTICE_FORCE, ADJ_POSITIONS, ADJ_NODAL_FORCE = 0, 1, 2, 3, 4
+ADJ_GRAD_BODY_FORCE, ADJ_GRAD_FORCE_SCALE = 5, 6
 STATUS = {
     0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_BICG_BREAKDOWN", 4: "ERR_BICG_NOCONV",
     5: "ERR_VOLUME", 6: "ERR_TOPOLOGY", 7: "ERR_STATE",
@@ -84,6 +87,16 @@ SIGNATURES = {
     "lbmibm_download_membrane": (C.c_int, [C.c_void_p] + [c_double_p] * 5 + [c_int_p]),
     "lbmibm_download_volume": (C.c_int, [C.c_void_p] + [c_double_p] * 4),
     "lbmibm_set_cell_force_scale": (C.c_int, [C.c_void_p, c_double_p]),
+    "lbmibm_set_body_force_field": (C.c_int, [C.c_void_p] + [c_double_p] * 3),
+    "lbmibm_adjoint_reset": (C.c_int, [C.c_void_p]),
+    "lbmibm_adjoint_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    "lbmibm_adjoint_set": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    "lbmibm_adj_lbm_d3q19": (C.c_int, [C.c_void_p, C.c_int]),
+    "lbmibm_adj_ibm_st1": (C.c_int, [C.c_void_p]),
+    "lbmibm_adj_ibm_st2": (C.c_int, [C.c_void_p]),
+    "lbmibm_adj_stress": (C.c_int, [C.c_void_p, C.c_int]),
+    "lbmibm_wall_shear": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_double_p]),
+    "lbmibm_adjoint_wall_shear": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int] + [c_double_p] * 5),
     "lbmibm_mesh_icosphere": (C.c_int, [C.c_int, c_int_p, c_int_p, c_double_p, c_int_p]),
     "lbmibm_mesh_biconcave": (C.c_int, [C.c_int, c_double_p]),
     "lbmibm_place_cells": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,

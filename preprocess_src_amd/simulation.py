@@ -198,3 +198,57 @@ class Simulation:
         m = np.zeros((self.nzp, self.nyp, self.nxp), bool)
         m[2:-2, 2:-2, 2:-2] = True
         return m.reshape(-1)
+
+    # ---- reverse mode ------------------------------------------------------
+    def _adj_size(self, which):
+        return {N.ADJ_POPULATIONS: self.np_ * Q, N.ADJ_VELOCITY: 3 * self.np_, N.ADJ_LATTICE_FORCE: 3 * self.np_,
+                N.ADJ_POSITIONS: 3 * self.nv, N.ADJ_NODAL_FORCE: 3 * self.nv,
+                N.ADJ_GRAD_BODY_FORCE: 3 * self.np_, N.ADJ_GRAD_FORCE_SCALE: self.nc}[which]
+
+    def set_body_force_field(self, gx=None, gy=None, gz=None):
+        """Per-node body-force field added to P e (padded [n] arrays); None removes it."""
+        if gx is None:
+            self._ck(self.L.lbmibm_set_body_force_field(self.ctx, None, None, None))
+            return
+        a = [np.ascontiguousarray(v, np.float64).reshape(-1) for v in (gx, gy, gz)]
+        self._ck(self.L.lbmibm_set_body_force_field(self.ctx, *[N.dp(v) for v in a]))
+
+    def adjoint_reset(self):
+        self._ck(self.L.lbmibm_adjoint_reset(self.ctx))
+
+    def adjoint(self, which):
+        out = np.zeros(self._adj_size(which))
+        self._ck(self.L.lbmibm_adjoint_get(self.ctx, int(which), N.dp(out)))
+        return out
+
+    def set_adjoint(self, which, a):
+        a = np.ascontiguousarray(a, np.float64).reshape(-1)
+        assert a.size == self._adj_size(which)
+        self._ck(self.L.lbmibm_adjoint_set(self.ctx, int(which), N.dp(a)))
+
+    def adj_lbm_d3q19(self, it):
+        self._ck(self.L.lbmibm_adj_lbm_d3q19(self.ctx, int(it)))
+
+    def adj_ibm_st1(self):
+        self._ck(self.L.lbmibm_adj_ibm_st1(self.ctx))
+
+    def adj_ibm_st2(self):
+        self._ck(self.L.lbmibm_adj_ibm_st2(self.ctx))
+
+    def adj_stress(self, it):
+        self._ck(self.L.lbmibm_adj_stress(self.ctx, int(it)))
+
+    def wall_shear(self, iter0, nsteps, window):
+        J = np.zeros(1)
+        self._ck(self.L.lbmibm_wall_shear(self.ctx, int(iter0), int(nsteps), int(window), N.dp(J)))
+        return float(J[0])
+
+    def adjoint_wall_shear(self, iter0, nsteps, window, checkpoint_every):
+        """J and dJ/d(body-force field) (3 padded [n] arrays) and dJ/d(per-cell force scale)."""
+        J = np.zeros(1)
+        g = [np.zeros(self.np_) for _ in range(3)]
+        ds = np.zeros(max(self.nc, 1))
+        self._ck(self.L.lbmibm_adjoint_wall_shear(self.ctx, int(iter0), int(nsteps), int(window),
+                                                  int(checkpoint_every), N.dp(J), *[N.dp(v) for v in g],
+                                                  N.dp(ds)))
+        return float(J[0]), g, ds[:self.nc]

@@ -483,3 +483,35 @@ def test_volume_correction_restores_the_reference_volume():
         tri = cells["ele"][3 * cells["eoff"][c]:3 * cells["eoff"][c + 1]].reshape(-1, 3) - cells["voff"][c]
         vol = np.einsum("ij,ij->i", X[tri[:, 0]] - g, np.cross(X[tri[:, 1]] - g, X[tri[:, 2]] - g)).sum() / 6
         assert abs(vol - V0[c]) / V0[c] < 1e-5
+
+
+# ---- body-force field and wall-shear objective (gradient checks) ---------------
+
+def test_uniform_body_force_field_equals_the_body_force():
+    """A field g = P e on every node reproduces the run with body force P e."""
+    lx, ly, lz, P = 8, 6, 6, 2e-4
+    a = OracleSim(lx, ly, lz, tau=0.8, p=P, volume_correction=False)
+    b = OracleSim(lx, ly, lz, tau=0.8, p=0.0, volume_correction=False)
+    a.init_fluid()
+    b.init_fluid()
+    b.set_body_force_field(np.full(b.n, P), np.zeros(b.n), np.zeros(b.n))
+    Ja = a.wall_shear(0, 30, 10)
+    Jb = b.wall_shear(0, 30, 10)
+    assert Ja > 0 and abs(Ja - Jb) <= 1e-14 * abs(Ja)
+    fa, fb = comp_view(a, a.fn.reshape(-1, Q)), comp_view(b, b.fn.reshape(-1, Q))
+    assert np.max(np.abs(fa - fb)) <= 1e-15
+
+
+def test_wall_shear_is_the_mean_first_node_velocity():
+    """The objective is 2 Df (dx/dt) e.u at the two wall-adjacent node layers,
+    averaged over them and over the window (checked against the macros)."""
+    lx, ly, lz = 6, 8, 4
+    o = OracleSim(lx, ly, lz, tau=0.9, p=1e-4, e=(0.6, 0.0, 0.8), volume_correction=False)
+    o.init_fluid()
+    J = o.wall_shear(0, 5, 1)
+    df = (2 * 0.9 - 1) / 6
+    u = comp_view(o, o.um)
+    w = comp_view(o, o.wm)
+    walls = [0, ly - 1]
+    expect = 2 * df * np.mean([0.6 * u[:, iy, :].mean() + 0.8 * w[:, iy, :].mean() for iy in walls])
+    assert abs(J - expect) <= 1e-13 * abs(expect)
