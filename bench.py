@@ -150,6 +150,48 @@ def tgv_init(shape, u0=0.01):
     return rep(r2), rep(u2), rep(v2)
 
 
+def tgv_field_errors(sim, shape, params, steps, precision):
+    """Per-field error of this run against an FP64 run of the same steps
+    (max-abs and L2-relative for rho, u_x, u_y, u_z and the populations), and
+    the decay of the velocity mode against exp(-2 nu k^2 t)."""
+    from preprocess_src_amd.simulation import Simulation
+    lx, ly, lz = shape
+    cq = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0],
+                   [-1, -1, 0], [-1, 1, 0], [1, -1, 0], [0, 1, 1], [0, -1, -1], [0, -1, 1], [0, 1, -1], [1, 0, 1],
+                   [-1, 0, -1], [1, 0, -1], [-1, 0, 1]], float)
+
+    def fields(s):
+        # moments of the post-stream populations (no force: u = j / rho)
+        f = s.populations().reshape(-1, Q)[s.comp_mask()]
+        rho = f.sum(axis=1)
+        j = f @ cq
+        return {"rho": rho, "u_x": j[:, 0] / rho, "u_y": j[:, 1] / rho, "u_z": j[:, 2] / rho, "f_q": f}
+    mine = fields(sim)
+    if precision == "f64":
+        ref = mine
+    else:
+        r = Simulation(lx, ly, lz, precision="f64", **params)
+        r.set_equilibrium(*tgv_init(shape))
+        r.run(0, steps, graph=True)
+        ref = fields(r)
+        r.close()
+    err = {}
+    for k in mine:
+        d = mine[k] - ref[k]
+        base = ref[k] - (1.0 if k == "rho" else 0.0)
+        err[k] = {"max_abs": float(np.max(np.abs(d))),
+                  "l2_rel": float(np.linalg.norm(d) / max(np.linalg.norm(base), 1e-300))}
+    k = 2 * np.pi / lx
+    xp = np.arange(lx) - lx / 2 + 0.5
+    yp = np.arange(ly) - ly / 2 + 0.5
+    X, Y = np.meshgrid(xp, yp, indexing="xy")
+    basis = np.broadcast_to(np.cos(k * X) * np.sin(k * Y), (lz, ly, lx)).reshape(-1)
+    amp = float((mine["u_x"] * basis).sum() / (basis * basis).sum())
+    nu = (params["tau"] - 0.5) / 3
+    err["decay"] = {"amplitude_over_u0": amp / 0.01, "analytic": float(np.exp(-2 * nu * k * k * steps))}
+    return err
+
+
 def lattice_bytes_per_node(precision, coupled):
     s = STORE_BYTES[precision]
     t = ARITH_BYTES[precision]
@@ -286,6 +328,9 @@ def main():
         "stages_ms": {k: round(v[0], 5) for k, v in stages.items()},
         "diagnostics": diag,
     }
+    if a.workload == "tgv":
+        print("[bench] per-field errors against FP64", file=sys.stderr, flush=True)
+        out["field_errors"] = tgv_field_errors(sim, shape, params, a.warmup + a.steps + a.stage_steps, a.precision)
     if not a.no_cpu:
         print("[bench] cpu baseline (FP64 oracle)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(shape, cells, params, a.cpu_seconds)

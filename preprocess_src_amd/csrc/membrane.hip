@@ -57,9 +57,79 @@ template <typename T> struct Corner;
 template <> struct Corner<float> { using type = float4; };
 template <> struct Corner<double> { using type = double4; };
 
+// One element's calc_tension quantities in FP64 (stress.cu:63-99): edges with
+// minimum image, the stored tension tk (rounded to T, as the reference's
+// calc_load reads td), gab = rg rounded to T, and the invariants for tp.
+template <typename T> struct ElemTension {
+    double v1[3], v2[3];
+    double t[4];   // tension before rounding
+    double tk[4];  // (double)(T) tension
+    double rg, gab, i1, i2;
+    __device__ __forceinline__ ElemTension(const double *__restrict__ xd, int ea, int eb, int ec,
+                                           const double *__restrict__ nr, const PeriodicBox &box,
+                                           const TensionArgs &a) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            v1[d] = xd[3 * eb + d] - xd[3 * ea + d];
+            v2[d] = xd[3 * ec + d] - xd[3 * ea + d];
+        }
+        min_img3<double>(v1, box);
+        min_img3<double>(v2, box);
+        const double n0 = nr[0], n1 = nr[1], n2 = nr[2], n3 = nr[3];
+        const double l1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+        const double l2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+        const double i12 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+        const double dv = (l1 * l1) * (l2 * l2) - i12 * i12;
+        const double dn = n0 * n3 - n1 * n2;
+        rg = sqrt(dv);
+        const double g0 = l1 * l1, g1 = i12, g3 = l2 * l2;
+        const double nd0 = g3 / dv, nd1 = -g1 / dv, nd3 = g0 / dv;
+        i1 = n0 * g0 + n1 * g1 + n2 * g1 + n3 * g3 - 2.0;
+        i2 = dv * dn - 1.0;
+        const double js = sqrt(i2 + 1.0);
+        const double nn[4] = {n0, n1, n2, n3}, nd[4] = {nd0, nd1, nd1, nd3};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!a.neo) t[k] = a.rgs * (i1 + 1.0) / js * nn[k] + (a.c * i2 - 1.0) * js * nd[k];
+            else t[k] = a.rgs * (nn[k] / js - nd[k] * js / (i2 + 1.0) / (i2 + 1.0));
+            tk[k] = (double)(T)t[k];
+        }
+        gab = (double)(T)rg;
+    }
+    // calc_load terms of corner en: sum_{alpha,beta} rg t^{alpha beta}
+    // (e_alpha dN_beta + e_beta dN_alpha) / 4, dN/dxi of the linear triangle
+    // (-1,-1), (1,0), (0,1)
+    __device__ __forceinline__ void corner(int en, double tb[3]) const {
+        const double F[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double vd[2] = {v1[d], v2[d]};
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int al = k >> 1, be = k & 1;
+                sum += gab * tk[k] * (vd[al] * F[en][be] + vd[be] * F[en][al]) / 4.0;
+            }
+            tb[d] = sum;
+        }
+    }
+    // principal-tension output accumulator (stress.cu:101-118)
+    __device__ __forceinline__ void accumulate_tp(T *__restrict__ tp, int j, int iter, const TensionArgs &a) const {
+        double t0 = 0.0, t1 = 0.0;
+        if ((iter - 1) % a.fileoutput != 0) { t0 = (double)tp[2 * j]; t1 = (double)tp[2 * j + 1]; }
+        const double judge = fabs((i1 + 2.0) * (i1 + 2.0) - 4.0 * (i2 + 1.0));
+        const double root = sqrt(judge);
+        const double l12 = 0.5 * (i1 + 2.0 + root), l22 = 0.5 * (i1 + 2.0 - root);
+        const double l11 = sqrt(l12), l21 = sqrt(l22);
+        tp[2 * j] = (T)(t0 + a.wtp * l11 / l21 * (l12 - 1.0 + a.c * l22 * i2));
+        tp[2 * j + 1] = (T)(t1 + a.wtp * l21 / l11 * (l22 - 1.0 + a.c * l12 * i2));
+    }
+};
+
 // The corner records go to the slot of the corner's vertex: cslot[3e+en] =
 // 6 j + s for vertex j = ele[3e+en] with e its s-th incident element, so the
-// per-vertex pass reads each vertex's records contiguously.
+// per-vertex pass reads each vertex's records contiguously.  (Used where the
+// records themselves are needed, by the adjoint; the forward uses k_assemble.)
 template <typename T>
 __global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, const int *__restrict__ ele,
                                               const double *__restrict__ nref, T *__restrict__ tt,
@@ -76,69 +146,19 @@ __global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, con
         jend = eoff[c + 1];
     }
     if (j >= jend) return;
-    const int ea = ele[3 * j], eb = ele[3 * j + 1], ec = ele[3 * j + 2];
-    double v1[3], v2[3];
+    const ElemTension<T> el(xd, ele[3 * j], ele[3 * j + 1], ele[3 * j + 2], nref + 4 * (size_t)j, box, a);
+    rg[j] = (T)el.rg;
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        v1[d] = xd[3 * eb + d] - xd[3 * ea + d];
-        v2[d] = xd[3 * ec + d] - xd[3 * ea + d];
-    }
-    min_img3<double>(v1, box);
-    min_img3<double>(v2, box);
-    const double n0 = nref[4 * j], n1 = nref[4 * j + 1], n2 = nref[4 * j + 2], n3 = nref[4 * j + 3];
-    const double l1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
-    const double l2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
-    const double i12 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
-    const double dv = (l1 * l1) * (l2 * l2) - i12 * i12;
-    const double dn = n0 * n3 - n1 * n2;
-    rg[j] = (T)sqrt(dv);
-    const double g0 = l1 * l1, g1 = i12, g3 = l2 * l2;
-    const double nd0 = g3 / dv, nd1 = -g1 / dv, nd3 = g0 / dv;
-    const double i1 = n0 * g0 + n1 * g1 + n2 * g1 + n3 * g3 - 2.0;
-    const double i2 = dv * dn - 1.0;
-    const double js = sqrt(i2 + 1.0);
-    const double nn[4] = {n0, n1, n2, n3}, nd[4] = {nd0, nd1, nd1, nd3};
-    double tk[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        double t;
-        if (!a.neo) t = a.rgs * (i1 + 1.0) / js * nn[k] + (a.c * i2 - 1.0) * js * nd[k];
-        else t = a.rgs * (nn[k] / js - nd[k] * js / (i2 + 1.0) / (i2 + 1.0));
-        tt[4 * j + k] = (T)t;
-        tk[k] = (double)(T)t;  // the stored tension, as the reference's calc_load reads it
-    }
-    // corners a, b, c: dN/dxi of the linear triangle (-1,-1), (1,0), (0,1)
-    const double gab = (double)(T)sqrt(dv);
-    const double F[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+    for (int k = 0; k < 4; k++) tt[4 * j + k] = (T)el.t[k];
 #pragma unroll
     for (int en = 0; en < 3; en++) {
         double tb[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const double vd[2] = {v1[d], v2[d]};
-            double sum = 0.0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int al = k >> 1, be = k & 1;
-                sum += gab * tk[k] * (vd[al] * F[en][be] + vd[be] * F[en][al]) / 4.0;
-            }
-            tb[d] = sum;
-        }
+        el.corner(en, tb);
         typename Corner<T>::type cv;
-        cv.x = (T)tb[0]; cv.y = (T)tb[1]; cv.z = (T)tb[2]; cv.w = (T)gab;
+        cv.x = (T)tb[0]; cv.y = (T)tb[1]; cv.z = (T)tb[2]; cv.w = (T)el.gab;
         corner[cslot[3 * (size_t)j + en]] = cv;
     }
-    if (tp) {  // principal-tension accumulator for output (stress.cu:101-118)
-        const int iter = st->iter;
-        double t0 = 0.0, t1 = 0.0;
-        if ((iter - 1) % a.fileoutput != 0) { t0 = (double)tp[2 * j]; t1 = (double)tp[2 * j + 1]; }
-        const double judge = fabs((i1 + 2.0) * (i1 + 2.0) - 4.0 * (i2 + 1.0));
-        const double root = sqrt(judge);
-        const double l12 = 0.5 * (i1 + 2.0 + root), l22 = 0.5 * (i1 + 2.0 - root);
-        const double l11 = sqrt(l12), l21 = sqrt(l22);
-        tp[2 * j] = (T)(t0 + a.wtp * l11 / l21 * (l12 - 1.0 + a.c * l22 * i2));
-        tp[2 * j + 1] = (T)(t1 + a.wtp * l21 / l11 * (l22 - 1.0 + a.c * l12 * i2));
-    }
+    if (tp) el.accumulate_tp(tp, j, st->iter, a);
 }
 
 // Per-vertex part of calc_load: the Jacobi-scaled row (stress.cu:263-268) and
@@ -177,6 +197,72 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
             tb[0] += (double)cv.x;
             tb[1] += (double)cv.y;
             tb[2] += (double)cv.z;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const unsigned ep = epair[6 * (size_t)jj + m];
+        double ra = 0.0, rb = 0.0;
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+            if ((int)(ep & 15u) == s) ra = rgs[s];
+            if ((int)(ep >> 4) == s) rb = rgs[s];
+        }
+        coef[(size_t)m * nv + jj] = ep == 0xffu ? T(0) : (T)((ra / 24.0 + rb / 24.0) / jac);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
+    dsv[jj] = (T)(ds * (double)gscale[vcell[jj]]);
+}
+
+// calc_tension + calc_load + q2f ring area fused per vertex: each thread
+// recomputes the tension of its (at most 6) incident elements, in the
+// reference's le order, and keeps only its own corner's terms, so no
+// per-corner records travel through memory.  Arithmetic and summation order
+// are those of k_elem + k_vert (results identical).  The tension and rg
+// outputs (td, rgd) and the tp accumulator are written by the element's
+// first-corner thread, and only when asked for (tt / tp non-null).
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ xd, const int *__restrict__ ele,
+                                                  const double *__restrict__ nref, const int *__restrict__ nln,
+                                                  const int *__restrict__ le, const unsigned char *__restrict__ epair,
+                                                  const int *__restrict__ vcell, const T *__restrict__ gscale,
+                                                  T *__restrict__ coef, T *__restrict__ bv, T *__restrict__ dsv,
+                                                  T *__restrict__ tt, T *__restrict__ rgo, T *__restrict__ tp, int nv,
+                                                  PeriodicBox box, TensionArgs a, const DevStep *__restrict__ st,
+                                                  const int *__restrict__ cells, const int *__restrict__ voff) {
+    int jj = blockIdx.x * blockDim.x + threadIdx.x, jend = nv;
+    if (cells) {
+        const int c = cells[blockIdx.y];
+        jj += voff[c];
+        jend = voff[c + 1];
+    }
+    if (jj >= jend) return;
+    const int n = nln[jj];
+    double rgs[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, tb[3] = {0.0, 0.0, 0.0}, jac = 0.0, ds = 0.0;
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        if (s >= n) continue;
+        const int e = le[6 * (size_t)jj + s];
+        const int ea = ele[3 * e], eb = ele[3 * e + 1], ec = ele[3 * e + 2];
+        const int en = ea == jj ? 0 : (eb == jj ? 1 : 2);
+        const ElemTension<T> el(xd, ea, eb, ec, nref + 4 * (size_t)e, box, a);
+        double c[3];
+        el.corner(en, c);
+        const double gab = (double)(T)el.gab;
+        rgs[s] = gab;
+        jac += gab / 12.0;
+        ds += gab / 2.0 / 3.0;
+        tb[0] += (double)(T)c[0];
+        tb[1] += (double)(T)c[1];
+        tb[2] += (double)(T)c[2];
+        if (en == 0) {
+            if (tt) {
+                rgo[e] = (T)el.rg;
+#pragma unroll
+                for (int k = 0; k < 4; k++) tt[4 * (size_t)e + k] = (T)el.t[k];
+            }
+            if (tp) el.accumulate_tp(tp, e, st->iter, a);
         }
     }
 #pragma unroll
@@ -1140,13 +1226,26 @@ template <typename T> __global__ void k_synth_rhs(T *__restrict__ b, size_t n) {
 
 // Tension, corner terms and rows of all cells (cells == nullptr) or of the
 // listed cells (n of them, at most max_e elements and max_v vertices each).
+// corners: also leave the per-(vertex, slot) corner records (two-pass form),
+// which the adjoint reads; otherwise the fused per-vertex kernel.  The
+// tension and rg outputs are kept with store_macros.
 template <typename T>
 lbmibm_status assemble_impl(lbmibm_ctx *ctx, const PeriodicBox &box, const TensionArgs &ta,
-                            const int *cells = nullptr, int n = 0, int max_e = 0, int max_v = 0) {
+                            const int *cells = nullptr, int n = 0, int max_e = 0, int max_v = 0,
+                            bool corners = false) {
     const lbmibm_params &p = ctx->p;
     const int ne = ctx->nt, nv = ctx->nv;
     const dim3 ge = cells ? dim3((max_e + 255) / 256, n) : dim3((ne + 255) / 256);
     const dim3 gv = cells ? dim3((max_v + 255) / 256, n) : dim3((nv + 255) / 256);
+    if (!corners) {
+        hipLaunchKernelGGL(k_assemble<T>, gv, dim3(256), 0, ctx->stream, (const double *)ctx->xd, ctx->ele,
+                           (const double *)ctx->nref, ctx->nln, ctx->le, ctx->epair, ctx->vcell,
+                           (const T *)ctx->gscale, (T *)ctx->coef, (T *)ctx->bv, (T *)ctx->dsv,
+                           p.store_macros ? (T *)ctx->tt : (T *)nullptr, (T *)ctx->rg,
+                           p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, nv, box, ta, ctx->step, cells, ctx->voff);
+        LBMIBM_LAUNCHED(ctx, "k_assemble");
+        return LBMIBM_OK;
+    }
     hipLaunchKernelGGL(k_elem<T>, ge, dim3(256), 0, ctx->stream, (const double *)ctx->xd, ctx->ele,
                        (const double *)ctx->nref, (T *)ctx->tt, (T *)ctx->rg,
                        p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ctx->cslot,
@@ -1482,7 +1581,9 @@ template <typename T> lbmibm_status forces_adj_impl(lbmibm_ctx *ctx, bool have_q
     const int ne = ctx->nt, nv = ctx->nv;
     // forward at the current positions: rows, ring areas, corner records, and
     // q unless the caller restored it (qv) from a recording of this step
-    lbmibm_status s = have_q ? assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx)) : forces_impl<T>(ctx);
+    lbmibm_status s = have_q ? LBMIBM_OK : forces_impl<T>(ctx);
+    // the corner records (rg per incident element) for the adjoint right-hand side
+    if (s == LBMIBM_OK) s = assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx), nullptr, 0, 0, 0, true);
     if (s != LBMIBM_OK) return s;
     hipLaunchKernelGGL(k_adj_load<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln,
                        (const typename Corner<T>::type *)ctx->corner, ctx->vcell, (const T *)ctx->dsv,

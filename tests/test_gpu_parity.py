@@ -358,3 +358,27 @@ def test_autotune_covers_the_large_cell_forms():
     cells = small_cells(lx, ly, lz, nc=1, radius=4.0, levels=[5], seed=64)
     g = gpu_of(lx, ly, lz, "f32", cells=cells)
     assert "class 2: using" in g.autotune_report()
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 2e-3), ("f32", 2e-3)])
+def test_poiseuille_channel_64x32x32_against_the_transient_series(prec, tol):
+    """The plain channel configuration: 64x32x32, half-way walls at y = 0.5 and
+    30.5 of the padded frame (H = 32 fluid nodes here, walls at +-LY/2), tau =
+    0.8, Guo body force 1e-6 along x, 1000 steps from rest; u_x(y) against the
+    start-up Fourier series (the same closed form that pins the oracle), and
+    against the FP64 oracle at 1e-10 (FP64) / 1e-5 (FP32) relative."""
+    from tests.test_oracle import _poiseuille_series
+    lx, ly, lz, tau, a = 64, 32, 32, 0.8, 1e-6
+    g = gpu_of(lx, ly, lz, prec, tau=tau, p=a / 2, volume_correction=False)
+    g.set_equilibrium()
+    g.run(0, 1000, graph=True)
+    u = g.macros()["um"].reshape(lz + 4, ly + 4, lx + 4)[2:-2, 2:-2, 2:-2]
+    assert np.ptp(u, axis=(0, 2)).max() <= 1e-12 * np.abs(u).max()
+    nu = (tau - 0.5) / 3
+    ua = _poiseuille_series(np.arange(ly) + 0.5, 1000, a, nu, ly)
+    assert np.max(np.abs(u[0, :, 0] - ua)) / ua.max() < tol
+    o = oracle_of(4, ly, 4, tau=tau, p=a / 2, volume_correction=False)
+    for it in range(1, 1001):
+        o.lbm_d3q19(it)
+    uo = o.un.reshape(o.nz, o.ny, o.nx)[2, 2:-2, 2]
+    assert np.max(np.abs(u[0, :, 0] - uo)) / np.abs(uo).max() < (1e-10 if prec == "f64" else 1e-5)
