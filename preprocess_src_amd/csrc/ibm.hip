@@ -379,14 +379,19 @@ constexpr int ACC_SX = 20, ACC_SXY = 176, ACC_N = ACC_SXY * TILE_Z;
 
 // Accumulation into the tile in 64-bit fixed point with a power-of-two scale,
 // so the integer sums do not depend on the order of the additions.
-//   FP32: the scale keeps |sum| below 2^50; the product w*(f*scale) is formed
+//   FP32: the scale keeps |sum| below 2^47; the product w*(f*scale) is formed
 //   by one FP64 FMA with 1.5*2^52 added, which leaves round(w*f*scale) in the
-//   low mantissa bits: subtracting the constant's bit pattern gives the
-//   two's-complement integer (valid for |value| < 2^51).
+//   low mantissa bits (valid for |value| < 2^51), see below.
 //   FP64: the scale keeps |sum| below 2^62 and llrint converts.
 template <typename T> struct FixedPoint;
+// FP32: t = fma(w, f scale, 1.5 2^52) holds k = round(w f scale) as
+// bits(t) = 0x4338 << 48 + k for |k| < 2^51.  The constant part is a multiple
+// of 2^48, so the raw bits are summed as they are and the low 48 bits of the
+// sum are the sum of the k modulo 2^48: exact whenever the final sum is below
+// 2^47 in magnitude, which the per-tile scale guarantees (kBits = 47), however
+// the partial sums wrap.  No per-term subtraction.
 template <> struct FixedPoint<float> {
-    static constexpr int kBits = 50;
+    static constexpr int kBits = 47;
     double f[3];
     __device__ __forceinline__ FixedPoint(float f0, float f1, float f2, double scale) {
         f[0] = (double)f0 * scale; f[1] = (double)f1 * scale; f[2] = (double)f2 * scale;
@@ -396,9 +401,11 @@ template <> struct FixedPoint<float> {
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const double t = fma(wd, f[d], 6755399441055744.0);  // + 1.5 * 2^52
-            atomicAdd(&acc[d * ACC_N + li],
-                      (unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull);
+            atomicAdd(&acc[d * ACC_N + li], (unsigned long long)__double_as_longlong(t));
         }
+    }
+    __device__ __forceinline__ static long long value(unsigned long long a) {
+        return (long long)(a << 16) >> 16;  // sign-extend the low 48 bits
     }
 };
 template <> struct FixedPoint<double> {
@@ -411,6 +418,7 @@ template <> struct FixedPoint<double> {
 #pragma unroll
         for (int d = 0; d < 3; d++) atomicAdd(&acc[d * ACC_N + li], (unsigned long long)llrint(w * f[d] * scale));
     }
+    __device__ __forceinline__ static long long value(unsigned long long a) { return (long long)a; }
 };
 
 template <typename T>
@@ -543,7 +551,8 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const SpreadRec<T> *__rest
         const size_t nidx = (size_t)(ox + la) + (size_t)g.nx * ((size_t)(oy + lb) + (size_t)g.ny * (oz + lc));
         const int li = la + ACC_SX * lb + ACC_SXY * lc;
 #pragma unroll
-        for (int d = 0; d < 3; d++) F[d * N + nidx] = (T)((double)(long long)acc[d * ACC_N + li] * inv_scale);
+        for (int d = 0; d < 3; d++)
+            F[d * N + nidx] = (T)((double)FixedPoint<T>::value(acc[d * ACC_N + li]) * inv_scale);
     }
 }
 
