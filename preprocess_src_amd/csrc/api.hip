@@ -189,6 +189,8 @@ void free_all(lbmibm_ctx *ctx) {
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_vfork) (void)hipEventDestroy(ctx->ev_vfork);
+    if (ctx->ev_vjoin) (void)hipEventDestroy(ctx->ev_vjoin);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
@@ -273,6 +275,8 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) return fail(LBMIBM_ERR_HIP);
     if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
     if (hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+    if (hipEventCreateWithFlags(&ctx->ev_vfork, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+    if (hipEventCreateWithFlags(&ctx->ev_vjoin, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
 
     const size_t szS = ctx->szS, szT = ctx->szT;
     CK(dev_alloc(ctx, &ctx->f[0], (size_t)Q * N * szS));

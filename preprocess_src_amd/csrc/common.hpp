@@ -49,7 +49,8 @@ struct lbmibm_ctx {
     // second stream (default priority) for the large-cell solves, forked from
     // and joined back into `stream` with events; captured graphs keep the fork
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;    // membrane solve
+    hipEvent_t ev_vfork = nullptr, ev_vjoin = nullptr;  // volume correction
     std::string err;
 
     // lattice (device)

@@ -143,18 +143,31 @@ __global__ __launch_bounds__(NT) void k_volume(const int *__restrict__ cells, in
 
 lbmibm_status run_impl(lbmibm_ctx *ctx, int mode) {
     VolArgs a{ctx->nx * ctx->p.dx, ctx->ny * ctx->p.dx, ctx->nz * ctx->p.dx, ctx->p.periodic_y};
-    // cells of up to VOL_SMALL vertices with 256 threads, larger ones with 1024
+    // cells of up to VOL_SMALL vertices with 256 threads, larger ones with
+    // 1024; the two classes touch disjoint cells, so the large one runs on the
+    // side stream (forked and joined with events) concurrently with the other
+    const bool fork = ctx->vol_n[0] > 0 && ctx->vol_n[1] > 0;
+    hipStream_t big = ctx->stream;
+    if (fork) {
+        LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_vfork, ctx->stream));
+        LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_vfork, 0));
+        big = ctx->side;
+    }
+    if (ctx->vol_n[1] > 0) {
+        hipLaunchKernelGGL(k_volume<1024>, dim3(ctx->vol_n[1]), dim3(1024), 0, big, ctx->vol_cells[1], mode,
+                           (double *)ctx->xs, ctx->nv, ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le,
+                           ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->vol_passes, ctx->status, a);
+        LBMIBM_LAUNCHED(ctx, "k_volume<1024>");
+    }
     if (ctx->vol_n[0] > 0) {
         hipLaunchKernelGGL(k_volume<256>, dim3(ctx->vol_n[0]), dim3(256), 0, ctx->stream, ctx->vol_cells[0], mode,
                            (double *)ctx->xs, ctx->nv, ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le,
                            ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->vol_passes, ctx->status, a);
         LBMIBM_LAUNCHED(ctx, "k_volume<256>");
     }
-    if (ctx->vol_n[1] > 0) {
-        hipLaunchKernelGGL(k_volume<1024>, dim3(ctx->vol_n[1]), dim3(1024), 0, ctx->stream, ctx->vol_cells[1], mode,
-                           (double *)ctx->xs, ctx->nv, ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le,
-                           ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->vol_passes, ctx->status, a);
-        LBMIBM_LAUNCHED(ctx, "k_volume<1024>");
+    if (fork) {
+        LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_vjoin, ctx->side));
+        LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_vjoin, 0));
     }
     return LBMIBM_OK;
 }
