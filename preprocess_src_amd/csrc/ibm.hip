@@ -84,9 +84,12 @@ __device__ __forceinline__ void sincos_quarter(double t, double *s, double *c) {
 // floor(xtr/h)-1 (ibm.cu:103-121).  With t = frac(xtr/h) the distances are
 // 1+t, t, 1-t, 2-t, so the four cosines are -sin, cos, sin, -cos of pi t/2:
 // one sincos of an argument in [0, pi/2) per axis, evaluated in W.
+// xtr / h; h == 1 (lattice units) skips the FP64 division, with the same result
+__device__ __forceinline__ double over_h(double xtr, double h) { return h == 1.0 ? xtr : xtr / h; }
+
 template <typename W>
 __device__ __forceinline__ void stencil(double xtr, double h, int &first, W w[4], bool unit_spacing) {
-    const double q = unit_spacing ? xtr : xtr / h;
+    const double q = unit_spacing ? xtr : over_h(xtr, h);
     const double fl = floor(q);
     first = (int)fl - 1;
     const W t = (W)(q - fl);
@@ -101,7 +104,7 @@ __device__ __forceinline__ void stencil(double xtr, double h, int &first, W w[4]
 // First stencil node (padded frame, as stencil()) and fraction t = frac(xtr/h).
 template <typename W>
 __device__ __forceinline__ void stencil_first(double xtr, double h, int &first, W &t) {
-    const double q = xtr / h;
+    const double q = over_h(xtr, h);
     const double fl = floor(q);
     first = (int)fl - 1;
     t = (W)(q - fl);
@@ -728,7 +731,7 @@ __global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, d
 
 template <typename W>
 __device__ __forceinline__ void stencil_d(double xtr, double h, int &first, W w[4], W dw[4]) {
-    const double q = xtr / h;
+    const double q = over_h(xtr, h);
     const double fl = floor(q);
     first = (int)fl - 1;
     const W t = (W)(q - fl);
