@@ -678,19 +678,15 @@ __device__ __forceinline__ int opaque(int v) {
     asm volatile("" : "+v"(v));
     return v;
 }
-__device__ __forceinline__ double wide(float v) {
-    asm volatile("" : "+v"(v));
-    return (double)v;
-}
 
-template <int K, int KB, bool VREG>
+template <int K>
 __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells, const int *__restrict__ voff,
                                                    const float *__restrict__ coef,
                                                    const unsigned short *__restrict__ nbr,
                                                    const float *__restrict__ bv, float *__restrict__ x,
-                                                   float *__restrict__ pg, float *__restrict__ vg,
-                                                   SolveOut<float> out, int nv, int itrmax, double residual,
-                                                   int *__restrict__ iters, DevStatus *__restrict__ status) {
+                                                   float *__restrict__ pg, SolveOut<float> out, int nv, int itrmax,
+                                                   double residual, int *__restrict__ iters,
+                                                   DevStatus *__restrict__ status) {
     extern __shared__ double dyn[];
     __shared__ double sh[128];
     WgReduce red{sh, 0};
@@ -698,21 +694,10 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
     const int c = cells[blockIdx.x];
     const int v0 = voff[c], V = voff[c + 1] - v0;
     float *L = (float *)dyn;  // [V][3]
-    // r (then t) and, with VREG, v per slot; zero for the slots j >= V
-    float rt[K][3], vr[VREG ? K : 1][3];
-    // Slot k of this thread is vertex j = tid + k*1024.  Loads use the index
-    // clamped to V-1 so that they carry no branch; stores and sums are
-    // predicated on j < V.  A scheduling fence after every KB slots bounds
-    // how far the compiler hoists loads (all K slots at once would spill).
-    // Without VREG, v lives in memory (vg, owner thread only): 36 B more
-    // traffic per vertex and iteration, but registers for more loads in
-    // flight.  wide() converts without letting the compiler keep the FP64
-    // copies of the slots alive from one phase to the next.
-    auto fence = [](int k) {
-        if ((k + 1) % KB == 0) __builtin_amdgcn_sched_barrier(0);
-    };
-    // y = A L at vertex jc (global index g); 32-bit offsets from the uniform bases
-    auto spmv = [&](int jc, unsigned g, float y[3]) {
+    float rt[K][3], vr[K][3];  // r (then t), v
+    // y = A L at local vertex j (global index g); 32-bit offsets from the
+    // uniform bases
+    auto spmv = [&](int j, unsigned g, float y[3]) {
         float cm[6];
         int col[6];
 #pragma unroll
@@ -721,7 +706,7 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
             col[m] = nbr[(unsigned)m * (unsigned)nv + g];
         }
 #pragma unroll
-        for (int d = 0; d < 3; d++) y[d] = L[3 * jc + d];
+        for (int d = 0; d < 3; d++) y[d] = L[3 * j + d];
 #pragma unroll
         for (int m = 0; m < 6; m++)
 #pragma unroll
@@ -731,15 +716,17 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = threadIdx.x + k * NT;
-        const bool ok = j < V;
-        const unsigned g = (unsigned)(v0 + min(j, V - 1));
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const float b = ok ? bv[3 * g + d] : 0.0f;
-            if (ok) x[3 * g + d] = 0.0f;
-            rt[k][d] = b;
-            if (VREG) vr[k][d] = 0.0f;
-            acc[0] += (double)b * (double)b;
+        for (int d = 0; d < 3; d++) { rt[k][d] = 0.0f; vr[k][d] = 0.0f; }
+        if (j < V) {
+            const unsigned g = (unsigned)(v0 + j);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const float b = bv[3 * g + d];
+                x[3 * g + d] = 0.0f;
+                rt[k][d] = b;
+                acc[0] += (double)b * (double)b;
+            }
         }
     }
     red.sum<1>(*(double(*)[1])acc);
@@ -756,27 +743,23 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
     for (it = 0; it < itrmax; it++) {
         if (rr == 0.0) { flag = 1; break; }
         // p = r (first pass) or r + beta (p - omeg v), into LDS and memory
-        const bool first = it == 0;
-        const double beta = first ? 0.0 : (rr / rr_old) * (alfa / omeg);
+        const double beta = it > 0 ? (rr / rr_old) * (alfa / omeg) : 0.0;
         {
             const int t0 = opaque(threadIdx.x), b0 = opaque(v0);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int j = t0 + k * NT;
-                const bool ok = j < V;
-                const int jc = min(j, V - 1);
-                const unsigned g = (unsigned)(b0 + jc);
+                if (j < V) {
+                    const unsigned g = (unsigned)(b0 + j);
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const float po = pg[3 * g + d];
-                    const float vo = VREG ? vr[k][d] : vg[3 * g + d];
-                    const float pn = first ? rt[k][d] : (float)(wide(rt[k][d]) + beta * ((double)po - omeg * wide(vo)));
-                    if (ok) {
+                    for (int d = 0; d < 3; d++) {
+                        float pn = rt[k][d];
+                        if (it > 0)
+                            pn = (float)((double)rt[k][d] + beta * ((double)pg[3 * g + d] - omeg * (double)vr[k][d]));
                         pg[3 * g + d] = pn;
                         L[3 * j + d] = pn;
                     }
                 }
-                fence(k);
             }
         }
         __syncthreads();
@@ -787,38 +770,25 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int j = t0 + k * NT;
-                const bool ok = j < V;
-                const int jc = min(j, V - 1);
-                const unsigned g = (unsigned)(b0 + jc);
-                float y[3];
-                spmv(jc, g, y);
+                if (j < V) {
+                    const unsigned g = (unsigned)(b0 + j);
+                    spmv(j, g, vr[k]);
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const float vn = ok ? y[d] : 0.0f;
-                    if (VREG) vr[k][d] = vn;
-                    else if (ok) vg[3 * g + d] = vn;
-                    acc[0] += wide(vn) * (double)bv[3 * g + d];
+                    for (int d = 0; d < 3; d++) acc[0] += (double)vr[k][d] * (double)bv[3 * g + d];
                 }
-                fence(k);
             }
         }
         red.sum<1>(*(double(*)[1])acc);  // its barrier ends the reads of p
         alfa = rr / acc[0];
         // s = r - alfa v into LDS
         {
-            const int t0 = opaque(threadIdx.x), b0 = opaque(v0);
+            const int t0 = opaque(threadIdx.x);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int j = t0 + k * NT;
-                const int jc = min(j, V - 1);
-                const unsigned g = (unsigned)(b0 + jc);
+                if (j < V)
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const float vo = VREG ? vr[k][d] : vg[3 * g + d];
-                    const float sv = (float)(wide(rt[k][d]) - alfa * wide(vo));
-                    if (j < V) L[3 * j + d] = sv;
-                }
-                fence(k);
+                    for (int d = 0; d < 3; d++) L[3 * j + d] = (float)((double)rt[k][d] - alfa * (double)vr[k][d]);
             }
         }
         __syncthreads();
@@ -829,17 +799,14 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int j = t0 + k * NT;
-                const bool ok = j < V;
-                const int jc = min(j, V - 1);
-                float y[3];
-                spmv(jc, (unsigned)(b0 + jc), y);
+                if (j < V) {
+                    spmv(j, (unsigned)(b0 + j), rt[k]);
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    rt[k][d] = ok ? y[d] : 0.0f;
-                    acc[0] += (double)L[3 * jc + d] * wide(rt[k][d]);
-                    acc[1] += wide(rt[k][d]) * wide(rt[k][d]);
+                    for (int d = 0; d < 3; d++) {
+                        acc[0] += (double)L[3 * j + d] * (double)rt[k][d];
+                        acc[1] += (double)rt[k][d] * (double)rt[k][d];
+                    }
                 }
-                fence(k);
             }
         }
         red.sum<2>(acc);
@@ -851,20 +818,18 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int j = t0 + k * NT;
-                const bool ok = j < V;
-                const int jc = min(j, V - 1);
-                const unsigned g = (unsigned)(b0 + jc);
+                if (j < V) {
+                    const unsigned g = (unsigned)(b0 + j);
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const double si = (double)L[3 * jc + d];
-                    const float xn = (float)((double)x[3 * g + d] + alfa * (double)pg[3 * g + d] + omeg * si);
-                    if (ok) x[3 * g + d] = xn;
-                    const float rn = ok ? (float)(si - omeg * wide(rt[k][d])) : 0.0f;
-                    rt[k][d] = rn;
-                    acc[0] += (double)rn * (double)rn;
-                    acc[1] += (double)rn * (double)bv[3 * g + d];
+                    for (int d = 0; d < 3; d++) {
+                        const double si = (double)L[3 * j + d];
+                        x[3 * g + d] = (float)((double)x[3 * g + d] + alfa * (double)pg[3 * g + d] + omeg * si);
+                        const float rn = (float)(si - omeg * (double)rt[k][d]);
+                        rt[k][d] = rn;
+                        acc[0] += (double)rn * (double)rn;
+                        acc[1] += (double)rn * (double)bv[3 * g + d];
+                    }
                 }
-                fence(k);
             }
         }
         red.sum<2>(acc);  // its barrier ends the reads of s before p is written
@@ -1150,9 +1115,8 @@ lbmibm_status launch_onchip(lbmibm_ctx *ctx, int cls, int var, const SolveOut<T>
 
 // Forms of the large-cell class: 0 = vectors in global memory, 1 = p in LDS,
 // 2 = register-resident (FP32; 1 and 2 only when the largest cell fits).
-constexpr int BICG_NFORM = 6;
-constexpr const char *BICG_FORM_NAMES[BICG_NFORM] = {"global", "p-in-LDS", "resident/2", "resident/4",
-                                                     "resident-vmem/2", "resident-vmem/4"};
+constexpr int BICG_NFORM = 3;
+constexpr const char *BICG_FORM_NAMES[BICG_NFORM] = {"global", "p-in-LDS", "resident"};
 
 bool big_form_fits(const lbmibm_ctx *ctx, int form) {
     if (form == 0) return true;
@@ -1165,20 +1129,12 @@ template <typename T>
 lbmibm_status launch_big(lbmibm_ctx *ctx, int var, const SolveOut<T> &out, hipStream_t stream) {
     const lbmibm_params &p = ctx->p;
     if constexpr (sizeof(T) == 4) {
-        if (var >= 2 && big_form_fits(ctx, var)) {
+        if (var == 2 && big_form_fits(ctx, 2)) {
             const size_t lds = 3 * (size_t)ctx->bicg_maxv[2] * sizeof(float);
-#define LBMIBM_RES(KB, VREG)                                                                                      \
-    hipLaunchKernelGGL((k_bicg_res<BICG_RES_K, KB, VREG>), dim3(ctx->bicg_n[2]), dim3(1024), lds, stream,        \
-                       ctx->bicg_cells[2], ctx->voff, (const float *)ctx->coef, ctx->nbr, (const float *)ctx->bv, \
-                       (float *)ctx->rh, (float *)ctx->pv, (float *)ctx->vv, out, ctx->nv, p.itrmax, p.residual,  \
-                       ctx->bicg_iters, ctx->status)
-            switch (var) {
-                case 2: LBMIBM_RES(2, true); break;
-                case 3: LBMIBM_RES(4, true); break;
-                case 4: LBMIBM_RES(2, false); break;
-                default: LBMIBM_RES(4, false); break;
-            }
-#undef LBMIBM_RES
+            hipLaunchKernelGGL(k_bicg_res<BICG_RES_K>, dim3(ctx->bicg_n[2]), dim3(1024), lds, stream,
+                               ctx->bicg_cells[2], ctx->voff, (const float *)ctx->coef, ctx->nbr,
+                               (const float *)ctx->bv, (float *)ctx->rh, (float *)ctx->pv, out, ctx->nv, p.itrmax,
+                               p.residual, ctx->bicg_iters, ctx->status);
             LBMIBM_LAUNCHED(ctx, "k_bicg_res");
             return LBMIBM_OK;
         }
@@ -1487,6 +1443,41 @@ __global__ __launch_bounds__(256) void k_adj_load(const int *__restrict__ nln,
     if (gsa) atomicAdd(&gsa[vcell[jj]], sb * ds);
 }
 
+// Per cell (one work group): scale the adjoint right-hand side by a power of
+// two so that its largest entry is of order 1, and keep the scale.  The
+// adjoint loads of cells far from where the objective is measured can be
+// many orders of magnitude below those near it; in FP32 they would reach the
+// subnormal range, where the solver cannot reach a relative residual.  Power
+// of two scaling is exact, and the solution is scaled back after the solve.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_scale(const int *__restrict__ voff, T *__restrict__ bv,
+                                                   double *__restrict__ cscale) {
+    const int c = blockIdx.x;
+    const size_t b0 = 3 * (size_t)voff[c], b1 = 3 * (size_t)voff[c + 1];
+    __shared__ float wmax[4];
+    float m = 0.0f;
+    for (size_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) m = fmaxf(m, fabsf((float)bv[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    const double s = m > 0.0f ? ldexp(1.0, -ilogb((double)m)) : 1.0;
+    for (size_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) bv[i] = (T)((double)bv[i] * s);
+    if (threadIdx.x == 0) cscale[c] = s;
+}
+
+// Per vertex: undo the right-hand-side scale on the solution.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lam_unscale(const int *__restrict__ vcell, const double *__restrict__ cscale,
+                                                     T *__restrict__ lam, int nv) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nv) return;
+    const double s = 1.0 / cscale[vcell[j]];
+#pragma unroll
+    for (int k = 0; k < 3; k++) lam[3 * (size_t)j + k] = (T)((double)lam[3 * (size_t)j + k] * s);
+}
+
 // Per element: its position adjoints (rg terms and J_b^T lambda) into the
 // slots of its corners.
 template <typename T>
@@ -1589,6 +1580,9 @@ template <typename T> lbmibm_status forces_adj_impl(lbmibm_ctx *ctx, bool have_q
                        (const typename Corner<T>::type *)ctx->corner, ctx->vcell, (const T *)ctx->dsv,
                        (const T *)ctx->qv, (const T *)ctx->adj_cf, (T *)ctx->bv, ctx->adj_dsv, ctx->grad_gs, nv);
     LBMIBM_LAUNCHED(ctx, "k_adj_load");
+    hipLaunchKernelGGL(k_rhs_scale<T>, dim3(ctx->nc), dim3(256), 0, ctx->stream, ctx->voff, (T *)ctx->bv,
+                       ctx->adj_cscale);
+    LBMIBM_LAUNCHED(ctx, "k_rhs_scale");
     SolveOut<T> out = solve_out<T>(ctx);
     out.q = (T *)ctx->adj_lam;
     out.cf = (T *)ctx->adj_scratch;
@@ -1603,6 +1597,9 @@ template <typename T> lbmibm_status forces_adj_impl(lbmibm_ctx *ctx, bool have_q
         s = launch_big<T>(ctx, ctx->bicg_var[2], out, ctx->stream);
         if (s != LBMIBM_OK) return s;
     }
+    hipLaunchKernelGGL(k_lam_unscale<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->vcell,
+                       ctx->adj_cscale, (T *)ctx->adj_lam, nv);
+    LBMIBM_LAUNCHED(ctx, "k_lam_unscale");
     hipLaunchKernelGGL(k_elem_adj<T>, dim3((ne + 255) / 256), dim3(256), 0, ctx->stream, (const double *)ctx->xd,
                        ctx->ele, (const double *)ctx->nref, ctx->cslot, ctx->vcell, (const T *)ctx->gscale,
                        (const T *)ctx->adj_lam, (const T *)ctx->qv, ctx->adj_dsv, ctx->adj_corner, ne, box_of(ctx),
@@ -1659,10 +1656,7 @@ lbmibm_status membrane_setup(lbmibm_ctx *ctx) {
     }
     if (ctx->bicg_n[2] > 0 && big_form_fits(ctx, 2)) {
         const int lds = (int)(3 * (size_t)ctx->bicg_maxv[2] * sizeof(float));
-        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K, 2, true>, A, lds));
-        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K, 4, true>, A, lds));
-        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K, 2, false>, A, lds));
-        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K, 4, false>, A, lds));
+        LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K>, A, lds));
     }
     if (ctx->bicg_n[1] > 0) {
         const int lds = (int)(3 * 4 * (size_t)ctx->bicg_maxv[1] * sizeof(float));

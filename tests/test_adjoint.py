@@ -90,7 +90,7 @@ def test_lbm_vjp(walls):
 
 def _cell_case(seed, levels=(2, 2), lx=16, ly=14, lz=16, **kw):
     cells = small_cells(lx, ly, lz, nc=len(levels), radius=3.0, levels=list(levels), seed=seed)
-    base = dict(tau=0.8, b=0.5, c=10.0, residual=1e-14, volume_correction=False)
+    base = dict(tau=0.8, b=0.5, c=10.0, residual=1e-13, volume_correction=False)
     base.update(kw)
     g = gpu_of(lx, ly, lz, "f64", cells=cells, **base)
     return g, cells
@@ -195,7 +195,7 @@ CASE = dict(lx=16, ly=12, lz=14, nsteps=12, window=6)
 def _run_case(prec, seed=51):
     lx, ly, lz = CASE["lx"], CASE["ly"], CASE["lz"]
     cells = small_cells(lx, ly, lz, nc=2, radius=3.0, levels=[2, 2], seed=seed)
-    kw = dict(tau=0.8, p=1e-4, b=5e-2, c=10.0, residual=1e-14 if prec == "f64" else 1e-9,
+    kw = dict(tau=0.8, p=1e-4, b=5e-2, c=10.0, residual=1e-13 if prec == "f64" else 1e-9,
               volume_correction=False)
     return cells, kw, gpu_of(lx, ly, lz, prec, cells=cells, **kw)
 

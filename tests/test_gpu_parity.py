@@ -321,12 +321,12 @@ def test_spread_is_bitwise_reproducible():
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_every_solver_shape_matches_the_oracle(variant, monkeypatch):
     """Each candidate work-group shape of the on-chip solver classes and each
     form of the large-cell solver (the autotuner picks one of each) gives the
     oracle's loads and forces."""
-    monkeypatch.setenv("LBMIBM_BICG_SHAPES", f"{variant % 3},{variant % 3},{variant}")
+    monkeypatch.setenv("LBMIBM_BICG_SHAPES", f"{variant},{variant},{variant}")
     lx, ly, lz = 28, 24, 28
     cells = small_cells(lx, ly, lz, nc=4, radius=4.0, levels=[3, 4, 2, 5], seed=61)
     kw = dict(c=100.0, br=0.0)
