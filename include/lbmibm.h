@@ -81,9 +81,9 @@ typedef struct lbmibm_params {
     int    neohookean;       /* 0: SKALAK, 1: NEOHOOKEAN (stress.cu:95-99)                   */
     int    volume_correction;/* 1: lbmibm_step runs volume correction (main.cu:101-106)      */
     int    precision;        /* lbmibm_precision                                             */
-    int    store_macros;     /* 1: keep dn,un,vn,wn and the dm accumulation each step (the
-                                reference's outputs); 0: only the M-averaged velocity that
-                                the interpolation reads                                      */
+    int    store_macros;     /* 1: keep dn,un,vn,wn and the dm accumulation each step, the
+                                tension and rg of each element and the vertex velocity (the
+                                reference's outputs); 0: only what the next step reads      */
     int    fileoutput;       /* FILEOUTPUT period of the tp / qi accumulators (0: off)       */
 } lbmibm_params;
 
@@ -160,7 +160,9 @@ lbmibm_status lbmibm_upload_macros(lbmibm_ctx *ctx, const double *um, const doub
 lbmibm_status lbmibm_download_lattice_force(lbmibm_ctx *ctx, double *fx, double *fy, double *fz);
 lbmibm_status lbmibm_upload_lattice_force(lbmibm_ctx *ctx, const double *fx, const double *fy,
                                           const double *fz);
-/* Cell state; any pointer may be NULL.  x,y,z,u,v,w,fx,fy,fz [Nv]; xd [3Nv]. */
+/* Cell state; any pointer may be NULL.  x,y,z,u,v,w,fx,fy,fz [Nv]; xd [3Nv].
+ * The interpolated vertex velocity u,v,w (cel->u..) is kept with
+ * store_macros=1, like the other outputs that the step itself does not read. */
 lbmibm_status lbmibm_download_cells(lbmibm_ctx *ctx, double *x, double *y, double *z, double *u,
                                     double *v, double *w, double *fx, double *fy, double *fz, double *xd);
 lbmibm_status lbmibm_upload_cells(lbmibm_ctx *ctx, const double *x, const double *y, const double *z,

@@ -712,7 +712,7 @@ __global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, d
             if (yr < -a.ly / 2.0) yr += a.ly;
             if (yr > a.ly / 2.0) yr -= a.ly;
         }
-        uvw[j] = um; uvw[nv + j] = vm; uvw[2 * (size_t)nv + j] = wm;
+        if (uvw) { uvw[j] = um; uvw[nv + j] = vm; uvw[2 * (size_t)nv + j] = wm; }
         xs[j] = xr; xs[nv + j] = yr; xs[2 * (size_t)nv + j] = zr;
         xd[3 * (size_t)j] = xr; xd[3 * (size_t)j + 1] = yr; xd[3 * (size_t)j + 2] = zr;
     }
@@ -942,7 +942,8 @@ template <typename T> lbmibm_status interp_impl(lbmibm_ctx *ctx) {
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const size_t lds = 3 * (size_t)RN * sizeof(T);
     hipLaunchKernelGGL(k_interp_tiles<T>, dim3(ctx->ntiles), dim3(256), lds, ctx->stream, (const T *)ctx->U,
-                       (double *)ctx->xs, (T *)ctx->uvw, (double *)ctx->xd, ctx->tile_count, ctx->tile_off,
+                       (double *)ctx->xs, ctx->p.store_macros ? (T *)ctx->uvw : (T *)nullptr, (double *)ctx->xd,
+                       ctx->tile_count, ctx->tile_off,
                        ctx->home_list, ctx->nv, geom_of(ctx), tg, a, ctx->order_interp);
     LBMIBM_LAUNCHED(ctx, "k_interp_tiles");
     ctx->bins_valid = false;  // the positions moved
