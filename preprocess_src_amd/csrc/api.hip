@@ -151,18 +151,43 @@ lbmibm_status reset_status(lbmibm_ctx *ctx) {
     return LBMIBM_OK;
 }
 
-// One main.cu loop body on the stream, no host synchronisation.
-lbmibm_status enqueue_step(lbmibm_ctx *ctx, int iter, bool device_advance) {
+lbmibm_status join_volume(lbmibm_ctx *ctx) {
+    if (!ctx->vol_pending) return LBMIBM_OK;
+    ctx->vol_pending = false;
+    LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_pjoin, ctx->vstream));
+    LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_pjoin, 0));
+    return LBMIBM_OK;
+}
+
+// One main.cu loop body on the stream, no host synchronisation.  With
+// overlap_volume the volume correction is left running on vstream (joined by
+// the next step before it reads the positions, or by join_volume).  The
+// membrane force reads fem->xd, which the correction does not change
+// (volume_correction.cu:453-524 moves cel->x only), so the two may overlap;
+// bending reads cel->x, so with BR != 0 the correction is joined first.
+lbmibm_status enqueue_step(lbmibm_ctx *ctx, int iter, bool device_advance, bool overlap_volume) {
     lbmibm_status s = set_step(ctx, device_advance ? -1 : iter, false);
     const int M = ctx->p.m;
     if (s == LBMIBM_OK && (iter - 1) % M == 0 && ctx->nc > 0) {
-        s = membrane_forces(ctx);
+        if (ctx->p.br != 0.0) s = join_volume(ctx);
+        if (s == LBMIBM_OK) s = membrane_forces(ctx);
+        if (s == LBMIBM_OK) s = join_volume(ctx);
         if (s == LBMIBM_OK) s = ibm_spread(ctx);
     }
     if (s == LBMIBM_OK) s = lattice_step(ctx);
     if (s == LBMIBM_OK && iter % M == 0 && ctx->nc > 0) {
-        s = ibm_interp(ctx);
-        if (s == LBMIBM_OK && ctx->p.volume_correction) s = volume_run(ctx, 1);
+        s = join_volume(ctx);
+        if (s == LBMIBM_OK) s = ibm_interp(ctx);
+        if (s == LBMIBM_OK && ctx->p.volume_correction) {
+            if (overlap_volume && ctx->p.br == 0.0) {
+                LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_pfork, ctx->stream));
+                LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->vstream, ctx->ev_pfork, 0));
+                s = volume_run_on(ctx, 1, ctx->vstream);
+                ctx->vol_pending = true;
+            } else {
+                s = volume_run(ctx, 1);
+            }
+        }
     }
     return s;
 }
@@ -192,6 +217,9 @@ void free_all(lbmibm_ctx *ctx) {
     if (ctx->ev_vfork) (void)hipEventDestroy(ctx->ev_vfork);
     if (ctx->ev_vjoin) (void)hipEventDestroy(ctx->ev_vjoin);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->ev_pfork) (void)hipEventDestroy(ctx->ev_pfork);
+    if (ctx->ev_pjoin) (void)hipEventDestroy(ctx->ev_pjoin);
+    if (ctx->vstream) (void)hipStreamDestroy(ctx->vstream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
@@ -277,6 +305,9 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     if (hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
     if (hipEventCreateWithFlags(&ctx->ev_vfork, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
     if (hipEventCreateWithFlags(&ctx->ev_vjoin, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+    if (hipStreamCreateWithFlags(&ctx->vstream, hipStreamNonBlocking) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+    if (hipEventCreateWithFlags(&ctx->ev_pfork, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+    if (hipEventCreateWithFlags(&ctx->ev_pjoin, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
 
     const size_t szS = ctx->szS, szT = ctx->szT;
     CK(dev_alloc(ctx, &ctx->f[0], (size_t)Q * N * szS));
@@ -561,16 +592,20 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
     if (s != LBMIBM_OK) return s;
     const int M = ctx->p.m;
     // a replayable block is whole M-cycles with an even number of lattice
-    // steps, so that the population buffers are back in place
-    const int L = (M % 2 == 0) ? M : 2 * M;
+    // steps, so that the population buffers are back in place, repeated to at
+    // least 8 steps: a step's volume correction overlaps the next step's
+    // membrane force, except across block ends
+    const int L0 = (M % 2 == 0) ? M : 2 * M;
+    const int L = L0 * ((8 + L0 - 1) / L0);
     int it = iter0;
     const int end = iter0 + nsteps;
     s = set_step(ctx, iter0, false);
     // eager steps until the next step starts an M-cycle
     while (s == LBMIBM_OK && it < end && (use_graph == 0 || (it % M) != 0)) {
-        s = enqueue_step(ctx, it + 1, true);
+        s = enqueue_step(ctx, it + 1, true, true);
         it++;
     }
+    if (s == LBMIBM_OK) s = join_volume(ctx);  // nothing pending across a capture boundary
     if (s == LBMIBM_OK && use_graph && end - it >= L) {
         if (ctx->graph_ready && (ctx->graph_cur != ctx->cur || ctx->graph_force_zero != ctx->force_zero)) {
             (void)hipGraphExecDestroy(ctx->graph);
@@ -583,7 +618,8 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
             hipGraph_t g = nullptr;
             LBMIBM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             lbmibm_status cs = LBMIBM_OK;
-            for (int k = 0; k < L && cs == LBMIBM_OK; k++) cs = enqueue_step(ctx, it + k + 1, true);
+            for (int k = 0; k < L && cs == LBMIBM_OK; k++) cs = enqueue_step(ctx, it + k + 1, true, true);
+            if (cs == LBMIBM_OK) cs = join_volume(ctx);
             hipError_t e = hipStreamEndCapture(ctx->stream, &g);
             if (cs != LBMIBM_OK) return cs;
             if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
@@ -598,8 +634,14 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
         }
     }
     while (s == LBMIBM_OK && it < end) {
-        s = enqueue_step(ctx, it + 1, true);
+        s = enqueue_step(ctx, it + 1, true, true);
         it++;
+    }
+    if (s == LBMIBM_OK) {
+        s = join_volume(ctx);
+    } else {  // an enqueue failed: let anything left on the third stream finish
+        ctx->vol_pending = false;
+        (void)hipStreamSynchronize(ctx->vstream);
     }
     return s == LBMIBM_OK ? check_flags(ctx, nullptr) : s;
 }

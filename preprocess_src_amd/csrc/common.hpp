@@ -50,7 +50,14 @@ struct lbmibm_ctx {
     // and joined back into `stream` with events; captured graphs keep the fork
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;    // membrane solve
-    hipEvent_t ev_vfork = nullptr, ev_vjoin = nullptr;  // volume correction
+    hipEvent_t ev_vfork = nullptr, ev_vjoin = nullptr;  // volume correction classes
+    // third stream (default priority): inside lbmibm_run, the volume
+    // correction of one step overlaps the membrane force of the next, which
+    // reads fem->xd and not the corrected cel->x (when BR = 0); it is joined
+    // before the next spreading or interpolation reads the positions
+    hipStream_t vstream = nullptr;
+    hipEvent_t ev_pfork = nullptr, ev_pjoin = nullptr;
+    bool vol_pending = false;
     std::string err;
 
     // lattice (device)
@@ -163,6 +170,8 @@ int bicg_class(int prec, int nvert);                         // membrane.hip
 lbmibm_status membrane_setup(lbmibm_ctx *ctx);               // membrane.hip
 lbmibm_status membrane_autotune(lbmibm_ctx *ctx);            // membrane.hip
 lbmibm_status volume_run(lbmibm_ctx *ctx, int mode);         // volume.hip (0 init, 1 correct)
+// both classes on the given stream (not the context's main stream)
+lbmibm_status volume_run_on(lbmibm_ctx *ctx, int mode, hipStream_t stream);  // volume.hip
 lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool advance);  // api.hip
 // api.hip helpers shared with adjoint.hip
 lbmibm_status dev_alloc(lbmibm_ctx *ctx, void **p, size_t bytes);
@@ -174,7 +183,8 @@ long long push_target(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q);  //
 lbmibm_status sync_check(lbmibm_ctx *ctx);
 lbmibm_status check_flags(lbmibm_ctx *ctx, int *vol_iters);
 lbmibm_status reset_status(lbmibm_ctx *ctx);
-lbmibm_status enqueue_step(lbmibm_ctx *ctx, int iter, bool device_advance);
+lbmibm_status enqueue_step(lbmibm_ctx *ctx, int iter, bool device_advance, bool overlap_volume = false);
+lbmibm_status join_volume(lbmibm_ctx *ctx);  // joins a volume correction left on vstream
 
 inline lbmibm_status hip_fail(lbmibm_ctx *ctx, hipError_t e, const char *what) {
     ctx->err = std::string(what) + ": " + hipGetErrorString(e);

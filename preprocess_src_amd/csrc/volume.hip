@@ -141,13 +141,14 @@ __global__ __launch_bounds__(NT) void k_volume(const int *__restrict__ cells, in
     if (threadIdx.x == 0) { gx[3 * c] = g[0]; gx[3 * c + 1] = g[1]; gx[3 * c + 2] = g[2]; }
 }
 
-lbmibm_status run_impl(lbmibm_ctx *ctx, int mode) {
+lbmibm_status run_impl(lbmibm_ctx *ctx, int mode, hipStream_t stream) {
     VolArgs a{ctx->nx * ctx->p.dx, ctx->ny * ctx->p.dx, ctx->nz * ctx->p.dx, ctx->p.periodic_y};
     // cells of up to VOL_SMALL vertices with 256 threads, larger ones with
-    // 1024; the two classes touch disjoint cells, so the large one runs on the
-    // side stream (forked and joined with events) concurrently with the other
-    const bool fork = ctx->vol_n[0] > 0 && ctx->vol_n[1] > 0;
-    hipStream_t big = ctx->stream;
+    // 1024; the two classes touch disjoint cells, so on the main stream the
+    // large one runs on the side stream (forked and joined with events)
+    // concurrently with the other
+    const bool fork = stream == ctx->stream && ctx->vol_n[0] > 0 && ctx->vol_n[1] > 0;
+    hipStream_t big = stream;
     if (fork) {
         LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_vfork, ctx->stream));
         LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_vfork, 0));
@@ -160,7 +161,7 @@ lbmibm_status run_impl(lbmibm_ctx *ctx, int mode) {
         LBMIBM_LAUNCHED(ctx, "k_volume<1024>");
     }
     if (ctx->vol_n[0] > 0) {
-        hipLaunchKernelGGL(k_volume<256>, dim3(ctx->vol_n[0]), dim3(256), 0, ctx->stream, ctx->vol_cells[0], mode,
+        hipLaunchKernelGGL(k_volume<256>, dim3(ctx->vol_n[0]), dim3(256), 0, stream, ctx->vol_cells[0], mode,
                            (double *)ctx->xs, ctx->nv, ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le,
                            ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->vol_passes, ctx->status, a);
         LBMIBM_LAUNCHED(ctx, "k_volume<256>");
@@ -174,10 +175,12 @@ lbmibm_status run_impl(lbmibm_ctx *ctx, int mode) {
 
 }  // namespace
 
-lbmibm_status volume_run(lbmibm_ctx *ctx, int mode) {
+lbmibm_status volume_run(lbmibm_ctx *ctx, int mode) { return volume_run_on(ctx, mode, ctx->stream); }
+
+lbmibm_status volume_run_on(lbmibm_ctx *ctx, int mode, hipStream_t stream) {
     if (ctx->nc == 0) return LBMIBM_OK;
     if (mode == 1) ctx->bins_valid = false;  // the correction moves the vertices
-    return run_impl(ctx, mode);
+    return run_impl(ctx, mode, stream);
 }
 
 }  // namespace lbmibm

@@ -282,9 +282,11 @@ def test_step_parity(prec, m):
 def test_run_graph_matches_steps(prec):
     o, g1, nsteps = _step_pair(prec, m=1)
     _, g2, _ = _step_pair(prec, m=1)
-    for it in range(1, 9):
+    # 20 steps: two replays of the 8-step graph and 4 eager steps, with each
+    # volume correction overlapping the next membrane force, against single steps
+    for it in range(1, 21):
         g1.step(it)
-    g2.run(0, 8, graph=True)
+    g2.run(0, 20, graph=True)
     p1, p2 = g1.populations(), g2.populations()
     comp = o.bc == 0
     assert rel(comp_pops(o, p2), comp_pops(o, p1)) < (1e-13 if prec == "f64" else 1e-6)
