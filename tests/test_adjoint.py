@@ -243,7 +243,7 @@ def test_wall_shear_gradient_matches_gpu_fd_and_checkpoint_interval():
     J1, gH1, gs1 = g.adjoint_wall_shear(0, n, w, 4)
     _reset(g, cells)
     J2, gH2, gs2 = g.adjoint_wall_shear(0, n, w, 5)  # uneven last segment
-    assert J1 == J2
+    assert abs(J1 - J2) <= 1e-13 * abs(J1)  # the objective sums with atomics: order-dependent last bits
     for a, b in zip(gH1, gH2):
         assert np.max(np.abs(a - b)) <= 1e-10 * np.max(np.abs(a))
     assert np.max(np.abs(gs1 - gs2)) <= 1e-10 * np.max(np.abs(gs1))
