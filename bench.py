@@ -72,7 +72,8 @@ def adjoint_bench(a):
     from preprocess_src_amd.simulation import Simulation
     shape, cells, params, desc, ht = adjoint_workload()
     lx, ly, lz = shape
-    nsteps, window, K = 500, 200, 50
+    nsteps = a.adj_steps
+    window, K = (2 * nsteps) // 5, min(50, nsteps)
     sim = Simulation(lx, ly, lz, cells=cells, precision=a.precision, **params)
     x0 = np.concatenate([cells["x"], cells["y"], cells["z"]])
 
@@ -249,6 +250,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--adj-steps", type=int, default=500, help="steps of the adjoint workload (window 40%%)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path is single-GPU, "

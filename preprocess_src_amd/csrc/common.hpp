@@ -108,6 +108,7 @@ struct lbmibm_ctx {
     int *bicg_cells[3] = {nullptr, nullptr, nullptr};
     int bicg_n[3] = {0, 0, 0}, bicg_maxv[3] = {0, 0, 0};
     int bicg_var[3] = {0, 0, 0};  // shape variant of the on-chip classes, form of class 2 (autotuned)
+    int big_lds = 0;              // dynamic LDS of the class-2 forms that use LDS (the whole CU)
     int *onchip_cells = nullptr;  // cells of classes 0 and 1
     int n_onchip = 0;
     int cls_maxe[2] = {0, 0}, cls_maxv[2] = {0, 0};  // largest element / vertex count: on-chip, large

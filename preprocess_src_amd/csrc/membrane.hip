@@ -26,6 +26,8 @@
 // rounding of |x| ~ 10-100 positions into percent-level force errors.  The
 // stored tension, matrix rows, right-hand sides, BiCGSTAB vectors and nodal
 // forces use the arithmetic type T; inner products are FP64.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace lbmibm {
@@ -128,8 +130,7 @@ template <typename T> struct ElemTension {
 
 // The corner records go to the slot of the corner's vertex: cslot[3e+en] =
 // 6 j + s for vertex j = ele[3e+en] with e its s-th incident element, so the
-// per-vertex pass reads each vertex's records contiguously.  (Used where the
-// records themselves are needed, by the adjoint; the forward uses k_assemble.)
+// per-vertex pass reads each vertex's records contiguously.
 template <typename T>
 __global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, const int *__restrict__ ele,
                                               const double *__restrict__ nref, T *__restrict__ tt,
@@ -147,9 +148,11 @@ __global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, con
     }
     if (j >= jend) return;
     const ElemTension<T> el(xd, ele[3 * j], ele[3 * j + 1], ele[3 * j + 2], nref + 4 * (size_t)j, box, a);
-    rg[j] = (T)el.rg;
+    if (tt) {  // outputs (td, rgd), not read by the step
+        rg[j] = (T)el.rg;
 #pragma unroll
-    for (int k = 0; k < 4; k++) tt[4 * j + k] = (T)el.t[k];
+        for (int k = 0; k < 4; k++) tt[4 * j + k] = (T)el.t[k];
+    }
 #pragma unroll
     for (int en = 0; en < 3; en++) {
         double tb[3];
@@ -197,72 +200,6 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
             tb[0] += (double)cv.x;
             tb[1] += (double)cv.y;
             tb[2] += (double)cv.z;
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < 6; m++) {
-        const unsigned ep = epair[6 * (size_t)jj + m];
-        double ra = 0.0, rb = 0.0;
-#pragma unroll
-        for (int s = 0; s < 6; s++) {
-            if ((int)(ep & 15u) == s) ra = rgs[s];
-            if ((int)(ep >> 4) == s) rb = rgs[s];
-        }
-        coef[(size_t)m * nv + jj] = ep == 0xffu ? T(0) : (T)((ra / 24.0 + rb / 24.0) / jac);
-    }
-#pragma unroll
-    for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
-    dsv[jj] = (T)(ds * (double)gscale[vcell[jj]]);
-}
-
-// calc_tension + calc_load + q2f ring area fused per vertex: each thread
-// recomputes the tension of its (at most 6) incident elements, in the
-// reference's le order, and keeps only its own corner's terms, so no
-// per-corner records travel through memory.  Arithmetic and summation order
-// are those of k_elem + k_vert (results identical).  The tension and rg
-// outputs (td, rgd) and the tp accumulator are written by the element's
-// first-corner thread, and only when asked for (tt / tp non-null).
-template <typename T>
-__global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ xd, const int *__restrict__ ele,
-                                                  const double *__restrict__ nref, const int *__restrict__ nln,
-                                                  const int *__restrict__ le, const unsigned char *__restrict__ epair,
-                                                  const int *__restrict__ vcell, const T *__restrict__ gscale,
-                                                  T *__restrict__ coef, T *__restrict__ bv, T *__restrict__ dsv,
-                                                  T *__restrict__ tt, T *__restrict__ rgo, T *__restrict__ tp, int nv,
-                                                  PeriodicBox box, TensionArgs a, const DevStep *__restrict__ st,
-                                                  const int *__restrict__ cells, const int *__restrict__ voff) {
-    int jj = blockIdx.x * blockDim.x + threadIdx.x, jend = nv;
-    if (cells) {
-        const int c = cells[blockIdx.y];
-        jj += voff[c];
-        jend = voff[c + 1];
-    }
-    if (jj >= jend) return;
-    const int n = nln[jj];
-    double rgs[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, tb[3] = {0.0, 0.0, 0.0}, jac = 0.0, ds = 0.0;
-#pragma unroll
-    for (int s = 0; s < 6; s++) {
-        if (s >= n) continue;
-        const int e = le[6 * (size_t)jj + s];
-        const int ea = ele[3 * e], eb = ele[3 * e + 1], ec = ele[3 * e + 2];
-        const int en = ea == jj ? 0 : (eb == jj ? 1 : 2);
-        const ElemTension<T> el(xd, ea, eb, ec, nref + 4 * (size_t)e, box, a);
-        double c[3];
-        el.corner(en, c);
-        const double gab = (double)(T)el.gab;
-        rgs[s] = gab;
-        jac += gab / 12.0;
-        ds += gab / 2.0 / 3.0;
-        tb[0] += (double)(T)c[0];
-        tb[1] += (double)(T)c[1];
-        tb[2] += (double)(T)c[2];
-        if (en == 0) {
-            if (tt) {
-                rgo[e] = (T)el.rg;
-#pragma unroll
-                for (int k = 0; k < 4; k++) tt[4 * (size_t)e + k] = (T)el.t[k];
-            }
-            if (tp) el.accumulate_tp(tp, e, st->iter, a);
         }
     }
 #pragma unroll
@@ -1118,6 +1055,15 @@ lbmibm_status launch_onchip(lbmibm_ctx *ctx, int cls, int var, const SolveOut<T>
 constexpr int BICG_NFORM = 3;
 constexpr const char *BICG_FORM_NAMES[BICG_NFORM] = {"global", "p-in-LDS", "resident"};
 
+// Dynamic LDS of the large-cell forms: what the largest cell needs, raised
+// to the whole of the CU's LDS (ctx->big_lds, set up in membrane_setup), so
+// that no other work group (the on-chip classes run concurrently) shares the
+// CU and its issue slots with a large-cell solve.
+size_t big_lds(const lbmibm_ctx *ctx) {
+    const size_t need = 3 * (size_t)ctx->bicg_maxv[2] * sizeof(float);
+    return std::max(need, (size_t)ctx->big_lds);
+}
+
 bool big_form_fits(const lbmibm_ctx *ctx, int form) {
     if (form == 0) return true;
     if (ctx->prec == LBMIBM_F64) return false;
@@ -1130,7 +1076,7 @@ lbmibm_status launch_big(lbmibm_ctx *ctx, int var, const SolveOut<T> &out, hipSt
     const lbmibm_params &p = ctx->p;
     if constexpr (sizeof(T) == 4) {
         if (var == 2 && big_form_fits(ctx, 2)) {
-            const size_t lds = 3 * (size_t)ctx->bicg_maxv[2] * sizeof(float);
+            const size_t lds = big_lds(ctx);
             hipLaunchKernelGGL(k_bicg_res<BICG_RES_K>, dim3(ctx->bicg_n[2]), dim3(1024), lds, stream,
                                ctx->bicg_cells[2], ctx->voff, (const float *)ctx->coef, ctx->nbr,
                                (const float *)ctx->bv, (float *)ctx->rh, (float *)ctx->pv, out, ctx->nv, p.itrmax,
@@ -1139,7 +1085,7 @@ lbmibm_status launch_big(lbmibm_ctx *ctx, int var, const SolveOut<T> &out, hipSt
             return LBMIBM_OK;
         }
         if (var == 1 && big_form_fits(ctx, 1)) {
-            const size_t lds = 3 * (size_t)ctx->bicg_maxv[2] * sizeof(T);
+            const size_t lds = big_lds(ctx);
             hipLaunchKernelGGL((k_bicg_big<T, 4>), dim3(ctx->bicg_n[2]), dim3(1024), lds, stream, ctx->bicg_cells[2],
                                ctx->voff, (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->rh,
                                (T *)ctx->rv, (T *)ctx->vv, (T *)ctx->tv, out, ctx->nv, p.itrmax, p.residual,
@@ -1182,28 +1128,16 @@ template <typename T> __global__ void k_synth_rhs(T *__restrict__ b, size_t n) {
 
 // Tension, corner terms and rows of all cells (cells == nullptr) or of the
 // listed cells (n of them, at most max_e elements and max_v vertices each).
-// corners: also leave the per-(vertex, slot) corner records (two-pass form),
-// which the adjoint reads; otherwise the fused per-vertex kernel.  The
-// tension and rg outputs are kept with store_macros.
+// The tension and rg outputs are kept with store_macros.
 template <typename T>
 lbmibm_status assemble_impl(lbmibm_ctx *ctx, const PeriodicBox &box, const TensionArgs &ta,
-                            const int *cells = nullptr, int n = 0, int max_e = 0, int max_v = 0,
-                            bool corners = false) {
+                            const int *cells = nullptr, int n = 0, int max_e = 0, int max_v = 0) {
     const lbmibm_params &p = ctx->p;
     const int ne = ctx->nt, nv = ctx->nv;
     const dim3 ge = cells ? dim3((max_e + 255) / 256, n) : dim3((ne + 255) / 256);
     const dim3 gv = cells ? dim3((max_v + 255) / 256, n) : dim3((nv + 255) / 256);
-    if (!corners) {
-        hipLaunchKernelGGL(k_assemble<T>, gv, dim3(256), 0, ctx->stream, (const double *)ctx->xd, ctx->ele,
-                           (const double *)ctx->nref, ctx->nln, ctx->le, ctx->epair, ctx->vcell,
-                           (const T *)ctx->gscale, (T *)ctx->coef, (T *)ctx->bv, (T *)ctx->dsv,
-                           p.store_macros ? (T *)ctx->tt : (T *)nullptr, (T *)ctx->rg,
-                           p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, nv, box, ta, ctx->step, cells, ctx->voff);
-        LBMIBM_LAUNCHED(ctx, "k_assemble");
-        return LBMIBM_OK;
-    }
     hipLaunchKernelGGL(k_elem<T>, ge, dim3(256), 0, ctx->stream, (const double *)ctx->xd, ctx->ele,
-                       (const double *)ctx->nref, (T *)ctx->tt, (T *)ctx->rg,
+                       (const double *)ctx->nref, p.store_macros ? (T *)ctx->tt : (T *)nullptr, (T *)ctx->rg,
                        p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ctx->cslot,
                        (typename Corner<T>::type *)ctx->corner, ne, box, ta, ctx->step, cells, ctx->eoff);
     LBMIBM_LAUNCHED(ctx, "k_elem");
@@ -1572,9 +1506,7 @@ template <typename T> lbmibm_status forces_adj_impl(lbmibm_ctx *ctx, bool have_q
     const int ne = ctx->nt, nv = ctx->nv;
     // forward at the current positions: rows, ring areas, corner records, and
     // q unless the caller restored it (qv) from a recording of this step
-    lbmibm_status s = have_q ? LBMIBM_OK : forces_impl<T>(ctx);
-    // the corner records (rg per incident element) for the adjoint right-hand side
-    if (s == LBMIBM_OK) s = assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx), nullptr, 0, 0, 0, true);
+    lbmibm_status s = have_q ? assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx)) : forces_impl<T>(ctx);
     if (s != LBMIBM_OK) return s;
     hipLaunchKernelGGL(k_adj_load<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln,
                        (const typename Corner<T>::type *)ctx->corner, ctx->vcell, (const T *)ctx->dsv,
@@ -1650,12 +1582,24 @@ lbmibm_status membrane_setup(lbmibm_ctx *ctx) {
             LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_onchip<float, 1, 1024>, A, lds));
         }
     }
+    // large-cell forms with LDS: the whole CU (see big_lds), less their static LDS
+    ctx->big_lds = 0;
+    if (ctx->bicg_n[2] > 0 && (big_form_fits(ctx, 1) || big_form_fits(ctx, 2))) {
+        int dev = 0, lmax = 0;
+        LBMIBM_HIP(ctx, hipGetDevice(&dev));
+        LBMIBM_HIP(ctx, hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        hipFuncAttributes fa1{}, fa2{};
+        LBMIBM_HIP(ctx, hipFuncGetAttributes(&fa1, (const void *)k_bicg_big<float, 4>));
+        LBMIBM_HIP(ctx, hipFuncGetAttributes(&fa2, (const void *)k_bicg_res<BICG_RES_K>));
+        const int stat = (int)std::max(fa1.sharedSizeBytes, fa2.sharedSizeBytes);
+        ctx->big_lds = std::max(0, lmax - stat);
+    }
     if (ctx->bicg_n[2] > 0 && big_form_fits(ctx, 1)) {
-        const int lds = (int)(3 * (size_t)ctx->bicg_maxv[2] * sizeof(float));
+        const int lds = (int)big_lds(ctx);
         LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_big<float, 4>, A, lds));
     }
     if (ctx->bicg_n[2] > 0 && big_form_fits(ctx, 2)) {
-        const int lds = (int)(3 * (size_t)ctx->bicg_maxv[2] * sizeof(float));
+        const int lds = (int)big_lds(ctx);
         LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_bicg_res<BICG_RES_K>, A, lds));
     }
     if (ctx->bicg_n[1] > 0) {
