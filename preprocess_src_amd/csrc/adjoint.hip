@@ -80,7 +80,8 @@ lbmibm_status adj_alloc(lbmibm_ctx *ctx) {
         {&ctx->adj_F, 3 * N * T}, {(void **)&ctx->adj_x, 3 * nv * 8}, {&ctx->adj_cf, 3 * nv * T},
         {(void **)&ctx->grad_H, 3 * N * 8}, {(void **)&ctx->grad_gs, (size_t)ctx->nc * 8},
         {&ctx->adj_lam, 3 * nv * T}, {(void **)&ctx->adj_dsv, nv * 8}, {(void **)&ctx->adj_corner, 24 * nv * 8},
-        {&ctx->adj_scratch, 3 * nv * T}, {(void **)&ctx->adj_cscale, (size_t)ctx->nc * 8}};
+        {&ctx->adj_scratch, 3 * nv * T}, {(void **)&ctx->adj_cscale, (size_t)ctx->nc * 8},
+        {(void **)&ctx->adj_gpart, nv * 8}};
     for (auto &e : a) {
         lbmibm_status s = dev_alloc(ctx, e.p, e.bytes);
         if (s != LBMIBM_OK) return s;

@@ -88,6 +88,7 @@ struct lbmibm_ctx {
     double *adj_corner = nullptr;     // [6Nv][4] per-(vertex, slot) position adjoints
     void *adj_scratch = nullptr;      // [3][Nv] (T) discarded solver epilogue output
     double *adj_cscale = nullptr;     // [nc] power-of-two scale of each cell's adjoint right-hand side
+    double *adj_gpart = nullptr;      // [Nv] per-vertex terms of the force-scale gradient
 
     // cells (device): positions, metric and bending reference in FP64; the
     // other membrane arrays in the arithmetic type T

@@ -761,7 +761,7 @@ template <typename W> struct StencilD {
         z1 -= 2;
     }
     // visits every stencil node inside the lattice: f(node index, weight
-    // product, its gradient with respect to the position)
+    // product, its gradient with respect to the position, stencil offsets)
     template <typename Fn> __device__ __forceinline__ void each(const IbmGeom &g, Fn f) const {
         for (int kz = 0; kz < 4; kz++) {
             int iz = z1 + kz;
@@ -778,7 +778,7 @@ template <typename W> struct StencilD {
                     const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
                     const W w = wx[kx] * wy[ky] * wz[kz];
                     const W gx = dx[kx] * wy[ky] * wz[kz], gy = wx[kx] * dy[ky] * wz[kz], gz = wx[kx] * wy[ky] * dz[kz];
-                    f(n, w, gx, gy, gz);
+                    f(n, w, gx, gy, gz, kx, ky, kz);
                 }
             }
         }
@@ -799,7 +799,7 @@ __global__ __launch_bounds__(256) void k_spread_adj(const double *__restrict__ x
     const T f0 = cf[j], f1 = cf[nv + j], f2 = cf[2 * (size_t)nv + j];
     const T w64 = (T)(1.0 / (64.0 * g.dx * g.dx * g.dx));
     T b0 = T(0), b1 = T(0), b2 = T(0), x0 = T(0), x1 = T(0), x2 = T(0);
-    st.each(g, [&](size_t n, T w, T gx, T gy, T gz) {
+    st.each(g, [&](size_t n, T w, T gx, T gy, T gz, int, int, int) {
         const T a0 = Fa[n], a1 = Fa[N + n], a2 = Fa[2 * N + n];
         b0 += w * a0; b1 += w * a1; b2 += w * a2;
         const T s = a0 * f0 + a1 * f1 + a2 * f2;
@@ -818,68 +818,138 @@ __global__ __launch_bounds__(256) void k_spread_adj(const double *__restrict__ x
 // ua += phi^3(X_s) kb_s and X_s-bar = sum (u . kb_s) grad phi^3, which feeds
 // the stage before it (ibm.cu:430-517: X2 = X + h/2 k1, X3 = X + h/2 k2,
 // X4 = X + h k3, X' = X + h (k1 + 2 k2 + 2 k3 + k4)/6, h = M dt).
+// One work group per lattice tile takes the tile's home vertices (the bins
+// of the interpolation) and adds their velocity adjoint into an LDS box of
+// the tile plus its halo (the staging box of k_interp_tiles) with LDS
+// atomics; the box is then added to ua with one global atomic per node.
+// Stencils that leave the box add to ua directly.  Neighbouring vertices
+// share nodes, so adding every contribution to ua with global atomics would
+// serialise on them.
 template <typename T>
 __global__ __launch_bounds__(256) void k_advect_adj(const double *__restrict__ xs, const T *__restrict__ U,
-                                                    double *__restrict__ xa, T *__restrict__ ua, int nv,
-                                                    IbmGeom g, double h) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nv) return;
+                                                    double *__restrict__ xa, T *__restrict__ ua,
+                                                    const int *__restrict__ tile_count,
+                                                    const int *__restrict__ tile_off,
+                                                    const int *__restrict__ home_list, int nv, IbmGeom g,
+                                                    TileGeom tg, double h) {
+    extern __shared__ double dyn[];
+    T *Ub = (T *)dyn;  // [3][RZ][RY][RX] velocity adjoint of the box
+    const int tile = blockIdx.x;
+    const int cnt = tile_count[tile], off = tile_off[tile];
+    if (cnt == 0) return;  // whole work group leaves together
+    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+    const int bx = tx * TILE_X - RLO, by = ty * TILE_Y - RLO, bz = tz * TILE_Z - RLO;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
-    const double X[3] = {xs[j], xs[nv + j], xs[2 * (size_t)nv + j]};
-    auto interp = [&](const double p[3], double k[3]) {
-        const StencilD<T> st(g, p[0], p[1], p[2]);
-        T u0 = T(0), u1 = T(0), u2 = T(0);
-        st.each(g, [&](size_t n, T w, T, T, T) {
-            u0 += w * U[n]; u1 += w * U[N + n]; u2 += w * U[2 * N + n];
-        });
-        k[0] = (double)(u0 * T(1.0 / 64.0));
-        k[1] = (double)(u1 * T(1.0 / 64.0));
-        k[2] = (double)(u2 * T(1.0 / 64.0));
-    };
-    double P[4][3], k[3];
+    for (int k = threadIdx.x; k < 3 * RN; k += blockDim.x) Ub[k] = T(0);
+    __syncthreads();
+    const T q64 = T(1.0 / 64.0);
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const int j = home_list[off + i];
+        const double X[3] = {xs[j], xs[nv + j], xs[2 * (size_t)nv + j]};
+        auto interp = [&](const double p[3], double k[3]) {
+            const StencilD<T> st(g, p[0], p[1], p[2]);
+            T u0 = T(0), u1 = T(0), u2 = T(0);
+            st.each(g, [&](size_t n, T w, T, T, T, int, int, int) {
+                u0 += w * U[n]; u1 += w * U[N + n]; u2 += w * U[2 * N + n];
+            });
+            k[0] = (double)(u0 * q64);
+            k[1] = (double)(u1 * q64);
+            k[2] = (double)(u2 * q64);
+        };
+        double P[4][3], k[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) P[0][d] = X[d];
-    interp(P[0], k);
+        for (int d = 0; d < 3; d++) P[0][d] = X[d];
+        interp(P[0], k);
 #pragma unroll
-    for (int d = 0; d < 3; d++) P[1][d] = X[d] + k[d] * h / 2.0;
-    interp(P[1], k);
+        for (int d = 0; d < 3; d++) P[1][d] = X[d] + k[d] * h / 2.0;
+        interp(P[1], k);
 #pragma unroll
-    for (int d = 0; d < 3; d++) P[2][d] = X[d] + k[d] * h / 2.0;
-    interp(P[2], k);
+        for (int d = 0; d < 3; d++) P[2][d] = X[d] + k[d] * h / 2.0;
+        interp(P[2], k);
 #pragma unroll
-    for (int d = 0; d < 3; d++) P[3][d] = X[d] + k[d] * h;
-    const double xb[3] = {xa[j], xa[nv + j], xa[2 * (size_t)nv + j]};
-    double acc[3] = {xb[0], xb[1], xb[2]};
-    double kb[4][3];
-    const double cw[4] = {h / 6.0, h / 3.0, h / 3.0, h / 6.0};
+        for (int d = 0; d < 3; d++) P[3][d] = X[d] + k[d] * h;
+        const double xb[3] = {xa[j], xa[nv + j], xa[2 * (size_t)nv + j]};
+        double acc[3] = {xb[0], xb[1], xb[2]};
+        double kb[4][3];
+        const double cw[4] = {h / 6.0, h / 3.0, h / 3.0, h / 6.0};
 #pragma unroll
-    for (int s = 0; s < 4; s++)
+        for (int s = 0; s < 4; s++)
 #pragma unroll
-        for (int d = 0; d < 3; d++) kb[s][d] = cw[s] * xb[d];
-    const double back[4] = {0.0, h / 2.0, h / 2.0, h};  // X_s = X + back[s] k_{s-1}
-    for (int s = 3; s >= 0; s--) {
-        const StencilD<T> st(g, P[s][0], P[s][1], P[s][2]);
-        const T k0 = (T)kb[s][0], k1 = (T)kb[s][1], k2 = (T)kb[s][2];
-        const T q64 = T(1.0 / 64.0);
-        T g0 = T(0), g1 = T(0), g2 = T(0);
-        st.each(g, [&](size_t n, T w, T gx, T gy, T gz) {
-            const T wq = w * q64;
-            atomicAdd(&ua[n], wq * k0);
-            atomicAdd(&ua[N + n], wq * k1);
-            atomicAdd(&ua[2 * N + n], wq * k2);
-            const T sdot = U[n] * k0 + U[N + n] * k1 + U[2 * N + n] * k2;
-            g0 += sdot * gx; g1 += sdot * gy; g2 += sdot * gz;
-        });
-        const double xs_[3] = {(double)(g0 * q64), (double)(g1 * q64), (double)(g2 * q64)};
+            for (int d = 0; d < 3; d++) kb[s][d] = cw[s] * xb[d];
+        const double back[4] = {0.0, h / 2.0, h / 2.0, h};  // X_s = X + back[s] k_{s-1}
+        for (int s = 3; s >= 0; s--) {
+            const StencilD<T> st(g, P[s][0], P[s][1], P[s][2]);
+            const T k0 = (T)kb[s][0], k1 = (T)kb[s][1], k2 = (T)kb[s][2];
+            // box coordinates of the first stencil node (as interp_tile)
+            int rx = st.x1 - bx, ry = st.y1 - by, rz = st.z1 - bz;
+            if (rx < 0) rx += g.nx; else if (rx >= g.nx) rx -= g.nx;
+            if (rz < 0) rz += g.nz; else if (rz >= g.nz) rz -= g.nz;
+            if (!g.walls) { if (ry < 0) ry += g.ny; else if (ry >= g.ny) ry -= g.ny; }
+            const bool inbox = rx >= 0 && rx + 3 < RX && ry >= 0 && ry + 3 < RY && rz >= 0 && rz + 3 < RZ;
+            T g0 = T(0), g1 = T(0), g2 = T(0);
+            // the stencil loop of StencilD::each, written out so that the LDS
+            // atomics address the box directly
+            for (int kz = 0; kz < 4; kz++) {
+                int iz = st.z1 + kz;
+                if (!wrap_index(iz, g.nz)) continue;
+                for (int ky = 0; ky < 4; ky++) {
+                    int iy = st.y1 + ky;
+                    if (g.walls) {
+                        if (iy < 0 || iy >= g.ny) continue;
+                    } else if (!wrap_index(iy, g.ny)) continue;
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            acc[d] += xs_[d];
-            if (s > 0) kb[s - 1][d] += back[s] * xs_[d];
+                    for (int kx = 0; kx < 4; kx++) {
+                        int ix = st.x1 + kx;
+                        if (!wrap_index(ix, g.nx)) continue;
+                        const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
+                        const T w = st.wx[kx] * st.wy[ky] * st.wz[kz];
+                        const T wq = w * q64;
+                        if (inbox) {  // node (x1+kx, y1+ky, z1+kz) is box node (rx+kx, ry+ky, rz+kz)
+                            const int bi = (rx + kx) + RX * ((ry + ky) + RY * (rz + kz));
+                            atomicAdd((T *)dyn + bi, wq * k0);
+                            atomicAdd((T *)dyn + RN + bi, wq * k1);
+                            atomicAdd((T *)dyn + 2 * RN + bi, wq * k2);
+                        } else {
+                            atomicAdd(&ua[n], wq * k0);
+                            atomicAdd(&ua[N + n], wq * k1);
+                            atomicAdd(&ua[2 * N + n], wq * k2);
+                        }
+                        const T sdot = U[n] * k0 + U[N + n] * k1 + U[2 * N + n] * k2;
+                        g0 += sdot * st.dx[kx] * st.wy[ky] * st.wz[kz];
+                        g1 += sdot * st.wx[kx] * st.dy[ky] * st.wz[kz];
+                        g2 += sdot * st.wx[kx] * st.wy[ky] * st.dz[kz];
+                    }
+                }
+            }
+            const double xs_[3] = {(double)(g0 * q64), (double)(g1 * q64), (double)(g2 * q64)};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                acc[d] += xs_[d];
+                if (s > 0) kb[s - 1][d] += back[s] * xs_[d];
+            }
         }
+        xa[j] = acc[0];
+        xa[nv + j] = acc[1];
+        xa[2 * (size_t)nv + j] = acc[2];
     }
-    xa[j] = acc[0];
-    xa[nv + j] = acc[1];
-    xa[2 * (size_t)nv + j] = acc[2];
+    __syncthreads();
+    // the box into ua (the box may wrap onto itself on small lattices: atomics)
+    for (int k = threadIdx.x; k < RN; k += blockDim.x) {
+        const T a0 = Ub[k], a1 = Ub[RN + k], a2 = Ub[2 * RN + k];
+        if (a0 == T(0) && a1 == T(0) && a2 == T(0)) continue;
+        const int rx = k % RX, ry = (k / RX) % RY, rz = k / (RX * RY);
+        const int ix = pmod(bx + rx, g.nx), iz = pmod(bz + rz, g.nz);
+        int iy = by + ry;
+        if (g.walls) {
+            if (iy < 0 || iy >= g.ny) continue;
+        } else {
+            iy = pmod(iy, g.ny);
+        }
+        const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
+        atomicAdd(&ua[n], a0);
+        atomicAdd(&ua[N + n], a1);
+        atomicAdd(&ua[2 * N + n], a2);
+    }
 }
 
 IbmGeom geom_of(const lbmibm_ctx *ctx) {
@@ -961,9 +1031,15 @@ template <typename T> lbmibm_status spread_adj_impl(lbmibm_ctx *ctx) {
 
 template <typename T> lbmibm_status advect_adj_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return LBMIBM_OK;
-    hipLaunchKernelGGL(k_advect_adj<T>, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream,
-                       (const double *)ctx->xs, (const T *)ctx->U, ctx->adj_x, (T *)ctx->adj_u, ctx->nv,
-                       geom_of(ctx), ctx->p.m * ctx->p.dt);
+    if (!ctx->bins_valid) {
+        lbmibm_status s = bin_impl<T>(ctx);
+        if (s != LBMIBM_OK) return s;
+    }
+    TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
+    const size_t lds = 3 * (size_t)RN * sizeof(T);
+    hipLaunchKernelGGL(k_advect_adj<T>, dim3(ctx->ntiles), dim3(256), lds, ctx->stream, (const double *)ctx->xs,
+                       (const T *)ctx->U, ctx->adj_x, (T *)ctx->adj_u, ctx->tile_count, ctx->tile_off,
+                       ctx->home_list, ctx->nv, geom_of(ctx), tg, ctx->p.m * ctx->p.dt);
     LBMIBM_LAUNCHED(ctx, "k_advect_adj");
     return LBMIBM_OK;
 }
@@ -988,6 +1064,10 @@ lbmibm_status ibm_setup(lbmibm_ctx *ctx) {
     LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_interp_tiles<double>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds64));
     LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_interp_tiles<float>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds32));
+    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_advect_adj<double>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds64));
+    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_advect_adj<float>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds32));
     return LBMIBM_OK;
 }

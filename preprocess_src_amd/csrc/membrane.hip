@@ -1352,10 +1352,9 @@ __device__ __forceinline__ void elem_rhs(const R v1[3], const R v2[3], const dou
 template <typename T>
 __global__ __launch_bounds__(256) void k_adj_load(const int *__restrict__ nln,
                                                   const typename Corner<T>::type *__restrict__ corner,
-                                                  const int *__restrict__ vcell, const T *__restrict__ dsv,
-                                                  const T *__restrict__ q, const T *__restrict__ cfa,
-                                                  T *__restrict__ bv, double *__restrict__ dsva,
-                                                  double *__restrict__ gsa, int nv) {
+                                                  const T *__restrict__ dsv, const T *__restrict__ q,
+                                                  const T *__restrict__ cfa, T *__restrict__ bv,
+                                                  double *__restrict__ dsva, double *__restrict__ gpart, int nv) {
     const int jj = blockIdx.x * blockDim.x + threadIdx.x;
     if (jj >= nv) return;
     const int n = nln[jj];
@@ -1374,7 +1373,7 @@ __global__ __launch_bounds__(256) void k_adj_load(const int *__restrict__ nln,
         sb -= a * (double)q[3 * (size_t)jj + k];
     }
     dsva[jj] = sb;
-    if (gsa) atomicAdd(&gsa[vcell[jj]], sb * ds);
+    gpart[jj] = sb * ds;  // this vertex's term of the cell's force-scale gradient
 }
 
 // Per cell (one work group): scale the adjoint right-hand side by a power of
@@ -1383,22 +1382,36 @@ __global__ __launch_bounds__(256) void k_adj_load(const int *__restrict__ nln,
 // many orders of magnitude below those near it; in FP32 they would reach the
 // subnormal range, where the solver cannot reach a relative residual.  Power
 // of two scaling is exact, and the solution is scaled back after the solve.
+// The same work group also adds the cell's force-scale gradient (its
+// vertices' terms gpart, in a fixed order) to gsa, without atomics.
 template <typename T>
 __global__ __launch_bounds__(256) void k_rhs_scale(const int *__restrict__ voff, T *__restrict__ bv,
-                                                   double *__restrict__ cscale) {
+                                                   double *__restrict__ cscale, const double *__restrict__ gpart,
+                                                   double *__restrict__ gsa) {
     const int c = blockIdx.x;
-    const size_t b0 = 3 * (size_t)voff[c], b1 = 3 * (size_t)voff[c + 1];
+    const size_t v0 = voff[c], v1 = voff[c + 1];
+    const size_t b0 = 3 * v0, b1 = 3 * v1;
     __shared__ float wmax[4];
+    __shared__ double wsum[4];
     float m = 0.0f;
+    double gs = 0.0;
     for (size_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) m = fmaxf(m, fabsf((float)bv[i]));
+    for (size_t i = v0 + threadIdx.x; i < v1; i += blockDim.x) gs += gpart[i];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    gs = wave_sum(gs);
+    if ((threadIdx.x & 63) == 0) {
+        wmax[threadIdx.x >> 6] = m;
+        wsum[threadIdx.x >> 6] = gs;
+    }
     __syncthreads();
     m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
     const double s = m > 0.0f ? ldexp(1.0, -ilogb((double)m)) : 1.0;
     for (size_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) bv[i] = (T)((double)bv[i] * s);
-    if (threadIdx.x == 0) cscale[c] = s;
+    if (threadIdx.x == 0) {
+        cscale[c] = s;
+        if (gsa) gsa[c] += ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+    }
 }
 
 // Per vertex: undo the right-hand-side scale on the solution.
@@ -1509,11 +1522,11 @@ template <typename T> lbmibm_status forces_adj_impl(lbmibm_ctx *ctx, bool have_q
     lbmibm_status s = have_q ? assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx)) : forces_impl<T>(ctx);
     if (s != LBMIBM_OK) return s;
     hipLaunchKernelGGL(k_adj_load<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln,
-                       (const typename Corner<T>::type *)ctx->corner, ctx->vcell, (const T *)ctx->dsv,
-                       (const T *)ctx->qv, (const T *)ctx->adj_cf, (T *)ctx->bv, ctx->adj_dsv, ctx->grad_gs, nv);
+                       (const typename Corner<T>::type *)ctx->corner, (const T *)ctx->dsv, (const T *)ctx->qv,
+                       (const T *)ctx->adj_cf, (T *)ctx->bv, ctx->adj_dsv, ctx->adj_gpart, nv);
     LBMIBM_LAUNCHED(ctx, "k_adj_load");
     hipLaunchKernelGGL(k_rhs_scale<T>, dim3(ctx->nc), dim3(256), 0, ctx->stream, ctx->voff, (T *)ctx->bv,
-                       ctx->adj_cscale);
+                       ctx->adj_cscale, ctx->adj_gpart, ctx->grad_gs);
     LBMIBM_LAUNCHED(ctx, "k_rhs_scale");
     SolveOut<T> out = solve_out<T>(ctx);
     out.q = (T *)ctx->adj_lam;
