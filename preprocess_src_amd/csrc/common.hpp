@@ -52,9 +52,9 @@ struct lbmibm_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;    // membrane solve
     hipEvent_t ev_vfork = nullptr, ev_vjoin = nullptr;  // volume correction classes
     // third stream (default priority): inside lbmibm_run, the volume
-    // correction of one step overlaps the membrane force of the next, which
-    // reads fem->xd and not the corrected cel->x (when BR = 0); it is joined
-    // before the next spreading or interpolation reads the positions
+    // correction of one step and the binning of its positions overlap the
+    // membrane force of the next, which reads fem->xd and not the corrected
+    // cel->x (when BR = 0); joined before the next spreading or interpolation
     hipStream_t vstream = nullptr;
     hipEvent_t ev_pfork = nullptr, ev_pjoin = nullptr;
     bool vol_pending = false;
@@ -137,7 +137,7 @@ struct lbmibm_ctx {
     int *tile_count = nullptr, *tile_off = nullptr, *tile_cur = nullptr;  // [ntiles]
     int *tile_cand = nullptr;                          // [ntiles] spreading candidates
     int *order_spread = nullptr, *order_interp = nullptr;  // [ntiles] heavy-first tile orders
-    float *blk_fmax = nullptr;               // [nblk_bin] max |f| per binning block
+    int *order_calls = nullptr;              // device counter of binnings (tile-order refresh)
     unsigned *absmax = nullptr;
     bool bins_valid = false;                 // bins match the current positions
 
@@ -161,7 +161,7 @@ lbmibm_status lattice_fill(lbmibm_ctx *ctx, void *ptr, size_t count, double valu
 lbmibm_status lattice_adjoint(lbmibm_ctx *ctx, const void *ga_out, void *ga_in);  // lattice.hip
 lbmibm_status ibm_spread(lbmibm_ctx *ctx);                   // ibm.hip
 lbmibm_status ibm_interp(lbmibm_ctx *ctx);                   // ibm.hip
-lbmibm_status ibm_bin(lbmibm_ctx *ctx);                      // ibm.hip
+lbmibm_status ibm_bin(lbmibm_ctx *ctx, hipStream_t stream);  // ibm.hip (positions only)
 lbmibm_status ibm_setup(lbmibm_ctx *ctx);                    // ibm.hip
 lbmibm_status ibm_spread_adjoint(lbmibm_ctx *ctx);           // ibm.hip
 lbmibm_status ibm_interp_adjoint(lbmibm_ctx *ctx);           // ibm.hip

@@ -196,14 +196,11 @@ __device__ __forceinline__ int wave_aggregated_add(int *counters, int key, bool 
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs, const T *__restrict__ cf, int nv,
-                                                   IbmGeom g, TileGeom tg, int *__restrict__ home,
-                                                   int *__restrict__ tile_count, float *__restrict__ blk_fmax) {
-    __shared__ float shm[4];
+__global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs, int nv, IbmGeom g, TileGeom tg,
+                                                   int *__restrict__ home, int *__restrict__ tile_count) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = v < nv;
     int t = 0;
-    float fm = 0.0f;
     if (on) {
         int x1, y1, z1;
         T wx[4], wy[4], wz[4];
@@ -220,15 +217,8 @@ __global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs
         yh = min(max(yh, 0), g.ny - 1);
         t = xw / TILE_X + tg.ntx * (yh / TILE_Y + tg.nty * (zw / TILE_Z));
         home[v] = t;
-        fm = fmaxf(fabsf((float)cf[v]), fmaxf(fabsf((float)cf[nv + v]), fabsf((float)cf[2 * (size_t)nv + v])));
-        fm *= 1.0001f;  // margin for the float rounding of |f|
     }
     (void)wave_aggregated_add(tile_count, t, on);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) fm = fmaxf(fm, __shfl_xor(fm, o));
-    if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = fm;
-    __syncthreads();
-    if (threadIdx.x == 0) blk_fmax[blockIdx.x] = fmaxf(fmaxf(shm[0], shm[1]), fmaxf(shm[2], shm[3]));
 }
 
 // Heavy-first order of the tiles: a counting sort on a 256-level key of the
@@ -264,23 +254,16 @@ __device__ void heavy_first_order(const int *load, int n, int *hist, int *order)
 
 __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile_count, int *__restrict__ tile_off,
                                                     int *__restrict__ tile_cur, int ntiles,
-                                                    const float *__restrict__ blk_fmax, int nblk,
                                                     unsigned *__restrict__ absmax) {
-    // exclusive scan over tiles in one work group, 1024 entries per round,
-    // and the maximum of the per-block force maxima
+    // exclusive scan over tiles in one work group, 1024 entries per round;
+    // also clears the force maximum that k_bin_forces raises
     __shared__ int sh[1024];
-    __shared__ float shm[1024];
     __shared__ int carry;
-    float fm = 0.0f;
-    for (int i = threadIdx.x; i < nblk; i += 1024) fm = fmaxf(fm, blk_fmax[i]);
-    shm[threadIdx.x] = fm;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-        if (threadIdx.x < o) shm[threadIdx.x] = fmaxf(shm[threadIdx.x], shm[threadIdx.x + o]);
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        carry = 0;
+        *absmax = 0u;
     }
-    if (threadIdx.x == 0) *absmax = __float_as_uint(shm[0]);
+    __syncthreads();
     for (int base = 0; base < ntiles; base += 1024) {
         const int i = base + threadIdx.x;
         const int v = i < ntiles ? tile_count[i] : 0;
@@ -305,17 +288,20 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile
 // The heavy-first tile orders of the spreading (by candidate count: own bin
 // and the bins below on each axis) and of the interpolation (by own count).
 // They only steer the dispatch order, and tile loads change slowly with the
-// cells, so they are refreshed every TILE_ORDER_PERIOD steps (the step counter
-// is read on the device, so captured graphs refresh them too); any
+// cells, so they are refreshed on every TILE_ORDER_PERIOD-th binning (a call
+// counter on the device, so captured graphs refresh them too); any
 // permutation is a valid order, the initial one is the identity.
 constexpr int TILE_ORDER_PERIOD = 16;
 
 __global__ __launch_bounds__(1024) void k_tile_order(const int *__restrict__ tile_count, int ntiles, IbmGeom g,
                                                      TileGeom tg, int *__restrict__ tile_cand,
                                                      int *__restrict__ order_spread, int *__restrict__ order_interp,
-                                                     const DevStep *__restrict__ st) {
+                                                     int *__restrict__ calls) {
     __shared__ int sh[256];
-    if ((st->iter - 1) % TILE_ORDER_PERIOD != 0) return;
+    __shared__ int call;
+    if (threadIdx.x == 0) call = (*calls)++;
+    __syncthreads();
+    if (call % TILE_ORDER_PERIOD != 0) return;
     for (int t = threadIdx.x; t < ntiles; t += 1024) {
         const int tx = t % tg.ntx, ty = (t / tg.ntx) % tg.nty, tz = t / (tg.ntx * tg.nty);
         int sxs[4], sys[4], szs[4];
@@ -341,11 +327,10 @@ template <typename T> struct SpreadRec {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bin_scatter(const double *__restrict__ xs, const T *__restrict__ cf,
-                                                     const int *__restrict__ home, int nv, IbmGeom g,
-                                                     const int *__restrict__ tile_off, int *__restrict__ tile_cur,
-                                                     int *__restrict__ home_list, short4 *__restrict__ node0,
-                                                     SpreadRec<T> *__restrict__ rec) {
+__global__ __launch_bounds__(256) void k_bin_scatter(const double *__restrict__ xs, const int *__restrict__ home,
+                                                     int nv, IbmGeom g, const int *__restrict__ tile_off,
+                                                     int *__restrict__ tile_cur, int *__restrict__ home_list,
+                                                     short4 *__restrict__ node0, SpreadRec<T> *__restrict__ rec) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = v < nv;
     const int t = on ? home[v] : 0;
@@ -354,10 +339,10 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const double *__restrict__ 
     const int pos = tile_off[t] + r;
     home_list[pos] = v;
     int x1, y1, z1;
-    SpreadRec<T> sr;
-    stencil_first<T>(xs[v] + (double)(g.nxp - 1) * g.dx / 2.0, g.dx, x1, sr.t[0]);
-    stencil_first<T>(xs[nv + v] + (double)(g.nyp - 1) * g.dx / 2.0, g.dx, y1, sr.t[1]);
-    stencil_first<T>(xs[2 * (size_t)nv + v] + (double)(g.nzp - 1) * g.dx / 2.0, g.dx, z1, sr.t[2]);
+    T t3[3];
+    stencil_first<T>(xs[v] + (double)(g.nxp - 1) * g.dx / 2.0, g.dx, x1, t3[0]);
+    stencil_first<T>(xs[nv + v] + (double)(g.nyp - 1) * g.dx / 2.0, g.dx, y1, t3[1]);
+    stencil_first<T>(xs[2 * (size_t)nv + v] + (double)(g.nzp - 1) * g.dx / 2.0, g.dx, z1, t3[2]);
     x1 -= 2; y1 -= 2; z1 -= 2;  // unpadded
     int xw = x1, zw = z1;
     wrap_index(xw, g.nx);
@@ -365,8 +350,30 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const double *__restrict__ 
     xw = min(max(xw, -30000), 30000);
     zw = min(max(zw, -30000), 30000);
     node0[pos] = make_short4((short)xw, (short)min(max(y1 + 3, -30000), 30000), (short)zw, 0);
-    sr.f[0] = cf[v]; sr.f[1] = cf[nv + v]; sr.f[2] = cf[2 * (size_t)nv + v];
-    rec[pos] = sr;
+    rec[pos].t[0] = t3[0]; rec[pos].t[1] = t3[1]; rec[pos].t[2] = t3[2];
+}
+
+// The force part of the spreading records, in bin order, and max |f| (the
+// binning itself only depends on the positions and may run earlier).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_forces(const int *__restrict__ home_list, const T *__restrict__ cf, int nv,
+                                                    SpreadRec<T> *__restrict__ rec, unsigned *__restrict__ absmax) {
+    __shared__ float shm[4];
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    float fm = 0.0f;
+    if (pos < nv) {
+        const int v = home_list[pos];
+        const T f0 = cf[v], f1 = cf[nv + v], f2 = cf[2 * (size_t)nv + v];
+        rec[pos].f[0] = f0; rec[pos].f[1] = f1; rec[pos].f[2] = f2;
+        fm = fmaxf(fabsf((float)f0), fmaxf(fabsf((float)f1), fabsf((float)f2)));
+        fm *= 1.0001f;  // margin for the float rounding of |f|
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) fm = fmaxf(fm, __shfl_xor(fm, o));
+    if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = fm;
+    __syncthreads();
+    // non-negative floats order as their bit patterns: an integer maximum
+    if (threadIdx.x == 0) atomicMax(absmax, __float_as_uint(fmaxf(fmaxf(shm[0], shm[1]), fmaxf(shm[2], shm[3]))));
 }
 
 __global__ void k_zero_int(int *__restrict__ p, int n) {
@@ -763,9 +770,11 @@ template <typename W> struct StencilD {
     // visits every stencil node inside the lattice: f(node index, weight
     // product, its gradient with respect to the position, stencil offsets)
     template <typename Fn> __device__ __forceinline__ void each(const IbmGeom &g, Fn f) const {
+#pragma unroll
         for (int kz = 0; kz < 4; kz++) {
             int iz = z1 + kz;
             if (!wrap_index(iz, g.nz)) continue;
+#pragma unroll
             for (int ky = 0; ky < 4; ky++) {
                 int iy = y1 + ky;
                 if (g.walls) {
@@ -877,7 +886,8 @@ __global__ __launch_bounds__(256) void k_advect_adj(const double *__restrict__ x
 #pragma unroll
             for (int d = 0; d < 3; d++) kb[s][d] = cw[s] * xb[d];
         const double back[4] = {0.0, h / 2.0, h / 2.0, h};  // X_s = X + back[s] k_{s-1}
-        for (int s = 3; s >= 0; s--) {
+#pragma unroll
+        for (int s = 3; s >= 0; s--) {  // unrolled: P and kb stay in registers
             const StencilD<T> st(g, P[s][0], P[s][1], P[s][2]);
             const T k0 = (T)kb[s][0], k1 = (T)kb[s][1], k2 = (T)kb[s][2];
             // box coordinates of the first stencil node (as interp_tile)
@@ -888,10 +898,13 @@ __global__ __launch_bounds__(256) void k_advect_adj(const double *__restrict__ x
             const bool inbox = rx >= 0 && rx + 3 < RX && ry >= 0 && ry + 3 < RY && rz >= 0 && rz + 3 < RZ;
             T g0 = T(0), g1 = T(0), g2 = T(0);
             // the stencil loop of StencilD::each, written out so that the LDS
-            // atomics address the box directly
+            // atomics address the box directly; unrolled so that the weight
+            // arrays are indexed by constants (registers, not scratch)
+#pragma unroll
             for (int kz = 0; kz < 4; kz++) {
                 int iz = st.z1 + kz;
                 if (!wrap_index(iz, g.nz)) continue;
+#pragma unroll
                 for (int ky = 0; ky < 4; ky++) {
                     int iy = st.y1 + ky;
                     if (g.walls) {
@@ -961,25 +974,27 @@ IbmGeom geom_of(const lbmibm_ctx *ctx) {
     return g;
 }
 
-template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
+// Binning of the vertices by the tile of their first stencil node, from the
+// positions only, on the given stream.
+template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx, hipStream_t stream) {
     const IbmGeom g = geom_of(ctx);
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const int nv = ctx->nv;
-    hipLaunchKernelGGL(k_zero_int, dim3((ctx->ntiles + 255) / 256), dim3(256), 0, ctx->stream, ctx->tile_count,
+    hipLaunchKernelGGL(k_zero_int, dim3((ctx->ntiles + 255) / 256), dim3(256), 0, stream, ctx->tile_count,
                        ctx->ntiles);
     LBMIBM_LAUNCHED(ctx, "k_zero_int");
-    hipLaunchKernelGGL(k_bin_count<T>, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
-                       (const T *)ctx->cf, nv, g, tg, ctx->home, ctx->tile_count, ctx->blk_fmax);
+    hipLaunchKernelGGL(k_bin_count<T>, dim3(ctx->nblk_bin), dim3(256), 0, stream, (const double *)ctx->xs, nv, g, tg,
+                       ctx->home, ctx->tile_count);
     LBMIBM_LAUNCHED(ctx, "k_bin_count");
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->tile_count, ctx->tile_off,
-                       ctx->tile_cur, ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->tile_off, ctx->tile_cur,
+                       ctx->ntiles, ctx->absmax);
     LBMIBM_LAUNCHED(ctx, "k_tile_scan");
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, ctx->stream, ctx->tile_count, ctx->ntiles, g, tg,
-                       ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->step);
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->ntiles, g, tg,
+                       ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->order_calls);
     LBMIBM_LAUNCHED(ctx, "k_tile_order");
-    hipLaunchKernelGGL(k_bin_scatter<T>, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, (const double *)ctx->xs,
-                       (const T *)ctx->cf, ctx->home, nv, g, ctx->tile_off, ctx->tile_cur, ctx->home_list,
-                       ctx->node0, (SpreadRec<T> *)ctx->srec);
+    hipLaunchKernelGGL(k_bin_scatter<T>, dim3(ctx->nblk_bin), dim3(256), 0, stream, (const double *)ctx->xs,
+                       ctx->home, nv, g, ctx->tile_off, ctx->tile_cur, ctx->home_list, ctx->node0,
+                       (SpreadRec<T> *)ctx->srec);
     LBMIBM_LAUNCHED(ctx, "k_bin_scatter");
     ctx->bins_valid = true;
     return LBMIBM_OK;
@@ -987,9 +1002,12 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
 
 template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return lattice_fill(ctx, ctx->F, 3 * (size_t)ctx->N, 0.0, false);
-    // the nodal forces change between calls: the binning pass also takes max |f|
-    lbmibm_status s = bin_impl<T>(ctx);
+    lbmibm_status s = ctx->bins_valid ? LBMIBM_OK : bin_impl<T>(ctx, ctx->stream);
     if (s != LBMIBM_OK) return s;
+    // the nodal forces change with every call: their part of the records and max |f|
+    hipLaunchKernelGGL(k_bin_forces<T>, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, ctx->home_list,
+                       (const T *)ctx->cf, ctx->nv, (SpreadRec<T> *)ctx->srec, ctx->absmax);
+    LBMIBM_LAUNCHED(ctx, "k_bin_forces");
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     hipLaunchKernelGGL(k_spread_tiles<T>, dim3(ctx->ntiles), dim3(256), 0, ctx->stream,
                        (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off, (T *)ctx->F,
@@ -1001,7 +1019,7 @@ template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
 template <typename T> lbmibm_status interp_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return LBMIBM_OK;
     if (!ctx->bins_valid) {
-        lbmibm_status s = bin_impl<T>(ctx);
+        lbmibm_status s = bin_impl<T>(ctx, ctx->stream);
         if (s != LBMIBM_OK) return s;
     }
     AdvectArgs a;
@@ -1032,7 +1050,7 @@ template <typename T> lbmibm_status spread_adj_impl(lbmibm_ctx *ctx) {
 template <typename T> lbmibm_status advect_adj_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return LBMIBM_OK;
     if (!ctx->bins_valid) {
-        lbmibm_status s = bin_impl<T>(ctx);
+        lbmibm_status s = bin_impl<T>(ctx, ctx->stream);
         if (s != LBMIBM_OK) return s;
     }
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
@@ -1072,9 +1090,9 @@ lbmibm_status ibm_setup(lbmibm_ctx *ctx) {
     return LBMIBM_OK;
 }
 
-lbmibm_status ibm_bin(lbmibm_ctx *ctx) {
+lbmibm_status ibm_bin(lbmibm_ctx *ctx, hipStream_t stream) {
     if (ctx->nv == 0) return LBMIBM_OK;
-    return ctx->prec == LBMIBM_F64 ? bin_impl<double>(ctx) : bin_impl<float>(ctx);
+    return ctx->prec == LBMIBM_F64 ? bin_impl<double>(ctx, stream) : bin_impl<float>(ctx, stream);
 }
 
 lbmibm_status ibm_spread(lbmibm_ctx *ctx) {
