@@ -239,13 +239,30 @@ def cpu_baseline(shape, cells, params, seconds):
                        f"({t_run:.1f} s timed, {setup:.1f} s set-up excluded)")
 
 
+def copy_calibration(nbytes=256 << 20, reps=3):
+    """A device-to-device copy of known size: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE per dispatch against nbytes give the factors that
+    turn those counters into HBM bytes."""
+    import torch
+    a = torch.ones(nbytes // 4, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "dispatches": reps}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64", "f16"])
-    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv", "adjoint"])
+    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv", "adjoint", "copy"],
+                    help="copy: a plain 256 MiB device copy, the calibration run of the FETCH_SIZE / "
+                         "WRITE_SIZE counters")
     ap.add_argument("--stage-steps", type=int, default=6)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -256,6 +273,10 @@ def main():
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path is single-GPU, "
               "multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "copy":
+        copy_calibration()
+        return
 
     from preprocess_src_amd.simulation import Simulation
 

@@ -81,7 +81,8 @@ lbmibm_status adj_alloc(lbmibm_ctx *ctx) {
         {(void **)&ctx->grad_H, 3 * N * 8}, {(void **)&ctx->grad_gs, (size_t)ctx->nc * 8},
         {&ctx->adj_lam, 3 * nv * T}, {(void **)&ctx->adj_dsv, nv * 8}, {(void **)&ctx->adj_corner, 24 * nv * 8},
         {&ctx->adj_scratch, 3 * nv * T}, {(void **)&ctx->adj_cscale, (size_t)ctx->nc * 8},
-        {(void **)&ctx->adj_gpart, nv * 8}};
+        {(void **)&ctx->adj_gpart, nv * 8}, {(void **)&ctx->adj_xmax, 256},
+        {&ctx->adj_rec, 4 * nv * 48}};
     for (auto &e : a) {
         lbmibm_status s = dev_alloc(ctx, e.p, e.bytes);
         if (s != LBMIBM_OK) return s;

@@ -179,12 +179,12 @@ lbmibm_status enqueue_step(lbmibm_ctx *ctx, int iter, bool device_advance, bool 
         s = join_volume(ctx);
         if (s == LBMIBM_OK) s = ibm_interp(ctx);
         if (s == LBMIBM_OK && overlap_volume && ctx->p.br == 0.0) {
-            // the volume correction and the binning of the final positions for
-            // the next spreading, overlapping the next membrane force
+            // the volume correction overlapping the next membrane force (the
+            // binning of the corrected positions stays with the spreading: on
+            // this stream it took CUs from the large-cell solve)
             LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_pfork, ctx->stream));
             LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->vstream, ctx->ev_pfork, 0));
             if (ctx->p.volume_correction) s = volume_run_on(ctx, 1, ctx->vstream);
-            if (s == LBMIBM_OK) s = ibm_bin(ctx, ctx->vstream);
             ctx->vol_pending = true;
         } else if (s == LBMIBM_OK && ctx->p.volume_correction) {
             s = volume_run(ctx, 1);
@@ -204,12 +204,12 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->qi, ctx->coef, ctx->bv, ctx->rv, ctx->rh, ctx->pv, ctx->vv, ctx->tv, ctx->gscale,
                     ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->bicg_iters, ctx->vol_passes,
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
-                    ctx->tile_off, ctx->tile_cur, ctx->order_calls, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
+                    ctx->tile_off, ctx->tile_cur, ctx->order_calls, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
                     ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv,
                     ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair,
                     ctx->srec, ctx->onchip_cells, ctx->Hb, ctx->adj_f[0], ctx->adj_f[1], ctx->adj_u, ctx->adj_F,
                     ctx->adj_x, ctx->adj_cf, ctx->grad_H, ctx->grad_gs, ctx->adj_lam, ctx->adj_dsv,
-                    ctx->adj_corner, ctx->adj_scratch, ctx->adj_cscale, ctx->adj_gpart};
+                    ctx->adj_corner, ctx->adj_scratch, ctx->adj_cscale, ctx->adj_gpart, ctx->adj_xmax, ctx->adj_rec};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -498,6 +498,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
             const int zero = 0;
             CK(up_i(&ctx->order_calls, &zero, 1));
         }
+        CK(dev_alloc(ctx, (void **)&ctx->blk_fmax, (size_t)ctx->nblk_bin * sizeof(float)));
         CK(dev_alloc(ctx, (void **)&ctx->absmax, sizeof(unsigned)));
         CK(ibm_setup(ctx));
         CK(dev_alloc(ctx, (void **)&ctx->normal, 3 * (size_t)nt * sizeof(double)));

@@ -89,6 +89,8 @@ struct lbmibm_ctx {
     void *adj_scratch = nullptr;      // [3][Nv] (T) discarded solver epilogue output
     double *adj_cscale = nullptr;     // [nc] power-of-two scale of each cell's adjoint right-hand side
     double *adj_gpart = nullptr;      // [Nv] per-vertex terms of the force-scale gradient
+    unsigned *adj_xmax = nullptr;     // [1] max over vertices of sum_s |kb_s| (float bits), advection adjoint scale
+    void *adj_rec = nullptr;          // [4 Nv] stage records of the advection adjoint
 
     // cells (device): positions, metric and bending reference in FP64; the
     // other membrane arrays in the arithmetic type T
@@ -138,6 +140,7 @@ struct lbmibm_ctx {
     int *tile_cand = nullptr;                          // [ntiles] spreading candidates
     int *order_spread = nullptr, *order_interp = nullptr;  // [ntiles] heavy-first tile orders
     int *order_calls = nullptr;              // device counter of binnings (tile-order refresh)
+    float *blk_fmax = nullptr;               // [nblk_bin] max |f| per binning block
     unsigned *absmax = nullptr;
     bool bins_valid = false;                 // bins match the current positions
 
@@ -161,7 +164,6 @@ lbmibm_status lattice_fill(lbmibm_ctx *ctx, void *ptr, size_t count, double valu
 lbmibm_status lattice_adjoint(lbmibm_ctx *ctx, const void *ga_out, void *ga_in);  // lattice.hip
 lbmibm_status ibm_spread(lbmibm_ctx *ctx);                   // ibm.hip
 lbmibm_status ibm_interp(lbmibm_ctx *ctx);                   // ibm.hip
-lbmibm_status ibm_bin(lbmibm_ctx *ctx, hipStream_t stream);  // ibm.hip (positions only)
 lbmibm_status ibm_setup(lbmibm_ctx *ctx);                    // ibm.hip
 lbmibm_status ibm_spread_adjoint(lbmibm_ctx *ctx);           // ibm.hip
 lbmibm_status ibm_interp_adjoint(lbmibm_ctx *ctx);           // ibm.hip

@@ -196,11 +196,14 @@ __device__ __forceinline__ int wave_aggregated_add(int *counters, int key, bool 
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs, int nv, IbmGeom g, TileGeom tg,
-                                                   int *__restrict__ home, int *__restrict__ tile_count) {
+__global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs, const T *__restrict__ cf, int nv,
+                                                   IbmGeom g, TileGeom tg, int *__restrict__ home,
+                                                   int *__restrict__ tile_count, float *__restrict__ blk_fmax) {
+    __shared__ float shm[4];
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = v < nv;
     int t = 0;
+    float fm = 0.0f;
     if (on) {
         int x1, y1, z1;
         T wx[4], wy[4], wz[4];
@@ -217,8 +220,16 @@ __global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs
         yh = min(max(yh, 0), g.ny - 1);
         t = xw / TILE_X + tg.ntx * (yh / TILE_Y + tg.nty * (zw / TILE_Z));
         home[v] = t;
+        fm = fmaxf(fabsf((float)cf[v]), fmaxf(fabsf((float)cf[nv + v]), fabsf((float)cf[2 * (size_t)nv + v])));
+        fm *= 1.0001f;  // margin for the float rounding of |f|
     }
     (void)wave_aggregated_add(tile_count, t, on);
+    // per-block max |f| (one store per block; k_tile_scan reduces them)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) fm = fmaxf(fm, __shfl_xor(fm, o));
+    if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = fm;
+    __syncthreads();
+    if (threadIdx.x == 0) blk_fmax[blockIdx.x] = fmaxf(fmaxf(shm[0], shm[1]), fmaxf(shm[2], shm[3]));
 }
 
 // Heavy-first order of the tiles: a counting sort on a 256-level key of the
@@ -254,16 +265,23 @@ __device__ void heavy_first_order(const int *load, int n, int *hist, int *order)
 
 __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile_count, int *__restrict__ tile_off,
                                                     int *__restrict__ tile_cur, int ntiles,
+                                                    const float *__restrict__ blk_fmax, int nblk,
                                                     unsigned *__restrict__ absmax) {
-    // exclusive scan over tiles in one work group, 1024 entries per round;
-    // also clears the force maximum that k_bin_forces raises
+    // exclusive scan over tiles in one work group, 1024 entries per round,
+    // and the maximum of the per-block force maxima
     __shared__ int sh[1024];
+    __shared__ float shm[1024];
     __shared__ int carry;
-    if (threadIdx.x == 0) {
-        carry = 0;
-        *absmax = 0u;
-    }
+    float fm = 0.0f;
+    for (int i = threadIdx.x; i < nblk; i += 1024) fm = fmaxf(fm, blk_fmax[i]);
+    shm[threadIdx.x] = fm;
+    if (threadIdx.x == 0) carry = 0;
     __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) shm[threadIdx.x] = fmaxf(shm[threadIdx.x], shm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *absmax = __float_as_uint(shm[0]);
     for (int base = 0; base < ntiles; base += 1024) {
         const int i = base + threadIdx.x;
         const int v = i < ntiles ? tile_count[i] : 0;
@@ -327,34 +345,38 @@ template <typename T> struct SpreadRec {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bin_scatter(const double *__restrict__ xs, const int *__restrict__ home,
-                                                     int nv, IbmGeom g, const int *__restrict__ tile_off,
-                                                     int *__restrict__ tile_cur, int *__restrict__ home_list,
-                                                     short4 *__restrict__ node0, SpreadRec<T> *__restrict__ rec) {
+__global__ __launch_bounds__(256) void k_bin_scatter(const double *__restrict__ xs, const T *__restrict__ cf,
+                                                     const int *__restrict__ home, int nv, IbmGeom g,
+                                                     const int *__restrict__ tile_off, int *__restrict__ tile_cur,
+                                                     int *__restrict__ home_list, short4 *__restrict__ node0,
+                                                     SpreadRec<T> *__restrict__ rec) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = v < nv;
     const int t = on ? home[v] : 0;
     const int r = wave_aggregated_add(tile_cur, t, on);
-    if (!on) return;
-    const int pos = tile_off[t] + r;
-    home_list[pos] = v;
-    int x1, y1, z1;
-    T t3[3];
-    stencil_first<T>(xs[v] + (double)(g.nxp - 1) * g.dx / 2.0, g.dx, x1, t3[0]);
-    stencil_first<T>(xs[nv + v] + (double)(g.nyp - 1) * g.dx / 2.0, g.dx, y1, t3[1]);
-    stencil_first<T>(xs[2 * (size_t)nv + v] + (double)(g.nzp - 1) * g.dx / 2.0, g.dx, z1, t3[2]);
-    x1 -= 2; y1 -= 2; z1 -= 2;  // unpadded
-    int xw = x1, zw = z1;
-    wrap_index(xw, g.nx);
-    wrap_index(zw, g.nz);
-    xw = min(max(xw, -30000), 30000);
-    zw = min(max(zw, -30000), 30000);
-    node0[pos] = make_short4((short)xw, (short)min(max(y1 + 3, -30000), 30000), (short)zw, 0);
-    rec[pos].t[0] = t3[0]; rec[pos].t[1] = t3[1]; rec[pos].t[2] = t3[2];
+    if (on) {
+        const int pos = tile_off[t] + r;
+        home_list[pos] = v;
+        int x1, y1, z1;
+        SpreadRec<T> sr;
+        stencil_first<T>(xs[v] + (double)(g.nxp - 1) * g.dx / 2.0, g.dx, x1, sr.t[0]);
+        stencil_first<T>(xs[nv + v] + (double)(g.nyp - 1) * g.dx / 2.0, g.dx, y1, sr.t[1]);
+        stencil_first<T>(xs[2 * (size_t)nv + v] + (double)(g.nzp - 1) * g.dx / 2.0, g.dx, z1, sr.t[2]);
+        x1 -= 2; y1 -= 2; z1 -= 2;  // unpadded
+        int xw = x1, zw = z1;
+        wrap_index(xw, g.nx);
+        wrap_index(zw, g.nz);
+        xw = min(max(xw, -30000), 30000);
+        zw = min(max(zw, -30000), 30000);
+        node0[pos] = make_short4((short)xw, (short)min(max(y1 + 3, -30000), 30000), (short)zw, 0);
+        // the forces read in vertex order (coalesced), written in bin order
+        sr.f[0] = cf[v]; sr.f[1] = cf[nv + v]; sr.f[2] = cf[2 * (size_t)nv + v];
+        rec[pos] = sr;
+    }
 }
 
-// The force part of the spreading records, in bin order, and max |f| (the
-// binning itself only depends on the positions and may run earlier).
+// The force part of the spreading records, in bin order, and max |f|, when
+// the bins of unchanged positions are reused for new forces (a gather of cf).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bin_forces(const int *__restrict__ home_list, const T *__restrict__ cf, int nv,
                                                     SpreadRec<T> *__restrict__ rec, unsigned *__restrict__ absmax) {
@@ -827,33 +849,33 @@ __global__ __launch_bounds__(256) void k_spread_adj(const double *__restrict__ x
 // ua += phi^3(X_s) kb_s and X_s-bar = sum (u . kb_s) grad phi^3, which feeds
 // the stage before it (ibm.cu:430-517: X2 = X + h/2 k1, X3 = X + h/2 k2,
 // X4 = X + h k3, X' = X + h (k1 + 2 k2 + 2 k3 + k4)/6, h = M dt).
-// One work group per lattice tile takes the tile's home vertices (the bins
-// of the interpolation) and adds their velocity adjoint into an LDS box of
-// the tile plus its halo (the staging box of k_interp_tiles) with LDS
-// atomics; the box is then added to ua with one global atomic per node.
-// Stencils that leave the box add to ua directly.  Neighbouring vertices
-// share nodes, so adding every contribution to ua with global atomics would
-// serialise on them.
+// Two kernels.  k_advect_adj_vert, one thread per vertex in bin order, does
+// the gathers (the stage positions, X-bar) and writes each stage's position
+// and kb_s to a record; k_advect_adj_scatter, one work group per tile over
+// its home vertices, adds the velocity adjoint of the records.
+
+// a[k] for a runtime k by selects (a runtime index would put a in scratch)
+template <typename T> __device__ __forceinline__ T sel4(const T (&a)[4], int k) {
+    return k == 0 ? a[0] : k == 1 ? a[1] : k == 2 ? a[2] : a[3];
+}
+
+template <typename T> struct AdvRec {
+    double p[3];  // stage position
+    T k[3];       // kb_s
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_advect_adj(const double *__restrict__ xs, const T *__restrict__ U,
-                                                    double *__restrict__ xa, T *__restrict__ ua,
-                                                    const int *__restrict__ tile_count,
-                                                    const int *__restrict__ tile_off,
-                                                    const int *__restrict__ home_list, int nv, IbmGeom g,
-                                                    TileGeom tg, double h) {
-    extern __shared__ double dyn[];
-    T *Ub = (T *)dyn;  // [3][RZ][RY][RX] velocity adjoint of the box
-    const int tile = blockIdx.x;
-    const int cnt = tile_count[tile], off = tile_off[tile];
-    if (cnt == 0) return;  // whole work group leaves together
-    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
-    const int bx = tx * TILE_X - RLO, by = ty * TILE_Y - RLO, bz = tz * TILE_Z - RLO;
-    const size_t N = (size_t)g.nx * g.ny * g.nz;
-    for (int k = threadIdx.x; k < 3 * RN; k += blockDim.x) Ub[k] = T(0);
-    __syncthreads();
-    const T q64 = T(1.0 / 64.0);
-    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-        const int j = home_list[off + i];
+__global__ __launch_bounds__(256) void k_advect_adj_vert(const double *__restrict__ xs, const T *__restrict__ U,
+                                                         double *__restrict__ xa, const int *__restrict__ home_list,
+                                                         int nv, IbmGeom g, double h, AdvRec<T> *__restrict__ rec,
+                                                         unsigned *__restrict__ kbmax) {
+    __shared__ float shm[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // position in bin order
+    float km = 0.0f;
+    if (i < nv) {
+        const int j = home_list[i];
+        const size_t N = (size_t)g.nx * g.ny * g.nz;
+        const T q64 = T(1.0 / 64.0);
         const double X[3] = {xs[j], xs[nv + j], xs[2 * (size_t)nv + j]};
         auto interp = [&](const double p[3], double k[3]) {
             const StencilD<T> st(g, p[0], p[1], p[2]);
@@ -887,53 +909,19 @@ __global__ __launch_bounds__(256) void k_advect_adj(const double *__restrict__ x
             for (int d = 0; d < 3; d++) kb[s][d] = cw[s] * xb[d];
         const double back[4] = {0.0, h / 2.0, h / 2.0, h};  // X_s = X + back[s] k_{s-1}
 #pragma unroll
-        for (int s = 3; s >= 0; s--) {  // unrolled: P and kb stay in registers
-            const StencilD<T> st(g, P[s][0], P[s][1], P[s][2]);
+        for (int s = 3; s >= 0; s--) {  // kb_s is final here (stage s+1 came first)
             const T k0 = (T)kb[s][0], k1 = (T)kb[s][1], k2 = (T)kb[s][2];
-            // box coordinates of the first stencil node (as interp_tile)
-            int rx = st.x1 - bx, ry = st.y1 - by, rz = st.z1 - bz;
-            if (rx < 0) rx += g.nx; else if (rx >= g.nx) rx -= g.nx;
-            if (rz < 0) rz += g.nz; else if (rz >= g.nz) rz -= g.nz;
-            if (!g.walls) { if (ry < 0) ry += g.ny; else if (ry >= g.ny) ry -= g.ny; }
-            const bool inbox = rx >= 0 && rx + 3 < RX && ry >= 0 && ry + 3 < RY && rz >= 0 && rz + 3 < RZ;
+            AdvRec<T> r;
+            r.p[0] = P[s][0]; r.p[1] = P[s][1]; r.p[2] = P[s][2];
+            r.k[0] = k0; r.k[1] = k1; r.k[2] = k2;
+            rec[4 * (size_t)i + s] = r;
+            km += fmaxf(fabsf((float)k0), fmaxf(fabsf((float)k1), fabsf((float)k2)));
+            const StencilD<T> st(g, P[s][0], P[s][1], P[s][2]);
             T g0 = T(0), g1 = T(0), g2 = T(0);
-            // the stencil loop of StencilD::each, written out so that the LDS
-            // atomics address the box directly; unrolled so that the weight
-            // arrays are indexed by constants (registers, not scratch)
-#pragma unroll
-            for (int kz = 0; kz < 4; kz++) {
-                int iz = st.z1 + kz;
-                if (!wrap_index(iz, g.nz)) continue;
-#pragma unroll
-                for (int ky = 0; ky < 4; ky++) {
-                    int iy = st.y1 + ky;
-                    if (g.walls) {
-                        if (iy < 0 || iy >= g.ny) continue;
-                    } else if (!wrap_index(iy, g.ny)) continue;
-#pragma unroll
-                    for (int kx = 0; kx < 4; kx++) {
-                        int ix = st.x1 + kx;
-                        if (!wrap_index(ix, g.nx)) continue;
-                        const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
-                        const T w = st.wx[kx] * st.wy[ky] * st.wz[kz];
-                        const T wq = w * q64;
-                        if (inbox) {  // node (x1+kx, y1+ky, z1+kz) is box node (rx+kx, ry+ky, rz+kz)
-                            const int bi = (rx + kx) + RX * ((ry + ky) + RY * (rz + kz));
-                            atomicAdd((T *)dyn + bi, wq * k0);
-                            atomicAdd((T *)dyn + RN + bi, wq * k1);
-                            atomicAdd((T *)dyn + 2 * RN + bi, wq * k2);
-                        } else {
-                            atomicAdd(&ua[n], wq * k0);
-                            atomicAdd(&ua[N + n], wq * k1);
-                            atomicAdd(&ua[2 * N + n], wq * k2);
-                        }
-                        const T sdot = U[n] * k0 + U[N + n] * k1 + U[2 * N + n] * k2;
-                        g0 += sdot * st.dx[kx] * st.wy[ky] * st.wz[kz];
-                        g1 += sdot * st.wx[kx] * st.dy[ky] * st.wz[kz];
-                        g2 += sdot * st.wx[kx] * st.wy[ky] * st.dz[kz];
-                    }
-                }
-            }
+            st.each(g, [&](size_t n, T, T gx, T gy, T gz, int, int, int) {
+                const T sdot = U[n] * k0 + U[N + n] * k1 + U[2 * N + n] * k2;
+                g0 += sdot * gx; g1 += sdot * gy; g2 += sdot * gz;
+            });
             const double xs_[3] = {(double)(g0 * q64), (double)(g1 * q64), (double)(g2 * q64)};
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -944,11 +932,155 @@ __global__ __launch_bounds__(256) void k_advect_adj(const double *__restrict__ x
         xa[j] = acc[0];
         xa[nv + j] = acc[1];
         xa[2 * (size_t)nv + j] = acc[2];
+        km *= 1.0001f;  // margin for the float rounding
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) km = fmaxf(km, __shfl_xor(km, o));
+    if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = km;
+    __syncthreads();
+    // non-negative floats order as their bit patterns: an integer maximum
+    if (threadIdx.x == 0) atomicMax(kbmax, __float_as_uint(fmaxf(fmaxf(shm[0], shm[1]), fmaxf(shm[2], shm[3]))));
+}
+
+// One work group per tile takes its home vertices (the bins of the
+// interpolation) and adds ua += sum_s phi^3(X_s) kb_s / 64 into an LDS box
+// of the tile plus its halo (the staging box of k_interp_tiles), then adds
+// the box to ua with one global atomic per node; stencils that leave the box
+// add to ua directly.  The four stage stencils of a vertex nearly always
+// start at the same node (the stages move it by h |u| << 1): then the four
+// terms are summed per node and added once.  Lane l of wave w takes column
+// 8l + w of the bin read as 512 columns, so the lanes of a wave hold vertices
+// far apart, whose nodes rarely coincide (LDS atomics to one address
+// serialise).  FP32 accumulates in 64-bit fixed point, as the spreading
+// (raw bits of fma(v, scale, 1.5 2^52), whose low 48 bits sum exactly), with
+// the scale from the bound cnt max_v sum_s |kb_s| / 8 of a node's sum (a
+// weight product is at most 8/64); FP64 uses FP64 LDS atomics.
+constexpr int ADV_WG = 512;  // work-group size of k_advect_adj_scatter
+
+template <typename T>
+__global__ __launch_bounds__(ADV_WG) void k_advect_adj_scatter(const AdvRec<T> *__restrict__ rec, T *__restrict__ ua,
+                                                            const int *__restrict__ tile_count,
+                                                            const int *__restrict__ tile_off, IbmGeom g,
+                                                            TileGeom tg, const unsigned *__restrict__ kbmax,
+                                                            const int *__restrict__ order) {
+    extern __shared__ double dyn[];
+    constexpr bool FIX = sizeof(T) == 4;
+    unsigned long long *Ub = (unsigned long long *)dyn;  // [3][RZ][RY][RX]
+    double *Ud = dyn;
+    const int tile = order[blockIdx.x];
+    const int cnt = tile_count[tile], off = tile_off[tile];
+    if (cnt == 0) return;  // whole work group leaves together
+    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+    const int bx = tx * TILE_X - RLO, by = ty * TILE_Y - RLO, bz = tz * TILE_Z - RLO;
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    double scale = 1.0;
+    if (FIX) {
+        const double km = (double)__uint_as_float(*kbmax);
+        if (km > 0.0) scale = ldexp(1.0, 46 - ilogb((double)cnt * km / 8.0));
+    }
+    for (int k = threadIdx.x; k < 3 * RN; k += blockDim.x) Ub[k] = 0ull;
+    __syncthreads();
+    const T q64 = T(1.0 / 64.0);
+    constexpr int NW = ADV_WG / 64;
+    const int R = (cnt + ADV_WG - 1) / ADV_WG;
+    const int col = NW * (threadIdx.x & 63) + (threadIdx.x >> 6);
+    for (int r = 0; r < R; r++) {
+        const int i = col * R + r;
+        if (i >= cnt) break;
+        int x1[4], y1[4], z1[4];
+        T wx[4][4], wy[4][4], wz[4][4], kk[4][3];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const AdvRec<T> rr = rec[4 * (size_t)(off + i) + s];
+            stencil3<T>(g, rr.p[0], rr.p[1], rr.p[2], false, x1[s], y1[s], z1[s], wx[s], wy[s], wz[s]);
+            kk[s][0] = rr.k[0] * q64; kk[s][1] = rr.k[1] * q64; kk[s][2] = rr.k[2] * q64;
+        }
+        bool same = true;
+#pragma unroll
+        for (int s = 1; s < 4; s++) same = same && x1[s] == x1[0] && y1[s] == y1[0] && z1[s] == z1[0];
+        // one walk of the common stencil with all four stages, or one walk per
+        // stage with the other stages masked out
+        const int ns = same ? 1 : 4;
+        for (int it = 0; it < ns; it++) {
+            T m[4];
+            int sx = x1[0], sy = y1[0], sz = z1[0];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                m[s] = (same || s == it) ? T(1) : T(0);
+                if (!same && s == it) { sx = x1[s]; sy = y1[s]; sz = z1[s]; }
+            }
+            // box coordinates of the first stencil node (as interp_tile)
+            int rx = sx - bx, ry = sy - by, rz = sz - bz;
+            if (rx < 0) rx += g.nx; else if (rx >= g.nx) rx -= g.nx;
+            if (rz < 0) rz += g.nz; else if (rz >= g.nz) rz -= g.nz;
+            if (!g.walls) { if (ry < 0) ry += g.ny; else if (ry >= g.ny) ry -= g.ny; }
+            const bool inbox = rx >= 0 && rx + 3 < RX && ry >= 0 && ry + 3 < RY && rz >= 0 && rz + 3 < RZ;
+#pragma unroll 1
+            for (int kz = 0; kz < 4; kz++) {
+                int iz = sz + kz;
+                if (!wrap_index(iz, g.nz)) continue;
+                T cz[4][3];  // mask * kb_s/64 * wz_s
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const T w = m[s] * sel4(wz[s], kz);
+                    cz[s][0] = w * kk[s][0]; cz[s][1] = w * kk[s][1]; cz[s][2] = w * kk[s][2];
+                }
+#pragma unroll
+                for (int ky = 0; ky < 4; ky++) {
+                    int iy = sy + ky;
+                    if (g.walls) {
+                        if (iy < 0 || iy >= g.ny) continue;
+                    } else if (!wrap_index(iy, g.ny)) continue;
+                    T cy[4][3];
+#pragma unroll
+                    for (int s = 0; s < 4; s++)
+#pragma unroll
+                        for (int d = 0; d < 3; d++) cy[s][d] = wy[s][ky] * cz[s][d];
+#pragma unroll
+                    for (int kx = 0; kx < 4; kx++) {
+                        int ix = sx + kx;
+                        if (!wrap_index(ix, g.nx)) continue;
+                        const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
+                        const int bi = (rx + kx) + RX * ((ry + ky) + RY * (rz + kz));
+                        T v0 = T(0), v1 = T(0), v2 = T(0);
+#pragma unroll
+                        for (int s = 0; s < 4; s++) {
+                            v0 += wx[s][kx] * cy[s][0]; v1 += wx[s][kx] * cy[s][1]; v2 += wx[s][kx] * cy[s][2];
+                        }
+                        // into the box when the stencil lies in it, else into ua (written
+                        // out here: the LDS atomics must see the LDS pointer itself)
+                        if (inbox) {
+                            if (FIX) {
+                                const double big = 6755399441055744.0;  // 1.5 2^52
+                                atomicAdd(Ub + bi,
+                                          (unsigned long long)__double_as_longlong(fma((double)v0, scale, big)));
+                                atomicAdd(Ub + RN + bi,
+                                          (unsigned long long)__double_as_longlong(fma((double)v1, scale, big)));
+                                atomicAdd(Ub + 2 * RN + bi,
+                                          (unsigned long long)__double_as_longlong(fma((double)v2, scale, big)));
+                            } else {
+                                atomicAdd(Ud + bi, (double)v0);
+                                atomicAdd(Ud + RN + bi, (double)v1);
+                                atomicAdd(Ud + 2 * RN + bi, (double)v2);
+                            }
+                        } else {
+                            atomicAdd(&ua[n], v0);
+                            atomicAdd(&ua[N + n], v1);
+                            atomicAdd(&ua[2 * N + n], v2);
+                        }
+                    }
+                }
+            }
+        }
     }
     __syncthreads();
     // the box into ua (the box may wrap onto itself on small lattices: atomics)
+    const double inv_scale = 1.0 / scale;
     for (int k = threadIdx.x; k < RN; k += blockDim.x) {
-        const T a0 = Ub[k], a1 = Ub[RN + k], a2 = Ub[2 * RN + k];
+        auto val = [&](int i) {
+            return FIX ? (T)((double)((long long)(Ub[i] << 16) >> 16) * inv_scale) : (T)Ud[i];
+        };
+        const T a0 = val(k), a1 = val(RN + k), a2 = val(2 * RN + k);
         if (a0 == T(0) && a1 == T(0) && a2 == T(0)) continue;
         const int rx = k % RX, ry = (k / RX) % RY, rz = k / (RX * RY);
         const int ix = pmod(bx + rx, g.nx), iz = pmod(bz + rz, g.nz);
@@ -974,27 +1106,28 @@ IbmGeom geom_of(const lbmibm_ctx *ctx) {
     return g;
 }
 
-// Binning of the vertices by the tile of their first stencil node, from the
-// positions only, on the given stream.
-template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx, hipStream_t stream) {
+// Binning of the vertices by the tile of their first stencil node, with the
+// spreading records of the current nodal forces and max |f|.
+template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
+    hipStream_t stream = ctx->stream;
     const IbmGeom g = geom_of(ctx);
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const int nv = ctx->nv;
     hipLaunchKernelGGL(k_zero_int, dim3((ctx->ntiles + 255) / 256), dim3(256), 0, stream, ctx->tile_count,
                        ctx->ntiles);
     LBMIBM_LAUNCHED(ctx, "k_zero_int");
-    hipLaunchKernelGGL(k_bin_count<T>, dim3(ctx->nblk_bin), dim3(256), 0, stream, (const double *)ctx->xs, nv, g, tg,
-                       ctx->home, ctx->tile_count);
+    hipLaunchKernelGGL(k_bin_count<T>, dim3(ctx->nblk_bin), dim3(256), 0, stream, (const double *)ctx->xs,
+                       (const T *)ctx->cf, nv, g, tg, ctx->home, ctx->tile_count, ctx->blk_fmax);
     LBMIBM_LAUNCHED(ctx, "k_bin_count");
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->tile_off, ctx->tile_cur,
-                       ctx->ntiles, ctx->absmax);
+                       ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax);
     LBMIBM_LAUNCHED(ctx, "k_tile_scan");
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->ntiles, g, tg,
                        ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->order_calls);
     LBMIBM_LAUNCHED(ctx, "k_tile_order");
     hipLaunchKernelGGL(k_bin_scatter<T>, dim3(ctx->nblk_bin), dim3(256), 0, stream, (const double *)ctx->xs,
-                       ctx->home, nv, g, ctx->tile_off, ctx->tile_cur, ctx->home_list, ctx->node0,
-                       (SpreadRec<T> *)ctx->srec);
+                       (const T *)ctx->cf, ctx->home, nv, g, ctx->tile_off, ctx->tile_cur, ctx->home_list,
+                       ctx->node0, (SpreadRec<T> *)ctx->srec);
     LBMIBM_LAUNCHED(ctx, "k_bin_scatter");
     ctx->bins_valid = true;
     return LBMIBM_OK;
@@ -1002,12 +1135,16 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx, hipStream_t stream
 
 template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return lattice_fill(ctx, ctx->F, 3 * (size_t)ctx->N, 0.0, false);
-    lbmibm_status s = ctx->bins_valid ? LBMIBM_OK : bin_impl<T>(ctx, ctx->stream);
-    if (s != LBMIBM_OK) return s;
-    // the nodal forces change with every call: their part of the records and max |f|
-    hipLaunchKernelGGL(k_bin_forces<T>, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, ctx->home_list,
-                       (const T *)ctx->cf, ctx->nv, (SpreadRec<T> *)ctx->srec, ctx->absmax);
-    LBMIBM_LAUNCHED(ctx, "k_bin_forces");
+    if (!ctx->bins_valid) {
+        lbmibm_status s = bin_impl<T>(ctx);
+        if (s != LBMIBM_OK) return s;
+    } else {  // same positions, new forces: refresh the force part of the records
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->absmax, 1);
+        LBMIBM_LAUNCHED(ctx, "k_zero_int");
+        hipLaunchKernelGGL(k_bin_forces<T>, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, ctx->home_list,
+                           (const T *)ctx->cf, ctx->nv, (SpreadRec<T> *)ctx->srec, ctx->absmax);
+        LBMIBM_LAUNCHED(ctx, "k_bin_forces");
+    }
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     hipLaunchKernelGGL(k_spread_tiles<T>, dim3(ctx->ntiles), dim3(256), 0, ctx->stream,
                        (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off, (T *)ctx->F,
@@ -1019,7 +1156,7 @@ template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
 template <typename T> lbmibm_status interp_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return LBMIBM_OK;
     if (!ctx->bins_valid) {
-        lbmibm_status s = bin_impl<T>(ctx, ctx->stream);
+        lbmibm_status s = bin_impl<T>(ctx);
         if (s != LBMIBM_OK) return s;
     }
     AdvectArgs a;
@@ -1050,15 +1187,22 @@ template <typename T> lbmibm_status spread_adj_impl(lbmibm_ctx *ctx) {
 template <typename T> lbmibm_status advect_adj_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return LBMIBM_OK;
     if (!ctx->bins_valid) {
-        lbmibm_status s = bin_impl<T>(ctx, ctx->stream);
+        lbmibm_status s = bin_impl<T>(ctx);
         if (s != LBMIBM_OK) return s;
     }
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->adj_xmax, 1);
+    LBMIBM_LAUNCHED(ctx, "k_zero_int");
+    const IbmGeom g = geom_of(ctx);
+    hipLaunchKernelGGL(k_advect_adj_vert<T>, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const double *)ctx->xs, (const T *)ctx->U, ctx->adj_x, ctx->home_list, ctx->nv, g,
+                       ctx->p.m * ctx->p.dt, (AdvRec<T> *)ctx->adj_rec, ctx->adj_xmax);
+    LBMIBM_LAUNCHED(ctx, "k_advect_adj_vert");
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
-    const size_t lds = 3 * (size_t)RN * sizeof(T);
-    hipLaunchKernelGGL(k_advect_adj<T>, dim3(ctx->ntiles), dim3(256), lds, ctx->stream, (const double *)ctx->xs,
-                       (const T *)ctx->U, ctx->adj_x, (T *)ctx->adj_u, ctx->tile_count, ctx->tile_off,
-                       ctx->home_list, ctx->nv, geom_of(ctx), tg, ctx->p.m * ctx->p.dt);
-    LBMIBM_LAUNCHED(ctx, "k_advect_adj");
+    const size_t lds = 3 * (size_t)RN * sizeof(unsigned long long);
+    hipLaunchKernelGGL(k_advect_adj_scatter<T>, dim3(ctx->ntiles), dim3(ADV_WG), lds, ctx->stream,
+                       (const AdvRec<T> *)ctx->adj_rec, (T *)ctx->adj_u, ctx->tile_count, ctx->tile_off, g, tg,
+                       ctx->adj_xmax, ctx->order_interp);
+    LBMIBM_LAUNCHED(ctx, "k_advect_adj_scatter");
     return LBMIBM_OK;
 }
 
@@ -1083,16 +1227,11 @@ lbmibm_status ibm_setup(lbmibm_ctx *ctx) {
                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds64));
     LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_interp_tiles<float>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds32));
-    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_advect_adj<double>,
+    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_advect_adj_scatter<double>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds64));
-    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_advect_adj<float>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds32));
+    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_advect_adj_scatter<float>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds64));
     return LBMIBM_OK;
-}
-
-lbmibm_status ibm_bin(lbmibm_ctx *ctx, hipStream_t stream) {
-    if (ctx->nv == 0) return LBMIBM_OK;
-    return ctx->prec == LBMIBM_F64 ? bin_impl<double>(ctx, stream) : bin_impl<float>(ctx, stream);
 }
 
 lbmibm_status ibm_spread(lbmibm_ctx *ctx) {
