@@ -753,10 +753,10 @@ __global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, d
 // weights are (1 - sin, 1 + cos, 1 + sin, 1 - cos)(pi t/2), and their
 // derivatives (pi/2h) (-cos, -sin, cos, sin)(pi t/2) agree at t = 0 and 1
 // with those of the neighbouring stencil, so the stencil switch adds no term.
-// Both adjoints walk the 4x4x4 stencil of each vertex in global memory (one
-// thread per vertex): the spread adjoint is a gather, the advection adjoint
-// gathers for the position terms and adds the velocity adjoint to the nodes
-// with atomics.
+// The spread adjoint is a gather over each vertex's 4x4x4 stencil in global
+// memory (one thread per vertex).  The advection adjoint gathers for the
+// position terms the same way, and adds the velocity adjoint to the nodes
+// per tile in LDS (k_advect_adj_scatter).
 
 template <typename W>
 __device__ __forceinline__ void stencil_d(double xtr, double h, int &first, W w[4], W dw[4]) {
