@@ -113,11 +113,14 @@ const char   *lbmibm_last_error(const lbmibm_ctx *ctx);
 /* No reference counterpart.  lbmibm_create times the candidate work-group
  * shapes of the on-chip membrane solvers and the forms of the large-cell
  * solver (0 vectors in memory, 1 p in LDS, 2 register-resident) on the
- * cells' own matrices and keeps the fastest.  In the environment,
- * LBMIBM_AUTOTUNE=0 skips it and uses the first shapes and form 0;
+ * cells' own matrices, and the two forms of the interpolation (0 tiles staged
+ * in LDS, 1 one thread per vertex reading the field's rows) on the initial
+ * positions, and keeps the fastest of each.  In the environment,
+ * LBMIBM_AUTOTUNE=0 skips it and uses the first shapes and forms 0;
  * LBMIBM_BICG_SHAPES="a,b[,c]" fixes the variants of the two on-chip classes
  * and the large-cell form (a form the largest cell does not fit falls back to
- * form 0).  This returns the timings and choices as text, owned by ctx. */
+ * form 0); LBMIBM_INTERP_FORM=0|1 fixes the interpolation form.  This returns
+ * the timings and choices as text, owned by ctx. */
 const char   *lbmibm_autotune_report(const lbmibm_ctx *ctx);
 /* Sizes: padded lattice nodes n, vertices, elements, cells. */
 void          lbmibm_sizes(const lbmibm_ctx *ctx, long long *n_padded, int *nv, int *nt, int *nc);

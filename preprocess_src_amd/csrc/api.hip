@@ -524,6 +524,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     }
     CK(set_step(ctx, 0, false));
     CK(membrane_autotune(ctx));
+    CK(ibm_autotune(ctx));
     CK(reset_status(ctx));
     CK(sync_check(ctx));
 #undef CK
@@ -620,6 +621,9 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
         if (!ctx->graph_ready) {
             ctx->graph_cur = ctx->cur;
             ctx->graph_force_zero = ctx->force_zero;
+            // the block's first spreading must bin: on replay the positions
+            // are the previous block's, whatever the host flag says now
+            ctx->bins_valid = false;
             hipGraph_t g = nullptr;
             LBMIBM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             lbmibm_status cs = LBMIBM_OK;

@@ -111,6 +111,7 @@ struct lbmibm_ctx {
     int *bicg_cells[3] = {nullptr, nullptr, nullptr};
     int bicg_n[3] = {0, 0, 0}, bicg_maxv[3] = {0, 0, 0};
     int bicg_var[3] = {0, 0, 0};  // shape variant of the on-chip classes, form of class 2 (autotuned)
+    int interp_form = 0;          // 0: tiles staged in LDS, 1: one thread per vertex, rows from memory (autotuned)
     int big_lds = 0;              // dynamic LDS of the class-2 forms that use LDS (the whole CU)
     int *onchip_cells = nullptr;  // cells of classes 0 and 1
     int n_onchip = 0;
@@ -173,6 +174,7 @@ lbmibm_status membrane_forces_adjoint(lbmibm_ctx *ctx, bool have_q);  // membran
 int bicg_class(int prec, int nvert);                         // membrane.hip
 lbmibm_status membrane_setup(lbmibm_ctx *ctx);               // membrane.hip
 lbmibm_status membrane_autotune(lbmibm_ctx *ctx);            // membrane.hip
+lbmibm_status ibm_autotune(lbmibm_ctx *ctx);                 // ibm.hip
 lbmibm_status volume_run(lbmibm_ctx *ctx, int mode);         // volume.hip (0 init, 1 correct)
 // both classes on the given stream (not the context's main stream)
 lbmibm_status volume_run_on(lbmibm_ctx *ctx, int mode, hipStream_t stream);  // volume.hip

@@ -672,8 +672,120 @@ struct AdvectArgs {
     int periodic_y;
 };
 
+// Four consecutive values of one stencil row, read with one (unaligned)
+// wide load.
+template <typename T> struct Row4 {
+    T v[4];
+} __attribute__((aligned(sizeof(T))));
+
+// Interpolation straight from the velocity field: one wide load per stencil
+// row and component, the same products and summation order as interp_tile.
+// A row that wraps in x is read node by node.
 template <typename T>
-__global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, double *__restrict__ xs,
+__device__ __forceinline__ void interp_rows(const T *__restrict__ U, size_t N, const IbmGeom &g, double x,
+                                            double y, double z, bool unit, T &ur, T &vr, T &wr) {
+    int x1, y1, z1;
+    T wx[4], wy[4], wz[4];
+    stencil3<T>(g, x, y, z, unit, x1, y1, z1, wx, wy, wz);
+    const bool whole = x1 >= 0 && x1 + 3 < g.nx;
+    int ixw[4];
+    bool okx[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        ixw[k] = x1 + k;
+        okx[k] = wrap_index(ixw[k], g.nx);
+    }
+    ur = vr = wr = T(0);
+#pragma unroll
+    for (int kz = 0; kz < 4; kz++) {
+        int iz = z1 + kz;
+        if (!wrap_index(iz, g.nz)) continue;
+        const T wz64 = wz[kz] * T(1.0 / 64.0);
+#pragma unroll
+        for (int ky = 0; ky < 4; ky++) {
+            int iy = y1 + ky;
+            if (g.walls) {
+                if (iy < 0 || iy >= g.ny) continue;
+            } else if (!wrap_index(iy, g.ny)) continue;
+            const T wyz = wy[ky] * wz64;
+            const size_t rowb = (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
+            T ru[4], rv[4], rw[4];
+            if (whole) {
+                const Row4<T> a = *(const Row4<T> *)(U + rowb + x1);
+                const Row4<T> b = *(const Row4<T> *)(U + N + rowb + x1);
+                const Row4<T> c = *(const Row4<T> *)(U + 2 * N + rowb + x1);
+#pragma unroll
+                for (int kx = 0; kx < 4; kx++) { ru[kx] = a.v[kx]; rv[kx] = b.v[kx]; rw[kx] = c.v[kx]; }
+            } else {
+#pragma unroll
+                for (int kx = 0; kx < 4; kx++) {
+                    const size_t n = rowb + (okx[kx] ? ixw[kx] : 0);
+                    ru[kx] = okx[kx] ? U[n] : T(0);
+                    rv[kx] = okx[kx] ? U[N + n] : T(0);
+                    rw[kx] = okx[kx] ? U[2 * N + n] : T(0);
+                }
+            }
+#pragma unroll
+            for (int kx = 0; kx < 4; kx++) {
+                const T dlt = wx[kx] * wyz;
+                ur += dlt * ru[kx];
+                vr += dlt * rv[kx];
+                wr += dlt * rw[kx];
+            }
+        }
+    }
+}
+
+// The RK stages of vertex j (ibm.cu:430-517, 524-551, 581-608) with the
+// interpolation ip(x, y, z, unit, u, v, w), then the periodic wrap of the new
+// position (ibm.cu:616-632) and its stores.  xin and xo may be the same array
+// (each vertex is read before it is written, by the same thread).
+template <typename T, typename Interp>
+__device__ __forceinline__ void advect_vertex(int j, const double *xin, double *xo, T *__restrict__ uvw,
+                                              double *__restrict__ xd, int nv, const AdvectArgs &a, Interp ip) {
+    const double M = a.M, dt = a.dt;
+    const bool unit = a.rk != 4;
+    const double xn = xin[j], yn = xin[nv + j], zn = xin[2 * (size_t)nv + j];
+    // stage s is taken at X + k_{s-1} (M dt/2, M dt/2, M dt) for RK4 and
+    // X + k_1 M for RK2, each offset evaluated as (k M) dt / 2 in that order;
+    // the stage sum is (k1 + 2 k2 + 2 k3 + k4)/6, (k1 + k2)/2, k1.  One loop
+    // keeps a single inlined copy of the interpolation.
+    const int ns = a.rk == 4 ? 4 : (a.rk == 2 ? 2 : 1);
+    T ku = T(0), kv = T(0), kw = T(0), su = T(0), sv = T(0), sw = T(0);
+#pragma unroll 1
+    for (int s = 0; s < ns; s++) {
+        double ox = 0.0, oy = 0.0, oz = 0.0;
+        if (s > 0) {
+            ox = (double)ku * M; oy = (double)kv * M; oz = (double)kw * M;
+            if (a.rk == 4) {
+                ox *= dt; oy *= dt; oz *= dt;
+                if (s < 3) { ox /= 2.0; oy /= 2.0; oz /= 2.0; }
+            }
+        }
+        ip(xn + ox, yn + oy, zn + oz, unit, ku, kv, kw);
+        const T wt = (a.rk == 4 && (s == 1 || s == 2)) ? T(2) : T(1);
+        su += wt * ku; sv += wt * kv; sw += wt * kw;
+    }
+    const T den = a.rk == 4 ? T(6) : (a.rk == 2 ? T(2) : T(1));
+    const T um = su / den, vm = sv / den, wm = sw / den;
+    double xr = xn + (double)um * M * dt, yr = yn + (double)vm * M * dt, zr = zn + (double)wm * M * dt;
+    if (xr < -a.lx / 2.0) xr += a.lx;
+    if (xr > a.lx / 2.0) xr -= a.lx;
+    if (zr < -a.lz / 2.0) zr += a.lz;
+    if (zr > a.lz / 2.0) zr -= a.lz;
+    if (a.periodic_y) {
+        if (yr < -a.ly / 2.0) yr += a.ly;
+        if (yr > a.ly / 2.0) yr -= a.ly;
+    }
+    if (uvw) { uvw[j] = um; uvw[nv + j] = vm; uvw[2 * (size_t)nv + j] = wm; }
+    xo[j] = xr; xo[nv + j] = yr; xo[2 * (size_t)nv + j] = zr;
+    xd[3 * (size_t)j] = xr; xd[3 * (size_t)j + 1] = yr; xd[3 * (size_t)j + 2] = zr;
+}
+
+// Interpolation form 0: one work group per tile stages the tile's velocity
+// plus halo in LDS once and runs its home vertices from there.
+template <typename T>
+__global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, const double *xin, double *xo,
                                                       T *__restrict__ uvw, double *__restrict__ xd,
                                                       const int *__restrict__ tile_count,
                                                       const int *__restrict__ tile_off,
@@ -703,48 +815,28 @@ __global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, d
         Us[k] = u; Us[RN + k] = v; Us[2 * RN + k] = w;
     }
     __syncthreads();
-    const double M = a.M, dt = a.dt;
-    const bool unit = a.rk != 4;
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-        const int j = home_list[off + i];
-        const double xn = xs[j], yn = xs[nv + j], zn = xs[2 * (size_t)nv + j];
-        T um, vm, wm;
-        if (a.rk == 4) {
-            T u1, v1, w1, u2, v2, w2, u3, v3, w3, u4, v4, w4;
-            interp_tile<T>(U, N, Us, g, bx, by, bz, xn, yn, zn, false, u1, v1, w1);
-            interp_tile<T>(U, N, Us, g, bx, by, bz, xn + (double)u1 * M * dt / 2.0, yn + (double)v1 * M * dt / 2.0,
-                           zn + (double)w1 * M * dt / 2.0, false, u2, v2, w2);
-            interp_tile<T>(U, N, Us, g, bx, by, bz, xn + (double)u2 * M * dt / 2.0, yn + (double)v2 * M * dt / 2.0,
-                           zn + (double)w2 * M * dt / 2.0, false, u3, v3, w3);
-            interp_tile<T>(U, N, Us, g, bx, by, bz, xn + (double)u3 * M * dt, yn + (double)v3 * M * dt,
-                           zn + (double)w3 * M * dt, false, u4, v4, w4);
-            um = (u1 + T(2) * u2 + T(2) * u3 + u4) / T(6);
-            vm = (v1 + T(2) * v2 + T(2) * v3 + v4) / T(6);
-            wm = (w1 + T(2) * w2 + T(2) * w3 + w4) / T(6);
-        } else if (a.rk == 2) {
-            T u1, v1, w1, u2, v2, w2;
-            interp_tile<T>(U, N, Us, g, bx, by, bz, xn, yn, zn, unit, u1, v1, w1);
-            interp_tile<T>(U, N, Us, g, bx, by, bz, xn + (double)u1 * M, yn + (double)v1 * M, zn + (double)w1 * M,
-                           unit, u2, v2, w2);
-            um = (u1 + u2) / T(2);
-            vm = (v1 + v2) / T(2);
-            wm = (w1 + w2) / T(2);
-        } else {
-            interp_tile<T>(U, N, Us, g, bx, by, bz, xn, yn, zn, unit, um, vm, wm);
-        }
-        double xr = xn + (double)um * M * dt, yr = yn + (double)vm * M * dt, zr = zn + (double)wm * M * dt;
-        if (xr < -a.lx / 2.0) xr += a.lx;
-        if (xr > a.lx / 2.0) xr -= a.lx;
-        if (zr < -a.lz / 2.0) zr += a.lz;
-        if (zr > a.lz / 2.0) zr -= a.lz;
-        if (a.periodic_y) {
-            if (yr < -a.ly / 2.0) yr += a.ly;
-            if (yr > a.ly / 2.0) yr -= a.ly;
-        }
-        if (uvw) { uvw[j] = um; uvw[nv + j] = vm; uvw[2 * (size_t)nv + j] = wm; }
-        xs[j] = xr; xs[nv + j] = yr; xs[2 * (size_t)nv + j] = zr;
-        xd[3 * (size_t)j] = xr; xd[3 * (size_t)j + 1] = yr; xd[3 * (size_t)j + 2] = zr;
+        advect_vertex<T>(home_list[off + i], xin, xo, uvw, xd, nv, a,
+                         [&](double x, double y, double z, bool unit, T &u, T &v, T &w) {
+                             interp_tile<T>(U, N, Us, g, bx, by, bz, x, y, z, unit, u, v, w);
+                         });
     }
+}
+
+// Interpolation form 1: one thread per vertex in bin order (neighbouring
+// lanes read neighbouring rows), rows read from the field through the caches.
+template <typename T>
+__global__ __launch_bounds__(256) void k_interp_vert(const T *__restrict__ U, const double *xin, double *xo,
+                                                     T *__restrict__ uvw, double *__restrict__ xd,
+                                                     const int *__restrict__ home_list, int nv, IbmGeom g,
+                                                     AdvectArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    advect_vertex<T>(home_list[i], xin, xo, uvw, xd, nv, a,
+                     [&](double x, double y, double z, bool unit, T &u, T &v, T &w) {
+                         interp_rows<T>(U, N, g, x, y, z, unit, u, v, w);
+                     });
 }
 
 // ---- adjoints (vector-Jacobian products) ------------------------------------
@@ -1153,26 +1245,87 @@ template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
     return LBMIBM_OK;
 }
 
+// One interpolation + advection of all vertices with the given form, from
+// xin into xo / xd (uvw may be null).
+template <typename T>
+lbmibm_status interp_launch(lbmibm_ctx *ctx, int form, const double *xin, double *xo, T *uvw, double *xd) {
+    AdvectArgs a;
+    a.dt = ctx->p.dt; a.M = ctx->p.m;
+    a.lx = ctx->nx * ctx->p.dx; a.ly = ctx->ny * ctx->p.dx; a.lz = ctx->nz * ctx->p.dx;
+    a.rk = ctx->p.runge_kutta;
+    a.periodic_y = ctx->p.periodic_y;
+    if (form == 1) {
+        hipLaunchKernelGGL(k_interp_vert<T>, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream,
+                           (const T *)ctx->U, xin, xo, uvw, xd, ctx->home_list, ctx->nv, geom_of(ctx), a);
+        LBMIBM_LAUNCHED(ctx, "k_interp_vert");
+        return LBMIBM_OK;
+    }
+    TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
+    const size_t lds = 3 * (size_t)RN * sizeof(T);
+    hipLaunchKernelGGL(k_interp_tiles<T>, dim3(ctx->ntiles), dim3(256), lds, ctx->stream, (const T *)ctx->U, xin,
+                       xo, uvw, xd, ctx->tile_count, ctx->tile_off, ctx->home_list, ctx->nv, geom_of(ctx), tg, a,
+                       ctx->order_interp);
+    LBMIBM_LAUNCHED(ctx, "k_interp_tiles");
+    return LBMIBM_OK;
+}
+
 template <typename T> lbmibm_status interp_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return LBMIBM_OK;
     if (!ctx->bins_valid) {
         lbmibm_status s = bin_impl<T>(ctx);
         if (s != LBMIBM_OK) return s;
     }
-    AdvectArgs a;
-    a.dt = ctx->p.dt; a.M = ctx->p.m;
-    a.lx = ctx->nx * ctx->p.dx; a.ly = ctx->ny * ctx->p.dx; a.lz = ctx->nz * ctx->p.dx;
-    a.rk = ctx->p.runge_kutta;
-    a.periodic_y = ctx->p.periodic_y;
-    TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
-    const size_t lds = 3 * (size_t)RN * sizeof(T);
-    hipLaunchKernelGGL(k_interp_tiles<T>, dim3(ctx->ntiles), dim3(256), lds, ctx->stream, (const T *)ctx->U,
-                       (double *)ctx->xs, ctx->p.store_macros ? (T *)ctx->uvw : (T *)nullptr, (double *)ctx->xd,
-                       ctx->tile_count, ctx->tile_off,
-                       ctx->home_list, ctx->nv, geom_of(ctx), tg, a, ctx->order_interp);
-    LBMIBM_LAUNCHED(ctx, "k_interp_tiles");
+    lbmibm_status s = interp_launch<T>(ctx, ctx->interp_form, (const double *)ctx->xs, (double *)ctx->xs,
+                                       ctx->p.store_macros ? (T *)ctx->uvw : (T *)nullptr, (double *)ctx->xd);
     ctx->bins_valid = false;  // the positions moved
-    return LBMIBM_OK;
+    return s;
+}
+
+// Times the interpolation forms on the current positions and velocity field,
+// writing to scratch arrays (the vertices do not move), and keeps the faster.
+template <typename T> lbmibm_status interp_autotune_impl(lbmibm_ctx *ctx) {
+    if (!ctx->bins_valid) {
+        lbmibm_status s = bin_impl<T>(ctx);
+        if (s != LBMIBM_OK) return s;
+    }
+    double *xo = nullptr, *xd = nullptr;
+    const size_t bytes = 3 * (size_t)ctx->nv * sizeof(double);
+    LBMIBM_HIP(ctx, hipMalloc((void **)&xo, bytes));
+    if (hipMalloc((void **)&xd, bytes) != hipSuccess) {
+        (void)hipFree(xo);
+        return LBMIBM_ERR_HIP;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    lbmibm_status s = LBMIBM_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) s = LBMIBM_ERR_HIP;
+    static const char *names[2] = {"tiles in LDS", "rows from memory"};
+    char line[160];
+    float best = 0.0f;
+    for (int f = 0; f < 2 && s == LBMIBM_OK; f++) {
+        s = interp_launch<T>(ctx, f, (const double *)ctx->xs, xo, (T *)nullptr, xd);  // warm-up
+        if (s != LBMIBM_OK) break;
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) { s = LBMIBM_ERR_HIP; break; }
+        for (int r = 0; r < 3 && s == LBMIBM_OK; r++)
+            s = interp_launch<T>(ctx, f, (const double *)ctx->xs, xo, (T *)nullptr, xd);
+        float ms = 0.0f;
+        if (s == LBMIBM_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                               hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            s = LBMIBM_ERR_HIP;
+        if (s != LBMIBM_OK) break;
+        ms /= 3.0f;
+        snprintf(line, sizeof line, "interp form %s: %.4f ms\n", names[f], ms);
+        ctx->tune_report += line;
+        if (f == 0 || ms < best) { best = ms; ctx->interp_form = f; }
+    }
+    if (s == LBMIBM_OK) {
+        snprintf(line, sizeof line, "interp: using %s\n", names[ctx->interp_form]);
+        ctx->tune_report += line;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(xo);
+    (void)hipFree(xd);
+    return s;
 }
 
 template <typename T> lbmibm_status spread_adj_impl(lbmibm_ctx *ctx) {
@@ -1238,6 +1391,22 @@ lbmibm_status ibm_spread(lbmibm_ctx *ctx) {
     lbmibm_status s = ctx->prec == LBMIBM_F64 ? spread_impl<double>(ctx) : spread_impl<float>(ctx);
     if (s == LBMIBM_OK) ctx->force_zero = (ctx->nv == 0);
     return s;
+}
+
+lbmibm_status ibm_autotune(lbmibm_ctx *ctx) {
+    ctx->interp_form = 0;
+    if (ctx->nv == 0) return LBMIBM_OK;
+    // LBMIBM_INTERP_FORM=0|1 fixes the form; LBMIBM_AUTOTUNE=0 keeps form 0
+    const char *fixed = getenv("LBMIBM_INTERP_FORM");
+    if (fixed && fixed[0]) {
+        if ((fixed[0] != '0' && fixed[0] != '1') || fixed[1]) return LBMIBM_ERR_ARG;
+        ctx->interp_form = fixed[0] - '0';
+        ctx->tune_report += "interp form fixed by LBMIBM_INTERP_FORM\n";
+        return LBMIBM_OK;
+    }
+    const char *env = getenv("LBMIBM_AUTOTUNE");
+    if (env && env[0] == '0') return LBMIBM_OK;
+    return ctx->prec == LBMIBM_F64 ? interp_autotune_impl<double>(ctx) : interp_autotune_impl<float>(ctx);
 }
 
 lbmibm_status ibm_interp(lbmibm_ctx *ctx) {

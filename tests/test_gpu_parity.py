@@ -129,12 +129,17 @@ def _random_velocity_field(o, seed, amp=1e-2):
     return o.um.copy(), o.vm.copy(), o.wm.copy()
 
 
+@pytest.mark.parametrize("form", [0, 1])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("rk,m,dx,shape", [(4, 1, 1.0, SHAPES[0]), (4, 2, 0.5, SHAPES[0]), (2, 1, 1.0, SHAPES[0]),
                                            (1, 3, 1.0, SHAPES[0]), (4, 1, 1.0, SHAPES[1]), (4, 1, 1.0, SHAPES[2])])
-def test_ibm_interp_parity(prec, rk, m, dx, shape):
+def test_ibm_interp_parity(prec, rk, m, dx, shape, form, monkeypatch):
+    """Both interpolation forms (LDS-staged tiles, rows read from memory; the
+    autotuner picks one) against the oracle, with cells across the periodic
+    and wall boundaries."""
+    monkeypatch.setenv("LBMIBM_INTERP_FORM", str(form))
     lx, ly, lz, py = shape
-    print(f"[interp parity] prec={prec} rk={rk} m={m} dx={dx} shape={shape}", flush=True)
+    print(f"[interp parity] prec={prec} rk={rk} m={m} dx={dx} shape={shape} form={form}", flush=True)
     cells = _cells_across_boundaries(lx * dx, ly * dx, lz * dx, radius=3.0 * dx, periodic_y=py)
     kw = dict(runge_kutta=rk, m=m, dx=dx, cr=0.7, periodic_y=py)
     o = oracle_of(lx, ly, lz, cells=cells, **kw)
@@ -353,6 +358,7 @@ def test_autotune_reports_a_choice_per_class():
     g = gpu_of(lx, ly, lz, "f32", cells=cells)
     rep = g.autotune_report()
     assert "class 0: using" in rep and "class 1: using" in rep, rep
+    assert "interp: using" in rep, rep
 
 
 def test_autotune_covers_the_large_cell_forms():
