@@ -114,13 +114,15 @@ const char   *lbmibm_last_error(const lbmibm_ctx *ctx);
  * shapes of the on-chip membrane solvers and the forms of the large-cell
  * solver (0 vectors in memory, 1 p in LDS, 2 register-resident) on the
  * cells' own matrices, and the two forms of the interpolation (0 tiles staged
- * in LDS, 1 one thread per vertex reading the field's rows) on the initial
- * positions, and keeps the fastest of each.  In the environment,
+ * in LDS, 1 one thread per vertex reading the field's rows) and of the
+ * spreading (0 source bins per tile, 1 home boxes summed by a gather) on the
+ * initial positions, and keeps the fastest of each.  In the environment,
  * LBMIBM_AUTOTUNE=0 skips it and uses the first shapes and forms 0;
  * LBMIBM_BICG_SHAPES="a,b[,c]" fixes the variants of the two on-chip classes
  * and the large-cell form (a form the largest cell does not fit falls back to
- * form 0); LBMIBM_INTERP_FORM=0|1 fixes the interpolation form.  This returns
- * the timings and choices as text, owned by ctx. */
+ * form 0); LBMIBM_INTERP_FORM=0|1 and LBMIBM_SPREAD_FORM=0|1 fix those forms
+ * (spread form 1 is refused on lattices much smaller than its boxes).  This
+ * returns the timings and choices as text, owned by ctx. */
 const char   *lbmibm_autotune_report(const lbmibm_ctx *ctx);
 /* Sizes: padded lattice nodes n, vertices, elements, cells. */
 void          lbmibm_sizes(const lbmibm_ctx *ctx, long long *n_padded, int *nv, int *nt, int *nc);

@@ -457,52 +457,73 @@ lbmibm_status lbmibm_adjoint_wall_shear(lbmibm_ctx *ctx, int iter0, int nsteps, 
     if (s == LBMIBM_OK) s = check_flags(ctx, nullptr);
     if (s == LBMIBM_OK && J && hipMemcpy(J, acc, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         s = LBMIBM_ERR_HIP;
-    // reverse sweep
+    // reverse sweep.  The recompute writes each step's state straight into
+    // the tape (the buffers the steps write are pointed at the tape slots)
+    // and the reverse steps point at the slots again: no copies.  A step's
+    // populations and positions come from the previous step's outputs, the
+    // first step's from the checkpoint.
+    struct Owned {  // restores the context's own buffers on every way out
+        lbmibm_ctx *c;
+        void *f0, *f1, *U, *F, *cf, *qv, *xs;
+        bool active;
+        void restore() {
+            if (!active) return;
+            c->f[0] = f0; c->f[1] = f1; c->U = U; c->F = F; c->cf = cf; c->qv = qv; c->xs = xs;
+            c->xs_out = nullptr;
+            c->bins_valid = false;
+            active = false;
+        }
+        ~Owned() { restore(); }
+    } owned{ctx, ctx->f[0], ctx->f[1], ctx->U, ctx->F, ctx->cf, ctx->qv, ctx->xs, true};
     if (s == LBMIBM_OK) s = adj_reset(ctx);
     for (int seg = nseg - 1; seg >= 0 && s == LBMIBM_OK; seg--) {
         const int k0 = seg * K, len = std::min(K, nsteps - k0);
-        s = d2d(ctx, ctx->f[ctx->cur], ck_pop(seg), popB);
-        if (s == LBMIBM_OK) s = d2d(ctx, ctx->xs, ck_x(seg), xB);
-        if (s == LBMIBM_OK) s = positions_from_soa(ctx);
-        // recompute the segment, recording each step's inputs
+        auto popk = [&](int k) -> void * { return k == 0 ? (void *)ck_pop(seg) : (void *)tp(k, T_POP); };
+        auto xk = [&](int k) -> double * { return k == 0 ? (double *)ck_x(seg) : (double *)tp(k, T_X); };
+        ctx->xs = xk(0);
+        s = positions_from_soa(ctx);
+        // recompute the segment
         for (int k = 0; k < len && s == LBMIBM_OK; k++) {
             const int iter = iter0 + k0 + k + 1;
-            s = d2d(ctx, tp(k, T_POP), ctx->f[ctx->cur], popB);
-            if (s == LBMIBM_OK) s = d2d(ctx, tp(k, T_X), ctx->xs, xB);
-            if (s == LBMIBM_OK) s = set_step(ctx, iter, false);
+            ctx->f[ctx->cur] = popk(k);
+            ctx->f[1 - ctx->cur] = k + 1 < len ? popk(k + 1) : owned.f0;  // the last output is not needed
+            ctx->xs = xk(k);
+            ctx->xs_out = k + 1 < len ? xk(k + 1) : (double *)owned.xs;
+            ctx->bins_valid = false;
+            ctx->cf = tp(k, T_CF); ctx->qv = tp(k, T_Q); ctx->F = tp(k, T_F); ctx->U = tp(k, T_U);
+            s = set_step(ctx, iter, false);
             if (s == LBMIBM_OK && ctx->nc > 0) s = membrane_forces(ctx);
             if (s == LBMIBM_OK && ctx->nc > 0) s = ibm_spread(ctx);
-            if (s == LBMIBM_OK) s = d2d(ctx, tp(k, T_CF), ctx->cf, vB);
-            if (s == LBMIBM_OK && ctx->nc > 0) s = d2d(ctx, tp(k, T_Q), ctx->qv, vB);
-            if (s == LBMIBM_OK) s = d2d(ctx, tp(k, T_F), ctx->F, fldB);
             if (s == LBMIBM_OK) s = lattice_step(ctx);
-            if (s == LBMIBM_OK) s = d2d(ctx, tp(k, T_U), ctx->U, fldB);
             if (s == LBMIBM_OK && ctx->nc > 0) s = ibm_interp(ctx);
         }
+        ctx->xs_out = nullptr;
         for (int k = len - 1; k >= 0 && s == LBMIBM_OK; k--) {
             const int iter = iter0 + k0 + k + 1;
             // advection: positions before it and the velocity it read
-            s = d2d(ctx, ctx->xs, tp(k, T_X), xB);
-            if (s == LBMIBM_OK) s = positions_from_soa(ctx);
-            if (s == LBMIBM_OK) s = d2d(ctx, ctx->U, tp(k, T_U), fldB);
+            ctx->xs = xk(k);
+            s = positions_from_soa(ctx);
+            ctx->U = tp(k, T_U);
             if (s == LBMIBM_OK && ctx->nc > 0) s = ibm_interp_adjoint(ctx);
             if (s == LBMIBM_OK && k0 + k >= nsteps - window) s = wall_seed(ctx, coef);
             // lattice step from its pre-step populations and force
-            if (s == LBMIBM_OK) s = d2d(ctx, ctx->f[ctx->cur], tp(k, T_POP), popB);
-            if (s == LBMIBM_OK) s = d2d(ctx, ctx->F, tp(k, T_F), fldB);
+            ctx->f[ctx->cur] = popk(k);
+            ctx->F = tp(k, T_F);
             if (s == LBMIBM_OK) {
                 ctx->force_zero = ctx->nc == 0;
                 s = adj_lattice(ctx, iter);
             }
             if (s == LBMIBM_OK && ctx->nc > 0) {
-                s = d2d(ctx, ctx->cf, tp(k, T_CF), vB);
-                if (s == LBMIBM_OK) s = ibm_spread_adjoint(ctx);
-                if (s == LBMIBM_OK) s = d2d(ctx, ctx->qv, tp(k, T_Q), vB);
+                ctx->cf = tp(k, T_CF);
+                s = ibm_spread_adjoint(ctx);
+                ctx->qv = tp(k, T_Q);
                 if (s == LBMIBM_OK) s = membrane_forces_adjoint(ctx, true);
             }
         }
         if (s == LBMIBM_OK) s = check_flags(ctx, nullptr);
     }
+    // the context's own buffers back (the tape is freed below)
+    owned.restore();
     // gradients out, forward end state back
     if (s == LBMIBM_OK && (gx || gy || gz)) {
         std::vector<double> g(3 * (size_t)ctx->np);

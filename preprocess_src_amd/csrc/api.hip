@@ -209,7 +209,7 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair,
                     ctx->srec, ctx->onchip_cells, ctx->Hb, ctx->adj_f[0], ctx->adj_f[1], ctx->adj_u, ctx->adj_F,
                     ctx->adj_x, ctx->adj_cf, ctx->grad_H, ctx->grad_gs, ctx->adj_lam, ctx->adj_dsv,
-                    ctx->adj_corner, ctx->adj_scratch, ctx->adj_cscale, ctx->adj_gpart, ctx->adj_xmax, ctx->adj_rec};
+                    ctx->adj_corner, ctx->adj_scratch, ctx->adj_cscale, ctx->adj_gpart, ctx->adj_xmax, ctx->adj_rec, ctx->sbox, ctx->sb_cand};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);

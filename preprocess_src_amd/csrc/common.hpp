@@ -89,6 +89,7 @@ struct lbmibm_ctx {
     void *adj_scratch = nullptr;      // [3][Nv] (T) discarded solver epilogue output
     double *adj_cscale = nullptr;     // [nc] power-of-two scale of each cell's adjoint right-hand side
     double *adj_gpart = nullptr;      // [Nv] per-vertex terms of the force-scale gradient
+    double *xs_out = nullptr;         // when set, the interpolation writes the new positions here (adjoint tape)
     unsigned *adj_xmax = nullptr;     // [1] max over vertices of sum_s |kb_s| (float bits), advection adjoint scale
     void *adj_rec = nullptr;          // [4 Nv] stage records of the advection adjoint
 
@@ -112,6 +113,9 @@ struct lbmibm_ctx {
     int bicg_n[3] = {0, 0, 0}, bicg_maxv[3] = {0, 0, 0};
     int bicg_var[3] = {0, 0, 0};  // shape variant of the on-chip classes, form of class 2 (autotuned)
     int interp_form = 0;          // 0: tiles staged in LDS, 1: one thread per vertex, rows from memory (autotuned)
+    int spread_form = 0;          // 0: source bins per tile, 1: home boxes + gather (autotuned)
+    void *sbox = nullptr;         // [ntiles][3][SB_N] staging boxes of spread form 1
+    int *sb_cand = nullptr;       // [(nx+ny+nz)*4] box entries per coordinate (spread form 1)
     int big_lds = 0;              // dynamic LDS of the class-2 forms that use LDS (the whole CU)
     int *onchip_cells = nullptr;  // cells of classes 0 and 1
     int n_onchip = 0;

@@ -428,12 +428,12 @@ template <> struct FixedPoint<float> {
     __device__ __forceinline__ FixedPoint(float f0, float f1, float f2, double scale) {
         f[0] = (double)f0 * scale; f[1] = (double)f1 * scale; f[2] = (double)f2 * scale;
     }
-    __device__ __forceinline__ void add(unsigned long long *acc, int li, float w) const {
+    __device__ __forceinline__ void add(unsigned long long *acc, int li, float w, int stride = ACC_N) const {
         const double wd = (double)w;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const double t = fma(wd, f[d], 6755399441055744.0);  // + 1.5 * 2^52
-            atomicAdd(&acc[d * ACC_N + li], (unsigned long long)__double_as_longlong(t));
+            atomicAdd(&acc[d * stride + li], (unsigned long long)__double_as_longlong(t));
         }
     }
     __device__ __forceinline__ static long long value(unsigned long long a) {
@@ -446,9 +446,9 @@ template <> struct FixedPoint<double> {
     __device__ __forceinline__ FixedPoint(double f0, double f1, double f2, double s) : scale(s) {
         f[0] = f0; f[1] = f1; f[2] = f2;
     }
-    __device__ __forceinline__ void add(unsigned long long *acc, int li, double w) const {
+    __device__ __forceinline__ void add(unsigned long long *acc, int li, double w, int stride = ACC_N) const {
 #pragma unroll
-        for (int d = 0; d < 3; d++) atomicAdd(&acc[d * ACC_N + li], (unsigned long long)llrint(w * f[d] * scale));
+        for (int d = 0; d < 3; d++) atomicAdd(&acc[d * stride + li], (unsigned long long)llrint(w * f[d] * scale));
     }
     __device__ __forceinline__ static long long value(unsigned long long a) { return (long long)a; }
 };
@@ -586,6 +586,122 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const SpreadRec<T> *__rest
         for (int d = 0; d < 3; d++)
             F[d * N + nidx] = (T)((double)FixedPoint<T>::value(acc[d * ACC_N + li]) * inv_scale);
     }
+}
+
+// Spreading, form 1.  One work group per tile takes its home vertices only
+// and adds each one's whole 4x4x4 stencil into an LDS box of the tile plus
+// the 3 nodes above it on each axis (its stencils cannot leave that box), in
+// the fixed point of form 0, with no per-node masks; the box is then written
+// to a staging array, and k_spread_gather sums for each lattice node the box
+// entries that map to it.  Form 0 visits a vertex once per tile its stencil
+// touches (2.2 tiles on average with 16x8x8 tiles), each visit walking the 64
+// nodes under masks.  Lane l of wave w takes column NW l + w of the bin read
+// as WG columns, so that the lanes of a wave hold vertices far apart.
+constexpr int SB_X = TILE_X + 3, SB_Y = TILE_Y + 3, SB_Z = TILE_Z + 3, SB_N = SB_X * SB_Y * SB_Z;
+constexpr int SPREAD_BOX_WG = 512;
+constexpr int SB_MAXC = 4;  // box entries per lattice coordinate and axis (k_spread_gather)
+
+template <typename T>
+__global__ __launch_bounds__(SPREAD_BOX_WG) void k_spread_box(const SpreadRec<T> *__restrict__ rec,
+                                                              const short4 *__restrict__ node0,
+                                                              const int *__restrict__ tile_count,
+                                                              const int *__restrict__ tile_off, T *__restrict__ S,
+                                                              IbmGeom g, TileGeom tg,
+                                                              const unsigned *__restrict__ absmax,
+                                                              const int *__restrict__ order) {
+    __shared__ unsigned long long acc[3 * SB_N];
+    const int tile = order[blockIdx.x];
+    const int cnt = tile_count[tile], off = tile_off[tile];
+    if (cnt == 0) return;  // whole work group leaves together; the gather skips empty tiles
+    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+    const int ox = tx * TILE_X, oy = ty * TILE_Y, oz = tz * TILE_Z;
+    for (int k = threadIdx.x; k < 3 * SB_N; k += blockDim.x) acc[k] = 0ull;
+    // a box node takes at most one term per home vertex, each at most
+    // |f|max / (8 dx^3): the power of two keeps the sums below 2^kBits
+    const double fmax = (double)__uint_as_float(*absmax);
+    double scale = 1.0;
+    if (fmax > 0.0) {
+        const double bound = (double)cnt * fmax / (8.0 * g.dx * g.dx * g.dx);
+        scale = ldexp(1.0, FixedPoint<T>::kBits - 1 - ilogb(bound));
+    }
+    const T w64 = (T)(1.0 / (64.0 * g.dx * g.dx * g.dx));
+    __syncthreads();
+    constexpr int NW = SPREAD_BOX_WG / 64;
+    const int R = (cnt + SPREAD_BOX_WG - 1) / SPREAD_BOX_WG;
+    const int col = NW * (threadIdx.x & 63) + (threadIdx.x >> 6);
+    for (int r = 0; r < R; r++) {
+        const int i = col * R + r;
+        if (i >= cnt) break;
+        const short4 n0 = node0[off + i];
+        const SpreadRec<T> sr = rec[off + i];
+        // box coordinates of the first stencil node (x, z wrapped at binning;
+        // y raw: with walls it may lie below the lattice in the first tile row)
+        const int bx = n0.x - ox, bz = n0.z - oz;
+        int y1 = n0.y - 3;
+        if (!g.walls) wrap_index(y1, g.ny);
+        const int by = y1 - oy;
+        if (bx < 0 || bx >= TILE_X || bz < 0 || bz >= TILE_Z || by < -3 || by >= TILE_Y) continue;  // not a valid home
+        T wx[4], wy[4], wz[4];
+        stencil_weights<T>(sr.t[0], wx);
+        stencil_weights<T>(sr.t[1], wy);
+        stencil_weights<T>(sr.t[2], wz);
+        const FixedPoint<T> fp(sr.f[0], sr.f[1], sr.f[2], scale);
+#pragma unroll
+        for (int kz = 0; kz < 4; kz++) {
+#pragma unroll
+            for (int ky = 0; ky < 4; ky++) {
+                if (by + ky < 0) continue;  // below the wall
+                const T wyz = wy[ky] * wz[kz] * w64;
+                const int row = bx + SB_X * ((by + ky) + SB_Y * (bz + kz));
+#pragma unroll
+                for (int kx = 0; kx < 4; kx++) fp.add(acc, row + kx, wx[kx] * wyz, SB_N);
+            }
+        }
+    }
+    __syncthreads();
+    const double inv_scale = 1.0 / scale;
+    T *St = S + (size_t)tile * 3 * SB_N;
+    for (int k = threadIdx.x; k < SB_N; k += blockDim.x)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            St[d * SB_N + k] = (T)((double)FixedPoint<T>::value(acc[d * SB_N + k]) * inv_scale);
+}
+
+// Sums, per lattice node, the box entries of k_spread_box that map to it, in
+// a fixed order (deterministic).  cx/cy/cz list for each coordinate of an axis
+// up to SB_MAXC entries (tile index along the axis) * 32 + box coordinate,
+// padded with -1 (built once on the host).
+template <typename T>
+__global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, const int *__restrict__ tile_count,
+                                                       const int *__restrict__ cx, const int *__restrict__ cy,
+                                                       const int *__restrict__ cz, T *__restrict__ F, IbmGeom g,
+                                                       TileGeom tg) {
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((size_t)g.nx * g.ny));
+    T a0 = T(0), a1 = T(0), a2 = T(0);
+    for (int iz = 0; iz < SB_MAXC; iz++) {
+        const int ez = cz[SB_MAXC * z + iz];
+        if (ez < 0) break;
+        for (int iy = 0; iy < SB_MAXC; iy++) {
+            const int ey = cy[SB_MAXC * y + iy];
+            if (ey < 0) break;
+            for (int ix = 0; ix < SB_MAXC; ix++) {
+                const int ex = cx[SB_MAXC * x + ix];
+                if (ex < 0) break;
+                const int t = (ex >> 5) + tg.ntx * ((ey >> 5) + tg.nty * (ez >> 5));
+                if (tile_count[t] == 0) continue;
+                const size_t b = (size_t)t * 3 * SB_N + (ex & 31) + SB_X * ((ey & 31) + SB_Y * (ez & 31));
+                a0 += S[b];
+                a1 += S[b + SB_N];
+                a2 += S[b + 2 * SB_N];
+            }
+        }
+    }
+    F[n] = a0;
+    F[N + n] = a1;
+    F[2 * N + n] = a2;
 }
 
 // ---- interpolation + advection ------------------------------------------------
@@ -870,6 +986,11 @@ __device__ __forceinline__ void stencil_d(double xtr, double h, int &first, W w[
 }
 
 // Stencil of a position as stencil3 (spacing dx) with weight derivatives.
+// a[k] for a runtime k by selects (a runtime index would put a in scratch)
+template <typename T> __device__ __forceinline__ T sel4(const T (&a)[4], int k) {
+    return k == 0 ? a[0] : k == 1 ? a[1] : k == 2 ? a[2] : a[3];
+}
+
 template <typename W> struct StencilD {
     int x1, y1, z1;
     W wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
@@ -906,7 +1027,37 @@ template <typename W> struct StencilD {
             }
         }
     }
+    // the 16 nodes of z-slice kz only (same products as each)
+    template <typename Fn> __device__ __forceinline__ void slice(const IbmGeom &g, int kz, Fn f) const {
+        int iz = z1 + kz;
+        if (!wrap_index(iz, g.nz)) return;
+        const W wzk = sel4(wz, kz), dzk = sel4(dz, kz);
+#pragma unroll
+        for (int ky = 0; ky < 4; ky++) {
+            int iy = y1 + ky;
+            if (g.walls) {
+                if (iy < 0 || iy >= g.ny) continue;
+            } else if (!wrap_index(iy, g.ny)) continue;
+#pragma unroll
+            for (int kx = 0; kx < 4; kx++) {
+                int ix = x1 + kx;
+                if (!wrap_index(ix, g.nx)) continue;
+                const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
+                const W w = wx[kx] * wy[ky] * wzk;
+                const W gx = dx[kx] * wy[ky] * wzk, gy = wx[kx] * dy[ky] * wzk, gz = wx[kx] * wy[ky] * dzk;
+                f(n, w, gx, gy, gz);
+            }
+        }
+    }
 };
+
+// Sum over the 4 lanes of an aligned quad; every lane gets the same bits
+// ((a + b) + (c + d) in each, addition being commutative).
+template <typename T> __device__ __forceinline__ T quad_sum(T v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    return v;
+}
 
 // Adjoint of ibm_st1 at the current positions and nodal forces: given Fa (the
 // adjoint of the lattice force), cfa = sum D Fa is the adjoint of the nodal
@@ -941,30 +1092,30 @@ __global__ __launch_bounds__(256) void k_spread_adj(const double *__restrict__ x
 // ua += phi^3(X_s) kb_s and X_s-bar = sum (u . kb_s) grad phi^3, which feeds
 // the stage before it (ibm.cu:430-517: X2 = X + h/2 k1, X3 = X + h/2 k2,
 // X4 = X + h k3, X' = X + h (k1 + 2 k2 + 2 k3 + k4)/6, h = M dt).
-// Two kernels.  k_advect_adj_vert, one thread per vertex in bin order, does
+// Two kernels.  k_advect_adj_vert, four lanes per vertex in bin order, does
 // the gathers (the stage positions, X-bar) and writes each stage's position
 // and kb_s to a record; k_advect_adj_scatter, one work group per tile over
 // its home vertices, adds the velocity adjoint of the records.
-
-// a[k] for a runtime k by selects (a runtime index would put a in scratch)
-template <typename T> __device__ __forceinline__ T sel4(const T (&a)[4], int k) {
-    return k == 0 ? a[0] : k == 1 ? a[1] : k == 2 ? a[2] : a[3];
-}
 
 template <typename T> struct AdvRec {
     double p[3];  // stage position
     T k[3];       // kb_s
 };
 
+// Four lanes per vertex (bin order): lane z of the quad sums the z-th slice
+// of every stencil, and quad_sum completes the sums in all four lanes, so
+// the four follow the same stage chain.  One lane per vertex left most of the
+// machine idle on the small cell counts the gradient is taken on.
 template <typename T>
 __global__ __launch_bounds__(256) void k_advect_adj_vert(const double *__restrict__ xs, const T *__restrict__ U,
                                                          double *__restrict__ xa, const int *__restrict__ home_list,
                                                          int nv, IbmGeom g, double h, AdvRec<T> *__restrict__ rec,
                                                          unsigned *__restrict__ kbmax) {
     __shared__ float shm[4];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // position in bin order
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> 2, sub = t & 3;  // vertex in bin order, stencil z-slice
     float km = 0.0f;
-    if (i < nv) {
+    if (i < nv) {  // the same for the four lanes of a quad
         const int j = home_list[i];
         const size_t N = (size_t)g.nx * g.ny * g.nz;
         const T q64 = T(1.0 / 64.0);
@@ -972,12 +1123,12 @@ __global__ __launch_bounds__(256) void k_advect_adj_vert(const double *__restric
         auto interp = [&](const double p[3], double k[3]) {
             const StencilD<T> st(g, p[0], p[1], p[2]);
             T u0 = T(0), u1 = T(0), u2 = T(0);
-            st.each(g, [&](size_t n, T w, T, T, T, int, int, int) {
+            st.slice(g, sub, [&](size_t n, T w, T, T, T) {
                 u0 += w * U[n]; u1 += w * U[N + n]; u2 += w * U[2 * N + n];
             });
-            k[0] = (double)(u0 * q64);
-            k[1] = (double)(u1 * q64);
-            k[2] = (double)(u2 * q64);
+            k[0] = (double)(quad_sum(u0) * q64);
+            k[1] = (double)(quad_sum(u1) * q64);
+            k[2] = (double)(quad_sum(u2) * q64);
         };
         double P[4][3], k[3];
 #pragma unroll
@@ -1003,27 +1154,32 @@ __global__ __launch_bounds__(256) void k_advect_adj_vert(const double *__restric
 #pragma unroll
         for (int s = 3; s >= 0; s--) {  // kb_s is final here (stage s+1 came first)
             const T k0 = (T)kb[s][0], k1 = (T)kb[s][1], k2 = (T)kb[s][2];
-            AdvRec<T> r;
-            r.p[0] = P[s][0]; r.p[1] = P[s][1]; r.p[2] = P[s][2];
-            r.k[0] = k0; r.k[1] = k1; r.k[2] = k2;
-            rec[4 * (size_t)i + s] = r;
+            if (sub == s) {  // the quad's lanes write one record each
+                AdvRec<T> r;
+                r.p[0] = P[s][0]; r.p[1] = P[s][1]; r.p[2] = P[s][2];
+                r.k[0] = k0; r.k[1] = k1; r.k[2] = k2;
+                rec[4 * (size_t)i + s] = r;
+            }
             km += fmaxf(fabsf((float)k0), fmaxf(fabsf((float)k1), fabsf((float)k2)));
             const StencilD<T> st(g, P[s][0], P[s][1], P[s][2]);
             T g0 = T(0), g1 = T(0), g2 = T(0);
-            st.each(g, [&](size_t n, T, T gx, T gy, T gz, int, int, int) {
+            st.slice(g, sub, [&](size_t n, T, T gx, T gy, T gz) {
                 const T sdot = U[n] * k0 + U[N + n] * k1 + U[2 * N + n] * k2;
                 g0 += sdot * gx; g1 += sdot * gy; g2 += sdot * gz;
             });
-            const double xs_[3] = {(double)(g0 * q64), (double)(g1 * q64), (double)(g2 * q64)};
+            const double xs_[3] = {(double)(quad_sum(g0) * q64), (double)(quad_sum(g1) * q64),
+                                   (double)(quad_sum(g2) * q64)};
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 acc[d] += xs_[d];
                 if (s > 0) kb[s - 1][d] += back[s] * xs_[d];
             }
         }
-        xa[j] = acc[0];
-        xa[nv + j] = acc[1];
-        xa[2 * (size_t)nv + j] = acc[2];
+        if (sub == 0) {
+            xa[j] = acc[0];
+            xa[nv + j] = acc[1];
+            xa[2 * (size_t)nv + j] = acc[2];
+        }
         km *= 1.0001f;  // margin for the float rounding
     }
 #pragma unroll
@@ -1225,6 +1381,28 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
     return LBMIBM_OK;
 }
 
+// The spreading of the binned records into F with the given form.
+template <typename T> lbmibm_status spread_launch(lbmibm_ctx *ctx, int form) {
+    TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
+    const IbmGeom g = geom_of(ctx);
+    if (form == 1) {
+        hipLaunchKernelGGL(k_spread_box<T>, dim3(ctx->ntiles), dim3(SPREAD_BOX_WG), 0, ctx->stream,
+                           (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off,
+                           (T *)ctx->sbox, g, tg, ctx->absmax, ctx->order_interp);
+        LBMIBM_LAUNCHED(ctx, "k_spread_box");
+        hipLaunchKernelGGL(k_spread_gather<T>, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (const T *)ctx->sbox, ctx->tile_count, ctx->sb_cand, ctx->sb_cand + SB_MAXC * ctx->nx,
+                           ctx->sb_cand + SB_MAXC * (ctx->nx + ctx->ny), (T *)ctx->F, g, tg);
+        LBMIBM_LAUNCHED(ctx, "k_spread_gather");
+        return LBMIBM_OK;
+    }
+    hipLaunchKernelGGL(k_spread_tiles<T>, dim3(ctx->ntiles), dim3(256), 0, ctx->stream,
+                       (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off, (T *)ctx->F, g,
+                       tg, ctx->absmax, ctx->order_spread);
+    LBMIBM_LAUNCHED(ctx, "k_spread_tiles");
+    return LBMIBM_OK;
+}
+
 template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return lattice_fill(ctx, ctx->F, 3 * (size_t)ctx->N, 0.0, false);
     if (!ctx->bins_valid) {
@@ -1237,12 +1415,7 @@ template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
                            (const T *)ctx->cf, ctx->nv, (SpreadRec<T> *)ctx->srec, ctx->absmax);
         LBMIBM_LAUNCHED(ctx, "k_bin_forces");
     }
-    TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
-    hipLaunchKernelGGL(k_spread_tiles<T>, dim3(ctx->ntiles), dim3(256), 0, ctx->stream,
-                       (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off, (T *)ctx->F,
-                       geom_of(ctx), tg, ctx->absmax, ctx->order_spread);
-    LBMIBM_LAUNCHED(ctx, "k_spread_tiles");
-    return LBMIBM_OK;
+    return spread_launch<T>(ctx, ctx->spread_form);
 }
 
 // One interpolation + advection of all vertices with the given form, from
@@ -1275,7 +1448,8 @@ template <typename T> lbmibm_status interp_impl(lbmibm_ctx *ctx) {
         lbmibm_status s = bin_impl<T>(ctx);
         if (s != LBMIBM_OK) return s;
     }
-    lbmibm_status s = interp_launch<T>(ctx, ctx->interp_form, (const double *)ctx->xs, (double *)ctx->xs,
+    double *xo = ctx->xs_out ? ctx->xs_out : (double *)ctx->xs;
+    lbmibm_status s = interp_launch<T>(ctx, ctx->interp_form, (const double *)ctx->xs, xo,
                                        ctx->p.store_macros ? (T *)ctx->uvw : (T *)nullptr, (double *)ctx->xd);
     ctx->bins_valid = false;  // the positions moved
     return s;
@@ -1346,7 +1520,8 @@ template <typename T> lbmibm_status advect_adj_impl(lbmibm_ctx *ctx) {
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->adj_xmax, 1);
     LBMIBM_LAUNCHED(ctx, "k_zero_int");
     const IbmGeom g = geom_of(ctx);
-    hipLaunchKernelGGL(k_advect_adj_vert<T>, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_advect_adj_vert<T>, dim3((unsigned)((4 * (size_t)ctx->nv + 255) / 256)), dim3(256), 0,
+                       ctx->stream,
                        (const double *)ctx->xs, (const T *)ctx->U, ctx->adj_x, ctx->home_list, ctx->nv, g,
                        ctx->p.m * ctx->p.dt, (AdvRec<T> *)ctx->adj_rec, ctx->adj_xmax);
     LBMIBM_LAUNCHED(ctx, "k_advect_adj_vert");
@@ -1393,20 +1568,133 @@ lbmibm_status ibm_spread(lbmibm_ctx *ctx) {
     return s;
 }
 
+// Host tables of spread form 1: for each coordinate of each axis, the
+// (tile along the axis, box coordinate) pairs whose box node lands on it,
+// tiles ascending; false when a coordinate has more than SB_MAXC (lattices
+// much smaller than a box) or the tile counts do not fit the packing.
+static bool spread_box_tables(const lbmibm_ctx *ctx, std::vector<int> &out) {
+    const int n[3] = {ctx->nx, ctx->ny, ctx->nz}, nt[3] = {ctx->ntx, ctx->nty, ctx->ntz};
+    const int tile[3] = {TILE_X, TILE_Y, TILE_Z}, R[3] = {SB_X, SB_Y, SB_Z};
+    const bool wrap[3] = {true, ctx->p.periodic_y != 0, true};
+    out.assign((size_t)SB_MAXC * (n[0] + n[1] + n[2]), -1);
+    size_t base = 0;
+    for (int a = 0; a < 3; a++) {
+        if (nt[a] >= (1 << 25)) return false;
+        for (int t = 0; t < nt[a]; t++)
+            for (int k = 0; k < R[a]; k++) {
+                int cc = t * tile[a] + k;
+                if (cc >= n[a]) {
+                    if (!wrap[a]) continue;
+                    cc %= n[a];
+                }
+                int *e = &out[base + (size_t)SB_MAXC * cc];
+                int slot = 0;
+                while (slot < SB_MAXC && e[slot] >= 0) slot++;
+                if (slot == SB_MAXC) return false;
+                e[slot] = t * 32 + k;
+            }
+        base += (size_t)SB_MAXC * n[a];
+    }
+    return true;
+}
+
+// Allocates the staging boxes and uploads the tables of spread form 1;
+// *ok = false where the form does not apply.
+static lbmibm_status spread_box_setup(lbmibm_ctx *ctx, bool *ok) {
+    *ok = false;
+    std::vector<int> tab;
+    if (!spread_box_tables(ctx, tab)) return LBMIBM_OK;
+    if (!ctx->sb_cand) {
+        lbmibm_status s = dev_alloc(ctx, (void **)&ctx->sb_cand, tab.size() * sizeof(int));
+        if (s != LBMIBM_OK) return s;
+        LBMIBM_HIP(ctx, hipMemcpy(ctx->sb_cand, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (!ctx->sbox) {
+        lbmibm_status s = dev_alloc(ctx, &ctx->sbox, (size_t)ctx->ntiles * 3 * SB_N * ctx->szT);
+        if (s != LBMIBM_OK) return s;
+    }
+    *ok = true;
+    return LBMIBM_OK;
+}
+
+// Times the spreading forms on the binned initial positions (the force
+// field they write is overwritten by the first step) and keeps the faster.
+template <typename T> lbmibm_status spread_autotune_impl(lbmibm_ctx *ctx) {
+    if (!ctx->bins_valid) {
+        lbmibm_status s = bin_impl<T>(ctx);
+        if (s != LBMIBM_OK) return s;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    lbmibm_status s = LBMIBM_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) s = LBMIBM_ERR_HIP;
+    static const char *names[2] = {"source bins per tile", "home boxes + gather"};
+    char line[160];
+    float best = 0.0f;
+    for (int f = 0; f < 2 && s == LBMIBM_OK; f++) {
+        s = spread_launch<T>(ctx, f);  // warm-up
+        if (s != LBMIBM_OK) break;
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) { s = LBMIBM_ERR_HIP; break; }
+        for (int r = 0; r < 3 && s == LBMIBM_OK; r++) s = spread_launch<T>(ctx, f);
+        float ms = 0.0f;
+        if (s == LBMIBM_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                               hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            s = LBMIBM_ERR_HIP;
+        if (s != LBMIBM_OK) break;
+        ms /= 3.0f;
+        snprintf(line, sizeof line, "spread form %s: %.4f ms\n", names[f], ms);
+        ctx->tune_report += line;
+        if (f == 0 || ms < best) { best = ms; ctx->spread_form = f; }
+    }
+    if (s == LBMIBM_OK) {
+        snprintf(line, sizeof line, "spread: using %s\n", names[ctx->spread_form]);
+        ctx->tune_report += line;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return s;
+}
+
+// a form fixed in the environment: "0" or "1" into *form, -1 when unset
+static lbmibm_status env_form(const char *name, int *form) {
+    *form = -1;
+    const char *v = getenv(name);
+    if (!v || !v[0]) return LBMIBM_OK;
+    if ((v[0] != '0' && v[0] != '1') || v[1]) return LBMIBM_ERR_ARG;
+    *form = v[0] - '0';
+    return LBMIBM_OK;
+}
+
 lbmibm_status ibm_autotune(lbmibm_ctx *ctx) {
     ctx->interp_form = 0;
+    ctx->spread_form = 0;
     if (ctx->nv == 0) return LBMIBM_OK;
-    // LBMIBM_INTERP_FORM=0|1 fixes the form; LBMIBM_AUTOTUNE=0 keeps form 0
-    const char *fixed = getenv("LBMIBM_INTERP_FORM");
-    if (fixed && fixed[0]) {
-        if ((fixed[0] != '0' && fixed[0] != '1') || fixed[1]) return LBMIBM_ERR_ARG;
-        ctx->interp_form = fixed[0] - '0';
-        ctx->tune_report += "interp form fixed by LBMIBM_INTERP_FORM\n";
-        return LBMIBM_OK;
-    }
+    // LBMIBM_INTERP_FORM / LBMIBM_SPREAD_FORM = 0|1 fix a form;
+    // LBMIBM_AUTOTUNE=0 keeps forms 0
+    int fi = -1, fs = -1;
+    lbmibm_status s = env_form("LBMIBM_INTERP_FORM", &fi);
+    if (s == LBMIBM_OK) s = env_form("LBMIBM_SPREAD_FORM", &fs);
+    if (s != LBMIBM_OK) return s;
+    bool box_ok = false;
+    s = spread_box_setup(ctx, &box_ok);
+    if (s != LBMIBM_OK) return s;
+    if (fs == 1 && !box_ok) return LBMIBM_ERR_ARG;  // the lattice is too small for the boxes
     const char *env = getenv("LBMIBM_AUTOTUNE");
-    if (env && env[0] == '0') return LBMIBM_OK;
-    return ctx->prec == LBMIBM_F64 ? interp_autotune_impl<double>(ctx) : interp_autotune_impl<float>(ctx);
+    const bool tune = !(env && env[0] == '0');
+    if (fi >= 0) {
+        ctx->interp_form = fi;
+        ctx->tune_report += "interp form fixed by LBMIBM_INTERP_FORM\n";
+    } else if (tune) {
+        s = ctx->prec == LBMIBM_F64 ? interp_autotune_impl<double>(ctx) : interp_autotune_impl<float>(ctx);
+        if (s != LBMIBM_OK) return s;
+    }
+    if (fs >= 0) {
+        ctx->spread_form = fs;
+        ctx->tune_report += "spread form fixed by LBMIBM_SPREAD_FORM\n";
+    } else if (tune && box_ok) {
+        s = ctx->prec == LBMIBM_F64 ? spread_autotune_impl<double>(ctx) : spread_autotune_impl<float>(ctx);
+        if (s != LBMIBM_OK) return s;
+    }
+    return LBMIBM_OK;
 }
 
 lbmibm_status ibm_interp(lbmibm_ctx *ctx) {

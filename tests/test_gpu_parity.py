@@ -81,10 +81,15 @@ def _set_positions(o, g, pos):
 SHAPES = [(16, 14, 12, False), (33, 17, 9, False), (18, 9, 10, True)]
 
 
+@pytest.mark.parametrize("form", [0, 1])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("lx,ly,lz,py", SHAPES)
-def test_ibm_spread_parity(prec, lx, ly, lz, py):
-    print(f"[spread parity] prec={prec} shape={lx}x{ly}x{lz} periodic_y={py}", flush=True)
+def test_ibm_spread_parity(prec, lx, ly, lz, py, form, monkeypatch):
+    """Both spreading forms (source bins per tile; home boxes + gather, the
+    autotuner picks one) against the oracle, on lattices with partial tiles,
+    boxes that wrap onto themselves, walls and periodic y."""
+    monkeypatch.setenv("LBMIBM_SPREAD_FORM", str(form))
+    print(f"[spread parity] prec={prec} shape={lx}x{ly}x{lz} periodic_y={py} form={form}", flush=True)
     cells = _cells_across_boundaries(lx, ly, lz, periodic_y=py)
     o = oracle_of(lx, ly, lz, cells=cells, periodic_y=py)
     g = gpu_of(lx, ly, lz, prec, cells=cells, periodic_y=py)
@@ -312,8 +317,11 @@ def test_bicgstab_not_converged_is_reported():
     assert e.value.code == 4
 
 
-def test_spread_is_bitwise_reproducible():
-    """Fixed-point LDS accumulation: repeated spreads give identical bits."""
+@pytest.mark.parametrize("form", [0, 1])
+def test_spread_is_bitwise_reproducible(form, monkeypatch):
+    """Fixed-point LDS accumulation (and, in form 1, a fixed summation order
+    over the boxes): repeated spreads give identical bits."""
+    monkeypatch.setenv("LBMIBM_SPREAD_FORM", str(form))
     lx, ly, lz = 24, 20, 24
     cells = small_cells(lx, ly, lz, nc=3, radius=4.0, levels=[3, 2, 3], seed=51)
     g = gpu_of(lx, ly, lz, "f32", cells=cells)
@@ -358,7 +366,7 @@ def test_autotune_reports_a_choice_per_class():
     g = gpu_of(lx, ly, lz, "f32", cells=cells)
     rep = g.autotune_report()
     assert "class 0: using" in rep and "class 1: using" in rep, rep
-    assert "interp: using" in rep, rep
+    assert "interp: using" in rep and "spread: using" in rep, rep
 
 
 def test_autotune_covers_the_large_cell_forms():
