@@ -409,7 +409,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
                 for (int sl = 0; sl < n; sl++) {
                     const int e = le[6 * (size_t)j + sl];
                     const int *ev = cells->ele + 3 * (size_t)e;
-                    cslot[3 * (size_t)e + (ev[0] == j ? 0 : (ev[1] == j ? 1 : 2))] = 6 * j + sl;
+                    cslot[3 * (size_t)e + (ev[0] == j ? 0 : (ev[1] == j ? 1 : 2))] = sl * nv + j;
                 }
                 int m = 0;
                 for (int k = 0; k <= n && k < 7; k++) {
@@ -425,7 +425,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
                     }
                     if (m >= 6 || sa < 0 || sb < 0) return fail(LBMIBM_ERR_TOPOLOGY);
                     nbr16[(size_t)m * nv + j] = (unsigned short)(l - base);
-                    epair[6 * (size_t)j + m] = (unsigned char)(sa | (sb << 4));
+                    epair[(size_t)m * nv + j] = (unsigned char)(sa | (sb << 4));
                     m++;
                 }
                 for (; m < 6; m++) nbr16[(size_t)m * nv + j] = (unsigned short)(j - base);

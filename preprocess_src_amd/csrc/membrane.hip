@@ -129,8 +129,9 @@ template <typename T> struct ElemTension {
 };
 
 // The corner records go to the slot of the corner's vertex: cslot[3e+en] =
-// 6 j + s for vertex j = ele[3e+en] with e its s-th incident element, so the
-// per-vertex pass reads each vertex's records contiguously.
+// s nv + j for vertex j = ele[3e+en] with e its s-th incident element
+// (slot-major, so that the per-vertex pass reads slot s of a wave's vertices
+// contiguously).
 template <typename T>
 __global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, const int *__restrict__ ele,
                                               const double *__restrict__ nref, T *__restrict__ tt,
@@ -168,10 +169,12 @@ __global__ __launch_bounds__(256) void k_elem(const double *__restrict__ xd, con
 // right-hand side of each vertex from the corner terms of its incident
 // elements, summed in the reference's order (le list), plus the q2f ring area
 // (stress.cu:415-438: a third of the incident triangle areas, rg/6 each)
-// times the cell's force scale.  corner[6j+s] holds the record of vertex j
-// in its s-th incident element; epair[6j+m] holds the two slots s_A < s_B
-// whose elements share the edge to the m-th off-diagonal column (4 bits each,
-// 0xff for an unused column).
+// times the cell's force scale.  corner[s nv + j] holds the record of vertex
+// j in its s-th incident element; epair[m nv + j] holds the two slots
+// s_A < s_B whose elements share the edge to the m-th off-diagonal column (4
+// bits each, 0xff for an unused column).  rg/12 and rg/6 are formed as 2 and
+// 4 times rg/24 (exact: scaling by a power of two commutes with rounding),
+// one division per slot.
 template <typename T>
 __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const unsigned char *__restrict__ epair,
                                               const typename Corner<T>::type *__restrict__ corner,
@@ -187,16 +190,15 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
     }
     if (jj >= jend) return;
     const int n = nln[jj];
-    double rgs[6], tb[3] = {0.0, 0.0, 0.0}, jac = 0.0, ds = 0.0;
+    double r24[6], tb[3] = {0.0, 0.0, 0.0}, jac = 0.0, ds = 0.0;
 #pragma unroll
     for (int s = 0; s < 6; s++) {
-        rgs[s] = 0.0;
+        r24[s] = 0.0;
         if (s < n) {
-            const typename Corner<T>::type cv = corner[6 * (size_t)jj + s];
-            const double gab = (double)cv.w;
-            rgs[s] = gab;
-            jac += gab / 12.0;
-            ds += gab / 2.0 / 3.0;
+            const typename Corner<T>::type cv = corner[(size_t)s * nv + jj];
+            r24[s] = (double)cv.w / 24.0;
+            jac += 2.0 * r24[s];  // rg / 12
+            ds += 4.0 * r24[s];   // rg / 2 / 3
             tb[0] += (double)cv.x;
             tb[1] += (double)cv.y;
             tb[2] += (double)cv.z;
@@ -204,14 +206,14 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
     }
 #pragma unroll
     for (int m = 0; m < 6; m++) {
-        const unsigned ep = epair[6 * (size_t)jj + m];
+        const unsigned ep = epair[(size_t)m * nv + jj];
         double ra = 0.0, rb = 0.0;
 #pragma unroll
         for (int s = 0; s < 6; s++) {
-            if ((int)(ep & 15u) == s) ra = rgs[s];
-            if ((int)(ep >> 4) == s) rb = rgs[s];
+            if ((int)(ep & 15u) == s) ra = r24[s];
+            if ((int)(ep >> 4) == s) rb = r24[s];
         }
-        coef[(size_t)m * nv + jj] = ep == 0xffu ? T(0) : (T)((ra / 24.0 + rb / 24.0) / jac);
+        coef[(size_t)m * nv + jj] = ep == 0xffu ? T(0) : (T)((ra + rb) / jac);
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
@@ -1360,9 +1362,9 @@ __global__ __launch_bounds__(256) void k_adj_load(const int *__restrict__ nln,
     const int n = nln[jj];
     double jac = 0.0, ds = 0.0;
     for (int s = 0; s < n; s++) {
-        const double gab = (double)corner[6 * (size_t)jj + s].w;
-        jac += gab / 12.0;
-        ds += gab / 2.0 / 3.0;
+        const double r24 = (double)corner[(size_t)s * nv + jj].w / 24.0;
+        jac += 2.0 * r24;  // rg / 12, as k_vert
+        ds += 4.0 * r24;   // rg / 2 / 3
     }
     const double d = (double)dsv[jj];
     double sb = 0.0;
@@ -1506,7 +1508,7 @@ __global__ __launch_bounds__(256) void k_vert_adj(const int *__restrict__ nln, c
     const int n = nln[jj];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int s = 0; s < n; s++) {
-        const double *c = cadj + 4 * (6 * (size_t)jj + s);
+        const double *c = cadj + 4 * ((size_t)s * nv + jj);
         s0 += c[0]; s1 += c[1]; s2 += c[2];
     }
     xa[jj] += s0;
