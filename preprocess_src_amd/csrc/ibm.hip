@@ -680,23 +680,42 @@ __global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, 
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((size_t)g.nx * g.ny));
+    // all table entries first, then per (z, y) entry the tile counts and box
+    // values of every x entry, so that independent loads are in flight
+    // together; the sums keep the order (z, y, x entry)
+    int ex[SB_MAXC], ey[SB_MAXC], ez[SB_MAXC];
+#pragma unroll
+    for (int i = 0; i < SB_MAXC; i++) {
+        ex[i] = cx[SB_MAXC * x + i];
+        ey[i] = cy[SB_MAXC * y + i];
+        ez[i] = cz[SB_MAXC * z + i];
+    }
     T a0 = T(0), a1 = T(0), a2 = T(0);
+#pragma unroll
     for (int iz = 0; iz < SB_MAXC; iz++) {
-        const int ez = cz[SB_MAXC * z + iz];
-        if (ez < 0) break;
+        if (ez[iz] < 0) break;
+#pragma unroll
         for (int iy = 0; iy < SB_MAXC; iy++) {
-            const int ey = cy[SB_MAXC * y + iy];
-            if (ey < 0) break;
+            if (ey[iy] < 0) break;
+            const int tyz = tg.ntx * ((ey[iy] >> 5) + tg.nty * (ez[iz] >> 5));
+            const int byz = SB_X * ((ey[iy] & 31) + SB_Y * (ez[iz] & 31));
+            int cnt[SB_MAXC];
+#pragma unroll
+            for (int ix = 0; ix < SB_MAXC; ix++) cnt[ix] = ex[ix] >= 0 ? tile_count[(ex[ix] >> 5) + tyz] : 0;
+            T v0[SB_MAXC], v1[SB_MAXC], v2[SB_MAXC];
+#pragma unroll
             for (int ix = 0; ix < SB_MAXC; ix++) {
-                const int ex = cx[SB_MAXC * x + ix];
-                if (ex < 0) break;
-                const int t = (ex >> 5) + tg.ntx * ((ey >> 5) + tg.nty * (ez >> 5));
-                if (tile_count[t] == 0) continue;
-                const size_t b = (size_t)t * 3 * SB_N + (ex & 31) + SB_X * ((ey & 31) + SB_Y * (ez & 31));
-                a0 += S[b];
-                a1 += S[b + SB_N];
-                a2 += S[b + 2 * SB_N];
+                v0[ix] = v1[ix] = v2[ix] = T(0);
+                if (cnt[ix] > 0) {
+                    const size_t b = (size_t)((ex[ix] >> 5) + tyz) * 3 * SB_N + (ex[ix] & 31) + byz;
+                    v0[ix] = S[b];
+                    v1[ix] = S[b + SB_N];
+                    v2[ix] = S[b + 2 * SB_N];
+                }
             }
+#pragma unroll
+            for (int ix = 0; ix < SB_MAXC; ix++)
+                if (cnt[ix] > 0) { a0 += v0[ix]; a1 += v1[ix]; a2 += v2[ix]; }
         }
     }
     F[n] = a0;
@@ -716,7 +735,7 @@ constexpr int RLO = 1;  // nodes of the box below the tile
 // Out-of-line and not unrolled: it is the rare path, and inlining it into
 // every RK stage would multiply the kernel's code size.
 template <typename T>
-__device__ __noinline__ void interp_global(const T *__restrict__ U, size_t N, const IbmGeom &g, double x,
+__device__ __forceinline__ void interp_global(const T *__restrict__ U, size_t N, const IbmGeom &g, double x,
                                            double y, double z, bool unit, T &ur, T &vr, T &wr) {
     int x1, y1, z1;
     T wx[4], wy[4], wz[4];
