@@ -270,9 +270,10 @@ constexpr int BICG0_SHAPES[BICG_NVAR][2] = {{4, 256}, {2, 512}, {1, 1024}};
 constexpr int BICG1_SHAPES[BICG_NVAR][2] = {{6, 512}, {3, 1024}, {4, 768}};
 
 // On-chip BiCGSTAB for one cell of at most NT*KMAX vertices: every thread
-// keeps the rows of its vertices (x, r, p, v, t and the 6 off-diagonal
-// coefficients) in registers; p, r and b are mirrored in LDS ([V][4]) for
-// the neighbour gathers of the matrix-vector products.  matrix_solver.cu:
+// keeps the rows of its vertices (r, p, v, t and the 6 off-diagonal
+// coefficients) in registers; p (then r) and b are mirrored in LDS ([V][4])
+// for the neighbour gathers of the matrix-vector products, and x is kept in
+// LDS ([V][4]).  matrix_solver.cu:
 // 279-377 in the same update order; (r, r^) of the next iteration is fused
 // with the residual reduction; r^ = b because x starts at 0 (stress.cu:139).
 template <typename T, int KMAX, int NT>
@@ -287,11 +288,15 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
     WgReduce red{sh, 0};
     const int c = cells[blockIdx.x];
     const int v0 = voff[c], V = voff[c + 1] - v0;
-    T *Ps = (T *)dyn;              // [V][4]
-    T *Rs = Ps + 4 * (size_t)V;    // [V][4]
-    T *Bs = Rs + 4 * (size_t)V;    // [V][4] right-hand side = r^
+    // p for v = A p, then r for t = A r in the same array (the reduction
+    // after v = A p ends the reads of p; the one after t = A r those of r);
+    // x lives in LDS too, each vertex touched by its own thread only, which
+    // keeps the largest shape within its registers
+    T *Ps = (T *)dyn;              // [V][4] p, then r
+    T *Bs = Ps + 4 * (size_t)V;    // [V][4] right-hand side = r^
+    T *Xs = Bs + 4 * (size_t)V;    // [V][4] x
     const int tid = threadIdx.x;
-    T x[KMAX][3], r[KMAX][3], p[KMAX][3], v[KMAX][3], t[KMAX][3], cf[KMAX][6];
+    T r[KMAX][3], p[KMAX][3], v[KMAX][3], t[KMAX][3], cf[KMAX][6];
     unsigned nb[KMAX][3];  // two 16-bit column ids per register
     double acc[2] = {0.0, 0.0};
 #pragma unroll
@@ -308,10 +313,15 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const T bb = on ? bv[3 * g + d] : T(0);
-            x[k][d] = T(0); r[k][d] = bb; p[k][d] = bb; v[k][d] = T(0); t[k][d] = T(0);
+            r[k][d] = bb; p[k][d] = bb; v[k][d] = T(0); t[k][d] = T(0);
             acc[0] += (double)bb * (double)bb;
         }
-        if (on) { lds_store3(Ps, j, p[k]); lds_store3(Bs, j, p[k]); }
+        if (on) {
+            const T zero[3] = {T(0), T(0), T(0)};
+            lds_store3(Ps, j, p[k]);
+            lds_store3(Bs, j, p[k]);
+            lds_store3(Xs, j, zero);
+        }
     }
     red.sum<1>(*(double(*)[1])acc);  // its barrier also orders the LDS writes above
     const double bb = acc[0];
@@ -372,7 +382,7 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
             const int j = tid + k * NT;
 #pragma unroll
             for (int d = 0; d < 3; d++) r[k][d] = r[k][d] - al * v[k][d];
-            if (j < V) lds_store3(Rs, j, r[k]);
+            if (j < V) lds_store3(Ps, j, r[k]);
         }
         __syncthreads();
         // t = A r ; omeg = (r, t) / (t, t)
@@ -381,7 +391,7 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
         for (int k = 0; k < KMAX; k++) {
             const int j = tid + k * NT;
             if (j < V) {
-                spmv(Rs, k, r[k], t[k]);
+                spmv(Ps, k, r[k], t[k]);
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
                     acc[0] += (double)r[k][d] * (double)t[k][d];
@@ -397,12 +407,12 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
             const int j = tid + k * NT;
-            T b3[3] = {T(0), T(0), T(0)};
-            if (j < V) lds_load3(Bs, j, b3);
+            T b3[3] = {T(0), T(0), T(0)}, xv[3] = {T(0), T(0), T(0)};
+            if (j < V) { lds_load3(Bs, j, b3); lds_load3(Xs, j, xv); }
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const T ri = r[k][d];
-                x[k][d] = x[k][d] + al * p[k][d] + om * ri;
+                xv[d] = xv[d] + al * p[k][d] + om * ri;
                 const T rn = ri - om * t[k][d];
                 r[k][d] = rn;
                 if (j < V) {
@@ -410,6 +420,7 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
                     acc[1] += (double)rn * (double)b3[d];
                 }
             }
+            if (j < V) lds_store3(Xs, j, xv);
         }
         red.sum<2>(acc);
         rr_old = rr;
@@ -419,7 +430,11 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
         const int j = tid + k * NT;
-        if (j < V) out.put((size_t)v0 + j, x[k]);
+        if (j < V) {
+            T xv[3];
+            lds_load3(Xs, j, xv);
+            out.put((size_t)v0 + j, xv);
+        }
     }
     if (tid == 0) {
         iters[c] = conv ? it + 1 : it;
