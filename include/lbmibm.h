@@ -182,6 +182,40 @@ lbmibm_status lbmibm_download_volume(lbmibm_ctx *ctx, double *volume0, double *v
 /* Per-cell force scale (per-cell Gs / reference Gs), [nc]; NULL resets to 1. */
 lbmibm_status lbmibm_set_cell_force_scale(lbmibm_ctx *ctx, const double *scale);
 
+/* ---- the reference's binary files (etc.cu) ------------------------------
+ * Byte layouts of output_data / restart_file: native-endian int32 and FP64
+ * arrays one after another, no headers.  The cell sections assume one mesh
+ * for all cells (cel->vertex, cel->element): a context whose cells differ
+ * gets LBMIBM_ERR_ARG.  Fields this path does not model are written as the
+ * reference initialises them and ignored on reading: the VOF indicator
+ * vf, vf2 (0, init1.cu:396-411).  dm (the density sum) and the vertex
+ * velocities u, v, w are kept only with store_macros (0 otherwise); the
+ * per-cell rx, ry, rz, which no reference code assigns, are written as the
+ * mean of each cell's vertex positions. */
+/* bc codes of the padded lattice (cdo_init, init1.cu:164-178); the
+ * reference's numeric values were in the absent def.h. */
+typedef enum lbmibm_bc { LBMIBM_BC_COMP = 0, LBMIBM_BC_BUFF = 1, LBMIBM_BC_WALL = 2 } lbmibm_bc;
+lbmibm_status lbmibm_bc_mask(const lbmibm_ctx *ctx, int *bc);                 /* ltc->bcH [n]    */
+/* restart.bin (output_data, etc.cu:738-789): iter, fnum, fn [n*19], dm, um,
+ * vm, wm, vf, vf2 [n], then x, y, z [Nv], rx, ry, rz [nc], u, v, w [Nv],
+ * qi [nc*mn] (cell c at c*3V, mn = expand2pow2(3V), init2.cu:286), tp [2Nt]. */
+lbmibm_status lbmibm_write_restart(lbmibm_ctx *ctx, const char *path, int iter, int fnum);
+/* restart_file (etc.cu:164-323): reads restart.bin into the context
+ * (populations, um..wm, dm with store_macros, positions with fem->x from them,
+ * u, v, w, qi, tp) and returns iter and fnum, both 0 when fnum > 100
+ * (etc.cu:205).  The file size must match this lattice and these cells. */
+lbmibm_status lbmibm_read_restart(lbmibm_ctx *ctx, const char *path, int *iter, int *fnum);
+/* prof%04d.bin (etc.cu:547-560): iter, dm, um, vm, wm, vf, vf2 [n], bc [n]. */
+lbmibm_status lbmibm_write_prof(lbmibm_ctx *ctx, const char *path, int iter);
+/* cell%04d.bin (etc.cu:565-611): n, vertex, element, x, y, z, u, v, w [Nv],
+ * qi [3Nv], tp [2Nt], ele [3Nt] (global vertex ids). */
+lbmibm_status lbmibm_write_cell(lbmibm_ctx *ctx, const char *path);
+/* output_data's file part (etc.cu:544-789): dir/prof%04d.bin when
+ * fnum % 50 == 0, dir/cell/cell%04d.bin (the directory must exist), then
+ * dir/restart%04d.bin when fnum % 100 == 0 and dir/restart.bin otherwise;
+ * increments *fnum. */
+lbmibm_status lbmibm_output_data(lbmibm_ctx *ctx, const char *dir, int iter, int *fnum);
+
 /* ---- reverse mode (no reference counterpart) --------------------------
  * Adjoint (vector-Jacobian product) entry points for the main.cu loop body
  * with M = 1, RUNGE_KUTTA = 4, BR = 0, no volume correction and no FILEOUTPUT

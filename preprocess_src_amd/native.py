@@ -87,6 +87,12 @@ SIGNATURES = {
     "lbmibm_download_membrane": (C.c_int, [C.c_void_p] + [c_double_p] * 5 + [c_int_p]),
     "lbmibm_download_volume": (C.c_int, [C.c_void_p] + [c_double_p] * 4),
     "lbmibm_set_cell_force_scale": (C.c_int, [C.c_void_p, c_double_p]),
+    "lbmibm_bc_mask": (C.c_int, [C.c_void_p, c_int_p]),
+    "lbmibm_write_restart": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int]),
+    "lbmibm_read_restart": (C.c_int, [C.c_void_p, C.c_char_p, c_int_p, c_int_p]),
+    "lbmibm_write_prof": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "lbmibm_write_cell": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "lbmibm_output_data": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, c_int_p]),
     "lbmibm_set_body_force_field": (C.c_int, [C.c_void_p] + [c_double_p] * 3),
     "lbmibm_adjoint_reset": (C.c_int, [C.c_void_p]),
     "lbmibm_adjoint_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),

@@ -11,6 +11,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import os
+
 import numpy as np
 
 from . import native as N
@@ -192,6 +194,37 @@ class Simulation:
     def set_cell_force_scale(self, scale=None):
         s = None if scale is None else np.ascontiguousarray(scale, np.float64)
         self._ck(self.L.lbmibm_set_cell_force_scale(self.ctx, N.dp(s)))
+
+    # -- the reference's binary files (etc.cu) -------------------------------
+    def bc_mask(self):
+        """bc codes of the padded lattice (0 COMP, 1 BUFF, 2 WALL)."""
+        bc = np.zeros(self.nxp * self.nyp * self.nzp, np.int32)
+        self._ck(self.L.lbmibm_bc_mask(self.ctx, N.ip(bc)))
+        return bc
+
+    def write_restart(self, path, iter, fnum):
+        """restart.bin of output_data (etc.cu:738-789)."""
+        self._ck(self.L.lbmibm_write_restart(self.ctx, os.fsencode(path), int(iter), int(fnum)))
+
+    def read_restart(self, path):
+        """restart_file (etc.cu:164-323); returns (iter, fnum)."""
+        it, fn = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        self._ck(self.L.lbmibm_read_restart(self.ctx, os.fsencode(path), N.ip(it), N.ip(fn)))
+        return int(it[0]), int(fn[0])
+
+    def write_prof(self, path, iter):
+        """prof%04d.bin (etc.cu:547-560)."""
+        self._ck(self.L.lbmibm_write_prof(self.ctx, os.fsencode(path), int(iter)))
+
+    def write_cell(self, path):
+        """cell%04d.bin (etc.cu:565-611)."""
+        self._ck(self.L.lbmibm_write_cell(self.ctx, os.fsencode(path)))
+
+    def output_data(self, directory, iter, fnum):
+        """output_data's files with the reference's names; returns fnum + 1."""
+        f = np.array([fnum], np.int32)
+        self._ck(self.L.lbmibm_output_data(self.ctx, os.fsencode(directory), int(iter), N.ip(f)))
+        return int(f[0])
 
     def comp_mask(self):
         """COMP nodes of the padded reference lattice (init1.cu:164-178)."""
