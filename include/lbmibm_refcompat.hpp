@@ -11,8 +11,11 @@
  *   void stress(process*, domain*, cell*, fem*, lattice*)        stress.cu:580
  *   int  volume_correction(domain*, cell*, fem*)                 volume_correction.cu:453
  *   void volume_init(domain*, cell*)                             volume_correction.cu:427
+ *   void output_data(process*, domain*, lattice*, cell*, fem*, Wall*, Wall*)   etc.cu:449
+ *   void restart_file(process*, domain*, lattice*, cell*, fem*, Wall*, Wall*)  etc.cu:164
  * The adaptor's functions are templates over those struct types, so they
- * compile against whatever def.h defines; they read only prc->iter.  The
+ * compile against whatever def.h defines; they read only prc->iter (and
+ * prc->fnum, cdo->restart for the two file functions).  The
  * device state lives in one lbmibm_ctx created by lbmibm_ref::attach() after
  * the reference's own host initialisation (cdo_init, flu_init, cel_init,
  * fem_init), which replaces mem_init.  lbmibm_ref::pull_lattice() and
@@ -103,6 +106,26 @@ inline int volume_correction(Domain *, Cell *, Fem *) {
     int iterations = 0;
     lbmibm_ref::check(lbmibm_volume_correction(lbmibm_ref::context(), &iterations), "volume_correction");
     return iterations;
+}
+
+// The binary files of output_data into ./result with the reference's names
+// (prof%04d.bin every 50th file, cell/cell%04d.bin, restart.bin or
+// restart%04d.bin every 100th); increments prc->fnum.  The text files
+// (config.dat, center.dat, surface_volume.dat) are not written.
+template <class Process, class Domain, class Lattice, class Cell, class Fem, class Wall>
+inline void output_data(Process *prc, Domain *, Lattice *, Cell *, Fem *, Wall *, Wall *) {
+    std::printf("Result file %04d writing...\n", prc->fnum);
+    lbmibm_ref::check(lbmibm_output_data(lbmibm_ref::context(), "./result", prc->iter, &prc->fnum), "output_data");
+}
+
+// With cdo->restart set, ./result/restart.bin back into the context, and
+// prc->iter, prc->fnum from it (both 0 when fnum > 100, etc.cu:205).
+template <class Process, class Domain, class Lattice, class Cell, class Fem, class Wall>
+inline void restart_file(Process *prc, Domain *cdo, Lattice *, Cell *, Fem *, Wall *, Wall *) {
+    if (cdo->restart == 0) return;
+    std::printf("Initializing restart process\n");
+    lbmibm_ref::check(lbmibm_read_restart(lbmibm_ref::context(), "./result/restart.bin", &prc->iter, &prc->fnum),
+                      "restart_file");
 }
 
 #endif

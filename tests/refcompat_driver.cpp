@@ -10,11 +10,12 @@
 
 #include "lbmibm_refcompat.hpp"
 
-struct process { int iter; };
-struct domain {};
+struct process { int iter, fnum; };
+struct domain { int restart; };
 struct lattice {};
 struct cell {};
 struct fem {};
+struct Wall {};
 
 int main(int argc, char **argv) {
     lbmibm_params p;
@@ -27,12 +28,14 @@ int main(int argc, char **argv) {
     p.p = 5e-6;
     lbmibm_cells none{};
     lbmibm_ref::attach(p, none);
-    process prc{0};
-    domain cdo;
+    process prc{0, 0};
+    domain cdo{0};
     lattice ltc;
     cell cel;
     fem fm;
+    Wall wall;
     const int M = p.m;
+    output_data(&prc, &cdo, &ltc, &cel, &fm, &wall, &wall);  // main.cu:65, into ./result
     while (prc.iter < 20) {
         prc.iter++;
         if ((prc.iter - 1) % M == 0) stress(&prc, &cdo, &cel, &fm, &ltc);
@@ -40,7 +43,13 @@ int main(int argc, char **argv) {
         lbm_d3q19(&prc, &cdo, &ltc);
         if (prc.iter % M == 0) ibm_st2(&cdo, &ltc, &cel, &fm);
         if (prc.iter % M == 0) (void)volume_correction(&cdo, &cel, &fm);
+        if (prc.iter % 10 == 0) output_data(&prc, &cdo, &ltc, &cel, &fm, &wall, &wall);
     }
+    // restart.bin was last written at iter 20 with file number 2
+    process back{0, 0};
+    cdo.restart = 1;
+    restart_file(&back, &cdo, &ltc, &cel, &fm, &wall, &wall);
+    if (back.iter != 20 || back.fnum != 2) return 4;
     long long n = 0;
     int nv = 0, nt = 0, nc = 0;
     lbmibm_sizes(lbmibm_ref::context(), &n, &nv, &nt, &nc);

@@ -28,6 +28,10 @@ def test_adaptor_compiles_links_and_runs_host_side(tmp_path):
 @pytest.mark.gpu
 def test_adaptor_drives_the_reference_loop(tmp_path):
     exe = _build(tmp_path)
-    r = subprocess.run([exe, "run"], capture_output=True, text=True, timeout=120)
+    work = tmp_path / "work"
+    os.makedirs(work / "result" / "cell")
+    r = subprocess.run([exe, "run"], capture_output=True, text=True, timeout=120, cwd=work)
     assert r.returncode == 0, r.stdout + r.stderr
     assert float(r.stdout.split()[-1]) > 0.0
+    # output_data's files, named as the reference names them
+    assert sorted(os.listdir(work / "result")) == ["cell", "prof0000.bin", "restart.bin", "restart0000.bin"]
