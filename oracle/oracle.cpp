@@ -976,6 +976,10 @@ int orc_step(const orc_params *p, orc_state *s, int iter, int *vol_iters) {
     orc_lbm_step(p, s, iter);
     if (iter % p->m == 0 && s->nc > 0) {
         orc_ibm_st2(p, s);
+        if (p->vof_period > 0) {
+            orc_thinc_wlic(p, s);
+            if (iter % p->vof_period == 0 && orc_front_tracking(p, s, nullptr, nullptr) != 0) return 5;
+        }
         if (p->volume_correction) {
             const int k = orc_volume_correction(p, s);
             if (k < 0) return 3;

@@ -54,6 +54,10 @@ typedef struct orc_params {
     int    neohookean;      /* 0: SKALAK, 1: NEOHOOKEAN (stress.cu:95-99)                  */
     int    fileoutput;      /* FILEOUTPUT period of the tp/qi accumulators                 */
     int    volume_correction; /* 1: run volume_correction in orc_step (main.cu:101)        */
+    int    vof_period;      /* K: front tracking every K steps, THINC/WLIC every M steps  */
+                            /* (main.cu:93-97); 0: no VOF indicator (vf stays 0)          */
+    int    sor_itrmax;      /* ITRMAX, RESIDUAL of the red-black SOR (matrix_solver.cu:   */
+    double sor_residual;    /* 188-215), the same constants as BiCGSTAB's in the reference */
 } orc_params;
 
 /* Pointer set equivalent to the reference's lattice/cell/fem structs
@@ -111,6 +115,40 @@ void orc_ibm_st1(const orc_params *p, orc_state *s);
  * field, advect by M*dt, wrap x and z; writes x,y,z,u,v,w and xd.
  * ibm.cu:372-694 */
 void orc_ibm_st2(const orc_params *p, orc_state *s);
+/* ---- VOF indicator ------------------------------------------------------
+ * Slab connectivity only (walls in y, CapsuleSHEARFLOW).  Arrays are padded
+ * [n]; s->vf must be set. */
+/* front_tracking_source1-3: G = sum over elements of delta * (inward unit
+ * normal) * area, spread from the element centroid over the floor-1..ceil+1
+ * node box, then the x/z halo contributions folded onto the COMP images and
+ * copied back.  fronttrack.cu:274-499 */
+void orc_ft_gradient(const orc_params *p, const orc_state *s, double *gx, double *gy, double *gz);
+/* front_tracking_source4 + jacobi_source: b = div G / (-6/dx^2) on COMP, 0
+ * elsewhere.  fronttrack.cu:502-556, 314-333 */
+void orc_ft_rhs(const orc_params *p, const int *bc, const double *gx, const double *gy, const double *gz,
+                double *b);
+/* One redblacksor call on the Jacobi-scaled front-tracking matrix
+ * (fronttrack.cu:27-238): x[n] holds the guess on entry and the iterate on
+ * exit; the expand2pow2(n) - n identity rows of the padding start at `pad0`.
+ * Returns 0 (converged), 2 (ITRMAX).  matrix_solver.cu:112-216 */
+int orc_ft_sor(const orc_params *p, const int *bc, const double *b, double *x, double pad0, int *iters);
+/* front_tracking host loop: gradient, rhs, SOR from x = cnt for cnt = 0, 1,
+ * ... (error on the 11th attempt); vf = x.  Returns 0 or 5.
+ * fronttrack.cu:628-724 */
+int orc_front_tracking(const orc_params *p, orc_state *s, int *iters, int *attempts);
+/* normal3d_st1/st2: the 2x2x2-cube difference gradient, averaged over the 8
+ * cubes around the node; written for 1 <= i < n-1 per axis.
+ * thincwlic.cu:12-106 */
+void orc_vof_normals(const orc_params *p, const double *vf, double *n1x, double *n1y, double *n1z);
+/* The THINC part of wlic1d4md_st1 for the upwind cell value C, orientation
+ * alpha = +-1, face velocity uc and isgn (1: upwind cell below the face).
+ * thincwlic.cu:187-197 */
+double orc_thinc_flux(double c, double alpha, double uc, int isgn, int m, double dt, double dx);
+/* thinc_wlic: three directional sweeps (normals, fluxes, COMP update, BUFF
+ * copy / WALL zero) on s->vf with the M-averaged velocity (um, vm, wm).
+ * thincwlic.cu:108-313, 316-372 */
+void orc_thinc_wlic(const orc_params *p, orc_state *s);
+
 /* Single-point helpers used by the delta-function tests. */
 double orc_delta_weight(double r_over_dx);                 /* (1+cos(pi r/2))/4     */
 
@@ -154,8 +192,10 @@ int  orc_fem_topology(int nc, const int *voff, const int *eoff,
                       double *nref, int *ec, double *c0, double *s0);
 
 /* ---- one main.cu iteration body with the M gating (main.cu:73-110) ---- */
-/* Returns 0, or the stress flag (1/2) or 3 when volume correction failed.
- * vol_iters may be NULL. */
+/* With vof_period > 0: thinc_wlic after ibm_st2 (iter%M==0) and
+ * front_tracking (iter%K==0) before the volume correction (main.cu:93-97).
+ * Returns 0, or the stress flag (1/2), 3 when volume correction failed, 5
+ * when front tracking failed.  vol_iters may be NULL. */
 int  orc_step(const orc_params *p, orc_state *s, int iter, int *vol_iters);
 
 /* Discrete Skalak energy sum_e A0_e (1/4)(I1^2 + 2 I1 - 2 I2 + C I2^2)

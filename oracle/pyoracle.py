@@ -32,6 +32,7 @@ class OrcParams(C.Structure):
         ("c", C.c_double), ("rgs", C.c_double), ("br", C.c_double),
         ("itrmax", C.c_int), ("residual", C.c_double), ("runge_kutta", C.c_int),
         ("neohookean", C.c_int), ("fileoutput", C.c_int), ("volume_correction", C.c_int),
+        ("vof_period", C.c_int), ("sor_itrmax", C.c_int), ("sor_residual", C.c_double),
     ]
 
 
@@ -96,6 +97,14 @@ def lib():
             "orc_skalak_energy": (C.c_double, [P, S]),
             "orc_bending_energy": (C.c_double, [P, S]),
             "orc_num_threads": (C.c_int, []),
+            "orc_ft_gradient": (None, [P, S, c_double_p, c_double_p, c_double_p]),
+            "orc_ft_rhs": (None, [P, c_int_p, c_double_p, c_double_p, c_double_p, c_double_p]),
+            "orc_ft_sor": (C.c_int, [P, c_int_p, c_double_p, c_double_p, C.c_double, c_int_p]),
+            "orc_front_tracking": (C.c_int, [P, S, c_int_p, c_int_p]),
+            "orc_vof_normals": (None, [P, c_double_p, c_double_p, c_double_p, c_double_p]),
+            "orc_thinc_flux": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                            C.c_double, C.c_double]),
+            "orc_thinc_wlic": (None, [P, S]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -146,7 +155,7 @@ class OracleSim:
                  p=0.0, e=(1.0, 0.0, 0.0), force_modulated=False, flowstart=0, freq=1.0,
                  b=0.0, m=1, c=100.0, rgs=1.0, br=0.0, itrmax=1000, residual=1e-12,
                  runge_kutta=4, neohookean=False, fileoutput=1000, volume_correction=True,
-                 periodic_y=False, cells=None):
+                 periodic_y=False, cells=None, vof_period=0, sor_itrmax=None, sor_residual=None):
         if df is None:
             df = (2.0 * tau - 1.0) / 6.0  # tau = (6 Df + 1)/2 (init1.cu:140)
         P = OrcParams()
@@ -162,6 +171,9 @@ class OracleSim:
         P.c, P.rgs, P.br = c, rgs, br
         P.itrmax, P.residual, P.runge_kutta = itrmax, residual, runge_kutta
         P.neohookean, P.fileoutput, P.volume_correction = int(neohookean), fileoutput, int(volume_correction)
+        P.vof_period = int(vof_period)
+        P.sor_itrmax = int(itrmax if sor_itrmax is None else sor_itrmax)
+        P.sor_residual = float(residual if sor_residual is None else sor_residual)
         self.P = P
         self.nx, self.ny, self.nz = P.nx, P.ny, P.nz
         n = P.nx * P.ny * P.nz
@@ -170,13 +182,13 @@ class OracleSim:
         self.bc = np.zeros(n, np.int32)
         L.orc_bc_init(C.byref(P), ip(self.bc))
         self.bufs = {k: np.zeros(n * Q) for k in ("fa", "fb")}
-        for k in ("dn", "un", "vn", "wn", "dm", "um", "vm", "wm", "fx", "fy", "fz"):
+        for k in ("dn", "un", "vn", "wn", "dm", "um", "vm", "wm", "fx", "fy", "fz", "vf"):
             setattr(self, k, np.zeros(n))
         S = OrcState()
         S.fn, S.fm = dp(self.bufs["fa"]), dp(self.bufs["fb"])
         for k in ("dn", "un", "vn", "wn", "dm", "um", "vm", "wm", "fx", "fy", "fz"):
             setattr(S, k, dp(getattr(self, k)))
-        S.vf = None
+        S.vf = dp(self.vf)
         S.bc = ip(self.bc)
         S.nc = 0
         self.S = S
@@ -294,6 +306,39 @@ class OracleSim:
         rc = lib().orc_step(C.byref(self.P), C.byref(self.S), it, C.byref(k))
         return rc, k.value
 
+    # -- VOF indicator ---------------------------------------------------------
+    def front_tracking(self):
+        """vof_init / front_tracking: returns (status, SOR iterations, attempts)."""
+        its, att = C.c_int(0), C.c_int(0)
+        rc = lib().orc_front_tracking(C.byref(self.P), C.byref(self.S), C.byref(its), C.byref(att))
+        return rc, its.value, att.value
+
+    def thinc_wlic(self):
+        lib().orc_thinc_wlic(C.byref(self.P), C.byref(self.S))
+
+    def ft_gradient(self):
+        g = [np.zeros(self.n) for _ in range(3)]
+        lib().orc_ft_gradient(C.byref(self.P), C.byref(self.S), *[dp(a) for a in g])
+        return g
+
+    def ft_rhs(self, gx, gy, gz):
+        b = np.zeros(self.n)
+        lib().orc_ft_rhs(C.byref(self.P), ip(self.bc), *[dp(np.ascontiguousarray(a, np.float64)) for a in (gx, gy, gz)],
+                         dp(b))
+        return b
+
+    def ft_sor(self, b, x0, pad0=0.0):
+        x = np.ascontiguousarray(x0, np.float64).copy()
+        its = C.c_int(0)
+        rc = lib().orc_ft_sor(C.byref(self.P), ip(self.bc), dp(np.ascontiguousarray(b, np.float64)), dp(x),
+                              float(pad0), C.byref(its))
+        return rc, x, its.value
+
+    def vof_normals(self, vf):
+        out = [np.zeros(self.n) for _ in range(3)]
+        lib().orc_vof_normals(C.byref(self.P), dp(np.ascontiguousarray(vf, np.float64)), *[dp(a) for a in out])
+        return out
+
     def set_body_force_field(self, gx=None, gy=None, gz=None):
         """Per-node body-force field (padded [n]) added to P e; None removes it."""
         if gx is None:
@@ -331,6 +376,10 @@ def bicgstab(ptr, index, value, b, x0=None, itrmax=1000, residual=1e-12):
     rc = lib().orc_bicgstab(ip(ptr), ip(index), dp(value), dp(x), dp(np.ascontiguousarray(b, np.float64)),
                             n, itrmax, residual, C.byref(its))
     return rc, x, its.value
+
+
+def thinc_flux(c, alpha, uc, isgn, m, dt, dx):
+    return lib().orc_thinc_flux(float(c), float(alpha), float(uc), int(isgn), int(m), float(dt), float(dx))
 
 
 def delta_weight(r):

@@ -26,7 +26,9 @@ def oracle_of(lx, ly, lz, cells=None, **kw):
                   runge_kutta=p["runge_kutta"], volume_correction=p["volume_correction"],
                   periodic_y=p["periodic_y"], force_modulated=p.get("force_modulated", False),
                   flowstart=p.get("flowstart", 0), freq=p.get("freq", 1.0),
-                  fileoutput=p.get("fileoutput", 0) or 1000, cells=cells)
+                  fileoutput=p.get("fileoutput", 0) or 1000, cells=cells, lambda1=p.get("lambda1", 1.0),
+                  vof_period=p.get("vof_period", 0), sor_itrmax=p.get("sor_itrmax"),
+                  sor_residual=p.get("sor_residual"))
     o.init_fluid()
     return o
 

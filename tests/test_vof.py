@@ -1,0 +1,277 @@
+"""VOF indicator: front tracking, its red-black SOR solve and the THINC/WLIC
+advection (fronttrack.cu, matrix_solver.cu:112-216, thincwlic.cu), and the
+viscosity it sets (lbm.cu:1131).
+
+CPU: the oracle's restatement is pinned by what the mathematics fixes -- the
+THINC flux is the integral of the tanh profile whose cell mean is the
+volume fraction (quadrature, root-finding), the normals of a linear field
+are its gradient, the conservative update conserves the COMP sum, a planar
+front reduces to the 1-D scheme, the spread of one triangle is its area
+vector with the cosine weights, the SOR iterate is the direct solution of
+the reference's Jacobi-scaled matrix (scipy), and the indicator of a sphere
+is 1 inside, 0 outside with the sphere's volume.  GPU: the library's
+front tracking and THINC/WLIC against the oracle, and whole steps with a
+viscosity contrast.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.integrate
+import scipy.optimize
+import scipy.sparse
+import scipy.sparse.linalg
+
+from oracle import pyoracle as PO
+from tests.pairs import Q, gpu_of, oracle_of, rel
+
+BETA = 2.0
+
+
+def _tanh_profile(c, alpha):
+    """H(x) = (1 + alpha tanh(beta (x - xc)))/2 on [0, 1] with mean c."""
+    mean = lambda xc: 0.5 + 0.5 * alpha / BETA * (math.log(math.cosh(BETA * (1 - xc)))
+                                                   - math.log(math.cosh(BETA * xc))) - c
+    xc = scipy.optimize.brentq(mean, -20.0, 20.0, xtol=1e-14)
+    return lambda x: 0.5 * (1.0 + alpha * math.tanh(BETA * (x - xc)))
+
+
+@pytest.mark.parametrize("c", [0.05, 0.3, 0.5, 0.77, 0.96])
+@pytest.mark.parametrize("alpha", [1.0, -1.0])
+@pytest.mark.parametrize("courant", [0.31, 0.04, -0.22])
+def test_thinc_flux_is_the_integral_of_the_tanh_profile(c, alpha, courant):
+    # flux through the face during M dt: the upwind cell's profile over the
+    # segment that crosses it (below the face for uc > 0, above for uc < 0)
+    dx, dt, m = 0.5, 0.25, 2
+    uc = courant * dx / (m * dt)
+    H = _tanh_profile(c, alpha)
+    if courant > 0:
+        exact = dx * scipy.integrate.quad(H, 1.0 - courant, 1.0, epsabs=1e-14)[0]
+        isgn = 1
+    else:
+        exact = -dx * scipy.integrate.quad(H, 0.0, -courant, epsabs=1e-14)[0]
+        isgn = 0
+    got = PO.thinc_flux(c, alpha, uc, isgn, m, dt, dx)
+    # expf / coshf of the reference carry float rounding
+    assert abs(got - exact) < 2e-6 * dx, (got, exact)
+
+
+LX, LY, LZ = 16, 12, 14
+
+
+def _oracle(**kw):
+    return oracle_of(LX, LY, LZ, **kw)
+
+
+def test_normals_of_a_linear_field_are_its_gradient():
+    o = _oracle(dx=0.5)
+    X, Y, Z = o.node_coords()
+    g = (0.3, -1.1, 0.45)
+    n1 = o.vof_normals(g[0] * X + g[1] * Y + g[2] * Z)
+    inner = np.zeros((o.nz, o.ny, o.nx), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    inner = inner.reshape(-1)
+    for d in range(3):
+        assert np.max(np.abs(n1[d][inner] - g[d])) < 1e-12
+
+
+def _periodic_fill(o, a):
+    """BUFF nodes take their image's value, WALL nodes 0."""
+    nx, ny, nz = o.nx, o.ny, o.nz
+    a = a.reshape(nz, ny, nx).copy()
+    ix = np.r_[np.arange(nx - 4, nx - 2), np.arange(2, nx - 2), np.arange(2, 4)]
+    iz = np.r_[np.arange(nz - 4, nz - 2), np.arange(2, nz - 2), np.arange(2, 4)]
+    a = a[iz][:, :, ix]
+    a = a.reshape(-1)
+    a[o.bc == 2] = 0.0
+    return a
+
+
+def _smooth_field(o, seed, lo=0.0, hi=1.0):
+    rng = np.random.default_rng(seed)
+    X, Y, Z = o.node_coords()
+    f = np.zeros(o.n)
+    for _ in range(3):
+        k = rng.integers(1, 3, size=3) * 2 * np.pi / np.array([LX, LY, LZ]) * o.P.dx ** 0
+        f += rng.uniform(0.5, 1.0) * np.sin(k[0] * X + rng.uniform(0, 6)) * np.cos(k[2] * Z + rng.uniform(0, 6)) \
+            * np.cos(np.pi * Y / LY)
+    f = (f - f.min()) / (f.max() - f.min())
+    return _periodic_fill(o, lo + (hi - lo) * f)
+
+
+def _velocity(o, seed, amp=0.08):
+    rng = np.random.default_rng(seed)
+    X, Y, Z = o.node_coords()
+    out = []
+    for d in range(3):
+        k = 2 * np.pi * rng.integers(1, 3, size=2) / np.array([LX, LZ])
+        out.append(_periodic_fill(o, amp * (np.sin(k[0] * X + d) * np.cos(k[1] * Z) + 0.5 * rng.uniform(-1, 1))
+                                  * np.cos(np.pi * Y / (LY + 1))))
+    return out
+
+
+def test_thinc_conserves_the_comp_sum():
+    o = _oracle(m=2)
+    o.vf[:] = _smooth_field(o, 3, 0.0, 1.0)
+    o.um[:], o.vm[:], o.wm[:] = _velocity(o, 4)
+    comp = o.bc == 0
+    before = o.vf[comp].sum()
+    o.thinc_wlic()
+    assert abs(o.vf[comp].sum() - before) < 1e-12 * before
+    assert not np.array_equal(o.vf, _smooth_field(o, 3, 0.0, 1.0))
+
+
+def test_thinc_of_a_planar_front_is_the_one_dimensional_scheme():
+    o = _oracle(m=1)
+    X, Y, Z = o.node_coords()
+    x = X.reshape(o.nz, o.ny, o.nx)[0, 0]
+    # a periodic front profile in x only, with plateaus at 0 and 1
+    prof = 0.5 * (np.tanh(1.3 * np.sin(2 * np.pi * (x + 0.37) / LX) * 3) + 1.0)
+    vf = np.broadcast_to(prof, (o.nz, o.ny, o.nx)).reshape(-1).copy()
+    vf[o.bc == 2] = 0.0
+    o.vf[:] = vf
+    u0 = 0.21
+    o.um[:] = u0
+    o.vm[:] = 0.0
+    o.wm[:] = 0.0
+    o.thinc_wlic()
+    # 1-D: faces i (between i-1 and i), central-difference normal, alpha from
+    # the upwind cell's neighbours, THINC weight 1 where the normal is not 0
+    dx, dt = o.P.dx, o.P.dt
+    flux = np.zeros(o.nx)
+    for i in range(2, o.nx - 1):
+        ii = i - 1  # u0 > 0
+        c = prof[ii]
+        up = c * u0 * dt
+        nn = abs(prof[ii + 1] - prof[ii - 1]) / (2 * dx)
+        if c >= 1 - 1e-6 or c <= 1e-6 or nn <= 1e-6:
+            flux[i] = up
+        else:
+            alpha = 1.0 if prof[ii - 1] <= prof[ii + 1] else -1.0
+            flux[i] = PO.thinc_flux(c, alpha, u0, 1, 1, dt, dx)
+    new = prof.copy()
+    new[2:o.nx - 2] = prof[2:o.nx - 2] - (flux[3:o.nx - 1] - flux[2:o.nx - 2]) / dx
+    got = o.vf.reshape(o.nz, o.ny, o.nx)
+    # rows whose normals see no wall (the wall layers hold vf = 0)
+    for iz in (2, o.nz // 2):
+        for iy in (3, o.ny // 2):
+            assert np.max(np.abs(got[iz, iy, 2:-2] - new[2:-2])) < 1e-13
+
+
+def _sphere(center, radius, level=3):
+    from preprocess_src_amd import mesh
+    return mesh.assemble([level], np.array([center], float), np.eye(3)[None], radius, shape="sphere")
+
+
+def _one_element(o, pts):
+    """Restrict the oracle's cell set to element 0 with vertices pts (3x3)."""
+    c = o.cells
+    for k, v in enumerate(c["ele"][:3]):
+        c["x"][v], c["y"][v], c["z"][v] = pts[k]
+    o._eoff1 = np.array([0, 1], np.int32)
+    o.S.eoff = PO.ip(o._eoff1)
+    o.S.nc = 1
+
+
+def _weights(a):
+    k = np.arange(-1, 3)
+    return (1.0 + np.cos(np.pi * (k - a) / 2)) / 4.0
+
+
+@pytest.mark.parametrize("frac", [(0.0, 0.0, 0.0), (0.3, 0.55, 0.9)])
+def test_front_tracking_spread_of_one_triangle(frac):
+    dx = 0.5
+    o = _oracle(dx=dx, cells=_sphere((0.0, 0.0, 0.0), 2.0, level=2))
+    # centroid at padded node (7, 6, 8) plus frac
+    cen = np.array([(7 + frac[0]) * dx - (o.nx - 1) * dx / 2, (6 + frac[1]) * dx - (o.ny - 1) * dx / 2,
+                    (8 + frac[2]) * dx - (o.nz - 1) * dx / 2])
+    tri = cen + np.array([[0.31, -0.12, 0.05], [-0.2, 0.27, -0.11], [0.0, 0.0, 0.0]])
+    tri[2] = 3 * cen - tri[0] - tri[1]
+    _one_element(o, tri)
+    gx, gy, gz = o.ft_gradient()
+    e1, e2 = tri[1] - tri[0], tri[2] - tri[0]
+    area_vec = -np.cross(e1, e2) / 2.0  # inward for outward-wound triangles
+    G = np.stack([gx, gy, gz], 1).reshape(o.nz, o.ny, o.nx, 3)
+    assert np.allclose(G.reshape(-1, 3).sum(0) * dx ** 3, area_vec, rtol=0, atol=1e-14)
+    wx, wy, wz = (_weights(a) for a in frac)
+    box = G[8 - 1:8 + 3, 6 - 1:6 + 3, 7 - 1:7 + 3]
+    expect = (wz[:, None, None] * wy[None, :, None] * wx[None, None, :])[..., None] * area_vec / dx ** 3
+    assert np.max(np.abs(box - expect)) < 1e-14 / dx ** 3
+    assert np.count_nonzero(np.abs(G).sum(-1)) == np.count_nonzero(np.abs(expect).sum(-1))
+
+
+def test_front_tracking_spread_folds_the_halo_onto_the_images():
+    dx = 1.0
+    o = _oracle(dx=dx, cells=_sphere((0.0, 0.0, 0.0), 2.0, level=2))
+    # an element whose box straddles the +x and -z ends
+    cen = np.array([LX / 2 - 0.4, 0.7, -LZ / 2 + 0.6])
+    tri = cen + np.array([[0.3, -0.2, 0.1], [-0.25, 0.3, -0.15], [0.0, 0.0, 0.0]])
+    tri[2] = 3 * cen - tri[0] - tri[1]
+    _one_element(o, tri)
+    gx, gy, gz = o.ft_gradient()
+    area_vec = -np.cross(tri[1] - tri[0], tri[2] - tri[0]) / 2.0
+    comp = o.bc == 0
+    G = np.stack([gx, gy, gz], 1)
+    assert np.allclose(G[comp].sum(0) * dx ** 3, area_vec, rtol=0, atol=1e-14)
+    buff = np.flatnonzero(o.bc == 1)
+    img = _periodic_fill(o, np.arange(o.n, dtype=float)).astype(np.int64)
+    assert np.array_equal(G[buff], G[img[buff]])
+
+
+def _reference_matrix(o):
+    """front_tracking_matrix (fronttrack.cu:27-238) as an explicit CSR matrix
+    with the Jacobi scaling, for the slab (CapsuleSHEARFLOW) connectivity."""
+    n = o.n
+    mn = PO.lib().orc_expand2pow2(n)
+    dx = o.P.dx
+    nx, nxy = o.nx, o.nx * o.ny
+    img = _periodic_fill(o, np.arange(n, dtype=float)).astype(np.int64)
+    rows, cols, vals = [], [], []
+    for i in range(mn):
+        if i >= n or o.bc[i] == 2:
+            rows.append(i); cols.append(i); vals.append(1.0)
+        elif o.bc[i] == 0:
+            for j, v in ((i - nxy, 1), (i - nx, 1), (i - 1, 1), (i, -6), (i + 1, 1), (i + nx, 1), (i + nxy, 1)):
+                rows.append(i); cols.append(j); vals.append((v / dx ** 2) / (-6.0 / dx ** 2))
+        else:
+            rows += [i, i]; cols += [i, int(img[i])]; vals += [1.0, -1.0]
+    return scipy.sparse.csr_matrix((vals, (rows, cols)), shape=(mn, mn))
+
+
+def test_red_black_sor_reaches_the_direct_solution():
+    o = _oracle(dx=0.5, sor_itrmax=20000, sor_residual=1e-12)
+    rng = np.random.default_rng(5)
+    b = np.where(o.bc == 0, rng.standard_normal(o.n), 0.0)
+    rc, x, its = o.ft_sor(b, np.zeros(o.n))
+    assert rc == 0 and its > 10
+    A = _reference_matrix(o)
+    mn = A.shape[0]
+    xd = scipy.sparse.linalg.spsolve(A.tocsc(), np.r_[b, np.zeros(mn - o.n)])
+    assert rel(x, xd[:o.n]) < 1e-9
+    # a start from x = 3 (the retry guess) converges to the same solution
+    rc, x3, _ = o.ft_sor(b, np.full(o.n, 3.0), 3.0)
+    assert rc == 0 and rel(x3, xd[:o.n]) < 1e-9
+    # ITRMAX reached: flag 2
+    o.P.sor_itrmax = 5
+    assert o.ft_sor(b, np.zeros(o.n))[0] == 2
+
+
+def test_front_tracking_indicator_of_a_sphere():
+    R = 3.6
+    o = oracle_of(24, 20, 22, cells=_sphere((0.37, -0.61, 0.23), R), sor_itrmax=6000, sor_residual=1e-9)
+    rc, its, att = o.front_tracking()
+    assert rc == 0 and att == 1 and its > 100
+    X, Y, Z = o.node_coords()
+    r = np.sqrt((X - 0.37) ** 2 + (Y + 0.61) ** 2 + (Z - 0.23) ** 2)
+    comp = o.bc == 0
+    assert np.all(np.abs(o.vf[comp & (r < R - 2.2)] - 1.0) < 0.03)
+    assert np.all(np.abs(o.vf[comp & (r > R + 2.2)]) < 0.03)
+    c = o.cells
+    p = np.stack([c["x"], c["y"], c["z"]], 1)
+    t = c["ele"].reshape(-1, 3)
+    V = np.einsum("ij,ij->i", p[t[:, 0]], np.cross(p[t[:, 1]], p[t[:, 2]])).sum() / 6.0
+    assert abs(o.vf[comp].sum() - V) < 0.01 * V
+    # ITRMAX too small for every start: error 5 after the eleventh attempt
+    o.P.sor_itrmax = 3
+    rc, _, att = o.front_tracking()
+    assert rc == 5 and att == 11
