@@ -44,7 +44,9 @@ typedef enum lbmibm_status {
     LBMIBM_ERR_BICG_NOCONV = 4,    /* BiCGSTAB hit ITRMAX, flag 2 (matrix_solver.cu:374)     */
     LBMIBM_ERR_VOLUME = 5,         /* volume correction hit 1000 passes (:487-493)          */
     LBMIBM_ERR_TOPOLOGY = 6,       /* mesh topology check failed, error(3) (init2.cu:161)   */
-    LBMIBM_ERR_STATE = 7           /* call not valid in the current state                   */
+    LBMIBM_ERR_STATE = 7,          /* call not valid in the current state                   */
+    LBMIBM_ERR_VOF = 8             /* front-tracking SOR failed from every start, error(5)
+                                      (fronttrack.cu:707-720)                               */
 } lbmibm_status;
 
 typedef enum lbmibm_precision {
@@ -85,6 +87,13 @@ typedef struct lbmibm_params {
                                 tension and rg of each element and the vertex velocity (the
                                 reference's outputs); 0: only what the next step reads      */
     int    fileoutput;       /* FILEOUTPUT period of the tp / qi accumulators (0: off)       */
+    int    vof_period;       /* K: lbmibm_step runs THINC/WLIC every M steps and front
+                                tracking every K steps (main.cu:93-97); 0: no VOF indicator
+                                (vf = 0).  Needs cells and the slab (periodic_y = 0);
+                                LAMBDA1 != 1 needs it                                        */
+    int    sor_itrmax;       /* ITRMAX of the front-tracking red-black SOR                   */
+    double sor_residual;     /* its RESIDUAL (relative, matrix_solver.cu:188-215); the
+                                reference shares both constants with BiCGSTAB                */
 } lbmibm_params;
 
 /* Initial cell set: reference SoA positions and element vertex ids. */
@@ -100,7 +109,8 @@ typedef struct lbmibm_ctx lbmibm_ctx;
 
 /* Defaults for an lx*ly*lz slab channel: dx=dt=Cr=1, Df=0.1 (tau=0.8),
  * LAMBDA1=1, no body force, B=0, M=1, C=100, rg=1, BR=0, ITRMAX=1000,
- * RESIDUAL=1e-10, RK4, Skalak, volume correction on, FP32, store_macros=0. */
+ * RESIDUAL=1e-10, RK4, Skalak, volume correction on, FP32, store_macros=0,
+ * no VOF indicator (SOR ITRMAX=1000, RESIDUAL=1e-10). */
 void          lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz);
 
 /* cdo_init + flu_init + cel_init + fem_init + mem_init (init1.cu:13-1092,
@@ -182,13 +192,33 @@ lbmibm_status lbmibm_download_volume(lbmibm_ctx *ctx, double *volume0, double *v
 /* Per-cell force scale (per-cell Gs / reference Gs), [nc]; NULL resets to 1. */
 lbmibm_status lbmibm_set_cell_force_scale(lbmibm_ctx *ctx, const double *scale);
 
+/* ---- VOF indicator (vof_period > 0) --------------------------------------
+ * The indicator vf of the cells' interiors sets the relaxation rate of the
+ * collision, omega = 1/(3(1 + (LAMBDA1-1) vf) Df + 0.5) (lbm.cu:1131).  It
+ * is kept in FP64 whatever the lattice precision. */
+/* front_tracking (fronttrack.cu:628-724), also vof_init (init1.cu:1095-1123,
+ * once before the time loop): vf from the current positions by spreading
+ * the elements' inward area vectors, the divergence and a red-black SOR
+ * Poisson solve (matrix_solver.cu:112-216) started from x = 0, 1, 2, ... after
+ * each failure; LBMIBM_ERR_VOF after 10 failed starts.  *iterations: SOR
+ * iterations of the start that converged, *attempts: starts used (either
+ * may be NULL).  Synchronises the host once per batch of SOR iterations. */
+lbmibm_status lbmibm_front_tracking(lbmibm_ctx *ctx, int *iterations, int *attempts);
+/* thinc_wlic (thincwlic.cu:316-372): advects vf with the M-averaged
+ * velocity over M dt, one directional sweep per axis.  No host sync. */
+lbmibm_status lbmibm_thinc_wlic(lbmibm_ctx *ctx);
+/* ltc->vfH on the padded lattice [n]: COMP values, BUFF nodes their periodic
+ * image, WALL nodes 0.  Upload takes the COMP values. */
+lbmibm_status lbmibm_download_vf(lbmibm_ctx *ctx, double *vf);
+lbmibm_status lbmibm_upload_vf(lbmibm_ctx *ctx, const double *vf);
+
 /* ---- the reference's binary files (etc.cu) ------------------------------
  * Byte layouts of output_data / restart_file: native-endian int32 and FP64
  * arrays one after another, no headers.  The cell sections assume one mesh
  * for all cells (cel->vertex, cel->element): a context whose cells differ
  * gets LBMIBM_ERR_ARG.  Fields this path does not model are written as the
- * reference initialises them and ignored on reading: the VOF indicator
- * vf, vf2 (0, init1.cu:396-411).  dm (the density sum) and the vertex
+ * reference initialises them and ignored on reading: vf2, the indicator of a
+ * second species (0, init1.cu:396-411), and vf when vof_period = 0.  dm (the density sum) and the vertex
  * velocities u, v, w are kept only with store_macros (0 otherwise); the
  * per-cell rx, ry, rz, which no reference code assigns, are written as the
  * mean of each cell's vertex positions. */

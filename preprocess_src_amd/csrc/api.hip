@@ -178,6 +178,12 @@ lbmibm_status enqueue_step(lbmibm_ctx *ctx, int iter, bool device_advance, bool 
     if (s == LBMIBM_OK && iter % M == 0 && ctx->nc > 0) {
         s = join_volume(ctx);
         if (s == LBMIBM_OK) s = ibm_interp(ctx);
+        // THINC/WLIC, then front tracking every K steps, before the volume
+        // correction (main.cu:93-97); front tracking waits for the host
+        if (s == LBMIBM_OK && ctx->p.vof_period > 0) {
+            s = vof_thinc(ctx);
+            if (s == LBMIBM_OK && iter % ctx->p.vof_period == 0) s = vof_front_tracking(ctx, nullptr, nullptr);
+        }
         if (s == LBMIBM_OK && overlap_volume && ctx->p.br == 0.0) {
             // the volume correction overlapping the next membrane force (the
             // binning of the corrected positions stays with the spreading: on
@@ -209,7 +215,8 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair,
                     ctx->srec, ctx->onchip_cells, ctx->Hb, ctx->adj_f[0], ctx->adj_f[1], ctx->adj_u, ctx->adj_F,
                     ctx->adj_x, ctx->adj_cf, ctx->grad_H, ctx->grad_gs, ctx->adj_lam, ctx->adj_dsv,
-                    ctx->adj_corner, ctx->adj_scratch, ctx->adj_cscale, ctx->adj_gpart, ctx->adj_xmax, ctx->adj_rec, ctx->sbox, ctx->sb_cand};
+                    ctx->adj_corner, ctx->adj_scratch, ctx->adj_cscale, ctx->adj_gpart, ctx->adj_xmax, ctx->adj_rec, ctx->sbox, ctx->sb_cand,
+                    ctx->vof[0], ctx->vof[1], ctx->ft_g, ctx->ft_x, ctx->ft_b, ctx->ft_part, ctx->ft_ctl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -253,6 +260,9 @@ void lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz) {
     p->precision = LBMIBM_F32;
     p->store_macros = 0;
     p->fileoutput = 0;
+    p->vof_period = 0;
+    p->sor_itrmax = 1000;
+    p->sor_residual = 1.0e-10;
 }
 
 const char *lbmibm_last_error(const lbmibm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -274,7 +284,12 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         (p->runge_kutta != 1 && p->runge_kutta != 2 && p->runge_kutta != 4) || p->precision < 0 ||
         p->precision > 2 || p->itrmax < 1 || (p->force_modulated && p->freq == 0.0))
         return LBMIBM_ERR_ARG;
-    if (p->lambda1 != 1.0) return LBMIBM_ERR_ARG;  // needs the VOF indicator (front tracking)
+    // the viscosity contrast acts through the VOF indicator, which needs cells
+    // and the slab's walls (CapsuleSHEARFLOW, fronttrack.cu:87-125)
+    if (p->vof_period < 0 || (p->lambda1 != 1.0 && p->vof_period == 0)) return LBMIBM_ERR_ARG;
+    if (p->vof_period > 0 && (p->periodic_y || !cells || cells->nc < 1 || p->sor_itrmax < 1 ||
+                              !(p->sor_residual > 0.0)))
+        return LBMIBM_ERR_ARG;
     const long long N = (long long)p->lx * p->ly * p->lz;
     if (N >= (1LL << 32)) return LBMIBM_ERR_ARG;
     int ndev = 0;
@@ -525,6 +540,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     CK(set_step(ctx, 0, false));
     CK(membrane_autotune(ctx));
     CK(ibm_autotune(ctx));
+    if (p->vof_period > 0) CK(vof_alloc(ctx));
     CK(reset_status(ctx));
     CK(sync_check(ctx));
 #undef CK
@@ -597,6 +613,7 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
     lbmibm_status s = reset_status(ctx);
     if (s != LBMIBM_OK) return s;
     const int M = ctx->p.m;
+    if (ctx->p.vof_period > 0) use_graph = 0;  // front tracking synchronises with the host
     // a replayable block is whole M-cycles with an even number of lattice
     // steps, so that the population buffers are back in place, repeated to at
     // least 8 steps: a step's volume correction overlaps the next step's
@@ -653,6 +670,60 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
         (void)hipStreamSynchronize(ctx->vstream);
     }
     return s == LBMIBM_OK ? check_flags(ctx, nullptr) : s;
+}
+
+lbmibm_status lbmibm_front_tracking(lbmibm_ctx *ctx, int *iterations, int *attempts) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    if (ctx->p.vof_period <= 0 || ctx->nc < 1) {
+        ctx->err = "front tracking needs vof_period > 0";
+        return LBMIBM_ERR_STATE;
+    }
+    lbmibm_status s = join_volume(ctx);
+    if (s == LBMIBM_OK) s = vof_front_tracking(ctx, iterations, attempts);
+    return s == LBMIBM_OK ? sync_check(ctx) : s;
+}
+
+lbmibm_status lbmibm_thinc_wlic(lbmibm_ctx *ctx) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    if (ctx->p.vof_period <= 0) {
+        ctx->err = "THINC/WLIC needs vof_period > 0";
+        return LBMIBM_ERR_STATE;
+    }
+    return vof_thinc(ctx);
+}
+
+lbmibm_status lbmibm_download_vf(lbmibm_ctx *ctx, double *vf) {
+    if (!ctx || !vf) return LBMIBM_ERR_ARG;
+    const long long N = ctx->N;
+    std::vector<double> v((size_t)N, 0.0);
+    if (ctx->vof[0]) {
+        LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        LBMIBM_HIP(ctx, hipMemcpy(v.data(), ctx->vof[ctx->vof_cur], N * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (int pz = 0; pz < ctx->nzp; pz++)
+        for (int py = 0; py < ctx->nyp; py++)
+            for (int px = 0; px < ctx->nxp; px++) {
+                bool comp;
+                const long long i = padded_image(ctx, px, py, pz, comp);
+                vf[px + (long long)ctx->nxp * (py + (long long)ctx->nyp * pz)] = i < 0 ? 0.0 : v[i];
+            }
+    return LBMIBM_OK;
+}
+
+lbmibm_status lbmibm_upload_vf(lbmibm_ctx *ctx, const double *vf) {
+    if (!ctx || !vf) return LBMIBM_ERR_ARG;
+    if (!ctx->vof[0]) {
+        ctx->err = "vf needs vof_period > 0";
+        return LBMIBM_ERR_STATE;
+    }
+    std::vector<double> v((size_t)ctx->N);
+    for (int iz = 0; iz < ctx->nz; iz++)
+        for (int iy = 0; iy < ctx->ny; iy++)
+            for (int ix = 0; ix < ctx->nx; ix++)
+                v[ix + (size_t)ctx->nx * (iy + (size_t)ctx->ny * iz)] = vf[padded_index(ctx, ix, iy, iz)];
+    LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LBMIBM_HIP(ctx, hipMemcpy(ctx->vof[ctx->vof_cur], v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    return LBMIBM_OK;
 }
 
 lbmibm_status lbmibm_time_stages(lbmibm_ctx *ctx, int iter0, int nsteps, double *stage_ms, int *stage_calls) {

@@ -149,6 +149,16 @@ struct lbmibm_ctx {
     unsigned *absmax = nullptr;
     bool bins_valid = false;                 // bins match the current positions
 
+    // VOF indicator (vof_period > 0, allocated on first use): vf [N] FP64 in
+    // vof[vof_cur] and its partner for the THINC/WLIC sweeps; front-tracking
+    // work arrays on the padded lattice: fixed-point G [3][np], SOR iterate
+    // and right-hand side [np], per-block partial sums, the SOR status word
+    double *vof[2] = {nullptr, nullptr};
+    int vof_cur = 0;
+    unsigned long long *ft_g = nullptr;
+    double *ft_x = nullptr, *ft_b = nullptr, *ft_part = nullptr;
+    void *ft_ctl = nullptr;
+    int ft_nparts = 0;
     lbmibm::DevStep *step = nullptr;
     lbmibm::DevStatus *status = nullptr;
 
@@ -180,6 +190,9 @@ lbmibm_status membrane_setup(lbmibm_ctx *ctx);               // membrane.hip
 lbmibm_status membrane_autotune(lbmibm_ctx *ctx);            // membrane.hip
 lbmibm_status ibm_autotune(lbmibm_ctx *ctx);                 // ibm.hip
 lbmibm_status volume_run(lbmibm_ctx *ctx, int mode);         // volume.hip (0 init, 1 correct)
+lbmibm_status vof_alloc(lbmibm_ctx *ctx);                    // vof.hip
+lbmibm_status vof_front_tracking(lbmibm_ctx *ctx, int *iterations, int *attempts);  // vof.hip (host sync)
+lbmibm_status vof_thinc(lbmibm_ctx *ctx);                    // vof.hip
 // both classes on the given stream (not the context's main stream)
 lbmibm_status volume_run_on(lbmibm_ctx *ctx, int mode, hipStream_t stream);  // volume.hip
 lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool advance);  // api.hip

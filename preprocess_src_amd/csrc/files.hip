@@ -98,9 +98,9 @@ lbmibm_status close_out(lbmibm_ctx *ctx, Out &o, const char *path) {
 }
 
 // The fields of a lattice record: dm (the density sum, zero without
-// store_macros), um, vm, wm (the M-averaged velocity / Cr), and vf, vf2 (the
-// VOF indicator of the front tracking, not part of this path: written as 0,
-// the reference's initial value, init1.cu:396-411).
+// store_macros), um, vm, wm (the M-averaged velocity / Cr), vf (the VOF
+// indicator, 0 without vof_period) and vf2 (the indicator of a second cell
+// species, not modelled: 0, the reference's initial value, init1.cu:396-411).
 lbmibm_status put_lattice_fields(lbmibm_ctx *ctx, Out &o, long long n) {
     std::vector<double> dm(n, 0.0), um(n), vm(n), wm(n);
     lbmibm_status s = lbmibm_download_macros(ctx, nullptr, nullptr, nullptr, nullptr,
@@ -111,7 +111,15 @@ lbmibm_status put_lattice_fields(lbmibm_ctx *ctx, Out &o, long long n) {
     o.put(um);
     o.put(vm);
     o.put(wm);
-    o.zeros(2 * (size_t)n);
+    if (ctx->p.vof_period > 0) {
+        std::vector<double> vf(n);
+        s = lbmibm_download_vf(ctx, vf.data());
+        if (s != LBMIBM_OK) return s;
+        o.put(vf);
+    } else {
+        o.zeros((size_t)n);
+    }
+    o.zeros((size_t)n);
     return LBMIBM_OK;
 }
 
@@ -231,8 +239,10 @@ lbmibm_status lbmibm_read_restart(lbmibm_ctx *ctx, const char *path, int *iter, 
     if (fno > 100) { it = 0; fno = 0; }  // as restart_file (etc.cu:205)
     const std::vector<double> fn = take_d((size_t)L.n * Q);
     const std::vector<double> dm = take_d(L.n), um = take_d(L.n), vm = take_d(L.n), wm = take_d(L.n);
-    off += 2 * (size_t)L.n * sizeof(double);  // vf, vf2: no VOF on this path
+    const std::vector<double> vf = take_d(L.n);
+    off += (size_t)L.n * sizeof(double);  // vf2: one cell species here
     s = lbmibm_upload_populations(ctx, fn.data());
+    if (s == LBMIBM_OK && ctx->p.vof_period > 0) s = lbmibm_upload_vf(ctx, vf.data());
     if (s == LBMIBM_OK) s = lbmibm_upload_macros(ctx, um.data(), vm.data(), wm.data());
     if (s == LBMIBM_OK && ctx->p.store_macros) {
         std::vector<double> a(ctx->N);

@@ -54,9 +54,9 @@ struct LbmDims {
 template <typename S, typename T, bool HASF, bool MACROS, bool WALLS>
 __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__restrict__ fout, LbmDims d,
                                             const T *__restrict__ F, const T *__restrict__ H, T *__restrict__ U,
-                                            T *__restrict__ Ui, T *__restrict__ rho_out, T *__restrict__ rhom, T om,
+                                            T *__restrict__ Ui, T *__restrict__ rho_out, T *__restrict__ rhom, T om0,
                                             T inv_cr, T pex, T pey, T pez, T B, const DevStep *__restrict__ st,
-                                            int m) {
+                                            int m, const double *__restrict__ vf, T l1m1, T df) {
     const int lb = xcd_block(d.nblocks);
     if (lb >= d.nblocks) return;
     const unsigned i = (unsigned)lb * WG + threadIdx.x;
@@ -110,6 +110,8 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         ay += H[N + i];
         az += H[2 * N + i];
     }
+    // relaxation rate with the VOF indicator (lbm.cu:1131), uniform without
+    const T om = vf ? T(1) / (T(3) * (T(1) + l1m1 * (T)vf[i]) * df + T(0.5)) : om0;
     const T irho = T(1) / rho;
     // U = (j + G/2)/rho with the Guo force G = 2 rho (P e + B F) (lbm.cu:1116-1118)
     const T Ux = jx * irho + ax, Uy = jy * irho + ay, Uz = jz * irho + az;
@@ -302,13 +304,19 @@ template <typename S, typename T> lbmibm_status step_impl(lbmibm_ctx *ctx) {
     d.nblocks = (int)((ctx->N + WG - 1) / WG);
     const int grid = xcd_grid(d.nblocks);
     const double om = 1.0 / (3.0 * p.df + 0.5);  // lbm.cu:1131 with vf = 0
+    if (p.vof_period > 0 && !ctx->vof[0]) {
+        ctx->err = "the VOF indicator is not allocated";
+        return LBMIBM_ERR_STATE;
+    }
     const S *fin = (const S *)ctx->f[ctx->cur];
     S *fout = (S *)ctx->f[ctx->cur ^ 1];
     const bool hasf = !ctx->force_zero, macros = p.store_macros != 0, walls = p.periodic_y == 0;
     auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, ctx->stream, fin, fout, d, (const T *)ctx->F,
                            (const T *)ctx->Hb, (T *)ctx->U, (T *)ctx->Ui, (T *)ctx->rho, (T *)ctx->rhom, (T)om, (T)(1.0 / p.cr),
-                           (T)(p.p * p.ex), (T)(p.p * p.ey), (T)(p.p * p.ez), (T)p.b, ctx->step, p.m);
+                           (T)(p.p * p.ex), (T)(p.p * p.ey), (T)(p.p * p.ez), (T)p.b, ctx->step, p.m,
+                           (const double *)(p.vof_period > 0 ? ctx->vof[ctx->vof_cur] : nullptr), (T)(p.lambda1 - 1.0),
+                           (T)p.df);
     };
 #define LBM_CASE(HF, MC, WL) \
     if (hasf == HF && macros == MC && walls == WL) launch(k_lbm<S, T, HF, MC, WL>);

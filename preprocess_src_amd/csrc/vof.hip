@@ -1,0 +1,565 @@
+// VOF indicator of the cell interiors: front tracking (fronttrack.cu:274-724)
+// with its red-black SOR Poisson solve (matrix_solver.cu:112-216), and the
+// THINC/WLIC advection (thincwlic.cu:12-372).  Slab connectivity (walls in
+// y, periodic x and z), FP64 throughout.
+//
+// Front tracking works on the reference's padded lattice (nx+4)(ny+4)(nz+4)
+// so that its halo fold, its BUFF and WALL rows and its padding rows are the
+// reference's own:
+//   - the elements' inward area vectors are spread with the cosine delta by
+//     64-bit fixed-point atomics (integer sums do not depend on the order, so
+//     the field is deterministic; one unit is 2^-44 of 1/dx);
+//   - the x/z halo sums are folded onto their COMP images and copied back;
+//   - b = div G / (-6/dx^2) on COMP nodes;
+//   - the SOR runs matrix-free on the Jacobi-scaled rows: COMP x_i - (1/6)
+//     sum of the 6 neighbours = b_i, BUFF x_i - x_image = 0, WALL x_i = 0,
+//     and the expand2pow2(n) - n identity rows of the padding, whose common
+//     value is carried as one scalar.  Per colour, the COMP and WALL rows go
+//     first and the BUFF rows then read the updated image (the reference
+//     updates them in one kernel, with a read of a same-colour image that
+//     races with its update; this is one of the orders it can produce).
+//     The convergence test sqrt(r.r / b.b) < RESIDUAL runs on the device
+//     after every iteration; later kernels of a batch return at once, so the
+//     host reads the status once per batch.
+// THINC/WLIC works on the unpadded lattice: halo values are the periodic
+// images, which is what the reference's BUFF copies hold, and the wall
+// layers hold vf = 0.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace lbmibm {
+namespace {
+
+constexpr double FT_UNIT = 17592186044416.0;  // 2^44 fixed-point units per 1/dx
+constexpr int VWG = 256;
+constexpr int FT_MAX_ATTEMPTS = 10;
+
+struct FtGeom {
+    int nx, ny, nz;  // padded sizes
+    long long np;
+    double dx, lx, lz;  // spacing, physical LX, LZ
+};
+
+struct SorCtl {
+    int done;  // 0 running, 1 converged, 2 ITRMAX reached
+    int iter;
+    int conv_iter;
+    int pad;
+    double xpad;  // value of every padding row
+    double bb;    // b.b
+    double err;
+};
+
+__device__ double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += lds[k];
+    return s;
+}
+
+template <typename V> __global__ void k_vfill(V *__restrict__ p, size_t n, V v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// front_tracking_source2 (fronttrack.cu:336-441): 64 lanes per element, one
+// per node of the 4x4x4 box starting at floor(.)-1 (the box is 3 wide where
+// the centroid sits on a node: ceil = floor).
+__global__ __launch_bounds__(VWG) void k_ft_spread(const double *__restrict__ xs, int nv, const int *__restrict__ ele,
+                                                   int ne, FtGeom g, unsigned long long *__restrict__ G) {
+    const long long t = (long long)blockIdx.x * VWG + threadIdx.x;
+    const long long e = t >> 6;
+    if (e >= ne) return;
+    const int k = (int)(t & 63), kx = k & 3, ky = (k >> 2) & 3, kz = k >> 4;
+    const double *X = xs, *Y = xs + nv, *Z = xs + 2 * (size_t)nv;
+    const int j1 = ele[3 * e], j2 = ele[3 * e + 1], j3 = ele[3 * e + 2];
+    const double LX = g.lx, LZ = g.lz, dx = g.dx;
+    double tx1 = X[j1], tx2 = X[j2], tx3 = X[j3];
+    double ty1 = Y[j1], ty2 = Y[j2], ty3 = Y[j3];
+    double tz1 = Z[j1], tz2 = Z[j2], tz3 = Z[j3];
+    double xx1 = tx2 - tx1, yy1 = ty2 - ty1, zz1 = tz2 - tz1;
+    if (xx1 > LX * 0.5) { xx1 -= LX; tx1 += LX; }
+    else if (xx1 < -LX * 0.5) { xx1 += LX; tx2 += LX; }
+    if (zz1 > LZ * 0.5) { zz1 -= LZ; tz1 += LZ; }
+    else if (zz1 < -LZ * 0.5) { zz1 += LZ; tz2 += LZ; }
+    double xx2 = tx3 - tx1, yy2 = ty3 - ty1, zz2 = tz3 - tz1;
+    if (xx2 > LX * 0.5) { xx2 -= LX; tx1 += LX; }
+    else if (xx2 < -LX * 0.5) { xx2 += LX; tx3 += LX; }
+    if (zz2 > LZ * 0.5) { zz2 -= LZ; tz1 += LZ; }
+    else if (zz2 < -LZ * 0.5) { zz2 += LZ; tz3 += LZ; }
+    const double xtr = (tx1 + tx2 + tx3) / 3.0 + (double)(g.nx - 1) * dx / 2.0;
+    const double ytr = (ty1 + ty2 + ty3) / 3.0 + (double)(g.ny - 1) * dx / 2.0;
+    const double ztr = (tz1 + tz2 + tz3) / 3.0 + (double)(g.nz - 1) * dx / 2.0;
+    const int ix = (int)floor(xtr / dx) - 1 + kx, iy = (int)floor(ytr / dx) - 1 + ky, iz = (int)floor(ztr / dx) - 1 + kz;
+    if (ix > (int)ceil(xtr / dx) + 1 || iy > (int)ceil(ytr / dx) + 1 || iz > (int)ceil(ztr / dx) + 1) return;
+    if (ix < 0 || ix >= g.nx || iy < 0 || iy >= g.ny || iz < 0 || iz >= g.nz) return;
+    double nvx = -(yy1 * zz2 - zz1 * yy2);
+    double nvy = -(zz1 * xx2 - xx1 * zz2);
+    double nvz = -(xx1 * yy2 - yy1 * xx2);
+    const double nn = sqrt(nvx * nvx + nvy * nvy + nvz * nvz);
+    const double ds = nn / 2.0;
+    nvx /= nn; nvy /= nn; nvz /= nn;
+    const double xcos = 1.0 + cos(M_PI * ((double)ix * dx - xtr) / (2.0 * dx));
+    const double ycos = 1.0 + cos(M_PI * ((double)iy * dx - ytr) / (2.0 * dx));
+    const double zcos = 1.0 + cos(M_PI * ((double)iz * dx - ztr) / (2.0 * dx));
+    const double dlt = xcos * ycos * zcos / (64.0 * dx * dx * dx);
+    const double s = dx * FT_UNIT;
+    const long long i = ix + (long long)g.nx * (iy + (long long)g.ny * iz);
+    atomicAdd(G + i, (unsigned long long)__double2ll_rn(dlt * nvx * ds * s));
+    atomicAdd(G + g.np + i, (unsigned long long)__double2ll_rn(dlt * nvy * ds * s));
+    atomicAdd(G + 2 * g.np + i, (unsigned long long)__double2ll_rn(dlt * nvz * ds * s));
+}
+
+__device__ __forceinline__ bool ft_comp(const FtGeom &g, int ix, int iy, int iz) {
+    return ix >= 2 && ix <= g.nx - 3 && iy >= 2 && iy <= g.ny - 3 && iz >= 2 && iz <= g.nz - 3;
+}
+
+// front_tracking_source3 (fronttrack.cu:444-499) in exact integer sums.
+__global__ __launch_bounds__(VWG) void k_ft_fold(FtGeom g, unsigned long long *__restrict__ G) {
+    const long long i = (long long)blockIdx.x * VWG + threadIdx.x;
+    if (i >= g.np) return;
+    const int ix = (int)(i % g.nx), iy = (int)((i / g.nx) % g.ny), iz = (int)(i / ((long long)g.nx * g.ny));
+    if (!ft_comp(g, ix, iy, iz)) return;
+    const bool xl = ix <= 3, xh = ix >= g.nx - 4, zl = iz <= 3, zh = iz >= g.nz - 4;
+    if (!(xl || xh || zl || zh)) return;
+    const long long sx = g.nx - 4, sz = (long long)g.nx * g.ny * (g.nz - 4);
+    const long long ox = xl ? sx : -sx, oz = zl ? sz : -sz;
+    long long img[3] = {-1, -1, -1};
+    if ((xl || xh) && (zl || zh)) { img[0] = i + ox; img[1] = i + oz; img[2] = i + ox + oz; }
+    else if (xl || xh) img[0] = i + ox;
+    else img[0] = i + oz;
+    for (int c = 0; c < 3; c++) {
+        unsigned long long *Gc = G + c * g.np;
+        unsigned long long v = Gc[i];
+        for (int k = 0; k < 3; k++)
+            if (img[k] >= 0) v += Gc[img[k]];
+        Gc[i] = v;
+        for (int k = 0; k < 3; k++)
+            if (img[k] >= 0) Gc[img[k]] = v;
+    }
+}
+
+// front_tracking_source4 + jacobi_source (fronttrack.cu:502-556, 314-333);
+// per-block sums of b^2.
+__global__ __launch_bounds__(VWG) void k_ft_rhs(FtGeom g, const unsigned long long *__restrict__ G,
+                                                double *__restrict__ b, double *__restrict__ part) {
+    __shared__ double lds[VWG / 64];
+    const long long i = (long long)blockIdx.x * VWG + threadIdx.x;
+    double bi = 0.0;
+    if (i < g.np) {
+        const int ix = (int)(i % g.nx), iy = (int)((i / g.nx) % g.ny), iz = (int)(i / ((long long)g.nx * g.ny));
+        if (ft_comp(g, ix, iy, iz)) {
+            const double u = 1.0 / (g.dx * FT_UNIT);
+            const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
+            auto v = [&](int c, long long j) { return (double)(long long)G[c * g.np + j] * u; };
+            const double dg = (v(0, i + 1) - v(0, i - 1) + v(1, i + nx) - v(1, i - nx) + v(2, i + nxy) - v(2, i - nxy)) /
+                              (2.0 * g.dx);
+            bi = dg / (-6.0 / (g.dx * g.dx));
+        }
+        b[i] = bi;
+    }
+    const double s = block_sum(bi * bi, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_ft_start(SorCtl *__restrict__ ctl, const double *__restrict__ part, int nparts, double cnt) {
+    __shared__ double lds[VWG / 64];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += part[k];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) {
+        ctl->done = 0;
+        ctl->iter = 0;
+        ctl->conv_iter = 0;
+        ctl->xpad = cnt;
+        ctl->bb = s;
+        ctl->err = 0.0;
+    }
+}
+
+// One colour of the COMP and WALL rows.  One lane per x-pair of a row: the
+// lane takes the node of colour C, so a wave reads and writes a contiguous
+// stretch of the row.
+template <int C>
+__global__ __launch_bounds__(VWG) void k_sor_rows(FtGeom g, double *__restrict__ x, const double *__restrict__ b,
+                                                  const SorCtl *__restrict__ ctl, double *__restrict__ part, double a) {
+#pragma clang fp contract(off)
+    __shared__ double lds[VWG / 64];
+    if (ctl->done) return;
+    const int nxh = (g.nx + 1) >> 1;
+    const long long t = (long long)blockIdx.x * VWG + threadIdx.x;
+    const long long row = t / nxh;
+    double r2 = 0.0;
+    if (row < (long long)g.ny * g.nz) {
+        const int iy = (int)(row % g.ny), iz = (int)(row / g.ny);
+        const int ix = 2 * (int)(t % nxh) + ((C + iy + iz) & 1);
+        if (ix < g.nx) {
+            const long long i = ix + (long long)g.nx * row;
+            const bool wall = iy < 2 || iy > g.ny - 3;
+            const bool comp = !wall && ix >= 2 && ix <= g.nx - 3 && iz >= 2 && iz <= g.nz - 3;
+            if (comp || wall) {
+                double sum, bi = 0.0;
+                if (comp) {
+                    const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
+                    sum = 0.0;
+                    sum += a * x[i - nxy];
+                    sum += a * x[i - nx];
+                    sum += a * x[i - 1];
+                    sum += 1.0 * x[i];
+                    sum += a * x[i + 1];
+                    sum += a * x[i + nx];
+                    sum += a * x[i + nxy];
+                    bi = b[i];
+                } else {
+                    sum = 1.0 * x[i];
+                }
+                const double r = bi - sum;
+                x[i] = x[i] + 1.3 * r;
+                r2 = r * r;
+            }
+        }
+    }
+    const double s = block_sum(r2, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// One colour of the BUFF rows x_i - x_image = 0: the x halo (all z) then
+// the z halo (COMP x), y within the COMP range.
+template <int C>
+__global__ __launch_bounds__(VWG) void k_sor_buff(FtGeom g, double *__restrict__ x, const SorCtl *__restrict__ ctl,
+                                                  double *__restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double lds[VWG / 64];
+    if (ctl->done) return;
+    const long long t = (long long)blockIdx.x * VWG + threadIdx.x;
+    const long long ny4 = g.ny - 4, nx4 = g.nx - 4;
+    const long long n1 = 4 * ny4 * g.nz, n2 = 4 * nx4 * ny4;
+    double r2 = 0.0;
+    if (t < n1 + n2) {
+        int ix, iy, iz;
+        if (t < n1) {
+            const int a = (int)(t & 3);
+            const long long rest = t >> 2;
+            ix = a < 2 ? a : g.nx - 4 + a;
+            iy = 2 + (int)(rest % ny4);
+            iz = (int)(rest / ny4);
+        } else {
+            const long long u = t - n1;
+            const int a = (int)(u & 3);
+            const long long rest = u >> 2;
+            iz = a < 2 ? a : g.nz - 4 + a;
+            ix = 2 + (int)(rest % nx4);
+            iy = 2 + (int)(rest / nx4);
+        }
+        if (((ix + iy + iz) & 1) == C) {
+            const int jx = ix <= 1 ? ix + (g.nx - 4) : (ix >= g.nx - 2 ? ix - (g.nx - 4) : ix);
+            const int jz = iz <= 1 ? iz + (g.nz - 4) : (iz >= g.nz - 2 ? iz - (g.nz - 4) : iz);
+            const long long i = ix + (long long)g.nx * (iy + (long long)g.ny * iz);
+            const long long j = jx + (long long)g.nx * (iy + (long long)g.ny * jz);
+            const double r = 0.0 - (x[i] - x[j]);
+            x[i] = x[i] + 1.3 * r;
+            r2 = r * r;
+        }
+    }
+    const double s = block_sum(r2, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Residual norm of one iteration (the padding rows, all equal, contribute
+// npad x_pad^2) and the stop test of matrix_solver.cu:198-205.
+__global__ void k_sor_check(SorCtl *__restrict__ ctl, const double *__restrict__ part, int nparts, double npad,
+                            int itrmax, double residual) {
+#pragma clang fp contract(off)
+    __shared__ double lds[VWG / 64];
+    if (ctl->done) return;
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += part[k];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) {
+        const double xp = ctl->xpad;
+        const double rr = s + npad * (xp * xp);
+        ctl->xpad = xp + 1.3 * (0.0 - 1.0 * xp);
+        const int it = ctl->iter + 1;
+        ctl->iter = it;
+        const double err = sqrt(rr / ctl->bb);
+        ctl->err = err;
+        if (err < residual) {
+            ctl->done = 1;
+            ctl->conv_iter = it;
+        } else if (it >= itrmax) {
+            ctl->done = 2;
+        }
+    }
+}
+
+__global__ __launch_bounds__(VWG) void k_ft_extract(FtGeom g, const double *__restrict__ x, double *__restrict__ vf,
+                                                    int nx, int ny, long long N) {
+    const long long j = (long long)blockIdx.x * VWG + threadIdx.x;
+    if (j >= N) return;
+    const int ix = (int)(j % nx), iy = (int)((j / nx) % ny), iz = (int)(j / ((long long)nx * ny));
+    vf[j] = x[(ix + 2) + (long long)g.nx * ((iy + 2) + (long long)g.ny * (iz + 2))];
+}
+
+// ---- THINC/WLIC ---------------------------------------------------------
+
+struct VofDims {
+    int nx, ny, nz;  // unpadded
+    long long N;
+    int m;
+    double dt, dx;
+};
+
+__device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// thincwlic.cu:187-197 (expf / coshf of float arguments, as there)
+__device__ double thinc_flux(double c, double alpha, double uc, int isgn, int m, double dt, double dx) {
+#pragma clang fp contract(off)
+    const double beta = 2.0;
+    const double a1 = (double)expf((float)(beta * (2.0 * c - 1.0) / alpha));
+    const double a3 = (double)expf((float)beta);
+    const double xc = 0.5 / beta * log((a3 * a3 - a1 * a3) / (a1 * a3 - 1.0));
+    const double a4 = (double)coshf((float)(beta * ((double)isgn - uc * (double)m * dt / dx - xc)));
+    const double a5 = (double)coshf((float)(beta * ((double)isgn - xc)));
+    return 0.5 * (uc * (double)m * dt - alpha * dx / beta * log(a4 / a5));
+}
+
+// One directional sweep D of wlic1d4md_st1..st3 (thincwlic.cu:108-313):
+// each COMP node evaluates the fluxes of its two faces along D (the face
+// below node i is the reference's flux[i]) from vin and writes vout.
+template <typename T, int D>
+__global__ __launch_bounds__(VWG) void k_thinc(VofDims d, const double *__restrict__ vin, double *__restrict__ vout,
+                                               const T *__restrict__ u) {
+#pragma clang fp contract(off)
+    const long long j = (long long)blockIdx.x * VWG + threadIdx.x;
+    if (j >= d.N) return;
+    const int ix = (int)(j % d.nx), iy = (int)((j / d.nx) % d.ny), iz = (int)(j / ((long long)d.nx * d.ny));
+    auto V = [&](int x, int y, int z) -> double {
+        if (y < 0 || y >= d.ny) return 0.0;  // wall layers
+        x = wrapi(x, d.nx);
+        z = wrapi(z, d.nz);
+        return vin[x + (long long)d.nx * (y + (long long)d.ny * z)];
+    };
+    auto Uv = [&](int x, int y, int z) -> double {
+        x = wrapi(x, d.nx);
+        z = wrapi(z, d.nz);
+        return (double)u[x + (long long)d.nx * (y + (long long)d.ny * z)];
+    };
+    constexpr int ex = D == 0, ey = D == 1, ez = D == 2;
+    // flux through the face between (bx,by,bz) - e_D and (bx,by,bz)
+    auto face = [&](int bx, int by, int bz) -> double {
+        if (D == 1 && (by == 0 || by == d.ny)) return 0.0;  // a wall node on one side
+        const int ax = bx - ex, ay = by - ey, az = bz - ez;
+        const double uc = (Uv(bx, by, bz) + Uv(ax, ay, az)) / 2.0;
+        const int isgn = uc <= 0.0 ? 0 : 1;
+        const int px = isgn ? ax : bx, py = isgn ? ay : by, pz = isgn ? az : bz;
+        const double c = V(px, py, pz);
+        if (c >= 1.0 - 1.0e-6 || c <= 1.0e-6) return c * uc * (double)d.m * d.dt;
+        // normal3d_st1/st2 at the upwind node: 3x3x3 neighbourhood
+        double v[3][3][3];
+#pragma unroll
+        for (int c2 = 0; c2 < 3; c2++)
+#pragma unroll
+            for (int b2 = 0; b2 < 3; b2++)
+#pragma unroll
+                for (int a2 = 0; a2 < 3; a2++) v[c2][b2][a2] = V(px + a2 - 1, py + b2 - 1, pz + c2 - 1);
+        const double q = 0.25 / d.dx;
+        // cube differences at the corner node (1+a, 1+b, 1+c) of v
+        auto n2 = [&](int dir, int a, int b, int cc) -> double {
+            const int X = 1 + a, Y = 1 + b, Z = 1 + cc;
+            if (dir == 0)
+                return q * (v[Z][Y][X] - v[Z][Y][X - 1] + v[Z][Y - 1][X] - v[Z][Y - 1][X - 1] + v[Z - 1][Y][X] -
+                            v[Z - 1][Y][X - 1] + v[Z - 1][Y - 1][X] - v[Z - 1][Y - 1][X - 1]);
+            if (dir == 1)
+                return q * (v[Z][Y][X] - v[Z][Y - 1][X] + v[Z][Y][X - 1] - v[Z][Y - 1][X - 1] + v[Z - 1][Y][X] -
+                            v[Z - 1][Y - 1][X] + v[Z - 1][Y][X - 1] - v[Z - 1][Y - 1][X - 1]);
+            return q * (v[Z][Y][X] - v[Z - 1][Y][X] + v[Z][Y][X - 1] - v[Z - 1][Y][X - 1] + v[Z][Y - 1][X] -
+                        v[Z - 1][Y - 1][X] + v[Z][Y - 1][X - 1] - v[Z - 1][Y - 1][X - 1]);
+        };
+        auto n1 = [&](int dir) -> double {
+            return 0.125 * (n2(dir, 0, 0, 0) + n2(dir, 0, 0, 1) + n2(dir, 1, 0, 0) + n2(dir, 1, 0, 1) +
+                            n2(dir, 0, 1, 0) + n2(dir, 0, 1, 1) + n2(dir, 1, 1, 0) + n2(dir, 1, 1, 1));
+        };
+        const double n1x = n1(0), n1y = n1(1), n1z = n1(2);
+        const double nn = fabs(n1x) + fabs(n1y) + fabs(n1z);
+        if (nn <= 1.0e-6) return c * uc * (double)d.m * d.dt;
+        const double alpha = v[1 - ez][1 - ey][1 - ex] <= v[1 + ez][1 + ey][1 + ex] ? 1.0 : -1.0;
+        const double f = thinc_flux(c, alpha, uc, isgn, d.m, d.dt, d.dx);
+        const double weight = fabs(D == 0 ? n1x : (D == 1 ? n1y : n1z)) / nn;
+        return f * weight + c * uc * (double)d.m * d.dt * (1.0 - weight);
+    };
+    const double lo = face(ix, iy, iz), hi = face(ix + ex, iy + ey, iz + ez);
+    vout[j] = vin[j] - (hi - lo) / d.dx;
+}
+
+FtGeom ft_geom(const lbmibm_ctx *ctx) {
+    FtGeom g;
+    g.nx = ctx->nxp; g.ny = ctx->nyp; g.nz = ctx->nzp;
+    g.np = ctx->np;
+    g.dx = ctx->p.dx;
+    g.lx = ctx->p.lx * ctx->p.dx;
+    g.lz = ctx->p.lz * ctx->p.dx;
+    return g;
+}
+
+inline unsigned blocks_of(long long n) { return (unsigned)((n + VWG - 1) / VWG); }
+
+long long sor_row_threads(const FtGeom &g) { return (long long)((g.nx + 1) / 2) * g.ny * g.nz; }
+long long sor_buff_threads(const FtGeom &g) {
+    return 4LL * (g.ny - 4) * g.nz + 4LL * (g.nx - 4) * (g.ny - 4);
+}
+
+int expand_pow2(long long n) {  // etc.cu:425-446
+    int cnt = 0;
+    while (n > 1) {
+        if (n % 2 == 1) n++;
+        n /= 2;
+        cnt++;
+    }
+    return cnt;
+}
+
+}  // namespace
+
+lbmibm_status vof_alloc(lbmibm_ctx *ctx) {
+    if (ctx->vof[0]) return LBMIBM_OK;
+    const FtGeom g = ft_geom(ctx);
+    const long long nb_rows = blocks_of(sor_row_threads(g)), nb_buff = blocks_of(sor_buff_threads(g));
+    ctx->ft_nparts = (int)(2 * (nb_rows + nb_buff));
+    const long long nparts = std::max<long long>(ctx->ft_nparts, blocks_of(g.np));
+    lbmibm_status s = dev_alloc(ctx, (void **)&ctx->vof[0], ctx->N * sizeof(double));
+    if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->vof[1], ctx->N * sizeof(double));
+    if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->ft_g, 3 * g.np * sizeof(unsigned long long));
+    if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->ft_x, g.np * sizeof(double));
+    if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->ft_b, g.np * sizeof(double));
+    if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->ft_part, nparts * sizeof(double));
+    if (s == LBMIBM_OK) s = dev_alloc(ctx, &ctx->ft_ctl, sizeof(SorCtl));
+    if (s != LBMIBM_OK) return s;
+    for (int k = 0; k < 2; k++) {
+        hipLaunchKernelGGL(k_vfill<double>, dim3(blocks_of(ctx->N)), dim3(VWG), 0, ctx->stream, ctx->vof[k],
+                           (size_t)ctx->N, 0.0);
+        LBMIBM_LAUNCHED(ctx, "k_vfill");
+    }
+    ctx->vof_cur = 0;
+    return LBMIBM_OK;
+}
+
+lbmibm_status vof_front_tracking(lbmibm_ctx *ctx, int *iterations, int *attempts) {
+    if (iterations) *iterations = 0;
+    if (attempts) *attempts = 0;
+    lbmibm_status s = vof_alloc(ctx);
+    if (s != LBMIBM_OK) return s;
+    const FtGeom g = ft_geom(ctx);
+    hipStream_t st = ctx->stream;
+    unsigned long long *G = ctx->ft_g;
+    hipLaunchKernelGGL(k_vfill<unsigned long long>, dim3(blocks_of(3 * g.np)), dim3(VWG), 0, st, G,
+                       (size_t)(3 * g.np), 0ull);
+    LBMIBM_LAUNCHED(ctx, "k_vfill");
+    if (ctx->nt > 0) {
+        hipLaunchKernelGGL(k_ft_spread, dim3(blocks_of(64LL * ctx->nt)), dim3(VWG), 0, st, (const double *)ctx->xs,
+                           ctx->nv, ctx->ele, ctx->nt, g, G);
+        LBMIBM_LAUNCHED(ctx, "k_ft_spread");
+    }
+    hipLaunchKernelGGL(k_ft_fold, dim3(blocks_of(g.np)), dim3(VWG), 0, st, g, G);
+    LBMIBM_LAUNCHED(ctx, "k_ft_fold");
+    hipLaunchKernelGGL(k_ft_rhs, dim3(blocks_of(g.np)), dim3(VWG), 0, st, g, (const unsigned long long *)G, ctx->ft_b,
+                       ctx->ft_part);
+    LBMIBM_LAUNCHED(ctx, "k_ft_rhs");
+    // per-block sums of b^2 are in ft_part until k_ft_start adds them up
+    const int nb_b = (int)blocks_of(g.np);
+    const unsigned nb_rows = blocks_of(sor_row_threads(g)), nb_buff = blocks_of(sor_buff_threads(g));
+    double *part = ctx->ft_part;
+    double *p_r0 = part, *p_b0 = part + nb_rows, *p_r1 = part + nb_rows + nb_buff, *p_b1 = p_r1 + nb_rows;
+    SorCtl *ctl = (SorCtl *)ctx->ft_ctl;
+    const double a = (1.0 / (g.dx * g.dx)) / (-6.0 / (g.dx * g.dx));  // fronttrack.cu:97-110, 294-311
+    const long long mn = 1LL << expand_pow2(g.np);
+    const double npad = (double)(mn - g.np);
+    // b.b of the first start, kept for the restarts (b does not change)
+    hipLaunchKernelGGL(k_ft_start, dim3(1), dim3(VWG), 0, st, ctl, (const double *)part, nb_b, 0.0);
+    LBMIBM_LAUNCHED(ctx, "k_ft_start");
+    SorCtl h{};
+    LBMIBM_HIP(ctx, hipMemcpyAsync(&h, ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, st));
+    LBMIBM_HIP(ctx, hipStreamSynchronize(st));
+    const double bb = h.bb;
+    int cnt = 0;
+    for (;;) {
+        hipLaunchKernelGGL(k_vfill<double>, dim3(blocks_of(g.np)), dim3(VWG), 0, st, ctx->ft_x, (size_t)g.np,
+                           (double)cnt);
+        LBMIBM_LAUNCHED(ctx, "k_vfill");
+        h = SorCtl{};
+        h.xpad = (double)cnt;
+        h.bb = bb;
+        LBMIBM_HIP(ctx, hipMemcpyAsync(ctl, &h, sizeof(SorCtl), hipMemcpyHostToDevice, st));
+        int batch = 16;
+        int issued = 0;
+        for (;;) {
+            for (int k = 0; k < batch; k++) {
+                hipLaunchKernelGGL(k_sor_rows<0>, dim3(nb_rows), dim3(VWG), 0, st, g, ctx->ft_x,
+                                   (const double *)ctx->ft_b, (const SorCtl *)ctl, p_r0, a);
+                hipLaunchKernelGGL(k_sor_buff<0>, dim3(nb_buff), dim3(VWG), 0, st, g, ctx->ft_x, (const SorCtl *)ctl,
+                                   p_b0);
+                hipLaunchKernelGGL(k_sor_rows<1>, dim3(nb_rows), dim3(VWG), 0, st, g, ctx->ft_x,
+                                   (const double *)ctx->ft_b, (const SorCtl *)ctl, p_r1, a);
+                hipLaunchKernelGGL(k_sor_buff<1>, dim3(nb_buff), dim3(VWG), 0, st, g, ctx->ft_x, (const SorCtl *)ctl,
+                                   p_b1);
+                hipLaunchKernelGGL(k_sor_check, dim3(1), dim3(VWG), 0, st, ctl, (const double *)part,
+                                   (int)(2 * (nb_rows + nb_buff)), npad, ctx->p.sor_itrmax, ctx->p.sor_residual);
+            }
+            LBMIBM_LAUNCHED(ctx, "k_sor_*");
+            issued += batch;
+            LBMIBM_HIP(ctx, hipMemcpyAsync(&h, ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, st));
+            LBMIBM_HIP(ctx, hipStreamSynchronize(st));
+            if (h.done) break;
+            if (issued > ctx->p.sor_itrmax + 1) {  // the device test stops by ITRMAX
+                ctx->err = "front tracking: SOR status did not settle";
+                return LBMIBM_ERR_STATE;
+            }
+            batch = std::min(batch * 2, 256);
+        }
+        cnt++;
+        if (cnt > FT_MAX_ATTEMPTS) {  // fronttrack.cu:718: error(5) on the eleventh start
+            if (attempts) *attempts = cnt;
+            ctx->err = "front tracking: the red-black SOR did not converge from 10 starts (error 5)";
+            return LBMIBM_ERR_VOF;
+        }
+        if (h.done == 1) break;
+    }
+    hipLaunchKernelGGL(k_ft_extract, dim3(blocks_of(ctx->N)), dim3(VWG), 0, st, g, (const double *)ctx->ft_x,
+                       ctx->vof[ctx->vof_cur], ctx->nx, ctx->ny, ctx->N);
+    LBMIBM_LAUNCHED(ctx, "k_ft_extract");
+    if (iterations) *iterations = h.conv_iter;
+    if (attempts) *attempts = cnt;
+    return LBMIBM_OK;
+}
+
+lbmibm_status vof_thinc(lbmibm_ctx *ctx) {
+    lbmibm_status s = vof_alloc(ctx);
+    if (s != LBMIBM_OK) return s;
+    VofDims d{ctx->nx, ctx->ny, ctx->nz, ctx->N, ctx->p.m, ctx->p.dt, ctx->p.dx};
+    const unsigned nb = blocks_of(ctx->N);
+    const size_t N = (size_t)ctx->N;
+    auto sweep = [&](auto kern, auto u) {
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(VWG), 0, ctx->stream, d, (const double *)ctx->vof[ctx->vof_cur],
+                           ctx->vof[ctx->vof_cur ^ 1], u);
+        ctx->vof_cur ^= 1;
+    };
+    if (ctx->prec == LBMIBM_F64) {
+        const double *U = (const double *)ctx->U;
+        sweep(k_thinc<double, 0>, U);
+        sweep(k_thinc<double, 1>, U + N);
+        sweep(k_thinc<double, 2>, U + 2 * N);
+    } else {
+        const float *U = (const float *)ctx->U;
+        sweep(k_thinc<float, 0>, U);
+        sweep(k_thinc<float, 1>, U + N);
+        sweep(k_thinc<float, 2>, U + 2 * N);
+    }
+    LBMIBM_LAUNCHED(ctx, "k_thinc");
+    return LBMIBM_OK;
+}
+
+}  // namespace lbmibm

@@ -23,7 +23,7 @@ ADJ_POPULATIONS, ADJ_VELOCITY, ADJ_LATTICE_FORCE, ADJ_POSITIONS, ADJ_NODAL_FORCE
 ADJ_GRAD_BODY_FORCE, ADJ_GRAD_FORCE_SCALE = 5, 6
 STATUS = {
     0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_BICG_BREAKDOWN", 4: "ERR_BICG_NOCONV",
-    5: "ERR_VOLUME", 6: "ERR_TOPOLOGY", 7: "ERR_STATE",
+    5: "ERR_VOLUME", 6: "ERR_TOPOLOGY", 7: "ERR_STATE", 8: "ERR_VOF",
 }
 
 
@@ -46,7 +46,7 @@ class Params(C.Structure):
         ("b", C.c_double), ("m", C.c_int), ("c", C.c_double), ("rgs", C.c_double), ("br", C.c_double),
         ("itrmax", C.c_int), ("residual", C.c_double), ("runge_kutta", C.c_int), ("neohookean", C.c_int),
         ("volume_correction", C.c_int), ("precision", C.c_int), ("store_macros", C.c_int),
-        ("fileoutput", C.c_int),
+        ("fileoutput", C.c_int), ("vof_period", C.c_int), ("sor_itrmax", C.c_int), ("sor_residual", C.c_double),
     ]
 
 
@@ -93,6 +93,10 @@ SIGNATURES = {
     "lbmibm_write_prof": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "lbmibm_write_cell": (C.c_int, [C.c_void_p, C.c_char_p]),
     "lbmibm_output_data": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, c_int_p]),
+    "lbmibm_front_tracking": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
+    "lbmibm_thinc_wlic": (C.c_int, [C.c_void_p]),
+    "lbmibm_download_vf": (C.c_int, [C.c_void_p, c_double_p]),
+    "lbmibm_upload_vf": (C.c_int, [C.c_void_p, c_double_p]),
     "lbmibm_set_body_force_field": (C.c_int, [C.c_void_p] + [c_double_p] * 3),
     "lbmibm_adjoint_reset": (C.c_int, [C.c_void_p]),
     "lbmibm_adjoint_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),

@@ -24,7 +24,8 @@ PRECISIONS = {"f64": N.F64, "f32": N.F32, "f16": N.F16}
 def make_params(lx, ly, lz, *, precision="f32", tau=None, df=None, dx=1.0, cr=1.0, uc=1.0, p=0.0,
                 e=(1.0, 0.0, 0.0), force_modulated=False, flowstart=0, freq=1.0, b=0.0, m=1, c=100.0,
                 rgs=1.0, br=0.0, itrmax=1000, residual=1e-10, runge_kutta=4, neohookean=False,
-                volume_correction=True, store_macros=False, fileoutput=0, periodic_y=False, lambda1=1.0):
+                volume_correction=True, store_macros=False, fileoutput=0, periodic_y=False, lambda1=1.0,
+                vof_period=0, sor_itrmax=None, sor_residual=None):
     P = N.Params()
     N.lib().lbmibm_default_params(C.byref(P), int(lx), int(ly), int(lz))
     if df is None:
@@ -40,6 +41,9 @@ def make_params(lx, ly, lz, *, precision="f32", tau=None, df=None, dx=1.0, cr=1.
     P.neohookean, P.volume_correction = int(neohookean), int(volume_correction)
     P.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     P.store_macros, P.fileoutput = int(store_macros), int(fileoutput)
+    P.vof_period = int(vof_period)
+    P.sor_itrmax = int(itrmax if sor_itrmax is None else sor_itrmax)
+    P.sor_residual = float(residual if sor_residual is None else sor_residual)
     return P
 
 
@@ -225,6 +229,23 @@ class Simulation:
         f = np.array([fnum], np.int32)
         self._ck(self.L.lbmibm_output_data(self.ctx, os.fsencode(directory), int(iter), N.ip(f)))
         return int(f[0])
+
+    def front_tracking(self):
+        """vof_init / front_tracking; returns (SOR iterations, starts used)."""
+        its, att = C.c_int(0), C.c_int(0)
+        self._ck(self.L.lbmibm_front_tracking(self.ctx, C.byref(its), C.byref(att)))
+        return its.value, att.value
+
+    def thinc_wlic(self):
+        self._ck(self.L.lbmibm_thinc_wlic(self.ctx))
+
+    def vf(self):
+        a = np.zeros(self.np_)
+        self._ck(self.L.lbmibm_download_vf(self.ctx, N.dp(a)))
+        return a
+
+    def set_vf(self, vf):
+        self._ck(self.L.lbmibm_upload_vf(self.ctx, N.dp(np.ascontiguousarray(vf, np.float64))))
 
     def comp_mask(self):
         """COMP nodes of the padded reference lattice (init1.cu:164-178)."""

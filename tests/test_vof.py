@@ -275,3 +275,138 @@ def test_front_tracking_indicator_of_a_sphere():
     o.P.sor_itrmax = 3
     rc, _, att = o.front_tracking()
     assert rc == 5 and att == 11
+
+
+# ---- GPU: the library against the oracle ------------------------------------
+
+GL = (24, 20, 22)
+VOF_KW = dict(sor_itrmax=6000, sor_residual=1e-9)
+
+
+def _two_cells():
+    a = _sphere((0.37, -0.61, 0.23), 3.6)
+    # one mesh for both (the file layouts assume it): the second copy just
+    # inside the +x and -z ends, so that its delta stencils reach the halo
+    shift = np.array([GL[0] / 2 - 3.9, 0.4, -GL[2] / 2 + 3.85])
+    out = dict(voff=np.array([0, len(a["x"]), 2 * len(a["x"])], np.int32),
+               eoff=np.array([0, len(a["ele"]) // 3, 2 * (len(a["ele"]) // 3)], np.int32),
+               ele=np.r_[a["ele"], a["ele"] + len(a["x"])].astype(np.int32))
+    for k, sh in zip("xyz", shift):
+        out[k] = np.r_[a[k], a[k] - a[k].mean() + sh]
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_front_tracking_matches_the_oracle(prec):
+    print(f"[front tracking] prec={prec}", flush=True)
+    cells = _two_cells()
+    o = oracle_of(*GL, cells=cells, **VOF_KW)
+    g = gpu_of(*GL, prec, cells=cells, vof_period=4, **VOF_KW)
+    rc, its, att = o.front_tracking()
+    assert rc == 0
+    gits, gatt = g.front_tracking()
+    assert gatt == att and abs(gits - its) <= 1, (gits, its)
+    vo, vg = o.vf, g.vf()
+    keep = o.bc != 2
+    assert rel(vg[keep], vo[keep]) < 1e-8
+    assert not vg[o.bc == 2].any()
+    # the same again: deterministic
+    g.front_tracking()
+    assert np.array_equal(g.vf(), vg)
+
+
+@pytest.mark.gpu
+def test_front_tracking_failure_after_ten_starts():
+    from preprocess_src_amd import native as N
+    g = gpu_of(*GL, "f64", cells=_two_cells(), vof_period=4, sor_itrmax=2, sor_residual=1e-9)
+    with pytest.raises(N.LbmibmError) as e:
+        g.front_tracking()
+    assert e.value.code == 8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_thinc_wlic_matches_the_oracle(prec):
+    print(f"[thinc] prec={prec}", flush=True)
+    cells = _two_cells()
+    o = oracle_of(*GL, cells=cells, m=2, **VOF_KW)
+    g = gpu_of(*GL, prec, cells=cells, m=2, vof_period=4, **VOF_KW)
+    global LX, LY, LZ
+    saved = (LX, LY, LZ)
+    LX, LY, LZ = GL
+    try:
+        vf = _smooth_field(o, 7, -0.02, 1.02)
+        vel = _velocity(o, 8, amp=0.15)
+    finally:
+        LX, LY, LZ = saved
+    if prec == "f32":  # the library keeps um..wm in the lattice precision
+        vel = [v.astype(np.float32).astype(np.float64) for v in vel]
+    o.vf[:] = vf
+    o.um[:], o.vm[:], o.wm[:] = vel
+    g.set_vf(vf)
+    g.set_macros(*vel)
+    for _ in range(2):
+        o.thinc_wlic()
+        g.thinc_wlic()
+    keep = o.bc != 2
+    assert rel(g.vf()[keep], o.vf[keep]) < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-7), ("f32", 2e-5)])
+def test_steps_with_a_viscosity_contrast_match_the_oracle(prec, tol):
+    print(f"[vof steps] prec={prec}", flush=True)
+    cells = _two_cells()
+    kw = dict(tau=0.9, p=5e-6, b=5e-3, lambda1=5.0, vof_period=2, **VOF_KW)
+    o = oracle_of(*GL, cells=cells, **kw)
+    g = gpu_of(*GL, prec, cells=cells, **kw)
+    o.volume_init()
+    g.volume_init()
+    X, Y, Z = o.node_coords()
+    vel = 2e-3 * Y / (GL[1] / 2)
+    vel[o.bc == 2] = 0.0
+    o.init_fluid(None, vel, None, None)
+    g.set_equilibrium(None, vel, None, None)
+    assert o.front_tracking()[0] == 0
+    g.front_tracking()
+    for it in range(1, 5):
+        assert o.step(it)[0] == 0
+        g.step(it)
+    comp = o.bc == 0
+    assert rel(g.vf()[comp], o.vf[comp]) < max(tol, 1e-6)
+    fo, fg = o.fn.reshape(-1, Q)[comp], g.populations().reshape(-1, Q)[comp]
+    assert rel(fg - 1.0 / np.array([3] + [18] * 6 + [36] * 12), fo - 1.0 / np.array([3] + [18] * 6 + [36] * 12)) < tol * 50
+    so, sg = o.cells, g.cells_state()
+    for k in ("x", "y", "z"):
+        assert np.max(np.abs(sg[k] - so[k])) < tol * 10, k
+
+
+@pytest.mark.gpu
+def test_vof_arguments_and_states_are_checked(tmp_path):
+    from preprocess_src_amd import native as N
+    cells = _two_cells()
+    with pytest.raises(N.LbmibmError):
+        gpu_of(*GL, "f64", cells=cells, lambda1=3.0)  # a contrast needs the indicator
+    with pytest.raises(N.LbmibmError):
+        gpu_of(*GL, "f64", cells=cells, vof_period=2, periodic_y=True)
+    with pytest.raises(N.LbmibmError):
+        gpu_of(*GL, "f64", vof_period=2)  # no cells
+    plain = gpu_of(*GL, "f64", cells=cells)
+    with pytest.raises(N.LbmibmError):
+        plain.front_tracking()
+    assert not plain.vf().any()
+    g = gpu_of(*GL, "f64", cells=cells, vof_period=2, **VOF_KW)
+    with pytest.raises(N.LbmibmError):
+        g.adjoint_reset()
+        g.adj_lbm_d3q19(1)
+    g.front_tracking()
+    path = str(tmp_path / "restart.bin")
+    g.write_restart(path, 0, 0)
+    from oracle import reffiles as RF
+    nc, V, E = 2, len(cells["x"]) // 2, len(cells["ele"]) // 6
+    r = RF.parse_restart(open(path, "rb").read(), g.np_, nc, V, E)
+    assert np.array_equal(r["vf"], g.vf()) and r["vf"].any() and not r["vf2"].any()
+    h = gpu_of(*GL, "f64", cells=cells, vof_period=2, **VOF_KW)
+    h.read_restart(path)
+    assert np.array_equal(h.vf(), g.vf())
