@@ -10,11 +10,13 @@ seeded mesh mix of 50% 642-, 40% 2562- and 10% 10242-vertex meshes, radius
 populations stored as deviations, whole steps replayed from a hipGraph.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|f64|f16]
-                    [--workload suspension|tgv|adjoint] [--cpu-seconds S] [--no-cpu]
+                    [--workload suspension|tgv|adjoint|vof] [--cpu-seconds S] [--no-cpu]
 
 --workload adjoint times the checkpointed wall-shear gradient on a 128x64x64
 channel with 32 RBCs (500 steps, last 200 averaged, checkpoints every 50)
 against the forward run and checks it against a central difference.
+--workload vof times front tracking (red-black SOR), THINC/WLIC and steps
+with a viscosity contrast on the default suspension.
 """
 from __future__ import annotations
 
@@ -254,13 +256,71 @@ def copy_calibration(nbytes=256 << 20, reps=3):
           flush=True)
 
 
+def vof_bench(a):
+    """The VOF indicator on the default suspension: one front tracking
+    (spread, divergence, red-black SOR to sor_residual), THINC/WLIC sweeps
+    and whole steps with a viscosity contrast (LAMBDA1 = 5), which run eagerly
+    (front tracking waits for the host)."""
+    from preprocess_src_amd.simulation import Simulation
+    shape, cells, params, desc, ht = suspension_workload()
+    lx, ly, lz = shape
+    n = lx * ly * lz
+    npad = (lx + 4) * (ly + 4) * (lz + 4)
+    kw = dict(params, lambda1=5.0, vof_period=a.vof_period, sor_itrmax=a.sor_itrmax, sor_residual=a.sor_residual)
+    sim = Simulation(lx, ly, lz, cells=cells, precision=a.precision, **kw)
+    sim.volume_init()
+    sim.synchronize()
+    print("[bench] front tracking", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    its, att = sim.front_tracking()
+    t_ft = time.perf_counter() - t0
+    print(f"[bench] front tracking {t_ft:.3f} s, {its} SOR iterations, {att} start(s)", file=sys.stderr, flush=True)
+    reps = 20
+    sim.thinc_wlic()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.thinc_wlic()
+    sim.synchronize()
+    t_th = (time.perf_counter() - t0) / reps
+    steps = min(a.steps, a.vof_period - 1)
+    sim.run(0, 2, graph=False)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.run(2, steps, graph=False)
+    sim.synchronize()
+    t_step = (time.perf_counter() - t0) / steps
+    vf = sim.vf()
+    sor_bytes = 32 * npad  # per iteration: x read (8 B) + b and x written on each colour's half, both colours
+    th_bytes = 3 * n * (16 + ARITH_BYTES[a.precision])  # per call: 3 sweeps of vf read + write + one velocity
+    out = {
+        "workload": "vof",
+        "config": {"workload": desc + ", LAMBDA1=5", "lattice": [lx, ly, lz], "precision": a.precision,
+                   "vof_period": a.vof_period, "sor_itrmax": a.sor_itrmax, "sor_residual": a.sor_residual},
+        "front_tracking_ms": round(1e3 * t_ft, 3),
+        "sor_iterations": its, "sor_starts": att,
+        "sor_us_per_iteration": round(1e6 * t_ft / max(its, 1), 3),
+        "sor_roofline": {"bound": "hbm", "achieved": round(sor_bytes * its / t_ft / 1e9, 1),
+                         "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": round(sor_bytes * its / t_ft / HBM_PEAK, 4), "bytes_per_iteration": sor_bytes},
+        "thinc_ms": round(1e3 * t_th, 4),
+        "thinc_roofline": {"bound": "hbm", "achieved": round(th_bytes / t_th / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                           "unit": "GB/s", "frac": round(th_bytes / t_th / HBM_PEAK, 4), "bytes_per_call": th_bytes},
+        "step_ms_with_vof": round(1e3 * t_step, 4), "steps": steps,
+        "indicator_volume": round(float(vf.reshape(lz + 4, ly + 4, lx + 4)[2:-2, 2:-2, 2:-2].sum()), 1),
+        "cell_volume": round(float(ht * n), 1),
+    }
+    print(json.dumps(out), flush=True)
+    sim.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64", "f16"])
-    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv", "adjoint", "copy"],
+    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv", "adjoint", "copy", "vof"],
                     help="copy: a plain 256 MiB device copy, the calibration run of the FETCH_SIZE / "
                          "WRITE_SIZE counters")
     ap.add_argument("--stage-steps", type=int, default=6)
@@ -268,6 +328,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=500, help="steps of the adjoint workload (window 40%%)")
+    ap.add_argument("--vof-period", type=int, default=1000, help="K of the vof workload")
+    ap.add_argument("--sor-itrmax", type=int, default=200000)
+    ap.add_argument("--sor-residual", type=float, default=1e-10)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path is single-GPU, "
@@ -282,6 +345,9 @@ def main():
 
     if a.workload == "adjoint":
         adjoint_bench(a)
+        return
+    if a.workload == "vof":
+        vof_bench(a)
         return
     if a.workload == "suspension":
         shape, cells, params, desc, ht = suspension_workload()

@@ -288,8 +288,9 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     // and the slab's walls (CapsuleSHEARFLOW, fronttrack.cu:87-125)
     if (p->vof_period < 0 || (p->lambda1 != 1.0 && p->vof_period == 0)) return LBMIBM_ERR_ARG;
     if (p->vof_period > 0 && (p->periodic_y || !cells || cells->nc < 1 || p->sor_itrmax < 1 ||
-                              !(p->sor_residual > 0.0)))
-        return LBMIBM_ERR_ARG;
+                              !(p->sor_residual > 0.0) ||
+                              (long long)(p->lx + 4) * (p->ly + 4) * (p->lz + 4) >= (1LL << 31)))
+        return LBMIBM_ERR_ARG;  // the VOF kernels index the padded lattice in 32 bits
     const long long N = (long long)p->lx * p->ly * p->lz;
     if (N >= (1LL << 32)) return LBMIBM_ERR_ARG;
     int ndev = 0;

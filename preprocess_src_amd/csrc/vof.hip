@@ -34,6 +34,8 @@ namespace {
 constexpr double FT_UNIT = 17592186044416.0;  // 2^44 fixed-point units per 1/dx
 constexpr int VWG = 256;
 constexpr int FT_MAX_ATTEMPTS = 10;
+constexpr int SOR_CHUNKS = 4;  // x-pair chunks of VWG per work-group of k_sor_rows
+constexpr int CHECK_WG = 1024;
 
 struct FtGeom {
     int nx, ny, nz;  // padded sizes
@@ -184,28 +186,36 @@ __global__ void k_ft_start(SorCtl *__restrict__ ctl, const double *__restrict__ 
 
 // One colour of the COMP and WALL rows.  One lane per x-pair of a row: the
 // lane takes the node of colour C, so a wave reads and writes a contiguous
-// stretch of the row.
+// stretch of the row.  A work-group takes SOR_CHUNKS consecutive stretches
+// and the work-groups of one XCD a contiguous range of planes (their
+// neighbour planes stay in that XCD's L2); one partial sum of r^2 per
+// work-group.
 template <int C>
 __global__ __launch_bounds__(VWG) void k_sor_rows(FtGeom g, double *__restrict__ x, const double *__restrict__ b,
-                                                  const SorCtl *__restrict__ ctl, double *__restrict__ part, double a) {
+                                                  const SorCtl *__restrict__ ctl, double *__restrict__ part, double a,
+                                                  int nblocks) {
 #pragma clang fp contract(off)
     __shared__ double lds[VWG / 64];
     if (ctl->done) return;
-    const int nxh = (g.nx + 1) >> 1;
-    const long long t = (long long)blockIdx.x * VWG + threadIdx.x;
-    const long long row = t / nxh;
+    const int lb = xcd_block(nblocks);
+    if (lb >= nblocks) return;
+    const unsigned nxh = (unsigned)(g.nx + 1) >> 1, nyu = (unsigned)g.ny;
     double r2 = 0.0;
-    if (row < (long long)g.ny * g.nz) {
-        const int iy = (int)(row % g.ny), iz = (int)(row / g.ny);
-        const int ix = 2 * (int)(t % nxh) + ((C + iy + iz) & 1);
+    for (int k = 0; k < SOR_CHUNKS; k++) {
+    const unsigned t = ((unsigned)lb * SOR_CHUNKS + k) * VWG + threadIdx.x;
+    const unsigned row = t / nxh;
+    if (row < nyu * (unsigned)g.nz) {
+        const unsigned izu = row / nyu;
+        const int iy = (int)(row - izu * nyu), iz = (int)izu;
+        const int ix = 2 * (int)(t - row * nxh) + ((C + iy + iz) & 1);
         if (ix < g.nx) {
-            const long long i = ix + (long long)g.nx * row;
+            const unsigned i = (unsigned)ix + (unsigned)g.nx * row;
             const bool wall = iy < 2 || iy > g.ny - 3;
             const bool comp = !wall && ix >= 2 && ix <= g.nx - 3 && iz >= 2 && iz <= g.nz - 3;
             if (comp || wall) {
                 double sum, bi = 0.0;
                 if (comp) {
-                    const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
+                    const unsigned nx = g.nx, nxy = (unsigned)g.nx * g.ny;
                     sum = 0.0;
                     sum += a * x[i - nxy];
                     sum += a * x[i - nx];
@@ -220,12 +230,13 @@ __global__ __launch_bounds__(VWG) void k_sor_rows(FtGeom g, double *__restrict__
                 }
                 const double r = bi - sum;
                 x[i] = x[i] + 1.3 * r;
-                r2 = r * r;
+                r2 += r * r;
             }
         }
     }
+    }
     const double s = block_sum(r2, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (threadIdx.x == 0) part[lb] = s;
 }
 
 // One colour of the BUFF rows x_i - x_image = 0: the x halo (all z) then
@@ -272,10 +283,10 @@ __global__ __launch_bounds__(VWG) void k_sor_buff(FtGeom g, double *__restrict__
 
 // Residual norm of one iteration (the padding rows, all equal, contribute
 // npad x_pad^2) and the stop test of matrix_solver.cu:198-205.
-__global__ void k_sor_check(SorCtl *__restrict__ ctl, const double *__restrict__ part, int nparts, double npad,
-                            int itrmax, double residual) {
+__global__ __launch_bounds__(CHECK_WG) void k_sor_check(SorCtl *__restrict__ ctl, const double *__restrict__ part,
+                                                         int nparts, double npad, int itrmax, double residual) {
 #pragma clang fp contract(off)
-    __shared__ double lds[VWG / 64];
+    __shared__ double lds[CHECK_WG / 64];
     if (ctl->done) return;
     double s = 0.0;
     for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += part[k];
@@ -328,26 +339,26 @@ __device__ double thinc_flux(double c, double alpha, double uc, int isgn, int m,
     return 0.5 * (uc * (double)m * dt - alpha * dx / beta * log(a4 / a5));
 }
 
-// One directional sweep D of wlic1d4md_st1..st3 (thincwlic.cu:108-313):
-// each COMP node evaluates the fluxes of its two faces along D (the face
-// below node i is the reference's flux[i]) from vin and writes vout.
+// wlic1d4md_st1 (thincwlic.cu:108-214) of sweep D: the flux through the
+// face below each node along D (the reference's flux[i]), from vin.
 template <typename T, int D>
-__global__ __launch_bounds__(VWG) void k_thinc(VofDims d, const double *__restrict__ vin, double *__restrict__ vout,
-                                               const T *__restrict__ u) {
+__global__ __launch_bounds__(VWG) void k_thinc_flux(VofDims d, const double *__restrict__ vin,
+                                                    double *__restrict__ flux, const T *__restrict__ u) {
 #pragma clang fp contract(off)
-    const long long j = (long long)blockIdx.x * VWG + threadIdx.x;
-    if (j >= d.N) return;
-    const int ix = (int)(j % d.nx), iy = (int)((j / d.nx) % d.ny), iz = (int)(j / ((long long)d.nx * d.ny));
+    const unsigned j = blockIdx.x * VWG + threadIdx.x;
+    if (j >= (unsigned)d.N) return;
+    const unsigned r = j / (unsigned)d.nx, izu = r / (unsigned)d.ny;
+    const int ix = (int)(j - r * (unsigned)d.nx), iy = (int)(r - izu * (unsigned)d.ny), iz = (int)izu;
     auto V = [&](int x, int y, int z) -> double {
         if (y < 0 || y >= d.ny) return 0.0;  // wall layers
         x = wrapi(x, d.nx);
         z = wrapi(z, d.nz);
-        return vin[x + (long long)d.nx * (y + (long long)d.ny * z)];
+        return vin[(unsigned)x + (unsigned)d.nx * ((unsigned)y + (unsigned)d.ny * (unsigned)z)];
     };
     auto Uv = [&](int x, int y, int z) -> double {
         x = wrapi(x, d.nx);
         z = wrapi(z, d.nz);
-        return (double)u[x + (long long)d.nx * (y + (long long)d.ny * z)];
+        return (double)u[(unsigned)x + (unsigned)d.nx * ((unsigned)y + (unsigned)d.ny * (unsigned)z)];
     };
     constexpr int ex = D == 0, ey = D == 1, ez = D == 2;
     // flux through the face between (bx,by,bz) - e_D and (bx,by,bz)
@@ -392,8 +403,26 @@ __global__ __launch_bounds__(VWG) void k_thinc(VofDims d, const double *__restri
         const double weight = fabs(D == 0 ? n1x : (D == 1 ? n1y : n1z)) / nn;
         return f * weight + c * uc * (double)d.m * d.dt * (1.0 - weight);
     };
-    const double lo = face(ix, iy, iz), hi = face(ix + ex, iy + ey, iz + ez);
-    vout[j] = vin[j] - (hi - lo) / d.dx;
+    flux[j] = face(ix, iy, iz);
+}
+
+// wlic1d4md_st2 (thincwlic.cu:217-250) of sweep D; the face above the last
+// row along y is a wall face (flux 0), along x and z it is the periodic
+// image's lower face.
+template <int D>
+__global__ __launch_bounds__(VWG) void k_thinc_update(VofDims d, const double *__restrict__ vin,
+                                                      const double *__restrict__ flux, double *__restrict__ vout) {
+#pragma clang fp contract(off)
+    const unsigned j = blockIdx.x * VWG + threadIdx.x;
+    if (j >= (unsigned)d.N) return;
+    const unsigned r = j / (unsigned)d.nx, izu = r / (unsigned)d.ny;
+    const int ix = (int)(j - r * (unsigned)d.nx), iy = (int)(r - izu * (unsigned)d.ny), iz = (int)izu;
+    const unsigned nxy = (unsigned)d.nx * (unsigned)d.ny;
+    double hi;
+    if (D == 0) hi = flux[ix == d.nx - 1 ? j + 1 - (unsigned)d.nx : j + 1];
+    else if (D == 1) hi = iy == d.ny - 1 ? 0.0 : flux[j + (unsigned)d.nx];
+    else hi = flux[iz == d.nz - 1 ? j - nxy * (unsigned)(d.nz - 1) : j + nxy];
+    vout[j] = vin[j] - (hi - flux[j]) / d.dx;
 }
 
 FtGeom ft_geom(const lbmibm_ctx *ctx) {
@@ -409,6 +438,9 @@ FtGeom ft_geom(const lbmibm_ctx *ctx) {
 inline unsigned blocks_of(long long n) { return (unsigned)((n + VWG - 1) / VWG); }
 
 long long sor_row_threads(const FtGeom &g) { return (long long)((g.nx + 1) / 2) * g.ny * g.nz; }
+int sor_row_blocks(const FtGeom &g) {
+    return (int)((sor_row_threads(g) + (long long)VWG * SOR_CHUNKS - 1) / ((long long)VWG * SOR_CHUNKS));
+}
 long long sor_buff_threads(const FtGeom &g) {
     return 4LL * (g.ny - 4) * g.nz + 4LL * (g.nx - 4) * (g.ny - 4);
 }
@@ -428,7 +460,7 @@ int expand_pow2(long long n) {  // etc.cu:425-446
 lbmibm_status vof_alloc(lbmibm_ctx *ctx) {
     if (ctx->vof[0]) return LBMIBM_OK;
     const FtGeom g = ft_geom(ctx);
-    const long long nb_rows = blocks_of(sor_row_threads(g)), nb_buff = blocks_of(sor_buff_threads(g));
+    const long long nb_rows = sor_row_blocks(g), nb_buff = blocks_of(sor_buff_threads(g));
     ctx->ft_nparts = (int)(2 * (nb_rows + nb_buff));
     const long long nparts = std::max<long long>(ctx->ft_nparts, blocks_of(g.np));
     lbmibm_status s = dev_alloc(ctx, (void **)&ctx->vof[0], ctx->N * sizeof(double));
@@ -471,7 +503,8 @@ lbmibm_status vof_front_tracking(lbmibm_ctx *ctx, int *iterations, int *attempts
     LBMIBM_LAUNCHED(ctx, "k_ft_rhs");
     // per-block sums of b^2 are in ft_part until k_ft_start adds them up
     const int nb_b = (int)blocks_of(g.np);
-    const unsigned nb_rows = blocks_of(sor_row_threads(g)), nb_buff = blocks_of(sor_buff_threads(g));
+    const unsigned nb_rows = sor_row_blocks(g), nb_buff = blocks_of(sor_buff_threads(g));
+    const unsigned grid_rows = xcd_grid(nb_rows);
     double *part = ctx->ft_part;
     double *p_r0 = part, *p_b0 = part + nb_rows, *p_r1 = part + nb_rows + nb_buff, *p_b1 = p_r1 + nb_rows;
     SorCtl *ctl = (SorCtl *)ctx->ft_ctl;
@@ -498,15 +531,15 @@ lbmibm_status vof_front_tracking(lbmibm_ctx *ctx, int *iterations, int *attempts
         int issued = 0;
         for (;;) {
             for (int k = 0; k < batch; k++) {
-                hipLaunchKernelGGL(k_sor_rows<0>, dim3(nb_rows), dim3(VWG), 0, st, g, ctx->ft_x,
-                                   (const double *)ctx->ft_b, (const SorCtl *)ctl, p_r0, a);
+                hipLaunchKernelGGL(k_sor_rows<0>, dim3(grid_rows), dim3(VWG), 0, st, g, ctx->ft_x,
+                                   (const double *)ctx->ft_b, (const SorCtl *)ctl, p_r0, a, (int)nb_rows);
                 hipLaunchKernelGGL(k_sor_buff<0>, dim3(nb_buff), dim3(VWG), 0, st, g, ctx->ft_x, (const SorCtl *)ctl,
                                    p_b0);
-                hipLaunchKernelGGL(k_sor_rows<1>, dim3(nb_rows), dim3(VWG), 0, st, g, ctx->ft_x,
-                                   (const double *)ctx->ft_b, (const SorCtl *)ctl, p_r1, a);
+                hipLaunchKernelGGL(k_sor_rows<1>, dim3(grid_rows), dim3(VWG), 0, st, g, ctx->ft_x,
+                                   (const double *)ctx->ft_b, (const SorCtl *)ctl, p_r1, a, (int)nb_rows);
                 hipLaunchKernelGGL(k_sor_buff<1>, dim3(nb_buff), dim3(VWG), 0, st, g, ctx->ft_x, (const SorCtl *)ctl,
                                    p_b1);
-                hipLaunchKernelGGL(k_sor_check, dim3(1), dim3(VWG), 0, st, ctl, (const double *)part,
+                hipLaunchKernelGGL(k_sor_check, dim3(1), dim3(CHECK_WG), 0, st, ctl, (const double *)part,
                                    (int)(2 * (nb_rows + nb_buff)), npad, ctx->p.sor_itrmax, ctx->p.sor_residual);
             }
             LBMIBM_LAUNCHED(ctx, "k_sor_*");
@@ -542,21 +575,25 @@ lbmibm_status vof_thinc(lbmibm_ctx *ctx) {
     VofDims d{ctx->nx, ctx->ny, ctx->nz, ctx->N, ctx->p.m, ctx->p.dt, ctx->p.dx};
     const unsigned nb = blocks_of(ctx->N);
     const size_t N = (size_t)ctx->N;
-    auto sweep = [&](auto kern, auto u) {
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(VWG), 0, ctx->stream, d, (const double *)ctx->vof[ctx->vof_cur],
-                           ctx->vof[ctx->vof_cur ^ 1], u);
+    auto sweep = [&](auto kflux, auto kupd, auto u) {
+        const double *vin = ctx->vof[ctx->vof_cur];
+        hipLaunchKernelGGL(kflux, dim3(nb), dim3(VWG), 0, ctx->stream, d, vin, ctx->ft_x, u);
+        hipLaunchKernelGGL(kupd, dim3(nb), dim3(VWG), 0, ctx->stream, d, vin, (const double *)ctx->ft_x,
+                           ctx->vof[ctx->vof_cur ^ 1]);
         ctx->vof_cur ^= 1;
     };
+    // the flux of a sweep lives in the front-tracking iterate's array
+    // (np >= N doubles), which is free between front-tracking calls
     if (ctx->prec == LBMIBM_F64) {
         const double *U = (const double *)ctx->U;
-        sweep(k_thinc<double, 0>, U);
-        sweep(k_thinc<double, 1>, U + N);
-        sweep(k_thinc<double, 2>, U + 2 * N);
+        sweep(k_thinc_flux<double, 0>, k_thinc_update<0>, U);
+        sweep(k_thinc_flux<double, 1>, k_thinc_update<1>, U + N);
+        sweep(k_thinc_flux<double, 2>, k_thinc_update<2>, U + 2 * N);
     } else {
         const float *U = (const float *)ctx->U;
-        sweep(k_thinc<float, 0>, U);
-        sweep(k_thinc<float, 1>, U + N);
-        sweep(k_thinc<float, 2>, U + 2 * N);
+        sweep(k_thinc_flux<float, 0>, k_thinc_update<0>, U);
+        sweep(k_thinc_flux<float, 1>, k_thinc_update<1>, U + N);
+        sweep(k_thinc_flux<float, 2>, k_thinc_update<2>, U + 2 * N);
     }
     LBMIBM_LAUNCHED(ctx, "k_thinc");
     return LBMIBM_OK;
