@@ -49,8 +49,8 @@ inline void check(lbmibm_status s, const char *where) {
 }
 
 /* Creates the device context.  The caller fills p from its def.h constants
- * (LX/LY/LZ over dx, Cr, Df, P, B, M, C, GS, BR, ITRMAX, RESIDUAL,
- * RUNGE_KUTTA) and cells from cel->x/y/z and cel->ele (one mesh per cell
+ * (LX/LY/LZ over dx, Cr, Df, LAMBDA1, P, B, M, K, C, GS, BR, ITRMAX,
+ * RESIDUAL, RUNGE_KUTTA) and cells from cel->x/y/z and cel->ele (one mesh per cell
  * species or any mix, as voff/eoff say). */
 inline void attach(const lbmibm_params &p, const lbmibm_cells &cells) {
     check(lbmibm_create(&p, cells.nc > 0 ? &cells : nullptr, &context()), "lbmibm_create");
@@ -106,6 +106,25 @@ inline int volume_correction(Domain *, Cell *, Fem *) {
     int iterations = 0;
     lbmibm_ref::check(lbmibm_volume_correction(lbmibm_ref::context(), &iterations), "volume_correction");
     return iterations;
+}
+
+// vof_init (init1.cu:1095-1123) and the two VOF stages of the loop
+// (main.cu:93-97).  They need p.vof_period > 0 (K) at attach; lbmibm_step
+// calls the same two stages itself.
+template <class Domain, class Lattice, class Cell>
+inline void vof_init(Domain *, Lattice *, Cell *) {
+    std::printf("Initializing volume fraction ---------------------------------------\n");
+    lbmibm_ref::check(lbmibm_front_tracking(lbmibm_ref::context(), nullptr, nullptr), "vof_init");
+}
+
+template <class Domain, class Lattice, class Cell>
+inline void front_tracking(Domain *, Lattice *, Cell *) {
+    lbmibm_ref::check(lbmibm_front_tracking(lbmibm_ref::context(), nullptr, nullptr), "front_tracking");
+}
+
+template <class Domain, class Lattice>
+inline void thinc_wlic(Domain *, Lattice *) {
+    lbmibm_ref::check(lbmibm_thinc_wlic(lbmibm_ref::context()), "thinc_wlic");
 }
 
 // The binary files of output_data into ./result with the reference's names

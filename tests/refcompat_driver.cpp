@@ -21,9 +21,64 @@ int main(int argc, char **argv) {
     lbmibm_params p;
     lbmibm_default_params(&p, 16, 12, 10);
     if (p.lx != 16 || p.m != 1) return 2;
-    if (argc < 2 || std::strcmp(argv[1], "run") != 0) {
+    if (argc < 2 || (std::strcmp(argv[1], "run") != 0 && std::strcmp(argv[1], "vof") != 0)) {
         std::printf("adaptor ok\n");
         return 0;
+    }
+    if (std::strcmp(argv[1], "vof") == 0) {
+        // one spherical cell (radius 3) and a viscosity contrast: vof_init,
+        // then the loop with thinc_wlic every M and front_tracking every K
+        int nv = 0, nt = 0;
+        lbmibm_ref::check(lbmibm_mesh_icosphere(2, &nv, &nt, nullptr, nullptr), "icosphere");
+        std::vector<double> xyz(3 * (size_t)nv), x(nv), y(nv), z(nv);
+        std::vector<int> tri(3 * (size_t)nt);
+        lbmibm_ref::check(lbmibm_mesh_icosphere(2, &nv, &nt, xyz.data(), tri.data()), "icosphere");
+        for (int j = 0; j < nv; j++) {
+            x[j] = 3.0 * xyz[3 * j] + 0.3;
+            y[j] = 3.0 * xyz[3 * j + 1] - 0.2;
+            z[j] = 3.0 * xyz[3 * j + 2] + 0.1;
+        }
+        int voff[2] = {0, nv}, eoff[2] = {0, nt};
+        lbmibm_cells one{1, voff, eoff, x.data(), y.data(), z.data(), tri.data()};
+        p.lambda1 = 4.0;
+        p.vof_period = 5;  // K
+        p.sor_itrmax = 5000;
+        p.sor_residual = 1e-8;
+        p.volume_correction = 0;
+        lbmibm_ref::attach(p, one);
+        process prc{0, 0};
+        domain cdo{0};
+        lattice ltc;
+        cell cel;
+        fem fm;
+        const int M = p.m, K = p.vof_period;
+        vof_init(&cdo, &ltc, &cel);
+        while (prc.iter < 10) {
+            prc.iter++;
+            if ((prc.iter - 1) % M == 0) stress(&prc, &cdo, &cel, &fm, &ltc);
+            if ((prc.iter - 1) % M == 0) ibm_st1(&cdo, &ltc, &cel);
+            lbm_d3q19(&prc, &cdo, &ltc);
+            if (prc.iter % M == 0) ibm_st2(&cdo, &ltc, &cel, &fm);
+            if (prc.iter % M == 0) thinc_wlic(&cdo, &ltc);
+            if (prc.iter % K == 0) front_tracking(&cdo, &ltc, &cel);
+        }
+        long long n = 0;
+        lbmibm_sizes(lbmibm_ref::context(), &n, nullptr, nullptr, nullptr);
+        std::vector<double> vf(n);
+        lbmibm_ref::check(lbmibm_download_vf(lbmibm_ref::context(), vf.data()), "download_vf");
+        double vmax = 0.0, vsum = 0.0;
+        const int nx = p.lx + 4, ny = p.ly + 4, nz = p.lz + 4;
+        for (int iz = 2; iz < nz - 2; iz++)
+            for (int iy = 2; iy < ny - 2; iy++)
+                for (int ix = 2; ix < nx - 2; ix++) {
+                    const double v = vf[ix + (size_t)nx * (iy + (size_t)ny * iz)];
+                    vmax = v > vmax ? v : vmax;
+                    vsum += v;
+                }
+        std::printf("vf max %.4f sum %.3f\n", vmax, vsum);
+        lbmibm_ref::detach();
+        // a radius-3 sphere: vf near 1 at its centre, volume about 113
+        return vmax > 0.95 && vmax < 1.05 && vsum > 100.0 && vsum < 125.0 ? 0 : 5;
     }
     p.p = 5e-6;
     lbmibm_cells none{};

@@ -35,3 +35,14 @@ def test_adaptor_drives_the_reference_loop(tmp_path):
     assert float(r.stdout.split()[-1]) > 0.0
     # output_data's files, named as the reference names them
     assert sorted(os.listdir(work / "result")) == ["cell", "prof0000.bin", "restart.bin", "restart0000.bin"]
+
+
+@pytest.mark.gpu
+def test_adaptor_drives_the_vof_stages(tmp_path):
+    """vof_init, then main.cu's loop with thinc_wlic and front_tracking, for
+    one spherical cell with LAMBDA1 = 4: the indicator is 1 at the centre
+    and integrates to the sphere's volume."""
+    exe = _build(tmp_path)
+    r = subprocess.run([exe, "vof"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "vf max" in r.stdout
