@@ -69,7 +69,8 @@ typedef struct lbmibm_params {
     double p;                /* body force magnitude P (Guo force 2 rho P e)                 */
     double ex, ey, ez;       /* body force direction e                                       */
     int    force_modulated;  /* 1: e (1 + 0.5 sin(pi t/FREQ)) for iter > FLOWSTART, else 0
-                                (lbm.cu:1032-1035); 0: constant e (lbm.cu:518)               */
+                                (lbm.cu:1032-1035); 2: e for iter > FLOWSTART, else 0 (the
+                                tube, lbm_cylinder.cu:292-293); 0: constant e (lbm.cu:518)   */
     int    flowstart;        /* FLOWSTART                                                    */
     double freq;             /* FREQ                                                         */
     double b;                /* IBM coupling B (lbm.cu:1116-1118, 1135)                      */
@@ -94,6 +95,11 @@ typedef struct lbmibm_params {
     int    sor_itrmax;       /* ITRMAX of the front-tracking red-black SOR                   */
     double sor_residual;     /* its RESIDUAL (relative, matrix_solver.cu:188-215); the
                                 reference shares both constants with BiCGSTAB                */
+    double tube_radius;      /* > 0: the CapsuleCHANNELFLOW tube of radius Lc/2 (= R of
+                                lbm_cylinder.cu:167) along z: nodes with sqrt(x^2+y^2) > R
+                                are walls, z is periodic, links crossing r = R take the
+                                interpolated bounce-back (lbm_cylinder.cu:61-235); needs
+                                R <= min(LX,LY)/2 - dx, periodic_y = 0, no VOF. 0: off     */
 } lbmibm_params;
 
 /* Initial cell set: reference SoA positions and element vertex ids. */

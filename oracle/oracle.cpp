@@ -70,6 +70,33 @@ int orc_num_threads(void) { return omp_get_max_threads(); }
 
 double orc_delta_weight(double r) { return (1.0 + std::cos(M_PI * r / 2.0)) / 4.0; }
 
+double orc_tube_fraction(const orc_params *p, int i, int q) {
+    // lbm_cylinder.cu:147-186, with R = the tube radius
+    const Grid g = grid_of(p);
+    int ix, iy, iz;
+    g.coords(i, ix, iy, iz);
+    const double dxl = p->dx, R = p->tube_radius;
+    const double x_i = -p->lx / 2.0 - 1.5 * dxl + (double)ix * dxl;
+    const double y_i = -p->ly / 2.0 - 1.5 * dxl + (double)iy * dxl;
+    const double z_i = -p->lz / 2.0 - 1.5 * dxl + (double)iz * dxl;
+    const double x_j = -p->lx / 2.0 - 1.5 * dxl + (double)(ix + CX[q]) * dxl;
+    const double y_j = -p->ly / 2.0 - 1.5 * dxl + (double)(iy + CY[q]) * dxl;
+    const double z_j = -p->lz / 2.0 - 1.5 * dxl + (double)(iz + CZ[q]) * dxl;
+    const double dx = x_j - x_i, dy = y_j - y_i, dz = z_j - z_i;
+    const double r0 = std::sqrt(dx * dx + dy * dy + dz * dz);
+    const double Det = (x_i * dx + y_i * dy) * (x_i * dx + y_i * dy) - (dx * dx + dy * dy) * (x_i * x_i + y_i * y_i - R * R);
+    if (Det < 0.0) return -1.0;
+    const double t1 = (-x_i * dx - y_i * dy + std::sqrt(Det)) / (dx * dx + dy * dy);
+    const double t2 = (-x_i * dx - y_i * dy - std::sqrt(Det)) / (dx * dx + dy * dy);
+    double t;
+    if (t1 > 0.0 && t1 < 1.0) t = t1;
+    else if (t2 > 0.0 && t2 < 1.0) t = t2;
+    else return -1.0;
+    const double xw = x_i + t * dx, yw = y_i + t * dy, zw = z_i + t * dz;
+    const double r = std::sqrt((x_i - xw) * (x_i - xw) + (y_i - yw) * (y_i - yw) + (z_i - zw) * (z_i - zw));
+    return r / r0;
+}
+
 int orc_expand2pow2(int n) {
     // etc.cu:425-446
     int cnt = 0;
@@ -90,6 +117,21 @@ void orc_bc_init(const orc_params *p, int *bc) {
     // strictly inside (-LX/2, LX/2) exactly for ix in [2, nx-3].
     const Grid g = grid_of(p);
     const int n = g.nx * g.ny * g.nz;
+    if (p->tube) {
+        // init1.cu:217-229: the tube r <= Lc/2, BUFF in the z halo
+        for (int i = 0; i < n; i++) {
+            int ix, iy, iz;
+            g.coords(i, ix, iy, iz);
+            const double x = -p->lx / 2.0 - 1.5 * p->dx + (double)ix * p->dx;
+            const double y = -p->ly / 2.0 - 1.5 * p->dx + (double)iy * p->dx;
+            const double r = std::sqrt(x * x + y * y);
+            const bool inz = iz >= 2 && iz <= g.nz - 3;
+            if (r <= p->tube_radius && inz) bc[i] = ORC_COMP;
+            else if (r <= p->tube_radius) bc[i] = ORC_BUFF;
+            else bc[i] = ORC_WALL;
+        }
+        return;
+    }
     for (int i = 0; i < n; i++) {
         int ix, iy, iz;
         g.coords(i, ix, iy, iz);
@@ -132,9 +174,12 @@ void orc_lbm_step(const orc_params *p, orc_state *s, int iter) {
     const int n = g.nx * g.ny * g.nz;
     const double Cr = p->cr, Cr2 = Cr * Cr;
     double ui = p->ex, vi = p->ey, wi = p->ez;
-    if (p->force_modulated) {
+    if (p->force_modulated == 1) {
         double t = p->dt * (double)iter;
         double amp = (iter > p->flowstart) ? 1.0 + 0.5 * std::sin(M_PI * t / p->freq) : 0.0;
+        ui *= amp; vi *= amp; wi *= amp;
+    } else if (p->force_modulated == 2) {  // lbm_cylinder.cu:292-293
+        const double amp = iter > p->flowstart ? 1.0 : 0.0;
         ui *= amp; vi *= amp; wi *= amp;
     }
     const bool reset = ((iter - 1) % p->m) == 0;
@@ -190,6 +235,40 @@ void orc_lbm_step(const orc_params *p, orc_state *s, int iter) {
             else j = t * Q + q;
             fm[j] = fs;
         }
+    }
+
+    // The tube: linear interpolation on every link that crosses the wall
+    // (lbm_BounceBack_GPU, lbm_cylinder.cu:193-235).  All links read the
+    // values the push wrote and then write: where a node has wall nodes on
+    // both sides of a link the reference's threads race on those slots, and
+    // its f3 then reads a wall node's unused slot; here f2 and f3 are the
+    // node's own half-way-bounced value.
+    if (p->tube) {
+        std::vector<std::pair<size_t, double>> upd;
+        for (int i = 0; i < n; i++) {
+            if (s->bc[i] != ORC_COMP) continue;
+            int ix, iy, iz;
+            g.coords(i, ix, iy, iz);
+            for (int d = 1; d < Q; d++) {
+                const int t = g.idx(ix + CX[d], iy + CY[d], iz + CZ[d]);
+                if (s->bc[t] != ORC_WALL) continue;
+                const double r = orc_tube_fraction(p, i, d);
+                if (r < 0.0) continue;
+                const int q = opp(d);
+                int jn = g.idx(ix - CX[d], iy - CY[d], iz - CZ[d]);
+                const bool jwall = s->bc[jn] == ORC_WALL;
+                if (s->bc[jn] == ORC_BUFF) jn = g.image(jn);
+                const double f1 = fm[(size_t)i * Q + q];
+                const double f2 = fm[(size_t)i * Q + d];
+                const double f3 = jwall ? fm[(size_t)i * Q + d] : fm[(size_t)jn * Q + q];
+                double f;
+                if (r < 0.5) f = 2.0 * r * f1 + (1.0 - 2.0 * r) * f2;
+                else if (r <= 1.0) f = 1.0 / (2.0 * r) * f1 + (2.0 * r - 1.0) / (2.0 * r) * f3;
+                else f = f1;
+                upd.emplace_back((size_t)i * Q + q, f);
+            }
+        }
+        for (const auto &u : upd) fm[u.first] = u.second;
     }
 
     // BUFF nodes take the macros of their periodic image (lbm.cu:1316-1432).

@@ -39,7 +39,8 @@ typedef struct orc_params {
     double df, lambda1;     /* omega = 1/(3(1+(lambda1-1)vf)Df + 0.5)  (lbm.cu:1131)       */
     double p;               /* body force magnitude P (Guo force 2 rho P e, lbm.cu:590)  */
     double ex, ey, ez;      /* body force direction e                                      */
-    int    force_modulated; /* 1: e*(1+0.5 sin(pi t/FREQ)) after FLOWSTART (lbm.cu:1032)   */
+    int    force_modulated; /* 1: e*(1+0.5 sin(pi t/FREQ)) after FLOWSTART (lbm.cu:1032);  */
+                            /* 2: e after FLOWSTART, 0 before (lbm_cylinder.cu:292-293)     */
     int    flowstart;
     double freq;
     double b;               /* IBM force coupling B (lbm.cu:1116)                          */
@@ -58,6 +59,9 @@ typedef struct orc_params {
                             /* (main.cu:93-97); 0: no VOF indicator (vf stays 0)          */
     int    sor_itrmax;      /* ITRMAX, RESIDUAL of the red-black SOR (matrix_solver.cu:   */
     double sor_residual;    /* 188-215), the same constants as BiCGSTAB's in the reference */
+    int    tube;            /* 1: CapsuleCHANNELFLOW tube: COMP where sqrt(x^2+y^2) <= R,  */
+    double tube_radius;     /* periodic in z only, interpolated bounce-back on the links   */
+                            /* that cross r = R (init1.cu:190-236, lbm_cylinder.cu)         */
 } orc_params;
 
 /* Pointer set equivalent to the reference's lattice/cell/fem structs
@@ -148,6 +152,11 @@ double orc_thinc_flux(double c, double alpha, double uc, int isgn, int m, double
  * copy / WALL zero) on s->vf with the M-averaged velocity (um, vm, wm).
  * thincwlic.cu:108-313, 316-372 */
 void orc_thinc_wlic(const orc_params *p, orc_state *s);
+
+/* SetDistanceFromWall_GPU (lbm_cylinder.cu:118-190) for the link from padded
+ * node i along direction q: the fraction |x_i - x_wall| / |c_q dx| of the
+ * tube-wall crossing, or -1 where the reference records no crossing. */
+double orc_tube_fraction(const orc_params *p, int i, int q);
 
 /* Single-point helpers used by the delta-function tests. */
 double orc_delta_weight(double r_over_dx);                 /* (1+cos(pi r/2))/4     */

@@ -118,6 +118,7 @@ lbmibm_status adj_common(lbmibm_ctx *ctx) {
     lbmibm_status s = need(ctx, ctx->prec != LBMIBM_F16, "FP16 population storage has no adjoint");
     if (s == LBMIBM_OK) s = need(ctx, ctx->p.m == 1, "membrane sub-cycling M must be 1");
     if (s == LBMIBM_OK) s = need(ctx, ctx->p.vof_period == 0, "the VOF indicator has no adjoint");
+    if (s == LBMIBM_OK) s = need(ctx, ctx->p.tube_radius == 0.0, "the tube geometry has no adjoint");
     if (s == LBMIBM_OK) s = adj_alloc(ctx);
     return s;
 }

@@ -30,7 +30,8 @@ __global__ void k_set_step(DevStep *s, int iter, StepParams p) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int it = iter >= 0 ? iter : s->iter + 1;
     double amp = 1.0;
-    if (p.force_modulated) amp = it > p.flowstart ? 1.0 + 0.5 * sin(M_PI * (p.dt * (double)it) / p.freq) : 0.0;
+    if (p.force_modulated == 1) amp = it > p.flowstart ? 1.0 + 0.5 * sin(M_PI * (p.dt * (double)it) / p.freq) : 0.0;
+    else if (p.force_modulated == 2) amp = it > p.flowstart ? 1.0 : 0.0;  // lbm_cylinder.cu:292-293
     s->iter = it;
     s->amp = amp;
 }
@@ -88,6 +89,39 @@ lbmibm_status get_T(lbmibm_ctx *ctx, double *dst, const void *src, size_t n) {
     return LBMIBM_OK;
 }
 
+// The tube mask of cdo_init (init1.cu:217-229) at fluid-frame column (ix,
+// iy), which may lie in the halo: inside where sqrt(x^2 + y^2) <= R.
+bool tube_inside(const lbmibm_ctx *ctx, int ix, int iy) {
+    const double dx = ctx->p.dx;
+    const double x = -ctx->p.lx * dx / 2.0 - 1.5 * dx + (double)(ix + 2) * dx;
+    const double y = -ctx->p.ly * dx / 2.0 - 1.5 * dx + (double)(iy + 2) * dx;
+    return std::sqrt(x * x + y * y) <= ctx->p.tube_radius;
+}
+
+// SetDistanceFromWall_GPU (lbm_cylinder.cu:147-186) for the link from
+// column (ix, iy) along direction q (z taken at 0; the fraction does not
+// depend on it): the wall's fraction of the link, or -1 where the reference
+// records no crossing.
+double tube_fraction(const lbmibm_ctx *ctx, int ix, int iy, int q) {
+    const double dxl = ctx->p.dx, R = ctx->p.tube_radius;
+    const double x_i = -ctx->p.lx * dxl / 2.0 - 1.5 * dxl + (double)(ix + 2) * dxl;
+    const double y_i = -ctx->p.ly * dxl / 2.0 - 1.5 * dxl + (double)(iy + 2) * dxl;
+    const double x_j = -ctx->p.lx * dxl / 2.0 - 1.5 * dxl + (double)(ix + 2 + CXh[q]) * dxl;
+    const double y_j = -ctx->p.ly * dxl / 2.0 - 1.5 * dxl + (double)(iy + 2 + CYh[q]) * dxl;
+    const double dx = x_j - x_i, dy = y_j - y_i, dz = (double)CZh[q] * dxl;
+    const double r0 = std::sqrt(dx * dx + dy * dy + dz * dz);
+    const double Det = (x_i * dx + y_i * dy) * (x_i * dx + y_i * dy) - (dx * dx + dy * dy) * (x_i * x_i + y_i * y_i - R * R);
+    if (Det < 0.0) return -1.0;
+    const double t1 = (-x_i * dx - y_i * dy + std::sqrt(Det)) / (dx * dx + dy * dy);
+    const double t2 = (-x_i * dx - y_i * dy - std::sqrt(Det)) / (dx * dx + dy * dy);
+    double t;
+    if (t1 > 0.0 && t1 < 1.0) t = t1;
+    else if (t2 > 0.0 && t2 < 1.0) t = t2;
+    else return -1.0;
+    const double ex = t * dx, ey = t * dy, ez = t * dz;
+    return std::sqrt(ex * ex + ey * ey + ez * ez) / r0;
+}
+
 long long padded_index(const lbmibm_ctx *ctx, int ix, int iy, int iz) {
     return (long long)(ix + 2) + (long long)ctx->nxp * ((iy + 2) + (long long)ctx->nyp * (iz + 2));
 }
@@ -96,6 +130,10 @@ long long padded_index(const lbmibm_ctx *ctx, int ix, int iy, int iz) {
 // when it lies beyond a wall.
 long long pull_source(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q) {
     int sx = ix - CXh[q], sy = iy - CYh[q], sz = iz - CZh[q];
+    if (ctx->p.tube_radius > 0.0) {
+        if (!tube_inside(ctx, sx, sy)) return -1;
+        return (long long)sx + (long long)ctx->nx * (sy + (long long)ctx->ny * ((sz + ctx->nz) % ctx->nz));
+    }
     sx = (sx + ctx->nx) % ctx->nx;
     sz = (sz + ctx->nz) % ctx->nz;
     if (ctx->p.periodic_y) sy = (sy + ctx->ny) % ctx->ny;
@@ -105,6 +143,10 @@ long long pull_source(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q) {
 
 long long push_target(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q) {
     int tx = ix + CXh[q], ty = iy + CYh[q], tz = iz + CZh[q];
+    if (ctx->p.tube_radius > 0.0) {
+        if (!tube_inside(ctx, tx, ty)) return -1;
+        return (long long)tx + (long long)ctx->nx * (ty + (long long)ctx->ny * ((tz + ctx->nz) % ctx->nz));
+    }
     tx = (tx + ctx->nx) % ctx->nx;
     tz = (tz + ctx->nz) % ctx->nz;
     if (ctx->p.periodic_y) ty = (ty + ctx->ny) % ctx->ny;
@@ -116,6 +158,12 @@ long long push_target(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q) {
 // for a wall node.
 inline long long padded_image(const lbmibm_ctx *ctx, int px, int py, int pz, bool &is_comp) {
     auto w = [](int a, int n) { return a < 2 ? a + (n - 4) : (a > n - 3 ? a - (n - 4) : a); };
+    if (ctx->p.tube_radius > 0.0) {  // only z is periodic; outside the disc is wall
+        is_comp = false;
+        if (!tube_inside(ctx, px - 2, py - 2)) return -1;
+        is_comp = pz >= 2 && pz <= ctx->nzp - 3;
+        return (long long)(px - 2) + (long long)ctx->nx * ((py - 2) + (long long)ctx->ny * (w(pz, ctx->nzp) - 2));
+    }
     const bool iny = py >= 2 && py <= ctx->nyp - 3;
     is_comp = px >= 2 && px <= ctx->nxp - 3 && iny && pz >= 2 && pz <= ctx->nzp - 3;
     if (!iny && !ctx->p.periodic_y) return -1;
@@ -216,7 +264,8 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->srec, ctx->onchip_cells, ctx->Hb, ctx->adj_f[0], ctx->adj_f[1], ctx->adj_u, ctx->adj_F,
                     ctx->adj_x, ctx->adj_cf, ctx->grad_H, ctx->grad_gs, ctx->adj_lam, ctx->adj_dsv,
                     ctx->adj_corner, ctx->adj_scratch, ctx->adj_cscale, ctx->adj_gpart, ctx->adj_xmax, ctx->adj_rec, ctx->sbox, ctx->sb_cand,
-                    ctx->vof[0], ctx->vof[1], ctx->ft_g, ctx->ft_x, ctx->ft_b, ctx->ft_part, ctx->ft_ctl};
+                    ctx->vof[0], ctx->vof[1], ctx->ft_g, ctx->ft_x, ctx->ft_b, ctx->ft_part, ctx->ft_ctl,
+                    ctx->tube_col, ctx->tube_r};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -263,6 +312,7 @@ void lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz) {
     p->vof_period = 0;
     p->sor_itrmax = 1000;
     p->sor_residual = 1.0e-10;
+    p->tube_radius = 0.0;
 }
 
 const char *lbmibm_last_error(const lbmibm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -287,6 +337,11 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     // the viscosity contrast acts through the VOF indicator, which needs cells
     // and the slab's walls (CapsuleSHEARFLOW, fronttrack.cu:87-125)
     if (p->vof_period < 0 || (p->lambda1 != 1.0 && p->vof_period == 0)) return LBMIBM_ERR_ARG;
+    // the tube's halo columns must be walls (cdo_init tests only r <= Lc/2)
+    if (p->tube_radius < 0.0 || p->force_modulated < 0 || p->force_modulated > 2 ||
+        (p->tube_radius > 0.0 && (p->periodic_y || p->vof_period > 0 ||
+                                  p->tube_radius > 0.5 * std::min(p->lx, p->ly) * p->dx - p->dx)))
+        return LBMIBM_ERR_ARG;
     if (p->vof_period > 0 && (p->periodic_y || !cells || cells->nc < 1 || p->sor_itrmax < 1 ||
                               !(p->sor_residual > 0.0) ||
                               (long long)(p->lx + 4) * (p->ly + 4) * (p->lz + 4) >= (1LL << 31)))
@@ -542,6 +597,29 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     CK(membrane_autotune(ctx));
     CK(ibm_autotune(ctx));
     if (p->vof_period > 0) CK(vof_alloc(ctx));
+    if (p->tube_radius > 0.0) {
+        // per-column tube words: bit 0 inside, bit q the pull source of q
+        // outside; and the wall fractions of the cut links, by q
+        const size_t ncol = (size_t)ctx->nx * ctx->ny;
+        std::vector<unsigned> col(ncol, 0u);
+        ctx->tube_rh.assign(ncol * Q, -1.0);
+        for (int iy = 0; iy < ctx->ny; iy++)
+            for (int ix = 0; ix < ctx->nx; ix++) {
+                const size_t c = ix + (size_t)ctx->nx * iy;
+                if (!tube_inside(ctx, ix, iy)) continue;
+                unsigned w = 1u;
+                for (int q = 1; q < Q; q++)
+                    if (!tube_inside(ctx, ix - CXh[q], iy - CYh[q])) {
+                        w |= 1u << q;
+                        ctx->tube_rh[c * Q + q] = tube_fraction(ctx, ix, iy, opph(q));
+                    }
+                col[c] = w;
+            }
+        CK(dev_alloc(ctx, &ctx->tube_col, ncol * sizeof(unsigned)));
+        CK(dev_alloc(ctx, &ctx->tube_r, ncol * Q * sizeof(double)));
+        LBMIBM_HIP(ctx, hipMemcpy(ctx->tube_col, col.data(), ncol * sizeof(unsigned), hipMemcpyHostToDevice));
+        LBMIBM_HIP(ctx, hipMemcpy(ctx->tube_r, ctx->tube_rh.data(), ncol * Q * sizeof(double), hipMemcpyHostToDevice));
+    }
     CK(reset_status(ctx));
     CK(sync_check(ctx));
 #undef CK
@@ -816,7 +894,17 @@ lbmibm_status lbmibm_download_populations(lbmibm_ctx *ctx, double *fn) {
                 const long long pi = padded_index(ctx, ix, iy, iz);
                 for (int q = 0; q < Q; q++) {
                     const long long src = pull_source(ctx, ix, iy, iz, q);
-                    fn[pi * Q + q] = src < 0 ? fs[(size_t)opph(q) * N + i] : fs[(size_t)q * N + src];
+                    double f = src < 0 ? fs[(size_t)opph(q) * N + i] : fs[(size_t)q * N + src];
+                    if (src < 0 && ctx->p.tube_radius > 0.0 && tube_inside(ctx, ix, iy)) {
+                        // the interpolated bounce-back of the kernel (lbm_cylinder.cu:61-85)
+                        const double r = ctx->tube_rh[((size_t)ix + (size_t)ctx->nx * iy) * Q + q];
+                        const long long s2 = pull_source(ctx, ix, iy, iz, opph(q));
+                        const double own = fs[(size_t)q * N + i];
+                        const double f2 = s2 < 0 ? own : fs[(size_t)opph(q) * N + s2];
+                        if (r >= 0.0 && r < 0.5) f = 2.0 * r * f + (1.0 - 2.0 * r) * f2;
+                        else if (r >= 0.5 && r <= 1.0) f = 1.0 / (2.0 * r) * f + (2.0 * r - 1.0) / (2.0 * r) * own;
+                    }
+                    fn[pi * Q + q] = f;
                 }
             }
     return LBMIBM_OK;

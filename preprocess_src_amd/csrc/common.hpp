@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "lbmibm.h"
 
@@ -159,6 +160,10 @@ struct lbmibm_ctx {
     double *ft_x = nullptr, *ft_b = nullptr, *ft_part = nullptr;
     void *ft_ctl = nullptr;
     int ft_nparts = 0;
+    // tube geometry (tube_radius > 0): per-column words (bit 0 inside the
+    // disc, bit q the pull source of q is a wall) and wall fractions [col][19]
+    void *tube_col = nullptr, *tube_r = nullptr;
+    std::vector<double> tube_rh;  // host copy of the fractions
     lbmibm::DevStep *step = nullptr;
     lbmibm::DevStatus *status = nullptr;
 
@@ -202,6 +207,7 @@ lbmibm_status put_T(lbmibm_ctx *ctx, void *dst, const double *src, size_t n);  /
 lbmibm_status get_T(lbmibm_ctx *ctx, double *dst, const void *src, size_t n);  // device T -> host double
 long long padded_index(const lbmibm_ctx *ctx, int ix, int iy, int iz);
 long long pull_source(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q);  // -1 beyond a wall
+bool tube_inside(const lbmibm_ctx *ctx, int ix, int iy);  // tube mask at fluid-frame column (ix, iy)
 long long push_target(const lbmibm_ctx *ctx, int ix, int iy, int iz, int q);  // -1 beyond a wall
 lbmibm_status sync_check(lbmibm_ctx *ctx);
 lbmibm_status check_flags(lbmibm_ctx *ctx, int *vol_iters);

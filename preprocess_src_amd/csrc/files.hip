@@ -149,6 +149,10 @@ lbmibm_status lbmibm_bc_mask(const lbmibm_ctx *ctx, int *bc) {
                 const bool inx = px >= 2 && px <= ctx->nxp - 3, iny = py >= 2 && py <= ctx->nyp - 3,
                            inz = pz >= 2 && pz <= ctx->nzp - 3;
                 const long long i = px + (long long)ctx->nxp * (py + (long long)ctx->nyp * pz);
+                if (ctx->p.tube_radius > 0.0) {  // init1.cu:217-229
+                    bc[i] = !tube_inside(ctx, px - 2, py - 2) ? LBMIBM_BC_WALL : (inz ? LBMIBM_BC_COMP : LBMIBM_BC_BUFF);
+                    continue;
+                }
                 bc[i] = inx && iny && inz ? LBMIBM_BC_COMP
                                           : (iny || ctx->p.periodic_y ? LBMIBM_BC_BUFF : LBMIBM_BC_WALL);
             }

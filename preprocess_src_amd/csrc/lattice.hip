@@ -51,12 +51,33 @@ struct LbmDims {
     int nblocks;
 };
 
-template <typename S, typename T, bool HASF, bool MACROS, bool WALLS>
+// Geometry of k_lbm: periodic box, slab (half-way bounce-back walls at the
+// y ends) or the tube (nodes outside the disc are walls; links that cross
+// the disc's edge are interpolated from the node's own populations,
+// lbm_cylinder.cu:61-85, 193-235).
+enum { GEOM_PERIODIC = 0, GEOM_SLAB = 1, GEOM_TUBE = 2 };
+
+// BounceBackLinearInterpolation_DEV (lbm_cylinder.cu:61-85): r is the
+// wall's fraction of the link from the node, f1 the half-way bounced value,
+// f2 the value streamed in along the link, f3 the node's own value going
+// the other way; r < 0 marks a link without an interpolation.
+template <typename T> __device__ __forceinline__ T tube_interp(double r, T f1, T f2, T f3) {
+    if (r < 0.0) return f1;
+    const T rr = (T)r;
+    if (r < 0.5) return T(2) * rr * f1 + (T(1) - T(2) * rr) * f2;
+    if (r <= 1.0) return T(1) / (T(2) * rr) * f1 + (T(2) * rr - T(1)) / (T(2) * rr) * f3;
+    return f1;
+}
+
+template <typename S, typename T, bool HASF, bool MACROS, int GEOM>
 __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__restrict__ fout, LbmDims d,
                                             const T *__restrict__ F, const T *__restrict__ H, T *__restrict__ U,
                                             T *__restrict__ Ui, T *__restrict__ rho_out, T *__restrict__ rhom, T om0,
                                             T inv_cr, T pex, T pey, T pez, T B, const DevStep *__restrict__ st,
-                                            int m, const double *__restrict__ vf, T l1m1, T df) {
+                                            int m, const double *__restrict__ vf, T l1m1, T df,
+                                            const unsigned *__restrict__ tube_col,
+                                            const double *__restrict__ tube_r) {
+    constexpr bool WALLS = GEOM == GEOM_SLAB;
     const int lb = xcd_block(d.nblocks);
     if (lb >= d.nblocks) return;
     const unsigned i = (unsigned)lb * WG + threadIdx.x;
@@ -78,15 +99,38 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         yp = iy == ny - 1 ? 0 : iy + 1;
     }
 
+    // tube: bit 0 of the column word says the column is inside the disc,
+    // bit q that the pull source of direction q is a wall node
+    unsigned cut = 0;
+    if (GEOM == GEOM_TUBE) {
+        cut = tube_col[ix + nx * iy];
+        if (!(cut & 1u)) return;  // a wall node: never updated
+    }
+
     T g[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const unsigned sx = cx(q) == 1 ? xm : (cx(q) == -1 ? xp : ix);
         const unsigned sy = cy(q) == 1 ? ym : (cy(q) == -1 ? yp : iy);
         const unsigned sz = cz(q) == 1 ? zm : (cz(q) == -1 ? zp : iz);
-        const bool bb = WALLS && ((cy(q) == 1 && wall_lo) || (cy(q) == -1 && wall_hi));
+        const bool bb = (WALLS && ((cy(q) == 1 && wall_lo) || (cy(q) == -1 && wall_hi))) ||
+                        (GEOM == GEOM_TUBE && q > 0 && ((cut >> q) & 1u));
         const size_t src = bb ? (size_t)opp(q) * N + i : (size_t)q * N + (sx + nx * (sy + ny * sz));
         g[q] = Pop<S, T>::ld(fin + src, bb ? opp(q) : q);
+    }
+    if (GEOM == GEOM_TUBE && (cut & ~1u)) {
+        // g[q] of a cut link holds the half-way value (f1); f2 is the value
+        // pulled along the opposite direction, or the node's own f_q where
+        // that source is a wall too; f3 is the node's own f_q.  Only values
+        // of uncut directions are read as f2, so the in-place update is safe.
+        const double *rt = tube_r + (size_t)(ix + nx * iy) * Q;
+#pragma unroll
+        for (int q = 1; q < Q; q++) {
+            if (!((cut >> q) & 1u)) continue;
+            const T own = Pop<S, T>::ld(fin + (size_t)q * N + i, q);
+            const T f2 = ((cut >> opp(q)) & 1u) ? own : g[opp(q)];
+            g[q] = tube_interp<T>(rt[q], g[q], f2, own);
+        }
     }
 
     T drho = T(0), jx = T(0), jy = T(0), jz = T(0);
@@ -316,12 +360,14 @@ template <typename S, typename T> lbmibm_status step_impl(lbmibm_ctx *ctx) {
                            (const T *)ctx->Hb, (T *)ctx->U, (T *)ctx->Ui, (T *)ctx->rho, (T *)ctx->rhom, (T)om, (T)(1.0 / p.cr),
                            (T)(p.p * p.ex), (T)(p.p * p.ey), (T)(p.p * p.ez), (T)p.b, ctx->step, p.m,
                            (const double *)(p.vof_period > 0 ? ctx->vof[ctx->vof_cur] : nullptr), (T)(p.lambda1 - 1.0),
-                           (T)p.df);
+                           (T)p.df, (const unsigned *)ctx->tube_col, (const double *)ctx->tube_r);
     };
-#define LBM_CASE(HF, MC, WL) \
-    if (hasf == HF && macros == MC && walls == WL) launch(k_lbm<S, T, HF, MC, WL>);
-    LBM_CASE(true, true, true) LBM_CASE(true, true, false) LBM_CASE(true, false, true) LBM_CASE(true, false, false)
-    LBM_CASE(false, true, true) LBM_CASE(false, true, false) LBM_CASE(false, false, true) LBM_CASE(false, false, false)
+    const int geom = p.tube_radius > 0.0 ? GEOM_TUBE : (walls ? GEOM_SLAB : GEOM_PERIODIC);
+#define LBM_CASE(HF, MC, G) \
+    if (hasf == HF && macros == MC && geom == G) launch(k_lbm<S, T, HF, MC, G>);
+#define LBM_GEOMS(HF, MC) LBM_CASE(HF, MC, GEOM_PERIODIC) LBM_CASE(HF, MC, GEOM_SLAB) LBM_CASE(HF, MC, GEOM_TUBE)
+    LBM_GEOMS(true, true) LBM_GEOMS(true, false) LBM_GEOMS(false, true) LBM_GEOMS(false, false)
+#undef LBM_GEOMS
 #undef LBM_CASE
     LBMIBM_LAUNCHED(ctx, "k_lbm");
     ctx->cur ^= 1;  // pointer swap of lbm.cu:1575

@@ -28,7 +28,7 @@ def oracle_of(lx, ly, lz, cells=None, **kw):
                   flowstart=p.get("flowstart", 0), freq=p.get("freq", 1.0),
                   fileoutput=p.get("fileoutput", 0) or 1000, cells=cells, lambda1=p.get("lambda1", 1.0),
                   vof_period=p.get("vof_period", 0), sor_itrmax=p.get("sor_itrmax"),
-                  sor_residual=p.get("sor_residual"))
+                  sor_residual=p.get("sor_residual"), tube_radius=p.get("tube_radius", 0.0))
     o.init_fluid()
     return o
 
