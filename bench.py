@@ -10,13 +10,14 @@ seeded mesh mix of 50% 642-, 40% 2562- and 10% 10242-vertex meshes, radius
 populations stored as deviations, whole steps replayed from a hipGraph.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|f64|f16]
-                    [--workload suspension|tgv|adjoint|vof] [--cpu-seconds S] [--no-cpu]
+                    [--workload suspension|tgv|adjoint|vof|tube] [--cpu-seconds S] [--no-cpu]
 
 --workload adjoint times the checkpointed wall-shear gradient on a 128x64x64
 channel with 32 RBCs (500 steps, last 200 averaged, checkpoints every 50)
 against the forward run and checks it against a central difference.
 --workload vof times front tracking (red-black SOR), THINC/WLIC and steps
-with a viscosity contrast on the default suspension.
+with a viscosity contrast on the default suspension; --workload tube the
+lattice step of the interpolated-bounce-back tube.
 """
 from __future__ import annotations
 
@@ -314,13 +315,44 @@ def vof_bench(a):
     sim.close()
 
 
+def tube_bench(a):
+    """The CapsuleCHANNELFLOW tube: a 128 x 128 x 512 box holding a tube of
+    radius 63 lattice units along z, body force along z from the first step,
+    no cells (the lattice kernel with the interpolated bounce-back alone)."""
+    from preprocess_src_amd.simulation import Simulation
+    lx, ly, lz, R = 128, 128, 512, 63.0
+    sim = Simulation(lx, ly, lz, precision=a.precision, tau=1.0, p=1e-6, e=(0.0, 0.0, 1.0), force_modulated=2,
+                     flowstart=0, tube_radius=R)
+    fluid = int((sim.bc_mask() == 0).sum())
+    sim.run(0, a.warmup, graph=True)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.run(a.warmup, a.steps, graph=True)
+    sim.synchronize()
+    dt = time.perf_counter() - t0
+    stages = sim.time_stages(a.warmup + a.steps, a.stage_steps)
+    t_lbm = stages["lbm_d3q19"][0] * 1e-3
+    bpn = lattice_bytes_per_node(a.precision, False)
+    out = {
+        "workload": "tube",
+        "config": {"workload": f"tube r={R:g} lu in a {lx}x{ly}x{lz} box (z periodic, interpolated bounce-back), "
+                               f"body force along z, no cells", "precision": a.precision, "fluid_nodes": fluid},
+        "mlups": round(fluid * a.steps / dt / 1e6, 1), "ms_per_step": round(1e3 * dt / a.steps, 4),
+        "roofline": {"bound": "hbm", "kernel": "k_lbm (tube)", "achieved": round(fluid * bpn / t_lbm / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(fluid * bpn / t_lbm / HBM_PEAK, 4),
+                     "bytes_per_fluid_node": bpn, "avg_ms": round(stages["lbm_d3q19"][0], 5)},
+    }
+    print(json.dumps(out), flush=True)
+    sim.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64", "f16"])
-    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv", "adjoint", "copy", "vof"],
+    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv", "adjoint", "copy", "vof", "tube"],
                     help="copy: a plain 256 MiB device copy, the calibration run of the FETCH_SIZE / "
                          "WRITE_SIZE counters")
     ap.add_argument("--stage-steps", type=int, default=6)
@@ -348,6 +380,9 @@ def main():
         return
     if a.workload == "vof":
         vof_bench(a)
+        return
+    if a.workload == "tube":
+        tube_bench(a)
         return
     if a.workload == "suspension":
         shape, cells, params, desc, ht = suspension_workload()

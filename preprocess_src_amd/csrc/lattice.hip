@@ -123,13 +123,25 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         // pulled along the opposite direction, or the node's own f_q where
         // that source is a wall too; f3 is the node's own f_q.  Only values
         // of uncut directions are read as f2, so the in-place update is safe.
+        // all loads first (each under its lane mask), then the arithmetic,
+        // so that a wave waits once rather than once per cut direction
         const double *rt = tube_r + (size_t)(ix + nx * iy) * Q;
+        T own[Q];
+        double rq[Q];
+#pragma unroll
+        for (int q = 1; q < Q; q++) {
+            own[q] = T(0);
+            rq[q] = -1.0;
+            if ((cut >> q) & 1u) {
+                own[q] = Pop<S, T>::ld(fin + (size_t)q * N + i, q);
+                rq[q] = rt[q];
+            }
+        }
 #pragma unroll
         for (int q = 1; q < Q; q++) {
             if (!((cut >> q) & 1u)) continue;
-            const T own = Pop<S, T>::ld(fin + (size_t)q * N + i, q);
-            const T f2 = ((cut >> opp(q)) & 1u) ? own : g[opp(q)];
-            g[q] = tube_interp<T>(rt[q], g[q], f2, own);
+            const T f2 = ((cut >> opp(q)) & 1u) ? own[q] : g[opp(q)];
+            g[q] = tube_interp<T>(rq[q], g[q], f2, own[q]);
         }
     }
 
