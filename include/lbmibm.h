@@ -218,6 +218,43 @@ lbmibm_status lbmibm_thinc_wlic(lbmibm_ctx *ctx);
 lbmibm_status lbmibm_download_vf(lbmibm_ctx *ctx, double *vf);
 lbmibm_status lbmibm_upload_vf(lbmibm_ctx *ctx, const double *vf);
 
+/* ---- cytoadhesion (adhesion.cu; slab, linear spring law) ----------------
+ * Knob vertices bind receptors on the wall y = -LY/2 through springs_per_knob
+ * springs each, by the Bell model; the spring force joins the membrane force
+ * in the spreading (ibm_st1bb, ibm.cu:133-212).  Once set up, lbmibm_step
+ * runs Adhesion() after stress (main.cu:84-86).  Random numbers: splitmix64
+ * of (seed, iteration, spring), one draw per spring per call; a receptor
+ * wanted by several springs in one call goes to the lowest spring id
+ * (vertex * springs_per_knob + spring), the others try their next one.
+ * The reference draws from curand seeded by clock() and races on the
+ * receptor flag. */
+typedef struct lbmibm_adhesion_params {
+    double receptor_density;  /* RECEPTOR_DENSITY; receptor spacing sqrt(1/density)/RC   */
+    double rc;                /* RC: physical length of one unit                           */
+    double k_ad, l0;          /* K_AD (spring constant), L0 (rest length in dx)            */
+    double k_on0, k_off0;     /* K_ON0, K_OFF0 (rates, divided by the shear rate)          */
+    double chi_beta, kbt;     /* ChiBETA (reactive compliance), kBT                        */
+    double shear_rate;        /* cdo->sr                                                   */
+    double gs;                /* GS: the spring force is K_AD/GS (r - L0 dx) e             */
+    int    limit;             /* LIMIT: search radius in dx                                */
+    int    max_neighwall;     /* MAX_NEIGHWALL: receptors kept per knob, nearest first     */
+    int    springs_per_knob;  /* NUMSPRING_KNOB                                            */
+    double knob;              /* KNOB: expected knobs per cell when is_knob is NULL        */
+    unsigned long long seed;
+} lbmibm_adhesion_params;
+/* adhesion_init + wall_init (adhesion.cu:316-466): receptor grid, knobs
+ * (is_knob [Nv], the reference's ./input/icosa%d.dat flags, or NULL to draw
+ * them with probability KNOB / cell vertices), the neighbour search and the
+ * attachment of every spring to the nearest free receptors.  Needs cells and
+ * the slab (no periodic_y, no tube). */
+lbmibm_status lbmibm_adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *a, const unsigned char *is_knob);
+/* Adhesion() (adhesion.cu:772-790): spring force, then search + Bell. */
+lbmibm_status lbmibm_adhesion(lbmibm_ctx *ctx);
+/* Any pointer may be NULL: is_knob [Nv], id_wall / r_wall [Nv*MAXN], attach
+ * (receptor or -1) / r_ij [Nv*NS], is_attached [receptors], f_adh [3Nv] AoS. */
+lbmibm_status lbmibm_adhesion_state(lbmibm_ctx *ctx, unsigned char *is_knob, int *id_wall, double *r_wall,
+                                    int *attach, double *r_ij, int *is_attached, double *f_adh);
+
 /* ---- the reference's binary files (etc.cu) ------------------------------
  * Byte layouts of output_data / restart_file: native-endian int32 and FP64
  * arrays one after another, no headers.  The cell sections assume one mesh

@@ -322,9 +322,10 @@ void orc_ibm_st1(const orc_params *p, orc_state *s) {
                     const double ycos = 1.0 + std::cos(M_PI * yy / (2.0 * dx));
                     const double zcos = 1.0 + std::cos(M_PI * zz / (2.0 * dx));
                     const double dlt = xcos * ycos * zcos / (64.0 * dx * dx * dx);
-                    s->fx[i] += dlt * s->cfx[jid];
-                    s->fy[i] += dlt * s->cfy[jid];
-                    s->fz[i] += dlt * s->cfz[jid];
+                    const double *fa = s->adh ? s->adh->f_adh + 3 * (size_t)jid : nullptr;
+                    s->fx[i] += dlt * (fa ? s->cfx[jid] + fa[0] : s->cfx[jid]);
+                    s->fy[i] += dlt * (fa ? s->cfy[jid] + fa[1] : s->cfy[jid]);
+                    s->fz[i] += dlt * (fa ? s->cfz[jid] + fa[2] : s->cfz[jid]);
                 }
     }
     // ibm_st1c (ibm.cu:214-296): every COMP node near a periodic face adds
@@ -1050,6 +1051,7 @@ int orc_step(const orc_params *p, orc_state *s, int iter, int *vol_iters) {
     if ((iter - 1) % p->m == 0 && s->nc > 0) {
         const int f = orc_stress(p, s, iter);
         if (f) return f;
+        if (s->adh) orc_adhesion_step(p, s, s->adh, iter);  // main.cu:84-86
         orc_ibm_st1(p, s);
     }
     orc_lbm_step(p, s, iter);

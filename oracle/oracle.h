@@ -95,7 +95,34 @@ typedef struct orc_state {
     /* per-node body-force field [n] added to P e (Guo force 2 rho (P e + g)),
      * NULL = none; not in the reference, used by the gradient checks */
     const double *gbx, *gby, *gbz;
+    /* cytoadhesion (adhesion.cu) or NULL: orc_step runs Adhesion() after
+     * stress and spreads cfx + f_adh (ibm_st1bb, ibm.cu:133-212) */
+    struct orc_adhesion *adh;
 } orc_state;
+
+/* Cytoadhesion of the slab (CapsuleSHEARFLOW) with the linear spring law
+ * (adhesion.cu, LINEAR_SPRING).  Receptors on the wall y = -LY/2 on an
+ * nxw x nzw grid (wall_init, adhesion.cu:435-466).  Per vertex: knob flag,
+ * MAXN nearest receptors, springs_per_knob springs.  Randomness: the
+ * counter-based orc_adh_uniform(seed, iter, spring) in (0, 1], one draw per
+ * spring per call (curand_uniform in the reference, adhesion.cu:550-558).
+ * A receptor wanted by several springs in the same call goes to the lowest
+ * spring id (vertex * springs_per_knob + spring); the others try their
+ * next candidate (the reference resolves this with an atomicOr race). */
+typedef struct orc_adhesion {
+    double receptor_density, rc, k_ad, l0, k_on0, k_off0, chi_beta, kbt, shear_rate, gs;
+    int    limit, max_neighwall, springs_per_knob;
+    unsigned long long seed;
+    int    nxw, nzw;            /* receptor grid (set by orc_adhesion_init)  */
+    double dxw, dzw;
+    unsigned char *is_knob;     /* [Nv]            */
+    int    *id_wall;            /* [Nv*MAXN]       */
+    double *r_wall;             /* [Nv*MAXN]       */
+    int    *attach;             /* [Nv*NS], -1 free */
+    double *r_ij;               /* [Nv*NS]         */
+    int    *is_attached;        /* [nxw*nzw]       */
+    double *f_adh;              /* [3Nv] AoS       */
+} orc_adhesion;
 
 /* ---- lattice ---------------------------------------------------------- */
 /* cdo_init mask, slab (walls outside |y|<LY/2, BUFF halo in x,z) or fully
@@ -157,6 +184,18 @@ void orc_thinc_wlic(const orc_params *p, orc_state *s);
  * node i along direction q: the fraction |x_i - x_wall| / |c_q dx| of the
  * tube-wall crossing, or -1 where the reference records no crossing. */
 double orc_tube_fraction(const orc_params *p, int i, int q);
+
+/* Cytoadhesion entry points (orc_adhesion above).  The arrays are the
+ * caller's.  orc_adhesion_grid fills nxw, nzw, dxw, dzw and returns the
+ * receptor count.  orc_adhesion_init: adhesion_init's state, the neighbour
+ * search and the initial attachment to the nearest free receptors
+ * (adhesion.cu:316-404, 278-314).  orc_adhesion_step: Adhesion()
+ * (adhesion.cu:772-790): spring force, then the search and the Bell
+ * attach / detach (adhesion.cu:513-615). */
+int    orc_adhesion_grid(const orc_params *p, orc_adhesion *a);
+void   orc_adhesion_init(const orc_params *p, orc_state *s, orc_adhesion *a);
+void   orc_adhesion_step(const orc_params *p, orc_state *s, orc_adhesion *a, int iter);
+double orc_adh_uniform(unsigned long long seed, long long iter, long long sid);
 
 /* Single-point helpers used by the delta-function tests. */
 double orc_delta_weight(double r_over_dx);                 /* (1+cos(pi r/2))/4     */

@@ -56,6 +56,17 @@ class OrcState(C.Structure):
         ("volume0", c_double_p), ("volume", c_double_p), ("area", c_double_p), ("gx", c_double_p),
         ("bicg_iters", c_int_p),
         ("gbx", c_double_p), ("gby", c_double_p), ("gbz", c_double_p),
+        ("adh", C.c_void_p),
+    ]
+
+
+class OrcAdhesion(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("receptor_density", "rc", "k_ad", "l0", "k_on0", "k_off0", "chi_beta",
+                                          "kbt", "shear_rate", "gs")] + [
+        ("limit", C.c_int), ("max_neighwall", C.c_int), ("springs_per_knob", C.c_int), ("seed", C.c_ulonglong),
+        ("nxw", C.c_int), ("nzw", C.c_int), ("dxw", C.c_double), ("dzw", C.c_double),
+        ("is_knob", C.POINTER(C.c_ubyte)), ("id_wall", c_int_p), ("r_wall", c_double_p), ("attach", c_int_p),
+        ("r_ij", c_double_p), ("is_attached", c_int_p), ("f_adh", c_double_p),
     ]
 
 
@@ -107,6 +118,10 @@ def lib():
                                             C.c_double, C.c_double]),
             "orc_thinc_wlic": (None, [P, S]),
             "orc_tube_fraction": (C.c_double, [P, C.c_int, C.c_int]),
+            "orc_adhesion_grid": (C.c_int, [P, C.POINTER(OrcAdhesion)]),
+            "orc_adhesion_init": (None, [P, S, C.POINTER(OrcAdhesion)]),
+            "orc_adhesion_step": (None, [P, S, C.POINTER(OrcAdhesion), C.c_int]),
+            "orc_adh_uniform": (C.c_double, [C.c_ulonglong, C.c_longlong, C.c_longlong]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -342,6 +357,32 @@ class OracleSim:
         out = [np.zeros(self.n) for _ in range(3)]
         lib().orc_vof_normals(C.byref(self.P), dp(np.ascontiguousarray(vf, np.float64)), *[dp(a) for a in out])
         return out
+
+    # -- cytoadhesion ----------------------------------------------------------
+    def adhesion_setup(self, params, is_knob):
+        """adhesion_init with the given constants (dict of OrcAdhesion
+        fields) and knob flags [Nv]; orc_step then runs Adhesion()."""
+        A = OrcAdhesion()
+        for k, v in params.items():
+            setattr(A, k, v)
+        nrec = lib().orc_adhesion_grid(C.byref(self.P), C.byref(A))
+        nv = int(self.cells["voff"][-1])
+        ns, mn = A.springs_per_knob, A.max_neighwall
+        self._adh = dict(is_knob=np.ascontiguousarray(is_knob, np.uint8).copy(), id_wall=np.zeros(nv * mn, np.int32),
+                         r_wall=np.zeros(nv * mn), attach=np.zeros(nv * ns, np.int32), r_ij=np.zeros(nv * ns),
+                         is_attached=np.zeros(nrec, np.int32), f_adh=np.zeros(3 * nv))
+        A.is_knob = self._adh["is_knob"].ctypes.data_as(C.POINTER(C.c_ubyte))
+        for k in ("id_wall", "attach", "is_attached"):
+            setattr(A, k, ip(self._adh[k]))
+        for k in ("r_wall", "r_ij", "f_adh"):
+            setattr(A, k, dp(self._adh[k]))
+        self._adhs = A
+        lib().orc_adhesion_init(C.byref(self.P), C.byref(self.S), C.byref(A))
+        self.S.adh = C.addressof(A)
+        return self._adh
+
+    def adhesion_step(self, it):
+        lib().orc_adhesion_step(C.byref(self.P), C.byref(self.S), C.byref(self._adhs), it)
 
     def set_body_force_field(self, gx=None, gy=None, gz=None):
         """Per-node body-force field (padded [n]) added to P e; None removes it."""

@@ -19,7 +19,7 @@ LIB = os.path.join(PKG, "liblbmibm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["api.hip", "lattice.hip", "ibm.hip", "membrane.hip", "volume.hip", "adjoint.hip", "files.hip", "vof.hip", "mesh.cpp"]
+SOURCES = ["api.hip", "lattice.hip", "ibm.hip", "membrane.hip", "volume.hip", "adjoint.hip", "files.hip", "vof.hip", "adhesion.hip", "mesh.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
           "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
 HIP_ONLY = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1,0 +1,364 @@
+// Cytoadhesion of cell knobs to wall receptors (adhesion.cu), for the slab
+// (CapsuleSHEARFLOW) with the linear spring law: receptors on the wall
+// y = -LY/2 on an nxw x nzw grid (wall_init, adhesion.cu:441-458), knobs
+// with springs_per_knob springs, Bell-model attach / detach
+// (adhesion.cu:513-615) and the spring force that the spreading adds to the
+// membrane force (ibm_st1bb, ibm.cu:133-212).
+//
+// The reference leaves two things to the hardware; both are fixed here:
+//   - random numbers: splitmix64 of (seed, iteration, spring id), one draw
+//     per spring per call, mapped to (0, 1] (curand seeded from clock() in
+//     the reference);
+//   - a receptor wanted by several springs in one call goes to the lowest
+//     spring id (vertex * springs_per_knob + spring), the others trying
+//     their next candidate, in rounds (an atomicOr race in the reference).
+// Detachments go first; a spring that detaches does not re-attach in the
+// same call (one draw decides either branch in the reference).
+#include <climits>
+#include <vector>
+
+#include "common.hpp"
+
+namespace lbmibm {
+namespace {
+
+constexpr int AWG = 256;
+
+struct AdhDev {
+    double rc, k_ad, l0, k_on0, k_off0, chi_beta, kbt, shear_rate, gs;
+    int limit, maxn, ns, nxw, nzw;
+    double dxw, dzw, lx, ly, lz, dx, dtm;  // dtm = dt M
+    unsigned long long seed;
+    int nv;
+};
+
+__host__ __device__ __forceinline__ double adh_uniform(unsigned long long seed, long long iter, long long sid) {
+    // splitmix64 (Steele, Lea, Flood 2014) of the key, mapped to (0, 1]
+    unsigned long long z = seed ^ ((unsigned long long)iter * 0x9E3779B97F4A7C15ull) ^
+                           ((unsigned long long)sid * 0xD1B54A32D192ED03ull);
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (double)((z >> 11) + 1) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ void receptor(const AdhDev &a, int id, double &x, double &y, double &z) {
+    const int iz = id / a.nxw, ix = id - iz * a.nxw;
+    x = (ix - a.nxw / 2) * a.dxw;
+    y = -0.5 * a.ly;
+    z = (iz - a.nzw / 2) * a.dzw;
+}
+
+__device__ __forceinline__ double image_shift(double d, double L, double xi, double xj) {
+    if (fabs(d) > L / 2.0) {  // adhesion.cu:656-668
+        if (xi > xj) d += L;
+        else if (xi < xj) d -= L;
+    }
+    return d;
+}
+
+__device__ __forceinline__ double p_on(const AdhDev &a, double r, double &dr) {
+#pragma clang fp contract(off)
+    dr = fmax(0.0, (r - a.l0 * a.dx) * a.rc);  // adhesion.cu:562-573
+    const double f_ad = fabs(a.k_ad * dr);
+    const double k_on = a.k_on0 / a.shear_rate * exp(f_ad * (a.chi_beta - 0.5 * dr) / a.kbt);
+    return 1.0 - exp(-k_on * a.dtm);
+}
+
+// AdhesiveSpringForce_GPU (adhesion.cu:617-706) and the spreading force
+// cf + f_adh of ibm_st1bb.
+template <typename T>
+__global__ __launch_bounds__(AWG) void k_adh_force(AdhDev a, const double *__restrict__ xs,
+                                                   const T *__restrict__ cf, const unsigned char *__restrict__ knob,
+                                                   const int *__restrict__ attach, double *__restrict__ f_adh,
+                                                   T *__restrict__ fsum) {
+#pragma clang fp contract(off)
+    const int v = blockIdx.x * AWG + threadIdx.x;
+    if (v >= a.nv) return;
+    double f[3] = {0.0, 0.0, 0.0};
+    if (knob[v]) {
+        const double xi = xs[v], yi = xs[a.nv + v], zi = xs[2 * (size_t)a.nv + v];
+        for (int s = 0; s < a.ns; s++) {
+            const int id = attach[v * a.ns + s];
+            if (id == -1) continue;
+            double xj, yj, zj;
+            receptor(a, id, xj, yj, zj);
+            const double dx = image_shift(xj - xi, a.lx, xi, xj), dy = yj - yi, dz = image_shift(zj - zi, a.lz, zi, zj);
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double dr = r - a.l0 * a.dx;
+            f[0] += a.k_ad / a.gs * dr * dx / r;
+            f[1] += a.k_ad / a.gs * dr * dy / r;
+            f[2] += a.k_ad / a.gs * dr * dz / r;
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        f_adh[3 * (size_t)v + c] = f[c];
+        fsum[c * (size_t)a.nv + v] = (T)((double)cf[c * (size_t)a.nv + v] + f[c]);
+    }
+}
+
+// StoreNeighWall_CUBE_GPU (adhesion.cu:109-206): receptors within LIMIT dx
+// of a knob near the wall, nearest first, ties in scan order, at most MAXN.
+__global__ __launch_bounds__(AWG) void k_adh_search(AdhDev a, const double *__restrict__ xs,
+                                                    const unsigned char *__restrict__ knob, int *__restrict__ id_wall,
+                                                    double *__restrict__ r_wall) {
+#pragma clang fp contract(off)
+    const int v = blockIdx.x * AWG + threadIdx.x;
+    if (v >= a.nv) return;
+    int *ids = id_wall + (size_t)v * a.maxn;
+    double *rs = r_wall + (size_t)v * a.maxn;
+    for (int k = 0; k < a.maxn; k++) { ids[k] = -1; rs[k] = 1.0e15; }
+    if (!knob[v]) return;
+    const double xi = xs[v], yi = xs[a.nv + v], zi = xs[2 * (size_t)a.nv + v];
+    if (sqrt(yi * yi) < 0.5 * a.ly - a.limit * a.dx) return;
+    const int iz = (int)((zi - (-a.lz / 2.0)) / a.dzw);
+    const int span = a.limit * (int)(a.dx / a.dzw);
+    int cnt = 0;
+    for (int i = 0; i < 2 * span; i++) {
+        int jz = iz + i - (span - 1);
+        double zj = (jz - a.nzw / 2) * a.dzw;
+        if (jz < 0) { jz += a.nzw; zj += a.lz; }
+        else if (jz >= a.nzw) { jz -= a.nzw; zj -= a.lz; }
+        for (int jx = 0; jx < a.nxw; jx++) {
+            const int id = jz * a.nxw + jx;
+            double xj, yj, zr;
+            receptor(a, id, xj, yj, zr);
+            const double r = sqrt((xj - xi) * (xj - xi) + (yj - yi) * (yj - yi) + (zj - zi) * (zj - zi));
+            if (r > a.limit * a.dx) continue;
+            int pos = cnt < a.maxn ? cnt : a.maxn;
+            while (pos > 0 && rs[pos - 1] > r) pos--;
+            if (pos >= a.maxn) continue;
+            for (int k = (cnt < a.maxn ? cnt : a.maxn - 1); k > pos; k--) { ids[k] = ids[k - 1]; rs[k] = rs[k - 1]; }
+            ids[pos] = id;
+            rs[pos] = r;
+            if (cnt < a.maxn) cnt++;
+        }
+    }
+}
+
+// Detachment of attached springs (adhesion.cu:584-612, z image only); the
+// free springs become the candidates of the attachment rounds.
+__global__ __launch_bounds__(AWG) void k_adh_detach(AdhDev a, const double *__restrict__ xs,
+                                                    const unsigned char *__restrict__ knob, int *__restrict__ attach,
+                                                    double *__restrict__ r_ij, int *__restrict__ is_attached,
+                                                    int *__restrict__ pos, int *__restrict__ active,
+                                                    const DevStep *__restrict__ st, int init) {
+#pragma clang fp contract(off)
+    const int sid = blockIdx.x * AWG + threadIdx.x;
+    if (sid >= a.nv * a.ns) return;
+    const int v = sid / a.ns;
+    pos[sid] = 0;
+    active[sid] = 0;
+    if (!knob[v]) return;
+    const int id = attach[sid];
+    if (id == -1) {
+        active[sid] = 1;
+        return;
+    }
+    if (init) return;
+    double xj, yj, zj;
+    receptor(a, id, xj, yj, zj);
+    const double xi = xs[v], yi = xs[a.nv + v], zi = xs[2 * (size_t)a.nv + v];
+    const double dx = xj - xi, dy = yj - yi, dz = image_shift(zj - zi, a.lz, zi, zj);
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double dr = fmax(0.0, (r - a.l0 * a.dx) * a.rc);
+    const double f_ad = fabs(a.k_ad * dr);
+    const double k_off = a.k_off0 / a.shear_rate * exp(f_ad * a.chi_beta / a.kbt);
+    const double p_off = 1.0 - exp(-k_off * a.dtm);
+    if (p_off <= adh_uniform(a.seed, st->iter, sid)) return;
+    is_attached[id] = 0;
+    attach[sid] = -1;
+    r_ij[sid] = dr;
+}
+
+// One attachment round: each active spring proposes its next acceptable
+// free candidate and bids for it with its id.
+__global__ __launch_bounds__(AWG) void k_adh_propose(AdhDev a, const int *__restrict__ id_wall,
+                                                     const double *__restrict__ r_wall,
+                                                     const int *__restrict__ is_attached, int *__restrict__ pos,
+                                                     int *__restrict__ active, int *__restrict__ prop,
+                                                     int *__restrict__ claim, const DevStep *__restrict__ st,
+                                                     int init) {
+    const int sid = blockIdx.x * AWG + threadIdx.x;
+    if (sid >= a.nv * a.ns) return;
+    prop[sid] = -1;
+    if (!active[sid]) return;
+    const int v = sid / a.ns;
+    const double u = init ? 0.0 : adh_uniform(a.seed, st->iter, sid);
+    int c = pos[sid];
+    for (; c < a.maxn; c++) {
+        const int id = id_wall[(size_t)v * a.maxn + c];
+        if (id == -1) continue;
+        if (!init) {
+            double dr;
+            if (p_on(a, r_wall[(size_t)v * a.maxn + c], dr) <= u) continue;
+        }
+        if (is_attached[id]) continue;
+        break;
+    }
+    if (c >= a.maxn) {
+        active[sid] = 0;
+        return;
+    }
+    pos[sid] = c;
+    const int id = id_wall[(size_t)v * a.maxn + c];
+    prop[sid] = id;
+    atomicMin(claim + id, sid);
+}
+
+__global__ __launch_bounds__(AWG) void k_adh_resolve(AdhDev a, const double *__restrict__ r_wall,
+                                                     int *__restrict__ attach, double *__restrict__ r_ij,
+                                                     int *__restrict__ is_attached, int *__restrict__ pos,
+                                                     int *__restrict__ active, const int *__restrict__ prop,
+                                                     const int *__restrict__ claim, int init) {
+    const int sid = blockIdx.x * AWG + threadIdx.x;
+    if (sid >= a.nv * a.ns) return;
+    const int id = prop[sid];
+    if (id < 0) return;
+    if (claim[id] == sid) {
+        attach[sid] = id;
+        is_attached[id] = 1;
+        if (!init) {
+            double dr;
+            (void)p_on(a, r_wall[(size_t)(sid / a.ns) * a.maxn + pos[sid]], dr);
+            r_ij[sid] = dr;
+        }
+        active[sid] = 0;
+    } else {
+        pos[sid] = pos[sid] + 1;
+    }
+}
+
+__global__ __launch_bounds__(AWG) void k_adh_clear(int n, const int *__restrict__ prop, int *__restrict__ claim) {
+    const int sid = blockIdx.x * AWG + threadIdx.x;
+    if (sid >= n) return;
+    const int id = prop[sid];
+    if (id >= 0) claim[id] = INT_MAX;
+}
+
+template <typename V> __global__ void k_afill(V *__restrict__ p, size_t n, V v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+AdhDev adh_dev(const lbmibm_ctx *ctx) {
+    const lbmibm_adhesion_params &h = ctx->adh;
+    AdhDev a;
+    a.rc = h.rc; a.k_ad = h.k_ad; a.l0 = h.l0; a.k_on0 = h.k_on0; a.k_off0 = h.k_off0;
+    a.chi_beta = h.chi_beta; a.kbt = h.kbt; a.shear_rate = h.shear_rate; a.gs = h.gs;
+    a.limit = h.limit; a.maxn = h.max_neighwall; a.ns = h.springs_per_knob;
+    a.nxw = ctx->adh_nxw; a.nzw = ctx->adh_nzw; a.dxw = ctx->adh_dxw; a.dzw = ctx->adh_dzw;
+    a.dx = ctx->p.dx;
+    a.lx = ctx->p.lx * a.dx; a.ly = ctx->p.ly * a.dx; a.lz = ctx->p.lz * a.dx;
+    a.dtm = ctx->p.dt * (double)ctx->p.m;
+    a.seed = h.seed;
+    a.nv = ctx->nv;
+    return a;
+}
+
+inline unsigned ablocks(long long n) { return (unsigned)((n + AWG - 1) / AWG); }
+
+// Search, detachment and the attachment rounds (init: the nearest free
+// receptors with no draw, AttachmentPointAtWall_GPU, adhesion.cu:278-314).
+lbmibm_status adh_bell(lbmibm_ctx *ctx, int init) {
+    const AdhDev a = adh_dev(ctx);
+    hipStream_t st = ctx->stream;
+    const long long nsp = (long long)a.nv * a.ns;
+    hipLaunchKernelGGL(k_adh_search, dim3(ablocks(a.nv)), dim3(AWG), 0, st, a, (const double *)ctx->xs,
+                       (const unsigned char *)ctx->adh_knob, ctx->adh_idw, ctx->adh_rw);
+    hipLaunchKernelGGL(k_adh_detach, dim3(ablocks(nsp)), dim3(AWG), 0, st, a, (const double *)ctx->xs,
+                       (const unsigned char *)ctx->adh_knob, ctx->adh_attach, ctx->adh_rij, ctx->adh_isatt,
+                       ctx->adh_pos, ctx->adh_active, (const DevStep *)ctx->step, init);
+    for (int r = 0; r <= a.maxn; r++) {  // each round retires or advances every active spring
+        hipLaunchKernelGGL(k_adh_propose, dim3(ablocks(nsp)), dim3(AWG), 0, st, a, (const int *)ctx->adh_idw,
+                           (const double *)ctx->adh_rw, (const int *)ctx->adh_isatt, ctx->adh_pos, ctx->adh_active,
+                           ctx->adh_prop, ctx->adh_claim, (const DevStep *)ctx->step, init);
+        hipLaunchKernelGGL(k_adh_resolve, dim3(ablocks(nsp)), dim3(AWG), 0, st, a, (const double *)ctx->adh_rw,
+                           ctx->adh_attach, ctx->adh_rij, ctx->adh_isatt, ctx->adh_pos, ctx->adh_active,
+                           (const int *)ctx->adh_prop, (const int *)ctx->adh_claim, init);
+        hipLaunchKernelGGL(k_adh_clear, dim3(ablocks(nsp)), dim3(AWG), 0, st, (int)nsp, (const int *)ctx->adh_prop,
+                           ctx->adh_claim);
+    }
+    LBMIBM_LAUNCHED(ctx, "k_adh_*");
+    return LBMIBM_OK;
+}
+
+}  // namespace
+
+// Adhesion() (adhesion.cu:772-790): the spring force of the current
+// attachments (and the spreading force cf + f_adh), then the Bell update.
+lbmibm_status adhesion_step(lbmibm_ctx *ctx) {
+    const AdhDev a = adh_dev(ctx);
+    if (ctx->prec == LBMIBM_F64)
+        hipLaunchKernelGGL(k_adh_force<double>, dim3(ablocks(a.nv)), dim3(AWG), 0, ctx->stream, a,
+                           (const double *)ctx->xs, (const double *)ctx->cf, (const unsigned char *)ctx->adh_knob,
+                           (const int *)ctx->adh_attach, ctx->adh_fadh, (double *)ctx->adh_fsum);
+    else
+        hipLaunchKernelGGL(k_adh_force<float>, dim3(ablocks(a.nv)), dim3(AWG), 0, ctx->stream, a,
+                           (const double *)ctx->xs, (const float *)ctx->cf, (const unsigned char *)ctx->adh_knob,
+                           (const int *)ctx->adh_attach, ctx->adh_fadh, (float *)ctx->adh_fsum);
+    LBMIBM_LAUNCHED(ctx, "k_adh_force");
+    return adh_bell(ctx, 0);
+}
+
+lbmibm_status adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *h, const unsigned char *is_knob) {
+    ctx->adh = *h;
+    // wall_init, CapsuleSHEARFLOW (adhesion.cu:441-458)
+    const double dx = ctx->p.dx, LX = ctx->p.lx * dx, LZ = ctx->p.lz * dx;
+    const double dz = std::sqrt(1.0 / h->receptor_density) / h->rc;
+    ctx->adh_nzw = (int)(LZ / dz + 1.0e-03);
+    ctx->adh_nxw = (int)(LX / dz + 1.0e-03);
+    if (ctx->adh_nzw < 1 || ctx->adh_nxw < 1) {
+        ctx->err = "adhesion: receptor spacing larger than the box";
+        return LBMIBM_ERR_ARG;
+    }
+    ctx->adh_dzw = LZ / (double)ctx->adh_nzw;
+    ctx->adh_dxw = LX / (double)ctx->adh_nxw;
+    const long long nrec = (long long)ctx->adh_nxw * ctx->adh_nzw, nv = ctx->nv;
+    const long long nsp = nv * h->springs_per_knob, nn = nv * h->max_neighwall;
+    // knobs: the caller's flags (the reference's ./input/icosa file) or
+    // drawn with probability KNOB / (vertices of the cell)
+    std::vector<int> voff(ctx->nc + 1);
+    LBMIBM_HIP(ctx, hipMemcpy(voff.data(), ctx->voff, voff.size() * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<unsigned char> knob((size_t)nv, 0);
+    for (int c = 0; c < ctx->nc; c++)
+        for (int v = voff[c]; v < voff[c + 1]; v++) {
+            if (is_knob) {
+                knob[v] = is_knob[v] ? 1 : 0;
+            } else {  // iteration -1 is the knob draw's stream
+                knob[v] = adh_uniform(h->seed, -1, v) < h->knob / (double)(voff[c + 1] - voff[c]) ? 1 : 0;
+            }
+        }
+    lbmibm_status s = LBMIBM_OK;
+    auto al = [&](void **p, size_t bytes) {
+        if (s == LBMIBM_OK && !*p) s = dev_alloc(ctx, p, bytes);
+    };
+    al(&ctx->adh_knob, nv);
+    al((void **)&ctx->adh_idw, nn * sizeof(int));
+    al((void **)&ctx->adh_rw, nn * sizeof(double));
+    al((void **)&ctx->adh_attach, nsp * sizeof(int));
+    al((void **)&ctx->adh_rij, nsp * sizeof(double));
+    al((void **)&ctx->adh_pos, nsp * sizeof(int));
+    al((void **)&ctx->adh_active, nsp * sizeof(int));
+    al((void **)&ctx->adh_prop, nsp * sizeof(int));
+    al((void **)&ctx->adh_isatt, nrec * sizeof(int));
+    al((void **)&ctx->adh_claim, nrec * sizeof(int));
+    al((void **)&ctx->adh_fadh, 3 * nv * sizeof(double));
+    al(&ctx->adh_fsum, 3 * nv * ctx->szT);
+    if (s != LBMIBM_OK) return s;
+    LBMIBM_HIP(ctx, hipMemcpy(ctx->adh_knob, knob.data(), nv, hipMemcpyHostToDevice));
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nsp)), dim3(AWG), 0, st, ctx->adh_attach, (size_t)nsp, -1);
+    hipLaunchKernelGGL(k_afill<double>, dim3(ablocks(nsp)), dim3(AWG), 0, st, ctx->adh_rij, (size_t)nsp, -1.0e10);
+    hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nrec)), dim3(AWG), 0, st, ctx->adh_isatt, (size_t)nrec, 0);
+    hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nrec)), dim3(AWG), 0, st, ctx->adh_claim, (size_t)nrec, INT_MAX);
+    hipLaunchKernelGGL(k_afill<double>, dim3(ablocks(3 * nv)), dim3(AWG), 0, st, ctx->adh_fadh, (size_t)(3 * nv), 0.0);
+    LBMIBM_LAUNCHED(ctx, "k_afill");
+    ctx->adh_on = true;
+    s = adh_bell(ctx, 1);
+    if (s == LBMIBM_OK) LBMIBM_HIP(ctx, hipStreamSynchronize(st));
+    return s;
+}
+
+}  // namespace lbmibm

@@ -220,6 +220,7 @@ lbmibm_status enqueue_step(lbmibm_ctx *ctx, int iter, bool device_advance, bool 
         if (ctx->p.br != 0.0) s = join_volume(ctx);
         if (s == LBMIBM_OK) s = membrane_forces(ctx);
         if (s == LBMIBM_OK) s = join_volume(ctx);
+        if (s == LBMIBM_OK && ctx->adh_on) s = adhesion_step(ctx);  // Adhesion(), main.cu:84-86
         if (s == LBMIBM_OK) s = ibm_spread(ctx);
     }
     if (s == LBMIBM_OK) s = lattice_step(ctx);
@@ -265,7 +266,9 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->adj_x, ctx->adj_cf, ctx->grad_H, ctx->grad_gs, ctx->adj_lam, ctx->adj_dsv,
                     ctx->adj_corner, ctx->adj_scratch, ctx->adj_cscale, ctx->adj_gpart, ctx->adj_xmax, ctx->adj_rec, ctx->sbox, ctx->sb_cand,
                     ctx->vof[0], ctx->vof[1], ctx->ft_g, ctx->ft_x, ctx->ft_b, ctx->ft_part, ctx->ft_ctl,
-                    ctx->tube_col, ctx->tube_r};
+                    ctx->tube_col, ctx->tube_r, ctx->adh_knob, ctx->adh_idw, ctx->adh_attach, ctx->adh_pos,
+                    ctx->adh_active, ctx->adh_prop, ctx->adh_isatt, ctx->adh_claim, ctx->adh_rw, ctx->adh_rij,
+                    ctx->adh_fadh, ctx->adh_fsum};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -802,6 +805,50 @@ lbmibm_status lbmibm_upload_vf(lbmibm_ctx *ctx, const double *vf) {
                 v[ix + (size_t)ctx->nx * (iy + (size_t)ctx->ny * iz)] = vf[padded_index(ctx, ix, iy, iz)];
     LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     LBMIBM_HIP(ctx, hipMemcpy(ctx->vof[ctx->vof_cur], v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    return LBMIBM_OK;
+}
+
+lbmibm_status lbmibm_adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *a, const unsigned char *is_knob) {
+    if (!ctx || !a) return LBMIBM_ERR_ARG;
+    if (ctx->nc < 1 || ctx->p.periodic_y || ctx->p.tube_radius > 0.0 || !(a->receptor_density > 0.0) ||
+        !(a->rc > 0.0) || !(a->shear_rate > 0.0) || !(a->kbt > 0.0) || a->gs == 0.0 || a->limit < 1 ||
+        a->max_neighwall < 1 || a->springs_per_knob < 1) {
+        ctx->err = "adhesion: needs cells, the slab and positive constants";
+        return LBMIBM_ERR_ARG;
+    }
+    lbmibm_status s = join_volume(ctx);
+    if (s == LBMIBM_OK) s = adhesion_setup(ctx, a, is_knob);
+    ctx->graph_ready = false;
+    return s;
+}
+
+lbmibm_status lbmibm_adhesion(lbmibm_ctx *ctx) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    if (!ctx->adh_on) {
+        ctx->err = "adhesion: call lbmibm_adhesion_setup first";
+        return LBMIBM_ERR_STATE;
+    }
+    lbmibm_status s = join_volume(ctx);
+    return s == LBMIBM_OK ? adhesion_step(ctx) : s;
+}
+
+lbmibm_status lbmibm_adhesion_state(lbmibm_ctx *ctx, unsigned char *is_knob, int *id_wall, double *r_wall,
+                                    int *attach, double *r_ij, int *is_attached, double *f_adh) {
+    if (!ctx) return LBMIBM_ERR_ARG;
+    if (!ctx->adh_on) {
+        ctx->err = "adhesion: call lbmibm_adhesion_setup first";
+        return LBMIBM_ERR_STATE;
+    }
+    LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t nv = ctx->nv, nn = nv * ctx->adh.max_neighwall, ns = nv * ctx->adh.springs_per_knob;
+    const size_t nrec = (size_t)ctx->adh_nxw * ctx->adh_nzw;
+    if (is_knob) LBMIBM_HIP(ctx, hipMemcpy(is_knob, ctx->adh_knob, nv, hipMemcpyDeviceToHost));
+    if (id_wall) LBMIBM_HIP(ctx, hipMemcpy(id_wall, ctx->adh_idw, nn * sizeof(int), hipMemcpyDeviceToHost));
+    if (r_wall) LBMIBM_HIP(ctx, hipMemcpy(r_wall, ctx->adh_rw, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (attach) LBMIBM_HIP(ctx, hipMemcpy(attach, ctx->adh_attach, ns * sizeof(int), hipMemcpyDeviceToHost));
+    if (r_ij) LBMIBM_HIP(ctx, hipMemcpy(r_ij, ctx->adh_rij, ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (is_attached) LBMIBM_HIP(ctx, hipMemcpy(is_attached, ctx->adh_isatt, nrec * sizeof(int), hipMemcpyDeviceToHost));
+    if (f_adh) LBMIBM_HIP(ctx, hipMemcpy(f_adh, ctx->adh_fadh, 3 * nv * sizeof(double), hipMemcpyDeviceToHost));
     return LBMIBM_OK;
 }
 

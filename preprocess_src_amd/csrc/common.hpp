@@ -164,6 +164,20 @@ struct lbmibm_ctx {
     // disc, bit q the pull source of q is a wall) and wall fractions [col][19]
     void *tube_col = nullptr, *tube_r = nullptr;
     std::vector<double> tube_rh;  // host copy of the fractions
+    // cytoadhesion (after lbmibm_adhesion_setup): constants, receptor grid,
+    // per-vertex knob flags [Nv], receptor lists [Nv*MAXN], springs
+    // [Nv*NS] (receptor or -1, r_ij, round state), receptor flags and
+    // claims [nrec], f_adh [3Nv] AoS FP64 and the spreading force cf + f_adh
+    // [3][Nv] (T)
+    bool adh_on = false;
+    lbmibm_adhesion_params adh{};
+    int adh_nxw = 0, adh_nzw = 0;
+    double adh_dxw = 0.0, adh_dzw = 0.0;
+    void *adh_knob = nullptr;
+    int *adh_idw = nullptr, *adh_attach = nullptr, *adh_pos = nullptr, *adh_active = nullptr, *adh_prop = nullptr;
+    int *adh_isatt = nullptr, *adh_claim = nullptr;
+    double *adh_rw = nullptr, *adh_rij = nullptr, *adh_fadh = nullptr;
+    void *adh_fsum = nullptr;
     lbmibm::DevStep *step = nullptr;
     lbmibm::DevStatus *status = nullptr;
 
@@ -198,6 +212,10 @@ lbmibm_status volume_run(lbmibm_ctx *ctx, int mode);         // volume.hip (0 in
 lbmibm_status vof_alloc(lbmibm_ctx *ctx);                    // vof.hip
 lbmibm_status vof_front_tracking(lbmibm_ctx *ctx, int *iterations, int *attempts);  // vof.hip (host sync)
 lbmibm_status vof_thinc(lbmibm_ctx *ctx);                    // vof.hip
+lbmibm_status adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *a, const unsigned char *is_knob);  // adhesion.hip
+lbmibm_status adhesion_step(lbmibm_ctx *ctx);                // adhesion.hip
+// the nodal force the spreading reads: cf, or cf + f_adh with cytoadhesion
+inline const void *spread_force(const lbmibm_ctx *ctx) { return ctx->adh_on ? ctx->adh_fsum : ctx->cf; }
 // both classes on the given stream (not the context's main stream)
 lbmibm_status volume_run_on(lbmibm_ctx *ctx, int mode, hipStream_t stream);  // volume.hip
 lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool advance);  // api.hip

@@ -1384,7 +1384,7 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
                        ctx->ntiles);
     LBMIBM_LAUNCHED(ctx, "k_zero_int");
     hipLaunchKernelGGL(k_bin_count<T>, dim3(ctx->nblk_bin), dim3(256), 0, stream, (const double *)ctx->xs,
-                       (const T *)ctx->cf, nv, g, tg, ctx->home, ctx->tile_count, ctx->blk_fmax);
+                       (const T *)spread_force(ctx), nv, g, tg, ctx->home, ctx->tile_count, ctx->blk_fmax);
     LBMIBM_LAUNCHED(ctx, "k_bin_count");
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->tile_off, ctx->tile_cur,
                        ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax);
@@ -1393,7 +1393,7 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
                        ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->order_calls);
     LBMIBM_LAUNCHED(ctx, "k_tile_order");
     hipLaunchKernelGGL(k_bin_scatter<T>, dim3(ctx->nblk_bin), dim3(256), 0, stream, (const double *)ctx->xs,
-                       (const T *)ctx->cf, ctx->home, nv, g, ctx->tile_off, ctx->tile_cur, ctx->home_list,
+                       (const T *)spread_force(ctx), ctx->home, nv, g, ctx->tile_off, ctx->tile_cur, ctx->home_list,
                        ctx->node0, (SpreadRec<T> *)ctx->srec);
     LBMIBM_LAUNCHED(ctx, "k_bin_scatter");
     ctx->bins_valid = true;
@@ -1431,7 +1431,7 @@ template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->absmax, 1);
         LBMIBM_LAUNCHED(ctx, "k_zero_int");
         hipLaunchKernelGGL(k_bin_forces<T>, dim3(ctx->nblk_bin), dim3(256), 0, ctx->stream, ctx->home_list,
-                           (const T *)ctx->cf, ctx->nv, (SpreadRec<T> *)ctx->srec, ctx->absmax);
+                           (const T *)spread_force(ctx), ctx->nv, (SpreadRec<T> *)ctx->srec, ctx->absmax);
         LBMIBM_LAUNCHED(ctx, "k_bin_forces");
     }
     return spread_launch<T>(ctx, ctx->spread_form);

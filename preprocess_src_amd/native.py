@@ -51,6 +51,14 @@ class Params(C.Structure):
     ]
 
 
+class AdhesionParams(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("receptor_density", "rc", "k_ad", "l0", "k_on0", "k_off0", "chi_beta",
+                                          "kbt", "shear_rate", "gs")] + [
+        ("limit", C.c_int), ("max_neighwall", C.c_int), ("springs_per_knob", C.c_int), ("knob", C.c_double),
+        ("seed", C.c_ulonglong),
+    ]
+
+
 class Cells(C.Structure):
     _fields_ = [
         ("nc", C.c_int), ("voff", c_int_p), ("eoff", c_int_p),
@@ -98,6 +106,10 @@ SIGNATURES = {
     "lbmibm_thinc_wlic": (C.c_int, [C.c_void_p]),
     "lbmibm_download_vf": (C.c_int, [C.c_void_p, c_double_p]),
     "lbmibm_upload_vf": (C.c_int, [C.c_void_p, c_double_p]),
+    "lbmibm_adhesion_setup": (C.c_int, [C.c_void_p, C.POINTER(AdhesionParams), C.POINTER(C.c_ubyte)]),
+    "lbmibm_adhesion": (C.c_int, [C.c_void_p]),
+    "lbmibm_adhesion_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_ubyte), c_int_p, c_double_p, c_int_p, c_double_p,
+                                        c_int_p, c_double_p]),
     "lbmibm_set_body_force_field": (C.c_int, [C.c_void_p] + [c_double_p] * 3),
     "lbmibm_adjoint_reset": (C.c_int, [C.c_void_p]),
     "lbmibm_adjoint_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),

@@ -248,6 +248,33 @@ class Simulation:
     def set_vf(self, vf):
         self._ck(self.L.lbmibm_upload_vf(self.ctx, N.dp(np.ascontiguousarray(vf, np.float64))))
 
+    def adhesion_setup(self, params, is_knob=None):
+        """adhesion_init + wall_init; params: dict of AdhesionParams fields;
+        is_knob [Nv] or None (drawn with probability knob / cell vertices)."""
+        A = N.AdhesionParams()
+        for k, v in params.items():
+            setattr(A, k, v)
+        self._adh = A
+        kn = None
+        if is_knob is not None:
+            self._knob = np.ascontiguousarray(is_knob, np.uint8)
+            kn = self._knob.ctypes.data_as(C.POINTER(C.c_ubyte))
+        self._ck(self.L.lbmibm_adhesion_setup(self.ctx, C.byref(A), kn))
+
+    def adhesion(self):
+        self._ck(self.L.lbmibm_adhesion(self.ctx))
+
+    def adhesion_state(self, nrec):
+        A = self._adh
+        nv, mn, ns = self.nv, A.max_neighwall, A.springs_per_knob
+        out = dict(is_knob=np.zeros(nv, np.uint8), id_wall=np.zeros(nv * mn, np.int32), r_wall=np.zeros(nv * mn),
+                   attach=np.zeros(nv * ns, np.int32), r_ij=np.zeros(nv * ns), is_attached=np.zeros(nrec, np.int32),
+                   f_adh=np.zeros(3 * nv))
+        self._ck(self.L.lbmibm_adhesion_state(self.ctx, out["is_knob"].ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                              N.ip(out["id_wall"]), N.dp(out["r_wall"]), N.ip(out["attach"]),
+                                              N.dp(out["r_ij"]), N.ip(out["is_attached"]), N.dp(out["f_adh"])))
+        return out
+
     def comp_mask(self):
         """COMP nodes of the padded reference lattice (init1.cu:164-178)."""
         m = np.zeros((self.nzp, self.nyp, self.nxp), bool)

@@ -1,0 +1,220 @@
+"""Cytoadhesion (adhesion.cu, slab with the linear spring law; ibm_st1bb,
+ibm.cu:133-212).
+
+CPU: the oracle's restatement is pinned by the published splitmix64 output
+(its random numbers), the uniformity of the draws, a brute-force receptor
+search, the nearest-free initial attachment, the closed-form spring force,
+and the Bell rates: with a zero spring constant the fractions of springs
+that attach or detach in one call are p = 1 - exp(-k dt M).  GPU: the
+library's knobs, receptor lists, attachments (integers, exactly) and forces
+against the oracle over whole steps.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as PO
+from tests.pairs import gpu_of, oracle_of, rel
+
+LX, LY, LZ = 24, 16, 20
+ADH = dict(receptor_density=4.0, rc=1.0, k_ad=2e-3, l0=0.5, k_on0=0.6, k_off0=0.25, chi_beta=0.3, kbt=1.0,
+           shear_rate=1.0, gs=1.0, limit=2, max_neighwall=16, springs_per_knob=2, seed=20261202)
+
+
+def _cell(y0=-LY / 2 + 3.5, x0=0.3, z0=-0.4):
+    from preprocess_src_amd import mesh
+    return mesh.assemble([2], np.array([[x0, y0, z0]]), np.eye(3)[None], 3.0, shape="sphere")
+
+
+def _knobs(cells, frac=0.5, seed=3):
+    rng = np.random.default_rng(seed)
+    return (rng.random(len(cells["x"])) < frac).astype(np.uint8)
+
+
+def _receptors(o, A):
+    nxw, nzw = A.nxw, A.nzw
+    ids = np.arange(nxw * nzw)
+    iz, ix = ids // nxw, ids % nxw
+    return np.stack([(ix - nxw // 2) * A.dxw, np.full(ids.shape, -LY / 2), (iz - nzw // 2) * A.dzw], 1)
+
+
+def test_random_numbers_are_splitmix64():
+    # splitmix64 seeded with 1234567: first output 6457827717110365317
+    # (the reference implementation's published sequence)
+    u = PO.lib().orc_adh_uniform(1234567, 0, 0)
+    assert u == ((6457827717110365317 >> 11) + 1) / 2.0 ** 53
+    draws = np.array([PO.lib().orc_adh_uniform(99, it, sid) for it in range(50) for sid in range(400)])
+    assert draws.min() > 0.0 and draws.max() <= 1.0
+    assert abs(draws.mean() - 0.5) < 0.01 and abs(draws.var() - 1 / 12) < 0.005
+    hist = np.histogram(draws, bins=10, range=(0, 1))[0]
+    assert np.all(np.abs(hist - len(draws) / 10) < 5 * np.sqrt(len(draws) / 10))
+
+
+def _setup(knobs=None, **adh):
+    cells = _cell()
+    o = oracle_of(LX, LY, LZ, cells=cells)
+    st = o.adhesion_setup(dict(ADH, **adh), _knobs(cells) if knobs is None else knobs)
+    return o, cells, st
+
+
+def test_receptor_search_is_the_brute_force_neighbourhood():
+    o, cells, st = _setup()
+    A = o._adhs
+    rec = _receptors(o, A)
+    pts = np.stack([cells["x"], cells["y"], cells["z"]], 1)
+    lim = ADH["limit"]
+    MAXN = ADH["max_neighwall"]
+    seen = 0
+    for v in np.flatnonzero(st["is_knob"]):
+        ids = st["id_wall"][v * MAXN:(v + 1) * MAXN]
+        rs = st["r_wall"][v * MAXN:(v + 1) * MAXN]
+        d = np.linalg.norm(rec - pts[v], axis=1)
+        if abs(pts[v, 1]) < LY / 2 - lim:
+            assert np.all(ids == -1)
+            continue
+        got = ids[ids >= 0]
+        assert np.all(np.diff(rs[ids >= 0]) >= 0)
+        assert np.allclose(rs[ids >= 0], d[got], rtol=0, atol=1e-12) and np.all(d[got] <= lim)
+        # every receptor clearly inside the range and the scanned rows is there
+        want = np.flatnonzero(d <= lim - 0.6)
+        if 0 < len(want) <= MAXN:
+            assert set(want) <= set(got.tolist())
+        seen += len(got) > 0
+    assert seen >= 5
+
+
+def test_initial_attachment_takes_the_nearest_free_receptors():
+    o, cells, st = _setup()
+    NS, MAXN = ADH["springs_per_knob"], ADH["max_neighwall"]
+    att = st["attach"].reshape(-1, NS)
+    used = att[att >= 0]
+    assert len(used) == len(set(used.tolist())) and len(used) > 0
+    assert np.array_equal(np.flatnonzero(st["is_attached"]), np.sort(used))
+    taken = set()
+    for v in range(len(att)):  # ascending spring ids: the lowest id wins a receptor
+        cand = [c for c in st["id_wall"][v * MAXN:(v + 1) * MAXN] if c >= 0]
+        free = [c for c in cand if c not in taken]
+        want = free[:NS] + [-1] * (NS - len(free[:NS]))
+        assert att[v].tolist() == want, v
+        taken.update(c for c in want if c >= 0)
+
+
+def test_spring_force_is_the_linear_spring():
+    o, cells, st = _setup()
+    A = o._adhs
+    before = st["attach"].copy()  # the force uses the attachments the call starts from
+    o.adhesion_step(1)
+    rec = _receptors(o, A)
+    NS = ADH["springs_per_knob"]
+    pts = np.stack([cells["x"], cells["y"], cells["z"]], 1)
+    f = np.zeros_like(pts)
+    for v in range(len(pts)):
+        for s in range(NS):
+            rid = before[v * NS + s]
+            if rid < 0:
+                continue
+            d = rec[rid] - pts[v]
+            r = np.linalg.norm(d)
+            f[v] += ADH["k_ad"] / ADH["gs"] * (r - ADH["l0"]) * d / r
+    assert np.abs(f).max() > 0
+    assert np.allclose(st["f_adh"].reshape(-1, 3), f, rtol=1e-13, atol=1e-18)
+
+
+@pytest.mark.parametrize("which", ["on", "off"])
+def test_bell_rates_with_a_zero_spring_constant(which):
+    # f_ad = 0: p_on = 1 - exp(-k_on0/sr dt M), p_off = 1 - exp(-k_off0/sr dt M)
+    cells = _cell()
+    o = oracle_of(LX, LY, LZ, cells=cells)
+    knobs = np.ones(len(cells["x"]), np.uint8)
+    st = o.adhesion_setup(dict(ADH, k_ad=0.0, springs_per_knob=1, max_neighwall=1, k_on0=0.4, k_off0=0.9), knobs)
+    trials = hits = 0
+    for it in range(1, 60):
+        before = st["attach"].copy()
+        o.adhesion_step(it)
+        after = st["attach"]
+        if which == "off":
+            m = before >= 0
+            trials += m.sum()
+            hits += (after[m] == -1).sum()
+        else:
+            cand = st["id_wall"].reshape(-1, 1)[:, 0]
+            m = (before < 0) & (cand >= 0)
+            trials += m.sum()
+            hits += (after[m] >= 0).sum()
+    p = 1.0 - np.exp(-(0.9 if which == "off" else 0.4) * o.P.dt * o.P.m)
+    assert trials > 300
+    assert abs(hits / trials - p) < 4 * np.sqrt(p * (1 - p) / trials), (hits / trials, p)
+
+
+# ---- GPU: the library against the oracle ------------------------------------
+
+def _pair(prec, knobs="given", **kw):
+    cells = _cell()
+    o = oracle_of(LX, LY, LZ, cells=cells, **kw)
+    g = gpu_of(LX, LY, LZ, prec, cells=cells, **kw)
+    kn = _knobs(cells)
+    if knobs == "drawn":  # the library's draw: u(seed, -1, v) < KNOB / V
+        V = len(cells["x"])
+        kn = np.array([PO.lib().orc_adh_uniform(ADH["seed"], -1, v) < 40.0 / V for v in range(V)], np.uint8)
+        g.adhesion_setup(dict(ADH, knob=40.0))
+    else:
+        g.adhesion_setup(ADH, kn)
+    st = o.adhesion_setup(ADH, kn)
+    return o, g, st, o._adhs.nxw * o._adhs.nzw
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("knobs", ["given", "drawn"])
+def test_adhesion_setup_matches_the_oracle(knobs):
+    print(f"[adhesion setup] knobs={knobs}", flush=True)
+    o, g, st, nrec = _pair("f64", knobs)
+    gs = g.adhesion_state(nrec)
+    assert np.array_equal(gs["is_knob"], st["is_knob"])
+    assert np.array_equal(gs["id_wall"], st["id_wall"])
+    assert np.allclose(gs["r_wall"], st["r_wall"], rtol=1e-14, atol=0)
+    assert np.array_equal(gs["attach"], st["attach"]) and (gs["attach"] >= 0).sum() > 5
+    assert np.array_equal(gs["is_attached"], st["is_attached"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-11), ("f32", 2e-5)])
+def test_adhesion_steps_match_the_oracle(prec, tol):
+    print(f"[adhesion steps] prec={prec}", flush=True)
+    kw = dict(tau=0.9, p=5e-6, b=5e-3)
+    o, g, st, nrec = _pair(prec, **kw)
+    X, Y, Z = o.node_coords()
+    vel = 4e-3 * (Y + LY / 2) / LY
+    vel[o.bc == 2] = 0.0
+    o.init_fluid(None, vel, None, None)
+    g.set_equilibrium(None, vel, None, None)
+    o.volume_init()
+    g.volume_init()
+    changes = 0
+    prev = st["attach"].copy()
+    for it in range(1, 13):
+        assert o.step(it)[0] == 0
+        g.step(it)
+        changes += int((st["attach"] != prev).sum())
+        prev = st["attach"].copy()
+    gs = g.adhesion_state(nrec)
+    mism = (gs["attach"] != st["attach"]).mean()
+    assert mism == 0.0 if prec == "f64" else mism < 0.02
+    assert changes > 0  # springs attached and detached on the way
+    assert rel(gs["f_adh"], st["f_adh"]) < tol * 10
+    sg = g.cells_state()
+    for k in ("x", "y", "z"):
+        assert np.max(np.abs(sg[k] - o.cells[k])) < tol * 10, k
+
+
+@pytest.mark.gpu
+def test_adhesion_arguments_are_checked():
+    from preprocess_src_amd import native as N
+    g = gpu_of(LX, LY, LZ, "f64", cells=_cell(), periodic_y=True)
+    with pytest.raises(N.LbmibmError):
+        g.adhesion_setup(ADH)
+    h = gpu_of(LX, LY, LZ, "f64", cells=_cell())
+    with pytest.raises(N.LbmibmError):
+        h.adhesion()  # before set-up
+    h.adhesion_setup(ADH, _knobs(_cell()))
+    with pytest.raises(N.LbmibmError):
+        h.adjoint_reset()
+        h.adj_lbm_d3q19(1)
