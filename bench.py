@@ -10,14 +10,16 @@ seeded mesh mix of 50% 642-, 40% 2562- and 10% 10242-vertex meshes, radius
 populations stored as deviations, whole steps replayed from a hipGraph.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|f64|f16]
-                    [--workload suspension|tgv|adjoint|vof|tube] [--cpu-seconds S] [--no-cpu]
+                    [--workload suspension|tgv|adjoint|vof|tube|adhesion] [--cpu-seconds S]
+                    [--no-cpu]
 
 --workload adjoint times the checkpointed wall-shear gradient on a 128x64x64
 channel with 32 RBCs (500 steps, last 200 averaged, checkpoints every 50)
 against the forward run and checks it against a central difference.
 --workload vof times front tracking (red-black SOR), THINC/WLIC and steps
 with a viscosity contrast on the default suspension; --workload tube the
-lattice step of the interpolated-bounce-back tube.
+lattice step of the interpolated-bounce-back tube; --workload adhesion the
+suspension with cytoadhesion to the lower wall.
 """
 from __future__ import annotations
 
@@ -346,13 +348,45 @@ def tube_bench(a):
     sim.close()
 
 
+def adhesion_bench(a):
+    """The default suspension with cytoadhesion to the lower wall: receptors
+    every 0.5 lu, 40 knobs per cell drawn from the seed, 2 springs per knob,
+    16 receptors searched per knob within 2 lu; whole steps from the graph."""
+    from preprocess_src_amd.simulation import Simulation
+    shape, cells, params, desc, ht = suspension_workload()
+    lx, ly, lz = shape
+    n = lx * ly * lz
+    sim = Simulation(lx, ly, lz, cells=cells, precision=a.precision, **params)
+    sim.volume_init()
+    adh = dict(receptor_density=4.0, rc=1.0, k_ad=2e-3, l0=0.5, k_on0=0.6, k_off0=0.25, chi_beta=0.3, kbt=1.0,
+               shear_rate=1.0, gs=1.0, limit=2, max_neighwall=16, springs_per_knob=2, knob=40.0, seed=2024)
+    sim.adhesion_setup(adh)
+    nrec = int(lx / 0.5 + 1e-3) * int(lz / 0.5 + 1e-3)
+    sim.run(0, a.warmup, graph=True)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.run(a.warmup, a.steps, graph=True)
+    sim.synchronize()
+    dt = time.perf_counter() - t0
+    st = sim.adhesion_state(nrec)
+    out = {
+        "workload": "adhesion",
+        "config": {"workload": desc + ", cytoadhesion to the lower wall", "precision": a.precision, "adhesion": adh},
+        "mlups": round(n * a.steps / dt / 1e6, 2), "ms_per_step": round(1e3 * dt / a.steps, 4),
+        "knobs": int(st["is_knob"].sum()), "attached_springs": int((st["attach"] >= 0).sum()),
+        "receptors": nrec,
+    }
+    print(json.dumps(out), flush=True)
+    sim.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64", "f16"])
-    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv", "adjoint", "copy", "vof", "tube"],
+    ap.add_argument("--workload", default="suspension", choices=["suspension", "tgv", "adjoint", "copy", "vof", "tube", "adhesion"],
                     help="copy: a plain 256 MiB device copy, the calibration run of the FETCH_SIZE / "
                          "WRITE_SIZE counters")
     ap.add_argument("--stage-steps", type=int, default=6)
@@ -383,6 +417,9 @@ def main():
         return
     if a.workload == "tube":
         tube_bench(a)
+        return
+    if a.workload == "adhesion":
+        adhesion_bench(a)
         return
     if a.workload == "suspension":
         shape, cells, params, desc, ht = suspension_workload()

@@ -127,6 +127,21 @@ inline void thinc_wlic(Domain *, Lattice *) {
     lbmibm_ref::check(lbmibm_thinc_wlic(lbmibm_ref::context()), "thinc_wlic");
 }
 
+// Cytoadhesion: wall_init + adhesion_init in one call, given the def.h
+// constants and the knob flags of ./input/icosa%d.dat (or NULL to draw
+// them), then Adhesion() of the loop (main.cu:84-86), which lbmibm_step also
+// runs by itself once set up.
+namespace lbmibm_ref {
+inline void attach_adhesion(const lbmibm_adhesion_params &a, const unsigned char *is_knob) {
+    check(lbmibm_adhesion_setup(context(), &a, is_knob), "adhesion_init");
+}
+}  // namespace lbmibm_ref
+
+template <class Process, class Domain, class Cell, class Wall>
+inline void Adhesion(Process *, Domain *, Cell *, Wall *, Wall *) {
+    lbmibm_ref::check(lbmibm_adhesion(lbmibm_ref::context()), "Adhesion");
+}
+
 // The binary files of output_data into ./result with the reference's names
 // (prof%04d.bin every 50th file, cell/cell%04d.bin, restart.bin or
 // restart%04d.bin every 100th); increments prc->fnum.  The text files

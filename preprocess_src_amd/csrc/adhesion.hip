@@ -14,6 +14,7 @@
 //     their next candidate, in rounds (an atomicOr race in the reference).
 // Detachments go first; a spring that detaches does not re-attach in the
 // same call (one draw decides either branch in the reference).
+#include <algorithm>
 #include <climits>
 #include <vector>
 
@@ -105,22 +106,25 @@ __global__ __launch_bounds__(AWG) void k_adh_search(AdhDev a, const double *__re
                                                     double *__restrict__ r_wall) {
 #pragma clang fp contract(off)
     const int v = blockIdx.x * AWG + threadIdx.x;
-    if (v >= a.nv) return;
+    if (v >= a.nv || !knob[v]) return;  // other vertices keep the lists set up empty
     int *ids = id_wall + (size_t)v * a.maxn;
     double *rs = r_wall + (size_t)v * a.maxn;
     for (int k = 0; k < a.maxn; k++) { ids[k] = -1; rs[k] = 1.0e15; }
-    if (!knob[v]) return;
     const double xi = xs[v], yi = xs[a.nv + v], zi = xs[2 * (size_t)a.nv + v];
     if (sqrt(yi * yi) < 0.5 * a.ly - a.limit * a.dx) return;
     const int iz = (int)((zi - (-a.lz / 2.0)) / a.dzw);
     const int span = a.limit * (int)(a.dx / a.dzw);
+    // only columns within LIMIT dx in x can pass the range test; scanning
+    // them in the same increasing order keeps the list (and its ties)
+    const int jx0 = max(0, (int)floor((xi - a.limit * a.dx) / a.dxw) + a.nxw / 2 - 1);
+    const int jx1 = min(a.nxw - 1, (int)ceil((xi + a.limit * a.dx) / a.dxw) + a.nxw / 2 + 1);
     int cnt = 0;
     for (int i = 0; i < 2 * span; i++) {
         int jz = iz + i - (span - 1);
         double zj = (jz - a.nzw / 2) * a.dzw;
         if (jz < 0) { jz += a.nzw; zj += a.lz; }
         else if (jz >= a.nzw) { jz -= a.nzw; zj -= a.lz; }
-        for (int jx = 0; jx < a.nxw; jx++) {
+        for (int jx = jx0; jx <= jx1; jx++) {
             const int id = jz * a.nxw + jx;
             double xj, yj, zr;
             receptor(a, id, xj, yj, zr);
@@ -143,11 +147,12 @@ __global__ __launch_bounds__(AWG) void k_adh_detach(AdhDev a, const double *__re
                                                     const unsigned char *__restrict__ knob, int *__restrict__ attach,
                                                     double *__restrict__ r_ij, int *__restrict__ is_attached,
                                                     int *__restrict__ pos, int *__restrict__ active,
-                                                    const DevStep *__restrict__ st, int init) {
+                                                    const DevStep *__restrict__ st, int init,
+                                                    const int *__restrict__ klist, int nk) {
 #pragma clang fp contract(off)
-    const int sid = blockIdx.x * AWG + threadIdx.x;
-    if (sid >= a.nv * a.ns) return;
-    const int v = sid / a.ns;
+    const int t = blockIdx.x * AWG + threadIdx.x;
+    if (t >= nk * a.ns) return;
+    const int v = klist[t / a.ns], sid = v * a.ns + t % a.ns;
     pos[sid] = 0;
     active[sid] = 0;
     if (!knob[v]) return;
@@ -179,12 +184,15 @@ __global__ __launch_bounds__(AWG) void k_adh_propose(AdhDev a, const int *__rest
                                                      const int *__restrict__ is_attached, int *__restrict__ pos,
                                                      int *__restrict__ active, int *__restrict__ prop,
                                                      int *__restrict__ claim, const DevStep *__restrict__ st,
-                                                     int init) {
-    const int sid = blockIdx.x * AWG + threadIdx.x;
-    if (sid >= a.nv * a.ns) return;
+                                                     int init, int *__restrict__ nprop, int round,
+                                                     const int *__restrict__ klist, int nk) {
+    // after a round without proposals nothing changes any more
+    if (round > 0 && nprop[round - 1] == 0) return;
+    const int t = blockIdx.x * AWG + threadIdx.x;
+    if (t >= nk * a.ns) return;
+    const int v = klist[t / a.ns], sid = v * a.ns + t % a.ns;
     prop[sid] = -1;
     if (!active[sid]) return;
-    const int v = sid / a.ns;
     const double u = init ? 0.0 : adh_uniform(a.seed, st->iter, sid);
     int c = pos[sid];
     for (; c < a.maxn; c++) {
@@ -205,15 +213,20 @@ __global__ __launch_bounds__(AWG) void k_adh_propose(AdhDev a, const int *__rest
     const int id = id_wall[(size_t)v * a.maxn + c];
     prop[sid] = id;
     atomicMin(claim + id, sid);
+    atomicAdd(nprop + round, 1);
 }
 
 __global__ __launch_bounds__(AWG) void k_adh_resolve(AdhDev a, const double *__restrict__ r_wall,
                                                      int *__restrict__ attach, double *__restrict__ r_ij,
                                                      int *__restrict__ is_attached, int *__restrict__ pos,
                                                      int *__restrict__ active, const int *__restrict__ prop,
-                                                     const int *__restrict__ claim, int init) {
-    const int sid = blockIdx.x * AWG + threadIdx.x;
-    if (sid >= a.nv * a.ns) return;
+                                                     const int *__restrict__ claim, int init,
+                                                     const int *__restrict__ nprop, int round,
+                                                     const int *__restrict__ klist, int nk) {
+    if (nprop[round] == 0) return;
+    const int t = blockIdx.x * AWG + threadIdx.x;
+    if (t >= nk * a.ns) return;
+    const int sid = klist[t / a.ns] * a.ns + t % a.ns;
     const int id = prop[sid];
     if (id < 0) return;
     if (claim[id] == sid) {
@@ -230,9 +243,13 @@ __global__ __launch_bounds__(AWG) void k_adh_resolve(AdhDev a, const double *__r
     }
 }
 
-__global__ __launch_bounds__(AWG) void k_adh_clear(int n, const int *__restrict__ prop, int *__restrict__ claim) {
-    const int sid = blockIdx.x * AWG + threadIdx.x;
-    if (sid >= n) return;
+__global__ __launch_bounds__(AWG) void k_adh_clear(int ns, const int *__restrict__ prop, int *__restrict__ claim,
+                                                   const int *__restrict__ nprop, int round,
+                                                   const int *__restrict__ klist, int nk) {
+    if (nprop[round] == 0) return;
+    const int t = blockIdx.x * AWG + threadIdx.x;
+    if (t >= nk * ns) return;
+    const int sid = klist[t / ns] * ns + t % ns;
     const int id = prop[sid];
     if (id >= 0) claim[id] = INT_MAX;
 }
@@ -257,28 +274,33 @@ AdhDev adh_dev(const lbmibm_ctx *ctx) {
     return a;
 }
 
-inline unsigned ablocks(long long n) { return (unsigned)((n + AWG - 1) / AWG); }
+inline unsigned ablocks(long long n) { return n > 0 ? (unsigned)((n + AWG - 1) / AWG) : 1u; }
 
 // Search, detachment and the attachment rounds (init: the nearest free
 // receptors with no draw, AttachmentPointAtWall_GPU, adhesion.cu:278-314).
 lbmibm_status adh_bell(lbmibm_ctx *ctx, int init) {
     const AdhDev a = adh_dev(ctx);
     hipStream_t st = ctx->stream;
-    const long long nsp = (long long)a.nv * a.ns;
+    const long long nsp = (long long)ctx->adh_nk * a.ns;  // springs of the knobs
+    const int *kl = ctx->adh_klist;
+    const int nk = ctx->adh_nk;
     hipLaunchKernelGGL(k_adh_search, dim3(ablocks(a.nv)), dim3(AWG), 0, st, a, (const double *)ctx->xs,
                        (const unsigned char *)ctx->adh_knob, ctx->adh_idw, ctx->adh_rw);
     hipLaunchKernelGGL(k_adh_detach, dim3(ablocks(nsp)), dim3(AWG), 0, st, a, (const double *)ctx->xs,
                        (const unsigned char *)ctx->adh_knob, ctx->adh_attach, ctx->adh_rij, ctx->adh_isatt,
-                       ctx->adh_pos, ctx->adh_active, (const DevStep *)ctx->step, init);
+                       ctx->adh_pos, ctx->adh_active, (const DevStep *)ctx->step, init, kl, nk);
+    hipLaunchKernelGGL(k_afill<int>, dim3(1), dim3(64), 0, st, ctx->adh_nprop, (size_t)(a.maxn + 1), 0);
     for (int r = 0; r <= a.maxn; r++) {  // each round retires or advances every active spring
         hipLaunchKernelGGL(k_adh_propose, dim3(ablocks(nsp)), dim3(AWG), 0, st, a, (const int *)ctx->adh_idw,
                            (const double *)ctx->adh_rw, (const int *)ctx->adh_isatt, ctx->adh_pos, ctx->adh_active,
-                           ctx->adh_prop, ctx->adh_claim, (const DevStep *)ctx->step, init);
+                           ctx->adh_prop, ctx->adh_claim, (const DevStep *)ctx->step, init, ctx->adh_nprop, r, kl,
+                           nk);
         hipLaunchKernelGGL(k_adh_resolve, dim3(ablocks(nsp)), dim3(AWG), 0, st, a, (const double *)ctx->adh_rw,
                            ctx->adh_attach, ctx->adh_rij, ctx->adh_isatt, ctx->adh_pos, ctx->adh_active,
-                           (const int *)ctx->adh_prop, (const int *)ctx->adh_claim, init);
-        hipLaunchKernelGGL(k_adh_clear, dim3(ablocks(nsp)), dim3(AWG), 0, st, (int)nsp, (const int *)ctx->adh_prop,
-                           ctx->adh_claim);
+                           (const int *)ctx->adh_prop, (const int *)ctx->adh_claim, init,
+                           (const int *)ctx->adh_nprop, r, kl, nk);
+        hipLaunchKernelGGL(k_adh_clear, dim3(ablocks(nsp)), dim3(AWG), 0, st, a.ns, (const int *)ctx->adh_prop,
+                           ctx->adh_claim, (const int *)ctx->adh_nprop, r, kl, nk);
     }
     LBMIBM_LAUNCHED(ctx, "k_adh_*");
     return LBMIBM_OK;
@@ -330,6 +352,10 @@ lbmibm_status adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *h, c
                 knob[v] = adh_uniform(h->seed, -1, v) < h->knob / (double)(voff[c + 1] - voff[c]) ? 1 : 0;
             }
         }
+    std::vector<int> klist;
+    for (long long v = 0; v < nv; v++)
+        if (knob[v]) klist.push_back((int)v);
+    ctx->adh_nk = (int)klist.size();
     lbmibm_status s = LBMIBM_OK;
     auto al = [&](void **p, size_t bytes) {
         if (s == LBMIBM_OK && !*p) s = dev_alloc(ctx, p, bytes);
@@ -345,15 +371,25 @@ lbmibm_status adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *h, c
     al((void **)&ctx->adh_isatt, nrec * sizeof(int));
     al((void **)&ctx->adh_claim, nrec * sizeof(int));
     al((void **)&ctx->adh_fadh, 3 * nv * sizeof(double));
+    al((void **)&ctx->adh_nprop, (size_t)(h->max_neighwall + 1) * sizeof(int));
+    if (ctx->adh_klist) {  // a second set-up may have other knobs
+        (void)hipFree(ctx->adh_klist);
+        ctx->adh_klist = nullptr;
+    }
+    al((void **)&ctx->adh_klist, std::max<size_t>(klist.size(), 1) * sizeof(int));
     al(&ctx->adh_fsum, 3 * nv * ctx->szT);
     if (s != LBMIBM_OK) return s;
     LBMIBM_HIP(ctx, hipMemcpy(ctx->adh_knob, knob.data(), nv, hipMemcpyHostToDevice));
+    if (!klist.empty())
+        LBMIBM_HIP(ctx, hipMemcpy(ctx->adh_klist, klist.data(), klist.size() * sizeof(int), hipMemcpyHostToDevice));
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nsp)), dim3(AWG), 0, st, ctx->adh_attach, (size_t)nsp, -1);
     hipLaunchKernelGGL(k_afill<double>, dim3(ablocks(nsp)), dim3(AWG), 0, st, ctx->adh_rij, (size_t)nsp, -1.0e10);
     hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nrec)), dim3(AWG), 0, st, ctx->adh_isatt, (size_t)nrec, 0);
     hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nrec)), dim3(AWG), 0, st, ctx->adh_claim, (size_t)nrec, INT_MAX);
     hipLaunchKernelGGL(k_afill<double>, dim3(ablocks(3 * nv)), dim3(AWG), 0, st, ctx->adh_fadh, (size_t)(3 * nv), 0.0);
+    hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nn)), dim3(AWG), 0, st, ctx->adh_idw, (size_t)nn, -1);
+    hipLaunchKernelGGL(k_afill<double>, dim3(ablocks(nn)), dim3(AWG), 0, st, ctx->adh_rw, (size_t)nn, 1.0e15);
     LBMIBM_LAUNCHED(ctx, "k_afill");
     ctx->adh_on = true;
     s = adh_bell(ctx, 1);

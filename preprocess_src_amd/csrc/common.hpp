@@ -175,7 +175,8 @@ struct lbmibm_ctx {
     double adh_dxw = 0.0, adh_dzw = 0.0;
     void *adh_knob = nullptr;
     int *adh_idw = nullptr, *adh_attach = nullptr, *adh_pos = nullptr, *adh_active = nullptr, *adh_prop = nullptr;
-    int *adh_isatt = nullptr, *adh_claim = nullptr;
+    int *adh_isatt = nullptr, *adh_claim = nullptr, *adh_nprop = nullptr;  // nprop: proposals per round
+    int *adh_klist = nullptr, adh_nk = 0;  // the knob vertices
     double *adh_rw = nullptr, *adh_rij = nullptr, *adh_fadh = nullptr;
     void *adh_fsum = nullptr;
     lbmibm::DevStep *step = nullptr;
