@@ -218,3 +218,26 @@ def test_adhesion_arguments_are_checked():
     with pytest.raises(N.LbmibmError):
         h.adjoint_reset()
         h.adj_lbm_d3q19(1)
+
+
+@pytest.mark.gpu
+def test_adhesion_in_the_step_graph_matches_eager_steps():
+    """Whole steps replayed from the captured graph draw the same numbers
+    (the iteration comes from device memory) and claim the same receptors
+    as eager steps."""
+    print("[adhesion graph]", flush=True)
+    kw = dict(tau=0.9, p=5e-6, b=5e-3)
+    cells = _cell()
+    kn = _knobs(cells)
+    runs = []
+    for graph in (False, True):
+        g = gpu_of(LX, LY, LZ, "f64", cells=cells, **kw)
+        g.volume_init()
+        g.adhesion_setup(ADH, kn)
+        g.run(0, 24, graph=graph)
+        runs.append((g.adhesion_state(int(LX / 0.5) * int(LZ / 0.5)), g.cells_state()))
+    (a, ca), (b, cb) = runs
+    assert np.array_equal(a["attach"], b["attach"])
+    assert np.array_equal(a["f_adh"], b["f_adh"])
+    for k in ("x", "y", "z"):
+        assert np.array_equal(ca[k], cb[k]), k
