@@ -41,7 +41,17 @@ struct FtGeom {
     int nx, ny, nz;  // padded sizes
     long long np;
     double dx, lx, lz;  // spacing, physical LX, LZ
+    int nxh;            // (nx + 1) / 2: nodes of one colour per row
+    long long half;     // nxh * ny * nz: length of one colour's array
 };
+
+// The SOR iterate and right-hand side are stored by colour: node (ix, iy,
+// iz) of colour c = (ix + iy + iz) & 1 sits at c * half + (ix >> 1) +
+// nxh * (iy + ny * iz), so a colour's sweep reads and writes contiguous
+// stretches and its neighbours (all of the other colour) likewise.
+__device__ __forceinline__ long long packed(const FtGeom &g, int ix, int iy, int iz) {
+    return (long long)((ix + iy + iz) & 1) * g.half + (ix >> 1) + (long long)g.nxh * (iy + (long long)g.ny * iz);
+}
 
 struct SorCtl {
     int done;  // 0 running, 1 converged, 2 ITRMAX reached
@@ -163,7 +173,7 @@ __global__ __launch_bounds__(VWG) void k_ft_rhs(FtGeom g, const unsigned long lo
                               (2.0 * g.dx);
             bi = dg / (-6.0 / (g.dx * g.dx));
         }
-        b[i] = bi;
+        b[packed(g, ix, iy, iz)] = bi;
     }
     const double s = block_sum(bi * bi, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -209,27 +219,34 @@ __global__ __launch_bounds__(VWG) void k_sor_rows(FtGeom g, double *__restrict__
         const int iy = (int)(row - izu * nyu), iz = (int)izu;
         const int ix = 2 * (int)(t - row * nxh) + ((C + iy + iz) & 1);
         if (ix < g.nx) {
-            const unsigned i = (unsigned)ix + (unsigned)g.nx * row;
+            // own colour C at k + nxh*row; the neighbours, of colour 1-C, at
+            // their own x >> 1 in rows row, row -+ 1 (y), row -+ ny (z)
+            const unsigned k = t - row * nxh;
+            double *xc = x + (size_t)C * g.half;
+            const double *xo = x + (size_t)(1 - C) * g.half;
+            const unsigned i = k + nxh * row;
             const bool wall = iy < 2 || iy > g.ny - 3;
             const bool comp = !wall && ix >= 2 && ix <= g.nx - 3 && iz >= 2 && iz <= g.nz - 3;
             if (comp || wall) {
                 double sum, bi = 0.0;
                 if (comp) {
-                    const unsigned nx = g.nx, nxy = (unsigned)g.nx * g.ny;
+                    const unsigned rowz = nxh * (unsigned)g.ny;
+                    const unsigned xm = ((unsigned)ix - 1) >> 1, xp = ((unsigned)ix + 1) >> 1;
+                    const unsigned r0 = nxh * row;
                     sum = 0.0;
-                    sum += a * x[i - nxy];
-                    sum += a * x[i - nx];
-                    sum += a * x[i - 1];
-                    sum += 1.0 * x[i];
-                    sum += a * x[i + 1];
-                    sum += a * x[i + nx];
-                    sum += a * x[i + nxy];
-                    bi = b[i];
+                    sum += a * xo[k + r0 - rowz];
+                    sum += a * xo[k + r0 - nxh];
+                    sum += a * xo[xm + r0];
+                    sum += 1.0 * xc[i];
+                    sum += a * xo[xp + r0];
+                    sum += a * xo[k + r0 + nxh];
+                    sum += a * xo[k + r0 + rowz];
+                    bi = b[(size_t)C * g.half + i];
                 } else {
-                    sum = 1.0 * x[i];
+                    sum = 1.0 * xc[i];
                 }
                 const double r = bi - sum;
-                x[i] = x[i] + 1.3 * r;
+                xc[i] = xc[i] + 1.3 * r;
                 r2 += r * r;
             }
         }
@@ -270,8 +287,7 @@ __global__ __launch_bounds__(VWG) void k_sor_buff(FtGeom g, double *__restrict__
         if (((ix + iy + iz) & 1) == C) {
             const int jx = ix <= 1 ? ix + (g.nx - 4) : (ix >= g.nx - 2 ? ix - (g.nx - 4) : ix);
             const int jz = iz <= 1 ? iz + (g.nz - 4) : (iz >= g.nz - 2 ? iz - (g.nz - 4) : iz);
-            const long long i = ix + (long long)g.nx * (iy + (long long)g.ny * iz);
-            const long long j = jx + (long long)g.nx * (iy + (long long)g.ny * jz);
+            const long long i = packed(g, ix, iy, iz), j = packed(g, jx, iy, jz);
             const double r = 0.0 - (x[i] - x[j]);
             x[i] = x[i] + 1.3 * r;
             r2 = r * r;
@@ -313,7 +329,7 @@ __global__ __launch_bounds__(VWG) void k_ft_extract(FtGeom g, const double *__re
     const long long j = (long long)blockIdx.x * VWG + threadIdx.x;
     if (j >= N) return;
     const int ix = (int)(j % nx), iy = (int)((j / nx) % ny), iz = (int)(j / ((long long)nx * ny));
-    vf[j] = x[(ix + 2) + (long long)g.nx * ((iy + 2) + (long long)g.ny * (iz + 2))];
+    vf[j] = x[packed(g, ix + 2, iy + 2, iz + 2)];
 }
 
 // ---- THINC/WLIC ---------------------------------------------------------
@@ -432,6 +448,8 @@ FtGeom ft_geom(const lbmibm_ctx *ctx) {
     g.dx = ctx->p.dx;
     g.lx = ctx->p.lx * ctx->p.dx;
     g.lz = ctx->p.lz * ctx->p.dx;
+    g.nxh = (g.nx + 1) / 2;
+    g.half = (long long)g.nxh * g.ny * g.nz;
     return g;
 }
 
@@ -466,8 +484,8 @@ lbmibm_status vof_alloc(lbmibm_ctx *ctx) {
     lbmibm_status s = dev_alloc(ctx, (void **)&ctx->vof[0], ctx->N * sizeof(double));
     if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->vof[1], ctx->N * sizeof(double));
     if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->ft_g, 3 * g.np * sizeof(unsigned long long));
-    if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->ft_x, g.np * sizeof(double));
-    if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->ft_b, g.np * sizeof(double));
+    if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->ft_x, 2 * g.half * sizeof(double));
+    if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->ft_b, 2 * g.half * sizeof(double));
     if (s == LBMIBM_OK) s = dev_alloc(ctx, (void **)&ctx->ft_part, nparts * sizeof(double));
     if (s == LBMIBM_OK) s = dev_alloc(ctx, &ctx->ft_ctl, sizeof(SorCtl));
     if (s != LBMIBM_OK) return s;
@@ -520,8 +538,8 @@ lbmibm_status vof_front_tracking(lbmibm_ctx *ctx, int *iterations, int *attempts
     const double bb = h.bb;
     int cnt = 0;
     for (;;) {
-        hipLaunchKernelGGL(k_vfill<double>, dim3(blocks_of(g.np)), dim3(VWG), 0, st, ctx->ft_x, (size_t)g.np,
-                           (double)cnt);
+        hipLaunchKernelGGL(k_vfill<double>, dim3(blocks_of(2 * g.half)), dim3(VWG), 0, st, ctx->ft_x,
+                           (size_t)(2 * g.half), (double)cnt);
         LBMIBM_LAUNCHED(ctx, "k_vfill");
         h = SorCtl{};
         h.xpad = (double)cnt;
