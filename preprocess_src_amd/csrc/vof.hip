@@ -394,22 +394,25 @@ __global__ __launch_bounds__(VWG) void k_thinc_flux(VofDims d, const double *__r
             for (int b2 = 0; b2 < 3; b2++)
 #pragma unroll
                 for (int a2 = 0; a2 < 3; a2++) v[c2][b2][a2] = V(px + a2 - 1, py + b2 - 1, pz + c2 - 1);
-        const double q = 0.25 / d.dx;
-        // cube differences at the corner node (1+a, 1+b, 1+c) of v
-        auto n2 = [&](int dir, int a, int b, int cc) -> double {
-            const int X = 1 + a, Y = 1 + b, Z = 1 + cc;
-            if (dir == 0)
-                return q * (v[Z][Y][X] - v[Z][Y][X - 1] + v[Z][Y - 1][X] - v[Z][Y - 1][X - 1] + v[Z - 1][Y][X] -
-                            v[Z - 1][Y][X - 1] + v[Z - 1][Y - 1][X] - v[Z - 1][Y - 1][X - 1]);
-            if (dir == 1)
-                return q * (v[Z][Y][X] - v[Z][Y - 1][X] + v[Z][Y][X - 1] - v[Z][Y - 1][X - 1] + v[Z - 1][Y][X] -
-                            v[Z - 1][Y - 1][X] + v[Z - 1][Y][X - 1] - v[Z - 1][Y - 1][X - 1]);
-            return q * (v[Z][Y][X] - v[Z - 1][Y][X] + v[Z][Y][X - 1] - v[Z - 1][Y][X - 1] + v[Z][Y - 1][X] -
-                        v[Z - 1][Y - 1][X] + v[Z][Y - 1][X - 1] - v[Z - 1][Y - 1][X - 1]);
-        };
+        // normal3d_st1/st2 (thincwlic.cu:12-106): the 2x2x2-cube differences
+        // averaged over the 8 cubes around the node are the central
+        // difference along the axis weighted (1,2,1) x (1,2,1) across it,
+        // over 32 dx; summed in that form (9 differences per axis instead of
+        // 64 terms; the same value up to rounding)
         auto n1 = [&](int dir) -> double {
-            return 0.125 * (n2(dir, 0, 0, 0) + n2(dir, 0, 0, 1) + n2(dir, 1, 0, 0) + n2(dir, 1, 0, 1) +
-                            n2(dir, 0, 1, 0) + n2(dir, 0, 1, 1) + n2(dir, 1, 1, 0) + n2(dir, 1, 1, 1));
+            double acc = 0.0;
+#pragma unroll
+            for (int b2 = 0; b2 < 3; b2++)
+#pragma unroll
+                for (int c2 = 0; c2 < 3; c2++) {
+                    const double w = (double)((b2 == 1 ? 2 : 1) * (c2 == 1 ? 2 : 1));
+                    double diff;
+                    if (dir == 0) diff = v[c2][b2][2] - v[c2][b2][0];
+                    else if (dir == 1) diff = v[c2][2][b2] - v[c2][0][b2];
+                    else diff = v[2][c2][b2] - v[0][c2][b2];
+                    acc += w * diff;
+                }
+            return acc / (32.0 * d.dx);
         };
         const double n1x = n1(0), n1y = n1(1), n1z = n1(2);
         const double nn = fabs(n1x) + fabs(n1y) + fabs(n1z);
