@@ -11,7 +11,8 @@
 //     the reference);
 //   - a receptor wanted by several springs in one call goes to the lowest
 //     spring id (vertex * springs_per_knob + spring), the others trying
-//     their next candidate, in rounds (an atomicOr race in the reference).
+//     their next candidate, in rounds (an atomicOr race in the reference);
+//     the bids are 64-bit keys stamped with the call and round.
 // Detachments go first; a spring that detaches does not re-attach in the
 // same call (one draw decides either branch in the reference).
 #include <algorithm>
@@ -42,6 +43,19 @@ __host__ __device__ __forceinline__ double adh_uniform(unsigned long long seed, 
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z = z ^ (z >> 31);
     return (double)((z >> 11) + 1) * (1.0 / 9007199254740992.0);
+}
+
+// A bid of spring sid in round `round` of the call numbered `call` (a
+// device counter that every call advances): later rounds and calls outrank
+// every older bid, so claims need no clearing, and within a round the
+// lowest spring id is the largest key.
+__device__ __forceinline__ unsigned long long bid_key(unsigned long long call, int round, int maxn, int sid) {
+    const unsigned long long stamp = call * (unsigned long long)(maxn + 2) + (unsigned)round + 1u;
+    return (stamp << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)sid);
+}
+
+__global__ void k_adh_tick(unsigned long long *__restrict__ calls) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) calls[0] = calls[0] + 1ull;
 }
 
 __device__ __forceinline__ void receptor(const AdhDev &a, int id, double &x, double &y, double &z) {
@@ -183,9 +197,10 @@ __global__ __launch_bounds__(AWG) void k_adh_propose(AdhDev a, const int *__rest
                                                      const double *__restrict__ r_wall,
                                                      const int *__restrict__ is_attached, int *__restrict__ pos,
                                                      int *__restrict__ active, int *__restrict__ prop,
-                                                     int *__restrict__ claim, const DevStep *__restrict__ st,
+                                                     unsigned long long *__restrict__ claim, const DevStep *__restrict__ st,
                                                      int init, int *__restrict__ nprop, int round,
-                                                     const int *__restrict__ klist, int nk) {
+                                                     const int *__restrict__ klist, int nk,
+                                                     const unsigned long long *__restrict__ calls) {
     // after a round without proposals nothing changes any more
     if (round > 0 && nprop[round - 1] == 0) return;
     const int t = blockIdx.x * AWG + threadIdx.x;
@@ -212,7 +227,7 @@ __global__ __launch_bounds__(AWG) void k_adh_propose(AdhDev a, const int *__rest
     pos[sid] = c;
     const int id = id_wall[(size_t)v * a.maxn + c];
     prop[sid] = id;
-    atomicMin(claim + id, sid);
+    atomicMax(claim + id, bid_key(*calls, round, a.maxn, sid));
     atomicAdd(nprop + round, 1);
 }
 
@@ -220,16 +235,17 @@ __global__ __launch_bounds__(AWG) void k_adh_resolve(AdhDev a, const double *__r
                                                      int *__restrict__ attach, double *__restrict__ r_ij,
                                                      int *__restrict__ is_attached, int *__restrict__ pos,
                                                      int *__restrict__ active, const int *__restrict__ prop,
-                                                     const int *__restrict__ claim, int init,
+                                                     const unsigned long long *__restrict__ claim, int init,
                                                      const int *__restrict__ nprop, int round,
-                                                     const int *__restrict__ klist, int nk) {
+                                                     const int *__restrict__ klist, int nk,
+                                                     const unsigned long long *__restrict__ calls) {
     if (nprop[round] == 0) return;
     const int t = blockIdx.x * AWG + threadIdx.x;
     if (t >= nk * a.ns) return;
     const int sid = klist[t / a.ns] * a.ns + t % a.ns;
     const int id = prop[sid];
     if (id < 0) return;
-    if (claim[id] == sid) {
+    if (claim[id] == bid_key(*calls, round, a.maxn, sid)) {
         attach[sid] = id;
         is_attached[id] = 1;
         if (!init) {
@@ -241,17 +257,6 @@ __global__ __launch_bounds__(AWG) void k_adh_resolve(AdhDev a, const double *__r
     } else {
         pos[sid] = pos[sid] + 1;
     }
-}
-
-__global__ __launch_bounds__(AWG) void k_adh_clear(int ns, const int *__restrict__ prop, int *__restrict__ claim,
-                                                   const int *__restrict__ nprop, int round,
-                                                   const int *__restrict__ klist, int nk) {
-    if (nprop[round] == 0) return;
-    const int t = blockIdx.x * AWG + threadIdx.x;
-    if (t >= nk * ns) return;
-    const int sid = klist[t / ns] * ns + t % ns;
-    const int id = prop[sid];
-    if (id >= 0) claim[id] = INT_MAX;
 }
 
 template <typename V> __global__ void k_afill(V *__restrict__ p, size_t n, V v) {
@@ -290,17 +295,16 @@ lbmibm_status adh_bell(lbmibm_ctx *ctx, int init) {
                        (const unsigned char *)ctx->adh_knob, ctx->adh_attach, ctx->adh_rij, ctx->adh_isatt,
                        ctx->adh_pos, ctx->adh_active, (const DevStep *)ctx->step, init, kl, nk);
     hipLaunchKernelGGL(k_afill<int>, dim3(1), dim3(64), 0, st, ctx->adh_nprop, (size_t)(a.maxn + 1), 0);
+    hipLaunchKernelGGL(k_adh_tick, dim3(1), dim3(64), 0, st, ctx->adh_calls);
     for (int r = 0; r <= a.maxn; r++) {  // each round retires or advances every active spring
         hipLaunchKernelGGL(k_adh_propose, dim3(ablocks(nsp)), dim3(AWG), 0, st, a, (const int *)ctx->adh_idw,
                            (const double *)ctx->adh_rw, (const int *)ctx->adh_isatt, ctx->adh_pos, ctx->adh_active,
                            ctx->adh_prop, ctx->adh_claim, (const DevStep *)ctx->step, init, ctx->adh_nprop, r, kl,
-                           nk);
+                           nk, (const unsigned long long *)ctx->adh_calls);
         hipLaunchKernelGGL(k_adh_resolve, dim3(ablocks(nsp)), dim3(AWG), 0, st, a, (const double *)ctx->adh_rw,
                            ctx->adh_attach, ctx->adh_rij, ctx->adh_isatt, ctx->adh_pos, ctx->adh_active,
-                           (const int *)ctx->adh_prop, (const int *)ctx->adh_claim, init,
-                           (const int *)ctx->adh_nprop, r, kl, nk);
-        hipLaunchKernelGGL(k_adh_clear, dim3(ablocks(nsp)), dim3(AWG), 0, st, a.ns, (const int *)ctx->adh_prop,
-                           ctx->adh_claim, (const int *)ctx->adh_nprop, r, kl, nk);
+                           (const int *)ctx->adh_prop, (const unsigned long long *)ctx->adh_claim, init,
+                           (const int *)ctx->adh_nprop, r, kl, nk, (const unsigned long long *)ctx->adh_calls);
     }
     LBMIBM_LAUNCHED(ctx, "k_adh_*");
     return LBMIBM_OK;
@@ -369,7 +373,8 @@ lbmibm_status adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *h, c
     al((void **)&ctx->adh_active, nsp * sizeof(int));
     al((void **)&ctx->adh_prop, nsp * sizeof(int));
     al((void **)&ctx->adh_isatt, nrec * sizeof(int));
-    al((void **)&ctx->adh_claim, nrec * sizeof(int));
+    al((void **)&ctx->adh_claim, nrec * sizeof(unsigned long long));
+    al((void **)&ctx->adh_calls, sizeof(unsigned long long));
     al((void **)&ctx->adh_fadh, 3 * nv * sizeof(double));
     al((void **)&ctx->adh_nprop, (size_t)(h->max_neighwall + 1) * sizeof(int));
     if (ctx->adh_klist) {  // a second set-up may have other knobs
@@ -386,7 +391,9 @@ lbmibm_status adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *h, c
     hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nsp)), dim3(AWG), 0, st, ctx->adh_attach, (size_t)nsp, -1);
     hipLaunchKernelGGL(k_afill<double>, dim3(ablocks(nsp)), dim3(AWG), 0, st, ctx->adh_rij, (size_t)nsp, -1.0e10);
     hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nrec)), dim3(AWG), 0, st, ctx->adh_isatt, (size_t)nrec, 0);
-    hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nrec)), dim3(AWG), 0, st, ctx->adh_claim, (size_t)nrec, INT_MAX);
+    hipLaunchKernelGGL(k_afill<unsigned long long>, dim3(ablocks(nrec)), dim3(AWG), 0, st, ctx->adh_claim,
+                       (size_t)nrec, 0ull);
+    hipLaunchKernelGGL(k_afill<unsigned long long>, dim3(1), dim3(64), 0, st, ctx->adh_calls, (size_t)1, 0ull);
     hipLaunchKernelGGL(k_afill<double>, dim3(ablocks(3 * nv)), dim3(AWG), 0, st, ctx->adh_fadh, (size_t)(3 * nv), 0.0);
     hipLaunchKernelGGL(k_afill<int>, dim3(ablocks(nn)), dim3(AWG), 0, st, ctx->adh_idw, (size_t)nn, -1);
     hipLaunchKernelGGL(k_afill<double>, dim3(ablocks(nn)), dim3(AWG), 0, st, ctx->adh_rw, (size_t)nn, 1.0e15);

@@ -268,7 +268,7 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->vof[0], ctx->vof[1], ctx->ft_g, ctx->ft_x, ctx->ft_b, ctx->ft_part, ctx->ft_ctl,
                     ctx->tube_col, ctx->tube_r, ctx->adh_knob, ctx->adh_idw, ctx->adh_attach, ctx->adh_pos,
                     ctx->adh_active, ctx->adh_prop, ctx->adh_isatt, ctx->adh_claim, ctx->adh_rw, ctx->adh_rij,
-                    ctx->adh_fadh, ctx->adh_fsum, ctx->adh_nprop, ctx->adh_klist};
+                    ctx->adh_fadh, ctx->adh_fsum, ctx->adh_nprop, ctx->adh_klist, ctx->adh_calls};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);

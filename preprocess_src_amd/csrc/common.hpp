@@ -175,7 +175,9 @@ struct lbmibm_ctx {
     double adh_dxw = 0.0, adh_dzw = 0.0;
     void *adh_knob = nullptr;
     int *adh_idw = nullptr, *adh_attach = nullptr, *adh_pos = nullptr, *adh_active = nullptr, *adh_prop = nullptr;
-    int *adh_isatt = nullptr, *adh_claim = nullptr, *adh_nprop = nullptr;  // nprop: proposals per round
+    int *adh_isatt = nullptr, *adh_nprop = nullptr;  // nprop: proposals per round
+    unsigned long long *adh_claim = nullptr;          // per receptor: the highest bid key
+    unsigned long long *adh_calls = nullptr;          // [1] calls of the Bell update so far
     int *adh_klist = nullptr, adh_nk = 0;  // the knob vertices
     double *adh_rw = nullptr, *adh_rij = nullptr, *adh_fadh = nullptr;
     void *adh_fsum = nullptr;

@@ -241,3 +241,19 @@ def test_adhesion_in_the_step_graph_matches_eager_steps():
     assert np.array_equal(a["f_adh"], b["f_adh"])
     for k in ("x", "y", "z"):
         assert np.array_equal(ca[k], cb[k]), k
+
+
+@pytest.mark.gpu
+def test_adhesion_with_repeated_iteration_numbers_matches_the_oracle():
+    """Receptor claims do not depend on the iteration numbers being new: a
+    second pass over steps 1-4 in the same context claims as the oracle."""
+    print("[adhesion repeated]", flush=True)
+    o, g, st, nrec = _pair("f64", tau=0.9, p=5e-6, b=5e-3)
+    o.volume_init()
+    g.volume_init()
+    for it in list(range(1, 5)) * 2:
+        assert o.step(it)[0] == 0
+        g.step(it)
+    gs = g.adhesion_state(nrec)
+    assert np.array_equal(gs["attach"], st["attach"])
+    assert np.array_equal(gs["is_attached"], st["is_attached"])
