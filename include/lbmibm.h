@@ -218,7 +218,7 @@ lbmibm_status lbmibm_thinc_wlic(lbmibm_ctx *ctx);
 lbmibm_status lbmibm_download_vf(lbmibm_ctx *ctx, double *vf);
 lbmibm_status lbmibm_upload_vf(lbmibm_ctx *ctx, const double *vf);
 
-/* ---- cytoadhesion (adhesion.cu; slab, linear spring law) ----------------
+/* ---- cytoadhesion (adhesion.cu; slab, linear spring or WLC law) --------
  * Knob vertices bind receptors on the wall y = -LY/2 through springs_per_knob
  * springs each, by the Bell model; the spring force joins the membrane force
  * in the spreading (ibm_st1bb, ibm.cu:133-212).  Once set up, lbmibm_step
@@ -241,6 +241,9 @@ typedef struct lbmibm_adhesion_params {
     int    springs_per_knob;  /* NUMSPRING_KNOB                                            */
     double knob;              /* KNOB: expected knobs per cell when is_knob is NULL        */
     unsigned long long seed;
+    int    wlc;               /* 0: LINEAR_SPRING; 1: WLC (adhesion.cu:564-566, 677-690)   */
+    double lcont, f_wlc, ra;  /* WLC: contour length Lcont (in dx), fWLC, RA               */
+    double alpha1, alpha2;    /* WLC: wall repulsion ALPHA1, ALPHA2                        */
 } lbmibm_adhesion_params;
 /* adhesion_init + wall_init (adhesion.cu:316-466): receptor grid, knobs
  * (is_knob [Nv], the reference's ./input/icosa%d.dat flags, or NULL to draw

@@ -1,6 +1,6 @@
 // FP64 host ORACLE, cytoadhesion part (test infrastructure only; see
 // oracle.h).  A plain restatement of adhesion.cu for the slab
-// (CapsuleSHEARFLOW) with the linear spring law, in the reference's
+// (CapsuleSHEARFLOW) with the linear spring or WLC law, in the reference's
 // per-vertex arrays.  The two choices the reference leaves to the hardware
 // are fixed here and in the library alike: the random numbers come from a
 // counter-based generator, and a receptor wanted by several springs in one
@@ -71,10 +71,24 @@ void search(const orc_params *p, const orc_state *s, orc_adhesion *a, int v) {
     }
 }
 
+// The extension measure of the Bell rates: LINEAR_SPRING (adhesion.cu:562-
+// 563) or WLC with contour length LL = Lcont dx (adhesion.cu:564-566).
+double bell_extension(const orc_params *p, const orc_adhesion *a, double r) {
+    if (a->wlc) {
+        const double LL = a->lcont * p->dx;
+        return std::fmax(0.0, 0.25 / ((1.0 - r / LL) * (1.0 - r / LL)) + r / LL - 0.25);
+    }
+    return std::fmax(0.0, (r - a->l0 * p->dx) * a->rc);
+}
+
+double bell_force(const orc_adhesion *a, double dr) {
+    return a->wlc ? std::fabs(a->f_wlc * dr) : std::fabs(a->k_ad * dr);
+}
+
 double on_probability(const orc_params *p, const orc_adhesion *a, double r, double *dr_out) {
-    // adhesion.cu:562-573 (LINEAR_SPRING)
-    const double dr = std::fmax(0.0, (r - a->l0 * p->dx) * a->rc);
-    const double f_ad = std::fabs(a->k_ad * dr);
+    // adhesion.cu:562-573
+    const double dr = bell_extension(p, a, r);
+    const double f_ad = bell_force(a, dr);
     const double k_on = a->k_on0 / a->shear_rate * std::exp(f_ad * (a->chi_beta - 0.5 * dr) / a->kbt);
     *dr_out = dr;
     return 1.0 - std::exp(-k_on * p->dt * (double)p->m);
@@ -174,11 +188,14 @@ void orc_adhesion_init(const orc_params *p, orc_state *s, orc_adhesion *a) {
 void orc_adhesion_step(const orc_params *p, orc_state *s, orc_adhesion *a, int iter) {
     const int nv = s->voff[s->nc], NS = a->springs_per_knob;
     const double dx = p->dx;
-    // AdhesiveSpringForce_GPU (adhesion.cu:617-706), LINEAR_SPRING
+    // AdhesiveSpringForce_GPU (adhesion.cu:617-706).  With WLC the
+    // repulsive term is set only where |dy| < L0 dx and otherwise keeps the
+    // value of the knob's previous spring, as the reference's variable does.
     for (int v = 0; v < nv; v++) {
         double f[3] = {0.0, 0.0, 0.0};
         if (a->is_knob[v]) {
             const double xi = s->x[v], yi = s->y[v], zi = s->z[v];
+            double repulf = 0.0;
             for (int k = 0; k < NS; k++) {
                 const int id = a->attach[v * NS + k];
                 if (id == -1) continue;
@@ -187,6 +204,18 @@ void orc_adhesion_step(const orc_params *p, orc_state *s, orc_adhesion *a, int i
                 const double ddy = rc.y - yi;
                 const double ddz = min_image_signed(rc.z - zi, p->lz, zi, rc.z);
                 const double r = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                if (a->wlc) {
+                    const double LL = a->lcont * dx;
+                    const double dr = 0.25 / ((1.0 - r / LL) * (1.0 - r / LL)) + r / LL - 0.25;
+                    if (std::fabs(ddy) < a->l0 * dx) {
+                        const double ex = std::exp(-a->alpha2 * std::fabs(ddy));
+                        repulf = a->alpha1 / (a->gs * a->ra) * a->alpha2 * ex / (1.0 - ex) * ddy / std::fabs(ddy);
+                    }
+                    f[0] += a->f_wlc / (a->gs * a->ra) * dr * ddx / r;
+                    f[1] += a->f_wlc / (a->gs * a->ra) * dr * ddy / r + repulf;
+                    f[2] += a->f_wlc / (a->gs * a->ra) * dr * ddz / r;
+                    continue;
+                }
                 const double dr = r - a->l0 * dx;
                 f[0] += a->k_ad / a->gs * dr * ddx / r;
                 f[1] += a->k_ad / a->gs * dr * ddy / r;
@@ -208,8 +237,8 @@ void orc_adhesion_step(const orc_params *p, orc_state *s, orc_adhesion *a, int i
             const double ddx = rc.x - s->x[v], ddy = rc.y - s->y[v];
             const double ddz = min_image_signed(rc.z - s->z[v], p->lz, s->z[v], rc.z);
             const double r = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-            const double dr = std::fmax(0.0, (r - a->l0 * dx) * a->rc);
-            const double f_ad = std::fabs(a->k_ad * dr);
+            const double dr = bell_extension(p, a, r);
+            const double f_ad = bell_force(a, dr);
             const double k_off = a->k_off0 / a->shear_rate * std::exp(f_ad * a->chi_beta / a->kbt);
             const double p_off = 1.0 - std::exp(-k_off * p->dt * (double)p->m);
             if (p_off <= orc_adh_uniform(a->seed, iter, sid)) continue;

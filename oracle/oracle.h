@@ -101,7 +101,7 @@ typedef struct orc_state {
 } orc_state;
 
 /* Cytoadhesion of the slab (CapsuleSHEARFLOW) with the linear spring law
- * (adhesion.cu, LINEAR_SPRING).  Receptors on the wall y = -LY/2 on an
+ * (adhesion.cu, LINEAR_SPRING) or the worm-like chain (WLC).  Receptors on the wall y = -LY/2 on an
  * nxw x nzw grid (wall_init, adhesion.cu:435-466).  Per vertex: knob flag,
  * MAXN nearest receptors, springs_per_knob springs.  Randomness: the
  * counter-based orc_adh_uniform(seed, iter, spring) in (0, 1], one draw per
@@ -113,6 +113,8 @@ typedef struct orc_adhesion {
     double receptor_density, rc, k_ad, l0, k_on0, k_off0, chi_beta, kbt, shear_rate, gs;
     int    limit, max_neighwall, springs_per_knob;
     unsigned long long seed;
+    int    wlc;                 /* 0: LINEAR_SPRING, 1: WLC (adhesion.cu:564-567, 677-690) */
+    double lcont, f_wlc, ra, alpha1, alpha2;  /* WLC: Lcont, fWLC, RA, ALPHA1, ALPHA2   */
     int    nxw, nzw;            /* receptor grid (set by orc_adhesion_init)  */
     double dxw, dzw;
     unsigned char *is_knob;     /* [Nv]            */

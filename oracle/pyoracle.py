@@ -64,6 +64,8 @@ class OrcAdhesion(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("receptor_density", "rc", "k_ad", "l0", "k_on0", "k_off0", "chi_beta",
                                           "kbt", "shear_rate", "gs")] + [
         ("limit", C.c_int), ("max_neighwall", C.c_int), ("springs_per_knob", C.c_int), ("seed", C.c_ulonglong),
+        ("wlc", C.c_int), ("lcont", C.c_double), ("f_wlc", C.c_double), ("ra", C.c_double), ("alpha1", C.c_double),
+        ("alpha2", C.c_double),
         ("nxw", C.c_int), ("nzw", C.c_int), ("dxw", C.c_double), ("dzw", C.c_double),
         ("is_knob", C.POINTER(C.c_ubyte)), ("id_wall", c_int_p), ("r_wall", c_double_p), ("attach", c_int_p),
         ("r_ij", c_double_p), ("is_attached", c_int_p), ("f_adh", c_double_p),

@@ -1,5 +1,5 @@
 // Cytoadhesion of cell knobs to wall receptors (adhesion.cu), for the slab
-// (CapsuleSHEARFLOW) with the linear spring law: receptors on the wall
+// (CapsuleSHEARFLOW) with the linear spring or WLC law: receptors on the wall
 // y = -LY/2 on an nxw x nzw grid (wall_init, adhesion.cu:441-458), knobs
 // with springs_per_knob springs, Bell-model attach / detach
 // (adhesion.cu:513-615) and the spring force that the spreading adds to the
@@ -32,7 +32,23 @@ struct AdhDev {
     double dxw, dzw, lx, ly, lz, dx, dtm;  // dtm = dt M
     unsigned long long seed;
     int nv;
+    int wlc;                                      // 0 LINEAR_SPRING, 1 WLC
+    double lcont, f_wlc, ra, alpha1, alpha2;
 };
+
+// Extension measure and force of the Bell rates: LINEAR_SPRING
+// (adhesion.cu:562-563) or WLC with LL = Lcont dx (adhesion.cu:564-566).
+__device__ __forceinline__ double bell_extension(const AdhDev &a, double r) {
+#pragma clang fp contract(off)
+    if (a.wlc) {
+        const double LL = a.lcont * a.dx;
+        return fmax(0.0, 0.25 / ((1.0 - r / LL) * (1.0 - r / LL)) + r / LL - 0.25);
+    }
+    return fmax(0.0, (r - a.l0 * a.dx) * a.rc);
+}
+__device__ __forceinline__ double bell_force(const AdhDev &a, double dr) {
+    return a.wlc ? fabs(a.f_wlc * dr) : fabs(a.k_ad * dr);
+}
 
 __host__ __device__ __forceinline__ double adh_uniform(unsigned long long seed, long long iter, long long sid) {
     // splitmix64 (Steele, Lea, Flood 2014) of the key, mapped to (0, 1]
@@ -75,8 +91,8 @@ __device__ __forceinline__ double image_shift(double d, double L, double xi, dou
 
 __device__ __forceinline__ double p_on(const AdhDev &a, double r, double &dr) {
 #pragma clang fp contract(off)
-    dr = fmax(0.0, (r - a.l0 * a.dx) * a.rc);  // adhesion.cu:562-573
-    const double f_ad = fabs(a.k_ad * dr);
+    dr = bell_extension(a, r);  // adhesion.cu:562-573
+    const double f_ad = bell_force(a, dr);
     const double k_on = a.k_on0 / a.shear_rate * exp(f_ad * (a.chi_beta - 0.5 * dr) / a.kbt);
     return 1.0 - exp(-k_on * a.dtm);
 }
@@ -94,6 +110,9 @@ __global__ __launch_bounds__(AWG) void k_adh_force(AdhDev a, const double *__res
     double f[3] = {0.0, 0.0, 0.0};
     if (knob[v]) {
         const double xi = xs[v], yi = xs[a.nv + v], zi = xs[2 * (size_t)a.nv + v];
+        // WLC: the repulsive term keeps the previous spring's value where
+        // |dy| >= L0 dx, as the reference's variable does (adhesion.cu:677-690)
+        double repulf = 0.0;
         for (int s = 0; s < a.ns; s++) {
             const int id = attach[v * a.ns + s];
             if (id == -1) continue;
@@ -101,6 +120,18 @@ __global__ __launch_bounds__(AWG) void k_adh_force(AdhDev a, const double *__res
             receptor(a, id, xj, yj, zj);
             const double dx = image_shift(xj - xi, a.lx, xi, xj), dy = yj - yi, dz = image_shift(zj - zi, a.lz, zi, zj);
             const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (a.wlc) {
+                const double LL = a.lcont * a.dx;
+                const double dr = 0.25 / ((1.0 - r / LL) * (1.0 - r / LL)) + r / LL - 0.25;
+                if (fabs(dy) < a.l0 * a.dx) {
+                    const double ex = exp(-a.alpha2 * fabs(dy));
+                    repulf = a.alpha1 / (a.gs * a.ra) * a.alpha2 * ex / (1.0 - ex) * dy / fabs(dy);
+                }
+                f[0] += a.f_wlc / (a.gs * a.ra) * dr * dx / r;
+                f[1] += a.f_wlc / (a.gs * a.ra) * dr * dy / r + repulf;
+                f[2] += a.f_wlc / (a.gs * a.ra) * dr * dz / r;
+                continue;
+            }
             const double dr = r - a.l0 * a.dx;
             f[0] += a.k_ad / a.gs * dr * dx / r;
             f[1] += a.k_ad / a.gs * dr * dy / r;
@@ -181,8 +212,8 @@ __global__ __launch_bounds__(AWG) void k_adh_detach(AdhDev a, const double *__re
     const double xi = xs[v], yi = xs[a.nv + v], zi = xs[2 * (size_t)a.nv + v];
     const double dx = xj - xi, dy = yj - yi, dz = image_shift(zj - zi, a.lz, zi, zj);
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    const double dr = fmax(0.0, (r - a.l0 * a.dx) * a.rc);
-    const double f_ad = fabs(a.k_ad * dr);
+    const double dr = bell_extension(a, r);
+    const double f_ad = bell_force(a, dr);
     const double k_off = a.k_off0 / a.shear_rate * exp(f_ad * a.chi_beta / a.kbt);
     const double p_off = 1.0 - exp(-k_off * a.dtm);
     if (p_off <= adh_uniform(a.seed, st->iter, sid)) return;
@@ -276,6 +307,8 @@ AdhDev adh_dev(const lbmibm_ctx *ctx) {
     a.dtm = ctx->p.dt * (double)ctx->p.m;
     a.seed = h.seed;
     a.nv = ctx->nv;
+    a.wlc = h.wlc;
+    a.lcont = h.lcont; a.f_wlc = h.f_wlc; a.ra = h.ra; a.alpha1 = h.alpha1; a.alpha2 = h.alpha2;
     return a;
 }
 

@@ -55,7 +55,8 @@ class AdhesionParams(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("receptor_density", "rc", "k_ad", "l0", "k_on0", "k_off0", "chi_beta",
                                           "kbt", "shear_rate", "gs")] + [
         ("limit", C.c_int), ("max_neighwall", C.c_int), ("springs_per_knob", C.c_int), ("knob", C.c_double),
-        ("seed", C.c_ulonglong),
+        ("seed", C.c_ulonglong), ("wlc", C.c_int), ("lcont", C.c_double), ("f_wlc", C.c_double), ("ra", C.c_double),
+        ("alpha1", C.c_double), ("alpha2", C.c_double),
     ]
 
 

@@ -119,6 +119,34 @@ def test_spring_force_is_the_linear_spring():
     assert np.allclose(st["f_adh"].reshape(-1, 3), f, rtol=1e-13, atol=1e-18)
 
 
+WLC = dict(wlc=1, lcont=3.0, f_wlc=1e-3, ra=2.0, alpha1=1e-3, alpha2=2.0)
+
+
+def test_wlc_force_is_the_marko_siggia_interpolation():
+    # one spring per knob: f = fWLC/(GS RA) (1/(4(1-x)^2) - 1/4 + x) e with
+    # x = r/(Lcont dx) (Marko and Siggia 1995), plus the wall repulsion
+    # ALPHA1 ALPHA2 e^(-ALPHA2|dy|)/(1 - e^(-ALPHA2|dy|))/(GS RA) in y where
+    # |dy| < L0 dx
+    o, cells, st = _setup(springs_per_knob=1, **WLC)
+    A = o._adhs
+    before = st["attach"].copy()
+    o.adhesion_step(1)
+    rec = _receptors(o, A)
+    pts = np.stack([cells["x"], cells["y"], cells["z"]], 1)
+    f = np.zeros_like(pts)
+    c = WLC["f_wlc"] / (ADH["gs"] * WLC["ra"])
+    for v in np.flatnonzero(before >= 0):
+        d = rec[before[v]] - pts[v]
+        r = np.linalg.norm(d)
+        x = r / WLC["lcont"]
+        f[v] = c * (1 / (4 * (1 - x) ** 2) - 0.25 + x) * d / r
+        if abs(d[1]) < ADH["l0"]:
+            e = np.exp(-WLC["alpha2"] * abs(d[1]))
+            f[v, 1] += WLC["alpha1"] / (ADH["gs"] * WLC["ra"]) * WLC["alpha2"] * e / (1 - e) * np.sign(d[1])
+    assert (before >= 0).sum() > 5
+    assert np.allclose(st["f_adh"].reshape(-1, 3), f, rtol=1e-12, atol=1e-18)
+
+
 @pytest.mark.parametrize("which", ["on", "off"])
 def test_bell_rates_with_a_zero_spring_constant(which):
     # f_ad = 0: p_on = 1 - exp(-k_on0/sr dt M), p_off = 1 - exp(-k_off0/sr dt M)
@@ -147,18 +175,19 @@ def test_bell_rates_with_a_zero_spring_constant(which):
 
 # ---- GPU: the library against the oracle ------------------------------------
 
-def _pair(prec, knobs="given", **kw):
+def _pair(prec, knobs="given", extra=None, **kw):
     cells = _cell()
     o = oracle_of(LX, LY, LZ, cells=cells, **kw)
     g = gpu_of(LX, LY, LZ, prec, cells=cells, **kw)
     kn = _knobs(cells)
+    adh = dict(ADH, **(extra or {}))
     if knobs == "drawn":  # the library's draw: u(seed, -1, v) < KNOB / V
         V = len(cells["x"])
         kn = np.array([PO.lib().orc_adh_uniform(ADH["seed"], -1, v) < 40.0 / V for v in range(V)], np.uint8)
-        g.adhesion_setup(dict(ADH, knob=40.0))
+        g.adhesion_setup(dict(adh, knob=40.0))
     else:
-        g.adhesion_setup(ADH, kn)
-    st = o.adhesion_setup(ADH, kn)
+        g.adhesion_setup(adh, kn)
+    st = o.adhesion_setup(adh, kn)
     return o, g, st, o._adhs.nxw * o._adhs.nzw
 
 
@@ -176,11 +205,11 @@ def test_adhesion_setup_matches_the_oracle(knobs):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec,tol", [("f64", 1e-11), ("f32", 2e-5)])
-def test_adhesion_steps_match_the_oracle(prec, tol):
-    print(f"[adhesion steps] prec={prec}", flush=True)
+@pytest.mark.parametrize("prec,tol,law", [("f64", 1e-11, "linear"), ("f32", 2e-5, "linear"), ("f64", 1e-11, "wlc")])
+def test_adhesion_steps_match_the_oracle(prec, tol, law):
+    print(f"[adhesion steps] prec={prec} law={law}", flush=True)
     kw = dict(tau=0.9, p=5e-6, b=5e-3)
-    o, g, st, nrec = _pair(prec, **kw)
+    o, g, st, nrec = _pair(prec, extra=WLC if law == "wlc" else None, **kw)
     X, Y, Z = o.node_coords()
     vel = 4e-3 * (Y + LY / 2) / LY
     vel[o.bc == 2] = 0.0
