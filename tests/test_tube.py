@@ -143,3 +143,21 @@ def test_tube_arguments_are_checked():
     with pytest.raises(N.LbmibmError):
         g.adjoint_reset()
         g.adj_lbm_d3q19(1)
+
+
+@pytest.mark.gpu
+def test_tube_fp16_storage_matches_the_oracle():
+    """FP16 population storage (deviations scaled by 1/w) through the tube's
+    interpolated bounce-back, to the FP16 tolerances of the slab tests."""
+    print("[tube f16]", flush=True)
+    kw = dict(tube_radius=5.5, tau=0.8, p=2e-5, e=(0.0, 0.0, 1.0), force_modulated=2, flowstart=1)
+    o = oracle_of(14, 14, 8, **kw)
+    g = gpu_of(14, 14, 8, "f16", **kw)
+    for it in range(1, 21):
+        o.lbm_d3q19(it)
+        g.lbm_d3q19(it)
+    comp = o.bc == 0
+    assert rel(g.populations().reshape(-1, Q)[comp], o.fn.reshape(-1, Q)[comp]) < 2e-3
+    m = g.macros()
+    assert rel(m["wm"][comp], o.wm[comp]) < 5e-2
+    assert np.abs(o.wm[comp]).max() > 1e-6
