@@ -90,8 +90,8 @@ typedef struct lbmibm_params {
     int    fileoutput;       /* FILEOUTPUT period of the tp / qi accumulators (0: off)       */
     int    vof_period;       /* K: lbmibm_step runs THINC/WLIC every M steps and front
                                 tracking every K steps (main.cu:93-97); 0: no VOF indicator
-                                (vf = 0).  Needs cells and the slab (periodic_y = 0);
-                                LAMBDA1 != 1 needs it                                        */
+                                (vf = 0).  Needs cells and walls: the slab or the tube (not
+                                periodic_y); LAMBDA1 != 1 needs it                           */
     int    sor_itrmax;       /* ITRMAX of the front-tracking red-black SOR                   */
     double sor_residual;     /* its RESIDUAL (relative, matrix_solver.cu:188-215); the
                                 reference shares both constants with BiCGSTAB                */
@@ -99,7 +99,7 @@ typedef struct lbmibm_params {
                                 lbm_cylinder.cu:167) along z: nodes with sqrt(x^2+y^2) > R
                                 are walls, z is periodic, links crossing r = R take the
                                 interpolated bounce-back (lbm_cylinder.cu:61-235); needs
-                                R <= min(LX,LY)/2 - dx, periodic_y = 0, no VOF. 0: off     */
+                                R <= min(LX,LY)/2 - dx and periodic_y = 0. 0: off          */
 } lbmibm_params;
 
 /* Initial cell set: reference SoA positions and element vertex ids. */

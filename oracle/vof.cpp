@@ -1,8 +1,10 @@
 // FP64 host ORACLE, VOF indicator part (test infrastructure only; see
 // oracle.h).  A plain restatement of front tracking (fronttrack.cu), its
 // red-black SOR solve (matrix_solver.cu) and the THINC/WLIC advection
-// (thincwlic.cu) on the reference's padded lattice, for the slab
-// connectivity of CapsuleSHEARFLOW (walls in y, periodic x and z).
+// (thincwlic.cu) on the reference's padded lattice, for the slab of
+// CapsuleSHEARFLOW (walls in y, periodic x and z) and the tube of
+// CapsuleCHANNELFLOW (walls outside the disc, periodic z): everything else
+// follows the bc mask.
 //
 // The one choice the reference leaves open: its red-black SOR updates a BUFF
 // row (x_i - x_image = 0) in the same kernel as the COMP rows of its colour,
@@ -107,7 +109,8 @@ void orc_ft_gradient(const orc_params *p, const orc_state *s, double *gx, double
         int ix, iy, iz;
         g.coords(i, ix, iy, iz);
         int i1 = -1, i2 = -1, i3 = -1;
-        const bool xl = ix <= 3, xh = ix >= nx - 4, zl = iz <= 3, zh = iz >= nz - 4;
+        // the tube (CapsuleCHANNELFLOW, fronttrack.cu:466-482) folds z only
+        const bool xl = !p->tube && ix <= 3, xh = !p->tube && ix >= nx - 4, zl = iz <= 3, zh = iz >= nz - 4;
         const int ox = xl ? sx : -sx, oz = zl ? sz : -sz;
         if ((xl || xh) && (zl || zh)) { i1 = i + ox; i2 = i + oz; i3 = i + ox + oz; }
         else if (xl || xh) i1 = i + ox;

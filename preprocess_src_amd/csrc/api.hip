@@ -342,7 +342,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     if (p->vof_period < 0 || (p->lambda1 != 1.0 && p->vof_period == 0)) return LBMIBM_ERR_ARG;
     // the tube's halo columns must be walls (cdo_init tests only r <= Lc/2)
     if (p->tube_radius < 0.0 || p->force_modulated < 0 || p->force_modulated > 2 ||
-        (p->tube_radius > 0.0 && (p->periodic_y || p->vof_period > 0 ||
+        (p->tube_radius > 0.0 && (p->periodic_y ||
                                   p->tube_radius > 0.5 * std::min(p->lx, p->ly) * p->dx - p->dx)))
         return LBMIBM_ERR_ARG;
     if (p->vof_period > 0 && (p->periodic_y || !cells || cells->nc < 1 || p->sor_itrmax < 1 ||

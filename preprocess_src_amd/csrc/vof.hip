@@ -1,7 +1,8 @@
 // VOF indicator of the cell interiors: front tracking (fronttrack.cu:274-724)
 // with its red-black SOR Poisson solve (matrix_solver.cu:112-216), and the
-// THINC/WLIC advection (thincwlic.cu:12-372).  Slab connectivity (walls in
-// y, periodic x and z), FP64 throughout.
+// THINC/WLIC advection (thincwlic.cu:12-372).  The slab (walls in y,
+// periodic x and z) or the tube (walls outside the disc, periodic z), FP64
+// throughout.
 //
 // Front tracking works on the reference's padded lattice (nx+4)(ny+4)(nz+4)
 // so that its halo fold, its BUFF and WALL rows and its padding rows are the
@@ -43,7 +44,23 @@ struct FtGeom {
     double dx, lx, lz;  // spacing, physical LX, LZ
     int nxh;            // (nx + 1) / 2: nodes of one colour per row
     long long half;     // nxh * ny * nz: length of one colour's array
+    const unsigned *tube_col;  // the tube's column words (bit 0: inside), or null for the slab
 };
+
+enum { FT_COMP = 0, FT_BUFF = 1, FT_WALL = 2 };
+
+// bc class of a padded node: the slab (walls beyond the y range, BUFF in
+// the x/z halo) or the tube (walls outside the disc, BUFF in the z halo),
+// cdo_init (init1.cu:164-178, 217-229)
+__device__ __forceinline__ int ft_kind(const FtGeom &g, int ix, int iy, int iz) {
+    if (g.tube_col) {
+        const int ux = ix - 2, uy = iy - 2, nxu = g.nx - 4, nyu = g.ny - 4;
+        if (ux < 0 || ux >= nxu || uy < 0 || uy >= nyu || !(g.tube_col[ux + nxu * uy] & 1u)) return FT_WALL;
+        return iz >= 2 && iz <= g.nz - 3 ? FT_COMP : FT_BUFF;
+    }
+    if (iy < 2 || iy > g.ny - 3) return FT_WALL;
+    return ix >= 2 && ix <= g.nx - 3 && iz >= 2 && iz <= g.nz - 3 ? FT_COMP : FT_BUFF;
+}
 
 // The SOR iterate and right-hand side are stored by colour: node (ix, iy,
 // iz) of colour c = (ix + iy + iz) & 1 sits at c * half + (ix >> 1) +
@@ -128,7 +145,7 @@ __global__ __launch_bounds__(VWG) void k_ft_spread(const double *__restrict__ xs
 }
 
 __device__ __forceinline__ bool ft_comp(const FtGeom &g, int ix, int iy, int iz) {
-    return ix >= 2 && ix <= g.nx - 3 && iy >= 2 && iy <= g.ny - 3 && iz >= 2 && iz <= g.nz - 3;
+    return ft_kind(g, ix, iy, iz) == FT_COMP;
 }
 
 // front_tracking_source3 (fronttrack.cu:444-499) in exact integer sums.
@@ -137,7 +154,8 @@ __global__ __launch_bounds__(VWG) void k_ft_fold(FtGeom g, unsigned long long *_
     if (i >= g.np) return;
     const int ix = (int)(i % g.nx), iy = (int)((i / g.nx) % g.ny), iz = (int)(i / ((long long)g.nx * g.ny));
     if (!ft_comp(g, ix, iy, iz)) return;
-    const bool xl = ix <= 3, xh = ix >= g.nx - 4, zl = iz <= 3, zh = iz >= g.nz - 4;
+    // the tube folds z only (fronttrack.cu:466-482)
+    const bool xl = !g.tube_col && ix <= 3, xh = !g.tube_col && ix >= g.nx - 4, zl = iz <= 3, zh = iz >= g.nz - 4;
     if (!(xl || xh || zl || zh)) return;
     const long long sx = g.nx - 4, sz = (long long)g.nx * g.ny * (g.nz - 4);
     const long long ox = xl ? sx : -sx, oz = zl ? sz : -sz;
@@ -225,8 +243,8 @@ __global__ __launch_bounds__(VWG) void k_sor_rows(FtGeom g, double *__restrict__
             double *xc = x + (size_t)C * g.half;
             const double *xo = x + (size_t)(1 - C) * g.half;
             const unsigned i = k + nxh * row;
-            const bool wall = iy < 2 || iy > g.ny - 3;
-            const bool comp = !wall && ix >= 2 && ix <= g.nx - 3 && iz >= 2 && iz <= g.nz - 3;
+            const int kind = ft_kind(g, ix, iy, iz);
+            const bool wall = kind == FT_WALL, comp = kind == FT_COMP;
             if (comp || wall) {
                 double sum, bi = 0.0;
                 if (comp) {
@@ -284,7 +302,7 @@ __global__ __launch_bounds__(VWG) void k_sor_buff(FtGeom g, double *__restrict__
             ix = 2 + (int)(rest % nx4);
             iy = 2 + (int)(rest / nx4);
         }
-        if (((ix + iy + iz) & 1) == C) {
+        if (((ix + iy + iz) & 1) == C && ft_kind(g, ix, iy, iz) == FT_BUFF) {
             const int jx = ix <= 1 ? ix + (g.nx - 4) : (ix >= g.nx - 2 ? ix - (g.nx - 4) : ix);
             const int jz = iz <= 1 ? iz + (g.nz - 4) : (iz >= g.nz - 2 ? iz - (g.nz - 4) : iz);
             const long long i = packed(g, ix, iy, iz), j = packed(g, jx, iy, jz);
@@ -339,7 +357,15 @@ struct VofDims {
     long long N;
     int m;
     double dt, dx;
+    const unsigned *tube_col;  // the tube's column words, or null for the slab
 };
+
+// a wall node of the unpadded frame (beyond the slab's y range, or outside
+// the tube's disc)
+__device__ __forceinline__ bool vof_wall(const VofDims &d, int x, int y) {
+    if (d.tube_col) return x < 0 || x >= d.nx || y < 0 || y >= d.ny || !(d.tube_col[x + d.nx * y] & 1u);
+    return y < 0 || y >= d.ny;
+}
 
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
@@ -366,7 +392,7 @@ __global__ __launch_bounds__(VWG) void k_thinc_flux(VofDims d, const double *__r
     const unsigned r = j / (unsigned)d.nx, izu = r / (unsigned)d.ny;
     const int ix = (int)(j - r * (unsigned)d.nx), iy = (int)(r - izu * (unsigned)d.ny), iz = (int)izu;
     auto V = [&](int x, int y, int z) -> double {
-        if (y < 0 || y >= d.ny) return 0.0;  // wall layers
+        if (vof_wall(d, d.tube_col ? x : 0, y)) return 0.0;  // wall nodes hold vf = 0
         x = wrapi(x, d.nx);
         z = wrapi(z, d.nz);
         return vin[(unsigned)x + (unsigned)d.nx * ((unsigned)y + (unsigned)d.ny * (unsigned)z)];
@@ -379,8 +405,12 @@ __global__ __launch_bounds__(VWG) void k_thinc_flux(VofDims d, const double *__r
     constexpr int ex = D == 0, ey = D == 1, ez = D == 2;
     // flux through the face between (bx,by,bz) - e_D and (bx,by,bz)
     auto face = [&](int bx, int by, int bz) -> double {
-        if (D == 1 && (by == 0 || by == d.ny)) return 0.0;  // a wall node on one side
         const int ax = bx - ex, ay = by - ey, az = bz - ez;
+        if (d.tube_col) {  // a wall node on either side (thincwlic.cu:205)
+            if (vof_wall(d, bx, by) || vof_wall(d, ax, ay)) return 0.0;
+        } else if (D == 1 && (by == 0 || by == d.ny)) {
+            return 0.0;
+        }
         const double uc = (Uv(bx, by, bz) + Uv(ax, ay, az)) / 2.0;
         const int isgn = uc <= 0.0 ? 0 : 1;
         const int px = isgn ? ax : bx, py = isgn ? ay : by, pz = isgn ? az : bz;
@@ -437,6 +467,10 @@ __global__ __launch_bounds__(VWG) void k_thinc_update(VofDims d, const double *_
     const unsigned r = j / (unsigned)d.nx, izu = r / (unsigned)d.ny;
     const int ix = (int)(j - r * (unsigned)d.nx), iy = (int)(r - izu * (unsigned)d.ny), iz = (int)izu;
     const unsigned nxy = (unsigned)d.nx * (unsigned)d.ny;
+    if (d.tube_col && vof_wall(d, ix, iy)) {  // outside the tube: vf = 0
+        vout[j] = 0.0;
+        return;
+    }
     double hi;
     if (D == 0) hi = flux[ix == d.nx - 1 ? j + 1 - (unsigned)d.nx : j + 1];
     else if (D == 1) hi = iy == d.ny - 1 ? 0.0 : flux[j + (unsigned)d.nx];
@@ -453,6 +487,7 @@ FtGeom ft_geom(const lbmibm_ctx *ctx) {
     g.lz = ctx->p.lz * ctx->p.dx;
     g.nxh = (g.nx + 1) / 2;
     g.half = (long long)g.nxh * g.ny * g.nz;
+    g.tube_col = ctx->p.tube_radius > 0.0 ? (const unsigned *)ctx->tube_col : nullptr;
     return g;
 }
 
@@ -593,7 +628,8 @@ lbmibm_status vof_front_tracking(lbmibm_ctx *ctx, int *iterations, int *attempts
 lbmibm_status vof_thinc(lbmibm_ctx *ctx) {
     lbmibm_status s = vof_alloc(ctx);
     if (s != LBMIBM_OK) return s;
-    VofDims d{ctx->nx, ctx->ny, ctx->nz, ctx->N, ctx->p.m, ctx->p.dt, ctx->p.dx};
+    VofDims d{ctx->nx, ctx->ny, ctx->nz, ctx->N, ctx->p.m, ctx->p.dt, ctx->p.dx,
+              ctx->p.tube_radius > 0.0 ? (const unsigned *)ctx->tube_col : nullptr};
     const unsigned nb = blocks_of(ctx->N);
     const size_t N = (size_t)ctx->N;
     auto sweep = [&](auto kflux, auto kupd, auto u) {

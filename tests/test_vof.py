@@ -410,3 +410,33 @@ def test_vof_arguments_and_states_are_checked(tmp_path):
     h = gpu_of(*GL, "f64", cells=cells, vof_period=2, **VOF_KW)
     h.read_restart(path)
     assert np.array_equal(h.vf(), g.vf())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-7), ("f32", 2e-5)])
+def test_vof_in_the_tube_matches_the_oracle(prec, tol):
+    """CapsuleCHANNELFLOW: the indicator of a cell in the tube (walls outside
+    the disc, z periodic; the fold, the BUFF rows and the THINC copies in z
+    only) and steps with a viscosity contrast."""
+    print(f"[vof tube] prec={prec}", flush=True)
+    cells = _sphere((0.4, -0.3, 0.2), 3.0, level=3)
+    kw = dict(tube_radius=7.5, tau=0.9, p=2e-5, e=(0.0, 0.0, 1.0), force_modulated=2, flowstart=0, b=5e-3,
+              lambda1=5.0, vof_period=2, **VOF_KW)
+    o = oracle_of(18, 18, 14, cells=cells, **kw)
+    g = gpu_of(18, 18, 14, prec, cells=cells, **kw)
+    o.volume_init()
+    g.volume_init()
+    assert o.front_tracking()[0] == 0
+    g.front_tracking()
+    keep = o.bc != 2
+    assert rel(g.vf()[keep], o.vf[keep]) < 1e-8
+    assert not g.vf()[o.bc == 2].any()
+    comp = o.bc == 0
+    assert abs(o.vf[comp].sum() / (4 / 3 * np.pi * 27) - 1) < 0.03
+    for it in range(1, 5):
+        assert o.step(it)[0] == 0
+        g.step(it)
+    assert rel(g.vf()[comp], o.vf[comp]) < max(tol, 1e-6)
+    w = 1.0 / np.array([3] + [18] * 6 + [36] * 12)
+    fo, fg = o.fn.reshape(-1, Q)[comp] - w, g.populations().reshape(-1, Q)[comp] - w
+    assert rel(fg, fo) < tol * 50
