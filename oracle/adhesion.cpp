@@ -30,8 +30,14 @@ struct Rec {
     double x, y, z;
 };
 
+// Receptor id = iz * nxw + ix: the slab's wall y = -LY/2 (adhesion.cu:452-
+// 457) or, in the tube, the ring of radius R at angle ix * dxw (nxw = ntheta,
+// dxw = dtheta, adhesion.cu:475-490).
 inline Rec receptor(const orc_params *p, const orc_adhesion *a, int id) {
     const int iz = id / a->nxw, ix = id - iz * a->nxw;
+    if (p->tube)
+        return Rec{p->tube_radius * std::cos(ix * a->dxw), p->tube_radius * std::sin(ix * a->dxw),
+                   (iz - a->nzw / 2) * a->dzw};
     return Rec{(ix - a->nxw / 2) * a->dxw, -0.5 * p->ly, (iz - a->nzw / 2) * a->dzw};
 }
 
@@ -45,7 +51,9 @@ void search(const orc_params *p, const orc_state *s, orc_adhesion *a, int v) {
     for (int k = 0; k < MAXN; k++) { ids[k] = -1; rs[k] = 1.0e15; }  // InitNeighWall_GPU
     if (!a->is_knob[v]) return;
     const double dx = p->dx, xi = s->x[v], yi = s->y[v], zi = s->z[v];
-    if (std::sqrt(yi * yi) < 0.5 * p->ly - a->limit * dx) return;
+    if (p->tube ? std::sqrt(xi * xi + yi * yi) < p->tube_radius - a->limit * dx  // adhesion.cu:248-249
+                : std::sqrt(yi * yi) < 0.5 * p->ly - a->limit * dx)
+        return;
     const int iz = (int)((zi - (-p->lz / 2.0)) / a->dzw);
     const int span = a->limit * (int)(dx / a->dzw);
     int cnt = 0;
@@ -162,8 +170,15 @@ double orc_adh_uniform(unsigned long long seed, long long iter, long long sid) {
 }
 
 int orc_adhesion_grid(const orc_params *p, orc_adhesion *a) {
-    // wall_init, CapsuleSHEARFLOW (adhesion.cu:441-458)
     double dz = std::sqrt(1.0 / a->receptor_density) / a->rc;
+    if (p->tube) {  // wall_init, CapsuleCHANNELFLOW (adhesion.cu:467-480)
+        a->nzw = (int)(p->lz / dz + 1.0e-03);
+        a->dzw = p->lz / (double)a->nzw;
+        a->dxw = a->dzw / p->tube_radius;
+        a->nxw = (int)(2.0 * M_PI / a->dxw);
+        return a->nxw * a->nzw;
+    }
+    // wall_init, CapsuleSHEARFLOW (adhesion.cu:441-458)
     a->nzw = (int)(p->lz / dz + 1.0e-03);
     a->nxw = (int)(p->lx / dz + 1.0e-03);
     a->dzw = p->lz / (double)a->nzw;
@@ -200,7 +215,7 @@ void orc_adhesion_step(const orc_params *p, orc_state *s, orc_adhesion *a, int i
                 const int id = a->attach[v * NS + k];
                 if (id == -1) continue;
                 const Rec rc = receptor(p, a, id);
-                const double ddx = min_image_signed(rc.x - xi, p->lx, xi, rc.x);
+                const double ddx = p->tube ? rc.x - xi : min_image_signed(rc.x - xi, p->lx, xi, rc.x);
                 const double ddy = rc.y - yi;
                 const double ddz = min_image_signed(rc.z - zi, p->lz, zi, rc.z);
                 const double r = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);

@@ -34,6 +34,8 @@ struct AdhDev {
     int nv;
     int wlc;                                      // 0 LINEAR_SPRING, 1 WLC
     double lcont, f_wlc, ra, alpha1, alpha2;
+    int ring;                                     // the tube's receptor ring
+    double radius;
 };
 
 // Extension measure and force of the Bell rates: LINEAR_SPRING
@@ -74,10 +76,17 @@ __global__ void k_adh_tick(unsigned long long *__restrict__ calls) {
     if (threadIdx.x == 0 && blockIdx.x == 0) calls[0] = calls[0] + 1ull;
 }
 
+// Receptor id = iz * nxw + ix on the slab's wall y = -LY/2 (adhesion.cu:
+// 452-457) or on the tube's ring at angle ix * dxw (adhesion.cu:475-490).
 __device__ __forceinline__ void receptor(const AdhDev &a, int id, double &x, double &y, double &z) {
     const int iz = id / a.nxw, ix = id - iz * a.nxw;
-    x = (ix - a.nxw / 2) * a.dxw;
-    y = -0.5 * a.ly;
+    if (a.ring) {
+        x = a.radius * cos(ix * a.dxw);
+        y = a.radius * sin(ix * a.dxw);
+    } else {
+        x = (ix - a.nxw / 2) * a.dxw;
+        y = -0.5 * a.ly;
+    }
     z = (iz - a.nzw / 2) * a.dzw;
 }
 
@@ -118,7 +127,8 @@ __global__ __launch_bounds__(AWG) void k_adh_force(AdhDev a, const double *__res
             if (id == -1) continue;
             double xj, yj, zj;
             receptor(a, id, xj, yj, zj);
-            const double dx = image_shift(xj - xi, a.lx, xi, xj), dy = yj - yi, dz = image_shift(zj - zi, a.lz, zi, zj);
+            const double dx = a.ring ? xj - xi : image_shift(xj - xi, a.lx, xi, xj), dy = yj - yi;
+            const double dz = image_shift(zj - zi, a.lz, zi, zj);
             const double r = sqrt(dx * dx + dy * dy + dz * dz);
             if (a.wlc) {
                 const double LL = a.lcont * a.dx;
@@ -156,13 +166,15 @@ __global__ __launch_bounds__(AWG) void k_adh_search(AdhDev a, const double *__re
     double *rs = r_wall + (size_t)v * a.maxn;
     for (int k = 0; k < a.maxn; k++) { ids[k] = -1; rs[k] = 1.0e15; }
     const double xi = xs[v], yi = xs[a.nv + v], zi = xs[2 * (size_t)a.nv + v];
-    if (sqrt(yi * yi) < 0.5 * a.ly - a.limit * a.dx) return;
+    if (a.ring ? sqrt(xi * xi + yi * yi) < a.radius - a.limit * a.dx  // adhesion.cu:248-249
+               : sqrt(yi * yi) < 0.5 * a.ly - a.limit * a.dx)
+        return;
     const int iz = (int)((zi - (-a.lz / 2.0)) / a.dzw);
     const int span = a.limit * (int)(a.dx / a.dzw);
     // only columns within LIMIT dx in x can pass the range test; scanning
     // them in the same increasing order keeps the list (and its ties)
-    const int jx0 = max(0, (int)floor((xi - a.limit * a.dx) / a.dxw) + a.nxw / 2 - 1);
-    const int jx1 = min(a.nxw - 1, (int)ceil((xi + a.limit * a.dx) / a.dxw) + a.nxw / 2 + 1);
+    const int jx0 = a.ring ? 0 : max(0, (int)floor((xi - a.limit * a.dx) / a.dxw) + a.nxw / 2 - 1);
+    const int jx1 = a.ring ? a.nxw - 1 : min(a.nxw - 1, (int)ceil((xi + a.limit * a.dx) / a.dxw) + a.nxw / 2 + 1);
     int cnt = 0;
     for (int i = 0; i < 2 * span; i++) {
         int jz = iz + i - (span - 1);
@@ -309,6 +321,8 @@ AdhDev adh_dev(const lbmibm_ctx *ctx) {
     a.nv = ctx->nv;
     a.wlc = h.wlc;
     a.lcont = h.lcont; a.f_wlc = h.f_wlc; a.ra = h.ra; a.alpha1 = h.alpha1; a.alpha2 = h.alpha2;
+    a.ring = ctx->p.tube_radius > 0.0;
+    a.radius = ctx->p.tube_radius;
     return a;
 }
 
@@ -363,17 +377,26 @@ lbmibm_status adhesion_step(lbmibm_ctx *ctx) {
 
 lbmibm_status adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *h, const unsigned char *is_knob) {
     ctx->adh = *h;
-    // wall_init, CapsuleSHEARFLOW (adhesion.cu:441-458)
+    // wall_init (adhesion.cu:441-458 slab, 467-480 tube ring)
     const double dx = ctx->p.dx, LX = ctx->p.lx * dx, LZ = ctx->p.lz * dx;
     const double dz = std::sqrt(1.0 / h->receptor_density) / h->rc;
     ctx->adh_nzw = (int)(LZ / dz + 1.0e-03);
-    ctx->adh_nxw = (int)(LX / dz + 1.0e-03);
-    if (ctx->adh_nzw < 1 || ctx->adh_nxw < 1) {
+    if (ctx->adh_nzw < 1) {
         ctx->err = "adhesion: receptor spacing larger than the box";
         return LBMIBM_ERR_ARG;
     }
     ctx->adh_dzw = LZ / (double)ctx->adh_nzw;
-    ctx->adh_dxw = LX / (double)ctx->adh_nxw;
+    if (ctx->p.tube_radius > 0.0) {
+        ctx->adh_dxw = ctx->adh_dzw / ctx->p.tube_radius;  // dtheta
+        ctx->adh_nxw = (int)(2.0 * M_PI / ctx->adh_dxw);   // ntheta
+    } else {
+        ctx->adh_nxw = (int)(LX / dz + 1.0e-03);
+        ctx->adh_dxw = ctx->adh_nxw > 0 ? LX / (double)ctx->adh_nxw : 0.0;
+    }
+    if (ctx->adh_nxw < 1) {
+        ctx->err = "adhesion: receptor spacing larger than the box";
+        return LBMIBM_ERR_ARG;
+    }
     const long long nrec = (long long)ctx->adh_nxw * ctx->adh_nzw, nv = ctx->nv;
     const long long nsp = nv * h->springs_per_knob, nn = nv * h->max_neighwall;
     // knobs: the caller's flags (the reference's ./input/icosa file) or

@@ -810,10 +810,10 @@ lbmibm_status lbmibm_upload_vf(lbmibm_ctx *ctx, const double *vf) {
 
 lbmibm_status lbmibm_adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *a, const unsigned char *is_knob) {
     if (!ctx || !a) return LBMIBM_ERR_ARG;
-    if (ctx->nc < 1 || ctx->p.periodic_y || ctx->p.tube_radius > 0.0 || !(a->receptor_density > 0.0) ||
+    if (ctx->nc < 1 || ctx->p.periodic_y || !(a->receptor_density > 0.0) ||
         !(a->rc > 0.0) || !(a->shear_rate > 0.0) || !(a->kbt > 0.0) || a->gs == 0.0 || a->limit < 1 ||
         a->max_neighwall < 1 || a->springs_per_knob < 1) {
-        ctx->err = "adhesion: needs cells, the slab and positive constants";
+        ctx->err = "adhesion: needs cells, walls (the slab or the tube) and positive constants";
         return LBMIBM_ERR_ARG;
     }
     lbmibm_status s = join_volume(ctx);

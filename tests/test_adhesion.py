@@ -286,3 +286,67 @@ def test_adhesion_with_repeated_iteration_numbers_matches_the_oracle():
     gs = g.adhesion_state(nrec)
     assert np.array_equal(gs["attach"], st["attach"])
     assert np.array_equal(gs["is_attached"], st["is_attached"])
+
+
+def _tube_cell():
+    from preprocess_src_amd import mesh
+    return mesh.assemble([2], np.array([[4.1, 0.3, -0.2]]), np.eye(3)[None], 3.0, shape="sphere")
+
+
+TUBE = dict(tube_radius=7.5)
+
+
+def test_tube_ring_search_is_the_brute_force_neighbourhood():
+    # wall_init of the tube: the ring r = R at angles k dtheta, dtheta = dz/R
+    cells = _tube_cell()
+    o = oracle_of(18, 18, 14, cells=cells, **TUBE)
+    st = o.adhesion_setup(ADH, _knobs(cells))
+    A = o._adhs
+    R = TUBE["tube_radius"]
+    dz = 14 / int(14 / 0.5 + 1e-3)
+    assert abs(A.dzw - dz) < 1e-15 and abs(A.dxw - dz / R) < 1e-15 and A.nxw == int(2 * np.pi / (dz / R))
+    k = np.arange(A.nxw * A.nzw)
+    iz, it = k // A.nxw, k % A.nxw
+    rec = np.stack([R * np.cos(it * A.dxw), R * np.sin(it * A.dxw), (iz - A.nzw // 2) * A.dzw], 1)
+    pts = np.stack([cells["x"], cells["y"], cells["z"]], 1)
+    MAXN, seen = ADH["max_neighwall"], 0
+    for v in np.flatnonzero(st["is_knob"]):
+        ids = st["id_wall"][v * MAXN:(v + 1) * MAXN]
+        d = np.linalg.norm(rec - pts[v], axis=1)
+        got = ids[ids >= 0]
+        if np.hypot(pts[v, 0], pts[v, 1]) < R - ADH["limit"]:
+            assert len(got) == 0
+            continue
+        assert np.allclose(st["r_wall"][v * MAXN:(v + 1) * MAXN][ids >= 0], d[got], rtol=0, atol=1e-12)
+        want = np.flatnonzero(d <= ADH["limit"] - 0.6)
+        if 0 < len(want) <= MAXN:
+            assert set(want) <= set(got.tolist())
+        seen += len(got) > 0
+    assert seen >= 5
+
+
+@pytest.mark.gpu
+def test_adhesion_in_the_tube_matches_the_oracle():
+    print("[adhesion tube]", flush=True)
+    cells = _tube_cell()
+    kw = dict(TUBE, tau=0.9, p=2e-5, e=(0.0, 0.0, 1.0), force_modulated=2, flowstart=0, b=5e-3)
+    o = oracle_of(18, 18, 14, cells=cells, **kw)
+    g = gpu_of(18, 18, 14, "f64", cells=cells, **kw)
+    kn = _knobs(cells)
+    st = o.adhesion_setup(ADH, kn)
+    g.adhesion_setup(ADH, kn)
+    o.volume_init()
+    g.volume_init()
+    nrec = o._adhs.nxw * o._adhs.nzw
+    gs = g.adhesion_state(nrec)
+    assert np.array_equal(gs["id_wall"], st["id_wall"]) and np.array_equal(gs["attach"], st["attach"])
+    assert (st["attach"] >= 0).sum() > 5
+    for it in range(1, 9):
+        assert o.step(it)[0] == 0
+        g.step(it)
+    gs = g.adhesion_state(nrec)
+    assert np.array_equal(gs["attach"], st["attach"])
+    assert rel(gs["f_adh"], st["f_adh"]) < 1e-10
+    sg = g.cells_state()
+    for k in ("x", "y", "z"):
+        assert np.max(np.abs(sg[k] - o.cells[k])) < 1e-10, k
