@@ -218,8 +218,9 @@ lbmibm_status lbmibm_thinc_wlic(lbmibm_ctx *ctx);
 lbmibm_status lbmibm_download_vf(lbmibm_ctx *ctx, double *vf);
 lbmibm_status lbmibm_upload_vf(lbmibm_ctx *ctx, const double *vf);
 
-/* ---- cytoadhesion (adhesion.cu; slab, linear spring or WLC law) --------
- * Knob vertices bind receptors on the wall y = -LY/2 through springs_per_knob
+/* ---- cytoadhesion (adhesion.cu; slab wall or tube ring, linear or WLC) --
+ * Knob vertices bind receptors (on the wall y = -LY/2, or on the ring r = R
+ * of the tube) through springs_per_knob
  * springs each, by the Bell model; the spring force joins the membrane force
  * in the spreading (ibm_st1bb, ibm.cu:133-212).  Once set up, lbmibm_step
  * runs Adhesion() after stress (main.cu:84-86).  Random numbers: splitmix64
@@ -249,7 +250,7 @@ typedef struct lbmibm_adhesion_params {
  * (is_knob [Nv], the reference's ./input/icosa%d.dat flags, or NULL to draw
  * them with probability KNOB / cell vertices), the neighbour search and the
  * attachment of every spring to the nearest free receptors.  Needs cells and
- * the slab (no periodic_y, no tube). */
+ * walls (the slab or the tube; not periodic_y). */
 lbmibm_status lbmibm_adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *a, const unsigned char *is_knob);
 /* Adhesion() (adhesion.cu:772-790): spring force, then search + Bell. */
 lbmibm_status lbmibm_adhesion(lbmibm_ctx *ctx);
