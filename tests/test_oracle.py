@@ -515,3 +515,127 @@ def test_wall_shear_is_the_mean_first_node_velocity():
     walls = [0, ly - 1]
     expect = 2 * df * np.mean([0.6 * u[:, iy, :].mean() + 0.8 * w[:, iy, :].mean() for iy in walls])
     assert abs(J - expect) <= 1e-13 * abs(expect)
+
+
+# ---- pins of the RK stage offsets, the modulated force and the neo-Hookean law ----
+
+def _m1(t):
+    """Closed-form first moment of the cosine delta, sum_j (j - X) phi(X - j)
+    with t = frac(X) (in units of the spacing)."""
+    return -(t - 0.5 + (np.cos(np.pi * t / 2) - np.sin(np.pi * t / 2)) / 2)
+
+
+def _linear_interp(X, a, B, lo, h):
+    """Interpolation of the linear field u(x) = a + B x by the cosine delta:
+    a + B (X + h m1(t)), t the fraction of X in the node frame (nodes at
+    lo + j h); exact because sum phi = 1 and the first moment is m1."""
+    t = np.mod((X - lo[:, None]) / h, 1.0)
+    return a[:, None] + B @ (X + h * _m1(t))
+
+
+def _linear_field_oracle(rk, dx, m, cr):
+    lx, ly, lz = 16 * dx, 14 * dx, 12 * dx
+    cells = small_cells(lx, ly, lz, nc=1, radius=3.0 * dx, levels=[2], seed=71)
+    o = OracleSim(lx, ly, lz, dx=dx, tau=0.8, runge_kutta=rk, m=m, cr=cr, cells=cells)
+    rng = np.random.default_rng(72)
+    nv = o.cells["x"].size
+    # vertices anywhere in the interior, well clear of the padded edges
+    for k, L in (("x", lx), ("y", ly), ("z", lz)):
+        o.cells[k][:] = rng.uniform(-L / 2 + 3 * dx, L / 2 - 3 * dx, nv)
+    a = np.array([0.011, -0.007, 0.005])
+    B = np.array([[0.004, 0.021, -0.006], [-0.013, 0.003, 0.009], [0.008, -0.017, 0.002]]) / dx
+    X, Y, Z = o.node_coords()
+    o.um[:], o.vm[:], o.wm[:] = a[:, None] + B @ np.stack([X, Y, Z])  # every padded node
+    lo = np.array([-lx / 2 - 1.5 * dx, -ly / 2 - 1.5 * dx, -lz / 2 - 1.5 * dx])
+    return o, a, B, lo
+
+
+@pytest.mark.parametrize("dx,m,cr", [(1.0, 2, 0.7), (0.5, 3, 0.9)])
+def test_rk4_stages_on_a_linear_field(dx, m, cr):
+    """RK4 with the field frozen (ibm.cu:430-517): stages at X, X + k1 M dt/2,
+    X + k2 M dt/2, X + k3 M dt, then X + M dt (k1 + 2k2 + 2k3 + k4)/6.  The
+    field is linear, so each stage's velocity is a + B (X + h m1(t)) in
+    closed form; a slipped stage offset (k M dt for the second stage, k1 for
+    the third) moves the result by ~1e-4, far above the 1e-13 bound."""
+    o, a, B, lo = _linear_field_oracle(4, dx, m, cr)
+    X0 = np.stack([o.cells[k].copy() for k in "xyz"])
+    o.ibm_st2()
+    h, Mdt = dx, m * o.P.dt
+    k1 = _linear_interp(X0, a, B, lo, h)
+    k2 = _linear_interp(X0 + k1 * Mdt / 2, a, B, lo, h)
+    k3 = _linear_interp(X0 + k2 * Mdt / 2, a, B, lo, h)
+    k4 = _linear_interp(X0 + k3 * Mdt, a, B, lo, h)
+    um = (k1 + 2 * k2 + 2 * k3 + k4) / 6
+    X1 = X0 + um * Mdt
+    for d, k in enumerate("xyz"):
+        assert np.max(np.abs(o.cells[k] - X1[d])) < 1e-13, k
+        assert np.max(np.abs(o.cells["uvw"[d]] - um[d])) < 1e-14
+    # the slips named above are visible at this size
+    k3s = _linear_interp(X0 + k2 * Mdt, a, B, lo, h)
+    assert np.max(np.abs((k1 + 2 * k2 + 2 * k3s + k4) / 6 - um)) * Mdt > 1e-6
+
+
+def test_rk2_predictor_on_a_linear_field():
+    """RK2 (ibm.cu:520-564): the predictor is X + k1 M with no dt and the
+    stencil assumes unit spacing; the update is X + M dt (k1 + k2)/2."""
+    o, a, B, lo = _linear_field_oracle(2, 1.0, 2, 0.7)
+    X0 = np.stack([o.cells[k].copy() for k in "xyz"])
+    o.ibm_st2()
+    M, dt = 2, o.P.dt
+    k1 = _linear_interp(X0, a, B, lo, 1.0)
+    k2 = _linear_interp(X0 + k1 * M, a, B, lo, 1.0)
+    X1 = X0 + M * dt * (k1 + k2) / 2
+    for d, k in enumerate("xyz"):
+        assert np.max(np.abs(o.cells[k] - X1[d])) < 1e-13, k
+    k2dt = _linear_interp(X0 + k1 * M * dt, a, B, lo, 1.0)
+    assert np.max(np.abs(k2dt - k2)) * M * dt > 1e-6
+
+
+@pytest.mark.parametrize("flowstart", [0, 3])
+def test_modulated_body_force_follows_e_of_t(flowstart):
+    """force_modulated = 1 (lbm.cu:1032-1035): the force is P e (1 + 0.5
+    sin(pi t / FREQ)) with t = dt iter once iter > FLOWSTART, and 0 before.
+    In a fully periodic box the total momentum grows each step by exactly
+    sum 2 rho P e amp(iter), which fixes the gate, the time and the phase."""
+    P, e, freq, cr = 3e-5, np.array([0.6, 0.0, 0.8]), 7.0, 0.8
+    o = OracleSim(6, 5, 4, tau=0.7, p=P, e=tuple(e), cr=cr, periodic_y=True, force_modulated=1,
+                  flowstart=flowstart, freq=freq)
+    _random_state(o, 11)
+    m = o.comp_mask()
+    dt = o.P.dt
+    for it in range(1, 9):
+        f = o.fn.reshape(-1, Q)[m]
+        rho = f.sum(1)
+        J = (f @ CQ).sum(0)
+        o.lbm_d3q19(it)
+        dJ = (o.fn.reshape(-1, Q)[m] @ CQ).sum(0) - J
+        amp = 1.0 + 0.5 * np.sin(np.pi * dt * it / freq) if it > flowstart else 0.0
+        assert np.max(np.abs(dJ - 2 * rho.sum() * P * amp * e)) < 1e-13 * rho.sum(), it
+
+
+@pytest.mark.parametrize("rgs", [1.0, 1.7])
+def test_neohookean_equibiaxial_tension_matches_the_closed_form(rgs):
+    """NEOHOOKEAN (stress.cu:97-99): t = rgs (G^-1/J - g^-1 J/(I2+1)^2).  Under
+    x -> lam x, g = lam^2 G, J = lam^2, I2 + 1 = lam^4, so
+    t = rgs (1/lam^2 - 1/lam^8) G^-1."""
+    cells, o = _cell(neohookean=True, rgs=rgs)
+    lam = 1.013
+    cx = [cells[k].mean() for k in "xyz"]
+    o.cells["xd"][:] = _xd({k: cx[i] + lam * (cells[k] - cx[i]) for i, k in enumerate("xyz")})
+    o.calc_tension(1)
+    ref = rgs * (1 / lam ** 2 - 1 / lam ** 8) * o.cells["nref"]
+    assert np.max(np.abs(o.cells["t"] - ref)) < 1e-10 * np.abs(ref).max()
+
+
+def test_skalak_shear_term_alone_is_scaled_by_rgs():
+    """SKALAK (stress.cu:96): rgs multiplies only the (I1 + 1)/J term, not the
+    area term (C I2 - 1) J g^-1, so under x -> lam x,
+    t = [rgs (2 lam^2 - 1)/lam^2 + C (lam^4 - 1) - 1] G^-1."""
+    rgs, C_, lam = 1.7, 100.0, 1.01
+    cells, o = _cell(c=C_, rgs=rgs)
+    cx = [cells[k].mean() for k in "xyz"]
+    o.cells["xd"][:] = _xd({k: cx[i] + lam * (cells[k] - cx[i]) for i, k in enumerate("xyz")})
+    o.calc_tension(1)
+    coef = rgs * (2 * lam ** 2 - 1) / lam ** 2 + C_ * (lam ** 4 - 1) - 1
+    ref = coef * o.cells["nref"]
+    assert np.max(np.abs(o.cells["t"] - ref)) < 1e-10 * np.abs(ref).max()
