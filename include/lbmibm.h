@@ -100,6 +100,18 @@ typedef struct lbmibm_params {
                                 are walls, z is periodic, links crossing r = R take the
                                 interpolated bounce-back (lbm_cylinder.cu:61-235); needs
                                 R <= min(LX,LY)/2 - dx and periodic_y = 0. 0: off          */
+    int    tube_staircase;   /* with tube_radius > 0: 1 takes the half-way bounce-back of
+                                lbm_CapsuleCHANNELFLOW on the tube mask (lbm.cu:1050-1094)
+                                on every link into a wall node; 0 the interpolated bounce-
+                                back of lbm_cylinder.cu:61-235                               */
+    int    wall_motion;      /* slab walls sliding along z, with Ladd's momentum term
+                                -6 w rho Cr (c . u_w) on the bounced links (lbm.cu:951-970):
+                                0 at rest; 1 both walls, y = +LY/2 at +U and y = -LY/2 at -U
+                                (allwall, and lbm_SHEARFLOW, lbm.cu:455-468); 2 only
+                                y = +LY/2 at +U (topwall).  The walls rest while
+                                iter < FLOWSTART (sra = 0, lbm.cu:755).  Needs the slab     */
+    double wall_velocity;    /* U in units of Uc (lattice speed Cr U): sr LY/2 for allwall,
+                                sr LY for topwall, ub = 1 in lbm_SHEARFLOW                  */
 } lbmibm_params;
 
 /* Initial cell set: reference SoA positions and element vertex ids. */

@@ -185,6 +185,11 @@ void orc_lbm_step(const orc_params *p, orc_state *s, int iter) {
     const bool reset = ((iter - 1) % p->m) == 0;
     const double *fn = s->fn;
     double *fm = s->fm;
+    // sliding walls (lbm_CapsuleSHEARFLOW, lbm.cu:755, 951-970; lbm_SHEARFLOW,
+    // lbm.cu:455-468): z velocity of the wall above (+y) and below (-y)
+    const double sra = iter < p->flowstart ? 0.0 : p->wall_velocity;
+    const double uw_top = p->wall_motion ? sra : 0.0;
+    const double uw_bot = p->wall_motion == 1 ? -sra : 0.0;
 
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++) {
@@ -230,10 +235,18 @@ void orc_lbm_step(const orc_params *p, orc_state *s, int iter) {
             // Push with half-way bounce-back (lbm.cu:1273-1310).
             const int t = g.idx(ix + CX[q], iy + CY[q], iz + CZ[q]);
             int j;
-            if (s->bc[t] == ORC_WALL) j = i * Q + opp(q);
+            double fw = fs;
+            if (s->bc[t] == ORC_WALL) {
+                j = i * Q + opp(q);
+                // Ladd's term of a wall moving along z: f - 6 w td Cr (c . u_w)
+                if (p->wall_motion && !p->tube && CZ[q] != 0 && CY[q] != 0) {
+                    const double uw = CY[q] > 0 ? uw_top : uw_bot;
+                    fw = fs - 6.0 * td * Cr / WDIV[q] * (CZ[q] * uw);
+                }
+            }
             else if (s->bc[t] == ORC_BUFF) j = g.image(t) * Q + q;
             else j = t * Q + q;
-            fm[j] = fs;
+            fm[j] = fw;
         }
     }
 
@@ -243,7 +256,7 @@ void orc_lbm_step(const orc_params *p, orc_state *s, int iter) {
     // both sides of a link the reference's threads race on those slots, and
     // its f3 then reads a wall node's unused slot; here f2 and f3 are the
     // node's own half-way-bounced value.
-    if (p->tube) {
+    if (p->tube && !p->tube_staircase) {
         std::vector<std::pair<size_t, double>> upd;
         for (int i = 0; i < n; i++) {
             if (s->bc[i] != ORC_COMP) continue;

@@ -62,6 +62,11 @@ typedef struct orc_params {
     int    tube;            /* 1: CapsuleCHANNELFLOW tube: COMP where sqrt(x^2+y^2) <= R,  */
     double tube_radius;     /* periodic in z only, interpolated bounce-back on the links   */
                             /* that cross r = R (init1.cu:190-236, lbm_cylinder.cu)         */
+    int    tube_staircase;  /* with tube: 1 half-way bounce-back on the mask only          */
+                            /* (lbm_CapsuleCHANNELFLOW, lbm.cu:1050-1094, 1273-1310)       */
+    int    wall_motion;     /* slab walls sliding in z (lbm.cu:951-970): 0 none, 1 allwall */
+                            /* (+U at y = +LY/2, -U at y = -LY/2), 2 topwall (+U at top)  */
+    double wall_velocity;   /* U; sra = 0 while iter < FLOWSTART (lbm.cu:755)              */
 } orc_params;
 
 /* Pointer set equivalent to the reference's lattice/cell/fem structs

@@ -33,7 +33,8 @@ class OrcParams(C.Structure):
         ("itrmax", C.c_int), ("residual", C.c_double), ("runge_kutta", C.c_int),
         ("neohookean", C.c_int), ("fileoutput", C.c_int), ("volume_correction", C.c_int),
         ("vof_period", C.c_int), ("sor_itrmax", C.c_int), ("sor_residual", C.c_double),
-        ("tube", C.c_int), ("tube_radius", C.c_double),
+        ("tube", C.c_int), ("tube_radius", C.c_double), ("tube_staircase", C.c_int),
+        ("wall_motion", C.c_int), ("wall_velocity", C.c_double),
     ]
 
 
@@ -175,7 +176,7 @@ class OracleSim:
                  b=0.0, m=1, c=100.0, rgs=1.0, br=0.0, itrmax=1000, residual=1e-12,
                  runge_kutta=4, neohookean=False, fileoutput=1000, volume_correction=True,
                  periodic_y=False, cells=None, vof_period=0, sor_itrmax=None, sor_residual=None,
-                 tube_radius=0.0):
+                 tube_radius=0.0, tube_staircase=False, wall_motion=0, wall_velocity=0.0):
         if df is None:
             df = (2.0 * tau - 1.0) / 6.0  # tau = (6 Df + 1)/2 (init1.cu:140)
         P = OrcParams()
@@ -193,6 +194,8 @@ class OracleSim:
         P.neohookean, P.fileoutput, P.volume_correction = int(neohookean), fileoutput, int(volume_correction)
         P.vof_period = int(vof_period)
         P.tube, P.tube_radius = int(tube_radius > 0), float(tube_radius)
+        P.tube_staircase = int(tube_staircase)
+        P.wall_motion, P.wall_velocity = int(wall_motion), float(wall_velocity)
         P.sor_itrmax = int(itrmax if sor_itrmax is None else sor_itrmax)
         P.sor_residual = float(residual if sor_residual is None else sor_residual)
         self.P = P

@@ -119,6 +119,7 @@ lbmibm_status adj_common(lbmibm_ctx *ctx) {
     if (s == LBMIBM_OK) s = need(ctx, ctx->p.m == 1, "membrane sub-cycling M must be 1");
     if (s == LBMIBM_OK) s = need(ctx, ctx->p.vof_period == 0, "the VOF indicator has no adjoint");
     if (s == LBMIBM_OK) s = need(ctx, ctx->p.tube_radius == 0.0, "the tube geometry has no adjoint");
+    if (s == LBMIBM_OK) s = need(ctx, ctx->p.wall_motion == 0, "sliding walls have no adjoint");
     if (s == LBMIBM_OK) s = need(ctx, !ctx->adh_on, "cytoadhesion has no adjoint");
     if (s == LBMIBM_OK) s = adj_alloc(ctx);
     return s;

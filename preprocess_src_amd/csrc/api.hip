@@ -23,7 +23,7 @@ inline int opph(int q) { return q == 0 ? 0 : q + ((q % 2) << 1) - 1; }
 
 struct StepParams {
     int force_modulated, flowstart;
-    double dt, freq;
+    double dt, freq, wall_velocity;
 };
 
 __global__ void k_set_step(DevStep *s, int iter, StepParams p) {
@@ -34,6 +34,7 @@ __global__ void k_set_step(DevStep *s, int iter, StepParams p) {
     else if (p.force_modulated == 2) amp = it > p.flowstart ? 1.0 : 0.0;  // lbm_cylinder.cu:292-293
     s->iter = it;
     s->amp = amp;
+    s->wall = it < p.flowstart ? 0.0 : p.wall_velocity;  // sra (lbm.cu:755)
 }
 
 __global__ void k_reset_status(DevStatus *s) {
@@ -268,7 +269,7 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->vof[0], ctx->vof[1], ctx->ft_g, ctx->ft_x, ctx->ft_b, ctx->ft_part, ctx->ft_ctl,
                     ctx->tube_col, ctx->tube_r, ctx->adh_knob, ctx->adh_idw, ctx->adh_attach, ctx->adh_pos,
                     ctx->adh_active, ctx->adh_prop, ctx->adh_isatt, ctx->adh_claim, ctx->adh_rw, ctx->adh_rij,
-                    ctx->adh_fadh, ctx->adh_fsum, ctx->adh_nprop, ctx->adh_klist, ctx->adh_calls};
+                    ctx->adh_fadh, ctx->adh_fsum, ctx->adh_nprop, ctx->adh_klist, ctx->adh_calls, ctx->mw};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -287,7 +288,8 @@ void free_all(lbmibm_ctx *ctx) {
 
 namespace lbmibm {
 lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool) {
-    StepParams sp{ctx->p.force_modulated, ctx->p.flowstart, ctx->p.dt, ctx->p.freq};
+    StepParams sp{ctx->p.force_modulated, ctx->p.flowstart, ctx->p.dt, ctx->p.freq,
+                  ctx->p.wall_motion ? ctx->p.wall_velocity : 0.0};
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, ctx->stream, ctx->step, iter, sp);
     LBMIBM_LAUNCHED(ctx, "k_set_step");
     return LBMIBM_OK;
@@ -316,6 +318,9 @@ void lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz) {
     p->sor_itrmax = 1000;
     p->sor_residual = 1.0e-10;
     p->tube_radius = 0.0;
+    p->tube_staircase = 0;
+    p->wall_motion = 0;
+    p->wall_velocity = 0.0;
 }
 
 const char *lbmibm_last_error(const lbmibm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -344,6 +349,10 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     if (p->tube_radius < 0.0 || p->force_modulated < 0 || p->force_modulated > 2 ||
         (p->tube_radius > 0.0 && (p->periodic_y ||
                                   p->tube_radius > 0.5 * std::min(p->lx, p->ly) * p->dx - p->dx)))
+        return LBMIBM_ERR_ARG;
+    // sliding walls belong to the slab (CapsuleSHEARFLOW / SHEARFLOW)
+    if (p->wall_motion < 0 || p->wall_motion > 2 || p->tube_staircase < 0 || p->tube_staircase > 1 ||
+        (p->wall_motion != 0 && (p->periodic_y || p->tube_radius > 0.0)))
         return LBMIBM_ERR_ARG;
     if (p->vof_period > 0 && (p->periodic_y || !cells || cells->nc < 1 || p->sor_itrmax < 1 ||
                               !(p->sor_residual > 0.0) ||
@@ -393,6 +402,10 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         CK(dev_alloc(ctx, &ctx->Ui, 3 * (size_t)N * szT));
         CK(dev_alloc(ctx, &ctx->rho, (size_t)N * szT));
         CK(dev_alloc(ctx, &ctx->rhom, (size_t)N * szT));
+    }
+    if (p->wall_motion) {
+        CK(dev_alloc(ctx, &ctx->mw, 2 * (size_t)p->lx * p->lz * szT));
+        CK(lattice_fill(ctx, ctx->mw, 2 * (size_t)p->lx * p->lz, 0.0, false));
     }
     CK(dev_alloc(ctx, (void **)&ctx->step, sizeof(DevStep)));
     CK(dev_alloc(ctx, (void **)&ctx->status, sizeof(DevStatus)));
@@ -614,14 +627,16 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
                 for (int q = 1; q < Q; q++)
                     if (!tube_inside(ctx, ix - CXh[q], iy - CYh[q])) {
                         w |= 1u << q;
-                        ctx->tube_rh[c * Q + q] = tube_fraction(ctx, ix, iy, opph(q));
+                        if (!p->tube_staircase) ctx->tube_rh[c * Q + q] = tube_fraction(ctx, ix, iy, opph(q));
                     }
                 col[c] = w;
             }
         CK(dev_alloc(ctx, &ctx->tube_col, ncol * sizeof(unsigned)));
-        CK(dev_alloc(ctx, &ctx->tube_r, ncol * Q * sizeof(double)));
         LBMIBM_HIP(ctx, hipMemcpy(ctx->tube_col, col.data(), ncol * sizeof(unsigned), hipMemcpyHostToDevice));
-        LBMIBM_HIP(ctx, hipMemcpy(ctx->tube_r, ctx->tube_rh.data(), ncol * Q * sizeof(double), hipMemcpyHostToDevice));
+        if (!p->tube_staircase) {  // the staircase bounces every cut link half-way
+            CK(dev_alloc(ctx, &ctx->tube_r, ncol * Q * sizeof(double)));
+            LBMIBM_HIP(ctx, hipMemcpy(ctx->tube_r, ctx->tube_rh.data(), ncol * Q * sizeof(double), hipMemcpyHostToDevice));
+        }
     }
     CK(reset_status(ctx));
     CK(sync_check(ctx));
@@ -934,6 +949,13 @@ lbmibm_status lbmibm_download_populations(lbmibm_ctx *ctx, double *fn) {
     std::vector<double> fs((size_t)Q * N);
     lbmibm_status s = lattice_get_fstar(ctx, fs.data());
     if (s != LBMIBM_OK) return s;
+    // sliding walls: the term the next pull adds to the bounced links
+    std::vector<double> mw;
+    if (ctx->mw) {
+        mw.resize(2 * (size_t)ctx->nx * ctx->nz);
+        s = get_T(ctx, mw.data(), ctx->mw, mw.size());
+        if (s != LBMIBM_OK) return s;
+    }
     for (int iz = 0; iz < ctx->nz; iz++)
         for (int iy = 0; iy < ctx->ny; iy++)
             for (int ix = 0; ix < ctx->nx; ix++) {
@@ -942,6 +964,10 @@ lbmibm_status lbmibm_download_populations(lbmibm_ctx *ctx, double *fn) {
                 for (int q = 0; q < Q; q++) {
                     const long long src = pull_source(ctx, ix, iy, iz, q);
                     double f = src < 0 ? fs[(size_t)opph(q) * N + i] : fs[(size_t)q * N + src];
+                    if (src < 0 && !mw.empty() && CZh[q] != 0) {
+                        const size_t side = CYh[q] == 1 ? 0 : (size_t)ctx->nx * ctx->nz;
+                        f += (double)CZh[q] / WDh[q] * mw[side + ix + (size_t)ctx->nx * iz];
+                    }
                     if (src < 0 && ctx->p.tube_radius > 0.0 && tube_inside(ctx, ix, iy)) {
                         // the interpolated bounce-back of the kernel (lbm_cylinder.cu:61-85)
                         const double r = ctx->tube_rh[((size_t)ix + (size_t)ctx->nx * iy) * Q + q];

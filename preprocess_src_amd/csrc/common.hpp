@@ -16,12 +16,14 @@ constexpr int Q = 19;
 constexpr int WG = 256;  // work-group size of the streaming kernels (4 wave64)
 
 // Per-step values that kernels read from device memory, so that a captured
-// graph replays unchanged: the iteration counter and the body-force
-// amplitude e(t) of lbm.cu:1032-1035.
+// graph replays unchanged: the iteration counter, the body-force amplitude
+// e(t) of lbm.cu:1032-1035 and the sliding-wall speed sra U, 0 before
+// FLOWSTART (lbm.cu:755).
 struct DevStep {
     int iter;
     int pad;
     double amp;
+    double wall;
 };
 
 // Device status words, checked by the host after a synchronisation.
@@ -162,8 +164,12 @@ struct lbmibm_ctx {
     int ft_nparts = 0;
     // tube geometry (tube_radius > 0): per-column words (bit 0 inside the
     // disc, bit q the pull source of q is a wall) and wall fractions [col][19]
-    void *tube_col = nullptr, *tube_r = nullptr;
-    std::vector<double> tube_rh;  // host copy of the fractions
+    void *tube_col = nullptr, *tube_r = nullptr;  // tube_r stays null for the staircase
+    std::vector<double> tube_rh;  // host copy of the fractions (-1 everywhere for the staircase)
+    // sliding walls (wall_motion > 0): per wall-row node, 6 rho Cr u_w(iter) of
+    // the step that last collided it, [2][nx*nz] (T), bottom row then top row;
+    // the next pull adds w_q c_qz times it to the bounced links (lbm.cu:951-970)
+    void *mw = nullptr;
     // cytoadhesion (after lbmibm_adhesion_setup): constants, receptor grid,
     // per-vertex knob flags [Nv], receptor lists [Nv*MAXN], springs
     // [Nv*NS] (receptor or -1, r_ij, round state), receptor flags and

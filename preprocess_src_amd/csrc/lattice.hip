@@ -55,7 +55,8 @@ struct LbmDims {
 // y ends) or the tube (nodes outside the disc are walls; links that cross
 // the disc's edge are interpolated from the node's own populations,
 // lbm_cylinder.cu:61-85, 193-235).
-enum { GEOM_PERIODIC = 0, GEOM_SLAB = 1, GEOM_TUBE = 2 };
+// GEOM_SHEAR is the slab with walls sliding along z (lbm.cu:951-970).
+enum { GEOM_PERIODIC = 0, GEOM_SLAB = 1, GEOM_TUBE = 2, GEOM_SHEAR = 3 };
 
 // BounceBackLinearInterpolation_DEV (lbm_cylinder.cu:61-85): r is the
 // wall's fraction of the link from the node, f1 the half-way bounced value,
@@ -76,8 +77,9 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
                                             T inv_cr, T pex, T pey, T pez, T B, const DevStep *__restrict__ st,
                                             int m, const double *__restrict__ vf, T l1m1, T df,
                                             const unsigned *__restrict__ tube_col,
-                                            const double *__restrict__ tube_r) {
-    constexpr bool WALLS = GEOM == GEOM_SLAB;
+                                            const double *__restrict__ tube_r, T *__restrict__ mw, T mw_lo,
+                                            T mw_hi) {
+    constexpr bool WALLS = GEOM == GEOM_SLAB || GEOM == GEOM_SHEAR, MOVING = GEOM == GEOM_SHEAR;
     const int lb = xcd_block(d.nblocks);
     if (lb >= d.nblocks) return;
     const unsigned i = (unsigned)lb * WG + threadIdx.x;
@@ -118,7 +120,21 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         const size_t src = bb ? (size_t)opp(q) * N + i : (size_t)q * N + (sx + nx * (sy + ny * sz));
         g[q] = Pop<S, T>::ld(fin + src, bb ? opp(q) : q);
     }
-    if (GEOM == GEOM_TUBE && (cut & ~1u)) {
+    // sliding walls: Ladd's term on the links bounced at a wall moving along
+    // z, 6 w_q rho Cr (c_q . u_w) with the rho and u_w of the step that
+    // collided this node (lbm.cu:951-970), kept per wall-row node in mw
+    const size_t wcol = (size_t)ix + (size_t)nx * iz;
+    if (MOVING && wall_lo) {
+        const T m = mw[wcol];
+        g[11] += wq<T>(11) * m;  // c = (0, 1, 1)
+        g[14] -= wq<T>(14) * m;  // c = (0, 1, -1)
+    }
+    if (MOVING && wall_hi) {
+        const T m = mw[(size_t)nx * nz + wcol];
+        g[12] -= wq<T>(12) * m;  // c = (0, -1, -1)
+        g[13] += wq<T>(13) * m;  // c = (0, -1, 1)
+    }
+    if (GEOM == GEOM_TUBE && tube_r && (cut & ~1u)) {
         // g[q] of a cut link holds the half-way value (f1); f2 is the value
         // pulled along the opposite direction, or the node's own f_q where
         // that source is a wall too; f3 is the node's own f_q.  Only values
@@ -155,6 +171,11 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
     }
     const T rho = T(1) + drho;
     const T amp = (T)st->amp;
+    if (MOVING) {
+        const T uw = (T)st->wall;
+        if (wall_lo) mw[wcol] = mw_lo * rho * uw;
+        if (wall_hi) mw[(size_t)nx * nz + wcol] = mw_hi * rho * uw;
+    }
     T ax = pex * amp, ay = pey * amp, az = pez * amp;
     if (HASF) {
         ax += B * F[i];
@@ -372,12 +393,14 @@ template <typename S, typename T> lbmibm_status step_impl(lbmibm_ctx *ctx) {
                            (const T *)ctx->Hb, (T *)ctx->U, (T *)ctx->Ui, (T *)ctx->rho, (T *)ctx->rhom, (T)om, (T)(1.0 / p.cr),
                            (T)(p.p * p.ex), (T)(p.p * p.ey), (T)(p.p * p.ez), (T)p.b, ctx->step, p.m,
                            (const double *)(p.vof_period > 0 ? ctx->vof[ctx->vof_cur] : nullptr), (T)(p.lambda1 - 1.0),
-                           (T)p.df, (const unsigned *)ctx->tube_col, (const double *)ctx->tube_r);
+                           (T)p.df, (const unsigned *)ctx->tube_col, (const double *)ctx->tube_r, (T *)ctx->mw,
+                           (T)(p.wall_motion == 1 ? -6.0 * p.cr : 0.0), (T)(6.0 * p.cr));
     };
-    const int geom = p.tube_radius > 0.0 ? GEOM_TUBE : (walls ? GEOM_SLAB : GEOM_PERIODIC);
+    const int geom = p.tube_radius > 0.0 ? GEOM_TUBE : (walls ? (p.wall_motion ? GEOM_SHEAR : GEOM_SLAB) : GEOM_PERIODIC);
 #define LBM_CASE(HF, MC, G) \
     if (hasf == HF && macros == MC && geom == G) launch(k_lbm<S, T, HF, MC, G>);
-#define LBM_GEOMS(HF, MC) LBM_CASE(HF, MC, GEOM_PERIODIC) LBM_CASE(HF, MC, GEOM_SLAB) LBM_CASE(HF, MC, GEOM_TUBE)
+#define LBM_GEOMS(HF, MC) LBM_CASE(HF, MC, GEOM_PERIODIC) LBM_CASE(HF, MC, GEOM_SLAB) LBM_CASE(HF, MC, GEOM_TUBE) \
+    LBM_CASE(HF, MC, GEOM_SHEAR)
     LBM_GEOMS(true, true) LBM_GEOMS(true, false) LBM_GEOMS(false, true) LBM_GEOMS(false, false)
 #undef LBM_GEOMS
 #undef LBM_CASE
@@ -457,6 +480,8 @@ lbmibm_status lattice_set_state_from_fstar(lbmibm_ctx *ctx, const double *host_f
             case LBMIBM_F16: s = from_fstar_impl<__half, float>(ctx, dev); break;
         }
     }
+    // the uploaded state carries its wall terms already: the next pull adds none
+    if (s == LBMIBM_OK && ctx->mw) s = lattice_fill(ctx, ctx->mw, 2 * (size_t)ctx->nx * ctx->nz, 0.0, false);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess && s == LBMIBM_OK) s = LBMIBM_ERR_HIP;
     (void)hipFree(dev);
     if (s != LBMIBM_OK && ctx->err.empty()) ctx->err = "population upload failed";

@@ -47,7 +47,8 @@ class Params(C.Structure):
         ("itrmax", C.c_int), ("residual", C.c_double), ("runge_kutta", C.c_int), ("neohookean", C.c_int),
         ("volume_correction", C.c_int), ("precision", C.c_int), ("store_macros", C.c_int),
         ("fileoutput", C.c_int), ("vof_period", C.c_int), ("sor_itrmax", C.c_int), ("sor_residual", C.c_double),
-        ("tube_radius", C.c_double),
+        ("tube_radius", C.c_double), ("tube_staircase", C.c_int), ("wall_motion", C.c_int),
+        ("wall_velocity", C.c_double),
     ]
 
 

@@ -25,7 +25,8 @@ def make_params(lx, ly, lz, *, precision="f32", tau=None, df=None, dx=1.0, cr=1.
                 e=(1.0, 0.0, 0.0), force_modulated=False, flowstart=0, freq=1.0, b=0.0, m=1, c=100.0,
                 rgs=1.0, br=0.0, itrmax=1000, residual=1e-10, runge_kutta=4, neohookean=False,
                 volume_correction=True, store_macros=False, fileoutput=0, periodic_y=False, lambda1=1.0,
-                vof_period=0, sor_itrmax=None, sor_residual=None, tube_radius=0.0):
+                vof_period=0, sor_itrmax=None, sor_residual=None, tube_radius=0.0, tube_staircase=False,
+                wall_motion=0, wall_velocity=0.0):
     P = N.Params()
     N.lib().lbmibm_default_params(C.byref(P), int(lx), int(ly), int(lz))
     if df is None:
@@ -45,6 +46,8 @@ def make_params(lx, ly, lz, *, precision="f32", tau=None, df=None, dx=1.0, cr=1.
     P.sor_itrmax = int(itrmax if sor_itrmax is None else sor_itrmax)
     P.sor_residual = float(residual if sor_residual is None else sor_residual)
     P.tube_radius = float(tube_radius)
+    P.tube_staircase = int(tube_staircase)
+    P.wall_motion, P.wall_velocity = int(wall_motion), float(wall_velocity)
     return P
 
 

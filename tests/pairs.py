@@ -28,7 +28,9 @@ def oracle_of(lx, ly, lz, cells=None, **kw):
                   flowstart=p.get("flowstart", 0), freq=p.get("freq", 1.0),
                   fileoutput=p.get("fileoutput", 0) or 1000, cells=cells, lambda1=p.get("lambda1", 1.0),
                   vof_period=p.get("vof_period", 0), sor_itrmax=p.get("sor_itrmax"),
-                  sor_residual=p.get("sor_residual"), tube_radius=p.get("tube_radius", 0.0))
+                  sor_residual=p.get("sor_residual"), tube_radius=p.get("tube_radius", 0.0),
+                  tube_staircase=p.get("tube_staircase", False), wall_motion=p.get("wall_motion", 0),
+                  wall_velocity=p.get("wall_velocity", 0.0))
     o.init_fluid()
     return o
 
@@ -61,6 +63,18 @@ def rel(a, b):
     a, b = np.asarray(a), np.asarray(b)
     den = np.max(np.abs(b))
     return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+# D3Q19 weights (lbm.cu:219-237): populations are compared as deviations
+# f - w_q rho0 (rho0 = 1), the part that carries the flow, so that the rest
+# weight of max|f| ~ 1/3 does not dilute the error measure
+WQ = 1.0 / np.array([3.0] + [18.0] * 6 + [36.0] * 12)
+
+
+def pop_err(fa, fb):
+    """Max error of the population deviations of fa against fb ([n, 19]
+    arrays of the same nodes), relative to max |fb - w_q|."""
+    return rel(np.asarray(fa).reshape(-1, Q) - WQ, np.asarray(fb).reshape(-1, Q) - WQ)
 
 
 def small_cells(lx, ly, lz, nc=2, radius=3.0, levels=None, seed=11):

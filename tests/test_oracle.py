@@ -639,3 +639,127 @@ def test_skalak_shear_term_alone_is_scaled_by_rgs():
     coef = rgs * (2 * lam ** 2 - 1) / lam ** 2 + C_ * (lam ** 4 - 1) - 1
     ref = coef * o.cells["nref"]
     assert np.max(np.abs(o.cells["t"] - ref)) < 1e-10 * np.abs(ref).max()
+
+
+# ---- sliding walls (CapsuleSHEARFLOW / SHEARFLOW) -----------------------------------
+
+def _couette_series(eta, t, U, nu, H, top_only, nmax=2001):
+    """Start-up Couette flow between walls at eta = 0 and H, from rest, by the
+    sine series of u_t = nu u_yy: allwall (u = -U at 0, +U at H) or topwall
+    (0 at 0, U at H)."""
+    n = np.arange(1, nmax)[:, None]
+    dec = np.exp(-nu * (n * np.pi / H) ** 2 * t) * np.sin(n * np.pi * eta[None, :] / H)
+    if top_only:
+        steady = U * eta / H
+        b = 2 * U * (-1.0) ** n / (n * np.pi)
+    else:
+        steady = -U + 2 * U * eta / H
+        b = np.where(n % 2 == 0, 4 * U / (n * np.pi), 0.0)
+    return steady + (b * dec).sum(0)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_couette_start_up_follows_the_series(mode):
+    """Walls at y = +-LY/2 sliding along z (Ladd's term, lbm.cu:951-970), from
+    rest: u_z(y) after 600 steps against the series, to 1% of U.  A wrong sign
+    or a missing Cr in the wall term reverses or rescales the profile."""
+    H, tau, cr, U = 24, 0.8, 0.5, 0.04
+    o = OracleSim(4, H, 4, tau=tau, cr=cr, wall_motion=mode, wall_velocity=U)
+    o.init_fluid()
+    steps = 600
+    for it in range(1, steps + 1):
+        o.lbm_d3q19(it)
+    uz = comp_view(o, o.wn)
+    assert np.ptp(uz, axis=(0, 2)).max() < 1e-16
+    assert np.max(np.abs(comp_view(o, o.un))) < 1e-16
+    nu = (tau - 0.5) / 3
+    # lattice time: nu in lattice units, steps in lattice steps
+    ua = _couette_series(np.arange(H) + 0.5, steps, U, nu, H, top_only=(mode == 2))
+    assert np.max(np.abs(uz[0, :, 0] - ua)) < 1e-2 * U, np.max(np.abs(uz[0, :, 0] - ua)) / U
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_couette_steady_profile_is_linear(mode):
+    """Half-way bounce-back with Ladd's term gives the exact linear profile of
+    plane Couette flow: from the sheared start of flu_init (init1.cu:367-384,
+    u_z = sr y for allwall, sr (y + LY/2) for topwall), after long relaxation
+    u_z is linear to round-off and meets +-U at the half-way wall planes."""
+    H, tau, cr, sr = 12, 0.9, 0.5, 0.004
+    U = sr * H / 2 if mode == 1 else sr * H
+    o = OracleSim(4, H, 4, tau=tau, cr=cr, wall_motion=mode, wall_velocity=U)
+    _, Y, _ = o.node_coords()
+    o.init_fluid(None, None, None, sr * Y if mode == 1 else sr * (Y + H / 2))
+    for it in range(1, 3001):
+        o.lbm_d3q19(it)
+    y = np.arange(H) + 0.5 - H / 2
+    ref = sr * y if mode == 1 else sr * (y + H / 2)
+    uz = comp_view(o, o.wn)[0, :, 0]
+    assert np.max(np.abs(uz - ref)) < 1e-12 * U
+
+
+def test_walls_rest_before_flowstart():
+    """sra = 0 while iter < FLOWSTART (lbm.cu:755): the fluid stays at rest
+    through step FLOWSTART - 1, so after step n the state equals that of the
+    run with FLOWSTART = 0 after step n - FLOWSTART + 1."""
+    kw = dict(tau=0.8, cr=0.5, wall_motion=1, wall_velocity=0.03)
+    a = OracleSim(4, 10, 4, flowstart=5, **kw)
+    b = OracleSim(4, 10, 4, flowstart=0, **kw)
+    a.init_fluid()
+    b.init_fluid()
+    for it in range(1, 5):
+        a.lbm_d3q19(it)
+    m = a.comp_mask()
+    assert np.max(np.abs(comp_view(a, a.wn))) == 0.0
+    for it in range(5, 15):
+        a.lbm_d3q19(it)
+    for it in range(1, 11):
+        b.lbm_d3q19(it)
+    assert np.max(np.abs(a.fn.reshape(-1, Q)[m] - b.fn.reshape(-1, Q)[m])) < 1e-15
+
+
+def test_sliding_walls_conserve_mass_and_shear_stress_balances():
+    """Ladd's term moves momentum, not mass: sum f is conserved.  In the
+    steady state the z-momentum the top wall puts in per step equals what the
+    bottom wall takes out, so total z-momentum stops changing."""
+    H = 10
+    o = OracleSim(4, H, 4, tau=0.8, cr=0.5, wall_motion=1, wall_velocity=0.02)
+    _, Y, _ = o.node_coords()
+    o.init_fluid(None, None, None, 0.02 * Y / (H / 2))
+    m = o.comp_mask()
+    M0 = o.fn.reshape(-1, Q)[m].sum()
+    for it in range(1, 2001):
+        o.lbm_d3q19(it)
+    f = o.fn.reshape(-1, Q)[m]
+    assert abs(f.sum() - M0) < 1e-12 * M0
+    Jz = (f @ CQ)[:, 2].sum()
+    o.lbm_d3q19(2001)
+    Jz2 = (o.fn.reshape(-1, Q)[m] @ CQ)[:, 2].sum()
+    assert abs(Jz2 - Jz) < 1e-13 * M0
+
+
+def test_tube_staircase_is_half_way_bounce_back_on_the_mask():
+    """lbm_CapsuleCHANNELFLOW's tube (lbm.cu:1050-1094): every link into a
+    WALL node of the disc mask is bounced half-way, with no interpolation:
+    mass is conserved exactly, the flow keeps the square lattice's symmetry
+    x <-> y, and Poiseuille flow at R = 7.5 comes within 7% of the circular
+    profile's centreline, and further from it than the interpolated form
+    (the staircase wall sits inside r = R: 0.947 against 0.987 here)."""
+    R, L, tau, a = 7.5, 18, 0.8, 1e-6
+    o = OracleSim(L, L, 4, tau=tau, p=a / 2, e=(0, 0, 1), tube_radius=R, tube_staircase=True)
+    o.init_fluid()
+    m = o.comp_mask()
+    M0 = o.fn.reshape(-1, Q)[m].sum()
+    for it in range(1, 3001):
+        o.lbm_d3q19(it)
+    assert abs(o.fn.reshape(-1, Q)[m].sum() - M0) < 1e-12 * M0
+    w = o.wn.reshape(o.nz, o.ny, o.nx)[2]
+    assert np.max(np.abs(w - w.T)) < 1e-15
+    nu = (tau - 0.5) / 3
+    wmax = a * R * R / (4 * nu)
+    assert 0.93 < w.max() / wmax < 0.97
+    o2 = OracleSim(L, L, 4, tau=tau, p=a / 2, e=(0, 0, 1), tube_radius=R)
+    o2.init_fluid()
+    for it in range(1, 3001):
+        o2.lbm_d3q19(it)
+    w2 = o2.wn.reshape(o2.nz, o2.ny, o2.nx)[2]
+    assert abs(w2.max() / wmax - 1) < abs(w.max() / wmax - 1) / 3
