@@ -71,15 +71,15 @@ def adjoint_workload(nc=32, lx=128, ly=64, lz=64, radius=4.0, seed=2025):
     return (lx, ly, lz), cells, params, desc, ht
 
 
-def adjoint_bench(a):
+def adjoint_measure(precision, nsteps):
     """Cost of one checkpointed gradient of J relative to the forward run, and
-    its agreement with a central finite difference along a random direction."""
+    its agreement with a central finite difference along a random direction
+    of the body-force field (10% of P)."""
     from preprocess_src_amd.simulation import Simulation
     shape, cells, params, desc, ht = adjoint_workload()
     lx, ly, lz = shape
-    nsteps = a.adj_steps
     window, K = (2 * nsteps) // 5, min(50, nsteps)
-    sim = Simulation(lx, ly, lz, cells=cells, precision=a.precision, **params)
+    sim = Simulation(lx, ly, lz, cells=cells, precision=precision, **params)
     x0 = np.concatenate([cells["x"], cells["y"], cells["z"]])
 
     def reset():
@@ -100,7 +100,6 @@ def adjoint_bench(a):
     J2, gH, gs = sim.adjoint_wall_shear(0, nsteps, window, K)
     t_adj = time.perf_counter() - t0
     print(f"[bench] forward + adjoint: {t_adj:.3f} s", file=sys.stderr, flush=True)
-    # central difference along a random body-force direction (relative size 10% of P)
     rng = np.random.default_rng(7)
     m = sim.comp_mask()
     d = [0.1 * params["p"] * rng.standard_normal(sim.np_) * m for _ in range(3)]
@@ -113,30 +112,33 @@ def adjoint_bench(a):
     dd = (fd[0] - fd[1]) / 2.0
     ad = float(sum(g @ v for g, v in zip(gH, d)))
     n = lx * ly * lz
+    out = dict(workload=desc, precision=precision, steps=nsteps, window=window, checkpoint_every=K,
+               cells=sim.nc, vertices=sim.nv, hematocrit=round(ht, 4),
+               adjoint_over_forward=round((t_adj - t_fwd) / t_fwd, 3),
+               forward_s=round(t_fwd, 4), forward_plus_adjoint_s=round(t_adj, 4),
+               forward_ms_per_step=round(1e3 * t_fwd / nsteps, 4),
+               forward_mlups=round(n * nsteps / t_fwd / 1e6, 1), J=J, J_in_adjoint_call=J2,
+               fd_check={"directional_adjoint": ad, "central_difference": dd,
+                         "rel_err": abs(ad - dd) / max(abs(dd), 1e-300)},
+               grad_force_scale_range=[float(np.min(gs)), float(np.max(gs))])
+    sim.close()
+    return out
+
+
+def adjoint_bench(a):
+    r = adjoint_measure(a.precision, a.adj_steps)
     out = {
         "metric": "adjoint/forward cost (wall-shear gradient, checkpointed)",
-        "value": round((t_adj - t_fwd) / t_fwd, 3),
-        "unit": "x forward",
-        "n_gpus": 1,
-        "steps": nsteps,
-        "warmup": 20,
-        "ms_per_step": round(1e3 * t_fwd / nsteps, 4),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": a.precision,
-        "data": "synthetic (seeded cell placement, fluid at rest start)",
-        "config": {"workload": desc, "lattice": [lx, ly, lz], "cells": sim.nc, "vertices": sim.nv,
-                   "window": window, "checkpoint_every": K, "hematocrit": round(ht, 4)},
-        "forward_s": round(t_fwd, 4), "forward_plus_adjoint_s": round(t_adj, 4),
-        "forward_mlups": round(n * nsteps / t_fwd / 1e6, 1),
-        "J": J, "J_in_adjoint_call": J2,
-        "fd_check": {"directional_adjoint": ad, "central_difference": dd,
-                     "rel_err": abs(ad - dd) / max(abs(dd), 1e-300)},
-        "grad_force_scale_range": [float(np.min(gs)), float(np.max(gs))],
+        "value": r["adjoint_over_forward"], "unit": "x forward", "n_gpus": 1, "steps": r["steps"], "warmup": 20,
+        "ms_per_step": r["forward_ms_per_step"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": a.precision, "data": "synthetic (seeded cell placement, fluid at rest start)",
+        "config": {"workload": r["workload"], "lattice": [128, 64, 64], "cells": r["cells"],
+                   "vertices": r["vertices"], "window": r["window"], "checkpoint_every": r["checkpoint_every"],
+                   "hematocrit": r["hematocrit"]},
+        **{k: r[k] for k in ("forward_s", "forward_plus_adjoint_s", "forward_mlups", "J", "J_in_adjoint_call",
+                             "fd_check", "grad_force_scale_range")},
     }
     print(json.dumps(out), flush=True)
-    sim.close()
 
 
 def tgv_workload(n=256):
@@ -204,25 +206,91 @@ def lattice_bytes_per_node(precision, coupled):
     return 2 * Q * s + (3 * t * 2 if coupled else 3 * t)  # populations + (F read, u write) or u write
 
 
-def lbm_traffic_from_profile(precision):
-    """HBM bytes per k_lbm launch measured by rocprofv3 FETCH_SIZE (x2.0, the
-    copy-kernel calibration) and WRITE_SIZE on this workload, from the summary
-    committed in profiles/ (FP32 only; None when absent)."""
+def step_bytes(n, nv, nt, precision, coupled):
+    """Algorithmic HBM bytes of one whole step: populations read and written,
+    the lattice force written by the spreading and read by the collision, the
+    velocity written (12 B/node each in FP32), plus the membrane's 64 B per
+    vertex and 28 B per triangle."""
+    return n * (lattice_bytes_per_node(precision, coupled) + (3 * ARITH_BYTES[precision] if coupled else 0)) + \
+        (64 * nv + 28 * nt if coupled else 0)
+
+
+def traffic_from_profile(precision):
+    """HBM bytes per launch of each kernel measured by rocprofv3 FETCH_SIZE
+    (x2.0, the copy-kernel calibration) and WRITE_SIZE on this workload, from
+    the summary committed in profiles/ (FP32 only; {} when absent)."""
     path = os.path.join(ROOT, "profiles", "suspension_256x128x128_f32_hbm_traffic.txt")
+    out = {}
     if precision != "f32" or not os.path.exists(path):
-        return None
+        return out
+    table = False
     for line in open(path):
-        if line.startswith("k_lbm"):
+        if line.startswith("kernel "):
+            table = True
+            continue
+        if table and line.strip() and not line.startswith(" "):
             parts = line.split()
-            return round((float(parts[-3]) + float(parts[-1])) * 1e6)  # read MB (x2) + write MB
-    return None
+            if len(parts) >= 6 and parts[-1].replace(".", "").isdigit():
+                name = " ".join(parts[:-5])
+                out[name] = round((float(parts[-3]) + float(parts[-1])) * 1e6)
+        elif table and not line.strip():
+            table = False
+    return out
 
 
-def cpu_baseline(shape, cells, params, seconds):
+def kernel_table(kt, n, nv, nt, precision, prof):
+    """Per-kernel entries: launches per step and average duration (HIP events
+    on the launch's stream, eager steps), algorithmic bytes per launch where
+    the kernel has its own share of the step's dataflow, achieved bandwidth
+    and fraction of the HBM peak, and the rocprof-measured bytes per launch."""
+    t = ARITH_BYTES[precision]
+    algo = {
+        "k_lbm": n * lattice_bytes_per_node(precision, True),
+        # spreading: positions (24 B) and forces (12 B) of each vertex in, F out
+        "k_spread_box": 36 * nv, "k_spread_gather": 3 * t * n, "k_spread_tiles": 36 * nv + 3 * t * n,
+        # interpolation + advection: positions in (24 B), SoA and AoS positions
+        # out (48 B), the velocity field in
+        "k_interp_tiles": 72 * nv + 3 * t * n, "k_interp_vert": 72 * nv + 3 * t * n,
+    }
+    prof_name = {"k_lbm": "k_lbm<float, float, true, false, 1>", "k_spread_box": "k_spread_box<float>",
+                 "k_spread_gather": "k_spread_gather<float>", "k_interp_tiles": "k_interp_tiles<float>",
+                 "k_bicg_res": "k_bicg_res<11>", "k_elem": "k_elem<float>", "k_vert": "k_vert<float>"}
+    rows = []
+    for name, (per_step, ms) in sorted(kt.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        b = algo.get(name)
+        row = {"name": name, "launches_per_step": round(per_step, 3), "avg_ms": round(ms, 5),
+               "algorithmic_bytes": b}
+        if b:
+            row["achieved_GBps"] = round(b / (ms * 1e-3) / 1e9, 1)
+            row["frac"] = round(b / (ms * 1e-3) / HBM_PEAK, 4)
+        pb = prof.get(prof_name.get(name, ""))
+        if pb:
+            row["rocprof_bytes"] = pb
+        rows.append(row)
+    return rows
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return dict(cpu_model=model, nproc=avail, omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
+
+
+def cpu_baseline(shape, cells, params, seconds, nv=0, nt=0):
     """FP64 host oracle on the same workload for a bounded number of steps."""
     from oracle import pyoracle
     lx, ly, lz = shape
-    kw = {k: v for k, v in params.items() if k not in ("store_macros",)}
+    kw = {k: v for k, v in params.items() if k not in ("store_macros", "bicg_warm_start", "residual_floor")}
     t0 = time.time()
     o = pyoracle.OracleSim(lx, ly, lz, cells=cells, **kw)
     o.init_fluid()
@@ -238,10 +306,13 @@ def cpu_baseline(shape, cells, params, seconds):
         if rc != 0:
             raise RuntimeError(f"oracle step failed ({rc})")
     n = lx * ly * lz
+    b = step_bytes(n, nv, nt, "f64", cells is not None) * steps / t_run
     return dict(value=n * steps / t_run / 1e6, unit="MLUPS", cores=int(pyoracle.lib().orc_num_threads()),
-                kind="port",
+                kind="port", **cpu_info(), achieved_GBps=round(b / 1e9, 3),
+                roofline_frac=b / HBM_PEAK,
                 sample=f"{steps} full step(s) of the same workload in the FP64 OpenMP oracle "
-                       f"({t_run:.1f} s timed, {setup:.1f} s set-up excluded)")
+                       f"({t_run:.1f} s timed, {setup:.1f} s set-up excluded); GB/s from the FP64 step's "
+                       f"algorithmic bytes, fraction of the GPU's 8 TB/s HBM peak")
 
 
 def copy_calibration(nbytes=256 << 20, reps=3):
@@ -393,6 +464,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cold-start", action="store_true",
+                    help="BiCGSTAB from x0 = 0 every step, as the reference (default: from the previous loads)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the FP64 run and the adjoint measurement of the default workload")
     ap.add_argument("--adj-steps", type=int, default=500, help="steps of the adjoint workload (window 40%%)")
     ap.add_argument("--vof-period", type=int, default=1000, help="K of the vof workload")
     ap.add_argument("--sor-itrmax", type=int, default=200000)
@@ -423,49 +498,55 @@ def main():
         return
     if a.workload == "suspension":
         shape, cells, params, desc, ht = suspension_workload()
+        params["bicg_warm_start"] = not a.cold_start
     else:
         shape, cells, params, desc, ht = tgv_workload()
     lx, ly, lz = shape
     n = lx * ly * lz
-    t_setup = time.time()
-    sim = Simulation(lx, ly, lz, cells=cells, precision=a.precision, **params)
-    if a.workload == "tgv":
-        sim.set_equilibrium(*tgv_init(shape))
-    if cells is not None:
-        sim.volume_init()
-    print(f"[bench] set-up {time.time() - t_setup:.1f} s: {desc}, precision {a.precision}", file=sys.stderr,
-          flush=True)
-
     graph = not a.no_graph
-    sim.run(0, a.warmup, graph=graph)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    sim.run(a.warmup, a.steps, graph=graph)
-    sim.synchronize()
-    t1 = time.perf_counter()
-    dt = t1 - t0
+
+    def timed(precision):
+        t_setup = time.time()
+        sim = Simulation(lx, ly, lz, cells=cells, precision=precision, **params)
+        if a.workload == "tgv":
+            sim.set_equilibrium(*tgv_init(shape))
+        if cells is not None:
+            sim.volume_init()
+        print(f"[bench] set-up {time.time() - t_setup:.1f} s: {desc}, precision {precision}", file=sys.stderr,
+              flush=True)
+        sim.run(0, a.warmup, graph=graph)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.run(a.warmup, a.steps, graph=graph)
+        sim.synchronize()
+        dt = time.perf_counter() - t0
+        print(f"[bench] {precision}: timed {a.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
+        return sim, dt
+
+    sim, dt = timed(a.precision)
     ms_per_step = 1e3 * dt / a.steps
     mlups = n * a.steps / dt / 1e6
-    print(f"[bench] timed {a.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
-
-    # per-stage HIP-event timing on the library stream, right after the timed region
-    stages = sim.time_stages(a.warmup + a.steps, a.stage_steps)
+    coupled = cells is not None
+    nv, nt = sim.nv, sim.nt
+    sbytes = step_bytes(n, nv, nt, a.precision, coupled)
+    # per-stage and per-kernel HIP-event timing on the library's streams,
+    # eager steps right after the timed region
+    it0 = a.warmup + a.steps
+    stages = sim.time_stages(it0, a.stage_steps)
+    kt = sim.time_kernels(it0 + a.stage_steps, a.stage_steps)
     diag = {}
-    if cells is not None:
+    if coupled:
         its = sim.membrane()["bicg_iters"]
         nvc = np.diff(np.asarray(cells["voff"]))
         diag["bicg_iters_mean_by_vertices"] = {int(k): round(float(its[nvc == k].mean()), 2) for k in np.unique(nvc)}
         diag["autotune"] = [l for l in sim.autotune_report().splitlines() if l]
-    t_lbm = stages["lbm_d3q19"][0] * 1e-3
-    coupled = cells is not None
+    prof = traffic_from_profile(a.precision) if a.workload == "suspension" else {}
+    kernels = kernel_table(kt, n, nv, nt, a.precision, prof)
+    dominant = kernels[0]["name"] if kernels else None
+    lbm = kt.get("k_lbm", (1, stages["lbm_d3q19"][0]))[1]
     bpn = lattice_bytes_per_node(a.precision, coupled)
-    lbm_bytes = n * bpn
-    achieved = lbm_bytes / t_lbm
-    nv = sim.nv
-    nt = sim.nt
-    step_bytes = n * (bpn + (12 if coupled else 0)) + (64 * nv + 28 * nt)
     out = {
-        "metric": "MLUPS (coupled D3Q19+IBM+Skalak step)",
+        "metric": "MLUPS + % HBM roofline (D3Q19+IBM step, FP32 vs FP64); adjoint/forward cost",
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -480,23 +561,44 @@ def main():
         "config": {"workload": desc, "lattice": [lx, ly, lz], "cells": sim.nc, "vertices": nv, "triangles": nt,
                    "hematocrit": round(ht, 4), "precision": a.precision, "graph": graph,
                    "params": {k: (list(v) if isinstance(v, tuple) else v) for k, v in params.items()}},
-        "roofline": {"bound": "hbm", "kernel": "k_lbm (fused pull stream-collide)",
-                     "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": lbm_traffic_from_profile(a.precision) if a.workload == "suspension" else None,
-                     "bytes_per_node": bpn, "avg_ms": round(stages["lbm_d3q19"][0], 5)},
-        "step_roofline": {"algorithmic_bytes": step_bytes, "frac": round(step_bytes / (ms_per_step * 1e-3) / HBM_PEAK, 4)},
+        "roofline": {"bound": "hbm", "scope": "whole step (D3Q19 + IBM + membrane)",
+                     "achieved": round(sbytes / (ms_per_step * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(sbytes / (ms_per_step * 1e-3) / HBM_PEAK, 4),
+                     "traffic": None, "algorithmic_bytes_per_step": sbytes,
+                     "bytes_per_node": bpn + (3 * ARITH_BYTES[a.precision] if coupled else 0),
+                     "membrane_bytes": (64 * nv + 28 * nt) if coupled else 0},
+        "lattice_kernel_roofline": {"kernel": "k_lbm", "achieved": round(n * bpn / (lbm * 1e-3) / 1e9, 1),
+                                    "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                    "frac": round(n * bpn / (lbm * 1e-3) / HBM_PEAK, 4), "avg_ms": round(lbm, 5),
+                                    "bytes_per_node": bpn,
+                                    "traffic": prof.get("k_lbm<float, float, true, false, 1>")},
+        "dominant_kernel": dominant,
+        "kernels": kernels,
         "stages_ms": {k: round(v[0], 5) for k, v in stages.items()},
         "diagnostics": diag,
     }
+    sim.close()
     if a.workload == "tgv":
         print("[bench] per-field errors against FP64", file=sys.stderr, flush=True)
-        out["field_errors"] = tgv_field_errors(sim, shape, params, a.warmup + a.steps + a.stage_steps, a.precision)
+        s2, _ = timed(a.precision)
+        out["field_errors"] = tgv_field_errors(s2, shape, params, a.warmup + a.steps, a.precision)
+        s2.close()
+    if a.workload == "suspension" and not a.no_extra:
+        if a.precision != "f64":
+            s64, dt64 = timed("f64")
+            ms64 = 1e3 * dt64 / a.steps
+            b64 = step_bytes(n, nv, nt, "f64", coupled)
+            out["fp64"] = {"mlups": round(n * a.steps / dt64 / 1e6, 2), "ms_per_step": round(ms64, 4),
+                           "roofline_frac": round(b64 / (ms64 * 1e-3) / HBM_PEAK, 4),
+                           "algorithmic_bytes_per_step": b64, "fp32_over_fp64": round(ms64 / ms_per_step, 3)}
+            s64.close()
+        print("[bench] adjoint (128x64x64, 32 RBCs)", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_measure(a.precision, a.adj_steps)
     if not a.no_cpu:
         print("[bench] cpu baseline (FP64 oracle)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(shape, cells, params, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(shape, cells, params, a.cpu_seconds, nv if coupled else 0,
+                                           nt if coupled else 0)
     print(json.dumps(out), flush=True)
-    sim.close()
 
 
 if __name__ == "__main__":

@@ -112,6 +112,15 @@ typedef struct lbmibm_params {
                                 iter < FLOWSTART (sra = 0, lbm.cu:755).  Needs the slab     */
     double wall_velocity;    /* U in units of Uc (lattice speed Cr U): sr LY/2 for allwall,
                                 sr LY for topwall, ub = 1 in lbm_SHEARFLOW                  */
+    int    bicg_warm_start;  /* 1: each stress() BiCGSTAB starts from the cell's previous load
+                                q (x0 = q, r^ = r0 = b - A q; zero before the first solve);
+                                0: from x0 = 0 as stress.cu:139 (r^ = b).  Both stop at the
+                                same relative residual |r|/|b|, so their loads agree to it.
+                                The adjoint's solves always start from 0                     */
+    double residual_floor;   /* with FP32 solver arithmetic (FP32, FP16 storage) the stop test
+                                uses max(RESIDUAL, residual_floor): below about 1e-7 an FP32
+                                iterate no longer changes, only the recursively updated
+                                residual keeps falling.  0 keeps RESIDUAL.  FP64 ignores it */
 } lbmibm_params;
 
 /* Initial cell set: reference SoA positions and element vertex ids. */
@@ -128,7 +137,8 @@ typedef struct lbmibm_ctx lbmibm_ctx;
 /* Defaults for an lx*ly*lz slab channel: dx=dt=Cr=1, Df=0.1 (tau=0.8),
  * LAMBDA1=1, no body force, B=0, M=1, C=100, rg=1, BR=0, ITRMAX=1000,
  * RESIDUAL=1e-10, RK4, Skalak, volume correction on, FP32, store_macros=0,
- * no VOF indicator (SOR ITRMAX=1000, RESIDUAL=1e-10). */
+ * no VOF indicator (SOR ITRMAX=1000, RESIDUAL=1e-10), BiCGSTAB from x0 = 0,
+ * FP32 residual floor 1e-7. */
 void          lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz);
 
 /* cdo_init + flu_init + cel_init + fem_init + mem_init (init1.cu:13-1092,
@@ -177,6 +187,13 @@ lbmibm_status lbmibm_synchronize(lbmibm_ctx *ctx);
  * stage_calls[6], in the order stress, ibm_st1, lbm_d3q19, ibm_st2,
  * volume_correction, whole step. */
 lbmibm_status lbmibm_time_stages(lbmibm_ctx *ctx, int iter0, int nsteps, double *stage_ms, int *stage_calls);
+/* Per-kernel timer (no reference counterpart): runs steps iter0+1 ..
+ * iter0+nsteps eagerly, as lbmibm_step does, with a HIP event pair on its own
+ * stream around each launch of the path's main kernels (k_lbm, binning,
+ * k_spread_*, k_interp_*, k_elem, k_vert, the BiCGSTAB classes and forms,
+ * k_volume).  report receives one line per kernel, "name\tlaunches\tavg_ms",
+ * truncated to report_len bytes. */
+lbmibm_status lbmibm_time_kernels(lbmibm_ctx *ctx, int iter0, int nsteps, char *report, int report_len);
 
 /* ---- data exchange in the reference layouts --------------------------- */
 /* flu_init: f = f_eq(rho, u) on every node; rho,u,v,w padded [n] or NULL

@@ -36,6 +36,8 @@ struct AdhDev {
     double lcont, f_wlc, ra, alpha1, alpha2;
     int ring;                                     // the tube's receptor ring
     double radius;
+    const double *ring_xy;                        // [nxw][2]: R cos, R sin of each column (host libm)
+    double reach;                                 // angular half-width of the columns within LIMIT dx
 };
 
 // Extension measure and force of the Bell rates: LINEAR_SPRING
@@ -72,8 +74,18 @@ __device__ __forceinline__ unsigned long long bid_key(unsigned long long call, i
     return (stamp << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)sid);
 }
 
-__global__ void k_adh_tick(unsigned long long *__restrict__ calls) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) calls[0] = calls[0] + 1ull;
+// Advances the call counter. Stamps must stay below 2^32 (the key keeps 32
+// bits of stamp): when the next call's largest stamp would not, every claim
+// is cleared and the counter restarts at 0, whose stamps outrank a cleared
+// claim. With MAXN = 100 that happens once in about 42 million calls.
+__global__ void __launch_bounds__(256) k_adh_tick(unsigned long long *__restrict__ calls,
+                                                  unsigned long long *__restrict__ claim, long long nrec, int maxn) {
+    const unsigned long long next = calls[0] + 1ull;
+    const bool wrap = (next + 1ull) * (unsigned long long)(maxn + 2) > 0xFFFFFFFFull;
+    if (wrap)
+        for (long long i = threadIdx.x; i < nrec; i += blockDim.x) claim[i] = 0ull;
+    __syncthreads();
+    if (threadIdx.x == 0) calls[0] = wrap ? 0ull : next;
 }
 
 // Receptor id = iz * nxw + ix on the slab's wall y = -LY/2 (adhesion.cu:
@@ -81,8 +93,8 @@ __global__ void k_adh_tick(unsigned long long *__restrict__ calls) {
 __device__ __forceinline__ void receptor(const AdhDev &a, int id, double &x, double &y, double &z) {
     const int iz = id / a.nxw, ix = id - iz * a.nxw;
     if (a.ring) {
-        x = a.radius * cos(ix * a.dxw);
-        y = a.radius * sin(ix * a.dxw);
+        x = a.ring_xy[2 * ix];
+        y = a.ring_xy[2 * ix + 1];
     } else {
         x = (ix - a.nxw / 2) * a.dxw;
         y = -0.5 * a.ly;
@@ -171,17 +183,31 @@ __global__ __launch_bounds__(AWG) void k_adh_search(AdhDev a, const double *__re
         return;
     const int iz = (int)((zi - (-a.lz / 2.0)) / a.dzw);
     const int span = a.limit * (int)(a.dx / a.dzw);
-    // only columns within LIMIT dx in x can pass the range test; scanning
-    // them in the same increasing order keeps the list (and its ties)
-    const int jx0 = a.ring ? 0 : max(0, (int)floor((xi - a.limit * a.dx) / a.dxw) + a.nxw / 2 - 1);
-    const int jx1 = a.ring ? a.nxw - 1 : min(a.nxw - 1, (int)ceil((xi + a.limit * a.dx) / a.dxw) + a.nxw / 2 + 1);
+    // Only columns within LIMIT dx can pass the range test: an x window on
+    // the slab, an angular window around the knob on the ring (two ranges
+    // when it wraps past angle 0). Scanning them in increasing column order,
+    // as a scan of every column does, keeps the list and its ties.
+    int ra0, ra1, rb0 = 0, rb1 = -1;
+    if (a.ring) {
+        double t = atan2(yi, xi);
+        if (t < 0.0) t += 2.0 * M_PI;
+        const int j0 = (int)floor((t - a.reach) / a.dxw) - 1, j1 = (int)ceil((t + a.reach) / a.dxw) + 1;
+        if (j1 - j0 + 1 >= a.nxw) { ra0 = 0; ra1 = a.nxw - 1; }
+        else if (j0 < 0) { ra0 = 0; ra1 = j1; rb0 = j0 + a.nxw; rb1 = a.nxw - 1; }
+        else if (j1 >= a.nxw) { ra0 = 0; ra1 = j1 - a.nxw; rb0 = j0; rb1 = a.nxw - 1; }
+        else { ra0 = j0; ra1 = j1; }
+    } else {
+        ra0 = max(0, (int)floor((xi - a.limit * a.dx) / a.dxw) + a.nxw / 2 - 1);
+        ra1 = min(a.nxw - 1, (int)ceil((xi + a.limit * a.dx) / a.dxw) + a.nxw / 2 + 1);
+    }
     int cnt = 0;
     for (int i = 0; i < 2 * span; i++) {
         int jz = iz + i - (span - 1);
         double zj = (jz - a.nzw / 2) * a.dzw;
         if (jz < 0) { jz += a.nzw; zj += a.lz; }
         else if (jz >= a.nzw) { jz -= a.nzw; zj -= a.lz; }
-        for (int jx = jx0; jx <= jx1; jx++) {
+        for (int k = ra0, kend = ra1 + (rb1 - rb0 + 1); k <= kend; k++) {
+            const int jx = k <= ra1 ? k : rb0 + (k - ra1 - 1);
             const int id = jz * a.nxw + jx;
             double xj, yj, zr;
             receptor(a, id, xj, yj, zr);
@@ -323,6 +349,13 @@ AdhDev adh_dev(const lbmibm_ctx *ctx) {
     a.lcont = h.lcont; a.f_wlc = h.f_wlc; a.ra = h.ra; a.alpha1 = h.alpha1; a.alpha2 = h.alpha2;
     a.ring = ctx->p.tube_radius > 0.0;
     a.radius = ctx->p.tube_radius;
+    a.ring_xy = ctx->adh_ring;
+    a.reach = 0.0;
+    if (a.ring) {  // chord >= 2 sqrt(R rho) |sin(dtheta / 2)| with rho >= R - LIMIT dx for a searching knob
+        const double L = h.limit * a.dx, R = a.radius, rho = std::max(R - L, 1.0e-300);
+        const double arg = L / (2.0 * std::sqrt(R * rho));
+        a.reach = arg >= 1.0 ? M_PI : 2.0 * std::asin(arg);
+    }
     return a;
 }
 
@@ -342,7 +375,8 @@ lbmibm_status adh_bell(lbmibm_ctx *ctx, int init) {
                        (const unsigned char *)ctx->adh_knob, ctx->adh_attach, ctx->adh_rij, ctx->adh_isatt,
                        ctx->adh_pos, ctx->adh_active, (const DevStep *)ctx->step, init, kl, nk);
     hipLaunchKernelGGL(k_afill<int>, dim3(1), dim3(64), 0, st, ctx->adh_nprop, (size_t)(a.maxn + 1), 0);
-    hipLaunchKernelGGL(k_adh_tick, dim3(1), dim3(64), 0, st, ctx->adh_calls);
+    hipLaunchKernelGGL(k_adh_tick, dim3(1), dim3(256), 0, st, ctx->adh_calls, ctx->adh_claim,
+                       (long long)a.nxw * a.nzw, a.maxn);
     for (int r = 0; r <= a.maxn; r++) {  // each round retires or advances every active spring
         hipLaunchKernelGGL(k_adh_propose, dim3(ablocks(nsp)), dim3(AWG), 0, st, a, (const int *)ctx->adh_idw,
                            (const double *)ctx->adh_rw, (const int *)ctx->adh_isatt, ctx->adh_pos, ctx->adh_active,
@@ -416,9 +450,28 @@ lbmibm_status adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *h, c
     for (long long v = 0; v < nv; v++)
         if (knob[v]) klist.push_back((int)v);
     ctx->adh_nk = (int)klist.size();
+    // A second set-up may change every size (receptor count, MAXN, springs
+    // per knob, knobs): the previous buffers, and the step graph recorded with
+    // their addresses, are released once the device is done with them.
     lbmibm_status s = LBMIBM_OK;
+    if (ctx->adh_knob) {
+        LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+        ctx->graph_ready = false;
+        ctx->adh_on = false;
+        void **bufs[] = {&ctx->adh_knob, (void **)&ctx->adh_idw, (void **)&ctx->adh_rw, (void **)&ctx->adh_attach,
+                         (void **)&ctx->adh_rij, (void **)&ctx->adh_pos, (void **)&ctx->adh_active,
+                         (void **)&ctx->adh_prop, (void **)&ctx->adh_isatt, (void **)&ctx->adh_claim,
+                         (void **)&ctx->adh_calls, (void **)&ctx->adh_fadh, (void **)&ctx->adh_nprop,
+                         (void **)&ctx->adh_klist, &ctx->adh_fsum, (void **)&ctx->adh_ring};
+        for (void **p : bufs) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+        }
+    }
     auto al = [&](void **p, size_t bytes) {
-        if (s == LBMIBM_OK && !*p) s = dev_alloc(ctx, p, bytes);
+        if (s == LBMIBM_OK) s = dev_alloc(ctx, p, bytes);
     };
     al(&ctx->adh_knob, nv);
     al((void **)&ctx->adh_idw, nn * sizeof(int));
@@ -433,13 +486,20 @@ lbmibm_status adhesion_setup(lbmibm_ctx *ctx, const lbmibm_adhesion_params *h, c
     al((void **)&ctx->adh_calls, sizeof(unsigned long long));
     al((void **)&ctx->adh_fadh, 3 * nv * sizeof(double));
     al((void **)&ctx->adh_nprop, (size_t)(h->max_neighwall + 1) * sizeof(int));
-    if (ctx->adh_klist) {  // a second set-up may have other knobs
-        (void)hipFree(ctx->adh_klist);
-        ctx->adh_klist = nullptr;
-    }
     al((void **)&ctx->adh_klist, std::max<size_t>(klist.size(), 1) * sizeof(int));
     al(&ctx->adh_fsum, 3 * nv * ctx->szT);
+    std::vector<double> ring;
+    if (ctx->p.tube_radius > 0.0) {  // the columns' coordinates with the host's libm, as wall_init computes them
+        ring.resize(2 * (size_t)ctx->adh_nxw);
+        for (int ix = 0; ix < ctx->adh_nxw; ix++) {
+            ring[2 * ix] = ctx->p.tube_radius * std::cos(ix * ctx->adh_dxw);
+            ring[2 * ix + 1] = ctx->p.tube_radius * std::sin(ix * ctx->adh_dxw);
+        }
+        al((void **)&ctx->adh_ring, ring.size() * sizeof(double));
+    }
     if (s != LBMIBM_OK) return s;
+    if (!ring.empty())
+        LBMIBM_HIP(ctx, hipMemcpy(ctx->adh_ring, ring.data(), ring.size() * sizeof(double), hipMemcpyHostToDevice));
     LBMIBM_HIP(ctx, hipMemcpy(ctx->adh_knob, knob.data(), nv, hipMemcpyHostToDevice));
     if (!klist.empty())
         LBMIBM_HIP(ctx, hipMemcpy(ctx->adh_klist, klist.data(), klist.size() * sizeof(int), hipMemcpyHostToDevice));

@@ -237,6 +237,19 @@ lbmibm_status field_set(lbmibm_ctx *ctx, void *dst, bool dbl, const double *in) 
 
 }  // namespace
 
+namespace {
+// The gradient's forward runs and recomputes start every BiCGSTAB at x0 = 0
+// (bicg_warm_start is set aside while they run), so that a recomputed step
+// repeats its recorded solve and the finite-difference runs use the same
+// map as the adjoint.
+struct ColdSolves {
+    lbmibm_ctx *c;
+    bool prev;
+    explicit ColdSolves(lbmibm_ctx *ctx) : c(ctx), prev(ctx->solve_cold) { c->solve_cold = true; }
+    ~ColdSolves() { c->solve_cold = prev; }
+};
+}  // namespace
+
 extern "C" {
 
 lbmibm_status lbmibm_set_body_force_field(lbmibm_ctx *ctx, const double *gx, const double *gy, const double *gz) {
@@ -395,6 +408,7 @@ lbmibm_status lbmibm_wall_shear(lbmibm_ctx *ctx, int iter0, int nsteps, int wind
     if (s != LBMIBM_OK) return s;
     s = need(ctx, ctx->p.m == 1 && !ctx->p.volume_correction, "needs M = 1 and no volume correction");
     if (s != LBMIBM_OK) return s;
+    ColdSolves cold(ctx);
     double *acc = nullptr;
     LBMIBM_HIP(ctx, hipMalloc(&acc, sizeof(double)));
     s = zero(ctx, acc, sizeof(double));
@@ -419,6 +433,7 @@ lbmibm_status lbmibm_adjoint_wall_shear(lbmibm_ctx *ctx, int iter0, int nsteps, 
     if (s == LBMIBM_OK) s = need(ctx, ctx->p.fileoutput <= 0, "FILEOUTPUT accumulators must be off");
     if (s == LBMIBM_OK && ctx->nc > 0) s = need(ctx, ctx->p.runge_kutta == 4 && ctx->p.br == 0.0, "needs RK4, BR = 0");
     if (s != LBMIBM_OK) return s;
+    ColdSolves cold(ctx);
     const int K = std::min(checkpoint_every, nsteps);
     const int nseg = (nsteps + K - 1) / K;
     const size_t N = ctx->N, nv = ctx->nv, szS = ctx->szS, szT = ctx->szT;

@@ -269,7 +269,7 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->vof[0], ctx->vof[1], ctx->ft_g, ctx->ft_x, ctx->ft_b, ctx->ft_part, ctx->ft_ctl,
                     ctx->tube_col, ctx->tube_r, ctx->adh_knob, ctx->adh_idw, ctx->adh_attach, ctx->adh_pos,
                     ctx->adh_active, ctx->adh_prop, ctx->adh_isatt, ctx->adh_claim, ctx->adh_rw, ctx->adh_rij,
-                    ctx->adh_fadh, ctx->adh_fsum, ctx->adh_nprop, ctx->adh_klist, ctx->adh_calls, ctx->mw};
+                    ctx->adh_fadh, ctx->adh_fsum, ctx->adh_nprop, ctx->adh_klist, ctx->adh_calls, ctx->adh_ring, ctx->mw};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -321,6 +321,8 @@ void lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz) {
     p->tube_staircase = 0;
     p->wall_motion = 0;
     p->wall_velocity = 0.0;
+    p->bicg_warm_start = 0;
+    p->residual_floor = 1.0e-7;
 }
 
 const char *lbmibm_last_error(const lbmibm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -727,10 +729,12 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
     }
     if (s == LBMIBM_OK) s = join_volume(ctx);  // nothing pending across a capture boundary
     if (s == LBMIBM_OK && use_graph && end - it >= L) {
-        if (ctx->graph_ready && (ctx->graph_cur != ctx->cur || ctx->graph_force_zero != ctx->force_zero)) {
+        if (ctx->graph_ready && (ctx->graph_cur != ctx->cur || ctx->graph_force_zero != ctx->force_zero))
+            ctx->graph_ready = false;
+        if (!ctx->graph_ready && ctx->graph) {  // recorded before a state change: released once it has run
+            LBMIBM_HIP(ctx, hipStreamSynchronize(ctx->stream));
             (void)hipGraphExecDestroy(ctx->graph);
             ctx->graph = nullptr;
-            ctx->graph_ready = false;
         }
         if (!ctx->graph_ready) {
             ctx->graph_cur = ctx->cur;
@@ -908,6 +912,40 @@ lbmibm_status lbmibm_time_stages(lbmibm_ctx *ctx, int iter0, int nsteps, double 
         stage_ms[j] = calls[j] ? acc[j] / calls[j] : 0.0;
         if (stage_calls) stage_calls[j] = calls[j];
     }
+    return s == LBMIBM_OK ? check_flags(ctx, nullptr) : s;
+}
+
+lbmibm_status lbmibm_time_kernels(lbmibm_ctx *ctx, int iter0, int nsteps, char *report, int report_len) {
+    if (!ctx || nsteps < 1 || !report || report_len < 1) return LBMIBM_ERR_ARG;
+    report[0] = 0;
+    lbmibm_status s = reset_status(ctx);
+    if (s == LBMIBM_OK) s = join_volume(ctx);
+    ctx->kt_recs.clear();
+    ctx->kt_on = true;
+    for (int k = 0; k < nsteps && s == LBMIBM_OK; k++) s = enqueue_step(ctx, iter0 + k + 1, false);
+    ctx->kt_on = false;
+    if (s == LBMIBM_OK) s = join_volume(ctx);
+    if (hipDeviceSynchronize() != hipSuccess && s == LBMIBM_OK) s = LBMIBM_ERR_HIP;
+    // per name: launches and total time, in order of first appearance
+    std::vector<std::pair<std::string, std::pair<int, double>>> acc;
+    for (const auto &r : ctx->kt_recs) {
+        float ms = 0.0f;
+        if (s == LBMIBM_OK) (void)hipEventElapsedTime(&ms, r.a, r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+        auto it = std::find_if(acc.begin(), acc.end(), [&](const auto &e) { return e.first == r.name; });
+        if (it == acc.end()) acc.push_back({r.name, {1, (double)ms}});
+        else { it->second.first++; it->second.second += ms; }
+    }
+    ctx->kt_recs.clear();
+    std::string out;
+    char line[160];
+    for (const auto &e : acc) {
+        snprintf(line, sizeof line, "%s\t%d\t%.6f\n", e.first.c_str(), e.second.first,
+                 e.second.second / e.second.first);
+        out += line;
+    }
+    snprintf(report, (size_t)report_len, "%s", out.c_str());
     return s == LBMIBM_OK ? check_flags(ctx, nullptr) : s;
 }
 

@@ -186,6 +186,12 @@ struct lbmibm_ctx {
     unsigned long long *adh_calls = nullptr;          // [1] calls of the Bell update so far
     int *adh_klist = nullptr, adh_nk = 0;  // the knob vertices
     double *adh_rw = nullptr, *adh_rij = nullptr, *adh_fadh = nullptr;
+    double *adh_ring = nullptr;  // [nxw][2] ring receptor coordinates (tube)
+    bool solve_cold = false;     // the checkpointed adjoint's forward and recompute start every solve at 0
+    // per-kernel HIP-event timing of eager steps (lbmibm_time_kernels only)
+    struct KtRec { const char *name; hipEvent_t a, b; };
+    bool kt_on = false;
+    std::vector<KtRec> kt_recs;
     void *adh_fsum = nullptr;
     lbmibm::DevStep *step = nullptr;
     lbmibm::DevStatus *status = nullptr;
@@ -257,6 +263,32 @@ inline lbmibm_status hip_fail(lbmibm_ctx *ctx, hipError_t e, const char *what) {
     do {                                                                \
         hipError_t e_ = hipGetLastError();                              \
         if (e_ != hipSuccess) return ::lbmibm::hip_fail(ctx, e_, name); \
+    } while (0)
+
+// Timed launch: with the kernel timer on (eager steps of lbmibm_time_kernels,
+// never under capture) an event pair brackets the launch on its stream.
+inline hipEvent_t kt_begin(lbmibm_ctx *ctx, hipStream_t st) {
+    if (!ctx->kt_on) return nullptr;
+    hipEvent_t a = nullptr;
+    if (hipEventCreate(&a) != hipSuccess) return nullptr;
+    (void)hipEventRecord(a, st);
+    return a;
+}
+inline void kt_end(lbmibm_ctx *ctx, hipStream_t st, const char *name, hipEvent_t a) {
+    if (!a) return;
+    hipEvent_t b = nullptr;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return;
+    }
+    (void)hipEventRecord(b, st);
+    ctx->kt_recs.push_back({name, a, b});
+}
+#define LBMIBM_KT(ctx, st, name, ...)                                   \
+    do {                                                                \
+        hipEvent_t kt_a_ = ::lbmibm::kt_begin(ctx, st);                 \
+        __VA_ARGS__;                                                    \
+        ::lbmibm::kt_end(ctx, st, name, kt_a_);                         \
     } while (0)
 
 // Work-group remapping so that each of the 8 XCDs (which receive work groups

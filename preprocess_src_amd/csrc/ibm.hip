@@ -1380,6 +1380,7 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
     const IbmGeom g = geom_of(ctx);
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const int nv = ctx->nv;
+    const hipEvent_t kt = kt_begin(ctx, stream);
     hipLaunchKernelGGL(k_zero_int, dim3((ctx->ntiles + 255) / 256), dim3(256), 0, stream, ctx->tile_count,
                        ctx->ntiles);
     LBMIBM_LAUNCHED(ctx, "k_zero_int");
@@ -1396,6 +1397,7 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
                        (const T *)spread_force(ctx), ctx->home, nv, g, ctx->tile_off, ctx->tile_cur, ctx->home_list,
                        ctx->node0, (SpreadRec<T> *)ctx->srec);
     LBMIBM_LAUNCHED(ctx, "k_bin_scatter");
+    kt_end(ctx, stream, "binning", kt);
     ctx->bins_valid = true;
     return LBMIBM_OK;
 }
@@ -1405,19 +1407,23 @@ template <typename T> lbmibm_status spread_launch(lbmibm_ctx *ctx, int form) {
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const IbmGeom g = geom_of(ctx);
     if (form == 1) {
-        hipLaunchKernelGGL(k_spread_box<T>, dim3(ctx->ntiles), dim3(SPREAD_BOX_WG), 0, ctx->stream,
-                           (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off,
-                           (T *)ctx->sbox, g, tg, ctx->absmax, ctx->order_interp);
+        LBMIBM_KT(ctx, ctx->stream, "k_spread_box",
+                  hipLaunchKernelGGL(k_spread_box<T>, dim3(ctx->ntiles), dim3(SPREAD_BOX_WG), 0, ctx->stream,
+                                     (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off,
+                                     (T *)ctx->sbox, g, tg, ctx->absmax, ctx->order_interp));
         LBMIBM_LAUNCHED(ctx, "k_spread_box");
-        hipLaunchKernelGGL(k_spread_gather<T>, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0, ctx->stream,
-                           (const T *)ctx->sbox, ctx->tile_count, ctx->sb_cand, ctx->sb_cand + SB_MAXC * ctx->nx,
-                           ctx->sb_cand + SB_MAXC * (ctx->nx + ctx->ny), (T *)ctx->F, g, tg);
+        LBMIBM_KT(ctx, ctx->stream, "k_spread_gather",
+                  hipLaunchKernelGGL(k_spread_gather<T>, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0,
+                                     ctx->stream, (const T *)ctx->sbox, ctx->tile_count, ctx->sb_cand,
+                                     ctx->sb_cand + SB_MAXC * ctx->nx, ctx->sb_cand + SB_MAXC * (ctx->nx + ctx->ny),
+                                     (T *)ctx->F, g, tg));
         LBMIBM_LAUNCHED(ctx, "k_spread_gather");
         return LBMIBM_OK;
     }
-    hipLaunchKernelGGL(k_spread_tiles<T>, dim3(ctx->ntiles), dim3(256), 0, ctx->stream,
-                       (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off, (T *)ctx->F, g,
-                       tg, ctx->absmax, ctx->order_spread);
+    LBMIBM_KT(ctx, ctx->stream, "k_spread_tiles",
+              hipLaunchKernelGGL(k_spread_tiles<T>, dim3(ctx->ntiles), dim3(256), 0, ctx->stream,
+                                 (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off,
+                                 (T *)ctx->F, g, tg, ctx->absmax, ctx->order_spread));
     LBMIBM_LAUNCHED(ctx, "k_spread_tiles");
     return LBMIBM_OK;
 }
@@ -1447,16 +1453,18 @@ lbmibm_status interp_launch(lbmibm_ctx *ctx, int form, const double *xin, double
     a.rk = ctx->p.runge_kutta;
     a.periodic_y = ctx->p.periodic_y;
     if (form == 1) {
-        hipLaunchKernelGGL(k_interp_vert<T>, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream,
-                           (const T *)ctx->U, xin, xo, uvw, xd, ctx->home_list, ctx->nv, geom_of(ctx), a);
+        LBMIBM_KT(ctx, ctx->stream, "k_interp_vert",
+                  hipLaunchKernelGGL(k_interp_vert<T>, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream,
+                                     (const T *)ctx->U, xin, xo, uvw, xd, ctx->home_list, ctx->nv, geom_of(ctx), a));
         LBMIBM_LAUNCHED(ctx, "k_interp_vert");
         return LBMIBM_OK;
     }
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const size_t lds = 3 * (size_t)RN * sizeof(T);
-    hipLaunchKernelGGL(k_interp_tiles<T>, dim3(ctx->ntiles), dim3(256), lds, ctx->stream, (const T *)ctx->U, xin,
-                       xo, uvw, xd, ctx->tile_count, ctx->tile_off, ctx->home_list, ctx->nv, geom_of(ctx), tg, a,
-                       ctx->order_interp);
+    LBMIBM_KT(ctx, ctx->stream, "k_interp_tiles",
+              hipLaunchKernelGGL(k_interp_tiles<T>, dim3(ctx->ntiles), dim3(256), lds, ctx->stream,
+                                 (const T *)ctx->U, xin, xo, uvw, xd, ctx->tile_count, ctx->tile_off, ctx->home_list,
+                                 ctx->nv, geom_of(ctx), tg, a, ctx->order_interp));
     LBMIBM_LAUNCHED(ctx, "k_interp_tiles");
     return LBMIBM_OK;
 }

@@ -401,7 +401,8 @@ template <typename S, typename T> lbmibm_status step_impl(lbmibm_ctx *ctx) {
     if (hasf == HF && macros == MC && geom == G) launch(k_lbm<S, T, HF, MC, G>);
 #define LBM_GEOMS(HF, MC) LBM_CASE(HF, MC, GEOM_PERIODIC) LBM_CASE(HF, MC, GEOM_SLAB) LBM_CASE(HF, MC, GEOM_TUBE) \
     LBM_CASE(HF, MC, GEOM_SHEAR)
-    LBM_GEOMS(true, true) LBM_GEOMS(true, false) LBM_GEOMS(false, true) LBM_GEOMS(false, false)
+    LBMIBM_KT(ctx, ctx->stream, "k_lbm",
+              LBM_GEOMS(true, true) LBM_GEOMS(true, false) LBM_GEOMS(false, true) LBM_GEOMS(false, false));
 #undef LBM_GEOMS
 #undef LBM_CASE
     LBMIBM_LAUNCHED(ctx, "k_lbm");

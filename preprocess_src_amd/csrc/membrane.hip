@@ -275,14 +275,17 @@ constexpr int BICG1_SHAPES[BICG_NVAR][2] = {{6, 512}, {3, 1024}, {4, 768}};
 // for the neighbour gathers of the matrix-vector products, and x is kept in
 // LDS ([V][4]).  matrix_solver.cu:
 // 279-377 in the same update order; (r, r^) of the next iteration is fused
-// with the residual reduction; r^ = b because x starts at 0 (stress.cu:139).
+// with the residual reduction.  x starts at 0 (stress.cu:139), so r^ = b; with
+// warm != 0 it starts at the previous load out.q and r^ = r0 = b - A x0, the
+// stop test staying |r| / |b| < residual (a start that already passes it
+// returns x0 after no iteration).
 template <typename T, int KMAX, int NT>
 __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cells, const int *__restrict__ voff,
                                                       const T *__restrict__ coef,
                                                       const unsigned short *__restrict__ nbr,
                                                       const T *__restrict__ bv, SolveOut<T> out, int nv,
-                                                      int itrmax, double residual, int *__restrict__ iters,
-                                                      DevStatus *__restrict__ status) {
+                                                      int itrmax, double residual, int warm,
+                                                      int *__restrict__ iters, DevStatus *__restrict__ status) {
     extern __shared__ double dyn[];
     __shared__ double sh[128];
     WgReduce red{sh, 0};
@@ -313,14 +316,14 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const T bb = on ? bv[3 * g + d] : T(0);
-            r[k][d] = bb; p[k][d] = bb; v[k][d] = T(0); t[k][d] = T(0);
+            r[k][d] = bb; p[k][d] = bb; t[k][d] = T(0);
+            v[k][d] = (warm && on) ? out.q[3 * g + d] : T(0);  // x0 until the loop
             acc[0] += (double)bb * (double)bb;
         }
         if (on) {
-            const T zero[3] = {T(0), T(0), T(0)};
-            lds_store3(Ps, j, p[k]);
+            lds_store3(Ps, j, warm ? v[k] : p[k]);
             lds_store3(Bs, j, p[k]);
-            lds_store3(Xs, j, zero);
+            lds_store3(Xs, j, v[k]);
         }
     }
     red.sum<1>(*(double(*)[1])acc);  // its barrier also orders the LDS writes above
@@ -347,6 +350,34 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
     double rr = bb, rr_old = 0.0, alfa = 1.0, omeg = 1.0;
     int it = 0, flag = 0;
     bool conv = false;
+    if (warm) {  // r = p = r^ = b - A x0
+        acc[0] = 0.0;
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            const int j = tid + k * NT;
+            if (j < V) {
+                spmv(Ps, k, v[k], t[k]);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    r[k][d] = r[k][d] - t[k][d];
+                    p[k][d] = r[k][d];
+                    acc[0] += (double)r[k][d] * (double)r[k][d];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) { v[k][d] = T(0); t[k][d] = T(0); }
+        }
+        red.sum<1>(*(double(*)[1])acc);  // its barrier ends the gathers of x0
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            const int j = tid + k * NT;
+            if (j < V) { lds_store3(Ps, j, r[k]); lds_store3(Bs, j, r[k]); }
+        }
+        __syncthreads();
+        rr = acc[0];
+        if (sqrt(rr / bb) < residual) { conv = true; it = -1; }  // x0 passes: no iteration
+    }
+    if (!conv)
     for (it = 0; it < itrmax; it++) {
         if (rr == 0.0) { flag = 1; break; }
         if (it > 0) {
@@ -447,16 +478,18 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
 // matrix-vector product gathers from LDS and p never travels to memory; r, v,
 // t, x in global memory.  Every phase walks the thread's vertices in batches
 // of GB with all loads of a batch issued before use.  For cells whose p fits
-// LDS (V <= BIG_LDS_V); the autotuner times it against k_bicg_global.
+// LDS (V <= BIG_LDS_V); the autotuner times it against k_bicg_global.  With
+// warm != 0 the solve starts at x0 = out.q and r^ = r0 = b - A x0 replaces b
+// in bv (each vertex's entries are read by their own thread only).
 constexpr int BIG_LDS_V = 13568;  // 3 * 4 B * V within 160 KB, FP32
 
 template <typename T, int GB>
 __global__ __launch_bounds__(1024) void k_bicg_big(const int *__restrict__ cells, const int *__restrict__ voff,
                                                    const T *__restrict__ coef,
                                                    const unsigned short *__restrict__ nbr,
-                                                   const T *__restrict__ bv, T *__restrict__ x, T *__restrict__ r,
+                                                   T *__restrict__ bv, T *__restrict__ x, T *__restrict__ r,
                                                    T *__restrict__ v, T *__restrict__ t, SolveOut<T> out, int nv,
-                                                   int itrmax, double residual, int *__restrict__ iters,
+                                                   int itrmax, double residual, int warm, int *__restrict__ iters,
                                                    DevStatus *__restrict__ status) {
     extern __shared__ double dyn[];
     __shared__ double sh[128];
@@ -496,8 +529,8 @@ __global__ __launch_bounds__(1024) void k_bicg_big(const int *__restrict__ cells
         const size_t g = (size_t)v0 + j;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const T b = bv[3 * g + d];
-            x[3 * g + d] = T(0); r[3 * g + d] = b; P[(size_t)d * V + j] = b;
+            const T b = bv[3 * g + d], x0 = warm ? out.q[3 * g + d] : T(0);
+            x[3 * g + d] = x0; r[3 * g + d] = b; P[(size_t)d * V + j] = warm ? x0 : b;
             acc[0] += (double)b * (double)b;
         }
     }
@@ -512,6 +545,35 @@ __global__ __launch_bounds__(1024) void k_bicg_big(const int *__restrict__ cells
     double rr = bb, rr_old = 0.0, alfa = 1.0, omeg = 1.0;
     int it = 0, flag = 0;
     bool conv = false;
+    if (warm) {  // r = r^ = b - A x0 (x0 gathered from P), then p = r in P
+        acc[0] = 0.0;
+        for (int base = tid; base < V; base += GB * NT) {
+            T y[GB][3];
+            spmv_batch(base, getP, y);
+#pragma unroll
+            for (int i = 0; i < GB; i++) {
+                const int j = base + i * NT;
+                if (j < V) {
+                    const size_t g = (size_t)v0 + j;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const T rn = r[3 * g + d] - y[i][d];
+                        r[3 * g + d] = rn;
+                        bv[3 * g + d] = rn;
+                        acc[0] += (double)rn * (double)rn;
+                    }
+                }
+            }
+        }
+        red.sum<1>(*(double(*)[1])acc);  // its barrier ends the gathers of x0
+        for (int j = tid; j < V; j += NT)
+#pragma unroll
+            for (int d = 0; d < 3; d++) P[(size_t)d * V + j] = r[3 * ((size_t)v0 + j) + d];
+        __syncthreads();
+        rr = acc[0];
+        if (sqrt(rr / bb) < residual) { conv = true; it = -1; }  // x0 passes: no iteration
+    }
+    if (!conv)
     for (it = 0; it < itrmax; it++) {
         if (rr == 0.0) { flag = 1; break; }
         if (it > 0) {  // p = r + beta (p - omeg v)
@@ -624,7 +686,9 @@ __global__ __launch_bounds__(1024) void k_bicg_big(const int *__restrict__ cells
 // they need no barriers.  Per vertex and iteration that is 156 B of memory
 // traffic (two matrix reads, b twice, p three times, x twice) against about
 // 270 B for the p-in-LDS form.  Same arithmetic and update order as the
-// other forms (matrix_solver.cu:279-377).
+// other forms (matrix_solver.cu:279-377).  With warm != 0 the solve starts at
+// x0 = out.q and r^ = r0 = b - A x0 replaces b in bv (read by its owner
+// thread only).
 // A copy of v that the compiler cannot see through: addresses derived from it
 // are recomputed where they are used instead of being hoisted out of the
 // iteration loop and held in registers for the whole solve.
@@ -637,9 +701,9 @@ template <int K>
 __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells, const int *__restrict__ voff,
                                                    const float *__restrict__ coef,
                                                    const unsigned short *__restrict__ nbr,
-                                                   const float *__restrict__ bv, float *__restrict__ x,
+                                                   float *__restrict__ bv, float *__restrict__ x,
                                                    float *__restrict__ pg, SolveOut<float> out, int nv, int itrmax,
-                                                   double residual, int *__restrict__ iters,
+                                                   double residual, int warm, int *__restrict__ iters,
                                                    DevStatus *__restrict__ status) {
     extern __shared__ double dyn[];
     __shared__ double sh[128];
@@ -676,14 +740,15 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
             const unsigned g = (unsigned)(v0 + j);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const float b = bv[3 * g + d];
-                x[3 * g + d] = 0.0f;
+                const float b = bv[3 * g + d], x0 = warm ? out.q[3 * g + d] : 0.0f;
+                x[3 * g + d] = x0;
+                if (warm) L[3 * j + d] = x0;
                 rt[k][d] = b;
                 acc[0] += (double)b * (double)b;
             }
         }
     }
-    red.sum<1>(*(double(*)[1])acc);
+    red.sum<1>(*(double(*)[1])acc);  // its barrier also orders the writes of x0 to L
     const double bb = acc[0];
     if (bb == 0.0) {
         const float zero[3] = {0.0f, 0.0f, 0.0f};
@@ -694,6 +759,28 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
     double rr = bb, rr_old = 0.0, alfa = 1.0, omeg = 1.0;
     int it = 0, flag = 0;
     bool conv = false;
+    if (warm) {  // r = r^ = b - A x0
+        acc[0] = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = threadIdx.x + k * NT;
+            if (j < V) {
+                const unsigned g = (unsigned)(v0 + j);
+                spmv(j, g, vr[k]);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    rt[k][d] = rt[k][d] - vr[k][d];
+                    bv[3 * g + d] = rt[k][d];
+                    acc[0] += (double)rt[k][d] * (double)rt[k][d];
+                    vr[k][d] = 0.0f;
+                }
+            }
+        }
+        red.sum<1>(*(double(*)[1])acc);  // its barrier ends the gathers of x0 before p is written
+        rr = acc[0];
+        if (sqrt(rr / bb) < residual) { conv = true; it = -1; }  // x0 passes: no iteration
+    }
+    if (!conv)
     for (it = 0; it < itrmax; it++) {
         if (rr == 0.0) { flag = 1; break; }
         // p = r (first pass) or r + beta (p - omeg v), into LDS and memory
@@ -808,15 +895,16 @@ constexpr int BICG_RES_K = 11;
 
 // Same iteration for cells of any size with the vectors in global memory.
 // Each thread walks its vertices in batches of GB so that the loads of a
-// batch are in flight together.
+// batch are in flight together.  With warm != 0 the solve starts at
+// x0 = out.q and r^ = r0 = b - A x0 replaces b in bv.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ cells, const int *__restrict__ voff,
                                                       const T *__restrict__ coef,
                                                       const unsigned short *__restrict__ nbr,
-                                                      const T *__restrict__ bv, T *__restrict__ x,
+                                                      T *__restrict__ bv, T *__restrict__ x,
                                                       T *__restrict__ r, T *__restrict__ p, T *__restrict__ v,
                                                       T *__restrict__ t, SolveOut<T> out, int nv, int itrmax,
-                                                      double residual, int *__restrict__ iters,
+                                                      double residual, int warm, int *__restrict__ iters,
                                                       DevStatus *__restrict__ status) {
     __shared__ double sh[128];
     WgReduce red{sh, 0};
@@ -852,10 +940,10 @@ __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ ce
         for (int d = 0; d < 3; d++) {
             const size_t k = 3 * (size_t)vv + d;
             const T b = bv[k];
-            x[k] = T(0); r[k] = b; p[k] = b;
+            x[k] = warm ? out.q[k] : T(0); r[k] = b; p[k] = b;
             acc[0] += (double)b * (double)b;
         }
-    red.sum<1>(*(double(*)[1])acc);
+    red.sum<1>(*(double(*)[1])acc);  // its barrier also orders the writes above
     const double bb = acc[0];
     if (bb == 0.0) {
         for (int vv = v0 + tid; vv < v1; vv += nt) {
@@ -868,6 +956,28 @@ __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ ce
     double rr = bb, rr_old = 0.0, alfa = 1.0, omeg = 1.0;
     int it = 0, flag = 0;
     bool conv = false;
+    if (warm) {  // r = p = r^ = b - A x0
+        acc[0] = 0.0;
+        for (int base = v0 + tid; base < v1; base += GB * nt) {
+            T y[GB][3];
+            apply(x, base, y);
+#pragma unroll
+            for (int i = 0; i < GB; i++) {
+                const int vv = base + i * nt;
+                if (vv < v1)
+                    for (int d = 0; d < 3; d++) {
+                        const size_t k = 3 * (size_t)vv + d;
+                        const T rn = r[k] - y[i][d];
+                        r[k] = rn; p[k] = rn; bv[k] = rn;
+                        acc[0] += (double)rn * (double)rn;
+                    }
+            }
+        }
+        red.sum<1>(*(double(*)[1])acc);  // its barrier orders the writes of p before its gathers
+        rr = acc[0];
+        if (sqrt(rr / bb) < residual) { conv = true; it = -1; }  // x0 passes: no iteration
+    }
+    if (!conv)
     for (it = 0; it < itrmax; it++) {
         if (rr == 0.0) { flag = 1; break; }
         if (it > 0) {
@@ -1034,17 +1144,27 @@ __global__ __launch_bounds__(256) void k_bend(const double *__restrict__ xs, con
     }
 }
 
+// The stop test's relative residual: RESIDUAL, raised to the floor with FP32
+// solver arithmetic (below about 1e-7 an FP32 iterate stops changing).
+double solver_residual(const lbmibm_ctx *ctx) {
+    const lbmibm_params &p = ctx->p;
+    return ctx->prec == LBMIBM_F64 ? p.residual : std::max(p.residual, p.residual_floor);
+}
+
 // One on-chip class with the given shape variant.
 template <typename T>
-lbmibm_status launch_onchip(lbmibm_ctx *ctx, int cls, int var, const SolveOut<T> &out, hipStream_t stream) {
+lbmibm_status launch_onchip(lbmibm_ctx *ctx, int cls, int var, const SolveOut<T> &out, hipStream_t stream,
+                            int warm = 0) {
     const lbmibm_params &p = ctx->p;
+    const double residual = solver_residual(ctx);
     const size_t lds = 3 * 4 * (size_t)ctx->bicg_maxv[cls] * sizeof(T);
     const int n = ctx->bicg_n[cls];
 #define LBMIBM_ONCHIP(TT, K, NT)                                                                                \
     hipLaunchKernelGGL((k_bicg_onchip<TT, K, NT>), dim3(n), dim3(NT), lds, stream, ctx->bicg_cells[cls],          \
                        ctx->voff, (const TT *)ctx->coef, ctx->nbr, (const TT *)ctx->bv,                           \
-                       *reinterpret_cast<const SolveOut<TT> *>(&out), ctx->nv, p.itrmax, p.residual,             \
+                       *reinterpret_cast<const SolveOut<TT> *>(&out), ctx->nv, p.itrmax, residual, warm,         \
                        ctx->bicg_iters, ctx->status)
+    const hipEvent_t kt = kt_begin(ctx, stream);
     if (cls == 0) {
         switch (var) {
             case 0: LBMIBM_ONCHIP(T, BICG0_SHAPES[0][0], BICG0_SHAPES[0][1]); break;
@@ -1063,6 +1183,7 @@ lbmibm_status launch_onchip(lbmibm_ctx *ctx, int cls, int var, const SolveOut<T>
         }
     }
 #undef LBMIBM_ONCHIP
+    kt_end(ctx, stream, cls == 0 ? "k_bicg_onchip class 0" : "k_bicg_onchip class 1", kt);
     LBMIBM_LAUNCHED(ctx, "k_bicg_onchip");
     return LBMIBM_OK;
 }
@@ -1089,32 +1210,36 @@ bool big_form_fits(const lbmibm_ctx *ctx, int form) {
 }
 
 template <typename T>
-lbmibm_status launch_big(lbmibm_ctx *ctx, int var, const SolveOut<T> &out, hipStream_t stream) {
+lbmibm_status launch_big(lbmibm_ctx *ctx, int var, const SolveOut<T> &out, hipStream_t stream, int warm = 0) {
     const lbmibm_params &p = ctx->p;
+    const double residual = solver_residual(ctx);
     if constexpr (sizeof(T) == 4) {
         if (var == 2 && big_form_fits(ctx, 2)) {
             const size_t lds = big_lds(ctx);
-            hipLaunchKernelGGL(k_bicg_res<BICG_RES_K>, dim3(ctx->bicg_n[2]), dim3(1024), lds, stream,
-                               ctx->bicg_cells[2], ctx->voff, (const float *)ctx->coef, ctx->nbr,
-                               (const float *)ctx->bv, (float *)ctx->rh, (float *)ctx->pv, out, ctx->nv, p.itrmax,
-                               p.residual, ctx->bicg_iters, ctx->status);
+            LBMIBM_KT(ctx, stream, "k_bicg_res",
+                      hipLaunchKernelGGL(k_bicg_res<BICG_RES_K>, dim3(ctx->bicg_n[2]), dim3(1024), lds, stream,
+                                         ctx->bicg_cells[2], ctx->voff, (const float *)ctx->coef, ctx->nbr,
+                                         (float *)ctx->bv, (float *)ctx->rh, (float *)ctx->pv, out, ctx->nv,
+                                         p.itrmax, residual, warm, ctx->bicg_iters, ctx->status));
             LBMIBM_LAUNCHED(ctx, "k_bicg_res");
             return LBMIBM_OK;
         }
         if (var == 1 && big_form_fits(ctx, 1)) {
             const size_t lds = big_lds(ctx);
-            hipLaunchKernelGGL((k_bicg_big<T, 4>), dim3(ctx->bicg_n[2]), dim3(1024), lds, stream, ctx->bicg_cells[2],
-                               ctx->voff, (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->rh,
-                               (T *)ctx->rv, (T *)ctx->vv, (T *)ctx->tv, out, ctx->nv, p.itrmax, p.residual,
-                               ctx->bicg_iters, ctx->status);
+            LBMIBM_KT(ctx, stream, "k_bicg_big",
+                      hipLaunchKernelGGL((k_bicg_big<T, 4>), dim3(ctx->bicg_n[2]), dim3(1024), lds, stream,
+                                         ctx->bicg_cells[2], ctx->voff, (const T *)ctx->coef, ctx->nbr, (T *)ctx->bv,
+                                         (T *)ctx->rh, (T *)ctx->rv, (T *)ctx->vv, (T *)ctx->tv, out, ctx->nv,
+                                         p.itrmax, residual, warm, ctx->bicg_iters, ctx->status));
             LBMIBM_LAUNCHED(ctx, "k_bicg_big");
             return LBMIBM_OK;
         }
     }
-    hipLaunchKernelGGL(k_bicg_global<T>, dim3(ctx->bicg_n[2]), dim3(1024), 0, stream, ctx->bicg_cells[2], ctx->voff,
-                       (const T *)ctx->coef, ctx->nbr, (const T *)ctx->bv, (T *)ctx->rh, (T *)ctx->rv,
-                       (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, out, ctx->nv, p.itrmax, p.residual,
-                       ctx->bicg_iters, ctx->status);
+    LBMIBM_KT(ctx, stream, "k_bicg_global",
+              hipLaunchKernelGGL(k_bicg_global<T>, dim3(ctx->bicg_n[2]), dim3(1024), 0, stream, ctx->bicg_cells[2],
+                                 ctx->voff, (const T *)ctx->coef, ctx->nbr, (T *)ctx->bv, (T *)ctx->rh, (T *)ctx->rv,
+                                 (T *)ctx->pv, (T *)ctx->vv, (T *)ctx->tv, out, ctx->nv, p.itrmax, residual, warm,
+                                 ctx->bicg_iters, ctx->status));
     LBMIBM_LAUNCHED(ctx, "k_bicg_global");
     return LBMIBM_OK;
 }
@@ -1153,14 +1278,16 @@ lbmibm_status assemble_impl(lbmibm_ctx *ctx, const PeriodicBox &box, const Tensi
     const int ne = ctx->nt, nv = ctx->nv;
     const dim3 ge = cells ? dim3((max_e + 255) / 256, n) : dim3((ne + 255) / 256);
     const dim3 gv = cells ? dim3((max_v + 255) / 256, n) : dim3((nv + 255) / 256);
-    hipLaunchKernelGGL(k_elem<T>, ge, dim3(256), 0, ctx->stream, (const double *)ctx->xd, ctx->ele,
-                       (const double *)ctx->nref, p.store_macros ? (T *)ctx->tt : (T *)nullptr, (T *)ctx->rg,
-                       p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ctx->cslot,
-                       (typename Corner<T>::type *)ctx->corner, ne, box, ta, ctx->step, cells, ctx->eoff);
+    LBMIBM_KT(ctx, ctx->stream, "k_elem",
+              hipLaunchKernelGGL(k_elem<T>, ge, dim3(256), 0, ctx->stream, (const double *)ctx->xd, ctx->ele,
+                                 (const double *)ctx->nref, p.store_macros ? (T *)ctx->tt : (T *)nullptr,
+                                 (T *)ctx->rg, p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ctx->cslot,
+                                 (typename Corner<T>::type *)ctx->corner, ne, box, ta, ctx->step, cells, ctx->eoff));
     LBMIBM_LAUNCHED(ctx, "k_elem");
-    hipLaunchKernelGGL(k_vert<T>, gv, dim3(256), 0, ctx->stream, ctx->nln, ctx->epair,
-                       (const typename Corner<T>::type *)ctx->corner, ctx->vcell, (const T *)ctx->gscale,
-                       (T *)ctx->coef, (T *)ctx->bv, (T *)ctx->dsv, nv, cells, ctx->voff);
+    LBMIBM_KT(ctx, ctx->stream, "k_vert",
+              hipLaunchKernelGGL(k_vert<T>, gv, dim3(256), 0, ctx->stream, ctx->nln, ctx->epair,
+                                 (const typename Corner<T>::type *)ctx->corner, ctx->vcell, (const T *)ctx->gscale,
+                                 (T *)ctx->coef, (T *)ctx->bv, (T *)ctx->dsv, nv, cells, ctx->voff));
     LBMIBM_LAUNCHED(ctx, "k_vert");
     return LBMIBM_OK;
 }
@@ -1237,10 +1364,14 @@ template <typename T> lbmibm_status autotune_impl(lbmibm_ctx *ctx) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // the timed solves wrote loads of the synthetic systems: the first warm
+    // started solve begins at 0, as a cold one
+    if (s == LBMIBM_OK) LBMIBM_HIP(ctx, hipMemsetAsync(ctx->qv, 0, 3 * (size_t)ctx->nv * sizeof(T), ctx->stream));
     return s;
 }
 
-template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
+// warm: the solves start from the previous loads (bicg_warm_start).
+template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx, bool warm) {
     const lbmibm_params &p = ctx->p;
     const PeriodicBox box = box_of(ctx);
     const TensionArgs ta = tension_args(ctx);
@@ -1259,7 +1390,7 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
         // own buffer, the epilogue writes q
         LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
         LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-        lbmibm_status sb = launch_big<T>(ctx, ctx->bicg_var[2], out, ctx->side);
+        lbmibm_status sb = launch_big<T>(ctx, ctx->bicg_var[2], out, ctx->side, warm);
         if (sb != LBMIBM_OK) return sb;
         if (split && ctx->n_onchip > 0) {
             sb = assemble_impl<T>(ctx, box, ta, ctx->onchip_cells, ctx->n_onchip, ctx->cls_maxe[0], ctx->cls_maxv[0]);
@@ -1268,7 +1399,7 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx) {
     }
     for (int cls = 0; cls < 2; cls++)
         if (ctx->bicg_n[cls] > 0) {
-            lbmibm_status s = launch_onchip<T>(ctx, cls, ctx->bicg_var[cls], out, ctx->stream);
+            lbmibm_status s = launch_onchip<T>(ctx, cls, ctx->bicg_var[cls], out, ctx->stream, warm);
             if (s != LBMIBM_OK) return s;
         }
     if (ctx->bicg_n[2] > 0) {
@@ -1536,7 +1667,7 @@ template <typename T> lbmibm_status forces_adj_impl(lbmibm_ctx *ctx, bool have_q
     const int ne = ctx->nt, nv = ctx->nv;
     // forward at the current positions: rows, ring areas, corner records, and
     // q unless the caller restored it (qv) from a recording of this step
-    lbmibm_status s = have_q ? assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx)) : forces_impl<T>(ctx);
+    lbmibm_status s = have_q ? assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx)) : forces_impl<T>(ctx, false);
     if (s != LBMIBM_OK) return s;
     hipLaunchKernelGGL(k_adj_load<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln,
                        (const typename Corner<T>::type *)ctx->corner, (const T *)ctx->dsv, (const T *)ctx->qv,
@@ -1674,7 +1805,8 @@ lbmibm_status membrane_autotune(lbmibm_ctx *ctx) {
 
 lbmibm_status membrane_forces(lbmibm_ctx *ctx) {
     if (ctx->nc == 0) return LBMIBM_OK;
-    return ctx->prec == LBMIBM_F64 ? forces_impl<double>(ctx) : forces_impl<float>(ctx);
+    const bool warm = ctx->p.bicg_warm_start && !ctx->solve_cold;
+    return ctx->prec == LBMIBM_F64 ? forces_impl<double>(ctx, warm) : forces_impl<float>(ctx, warm);
 }
 
 }  // namespace lbmibm

@@ -155,15 +155,19 @@ lbmibm_status run_impl(lbmibm_ctx *ctx, int mode, hipStream_t stream) {
         big = ctx->side;
     }
     if (ctx->vol_n[1] > 0) {
-        hipLaunchKernelGGL(k_volume<1024>, dim3(ctx->vol_n[1]), dim3(1024), 0, big, ctx->vol_cells[1], mode,
-                           (double *)ctx->xs, ctx->nv, ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le,
-                           ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->vol_passes, ctx->status, a);
+        LBMIBM_KT(ctx, big, "k_volume<1024>",
+                  hipLaunchKernelGGL(k_volume<1024>, dim3(ctx->vol_n[1]), dim3(1024), 0, big, ctx->vol_cells[1], mode,
+                                     (double *)ctx->xs, ctx->nv, ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le,
+                                     ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->vol_passes,
+                                     ctx->status, a));
         LBMIBM_LAUNCHED(ctx, "k_volume<1024>");
     }
     if (ctx->vol_n[0] > 0) {
-        hipLaunchKernelGGL(k_volume<256>, dim3(ctx->vol_n[0]), dim3(256), 0, stream, ctx->vol_cells[0], mode,
-                           (double *)ctx->xs, ctx->nv, ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le,
-                           ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->vol_passes, ctx->status, a);
+        LBMIBM_KT(ctx, stream, "k_volume<256>",
+                  hipLaunchKernelGGL(k_volume<256>, dim3(ctx->vol_n[0]), dim3(256), 0, stream, ctx->vol_cells[0], mode,
+                                     (double *)ctx->xs, ctx->nv, ctx->voff, ctx->eoff, ctx->ele, ctx->nle, ctx->le,
+                                     ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->vol_passes,
+                                     ctx->status, a));
         LBMIBM_LAUNCHED(ctx, "k_volume<256>");
     }
     if (fork) {

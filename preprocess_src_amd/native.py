@@ -48,7 +48,7 @@ class Params(C.Structure):
         ("volume_correction", C.c_int), ("precision", C.c_int), ("store_macros", C.c_int),
         ("fileoutput", C.c_int), ("vof_period", C.c_int), ("sor_itrmax", C.c_int), ("sor_residual", C.c_double),
         ("tube_radius", C.c_double), ("tube_staircase", C.c_int), ("wall_motion", C.c_int),
-        ("wall_velocity", C.c_double),
+        ("wall_velocity", C.c_double), ("bicg_warm_start", C.c_int), ("residual_floor", C.c_double),
     ]
 
 
@@ -83,6 +83,7 @@ SIGNATURES = {
     "lbmibm_volume_init": (C.c_int, [C.c_void_p]),
     "lbmibm_volume_correction": (C.c_int, [C.c_void_p, c_int_p]),
     "lbmibm_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "lbmibm_time_kernels": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int]),
     "lbmibm_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "lbmibm_synchronize": (C.c_int, [C.c_void_p]),
     "lbmibm_time_stages": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p, c_int_p]),

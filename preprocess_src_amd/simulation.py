@@ -26,7 +26,7 @@ def make_params(lx, ly, lz, *, precision="f32", tau=None, df=None, dx=1.0, cr=1.
                 rgs=1.0, br=0.0, itrmax=1000, residual=1e-10, runge_kutta=4, neohookean=False,
                 volume_correction=True, store_macros=False, fileoutput=0, periodic_y=False, lambda1=1.0,
                 vof_period=0, sor_itrmax=None, sor_residual=None, tube_radius=0.0, tube_staircase=False,
-                wall_motion=0, wall_velocity=0.0):
+                wall_motion=0, wall_velocity=0.0, bicg_warm_start=False, residual_floor=1e-7):
     P = N.Params()
     N.lib().lbmibm_default_params(C.byref(P), int(lx), int(ly), int(lz))
     if df is None:
@@ -48,6 +48,7 @@ def make_params(lx, ly, lz, *, precision="f32", tau=None, df=None, dx=1.0, cr=1.
     P.tube_radius = float(tube_radius)
     P.tube_staircase = int(tube_staircase)
     P.wall_motion, P.wall_velocity = int(wall_motion), float(wall_velocity)
+    P.bicg_warm_start, P.residual_floor = int(bicg_warm_start), float(residual_floor)
     return P
 
 
@@ -131,6 +132,17 @@ class Simulation:
         calls = np.zeros(6, np.int32)
         self._ck(self.L.lbmibm_time_stages(self.ctx, int(iter0), int(nsteps), N.dp(ms), N.ip(calls)))
         return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(self.STAGES)}
+
+    def time_kernels(self, iter0, nsteps):
+        """{kernel: (launches per step, average ms per launch)} of eager steps,
+        timed with HIP events on each launch's stream."""
+        buf = C.create_string_buffer(8192)
+        self._ck(self.L.lbmibm_time_kernels(self.ctx, int(iter0), int(nsteps), buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split("\t")
+            out[name] = (int(n) / nsteps, float(ms))
+        return out
 
     def synchronize(self):
         self._ck(self.L.lbmibm_synchronize(self.ctx))

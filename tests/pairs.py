@@ -90,3 +90,28 @@ def perturb_cells(cells, seed, amp=0.05):
     for k in ("x", "y", "z"):
         out[k] = cells[k] + amp * rng.standard_normal(cells[k].shape)
     return out
+
+
+def dev_err(a, b, base=0.0):
+    """Error of a against b measured on the deviation b - base: max-abs error,
+    that over max |b - base| (rel) and the L2 ratio (l2_rel)."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    d = a - b
+    s = b - base
+    ms = float(np.max(np.abs(s))) if s.size else 0.0
+    ns = float(np.linalg.norm(s))
+    md = float(np.max(np.abs(d))) if d.size else 0.0
+    return dict(max_abs=md, rel=md / ms if ms > 0 else md, l2_rel=float(np.linalg.norm(d)) / ns if ns > 0 else md)
+
+
+def record_errors(case, errors):
+    """Appends a case's per-field errors as one JSON line to the file named by
+    LBMIBM_FIELD_ERRORS (nothing when unset), and prints them."""
+    import json
+    import os
+    line = json.dumps({"case": case, "errors": errors}, default=float)
+    print(line, flush=True)
+    path = os.environ.get("LBMIBM_FIELD_ERRORS")
+    if path:
+        with open(path, "a") as f:
+            f.write(line + "\n")

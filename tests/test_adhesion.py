@@ -273,6 +273,40 @@ def test_adhesion_in_the_step_graph_matches_eager_steps():
 
 
 @pytest.mark.gpu
+def test_second_setup_with_larger_sizes_matches_a_fresh_one():
+    """A second set-up with more receptors (density 4 -> 9), more receptors
+    per knob (MAXN 16 -> 40) and more springs per knob (2 -> 3), after graph
+    steps with the first, runs exactly as a context set up once with the
+    second parameters: every buffer is sized for the second set-up."""
+    print("[adhesion second set-up]", flush=True)
+    kw = dict(tau=0.9, p=5e-6, b=5e-3)
+    cells = _cell()
+    kn = _knobs(cells)
+    big = dict(ADH, receptor_density=9.0, max_neighwall=40, springs_per_knob=3)
+    nrec = int(LX / (1 / 3) + 1e-3) * int(LZ / (1 / 3) + 1e-3)
+    runs = []
+    for twice in (True, False):
+        g = gpu_of(LX, LY, LZ, "f64", cells=cells, **kw)
+        g.volume_init()
+        if twice:
+            g.adhesion_setup(ADH, kn)
+            g.run(0, 16, graph=True)
+            g.set_cells_state(x=cells["x"], y=cells["y"], z=cells["z"],
+                              xd=np.stack([cells["x"], cells["y"], cells["z"]], 1).reshape(-1))
+            g.set_equilibrium()
+        g.adhesion_setup(big, kn)
+        g.run(0, 16, graph=True)
+        runs.append((g.adhesion_state(nrec), g.cells_state()))
+        g.close()
+    (a, ca), (b, cb) = runs
+    assert (a["attach"] >= 0).sum() > 5
+    for k in ("id_wall", "attach", "is_attached", "f_adh"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("x", "y", "z"):
+        assert np.array_equal(ca[k], cb[k]), k
+
+
+@pytest.mark.gpu
 def test_adhesion_with_repeated_iteration_numbers_matches_the_oracle():
     """Receptor claims do not depend on the iteration numbers being new: a
     second pass over steps 1-4 in the same context claims as the oracle."""

@@ -1,0 +1,429 @@
+"""The five benchmark configurations at their full sizes, on the GPU, with the
+error of each field measured on the part that carries the flow:
+
+  - populations as deviations f - w_q (rho0 = 1), density as rho - 1, vertex
+    positions as displacements from the start, velocities and forces as they
+    are; each error is reported as max-abs, max-abs over the largest
+    deviation (rel) and the L2 ratio;
+  - the FP32 bound is the 1e-5 relative error of the north star; FP16
+    storage is reported per field and bounded below the size of its signal.
+
+C1  plain Poiseuille channel 64x32x32 (30 fluid rows between the wall rows),
+    FP64 and FP32, 1000 steps, against the start-up series and the oracle.
+C2  Taylor-Green vortex 256^3, tau 0.6, u0 0.01, 2000 steps: FP64 against
+    the analytic decay, FP32 and FP16 storage against the FP64 run.
+C3  128x64x64 channel with 64 RBCs of 2562 vertices: FP64 and FP32 against
+    the committed oracle fixture (tests/golden/make_c3.py, 40 steps), and
+    FP32 against FP64 after the configuration's 5000 steps.
+C4  the 256x128x128 suspension of 1024 cells with the 642 / 2562 / 10242
+    vertex mix: 3 FP32 steps against the FP64 oracle run on the host, spread
+    force conservation and the volume constraint of every cell.
+C5  the wall-shear gradient on the 128x64x64 channel with 32 RBCs (500
+    steps, last 200 averaged, checkpoints every 50): FP32 against FP64
+    gradients, central differences, and its cost against the forward run.
+
+With LBMIBM_FIELD_ERRORS set, every case appends its errors to that file.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from tests.pairs import WQ, dev_err, gpu_of, oracle_of, record_errors, rel
+
+pytestmark = pytest.mark.gpu
+
+Q = 19
+CQ = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0],
+               [-1, -1, 0], [-1, 1, 0], [1, -1, 0], [0, 1, 1], [0, -1, -1], [0, -1, 1], [0, 1, -1], [1, 0, 1],
+               [-1, 0, -1], [1, 0, -1], [-1, 0, 1]], float)
+
+
+def interior(a, shape, comps=1):
+    """COMP view of a padded [n] (comps=1) or [n][comps] array."""
+    lx, ly, lz = shape
+    v = a.reshape((lz + 4, ly + 4, lx + 4) + ((comps,) if comps > 1 else ()))
+    return v[2:-2, 2:-2, 2:-2]
+
+
+def pop_errors(fa, fb, shape):
+    """Deviation errors of two padded AoS population arrays on the COMP nodes,
+    slab by slab (bounded temporaries at 256^3)."""
+    A, B = interior(fa, shape, Q), interior(fb, shape, Q)
+    md = ms = sd = ss = 0.0
+    for z in range(A.shape[0]):
+        d = A[z] - B[z]
+        s = B[z] - WQ
+        md = max(md, float(np.abs(d).max()))
+        ms = max(ms, float(np.abs(s).max()))
+        sd += float((d * d).sum())
+        ss += float((s * s).sum())
+    return dict(max_abs=md, rel=md / ms, l2_rel=float(np.sqrt(sd / ss)))
+
+
+# ---------------------------------------------------------------- C1
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
+def test_c1_poiseuille_64x32x32_thirty_fluid_rows(prec, tol):
+    """64x32x32 lattice whose rows y = 0 and y = 31 are walls: 30 fluid rows,
+    half-way walls 30 apart (H = 30), tau 0.8, body force 1e-6 along x, from
+    rest, 1000 steps."""
+    from tests.test_oracle import _poiseuille_series
+    print(f"[C1] prec={prec}", flush=True)
+    lx, ly, lz, tau, a = 64, 30, 32, 0.8, 1e-6
+    g = gpu_of(lx, ly, lz, prec, tau=tau, p=a / 2, volume_correction=False)
+    g.set_equilibrium()
+    g.run(0, 1000, graph=True)
+    u = interior(g.macros()["um"], (lx, ly, lz))
+    assert np.ptp(u, axis=(0, 2)).max() <= 1e-12 * np.abs(u).max()
+    nu = (tau - 0.5) / 3
+    ua = _poiseuille_series(np.arange(ly) + 0.5, 1000, a, nu, ly)
+    e_series = float(np.max(np.abs(u[0, :, 0] - ua)) / ua.max())
+    o = oracle_of(4, ly, 4, tau=tau, p=a / 2, volume_correction=False)
+    for it in range(1, 1001):
+        o.lbm_d3q19(it)
+    uo = o.un.reshape(o.nz, o.ny, o.nx)[2, 2:-2, 2]
+    e = dev_err(u[0, :, 0], uo)
+    record_errors(f"C1 {prec}", {"u_x vs oracle": e, "u_x vs series": e_series})
+    assert e_series < 2e-3
+    assert e["rel"] < tol
+
+
+# ---------------------------------------------------------------- C2
+C2_SHAPE, C2_TAU, C2_U0, C2_STEPS = (256, 256, 256), 0.6, 0.01, 2000
+
+
+def tgv_start(shape, u0=C2_U0):
+    lx, ly, lz = shape
+    k = 2 * np.pi / lx
+    X, Y = np.meshgrid(np.arange(lx + 4) - 1.5 - lx / 2, np.arange(ly + 4) - 1.5 - ly / 2, indexing="xy")
+    rep = lambda a: np.broadcast_to(a, (lz + 4,) + a.shape).reshape(-1)
+    rho = 1.0 - 0.75 * u0 * u0 * (np.cos(2 * k * X) + np.cos(2 * k * Y))
+    return rep(rho), rep(u0 * np.cos(k * X) * np.sin(k * Y)), rep(-u0 * np.sin(k * X) * np.cos(k * Y))
+
+
+def tgv_run(prec):
+    from preprocess_src_amd.simulation import Simulation
+    lx, ly, lz = C2_SHAPE
+    s = Simulation(lx, ly, lz, precision=prec, tau=C2_TAU, periodic_y=True, store_macros=True)
+    rho, u, v = tgv_start(C2_SHAPE)
+    s.set_equilibrium(rho, u, v, None)
+    s.run(0, 20, graph=True)  # warm-up of the graph, then the timed remainder
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(20, C2_STEPS - 20, graph=True)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    mac = s.macros(instantaneous=True)
+    out = dict(f=s.populations(), rho=interior(mac["dn"], C2_SHAPE).copy(),
+               u=[interior(mac[k], C2_SHAPE).copy() for k in ("un", "vn", "wn")],
+               mlups=lx * ly * lz * (C2_STEPS - 20) / dt / 1e6)
+    s.close()
+    return out
+
+
+@pytest.fixture(scope="module")
+def c2_f64():
+    print("[C2] f64 reference run", flush=True)
+    return tgv_run("f64")
+
+
+def test_c2_taylor_green_f64_decays_as_the_analytic_vortex(c2_f64):
+    lx, ly, lz = C2_SHAPE
+    k = 2 * np.pi / lx
+    nu = (C2_TAU - 0.5) / 3
+    X, Y = np.meshgrid(np.arange(lx) + 0.5 - lx / 2, np.arange(ly) + 0.5 - ly / 2, indexing="xy")
+    decay = np.exp(-2 * nu * k * k * C2_STEPS)
+    ua = C2_U0 * decay * np.cos(k * X) * np.sin(k * Y)
+    va = -C2_U0 * decay * np.sin(k * X) * np.cos(k * Y)
+    u, v, w = c2_f64["u"]
+    basis = np.cos(k * X) * np.sin(k * Y)
+    amp = float((u * basis).sum() / (lz * (basis * basis).sum()) / C2_U0)
+    eu = float(max(np.abs(u - ua).max(), np.abs(v - va).max()) / (C2_U0 * decay))
+    record_errors("C2 f64 vs analytic", {"amplitude/u0": amp, "analytic": float(decay), "u_field_rel": eu,
+                                         "u_z_max": float(np.abs(w).max()), "mlups": c2_f64["mlups"]})
+    assert abs(amp / decay - 1) < 1e-3
+    assert eu < 1e-2
+    assert np.abs(w).max() < 1e-15
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_c2_taylor_green_precision_against_f64(prec, c2_f64):
+    print(f"[C2] {prec}", flush=True)
+    r = tgv_run(prec)
+    errs = {"rho-1": dev_err(r["rho"], c2_f64["rho"], 1.0)}
+    for name, a, b in zip(("u_x", "u_y", "u_z"), r["u"], c2_f64["u"]):
+        errs[name] = dev_err(a, b)
+    errs["u_z"]["rel"] = errs["u_z"]["max_abs"] / C2_U0  # u_z = 0: measured against u0
+    errs["f_q - w_q"] = pop_errors(r["f"], c2_f64["f"], C2_SHAPE)
+    errs["mlups"] = r["mlups"]
+    record_errors(f"C2 {prec} vs f64", errs)
+    if prec == "f32":
+        for k in ("u_x", "u_y", "u_z", "f_q - w_q"):
+            assert errs[k]["rel"] < 1e-5, k
+        assert errs["rho-1"]["rel"] < 1e-3
+    else:  # FP16 storage: errors well below the signal of each field
+        for k in ("u_x", "u_y", "f_q - w_q"):
+            assert errs[k]["rel"] < 2e-2, k
+        assert errs["u_z"]["rel"] < 2e-2
+        assert errs["rho-1"]["rel"] < 0.5
+
+
+# ---------------------------------------------------------------- C3
+def c3_fixture():
+    from tests.golden import make_c3 as M
+    return M, np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                   "c3_channel_64rbc_f64.npz"))
+
+
+def c3_sim(prec, cells, **extra):
+    from tests.golden import make_c3 as M
+    lx, ly, lz = M.SHAPE
+    g = gpu_of(lx, ly, lz, prec, cells=cells, **dict(M.PARAMS, **extra))
+    g.volume_init()
+    X = (np.arange(g.np_) // (lx + 4)) % (ly + 4) - 1.5 - ly / 2  # y of each padded node
+    g.set_equilibrium(None, M.poiseuille(X), None, None)
+    return g
+
+
+def c3_record(g):
+    from tests.golden import make_c3 as M
+    s = g.cells_state()
+    return M.record(g.populations(), g.macros(), s, g.volumes(), g.membrane()["bicg_iters"]), s
+
+
+def c3_compare(out, ref, x0, label):
+    """Per-field errors of a C3 record against another one (x0: the initial
+    positions of the recorded vertices)."""
+    errs = {"f_q - w_q": dev_err(out["f_sample"], ref["f_sample"], WQ)}
+    for k in ("um_z0", "vm_z0", "wm_z0"):
+        errs[k] = dev_err(out[k], ref[k])
+    disp = np.concatenate([ref[k] - x0[k] for k in "xyz"])
+    errs["displacement"] = dev_err(np.concatenate([out[k] - x0[k] for k in "xyz"]), disp)
+    errs["nodal force"] = dev_err(np.concatenate([out[k] for k in ("fx", "fy", "fz")]),
+                                  np.concatenate([ref[k] for k in ("fx", "fy", "fz")]))
+    errs["volume"] = float(np.max(np.abs(out["volume"] / ref["volume"] - 1)))
+    errs["area"] = float(np.max(np.abs(out["area"] / ref["area"] - 1)))
+    record_errors(label, errs)
+    return errs
+
+
+def test_c3_placement_reproduces_the_fixture_input():
+    M, G = c3_fixture()
+    cells = M.make_cells()
+    assert np.allclose(M.position_checksum(cells), G["checksum"], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-9), ("f32", 1e-5)])
+def test_c3_hip_path_matches_the_oracle_fixture(prec, tol):
+    M, G = c3_fixture()
+    print(f"[C3 fixture] prec={prec}", flush=True)
+    cells = M.make_cells()
+    g = c3_sim(prec, cells)
+    for it in range(1, M.STEPS + 1):
+        g.step(it)
+    out, _ = c3_record(g)
+    x0 = {k: cells[k][::M.VSTRIDE] for k in "xyz"}
+    errs = c3_compare(out, G, x0, f"C3 {prec} vs oracle fixture, {M.STEPS} steps")
+    for k in ("f_q - w_q", "um_z0", "vm_z0", "wm_z0", "displacement", "nodal force"):
+        assert errs[k]["rel"] < tol, k
+    assert errs["volume"] < tol and errs["area"] < tol
+    g.close()
+
+
+def test_c3_f32_against_f64_after_5000_steps():
+    M, _ = c3_fixture()
+    print("[C3 5000 steps] f64 and f32", flush=True)
+    cells = M.make_cells()
+    recs = {}
+    for prec in ("f64", "f32"):
+        g = c3_sim(prec, cells)
+        t0 = time.perf_counter()
+        g.run(0, 5000, graph=True)
+        g.synchronize()
+        recs[prec] = c3_record(g)[0]
+        recs[prec]["seconds"] = time.perf_counter() - t0
+        g.close()
+    x0 = {k: cells[k][::M.VSTRIDE] for k in "xyz"}
+    errs = c3_compare(recs["f32"], recs["f64"], x0, "C3 f32 vs f64, 5000 steps")
+    record_errors("C3 5000 steps timing", {p: {"s": recs[p]["seconds"],
+                                                "mlups": 128 * 64 * 64 * 5000 / recs[p]["seconds"] / 1e6}
+                                            for p in recs})
+    for k in ("f_q - w_q", "um_z0", "vm_z0", "wm_z0", "displacement"):
+        assert errs[k]["rel"] < 1e-5, k
+    assert errs["volume"] < 1e-5 and errs["area"] < 1e-5
+
+
+# ---------------------------------------------------------------- C4
+def c4_workload():
+    from preprocess_src_amd import mesh
+    lx, ly, lz, nc, radius, seed = 256, 128, 128, 1024, 7.0, 2024
+    levels = mesh.mesh_mix(nc, [0.5, 0.4, 0.1], [3, 4, 5], seed)
+    centres, rot = mesh.place(nc, radius, lx, ly, lz, seed, mode="grid", grid=(16, 8, 8))
+    cells = mesh.assemble(levels, centres, rot, radius)
+    params = dict(tau=1.0, p=0.5e-6, e=(1.0, 0.0, 0.0), b=2.5e-4, m=1, c=100.0, runge_kutta=4, residual=1e-10,
+                  itrmax=1000, volume_correction=True)
+    return (lx, ly, lz), cells, params
+
+
+def test_c4_suspension_three_steps_against_the_oracle():
+    """The bench suspension at full size (1024 cells, 2,423,808 vertices),
+    from the steady channel profile, 3 FP32 steps against 3 FP64 oracle steps
+    on the host."""
+    print("[C4] f32 vs oracle, 3 steps", flush=True)
+    shape, cells, params = c4_workload()
+    lx, ly, lz = shape
+    g = gpu_of(lx, ly, lz, "f32", cells=cells, **params)
+    o = oracle_of(lx, ly, lz, cells=cells, **params)
+    g.volume_init()
+    o.volume_init()
+    X, Y, Z = o.node_coords()
+    nu = (params["tau"] - 0.5) / 3
+    u = np.where(o.bc == 2, 0.0, 2 * params["p"] * (ly * ly / 4 - Y * Y) / (2 * nu))
+    o.init_fluid(None, u, None, None)
+    g.set_equilibrium(None, u, None, None)
+    x0 = np.concatenate([cells[k] for k in "xyz"])
+    t0 = time.perf_counter()
+    for it in range(1, 4):
+        assert o.step(it)[0] == 0
+    t_cpu = time.perf_counter() - t0
+    for it in range(1, 4):
+        g.step(it)
+    s = g.cells_state()
+    errs = {"f_q - w_q": pop_errors(g.populations(), o.fn, shape)}
+    mac = g.macros()
+    for k in ("um", "vm", "wm"):
+        errs[k] = dev_err(interior(mac[k], shape), interior(getattr(o, k), shape))
+    xg = np.concatenate([s[k] for k in "xyz"])
+    xo = np.concatenate([o.cells[k] for k in "xyz"])
+    errs["displacement"] = dev_err(xg - x0, xo - x0)
+    errs["nodal force"] = dev_err(np.concatenate([s[k] for k in ("fx", "fy", "fz")]),
+                                  np.concatenate([o.cells[k] for k in ("cfx", "cfy", "cfz")]))
+    v = g.volumes()
+    errs["volume"] = float(np.max(np.abs(v["volume"] / o.cells["volume"] - 1)))
+    errs["area"] = float(np.max(np.abs(v["area"] / o.cells["area"] - 1)))
+    errs["oracle_seconds_per_step"] = t_cpu / 3
+    record_errors("C4 f32 vs oracle, 3 steps", errs)
+    for k in ("f_q - w_q", "um", "vm", "wm", "displacement"):
+        assert errs[k]["rel"] < 1e-5, k
+    assert errs["nodal force"]["rel"] < 1e-4
+    assert errs["volume"] < 1e-6 and errs["area"] < 1e-6
+    g.close()
+
+
+def test_c4_spread_conserves_force_and_cells_keep_their_volume():
+    """Full-size invariants: the spread lattice force sums to the nodal
+    forces (partition of unity of the delta), and after 16 graph-replayed
+    steps every one of the 1024 cells is within the volume correction's
+    tolerance |V - V0| / V0 < 1e-5."""
+    print("[C4] invariants", flush=True)
+    shape, cells, params = c4_workload()
+    lx, ly, lz = shape
+    g = gpu_of(lx, ly, lz, "f32", cells=cells, **params)
+    g.volume_init()
+    g.run(0, 16, graph=True)
+    g.stress(17)
+    g.ibm_st1()
+    s = g.cells_state()
+    F = g.lattice_force()
+    for a, k in zip(F, ("fx", "fy", "fz")):
+        tot, ref = float(interior(a, shape).sum()), float(s[k].sum())
+        scale = float(np.abs(s[k]).sum())
+        assert abs(tot - ref) < 1e-5 * scale, k
+    v = g.volumes()
+    dv = np.abs(v["volume"] / v["volume0"] - 1)
+    record_errors("C4 invariants", {"max |V-V0|/V0": float(dv.max()), "cells": int(dv.size)})
+    assert dv.size == 1024 and dv.max() < 1e-5
+    g.close()
+
+
+# ---------------------------------------------------------------- C5
+def c5_setup(prec):
+    from preprocess_src_amd import mesh
+    from preprocess_src_amd.simulation import Simulation
+    lx, ly, lz, nc, radius = 128, 64, 64, 32, 4.0
+    centres, rot = mesh.place(nc, radius, lx, ly, lz, 2025)
+    cells = mesh.assemble(np.full(nc, 4), centres, rot, radius)
+    params = dict(tau=1.0, p=0.5e-6, e=(1.0, 0.0, 0.0), b=2.5e-4, m=1, c=100.0, runge_kutta=4, residual=1e-10,
+                  itrmax=1000, volume_correction=False)
+    sim = Simulation(lx, ly, lz, cells=cells, precision=prec, **params)
+    x0 = np.concatenate([cells["x"], cells["y"], cells["z"]]).reshape(3, -1)
+
+    def reset():
+        sim.set_equilibrium()
+        x, y, z = x0
+        sim.set_cells_state(x=x, y=y, z=z, xd=np.stack([x, y, z], 1).reshape(-1).copy())
+    return sim, reset, params
+
+
+C5_STEPS, C5_WINDOW, C5_K = 500, 200, 50
+
+
+@pytest.fixture(scope="module")
+def c5_f64():
+    print("[C5] f64 gradient", flush=True)
+    sim, reset, params = c5_setup("f64")
+    reset()
+    J, gH, gs = sim.adjoint_wall_shear(0, C5_STEPS, C5_WINDOW, C5_K)
+    out = dict(J=J, gH=gH, gs=gs, sim=sim, reset=reset, params=params)
+    yield out
+    sim.close()
+
+
+def test_c5_gradient_matches_central_differences(c5_f64):
+    """Directional derivative along a random body-force field (10% of P) and
+    the derivatives for three cells' Gs, each against a central difference
+    with relative step 1e-3, in FP64."""
+    sim, reset, p = c5_f64["sim"], c5_f64["reset"], c5_f64["params"]
+    rng = np.random.default_rng(7)
+    m = sim.comp_mask()
+    d = [0.1 * p["p"] * rng.standard_normal(sim.np_) * m for _ in range(3)]
+    eps = 1e-3
+    Js = []
+    for sgn in (1.0, -1.0):
+        reset()
+        sim.set_body_force_field(*[sgn * eps * v for v in d])
+        Js.append(sim.wall_shear(0, C5_STEPS, C5_WINDOW))
+    sim.set_body_force_field()
+    fd = (Js[0] - Js[1]) / (2 * eps)
+    ad = float(sum(g @ v for g, v in zip(c5_f64["gH"], d)))
+    errs = {"body force direction": {"adjoint": ad, "fd": fd, "rel": abs(ad - fd) / abs(fd)}}
+    for c in (0, 13, 31):
+        Jc = []
+        for sgn in (1.0, -1.0):
+            sc = np.ones(sim.nc)
+            sc[c] += sgn * eps
+            sim.set_cell_force_scale(sc)
+            reset()
+            Jc.append(sim.wall_shear(0, C5_STEPS, C5_WINDOW))
+        sim.set_cell_force_scale()
+        fdc = (Jc[0] - Jc[1]) / (2 * eps)
+        errs[f"Gs cell {c}"] = {"adjoint": float(c5_f64["gs"][c]), "fd": fdc,
+                                "rel": abs(c5_f64["gs"][c] - fdc) / max(abs(fdc), 1e-300)}
+    record_errors("C5 adjoint vs central differences (f64)", errs)
+    for k, e in errs.items():
+        assert e["rel"] < 1e-3, k
+
+
+def test_c5_f32_gradient_against_f64_and_its_cost(c5_f64):
+    print("[C5] f32 gradient and cost", flush=True)
+    sim, reset, _ = c5_setup("f32")
+    reset()
+    sim.wall_shear(0, 20, 10)  # warm-up
+    reset()
+    t0 = time.perf_counter()
+    J = sim.wall_shear(0, C5_STEPS, C5_WINDOW)
+    t_fwd = time.perf_counter() - t0
+    reset()
+    t0 = time.perf_counter()
+    J2, gH, gs = sim.adjoint_wall_shear(0, C5_STEPS, C5_WINDOW, C5_K)
+    t_both = time.perf_counter() - t0
+    sim.close()
+    gref = np.concatenate(c5_f64["gH"])
+    errs = {"J": abs(J2 / c5_f64["J"] - 1), "J forward vs adjoint call": abs(J2 / J - 1),
+            "dJ/dF": dev_err(np.concatenate(gH), gref), "dJ/dGs": dev_err(gs, c5_f64["gs"]),
+            "adjoint/forward": (t_both - t_fwd) / t_fwd, "forward_s": t_fwd, "forward_plus_adjoint_s": t_both}
+    record_errors("C5 f32 vs f64 gradient", errs)
+    assert errs["J"] < 1e-5 and errs["J forward vs adjoint call"] < 1e-9
+    assert errs["dJ/dF"]["l2_rel"] < 1e-3 and errs["dJ/dGs"]["rel"] < 1e-3
+    assert errs["adjoint/forward"] <= 3.0

@@ -49,3 +49,17 @@ def test_hip_path_matches_the_fixture(prec, tol):
     mac = g.macros()
     assert rel(mac["um"][comp], G["um"]) < tol * 10
     assert rel(g.volumes()["volume"], G["volume"]) < tol
+
+
+def test_oracle_reproduces_the_c3_fixture():
+    """The full-size 128x64x64 channel with 64 RBCs (tests/golden/make_c3.py):
+    the oracle still writes the committed numbers (about half a minute)."""
+    from tests.golden import make_c3 as M
+    C3 = np.load(os.path.join(HERE, "golden", "c3_channel_64rbc_f64.npz"))
+    cells = M.make_cells()
+    assert np.allclose(M.position_checksum(cells), C3["checksum"], rtol=1e-14, atol=0)
+    out = M.run_oracle(cells)
+    for k in ("f_sample", "um_z0", "x", "fx", "volume", "area", "centre"):
+        assert rel(out[k], C3[k]) < 1e-12, k
+    assert np.array_equal(out["vol_passes"], C3["vol_passes"])
+    assert np.array_equal(out["bicg_iters"], C3["bicg_iters"])

@@ -8,8 +8,8 @@ against a 2e-3 bound on the populations.
 import numpy as np
 import pytest
 
-from tests.pairs import (Q, gpu_of, oracle_of, perturb_cells, random_lattice_force, random_populations, rel,
-                         small_cells)
+from tests.pairs import (Q, gpu_of, oracle_of, perturb_cells, pop_err, random_lattice_force, random_populations,
+                         rel, small_cells)
 
 pytestmark = pytest.mark.gpu
 
@@ -286,6 +286,33 @@ def test_step_parity(prec, m):
         assert rel(s[a], o.cells[b]) < TOL[prec] * 10, a
     mac = g.macros()
     assert rel(mac["um"][comp], o.um[comp]) < TOL[prec] * 10
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_warm_started_solves_match_the_oracle(prec):
+    """BiCGSTAB started from the previous step's loads (bicg_warm_start)
+    stops at the same relative residual as the reference's start from 0, so
+    whole steps keep the tolerances of the cold start; from the second step
+    on it needs fewer iterations."""
+    o, g, nsteps = _step_pair(prec, m=1, bicg_warm_start=True)
+    _, h, _ = _step_pair(prec, m=1)
+    print(f"[warm start] prec={prec}", flush=True)
+    its_w, its_c = [], []
+    for it in range(1, nsteps + 1):
+        assert o.step(it)[0] == 0
+        g.step(it)
+        h.step(it)
+        its_w.append(g.membrane()["bicg_iters"].sum())
+        its_c.append(h.membrane()["bicg_iters"].sum())
+    comp = o.bc == 0
+    assert pop_err(comp_pops(o, g.populations()), comp_pops(o, o.fn)) < TOL[prec]
+    s = g.cells_state()
+    for k in "xyz":
+        assert np.max(np.abs(s[k] - o.cells[k])) < TOL[prec] * 24, k
+    for a, b in (("fx", "cfx"), ("fy", "cfy"), ("fz", "cfz")):
+        assert rel(s[a], o.cells[b]) < TOL[prec] * 10, a
+    assert its_w[0] == its_c[0]  # q = 0 before the first solve
+    assert sum(its_w[1:]) < 0.8 * sum(its_c[1:]), (its_w, its_c)
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
