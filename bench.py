@@ -247,12 +247,14 @@ def kernel_table(kt, n, nv, nt, precision, prof):
     algo = {
         "k_lbm": n * lattice_bytes_per_node(precision, True),
         # spreading: positions (24 B) and forces (12 B) of each vertex in, F out
-        "k_spread_box": 36 * nv, "k_spread_gather": 3 * t * n, "k_spread_tiles": 36 * nv + 3 * t * n,
+        "k_spread_box": 36 * nv, "k_spread_seg": 36 * nv, "k_spread_gather": 3 * t * n,
+        "k_spread_tiles": 36 * nv + 3 * t * n,
         # interpolation + advection: positions in (24 B), SoA and AoS positions
         # out (48 B), the velocity field in
         "k_interp_tiles": 72 * nv + 3 * t * n, "k_interp_vert": 72 * nv + 3 * t * n,
     }
     prof_name = {"k_lbm": "k_lbm<float, float, true, false, 1>", "k_spread_box": "k_spread_box<float>",
+                 "k_spread_seg": "k_spread_seg<float>",
                  "k_spread_gather": "k_spread_gather<float>", "k_interp_tiles": "k_interp_tiles<float>",
                  "k_bicg_res": "k_bicg_res<11>", "k_elem": "k_elem<float>", "k_vert": "k_vert<float>"}
     rows = []
@@ -465,7 +467,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cold-start", action="store_true",
-                    help="BiCGSTAB from x0 = 0 every step, as the reference (default: from the previous loads)")
+                    help="BiCGSTAB from x0 = 0 every step, as the reference (default: --warm-start)")
+    ap.add_argument("--warm-start", type=int, default=2, choices=[1, 2],
+                    help="BiCGSTAB start: 1 the previous loads, 2 their linear extrapolation from the last two")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the FP64 run and the adjoint measurement of the default workload")
     ap.add_argument("--adj-steps", type=int, default=500, help="steps of the adjoint workload (window 40%%)")
@@ -498,7 +502,7 @@ def main():
         return
     if a.workload == "suspension":
         shape, cells, params, desc, ht = suspension_workload()
-        params["bicg_warm_start"] = not a.cold_start
+        params["bicg_warm_start"] = 0 if a.cold_start else a.warm_start
     else:
         shape, cells, params, desc, ht = tgv_workload()
     lx, ly, lz = shape

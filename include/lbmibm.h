@@ -112,11 +112,13 @@ typedef struct lbmibm_params {
                                 iter < FLOWSTART (sra = 0, lbm.cu:755).  Needs the slab     */
     double wall_velocity;    /* U in units of Uc (lattice speed Cr U): sr LY/2 for allwall,
                                 sr LY for topwall, ub = 1 in lbm_SHEARFLOW                  */
-    int    bicg_warm_start;  /* 1: each stress() BiCGSTAB starts from the cell's previous load
-                                q (x0 = q, r^ = r0 = b - A q; zero before the first solve);
-                                0: from x0 = 0 as stress.cu:139 (r^ = b).  Both stop at the
-                                same relative residual |r|/|b|, so their loads agree to it.
-                                The adjoint's solves always start from 0                     */
+    int    bicg_warm_start;  /* start of each stress() BiCGSTAB: 0 from x0 = 0 as stress.cu:139
+                                (r^ = b); 1 from the cell's previous load q; 2 from the linear
+                                extrapolation 2 q_n - q_(n-1) of its last two loads (both zero
+                                before the first solves).  1 and 2 take r^ = r0 = b - A x0 and
+                                stop at the same relative residual |r|/|b| as 0, so their loads
+                                agree with it to that residual.  The adjoint's solves always
+                                start from 0                                                   */
     double residual_floor;   /* with FP32 solver arithmetic (FP32, FP16 storage) the stop test
                                 uses max(RESIDUAL, residual_floor): below about 1e-7 an FP32
                                 iterate no longer changes, only the recursively updated

@@ -256,7 +256,7 @@ namespace {
 void free_all(lbmibm_ctx *ctx) {
     void *ptrs[] = {ctx->f[0], ctx->f[1], ctx->F, ctx->U, ctx->Ui, ctx->rho, ctx->rhom, ctx->voff, ctx->eoff,
                     ctx->vcell, ctx->ele, ctx->ec, ctx->nln, ctx->ln, ctx->lo8, ctx->nle, ctx->le, ctx->xs,
-                    ctx->xd, ctx->uvw, ctx->cf, ctx->c0, ctx->s0, ctx->nref, ctx->tt, ctx->rg, ctx->tp, ctx->qv,
+                    ctx->xd, ctx->uvw, ctx->cf, ctx->c0, ctx->s0, ctx->nref, ctx->tt, ctx->rg, ctx->tp, ctx->qv, ctx->qp,
                     ctx->qi, ctx->coef, ctx->bv, ctx->rv, ctx->rh, ctx->pv, ctx->vv, ctx->tv, ctx->gscale,
                     ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->bicg_iters, ctx->vol_passes,
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
@@ -342,7 +342,8 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     *out = nullptr;
     if (p->lx < 4 || p->ly < 4 || p->lz < 4 || p->m < 1 || p->dx <= 0.0 || p->cr <= 0.0 ||
         (p->runge_kutta != 1 && p->runge_kutta != 2 && p->runge_kutta != 4) || p->precision < 0 ||
-        p->precision > 2 || p->itrmax < 1 || (p->force_modulated && p->freq == 0.0))
+        p->precision > 2 || p->itrmax < 1 || (p->force_modulated && p->freq == 0.0) || p->bicg_warm_start < 0 ||
+        p->bicg_warm_start > 2)
         return LBMIBM_ERR_ARG;
     // the viscosity contrast acts through the VOF indicator, which needs cells
     // and the slab's walls (CapsuleSHEARFLOW, fronttrack.cu:87-125)
@@ -483,6 +484,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         CK(up_T(&ctx->tp, nullptr, 2 * (size_t)nt));
         for (void **v : {&ctx->qv, &ctx->qi, &ctx->bv, &ctx->rv, &ctx->rh, &ctx->pv, &ctx->vv, &ctx->tv})
             CK(up_T(v, nullptr, 3 * (size_t)nv));
+        if (p->bicg_warm_start == 2) CK(up_T(&ctx->qp, nullptr, 3 * (size_t)nv));
         CK(up_T(&ctx->coef, nullptr, 6 * (size_t)nv));
         CK(up_T(&ctx->dsv, nullptr, (size_t)nv));
         // static system structure: cell-local column ids of the 6 off-diagonal

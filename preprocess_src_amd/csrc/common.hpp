@@ -108,6 +108,7 @@ struct lbmibm_ctx {
     void *c0 = nullptr, *s0 = nullptr, *nref = nullptr;  // FP64
     void *tt = nullptr, *rg = nullptr, *tp = nullptr;
     void *qv = nullptr, *qi = nullptr;                 // [3Nv]
+    void *qp = nullptr;                                // [3Nv] the loads before qv (bicg_warm_start 2)
     void *coef = nullptr;                              // [6][Nv] off-diagonal Jacobi-scaled coefficients
     unsigned short *nbr = nullptr;                     // [6][Nv] cell-local neighbour ids (self for unused)
     // BiCGSTAB cell classes: 0 on chip with one vertex per thread, 1 on chip
@@ -115,8 +116,10 @@ struct lbmibm_ctx {
     int *bicg_cells[3] = {nullptr, nullptr, nullptr};
     int bicg_n[3] = {0, 0, 0}, bicg_maxv[3] = {0, 0, 0};
     int bicg_var[3] = {0, 0, 0};  // shape variant of the on-chip classes, form of class 2 (autotuned)
-    int interp_form = 0;          // 0: tiles staged in LDS, 1: one thread per vertex, rows from memory (autotuned)
-    int spread_form = 0;          // 0: source bins per tile, 1: home boxes + gather (autotuned)
+    int interp_form = 0;          // 0: tiles staged in LDS, 1: one thread per vertex, rows from memory,
+                                  // 2: tiles staged in LDS, four lanes per vertex (autotuned)
+    int spread_form = 0;          // 0: source bins per tile, 1: home boxes + gather, 2: home boxes by
+                                  // sorted stencil runs + gather (autotuned)
     void *sbox = nullptr;         // [ntiles][3][SB_N] staging boxes of spread form 1
     int *sb_cand = nullptr;       // [(nx+ny+nz)*4] box entries per coordinate (spread form 1)
     int big_lds = 0;              // dynamic LDS of the class-2 forms that use LDS (the whole CU)

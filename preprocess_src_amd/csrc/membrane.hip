@@ -226,10 +226,23 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
 // scale.
 template <typename T> struct SolveOut {
     T *q, *cf, *qi;
+    T *qp;        // the loads of the solve before q (warm start 2), else null
     const T *ds;
     const DevStep *st;
     int nv, fo;   // fo = FILEOUTPUT period of qi (0: no accumulator)
     double wq;    // M / FILEOUTPUT
+    // Start value of entry i (= 3 g + d) of the solve: 0 (cold), the last
+    // load q (warm 1) or the linear extrapolation 2 q - qp from the last two
+    // loads (warm 2), which then moves q into qp.  Called once per entry, by
+    // the entry's owner thread, before the epilogue writes q.
+    __device__ __forceinline__ T start(size_t i, int warm) const {
+        if (warm == 0) return T(0);
+        const T last = q[i];
+        if (warm == 1) return last;
+        const T prev = qp[i];
+        qp[i] = last;
+        return (T)(2.0 * (double)last - (double)prev);
+    }
     __device__ __forceinline__ void put(size_t g, const T x[3]) const {
         const double d = (double)ds[g];
 #pragma unroll
@@ -317,7 +330,7 @@ __global__ __launch_bounds__(NT) void k_bicg_onchip(const int *__restrict__ cell
         for (int d = 0; d < 3; d++) {
             const T bb = on ? bv[3 * g + d] : T(0);
             r[k][d] = bb; p[k][d] = bb; t[k][d] = T(0);
-            v[k][d] = (warm && on) ? out.q[3 * g + d] : T(0);  // x0 until the loop
+            v[k][d] = on ? out.start(3 * (size_t)g + d, warm) : T(0);  // x0 until the loop
             acc[0] += (double)bb * (double)bb;
         }
         if (on) {
@@ -529,7 +542,7 @@ __global__ __launch_bounds__(1024) void k_bicg_big(const int *__restrict__ cells
         const size_t g = (size_t)v0 + j;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const T b = bv[3 * g + d], x0 = warm ? out.q[3 * g + d] : T(0);
+            const T b = bv[3 * g + d], x0 = out.start(3 * (size_t)g + d, warm);
             x[3 * g + d] = x0; r[3 * g + d] = b; P[(size_t)d * V + j] = warm ? x0 : b;
             acc[0] += (double)b * (double)b;
         }
@@ -740,7 +753,7 @@ __global__ __launch_bounds__(1024) void k_bicg_res(const int *__restrict__ cells
             const unsigned g = (unsigned)(v0 + j);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const float b = bv[3 * g + d], x0 = warm ? out.q[3 * g + d] : 0.0f;
+                const float b = bv[3 * g + d], x0 = out.start(3 * (size_t)g + d, warm);
                 x[3 * g + d] = x0;
                 if (warm) L[3 * j + d] = x0;
                 rt[k][d] = b;
@@ -940,7 +953,7 @@ __global__ __launch_bounds__(1024) void k_bicg_global(const int *__restrict__ ce
         for (int d = 0; d < 3; d++) {
             const size_t k = 3 * (size_t)vv + d;
             const T b = bv[k];
-            x[k] = warm ? out.q[k] : T(0); r[k] = b; p[k] = b;
+            x[k] = out.start(k, warm); r[k] = b; p[k] = b;
             acc[0] += (double)b * (double)b;
         }
     red.sum<1>(*(double(*)[1])acc);  // its barrier also orders the writes above
@@ -1249,6 +1262,7 @@ template <typename T> SolveOut<T> solve_out(lbmibm_ctx *ctx) {
     out.q = (T *)ctx->qv;
     out.cf = (T *)ctx->cf;
     out.qi = (T *)ctx->qi;
+    out.qp = (T *)ctx->qp;
     out.ds = (const T *)ctx->dsv;
     out.st = ctx->step;
     out.nv = ctx->nv;
@@ -1370,8 +1384,9 @@ template <typename T> lbmibm_status autotune_impl(lbmibm_ctx *ctx) {
     return s;
 }
 
-// warm: the solves start from the previous loads (bicg_warm_start).
-template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx, bool warm) {
+// warm: the start of the solves (bicg_warm_start: 0 from zero, 1 from the
+// previous loads, 2 extrapolated from the two previous loads).
+template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx, int warm) {
     const lbmibm_params &p = ctx->p;
     const PeriodicBox box = box_of(ctx);
     const TensionArgs ta = tension_args(ctx);
@@ -1805,7 +1820,7 @@ lbmibm_status membrane_autotune(lbmibm_ctx *ctx) {
 
 lbmibm_status membrane_forces(lbmibm_ctx *ctx) {
     if (ctx->nc == 0) return LBMIBM_OK;
-    const bool warm = ctx->p.bicg_warm_start && !ctx->solve_cold;
+    const int warm = ctx->solve_cold ? 0 : ctx->p.bicg_warm_start;
     return ctx->prec == LBMIBM_F64 ? forces_impl<double>(ctx, warm) : forces_impl<float>(ctx, warm);
 }
 

@@ -17,10 +17,85 @@ struct cell {};
 struct fem {};
 struct Wall {};
 
+template <class T> static bool get(std::FILE *f, T *v, size_t n) { return std::fread(v, sizeof(T), n, f) == n; }
+template <class T> static void put(std::FILE *f, const T *v, size_t n) { std::fwrite(v, sizeof(T), n, f); }
+
+// "cells <in> <out>": the case file <in> holds int32 lx, ly, lz, nc, precision,
+// steps, FP64 tau, P, B, C, RESIDUAL, int32 voff[nc+1], eoff[nc+1], FP64
+// x, y, z [Nv], int32 ele [3Nt].  The cells go through attach, volume_init
+// and main.cu's loop with the volume correction every M steps; <out> gets
+// FP64 fn [19n], um, vm, wm [n], x, y, z, fx, fy, fz [Nv], volume, area
+// [nc] and int32 the last correction's pass count.
+static int run_cells(const char *in, const char *out) {
+    std::FILE *f = std::fopen(in, "rb");
+    if (!f) return 10;
+    int hdr[6];
+    double par[5];
+    if (!get(f, hdr, 6) || !get(f, par, 5)) return 11;
+    const int nc = hdr[3], steps = hdr[5];
+    std::vector<int> voff(nc + 1), eoff(nc + 1);
+    if (!get(f, voff.data(), nc + 1) || !get(f, eoff.data(), nc + 1)) return 11;
+    const int nv = voff[nc], nt = eoff[nc];
+    std::vector<double> x(nv), y(nv), z(nv);
+    std::vector<int> ele(3 * (size_t)nt);
+    if (!get(f, x.data(), nv) || !get(f, y.data(), nv) || !get(f, z.data(), nv) || !get(f, ele.data(), 3 * (size_t)nt))
+        return 11;
+    std::fclose(f);
+    lbmibm_params p;
+    lbmibm_default_params(&p, hdr[0], hdr[1], hdr[2]);
+    p.precision = hdr[4];
+    p.df = (2.0 * par[0] - 1.0) / 6.0;  // tau = (6 Df + 1)/2
+    p.p = par[1];
+    p.b = par[2];
+    p.c = par[3];
+    p.residual = par[4];
+    p.volume_correction = 1;
+    p.store_macros = 1;
+    lbmibm_cells cs{nc, voff.data(), eoff.data(), x.data(), y.data(), z.data(), ele.data()};
+    lbmibm_ref::attach(p, cs);
+    process prc{0, 0};
+    domain cdo{0};
+    lattice ltc;
+    cell cel;
+    fem fm;
+    const int M = p.m;
+    int passes = 0;
+    volume_init(&cdo, &cel);
+    while (prc.iter < steps) {
+        prc.iter++;
+        if ((prc.iter - 1) % M == 0) stress(&prc, &cdo, &cel, &fm, &ltc);
+        if ((prc.iter - 1) % M == 0) ibm_st1(&cdo, &ltc, &cel);
+        lbm_d3q19(&prc, &cdo, &ltc);
+        if (prc.iter % M == 0) ibm_st2(&cdo, &ltc, &cel, &fm);
+        if (prc.iter % M == 0) passes = volume_correction(&cdo, &cel, &fm);
+    }
+    long long n = 0;
+    lbmibm_sizes(lbmibm_ref::context(), &n, nullptr, nullptr, nullptr);
+    std::vector<double> fn(19 * (size_t)n), um(n), vm(n), wm(n), fx(nv), fy(nv), fz(nv), v0(nc), vol(nc), area(nc);
+    lbmibm_ref::pull_lattice(fn.data(), nullptr, nullptr, nullptr, nullptr, nullptr, um.data(), vm.data(), wm.data());
+    lbmibm_ref::pull_cells(x.data(), y.data(), z.data(), nullptr, nullptr, nullptr, fx.data(), fy.data(), fz.data(),
+                           nullptr);
+    lbmibm_ref::check(lbmibm_download_volume(lbmibm_ref::context(), v0.data(), vol.data(), area.data(), nullptr),
+                      "download_volume");
+    lbmibm_ref::detach();
+    std::FILE *o = std::fopen(out, "wb");
+    if (!o) return 12;
+    put(o, fn.data(), fn.size());
+    for (auto *a : {&um, &vm, &wm}) put(o, a->data(), a->size());
+    for (auto *a : {&x, &y, &z, &fx, &fy, &fz}) put(o, a->data(), a->size());
+    put(o, vol.data(), vol.size());
+    put(o, area.data(), area.size());
+    put(o, &passes, 1);
+    std::fclose(o);
+    std::printf("cells %d vertices %d steps %d passes %d\n", nc, nv, steps, passes);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     lbmibm_params p;
     lbmibm_default_params(&p, 16, 12, 10);
     if (p.lx != 16 || p.m != 1) return 2;
+    if (argc == 4 && std::strcmp(argv[1], "cells") == 0) return run_cells(argv[2], argv[3]);
     if (argc < 2 || (std::strcmp(argv[1], "run") != 0 && std::strcmp(argv[1], "vof") != 0)) {
         std::printf("adaptor ok\n");
         return 0;
