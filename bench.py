@@ -247,14 +247,12 @@ def kernel_table(kt, n, nv, nt, precision, prof):
     algo = {
         "k_lbm": n * lattice_bytes_per_node(precision, True),
         # spreading: positions (24 B) and forces (12 B) of each vertex in, F out
-        "k_spread_box": 36 * nv, "k_spread_seg": 36 * nv, "k_spread_gather": 3 * t * n,
-        "k_spread_tiles": 36 * nv + 3 * t * n,
+        "k_spread_box": 36 * nv, "k_spread_gather": 3 * t * n, "k_spread_tiles": 36 * nv + 3 * t * n,
         # interpolation + advection: positions in (24 B), SoA and AoS positions
         # out (48 B), the velocity field in
         "k_interp_tiles": 72 * nv + 3 * t * n, "k_interp_vert": 72 * nv + 3 * t * n,
     }
     prof_name = {"k_lbm": "k_lbm<float, float, true, false, 1>", "k_spread_box": "k_spread_box<float>",
-                 "k_spread_seg": "k_spread_seg<float>",
                  "k_spread_gather": "k_spread_gather<float>", "k_interp_tiles": "k_interp_tiles<float>",
                  "k_bicg_res": "k_bicg_res<11>", "k_elem": "k_elem<float>", "k_vert": "k_vert<float>"}
     rows = []

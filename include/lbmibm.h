@@ -123,6 +123,10 @@ typedef struct lbmibm_params {
                                 uses max(RESIDUAL, residual_floor): below about 1e-7 an FP32
                                 iterate no longer changes, only the recursively updated
                                 residual keeps falling.  0 keeps RESIDUAL.  FP64 ignores it */
+    int    fp32_collide;     /* LBMIBM_F32 only: 0 collides the FP32-stored populations in FP64
+                                arithmetic (the lattice step stays bound by its FP32 HBM
+                                traffic, and the collision's FP32 rounding does not drift over
+                                long runs); 1 collides in FP32                              */
 } lbmibm_params;
 
 /* Initial cell set: reference SoA positions and element vertex ids. */
@@ -140,7 +144,7 @@ typedef struct lbmibm_ctx lbmibm_ctx;
  * LAMBDA1=1, no body force, B=0, M=1, C=100, rg=1, BR=0, ITRMAX=1000,
  * RESIDUAL=1e-10, RK4, Skalak, volume correction on, FP32, store_macros=0,
  * no VOF indicator (SOR ITRMAX=1000, RESIDUAL=1e-10), BiCGSTAB from x0 = 0,
- * FP32 residual floor 1e-7. */
+ * FP32 residual floor 1e-7, FP32 storage collided in FP64. */
 void          lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz);
 
 /* cdo_init + flu_init + cel_init + fem_init + mem_init (init1.cu:13-1092,

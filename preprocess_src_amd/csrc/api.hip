@@ -5,6 +5,7 @@
 // reference's padded / AoS layouts.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -260,7 +261,7 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->qi, ctx->coef, ctx->bv, ctx->rv, ctx->rh, ctx->pv, ctx->vv, ctx->tv, ctx->gscale,
                     ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->bicg_iters, ctx->vol_passes,
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
-                    ctx->tile_off, ctx->tile_cur, ctx->order_calls, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
+                    ctx->tile_off, ctx->tile_cur, ctx->order_calls, ctx->chunks, ctx->chunk_total, ctx->tile_chunk, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
                     ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv,
                     ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair,
                     ctx->srec, ctx->onchip_cells, ctx->Hb, ctx->adj_f[0], ctx->adj_f[1], ctx->adj_u, ctx->adj_F,
@@ -323,6 +324,7 @@ void lbmibm_default_params(lbmibm_params *p, int lx, int ly, int lz) {
     p->wall_velocity = 0.0;
     p->bicg_warm_start = 0;
     p->residual_floor = 1.0e-7;
+    p->fp32_collide = 0;
 }
 
 const char *lbmibm_last_error(const lbmibm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -578,6 +580,9 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         CK(dev_alloc(ctx, (void **)&ctx->tile_off, (size_t)ctx->ntiles * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->tile_cur, (size_t)ctx->ntiles * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->tile_cand, (size_t)ctx->ntiles * sizeof(int)));
+        ctx->nchunk_max = ctx->ntiles + (nv + INTERP_CHUNK - 1) / INTERP_CHUNK;
+        CK(dev_alloc(ctx, (void **)&ctx->chunks, (size_t)ctx->nchunk_max * sizeof(int2)));
+        CK(dev_alloc(ctx, (void **)&ctx->tile_chunk, ((size_t)ctx->ntiles + 1) * sizeof(int)));
         {
             // tile orders start as the identity (any permutation is valid)
             std::vector<int> ident(ctx->ntiles);
@@ -588,6 +593,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         {
             const int zero = 0;
             CK(up_i(&ctx->order_calls, &zero, 1));
+            CK(up_i(&ctx->chunk_total, &zero, 1));
         }
         CK(dev_alloc(ctx, (void **)&ctx->blk_fmax, (size_t)ctx->nblk_bin * sizeof(float)));
         CK(dev_alloc(ctx, (void **)&ctx->absmax, sizeof(unsigned)));
@@ -721,12 +727,15 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
     // membrane force, except across block ends
     const int L0 = (M % 2 == 0) ? M : 2 * M;
     const int L = L0 * ((8 + L0 - 1) / L0);
+    // LBMIBM_OVERLAP_VOLUME=0 keeps the volume correction on the stream
+    const char *ov = getenv("LBMIBM_OVERLAP_VOLUME");
+    const bool overlap = !(ov && ov[0] == '0');
     int it = iter0;
     const int end = iter0 + nsteps;
     s = set_step(ctx, iter0, false);
     // eager steps until the next step starts an M-cycle
     while (s == LBMIBM_OK && it < end && (use_graph == 0 || (it % M) != 0)) {
-        s = enqueue_step(ctx, it + 1, true, true);
+        s = enqueue_step(ctx, it + 1, true, overlap);
         it++;
     }
     if (s == LBMIBM_OK) s = join_volume(ctx);  // nothing pending across a capture boundary
@@ -747,7 +756,7 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
             hipGraph_t g = nullptr;
             LBMIBM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             lbmibm_status cs = LBMIBM_OK;
-            for (int k = 0; k < L && cs == LBMIBM_OK; k++) cs = enqueue_step(ctx, it + k + 1, true, true);
+            for (int k = 0; k < L && cs == LBMIBM_OK; k++) cs = enqueue_step(ctx, it + k + 1, true, overlap);
             if (cs == LBMIBM_OK) cs = join_volume(ctx);
             hipError_t e = hipStreamEndCapture(ctx->stream, &g);
             if (cs != LBMIBM_OK) return cs;
@@ -763,7 +772,7 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
         }
     }
     while (s == LBMIBM_OK && it < end) {
-        s = enqueue_step(ctx, it + 1, true, true);
+        s = enqueue_step(ctx, it + 1, true, overlap);
         it++;
     }
     if (s == LBMIBM_OK) {

@@ -36,6 +36,7 @@ struct DevStatus {
 // Lattice tiles of the immersed-boundary kernels.
 constexpr int VOL_SMALL = 3072;
 constexpr int TILE_X = 16, TILE_Y = 8, TILE_Z = 8;
+constexpr int INTERP_CHUNK = 512;  // home vertices per work group of the interpolation and spreading form 1
 
 }  // namespace lbmibm
 
@@ -116,10 +117,8 @@ struct lbmibm_ctx {
     int *bicg_cells[3] = {nullptr, nullptr, nullptr};
     int bicg_n[3] = {0, 0, 0}, bicg_maxv[3] = {0, 0, 0};
     int bicg_var[3] = {0, 0, 0};  // shape variant of the on-chip classes, form of class 2 (autotuned)
-    int interp_form = 0;          // 0: tiles staged in LDS, 1: one thread per vertex, rows from memory,
-                                  // 2: tiles staged in LDS, four lanes per vertex (autotuned)
-    int spread_form = 0;          // 0: source bins per tile, 1: home boxes + gather, 2: home boxes by
-                                  // sorted stencil runs + gather (autotuned)
+    int interp_form = 0;          // 0: tiles staged in LDS, 1: one thread per vertex, rows from memory (autotuned)
+    int spread_form = 0;          // 0: source bins per tile, 1: home boxes + gather (autotuned)
     void *sbox = nullptr;         // [ntiles][3][SB_N] staging boxes of spread form 1
     int *sb_cand = nullptr;       // [(nx+ny+nz)*4] box entries per coordinate (spread form 1)
     int big_lds = 0;              // dynamic LDS of the class-2 forms that use LDS (the whole CU)
@@ -151,6 +150,10 @@ struct lbmibm_ctx {
     int *tile_cand = nullptr;                          // [ntiles] spreading candidates
     int *order_spread = nullptr, *order_interp = nullptr;  // [ntiles] heavy-first tile orders
     int *order_calls = nullptr;              // device counter of binnings (tile-order refresh)
+    int2 *chunks = nullptr;                  // [nchunk_max] (tile, chunk) work list of the interpolation
+    int *chunk_total = nullptr;              // its length (device)
+    int *tile_chunk = nullptr;               // [ntiles + 1] first chunk of each tile, then the total
+    int nchunk_max = 0;                      // ntiles + ceil(Nv / INTERP_CHUNK)
     float *blk_fmax = nullptr;               // [nblk_bin] max |f| per binning block
     unsigned *absmax = nullptr;
     bool bins_valid = false;                 // bins match the current positions

@@ -263,15 +263,21 @@ __device__ void heavy_first_order(const int *load, int n, int *hist, int *order)
     __syncthreads();
 }
 
+// The work list of the interpolation and of spreading form 1: each tile's
+// home vertices in chunks of INTERP_CHUNK, one work group per chunk, so that
+// no work group holds more than a chunk however the cells crowd the tiles.
+// chunks[c] = (tile, chunk of the tile); the first *chunk_total entries are
+// valid; the chunks of tile t are tile_chunk[t] .. tile_chunk[t + 1] - 1.
 __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile_count, int *__restrict__ tile_off,
                                                     int *__restrict__ tile_cur, int ntiles,
                                                     const float *__restrict__ blk_fmax, int nblk,
-                                                    unsigned *__restrict__ absmax) {
-    // exclusive scan over tiles in one work group, 1024 entries per round,
-    // and the maximum of the per-block force maxima
-    __shared__ int sh[1024];
+                                                    unsigned *__restrict__ absmax, int2 *__restrict__ chunks,
+                                                    int *__restrict__ chunk_total, int *__restrict__ tile_chunk) {
+    // exclusive scans over tiles (vertices, chunks) in one work group, 1024
+    // entries per round, and the maximum of the per-block force maxima
+    __shared__ int sh[1024], shc[1024];
     __shared__ float shm[1024];
-    __shared__ int carry;
+    __shared__ int carry, carryc;
     float fm = 0.0f;
     for (int i = threadIdx.x; i < nblk; i += 1024) fm = fmaxf(fm, blk_fmax[i]);
     shm[threadIdx.x] = fm;
@@ -281,25 +287,42 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile
         if (threadIdx.x < o) shm[threadIdx.x] = fmaxf(shm[threadIdx.x], shm[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) *absmax = __float_as_uint(shm[0]);
+    if (threadIdx.x == 0) {
+        *absmax = __float_as_uint(shm[0]);
+        carryc = 0;
+    }
     for (int base = 0; base < ntiles; base += 1024) {
         const int i = base + threadIdx.x;
         const int v = i < ntiles ? tile_count[i] : 0;
+        const int nch = (v + INTERP_CHUNK - 1) / INTERP_CHUNK;
         sh[threadIdx.x] = v;
+        shc[threadIdx.x] = nch;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
             const int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            const int tc = threadIdx.x >= o ? shc[threadIdx.x - o] : 0;
             __syncthreads();
             sh[threadIdx.x] += t;
+            shc[threadIdx.x] += tc;
             __syncthreads();
         }
         if (i < ntiles) {
             tile_off[i] = carry + sh[threadIdx.x] - v;
             tile_cur[i] = 0;
+            const int c0 = carryc + shc[threadIdx.x] - nch;
+            tile_chunk[i] = c0;
+            for (int k = 0; k < nch; k++) chunks[c0 + k] = make_int2(i, k);
         }
         __syncthreads();
-        if (threadIdx.x == 1023) carry += sh[1023];
+        if (threadIdx.x == 1023) {
+            carry += sh[1023];
+            carryc += shc[1023];
+        }
         __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *chunk_total = carryc;
+        tile_chunk[ntiles] = carryc;
     }
 }
 
@@ -588,15 +611,19 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const SpreadRec<T> *__rest
     }
 }
 
-// Spreading, form 1.  One work group per tile takes its home vertices only
-// and adds each one's whole 4x4x4 stencil into an LDS box of the tile plus
-// the 3 nodes above it on each axis (its stencils cannot leave that box), in
-// the fixed point of form 0, with no per-node masks; the box is then written
-// to a staging array, and k_spread_gather sums for each lattice node the box
-// entries that map to it.  Form 0 visits a vertex once per tile its stencil
-// touches (2.2 tiles on average with 16x8x8 tiles), each visit walking the 64
-// nodes under masks.  Lane l of wave w takes column NW l + w of the bin read
-// as WG columns, so that the lanes of a wave hold vertices far apart.
+// Spreading, form 1.  One work group per chunk of a tile's home vertices
+// (the interpolation's work list, at most INTERP_CHUNK vertices) adds each
+// one's whole 4x4x4 stencil into an LDS box of the tile plus the 3 nodes
+// above it on each axis (its stencils cannot leave that box), in the fixed
+// point of form 0, with no per-node masks; the box is then written to the
+// chunk's staging slot, and k_spread_gather sums for each lattice node the
+// box entries that map to it, over the chunks of each tile in order.  Form 0
+// visits a vertex once per tile its stencil touches (2.2 tiles on average
+// with 16x8x8 tiles), each visit walking the 64 nodes under masks.  Chunks
+// keep a crowded tile from holding one CU for the whole launch: its LDS
+// atomics spread over as many CUs as it has chunks.  Lane l of wave w takes
+// column NW l + w of the chunk read as WG columns, so that the lanes of a
+// wave hold vertices far apart.
 constexpr int SB_X = TILE_X + 3, SB_Y = TILE_Y + 3, SB_Z = TILE_Z + 3, SB_N = SB_X * SB_Y * SB_Z;
 constexpr int SPREAD_BOX_WG = 512;
 constexpr int SB_MAXC = 4;  // box entries per lattice coordinate and axis (k_spread_gather)
@@ -608,11 +635,14 @@ __global__ __launch_bounds__(SPREAD_BOX_WG) void k_spread_box(const SpreadRec<T>
                                                               const int *__restrict__ tile_off, T *__restrict__ S,
                                                               IbmGeom g, TileGeom tg,
                                                               const unsigned *__restrict__ absmax,
-                                                              const int *__restrict__ order) {
+                                                              const int2 *__restrict__ chunks,
+                                                              const int *__restrict__ chunk_total) {
     __shared__ unsigned long long acc[3 * SB_N];
-    const int tile = order[blockIdx.x];
-    const int cnt = tile_count[tile], off = tile_off[tile];
-    if (cnt == 0) return;  // whole work group leaves together; the gather skips empty tiles
+    if ((int)blockIdx.x >= *chunk_total) return;  // whole work group leaves together
+    const int2 ch = chunks[blockIdx.x];
+    const int tile = ch.x;
+    const int off = tile_off[tile] + ch.y * INTERP_CHUNK;
+    const int cnt = min(INTERP_CHUNK, tile_count[tile] - ch.y * INTERP_CHUNK);
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
     const int ox = tx * TILE_X, oy = ty * TILE_Y, oz = tz * TILE_Z;
     for (int k = threadIdx.x; k < 3 * SB_N; k += blockDim.x) acc[k] = 0ull;
@@ -660,222 +690,7 @@ __global__ __launch_bounds__(SPREAD_BOX_WG) void k_spread_box(const SpreadRec<T>
     }
     __syncthreads();
     const double inv_scale = 1.0 / scale;
-    T *St = S + (size_t)tile * 3 * SB_N;
-    for (int k = threadIdx.x; k < SB_N; k += blockDim.x)
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-            St[d * SB_N + k] = (T)((double)FixedPoint<T>::value(acc[d * SB_N + k]) * inv_scale);
-}
-
-// Form 2, the same staging boxes as k_spread_box with fewer LDS atomics: a
-// tile's home vertices are counting-sorted in LDS by their first stencil
-// node (the key), so the vertices that share a stencil are consecutive.  A
-// wave then walks a slice of the sorted list with one lane per stencil node
-// (kx, ky, kz) = (lane & 3, (lane >> 2) & 3, lane >> 4): each vertex adds its
-// term for that node to the lane's fixed-point registers, and the registers
-// go into the box with one atomic per component when the key changes.  A
-// 10242-vertex cell of radius 7 lu puts about a dozen vertices on one key,
-// so the box sees one atomic per run of equal keys instead of one per vertex.
-// The terms and their fixed-point values are those of k_spread_box, and the
-// integer sums do not depend on the order, so the boxes are bitwise equal.
-// The lanes of a slice's vertices (64 at a time) load their records and
-// compute their stencil weights in parallel first; the walk reads them with
-// readlane.  More home vertices than SEG_CAP are sorted SEG_CAP at a time.
-constexpr int SEG_WG = 512;
-constexpr int SEG_CAP = 4096;
-constexpr int SEG_PER = SEG_CAP / SEG_WG;
-constexpr int SEG_NKEY = TILE_X * (TILE_Y + 3) * TILE_Z;  // first node in the tile, y from 3 below
-
-template <typename T> __device__ __forceinline__ T lane_value(T v, int q);
-template <> __device__ __forceinline__ float lane_value(float v, int q) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));
-}
-template <> __device__ __forceinline__ double lane_value(double v, int q) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffull), q);
-    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), q);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-template <typename T> struct SegTerm;
-// FP32: the raw bits of fma(w, f scale, 1.5 2^52), summed as FixedPoint<float>
-template <> struct SegTerm<float> {
-    static __device__ __forceinline__ unsigned long long term(float w, double fs, double) {
-        return (unsigned long long)__double_as_longlong(fma((double)w, fs, 6755399441055744.0));
-    }
-    static __device__ __forceinline__ double prescale(float f, double scale) { return (double)f * scale; }
-};
-template <> struct SegTerm<double> {
-    static __device__ __forceinline__ unsigned long long term(double w, double f, double scale) {
-        return (unsigned long long)llrint(w * f * scale);
-    }
-    static __device__ __forceinline__ double prescale(double f, double) { return f; }
-};
-
-// Exclusive scan of hist[0..SEG_NKEY) by the work group; returns the total.
-__device__ __forceinline__ int seg_scan(int *hist, int *wtot) {
-    constexpr int PER = (SEG_NKEY + SEG_WG - 1) / SEG_WG;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int v[PER], s = 0;
-#pragma unroll
-    for (int r = 0; r < PER; r++) {
-        const int k = PER * threadIdx.x + r;
-        v[r] = k < SEG_NKEY ? hist[k] : 0;
-        s += v[r];
-    }
-    int inc = s;  // inclusive scan over the wave
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(inc, o);
-        if (lane >= o) inc += t;
-    }
-    if (lane == 63) wtot[wave] = inc;
-    __syncthreads();
-    int base = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < SEG_WG / 64; w++) {
-        base += w < wave ? wtot[w] : 0;
-        total += wtot[w];
-    }
-    int run = base + inc - s;
-#pragma unroll
-    for (int r = 0; r < PER; r++) {
-        const int k = PER * threadIdx.x + r;
-        if (k < SEG_NKEY) hist[k] = run;
-        run += v[r];
-    }
-    return total;
-}
-
-template <typename T>
-__global__ __launch_bounds__(SEG_WG) void k_spread_seg(const SpreadRec<T> *__restrict__ rec,
-                                                       const short4 *__restrict__ node0,
-                                                       const int *__restrict__ tile_count,
-                                                       const int *__restrict__ tile_off, T *__restrict__ S,
-                                                       IbmGeom g, TileGeom tg, const unsigned *__restrict__ absmax,
-                                                       const int *__restrict__ order) {
-    __shared__ unsigned long long acc[3 * SB_N];
-    __shared__ int hist[SEG_NKEY];
-    __shared__ unsigned short list[SEG_CAP];
-    __shared__ int wtot[SEG_WG / 64];
-    const int tile = order[blockIdx.x];
-    const int cnt = tile_count[tile], off = tile_off[tile];
-    if (cnt == 0) return;  // whole work group leaves together; the gather skips empty tiles
-    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
-    const int ox = tx * TILE_X, oy = ty * TILE_Y, oz = tz * TILE_Z;
-    for (int k = threadIdx.x; k < 3 * SB_N; k += blockDim.x) acc[k] = 0ull;
-    const double fmax = (double)__uint_as_float(*absmax);
-    double scale = 1.0;
-    if (fmax > 0.0) {  // as k_spread_box
-        const double bound = (double)cnt * fmax / (8.0 * g.dx * g.dx * g.dx);
-        scale = ldexp(1.0, FixedPoint<T>::kBits - 1 - ilogb(bound));
-    }
-    const T w64 = (T)(1.0 / (64.0 * g.dx * g.dx * g.dx));
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
-    for (int c0 = 0; c0 < cnt; c0 += SEG_CAP) {
-        const int m = min(SEG_CAP, cnt - c0);
-        for (int k = threadIdx.x; k < SEG_NKEY; k += SEG_WG) hist[k] = 0;
-        __syncthreads();
-        int key[SEG_PER], rank[SEG_PER];
-#pragma unroll
-        for (int r = 0; r < SEG_PER; r++) {
-            const int i = threadIdx.x + r * SEG_WG;
-            key[r] = -1;
-            rank[r] = 0;
-            if (i < m) {
-                const short4 n0 = node0[off + c0 + i];
-                const int bx = n0.x - ox, bz = n0.z - oz;
-                int y1 = n0.y - 3;
-                if (!g.walls) wrap_index(y1, g.ny);
-                const int by = y1 - oy;
-                if (bx >= 0 && bx < TILE_X && bz >= 0 && bz < TILE_Z && by >= -3 && by < TILE_Y) {
-                    key[r] = bx + TILE_X * ((by + 3) + (TILE_Y + 3) * bz);
-                    rank[r] = atomicAdd(&hist[key[r]], 1);
-                }
-            }
-        }
-        __syncthreads();
-        const int total = seg_scan(hist, wtot);
-        __syncthreads();  // every offset written before the scatter reads them
-#pragma unroll
-        for (int r = 0; r < SEG_PER; r++)
-            if (key[r] >= 0) list[hist[key[r]] + rank[r]] = (unsigned short)(threadIdx.x + r * SEG_WG);
-        __syncthreads();
-        // contiguous slices of the sorted list, one per wave
-        const int per = (total + SEG_WG / 64 - 1) / (SEG_WG / 64);
-        const int b0 = min(total, wave * per), b1 = min(total, b0 + per);
-        unsigned long long a0 = 0ull, a1 = 0ull, a2 = 0ull;
-        int cur = -1;
-        for (int base = b0; base < b1; base += 64) {
-            const int j = base + lane;
-            int kj = -1;
-            T wx[4], wy[4], wz[4];
-            double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-            if (j < b1) {
-                const int i = list[j];
-                const short4 n0 = node0[off + c0 + i];
-                const SpreadRec<T> sr = rec[off + c0 + i];
-                int y1 = n0.y - 3;
-                if (!g.walls) wrap_index(y1, g.ny);
-                kj = (n0.x - ox) + TILE_X * ((y1 - oy + 3) + (TILE_Y + 3) * (n0.z - oz));
-                stencil_weights<T>(sr.t[0], wx);
-                stencil_weights<T>(sr.t[1], wy);
-                stencil_weights<T>(sr.t[2], wz);
-                f0 = SegTerm<T>::prescale(sr.f[0], scale);
-                f1 = SegTerm<T>::prescale(sr.f[1], scale);
-                f2 = SegTerm<T>::prescale(sr.f[2], scale);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; k++) wx[k] = wy[k] = wz[k] = T(0);
-            }
-            // this lane's weight of each axis, for the vertex of lane q below
-            const int nb = min(64, b1 - base);
-            for (int q = 0; q < nb; q++) {
-                const int kq = __builtin_amdgcn_readlane(kj, q);
-                if (kq != cur) {
-                    if (cur >= 0) {
-                        const int bx = cur % TILE_X, by = (cur / TILE_X) % (TILE_Y + 3) - 3,
-                                  bz = cur / (TILE_X * (TILE_Y + 3));
-                        if (by + ky >= 0) {  // below the wall otherwise
-                            const int li = bx + kx + SB_X * ((by + ky) + SB_Y * (bz + kz));
-                            atomicAdd(&acc[li], a0);
-                            atomicAdd(&acc[SB_N + li], a1);
-                            atomicAdd(&acc[2 * SB_N + li], a2);
-                        }
-                    }
-                    cur = kq;
-                    a0 = a1 = a2 = 0ull;
-                }
-                const T sx0 = lane_value(wx[0], q), sx1 = lane_value(wx[1], q), sx2 = lane_value(wx[2], q),
-                        sx3 = lane_value(wx[3], q);
-                const T sy0 = lane_value(wy[0], q), sy1 = lane_value(wy[1], q), sy2 = lane_value(wy[2], q),
-                        sy3 = lane_value(wy[3], q);
-                const T sz0 = lane_value(wz[0], q), sz1 = lane_value(wz[1], q), sz2 = lane_value(wz[2], q),
-                        sz3 = lane_value(wz[3], q);
-                const T ux = kx == 0 ? sx0 : kx == 1 ? sx1 : kx == 2 ? sx2 : sx3;
-                const T uy = ky == 0 ? sy0 : ky == 1 ? sy1 : ky == 2 ? sy2 : sy3;
-                const T uz = kz == 0 ? sz0 : kz == 1 ? sz1 : kz == 2 ? sz2 : sz3;
-                const T wyz = uy * uz * w64;
-                const T w = ux * wyz;
-                a0 += SegTerm<T>::term(w, lane_value(f0, q), scale);
-                a1 += SegTerm<T>::term(w, lane_value(f1, q), scale);
-                a2 += SegTerm<T>::term(w, lane_value(f2, q), scale);
-            }
-        }
-        if (cur >= 0) {
-            const int bx = cur % TILE_X, by = (cur / TILE_X) % (TILE_Y + 3) - 3, bz = cur / (TILE_X * (TILE_Y + 3));
-            if (by + ky >= 0) {
-                const int li = bx + kx + SB_X * ((by + ky) + SB_Y * (bz + kz));
-                atomicAdd(&acc[li], a0);
-                atomicAdd(&acc[SB_N + li], a1);
-                atomicAdd(&acc[2 * SB_N + li], a2);
-            }
-        }
-        __syncthreads();  // hist and list are reused by the next pass
-    }
-    const double inv_scale = 1.0 / scale;
-    T *St = S + (size_t)tile * 3 * SB_N;
+    T *St = S + (size_t)blockIdx.x * 3 * SB_N;
     for (int k = threadIdx.x; k < SB_N; k += blockDim.x)
 #pragma unroll
         for (int d = 0; d < 3; d++)
@@ -883,11 +698,13 @@ __global__ __launch_bounds__(SEG_WG) void k_spread_seg(const SpreadRec<T> *__res
 }
 
 // Sums, per lattice node, the box entries of k_spread_box that map to it, in
-// a fixed order (deterministic).  cx/cy/cz list for each coordinate of an axis
-// up to SB_MAXC entries (tile index along the axis) * 32 + box coordinate,
-// padded with -1 (built once on the host).
+// a fixed order (deterministic): per (z, y, x) entry the chunk boxes of its
+// tile in chunk order.  cx/cy/cz list for each coordinate of an axis up to
+// SB_MAXC entries (tile index along the axis) * 32 + box coordinate, padded
+// with -1 (built once on the host); tile_chunk[t] is the first chunk of tile
+// t, tile_chunk[t + 1] the end.
 template <typename T>
-__global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, const int *__restrict__ tile_count,
+__global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, const int *__restrict__ tile_chunk,
                                                        const int *__restrict__ cx, const int *__restrict__ cy,
                                                        const int *__restrict__ cz, T *__restrict__ F, IbmGeom g,
                                                        TileGeom tg) {
@@ -895,9 +712,9 @@ __global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, 
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((size_t)g.nx * g.ny));
-    // all table entries first, then per (z, y) entry the tile counts and box
-    // values of every x entry, so that independent loads are in flight
-    // together; the sums keep the order (z, y, x entry)
+    // all table entries first, then per (z, y) entry the chunk ranges and the
+    // first chunk's box values of every x entry, so that independent loads
+    // are in flight together; the sums keep the order (z, y, x entry, chunk)
     int ex[SB_MAXC], ey[SB_MAXC], ez[SB_MAXC];
 #pragma unroll
     for (int i = 0; i < SB_MAXC; i++) {
@@ -914,23 +731,36 @@ __global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, 
             if (ey[iy] < 0) break;
             const int tyz = tg.ntx * ((ey[iy] >> 5) + tg.nty * (ez[iz] >> 5));
             const int byz = SB_X * ((ey[iy] & 31) + SB_Y * (ez[iz] & 31));
-            int cnt[SB_MAXC];
+            int c0[SB_MAXC], c1[SB_MAXC];
 #pragma unroll
-            for (int ix = 0; ix < SB_MAXC; ix++) cnt[ix] = ex[ix] >= 0 ? tile_count[(ex[ix] >> 5) + tyz] : 0;
+            for (int ix = 0; ix < SB_MAXC; ix++) {
+                c0[ix] = c1[ix] = 0;
+                if (ex[ix] >= 0) {
+                    const int t = (ex[ix] >> 5) + tyz;
+                    c0[ix] = tile_chunk[t];
+                    c1[ix] = tile_chunk[t + 1];
+                }
+            }
             T v0[SB_MAXC], v1[SB_MAXC], v2[SB_MAXC];
 #pragma unroll
             for (int ix = 0; ix < SB_MAXC; ix++) {
                 v0[ix] = v1[ix] = v2[ix] = T(0);
-                if (cnt[ix] > 0) {
-                    const size_t b = (size_t)((ex[ix] >> 5) + tyz) * 3 * SB_N + (ex[ix] & 31) + byz;
+                if (c0[ix] < c1[ix]) {
+                    const size_t b = (size_t)c0[ix] * 3 * SB_N + (ex[ix] & 31) + byz;
                     v0[ix] = S[b];
                     v1[ix] = S[b + SB_N];
                     v2[ix] = S[b + 2 * SB_N];
                 }
             }
 #pragma unroll
-            for (int ix = 0; ix < SB_MAXC; ix++)
-                if (cnt[ix] > 0) { a0 += v0[ix]; a1 += v1[ix]; a2 += v2[ix]; }
+            for (int ix = 0; ix < SB_MAXC; ix++) {
+                if (c0[ix] >= c1[ix]) continue;
+                a0 += v0[ix]; a1 += v1[ix]; a2 += v2[ix];
+                for (int c = c0[ix] + 1; c < c1[ix]; c++) {  // the tile's further chunks
+                    const size_t b = (size_t)c * 3 * SB_N + (ex[ix] & 31) + byz;
+                    a0 += S[b]; a1 += S[b + SB_N]; a2 += S[b + 2 * SB_N];
+                }
+            }
         }
     }
     F[n] = a0;
@@ -1132,20 +962,26 @@ __device__ __forceinline__ void advect_vertex(int j, const double *xin, double *
     xd[3 * (size_t)j] = xr; xd[3 * (size_t)j + 1] = yr; xd[3 * (size_t)j + 2] = zr;
 }
 
-// Interpolation form 0: one work group per tile stages the tile's velocity
-// plus halo in LDS once and runs its home vertices from there.
+// Interpolation form 0: one work group per chunk of a tile's home vertices
+// (at most INTERP_CHUNK) stages the tile's velocity plus halo in LDS and runs
+// the chunk's vertices from there.  The chunks of one tile stage the same
+// box, which the L2 serves after the first; a crowded tile then spreads
+// over several CUs instead of holding one for the whole launch.
 template <typename T>
 __global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, const double *xin, double *xo,
                                                       T *__restrict__ uvw, double *__restrict__ xd,
                                                       const int *__restrict__ tile_count,
                                                       const int *__restrict__ tile_off,
                                                       const int *__restrict__ home_list, int nv, IbmGeom g,
-                                                      TileGeom tg, AdvectArgs a, const int *__restrict__ order) {
+                                                      TileGeom tg, AdvectArgs a, const int2 *__restrict__ chunks,
+                                                      const int *__restrict__ chunk_total) {
     extern __shared__ double dyn[];
     T *Us = (T *)dyn;  // [3][RZ][RY][RX]
-    const int tile = order[blockIdx.x];
-    const int cnt = tile_count[tile], off = tile_off[tile];
-    if (cnt == 0) return;  // whole work group leaves together
+    if ((int)blockIdx.x >= *chunk_total) return;  // whole work group leaves together
+    const int2 ch = chunks[blockIdx.x];
+    const int tile = ch.x;
+    const int off = tile_off[tile] + ch.y * INTERP_CHUNK;
+    const int cnt = min(INTERP_CHUNK, tile_count[tile] - ch.y * INTERP_CHUNK);
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
     const int bx = tx * TILE_X - RLO, by = ty * TILE_Y - RLO, bz = tz * TILE_Z - RLO;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
@@ -1179,165 +1015,6 @@ template <typename T> __device__ __forceinline__ T quad_sum(T v) {
     v += __shfl_xor(v, 1);
     v += __shfl_xor(v, 2);
     return v;
-}
-
-// Broadcast of lane a of each aligned quad to the quad (DPP quad_perm).
-template <int A> __device__ __forceinline__ int quad_bcast(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, A | (A << 2) | (A << 4) | (A << 6), 0xf, 0xf, false);
-}
-template <int A> __device__ __forceinline__ float quad_bcast(float v) {
-    return __int_as_float(quad_bcast<A>(__float_as_int(v)));
-}
-template <int A> __device__ __forceinline__ double quad_bcast(double v) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const int lo = quad_bcast<A>((int)(unsigned)(u & 0xffffffffull)), hi = quad_bcast<A>((int)(unsigned)(u >> 32));
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-// Interpolation form 2: the staged box of form 0, and four lanes per home
-// vertex.  Lane kz of a quad holds the kz-th z-slice of the vertex's stencil
-// (16 nodes x 3 components) in registers, loaded once, and the RK stages
-// reuse it while their first stencil node stays where it was: a stage moves
-// a vertex by |u| M dt << dx, so the four stages nearly always share it, and
-// the box is read a quarter as often as by form 0.  A stage on another node
-// reloads the slice, from the box or, outside it, from memory.  The stencil
-// of axis a (first node, sin and cos of its fraction) is computed by lane a
-// of the quad and broadcast; quad_sum completes each component's sum in all
-// four lanes, so they follow the same stage chain, and lane a writes
-// coordinate a.  The products are those of form 0's box path; the sums are
-// grouped by slice ((s0 + s1) + (s2 + s3)), so results agree with form 0 to
-// rounding.
-template <typename T>
-__global__ __launch_bounds__(256) void k_interp_quad(const T *__restrict__ U, const double *xin, double *xo,
-                                                     T *__restrict__ uvw, double *__restrict__ xd,
-                                                     const int *__restrict__ tile_count,
-                                                     const int *__restrict__ tile_off,
-                                                     const int *__restrict__ home_list, int nv, IbmGeom g,
-                                                     TileGeom tg, AdvectArgs a, const int *__restrict__ order) {
-    extern __shared__ double dyn[];
-    T *Us = (T *)dyn;  // [3][RZ][RY][RX]
-    const int tile = order[blockIdx.x];
-    const int cnt = tile_count[tile], off = tile_off[tile];
-    if (cnt == 0) return;  // whole work group leaves together
-    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
-    const int bx = tx * TILE_X - RLO, by = ty * TILE_Y - RLO, bz = tz * TILE_Z - RLO;
-    const size_t N = (size_t)g.nx * g.ny * g.nz;
-    for (int k = threadIdx.x; k < RN; k += blockDim.x) {
-        const int rx = k % RX, ry = (k / RX) % RY, rz = k / (RX * RY);
-        const int ix = pmod(bx + rx, g.nx), iz = pmod(bz + rz, g.nz);
-        int iy = by + ry;
-        bool ok = true;
-        if (g.walls) ok = iy >= 0 && iy < g.ny;
-        else iy = pmod(iy, g.ny);
-        T u = T(0), v = T(0), w = T(0);
-        if (ok) {
-            const size_t n = (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz);
-            u = U[n]; v = U[N + n]; w = U[2 * N + n];
-        }
-        Us[k] = u; Us[RN + k] = v; Us[2 * RN + k] = w;
-    }
-    __syncthreads();
-    const int sub = threadIdx.x & 3;           // z-slice of the stencil, and the axis of lanes 0..2
-    const int ax = sub < 3 ? sub : 0;
-    const int npad = ax == 0 ? g.nxp : (ax == 1 ? g.nyp : g.nzp);
-    const bool unit = a.rk != 4;
-    const double shift = unit ? (double)(npad - 1) / 2.0 : (double)(npad - 1) * g.dx / 2.0;
-    const double M = a.M, dt = a.dt;
-    const int ns = a.rk == 4 ? 4 : (a.rk == 2 ? 2 : 1);
-    const T den = a.rk == 4 ? T(6) : (a.rk == 2 ? T(2) : T(1));
-    for (int i = threadIdx.x >> 2; i < cnt; i += blockDim.x >> 2) {  // the four lanes of a quad share i
-        const int j = home_list[off + i];
-        const double Xa = xin[(size_t)ax * nv + j];  // this lane's axis
-        int lx1 = -0x40000000, ly1 = 0, lz1 = 0;     // first node of the loaded slice (none yet)
-        T sl[3][4][4];
-        T ku = T(0), kv = T(0), kw = T(0), su = T(0), sv = T(0), sw = T(0);
-#pragma unroll 1
-        for (int s = 0; s < ns; s++) {
-            double o = 0.0;
-            if (s > 0) {
-                o = (double)(ax == 0 ? ku : (ax == 1 ? kv : kw)) * M;
-                if (a.rk == 4) {
-                    o *= dt;
-                    if (s < 3) o /= 2.0;
-                }
-            }
-            // the stencil of this lane's axis (as stencil3)
-            const double xtr = (Xa + o) + shift;
-            const double q = unit ? xtr : over_h(xtr, g.dx);
-            const double fl = floor(q);
-            const int first = (int)fl - 1 - 2;
-            T sn, cs;
-            sincos_quarter((T)(q - fl), &sn, &cs);
-            const int x1 = quad_bcast<0>(first), y1 = quad_bcast<1>(first), z1 = quad_bcast<2>(first);
-            const T sx = quad_bcast<0>(sn), cx = quad_bcast<0>(cs), sy = quad_bcast<1>(sn), cy = quad_bcast<1>(cs),
-                    sz = quad_bcast<2>(sn), cz = quad_bcast<2>(cs);
-            const T wx[4] = {T(1) - sx, T(1) + cx, T(1) + sx, T(1) - cx};
-            const T wy[4] = {T(1) - sy, T(1) + cy, T(1) + sy, T(1) - cy};
-            const T wzk = sub == 0 ? T(1) - sz : (sub == 1 ? T(1) + cz : (sub == 2 ? T(1) + sz : T(1) - cz));
-            if (x1 != lx1 || y1 != ly1 || z1 != lz1) {  // (re)load the slice
-                lx1 = x1; ly1 = y1; lz1 = z1;
-                int rx = x1 - bx, rz = z1 - bz, ry = y1 - by;
-                if (rx < 0) rx += g.nx; else if (rx >= g.nx) rx -= g.nx;
-                if (rz < 0) rz += g.nz; else if (rz >= g.nz) rz -= g.nz;
-                if (!g.walls) { if (ry < 0) ry += g.ny; else if (ry >= g.ny) ry -= g.ny; }
-                if (rx >= 0 && rx + 3 < RX && ry >= 0 && ry + 3 < RY && rz >= 0 && rz + 3 < RZ) {
-                    const int base = rx + RX * (ry + RY * (rz + sub));
-#pragma unroll
-                    for (int ky = 0; ky < 4; ky++)
-#pragma unroll
-                        for (int kx = 0; kx < 4; kx++)
-#pragma unroll
-                            for (int c = 0; c < 3; c++) sl[c][ky][kx] = Us[c * RN + base + RX * ky + kx];
-                } else {  // outside the box: the slice from memory, zero beyond the walls
-                    int iz = z1 + sub;
-                    const bool okz = wrap_index(iz, g.nz);
-#pragma unroll
-                    for (int ky = 0; ky < 4; ky++) {
-                        int iy = y1 + ky;
-                        bool oky = okz;
-                        if (g.walls) oky = oky && iy >= 0 && iy < g.ny;
-                        else oky = oky && wrap_index(iy, g.ny);
-#pragma unroll
-                        for (int kx = 0; kx < 4; kx++) {
-                            int ix = x1 + kx;
-                            const bool ok = oky && wrap_index(ix, g.nx);
-                            const size_t n = ok ? (size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz) : 0;
-#pragma unroll
-                            for (int c = 0; c < 3; c++) sl[c][ky][kx] = ok ? U[c * N + n] : T(0);
-                        }
-                    }
-                }
-            }
-            const T wz64 = wzk * T(1.0 / 64.0);
-            T pu = T(0), pv = T(0), pw = T(0);
-#pragma unroll
-            for (int ky = 0; ky < 4; ky++) {
-                const T wyz = wy[ky] * wz64;
-#pragma unroll
-                for (int kx = 0; kx < 4; kx++) {
-                    const T dlt = wx[kx] * wyz;
-                    pu += dlt * sl[0][ky][kx];
-                    pv += dlt * sl[1][ky][kx];
-                    pw += dlt * sl[2][ky][kx];
-                }
-            }
-            ku = quad_sum(pu); kv = quad_sum(pv); kw = quad_sum(pw);
-            const T wt = (a.rk == 4 && (s == 1 || s == 2)) ? T(2) : T(1);
-            su += wt * ku; sv += wt * kv; sw += wt * kw;
-        }
-        if (sub < 3) {  // lane a: coordinate a (as advect_vertex)
-            const T mA = (sub == 0 ? su : (sub == 1 ? sv : sw)) / den;
-            double r = Xa + (double)mA * M * dt;
-            const double L = sub == 0 ? a.lx : (sub == 1 ? a.ly : a.lz);
-            if (sub != 1 || a.periodic_y) {
-                if (r < -L / 2.0) r += L;
-                if (r > L / 2.0) r -= L;
-            }
-            if (uvw) uvw[(size_t)sub * nv + j] = mA;
-            xo[(size_t)sub * nv + j] = r;
-            xd[3 * (size_t)j + sub] = r;
-        }
-    }
 }
 
 // Interpolation form 1: one thread per vertex in bin order (neighbouring
@@ -1762,7 +1439,8 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
                        (const T *)spread_force(ctx), nv, g, tg, ctx->home, ctx->tile_count, ctx->blk_fmax);
     LBMIBM_LAUNCHED(ctx, "k_bin_count");
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->tile_off, ctx->tile_cur,
-                       ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax);
+                       ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax, ctx->chunks, ctx->chunk_total,
+                       ctx->tile_chunk);
     LBMIBM_LAUNCHED(ctx, "k_tile_scan");
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->ntiles, g, tg,
                        ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->order_calls);
@@ -1780,23 +1458,15 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
 template <typename T> lbmibm_status spread_launch(lbmibm_ctx *ctx, int form) {
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const IbmGeom g = geom_of(ctx);
-    if (form == 1 || form == 2) {
-        if (form == 1) {
-            LBMIBM_KT(ctx, ctx->stream, "k_spread_box",
-                      hipLaunchKernelGGL(k_spread_box<T>, dim3(ctx->ntiles), dim3(SPREAD_BOX_WG), 0, ctx->stream,
-                                         (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count,
-                                         ctx->tile_off, (T *)ctx->sbox, g, tg, ctx->absmax, ctx->order_interp));
-            LBMIBM_LAUNCHED(ctx, "k_spread_box");
-        } else {
-            LBMIBM_KT(ctx, ctx->stream, "k_spread_seg",
-                      hipLaunchKernelGGL(k_spread_seg<T>, dim3(ctx->ntiles), dim3(SEG_WG), 0, ctx->stream,
-                                         (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count,
-                                         ctx->tile_off, (T *)ctx->sbox, g, tg, ctx->absmax, ctx->order_interp));
-            LBMIBM_LAUNCHED(ctx, "k_spread_seg");
-        }
+    if (form == 1) {
+        LBMIBM_KT(ctx, ctx->stream, "k_spread_box",
+                  hipLaunchKernelGGL(k_spread_box<T>, dim3(ctx->nchunk_max), dim3(SPREAD_BOX_WG), 0, ctx->stream,
+                                     (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off,
+                                     (T *)ctx->sbox, g, tg, ctx->absmax, ctx->chunks, ctx->chunk_total));
+        LBMIBM_LAUNCHED(ctx, "k_spread_box");
         LBMIBM_KT(ctx, ctx->stream, "k_spread_gather",
                   hipLaunchKernelGGL(k_spread_gather<T>, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0,
-                                     ctx->stream, (const T *)ctx->sbox, ctx->tile_count, ctx->sb_cand,
+                                     ctx->stream, (const T *)ctx->sbox, ctx->tile_chunk, ctx->sb_cand,
                                      ctx->sb_cand + SB_MAXC * ctx->nx, ctx->sb_cand + SB_MAXC * (ctx->nx + ctx->ny),
                                      (T *)ctx->F, g, tg));
         LBMIBM_LAUNCHED(ctx, "k_spread_gather");
@@ -1843,18 +1513,10 @@ lbmibm_status interp_launch(lbmibm_ctx *ctx, int form, const double *xin, double
     }
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const size_t lds = 3 * (size_t)RN * sizeof(T);
-    if (form == 2) {
-        LBMIBM_KT(ctx, ctx->stream, "k_interp_quad",
-                  hipLaunchKernelGGL(k_interp_quad<T>, dim3(ctx->ntiles), dim3(256), lds, ctx->stream,
-                                     (const T *)ctx->U, xin, xo, uvw, xd, ctx->tile_count, ctx->tile_off,
-                                     ctx->home_list, ctx->nv, geom_of(ctx), tg, a, ctx->order_interp));
-        LBMIBM_LAUNCHED(ctx, "k_interp_quad");
-        return LBMIBM_OK;
-    }
     LBMIBM_KT(ctx, ctx->stream, "k_interp_tiles",
-              hipLaunchKernelGGL(k_interp_tiles<T>, dim3(ctx->ntiles), dim3(256), lds, ctx->stream,
+              hipLaunchKernelGGL(k_interp_tiles<T>, dim3(ctx->nchunk_max), dim3(256), lds, ctx->stream,
                                  (const T *)ctx->U, xin, xo, uvw, xd, ctx->tile_count, ctx->tile_off, ctx->home_list,
-                                 ctx->nv, geom_of(ctx), tg, a, ctx->order_interp));
+                                 ctx->nv, geom_of(ctx), tg, a, ctx->chunks, ctx->chunk_total));
     LBMIBM_LAUNCHED(ctx, "k_interp_tiles");
     return LBMIBM_OK;
 }
@@ -1889,10 +1551,10 @@ template <typename T> lbmibm_status interp_autotune_impl(lbmibm_ctx *ctx) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     lbmibm_status s = LBMIBM_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) s = LBMIBM_ERR_HIP;
-    static const char *names[3] = {"tiles in LDS", "rows from memory", "tiles in LDS, four lanes per vertex"};
+    static const char *names[2] = {"tiles in LDS", "rows from memory"};
     char line[160];
     float best = 0.0f;
-    for (int f = 0; f < 3 && s == LBMIBM_OK; f++) {
+    for (int f = 0; f < 2 && s == LBMIBM_OK; f++) {
         s = interp_launch<T>(ctx, f, (const double *)ctx->xs, xo, (T *)nullptr, xd);  // warm-up
         if (s != LBMIBM_OK) break;
         if (hipEventRecord(e0, ctx->stream) != hipSuccess) { s = LBMIBM_ERR_HIP; break; }
@@ -1972,10 +1634,6 @@ lbmibm_status ibm_setup(lbmibm_ctx *ctx) {
                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds64));
     LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_interp_tiles<float>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds32));
-    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_interp_quad<double>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds64));
-    LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_interp_quad<float>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds32));
     LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_advect_adj_scatter<double>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds64));
     LBMIBM_HIP(ctx, hipFuncSetAttribute((const void *)k_advect_adj_scatter<float>,
@@ -2031,7 +1689,7 @@ static lbmibm_status spread_box_setup(lbmibm_ctx *ctx, bool *ok) {
         LBMIBM_HIP(ctx, hipMemcpy(ctx->sb_cand, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     if (!ctx->sbox) {
-        lbmibm_status s = dev_alloc(ctx, &ctx->sbox, (size_t)ctx->ntiles * 3 * SB_N * ctx->szT);
+        lbmibm_status s = dev_alloc(ctx, &ctx->sbox, (size_t)ctx->nchunk_max * 3 * SB_N * ctx->szT);
         if (s != LBMIBM_OK) return s;
     }
     *ok = true;
@@ -2048,11 +1706,10 @@ template <typename T> lbmibm_status spread_autotune_impl(lbmibm_ctx *ctx) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     lbmibm_status s = LBMIBM_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) s = LBMIBM_ERR_HIP;
-    static const char *names[3] = {"source bins per tile", "home boxes + gather",
-                                   "home boxes by sorted stencil runs + gather"};
+    static const char *names[2] = {"source bins per tile", "home boxes + gather"};
     char line[160];
     float best = 0.0f;
-    for (int f = 0; f < 3 && s == LBMIBM_OK; f++) {
+    for (int f = 0; f < 2 && s == LBMIBM_OK; f++) {
         s = spread_launch<T>(ctx, f);  // warm-up
         if (s != LBMIBM_OK) break;
         if (hipEventRecord(e0, ctx->stream) != hipSuccess) { s = LBMIBM_ERR_HIP; break; }
@@ -2076,7 +1733,7 @@ template <typename T> lbmibm_status spread_autotune_impl(lbmibm_ctx *ctx) {
     return s;
 }
 
-// a form fixed in the environment: "0" .. max into *form, -1 when unset
+// a form fixed in the environment: "0" or "1" into *form, -1 when unset
 static lbmibm_status env_form(const char *name, int *form, char max = '1') {
     *form = -1;
     const char *v = getenv(name);
@@ -2090,16 +1747,16 @@ lbmibm_status ibm_autotune(lbmibm_ctx *ctx) {
     ctx->interp_form = 0;
     ctx->spread_form = 0;
     if (ctx->nv == 0) return LBMIBM_OK;
-    // LBMIBM_INTERP_FORM = 0|1|2 and LBMIBM_SPREAD_FORM = 0|1|2 fix a form;
+    // LBMIBM_INTERP_FORM / LBMIBM_SPREAD_FORM = 0|1 fix a form;
     // LBMIBM_AUTOTUNE=0 keeps forms 0
     int fi = -1, fs = -1;
-    lbmibm_status s = env_form("LBMIBM_INTERP_FORM", &fi, '2');
-    if (s == LBMIBM_OK) s = env_form("LBMIBM_SPREAD_FORM", &fs, '2');
+    lbmibm_status s = env_form("LBMIBM_INTERP_FORM", &fi);
+    if (s == LBMIBM_OK) s = env_form("LBMIBM_SPREAD_FORM", &fs);
     if (s != LBMIBM_OK) return s;
     bool box_ok = false;
     s = spread_box_setup(ctx, &box_ok);
     if (s != LBMIBM_OK) return s;
-    if (fs >= 1 && !box_ok) return LBMIBM_ERR_ARG;  // the lattice is too small for the boxes
+    if (fs == 1 && !box_ok) return LBMIBM_ERR_ARG;  // the lattice is too small for the boxes
     const char *env = getenv("LBMIBM_AUTOTUNE");
     const bool tune = !(env && env[0] == '0');
     if (fi >= 0) {

@@ -70,15 +70,21 @@ template <typename T> __device__ __forceinline__ T tube_interp(double r, T f1, T
     return f1;
 }
 
-template <typename S, typename T, bool HASF, bool MACROS, int GEOM>
+// S: population storage, T: the lattice fields (F, H, U, macros), A: the
+// arithmetic of the collision.  FP32 storage collides in FP64: the kernel is
+// bound by its HBM traffic, which S and T set, and FP64 arithmetic removes
+// the drift that FP32 rounding of the collision accumulates over thousands
+// of steps (about 1e-8 of the velocity per step on a smooth flow), leaving
+// the rounding of the stored deviations, which does not drift.
+template <typename S, typename T, typename A, bool HASF, bool MACROS, int GEOM>
 __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__restrict__ fout, LbmDims d,
                                             const T *__restrict__ F, const T *__restrict__ H, T *__restrict__ U,
-                                            T *__restrict__ Ui, T *__restrict__ rho_out, T *__restrict__ rhom, T om0,
-                                            T inv_cr, T pex, T pey, T pez, T B, const DevStep *__restrict__ st,
-                                            int m, const double *__restrict__ vf, T l1m1, T df,
+                                            T *__restrict__ Ui, T *__restrict__ rho_out, T *__restrict__ rhom, A om0,
+                                            A inv_cr, A pex, A pey, A pez, A B, const DevStep *__restrict__ st,
+                                            int m, const double *__restrict__ vf, A l1m1, A df,
                                             const unsigned *__restrict__ tube_col,
-                                            const double *__restrict__ tube_r, T *__restrict__ mw, T mw_lo,
-                                            T mw_hi) {
+                                            const double *__restrict__ tube_r, T *__restrict__ mw, A mw_lo,
+                                            A mw_hi) {
     constexpr bool WALLS = GEOM == GEOM_SLAB || GEOM == GEOM_SHEAR, MOVING = GEOM == GEOM_SHEAR;
     const int lb = xcd_block(d.nblocks);
     if (lb >= d.nblocks) return;
@@ -109,7 +115,7 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         if (!(cut & 1u)) return;  // a wall node: never updated
     }
 
-    T g[Q];
+    A g[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const unsigned sx = cx(q) == 1 ? xm : (cx(q) == -1 ? xp : ix);
@@ -118,21 +124,21 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         const bool bb = (WALLS && ((cy(q) == 1 && wall_lo) || (cy(q) == -1 && wall_hi))) ||
                         (GEOM == GEOM_TUBE && q > 0 && ((cut >> q) & 1u));
         const size_t src = bb ? (size_t)opp(q) * N + i : (size_t)q * N + (sx + nx * (sy + ny * sz));
-        g[q] = Pop<S, T>::ld(fin + src, bb ? opp(q) : q);
+        g[q] = Pop<S, A>::ld(fin + src, bb ? opp(q) : q);
     }
     // sliding walls: Ladd's term on the links bounced at a wall moving along
     // z, 6 w_q rho Cr (c_q . u_w) with the rho and u_w of the step that
     // collided this node (lbm.cu:951-970), kept per wall-row node in mw
     const size_t wcol = (size_t)ix + (size_t)nx * iz;
     if (MOVING && wall_lo) {
-        const T m = mw[wcol];
-        g[11] += wq<T>(11) * m;  // c = (0, 1, 1)
-        g[14] -= wq<T>(14) * m;  // c = (0, 1, -1)
+        const A m = (A)mw[wcol];
+        g[11] += wq<A>(11) * m;  // c = (0, 1, 1)
+        g[14] -= wq<A>(14) * m;  // c = (0, 1, -1)
     }
     if (MOVING && wall_hi) {
-        const T m = mw[(size_t)nx * nz + wcol];
-        g[12] -= wq<T>(12) * m;  // c = (0, -1, -1)
-        g[13] += wq<T>(13) * m;  // c = (0, -1, 1)
+        const A m = (A)mw[(size_t)nx * nz + wcol];
+        g[12] -= wq<A>(12) * m;  // c = (0, -1, -1)
+        g[13] += wq<A>(13) * m;  // c = (0, -1, 1)
     }
     if (GEOM == GEOM_TUBE && tube_r && (cut & ~1u)) {
         // g[q] of a cut link holds the half-way value (f1); f2 is the value
@@ -142,26 +148,26 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         // all loads first (each under its lane mask), then the arithmetic,
         // so that a wave waits once rather than once per cut direction
         const double *rt = tube_r + (size_t)(ix + nx * iy) * Q;
-        T own[Q];
+        A own[Q];
         double rq[Q];
 #pragma unroll
         for (int q = 1; q < Q; q++) {
-            own[q] = T(0);
+            own[q] = A(0);
             rq[q] = -1.0;
             if ((cut >> q) & 1u) {
-                own[q] = Pop<S, T>::ld(fin + (size_t)q * N + i, q);
+                own[q] = Pop<S, A>::ld(fin + (size_t)q * N + i, q);
                 rq[q] = rt[q];
             }
         }
 #pragma unroll
         for (int q = 1; q < Q; q++) {
             if (!((cut >> q) & 1u)) continue;
-            const T f2 = ((cut >> opp(q)) & 1u) ? own[q] : g[opp(q)];
-            g[q] = tube_interp<T>(rq[q], g[q], f2, own[q]);
+            const A f2 = ((cut >> opp(q)) & 1u) ? own[q] : g[opp(q)];
+            g[q] = tube_interp<A>(rq[q], g[q], f2, own[q]);
         }
     }
 
-    T drho = T(0), jx = T(0), jy = T(0), jz = T(0);
+    A drho = A(0), jx = A(0), jy = A(0), jz = A(0);
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         drho += g[q];
@@ -169,46 +175,46 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         if (cy(q)) jy += cy(q) * g[q];
         if (cz(q)) jz += cz(q) * g[q];
     }
-    const T rho = T(1) + drho;
-    const T amp = (T)st->amp;
+    const A rho = A(1) + drho;
+    const A amp = (A)st->amp;
     if (MOVING) {
-        const T uw = (T)st->wall;
-        if (wall_lo) mw[wcol] = mw_lo * rho * uw;
-        if (wall_hi) mw[(size_t)nx * nz + wcol] = mw_hi * rho * uw;
+        const A uw = (A)st->wall;
+        if (wall_lo) mw[wcol] = (T)(mw_lo * rho * uw);
+        if (wall_hi) mw[(size_t)nx * nz + wcol] = (T)(mw_hi * rho * uw);
     }
-    T ax = pex * amp, ay = pey * amp, az = pez * amp;
+    A ax = pex * amp, ay = pey * amp, az = pez * amp;
     if (HASF) {
-        ax += B * F[i];
-        ay += B * F[N + i];
-        az += B * F[2 * N + i];
+        ax += B * (A)F[i];
+        ay += B * (A)F[N + i];
+        az += B * (A)F[2 * N + i];
     }
     if (H) {  // per-node body-force field, same convention as P e
-        ax += H[i];
-        ay += H[N + i];
-        az += H[2 * N + i];
+        ax += (A)H[i];
+        ay += (A)H[N + i];
+        az += (A)H[2 * N + i];
     }
     // relaxation rate with the VOF indicator (lbm.cu:1131), uniform without
-    const T om = vf ? T(1) / (T(3) * (T(1) + l1m1 * (T)vf[i]) * df + T(0.5)) : om0;
-    const T irho = T(1) / rho;
+    const A om = vf ? A(1) / (A(3) * (A(1) + l1m1 * (A)vf[i]) * df + A(0.5)) : om0;
+    const A irho = A(1) / rho;
     // U = (j + G/2)/rho with the Guo force G = 2 rho (P e + B F) (lbm.cu:1116-1118)
-    const T Ux = jx * irho + ax, Uy = jy * irho + ay, Uz = jz * irho + az;
-    const T uu = Ux * Ux + Uy * Uy + Uz * Uz;
-    const T Ua = Ux * ax + Uy * ay + Uz * az;
-    const T kf = (T(1) - om * T(0.5)) * T(2) * rho;
+    const A Ux = jx * irho + ax, Uy = jy * irho + ay, Uz = jz * irho + az;
+    const A uu = Ux * Ux + Uy * Uy + Uz * Uz;
+    const A Ua = Ux * ax + Uy * ay + Uz * az;
+    const A kf = (A(1) - om * A(0.5)) * A(2) * rho;
 
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        const T cu = (cx(q) ? cx(q) * Ux : T(0)) + (cy(q) ? cy(q) * Uy : T(0)) + (cz(q) ? cz(q) * Uz : T(0));
-        const T ca = (cx(q) ? cx(q) * ax : T(0)) + (cy(q) ? cy(q) * ay : T(0)) + (cz(q) ? cz(q) * az : T(0));
-        const T geq = wq<T>(q) * (drho + rho * (T(3) * cu + T(4.5) * cu * cu - T(1.5) * uu));
-        const T src = wq<T>(q) * kf * (T(3) * ca * (T(1) + T(3) * cu) - T(3) * Ua);
-        const T gs = g[q] + om * (geq - g[q]) + src;
-        fout[(size_t)q * N + i] = Pop<S, T>::st(gs, q);
+        const A cu = (cx(q) ? cx(q) * Ux : A(0)) + (cy(q) ? cy(q) * Uy : A(0)) + (cz(q) ? cz(q) * Uz : A(0));
+        const A ca = (cx(q) ? cx(q) * ax : A(0)) + (cy(q) ? cy(q) * ay : A(0)) + (cz(q) ? cz(q) * az : A(0));
+        const A geq = wq<A>(q) * (drho + rho * (A(3) * cu + A(4.5) * cu * cu - A(1.5) * uu));
+        const A src = wq<A>(q) * kf * (A(3) * ca * (A(1) + A(3) * cu) - A(3) * Ua);
+        const A gs = g[q] + om * (geq - g[q]) + src;
+        fout[(size_t)q * N + i] = Pop<S, A>::st(gs, q);
     }
 
     const int iter = st->iter;
     const bool reset = ((iter - 1) % m) == 0, last = (iter % m) == 0;
-    const T ux = Ux * inv_cr, uy = Uy * inv_cr, uz = Uz * inv_cr;
+    const T ux = (T)(Ux * inv_cr), uy = (T)(Uy * inv_cr), uz = (T)(Uz * inv_cr);
     if (m == 1) {
         U[i] = ux;
         U[N + i] = uy;
@@ -223,11 +229,11 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         U[2 * N + i] = a2;
     }
     if (MACROS) {
-        rho_out[i] = rho;
+        rho_out[i] = (T)rho;
         Ui[i] = ux;
         Ui[N + i] = uy;
         Ui[2 * N + i] = uz;
-        rhom[i] = reset ? rho : rhom[i] + rho;  // dm is never divided (lbm.cu:1455-1457)
+        rhom[i] = reset ? (T)rho : rhom[i] + (T)rho;  // dm is never divided (lbm.cu:1455-1457)
     }
 }
 
@@ -373,7 +379,7 @@ template <typename V> __global__ void k_fill(V *__restrict__ p, size_t n, V v) {
     if (i < n) p[i] = v;
 }
 
-template <typename S, typename T> lbmibm_status step_impl(lbmibm_ctx *ctx) {
+template <typename S, typename T, typename A> lbmibm_status step_impl(lbmibm_ctx *ctx) {
     const lbmibm_params &p = ctx->p;
     LbmDims d;
     d.N = (unsigned)ctx->N;
@@ -390,15 +396,15 @@ template <typename S, typename T> lbmibm_status step_impl(lbmibm_ctx *ctx) {
     const bool hasf = !ctx->force_zero, macros = p.store_macros != 0, walls = p.periodic_y == 0;
     auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, ctx->stream, fin, fout, d, (const T *)ctx->F,
-                           (const T *)ctx->Hb, (T *)ctx->U, (T *)ctx->Ui, (T *)ctx->rho, (T *)ctx->rhom, (T)om, (T)(1.0 / p.cr),
-                           (T)(p.p * p.ex), (T)(p.p * p.ey), (T)(p.p * p.ez), (T)p.b, ctx->step, p.m,
-                           (const double *)(p.vof_period > 0 ? ctx->vof[ctx->vof_cur] : nullptr), (T)(p.lambda1 - 1.0),
-                           (T)p.df, (const unsigned *)ctx->tube_col, (const double *)ctx->tube_r, (T *)ctx->mw,
-                           (T)(p.wall_motion == 1 ? -6.0 * p.cr : 0.0), (T)(6.0 * p.cr));
+                           (const T *)ctx->Hb, (T *)ctx->U, (T *)ctx->Ui, (T *)ctx->rho, (T *)ctx->rhom, (A)om, (A)(1.0 / p.cr),
+                           (A)(p.p * p.ex), (A)(p.p * p.ey), (A)(p.p * p.ez), (A)p.b, ctx->step, p.m,
+                           (const double *)(p.vof_period > 0 ? ctx->vof[ctx->vof_cur] : nullptr), (A)(p.lambda1 - 1.0),
+                           (A)p.df, (const unsigned *)ctx->tube_col, (const double *)ctx->tube_r, (T *)ctx->mw,
+                           (A)(p.wall_motion == 1 ? -6.0 * p.cr : 0.0), (A)(6.0 * p.cr));
     };
     const int geom = p.tube_radius > 0.0 ? GEOM_TUBE : (walls ? (p.wall_motion ? GEOM_SHEAR : GEOM_SLAB) : GEOM_PERIODIC);
 #define LBM_CASE(HF, MC, G) \
-    if (hasf == HF && macros == MC && geom == G) launch(k_lbm<S, T, HF, MC, G>);
+    if (hasf == HF && macros == MC && geom == G) launch(k_lbm<S, T, A, HF, MC, G>);
 #define LBM_GEOMS(HF, MC) LBM_CASE(HF, MC, GEOM_PERIODIC) LBM_CASE(HF, MC, GEOM_SLAB) LBM_CASE(HF, MC, GEOM_TUBE) \
     LBM_CASE(HF, MC, GEOM_SHEAR)
     LBMIBM_KT(ctx, ctx->stream, "k_lbm",
@@ -453,9 +459,10 @@ template <typename S, typename T> lbmibm_status to_fstar_impl(lbmibm_ctx *ctx, d
 
 lbmibm_status lattice_step(lbmibm_ctx *ctx) {
     switch (ctx->prec) {
-        case LBMIBM_F64: return step_impl<double, double>(ctx);
-        case LBMIBM_F32: return step_impl<float, float>(ctx);
-        case LBMIBM_F16: return step_impl<__half, float>(ctx);
+        case LBMIBM_F64: return step_impl<double, double, double>(ctx);
+        case LBMIBM_F32:
+            return ctx->p.fp32_collide ? step_impl<float, float, float>(ctx) : step_impl<float, float, double>(ctx);
+        case LBMIBM_F16: return step_impl<__half, float, float>(ctx);
     }
     return LBMIBM_ERR_ARG;
 }

@@ -48,7 +48,7 @@ class Params(C.Structure):
         ("volume_correction", C.c_int), ("precision", C.c_int), ("store_macros", C.c_int),
         ("fileoutput", C.c_int), ("vof_period", C.c_int), ("sor_itrmax", C.c_int), ("sor_residual", C.c_double),
         ("tube_radius", C.c_double), ("tube_staircase", C.c_int), ("wall_motion", C.c_int),
-        ("wall_velocity", C.c_double), ("bicg_warm_start", C.c_int), ("residual_floor", C.c_double),
+        ("wall_velocity", C.c_double), ("bicg_warm_start", C.c_int), ("residual_floor", C.c_double), ("fp32_collide", C.c_int),
     ]
 
 

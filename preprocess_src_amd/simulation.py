@@ -26,7 +26,8 @@ def make_params(lx, ly, lz, *, precision="f32", tau=None, df=None, dx=1.0, cr=1.
                 rgs=1.0, br=0.0, itrmax=1000, residual=1e-10, runge_kutta=4, neohookean=False,
                 volume_correction=True, store_macros=False, fileoutput=0, periodic_y=False, lambda1=1.0,
                 vof_period=0, sor_itrmax=None, sor_residual=None, tube_radius=0.0, tube_staircase=False,
-                wall_motion=0, wall_velocity=0.0, bicg_warm_start=False, residual_floor=1e-7):
+                wall_motion=0, wall_velocity=0.0, bicg_warm_start=False, residual_floor=1e-7,
+                fp32_collide=False):
     P = N.Params()
     N.lib().lbmibm_default_params(C.byref(P), int(lx), int(ly), int(lz))
     if df is None:
@@ -49,6 +50,7 @@ def make_params(lx, ly, lz, *, precision="f32", tau=None, df=None, dx=1.0, cr=1.
     P.tube_staircase = int(tube_staircase)
     P.wall_motion, P.wall_velocity = int(wall_motion), float(wall_velocity)
     P.bicg_warm_start, P.residual_floor = int(bicg_warm_start), float(residual_floor)
+    P.fp32_collide = int(fp32_collide)
     return P
 
 

@@ -5,8 +5,9 @@ Configuration: 128x64x64 fluid nodes, half-way bounce-back walls at y = +-32,
 periodic x and z, tau = 1, Guo body force 2 P = 1e-6 along x, 64 Evans-Fung
 biconcave cells of radius 4 lattice units on 2562-vertex / 5120-triangle
 icosphere meshes, placed by the library's seeded non-overlapping placement
-(seed 2026) with uniform random orientations, Skalak Gs = 5e-4 (B = Gs / 2),
-C = 100, BR = 0, RK4, M = 1, volume correction on, RESIDUAL 1e-12.  The fluid
+(seed 2026, clear of the periodic faces) with uniform random orientations,
+Skalak Gs = 5e-4 (B = Gs / 2), C = 100, BR = 0, RK4, M = 1, volume
+correction on, RESIDUAL 1e-12.  The fluid
 starts in the steady Poiseuille profile u(y) = 2P (H^2/4 - y^2) / (2 nu) so
 that the cells deform from the first step.  STEPS whole steps of the oracle
 are recorded.
@@ -45,9 +46,16 @@ PATH = os.path.join(HERE, "c3_channel_64rbc_f64.npz")
 
 
 def make_cells():
+    """The seeded placement in the box shrunk by RADIUS + 1 on the periodic
+    axes (x, z), so that no cell straddles a periodic face at the start.  A
+    straddling cell's initial vertices lie outside the box, where the
+    reference's padded frame has no nodes for part of their stencils (it
+    drops them until the first volume correction wraps the positions back),
+    while the library wraps every stencil node periodically."""
     from preprocess_src_amd import mesh
     lx, ly, lz = SHAPE
-    centres, rot = mesh.place(NC, RADIUS, lx, ly, lz, SEED)
+    m = 2.0 * (RADIUS + 1.0)
+    centres, rot = mesh.place(NC, RADIUS, lx - m, ly, lz - m, SEED)
     return mesh.assemble(np.full(NC, LEVEL), centres, rot, RADIUS)
 
 

@@ -81,14 +81,13 @@ def _set_positions(o, g, pos):
 SHAPES = [(16, 14, 12, False), (33, 17, 9, False), (18, 9, 10, True)]
 
 
-@pytest.mark.parametrize("form", [0, 1, 2])
+@pytest.mark.parametrize("form", [0, 1])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("lx,ly,lz,py", SHAPES)
 def test_ibm_spread_parity(prec, lx, ly, lz, py, form, monkeypatch):
-    """The spreading forms (source bins per tile; home boxes + gather; home
-    boxes by sorted stencil runs + gather; the autotuner picks one) against
-    the oracle, on lattices with partial tiles, boxes that wrap onto
-    themselves, walls and periodic y."""
+    """Both spreading forms (source bins per tile; home boxes + gather, the
+    autotuner picks one) against the oracle, on lattices with partial tiles,
+    boxes that wrap onto themselves, walls and periodic y."""
     monkeypatch.setenv("LBMIBM_SPREAD_FORM", str(form))
     print(f"[spread parity] prec={prec} shape={lx}x{ly}x{lz} periodic_y={py} form={form}", flush=True)
     cells = _cells_across_boundaries(lx, ly, lz, periodic_y=py)
@@ -135,15 +134,14 @@ def _random_velocity_field(o, seed, amp=1e-2):
     return o.um.copy(), o.vm.copy(), o.wm.copy()
 
 
-@pytest.mark.parametrize("form", [0, 1, 2])
+@pytest.mark.parametrize("form", [0, 1])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("rk,m,dx,shape", [(4, 1, 1.0, SHAPES[0]), (4, 2, 0.5, SHAPES[0]), (2, 1, 1.0, SHAPES[0]),
                                            (1, 3, 1.0, SHAPES[0]), (4, 1, 1.0, SHAPES[1]), (4, 1, 1.0, SHAPES[2])])
 def test_ibm_interp_parity(prec, rk, m, dx, shape, form, monkeypatch):
-    """The interpolation forms (LDS-staged tiles; rows read from memory;
-    LDS-staged tiles with four lanes per vertex; the autotuner picks one)
-    against the oracle, with cells across the periodic and wall
-    boundaries."""
+    """Both interpolation forms (LDS-staged tiles, rows read from memory; the
+    autotuner picks one) against the oracle, with cells across the periodic
+    and wall boundaries."""
     monkeypatch.setenv("LBMIBM_INTERP_FORM", str(form))
     lx, ly, lz, py = shape
     print(f"[interp parity] prec={prec} rk={rk} m={m} dx={dx} shape={shape} form={form}", flush=True)
@@ -352,41 +350,7 @@ def test_bicgstab_not_converged_is_reported():
     assert e.value.code == 4
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_spread_sorted_runs_equal_home_boxes_bitwise(prec, monkeypatch):
-    """Form 2 adds the same fixed-point terms as form 1, grouped by stencil,
-    so the lattice forces are bitwise equal.  A 10242-vertex cell of radius
-    2.5 has all its vertices at home in one 16x8x8 tile, more than one
-    sorting pass holds (4096), so the passes are exercised; the walls cut
-    the stencils of the other cell near y = -LY/2."""
-    from preprocess_src_amd import mesh
-    lx, ly, lz = 32, 24, 24
-    centres = np.array([[-7.7, -8.0, -8.0], [8.3, -9.0, 4.4]])
-    rot = np.stack([np.eye(3)] * 2)
-    cells = mesh.assemble([5, 4], centres, rot, 2.5)
-    rng = np.random.default_rng(71)
-    n = cells["x"].size
-    f = [rng.standard_normal(n) for _ in range(3)]
-    out = {}
-    for form in (1, 2):
-        print(f"[spread forms] prec={prec} form={form}", flush=True)
-        monkeypatch.setenv("LBMIBM_SPREAD_FORM", str(form))
-        g = gpu_of(lx, ly, lz, prec, cells=cells)
-        g.set_cells_state(fx=f[0], fy=f[1], fz=f[2])
-        g.ibm_st1()
-        out[form] = [a.copy() for a in g.lattice_force()]
-        g.close()
-    for a, b in zip(out[1], out[2]):
-        assert np.array_equal(a, b)
-    o = oracle_of(lx, ly, lz, cells=cells)
-    o.cells["cfx"][:], o.cells["cfy"][:], o.cells["cfz"][:] = f
-    o.ibm_st1()
-    comp = o.bc == 0
-    for a, b in zip(out[2], (o.fx, o.fy, o.fz)):
-        assert rel(a[comp], b[comp]) < TOL[prec]
-
-
-@pytest.mark.parametrize("form", [0, 1, 2])
+@pytest.mark.parametrize("form", [0, 1])
 def test_spread_is_bitwise_reproducible(form, monkeypatch):
     """Fixed-point LDS accumulation (and, in form 1, a fixed summation order
     over the boxes): repeated spreads give identical bits."""
