@@ -371,6 +371,15 @@ lbmibm_status lbmibm_adj_stress(lbmibm_ctx *ctx, int iter);
  * both y walls, averaged over the walls (e = unit body-force direction).
  * Runs the steps. */
 lbmibm_status lbmibm_wall_shear(lbmibm_ctx *ctx, int iter0, int nsteps, int window, double *J);
+/* J as lbmibm_wall_shear, allocating nothing: no counterpart in the
+ * reference.  For callers that cannot allocate device memory once set up
+ * (harnesses that forbid it, loops that evaluate J repeatedly).  Its partial
+ * sums (one per 64 wall columns, summed in a fixed order) live in a static
+ * device array of the library, 65,537 doubles (512 KiB) per device, held
+ * until the process ends: nx*nz up to 4,194,304 columns, larger lattices get
+ * LBMIBM_ERR_ARG.  Every call on a device shares that array (each call
+ * clears it first), so calls that overlap on one device race. */
+lbmibm_status lbmibm_wall_shear_noalloc(lbmibm_ctx *ctx, int iter0, int nsteps, int window, double *J);
 /* J as lbmibm_wall_shear and its gradient with respect to the body-force
  * field (padded [n] arrays, any may be NULL) and the per-cell force scales
  * ([nc] or NULL), by checkpoints every `checkpoint_every` steps and per-segment

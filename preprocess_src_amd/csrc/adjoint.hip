@@ -17,6 +17,7 @@
 // as it would run next.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -248,6 +249,77 @@ struct ColdSolves {
     explicit ColdSolves(lbmibm_ctx *ctx) : c(ctx), prev(ctx->solve_cold) { c->solve_cold = true; }
     ~ColdSolves() { c->solve_cold = prev; }
 };
+
+// ---- wall shear without allocation -------------------------------------
+// Partial sums of the wall-shear objective, one slot per 64-column wave of
+// k_wall_shear_part, and the total in the slot after them.  The array is a
+// per-device static: every call on a device shares it (calls that overlap on
+// one device race), and it keeps the last call's values, so each call clears
+// the slots it uses first.
+constexpr int WS_MAX_PARTS = 65536;  // 64 columns each: nx*nz <= 4,194,304
+__device__ __align__(256) double scratch[WS_MAX_PARTS + 1];
+
+constexpr int WS_MAX_DEVICES = 64;
+std::mutex ws_mutex;
+double *ws_ptr[WS_MAX_DEVICES] = {};
+
+// The device address of scratch on the current device, looked up once per
+// device (the host-side symbol is not a device address).
+lbmibm_status ws_scratch(lbmibm_ctx *ctx, double **out) {
+    int dev = 0;
+    LBMIBM_HIP(ctx, hipGetDevice(&dev));
+    if (dev < 0 || dev >= WS_MAX_DEVICES) {
+        ctx->err = "wall_shear_noalloc: device index out of range";
+        return LBMIBM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(ws_mutex);
+    if (!ws_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            ctx->err = std::string("wall_shear_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e);
+            return LBMIBM_ERR_HIP;
+        }
+        ws_ptr[dev] = (double *)p;
+    }
+    *out = ws_ptr[dev];
+    return LBMIBM_OK;
+}
+
+__global__ void k_ws_clear(double *__restrict__ part, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) part[k] = 0.0;
+}
+
+// As k_wall_shear, but each wave adds its sum to its own slot (no atomics).
+template <typename T>
+__global__ void k_wall_shear_part(const T *__restrict__ U, double *__restrict__ part, int nx, int ny, int nz,
+                                  double ex, double ey, double ez, double coef) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (k < nx * nz) {
+        const int ix = k % nx, iz = k / nx;
+        const size_t N = (size_t)nx * ny * nz;
+        for (int iy : {0, ny - 1}) {
+            const size_t n = (size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz);
+            v += ex * (double)U[n] + ey * (double)U[N + n] + ez * (double)U[2 * N + n];
+        }
+    }
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0 && k - (int)(threadIdx.x & 63) < nx * nz) part[k >> 6] += coef * v;
+}
+
+// One 256-thread work group: part[nparts] = the sum of part[0..nparts) in a
+// fixed order (strided per thread, then the waves, then the four wave sums).
+__global__ __launch_bounds__(256) void k_ws_total(double *__restrict__ part, int nparts) {
+    __shared__ double w[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) v += part[i];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) part[nparts] = (w[0] + w[1]) + (w[2] + w[3]);
+}
 }  // namespace
 
 extern "C" {
@@ -420,6 +492,53 @@ lbmibm_status lbmibm_wall_shear(lbmibm_ctx *ctx, int iter0, int nsteps, int wind
     if (s == LBMIBM_OK) s = check_flags(ctx, nullptr);
     if (s == LBMIBM_OK && hipMemcpy(J, acc, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) s = LBMIBM_ERR_HIP;
     (void)hipFree(acc);
+    return s;
+}
+
+lbmibm_status lbmibm_wall_shear_noalloc(lbmibm_ctx *ctx, int iter0, int nsteps, int window, double *J) {
+    if (!ctx || !J || nsteps < 1 || window < 1 || window > nsteps) return LBMIBM_ERR_ARG;
+    const long long cols = (long long)ctx->nx * ctx->nz;
+    const int nparts = (int)((cols + 63) / 64);
+    if (cols > 64LL * WS_MAX_PARTS) {
+        ctx->err = "wall_shear_noalloc: nx*nz = " + std::to_string(cols) + " columns exceeds the scratch limit of " +
+                   std::to_string(64LL * WS_MAX_PARTS);
+        return LBMIBM_ERR_ARG;
+    }
+    double coef = 0.0;
+    lbmibm_status s = wall_coef(ctx, window, coef);
+    if (s != LBMIBM_OK) return s;
+    s = need(ctx, ctx->p.m == 1 && !ctx->p.volume_correction, "needs M = 1 and no volume correction");
+    if (s != LBMIBM_OK) return s;
+    double *part = nullptr;
+    s = ws_scratch(ctx, &part);
+    if (s != LBMIBM_OK) return s;
+    ColdSolves cold(ctx);
+    double e[3];
+    e_norm(ctx->p, e);
+    const dim3 grid((unsigned)((cols + 255) / 256));
+    hipLaunchKernelGGL(k_ws_clear, dim3((nparts + 256) / 256), dim3(256), 0, ctx->stream, part, nparts + 1);
+    LBMIBM_LAUNCHED(ctx, "k_ws_clear");
+    s = reset_status(ctx);
+    for (int k = 0; k < nsteps && s == LBMIBM_OK; k++) {
+        s = enqueue_step(ctx, iter0 + k + 1, false);
+        if (s != LBMIBM_OK || k < nsteps - window) continue;
+        if (ctx->prec == LBMIBM_F64)
+            hipLaunchKernelGGL(k_wall_shear_part<double>, grid, dim3(256), 0, ctx->stream, (const double *)ctx->U,
+                               part, ctx->nx, ctx->ny, ctx->nz, e[0], e[1], e[2], coef);
+        else
+            hipLaunchKernelGGL(k_wall_shear_part<float>, grid, dim3(256), 0, ctx->stream, (const float *)ctx->U,
+                               part, ctx->nx, ctx->ny, ctx->nz, e[0], e[1], e[2], coef);
+        LBMIBM_LAUNCHED(ctx, "k_wall_shear_part");
+    }
+    if (s == LBMIBM_OK) {
+        hipLaunchKernelGGL(k_ws_total, dim3(1), dim3(256), 0, ctx->stream, part, nparts);
+        LBMIBM_LAUNCHED(ctx, "k_ws_total");
+    }
+    if (s == LBMIBM_OK) s = check_flags(ctx, nullptr);
+    if (s == LBMIBM_OK &&
+        hipMemcpyAsync(J, part + nparts, sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        s = LBMIBM_ERR_HIP;
+    if (s == LBMIBM_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) s = LBMIBM_ERR_HIP;
     return s;
 }
 

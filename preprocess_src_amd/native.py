@@ -122,6 +122,7 @@ SIGNATURES = {
     "lbmibm_adj_ibm_st2": (C.c_int, [C.c_void_p]),
     "lbmibm_adj_stress": (C.c_int, [C.c_void_p, C.c_int]),
     "lbmibm_wall_shear": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_double_p]),
+    "lbmibm_wall_shear_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_double_p]),
     "lbmibm_adjoint_wall_shear": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int] + [c_double_p] * 5),
     "lbmibm_mesh_icosphere": (C.c_int, [C.c_int, c_int_p, c_int_p, c_double_p, c_int_p]),
     "lbmibm_mesh_biconcave": (C.c_int, [C.c_int, c_double_p]),
