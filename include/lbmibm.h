@@ -293,6 +293,12 @@ lbmibm_status lbmibm_adhesion(lbmibm_ctx *ctx);
  * (receptor or -1) / r_ij [Nv*NS], is_attached [receptors], f_adh [3Nv] AoS. */
 lbmibm_status lbmibm_adhesion_state(lbmibm_ctx *ctx, unsigned char *is_knob, int *id_wall, double *r_wall,
                                     int *attach, double *r_ij, int *is_attached, double *f_adh);
+/* The file output of Adhesion() (adhesion.cu:785-822): writes
+ * dir/adhesion/adhe%05d.bin (fnum_ad, iter, f_adh [3Nv], attachment_point
+ * [Nv*NS], is_knob [Nv] as bytes, is_attached [receptors], r_ij [Nv*NS]) and
+ * dir/adhesion/restart_adhe.dat, then increments *fnum_ad.  The reference
+ * calls it when (iter-1) % FILEOUTPUT_ADHESION == 0. */
+lbmibm_status lbmibm_write_adhesion(lbmibm_ctx *ctx, const char *dir, int iter, int *fnum_ad);
 
 /* ---- the reference's binary files (etc.cu) ------------------------------
  * Byte layouts of output_data / restart_file: native-endian int32 and FP64

@@ -130,16 +130,29 @@ inline void thinc_wlic(Domain *, Lattice *) {
 // Cytoadhesion: wall_init + adhesion_init in one call, given the def.h
 // constants and the knob flags of ./input/icosa%d.dat (or NULL to draw
 // them), then Adhesion() of the loop (main.cu:84-86), which lbmibm_step also
-// runs by itself once set up.
+// runs by itself once set up.  With FILEOUTPUT_ADHESION > 0, Adhesion()
+// writes ./result/adhesion/adhe%05d.bin and restart_adhe.dat when
+// (iter-1) % FILEOUTPUT_ADHESION == 0 and advances prc->fnum_ad
+// (adhesion.cu:785-822).
 namespace lbmibm_ref {
-inline void attach_adhesion(const lbmibm_adhesion_params &a, const unsigned char *is_knob) {
+inline int &adhesion_file_period() {
+    static int k = 0;
+    return k;
+}
+inline void attach_adhesion(const lbmibm_adhesion_params &a, const unsigned char *is_knob,
+                            int fileoutput_adhesion = 0) {
     check(lbmibm_adhesion_setup(context(), &a, is_knob), "adhesion_init");
+    adhesion_file_period() = fileoutput_adhesion;
 }
 }  // namespace lbmibm_ref
 
 template <class Process, class Domain, class Cell, class Wall>
-inline void Adhesion(Process *, Domain *, Cell *, Wall *, Wall *) {
+inline void Adhesion(Process *prc, Domain *, Cell *, Wall *, Wall *) {
     lbmibm_ref::check(lbmibm_adhesion(lbmibm_ref::context()), "Adhesion");
+    const int k = lbmibm_ref::adhesion_file_period();
+    if (k > 0 && (prc->iter - 1) % k == 0)
+        lbmibm_ref::check(lbmibm_write_adhesion(lbmibm_ref::context(), "./result", prc->iter, &prc->fnum_ad),
+                          "Adhesion file output");
 }
 
 // The binary files of output_data into ./result with the reference's names

@@ -153,3 +153,33 @@ def parse_cell(buf: bytes) -> dict:
     out["ele"] = r.take(np.int32, 3 * nc * E)
     assert r.done(), "trailing bytes"
     return out
+
+
+def adhesion_bytes(fnum_ad, iter_, f_adh, attach, is_knob, is_attached, r_ij) -> bytes:
+    """adhe%05d.bin of Adhesion() (adhesion.cu:785-810) from the adhesion
+    arrays: fnum_ad, iter (int32), f_adh [3 Nv] (AoS, FP64), attachment_point
+    [Nv NS] (int32), is_knob [Nv] (bool, one byte), the receptors'
+    is_attached (int32) and r_ij [Nv NS] (FP64)."""
+    return b"".join([np.array([fnum_ad, iter_], np.int32).tobytes(),
+                     np.ascontiguousarray(f_adh, np.float64).tobytes(),
+                     np.ascontiguousarray(attach, np.int32).tobytes(),
+                     np.ascontiguousarray(is_knob, np.uint8).tobytes(),
+                     np.ascontiguousarray(is_attached, np.int32).tobytes(),
+                     np.ascontiguousarray(r_ij, np.float64).tobytes()])
+
+
+def parse_adhesion(buf: bytes, nv, ns, nrec) -> dict:
+    """Inverse of adhesion_bytes for Nv vertices, NS springs per knob and
+    nrec receptors."""
+    assert len(buf) == 8 + 24 * nv + 4 * nv * ns + nv + 4 * nrec + 8 * nv * ns, len(buf)
+    o = 0
+
+    def take(dtype, count):
+        nonlocal o
+        a = np.frombuffer(buf, dtype, count, o)
+        o += a.nbytes
+        return a
+
+    fnum_ad, iter_ = (int(v) for v in take(np.int32, 2))
+    return dict(fnum_ad=fnum_ad, iter=iter_, f_adh=take(np.float64, 3 * nv), attach=take(np.int32, nv * ns),
+                is_knob=take(np.uint8, nv), is_attached=take(np.int32, nrec), r_ij=take(np.float64, nv * ns))

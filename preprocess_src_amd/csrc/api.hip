@@ -261,9 +261,10 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->qi, ctx->coef, ctx->bv, ctx->rv, ctx->rh, ctx->pv, ctx->vv, ctx->tv, ctx->gscale,
                     ctx->normal, ctx->vol0, ctx->vol, ctx->area, ctx->gx, ctx->bicg_iters, ctx->vol_passes,
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
-                    ctx->tile_off, ctx->tile_cur, ctx->order_calls, ctx->chunks, ctx->chunk_total, ctx->tile_chunk, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
+                    ctx->tile_off, ctx->tile_cur, ctx->order_calls, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
                     ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv,
                     ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair,
+                    ctx->asm_chunks, ctx->asm_elems, ctx->asm_list[0], ctx->asm_list[1], ctx->asm_list[2], ctx->eslot,
                     ctx->srec, ctx->onchip_cells, ctx->Hb, ctx->adj_f[0], ctx->adj_f[1], ctx->adj_u, ctx->adj_F,
                     ctx->adj_x, ctx->adj_cf, ctx->grad_H, ctx->grad_gs, ctx->adj_lam, ctx->adj_dsv,
                     ctx->adj_corner, ctx->adj_scratch, ctx->adj_cscale, ctx->adj_gpart, ctx->adj_xmax, ctx->adj_rec, ctx->sbox, ctx->sb_cand,
@@ -496,13 +497,15 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         {
             std::vector<unsigned short> nbr16(6 * (size_t)nv);
             std::vector<int> cslot(3 * (size_t)nt, 0);
-            std::vector<unsigned char> epair(6 * (size_t)nv, 0xff);
+            std::vector<unsigned char> epair(6 * (size_t)nv, 0xff), eslot(3 * (size_t)nt, 0);
             for (int j = 0; j < nv; j++) {
                 const int n = nln[j], base = cells->voff[vcell[j]];
                 for (int sl = 0; sl < n; sl++) {
                     const int e = le[6 * (size_t)j + sl];
                     const int *ev = cells->ele + 3 * (size_t)e;
-                    cslot[3 * (size_t)e + (ev[0] == j ? 0 : (ev[1] == j ? 1 : 2))] = sl * nv + j;
+                    const size_t k = 3 * (size_t)e + (ev[0] == j ? 0 : (ev[1] == j ? 1 : 2));
+                    cslot[k] = sl * nv + j;
+                    eslot[k] = (unsigned char)sl;
                 }
                 int m = 0;
                 for (int k = 0; k <= n && k < 7; k++) {
@@ -530,6 +533,8 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
             CK(dev_alloc(ctx, (void **)&ctx->epair, epair.size()));
             LBMIBM_HIP(ctx, hipMemcpy(ctx->epair, epair.data(), epair.size(), hipMemcpyHostToDevice));
             CK(dev_alloc(ctx, &ctx->corner, 6 * (size_t)nv * 4 * szT));
+            CK(dev_alloc(ctx, (void **)&ctx->eslot, eslot.size()));
+            LBMIBM_HIP(ctx, hipMemcpy(ctx->eslot, eslot.data(), eslot.size(), hipMemcpyHostToDevice));
         }
         // cells solved on chip and the rest
         std::vector<int> cls[3];
@@ -556,6 +561,38 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
                 ctx->cls_maxv[big] = std::max(ctx->cls_maxv[big], cells->voff[c + 1] - cells->voff[c]);
             }
         }
+        {
+            // fused assembly: each cell's vertices in chunks of ASM_CH with the
+            // elements incident to them (ascending), and the chunk lists of
+            // all cells, the large cells (class 2) and the on-chip cells
+            std::vector<int4> chunks;
+            std::vector<int> elems, lists[3], mark(nt, -1);
+            std::vector<char> big(nc, 0);
+            for (int c : cls[2]) big[c] = 1;
+            for (int c = 0; c < nc; c++)
+                for (int v0 = cells->voff[c]; v0 < cells->voff[c + 1]; v0 += ASM_CH) {
+                    const int cnv = std::min(ASM_CH, cells->voff[c + 1] - v0), k = (int)chunks.size();
+                    const size_t e0 = elems.size();
+                    for (int j = v0; j < v0 + cnv; j++)
+                        for (int sl = 0; sl < nln[j]; sl++) {
+                            const int e = le[6 * (size_t)j + sl];
+                            if (mark[e] != k) { mark[e] = k; elems.push_back(e); }
+                        }
+                    std::sort(elems.begin() + e0, elems.end());
+                    chunks.push_back(make_int4(v0, cnv, (int)e0, (int)(elems.size() - e0)));
+                    lists[0].push_back(k);
+                    lists[big[c] ? 1 : 2].push_back(k);
+                }
+            if (elems.size() > (size_t)INT32_MAX) return fail(LBMIBM_ERR_ARG);
+            CK(dev_alloc(ctx, (void **)&ctx->asm_chunks, chunks.size() * sizeof(int4)));
+            LBMIBM_HIP(ctx, hipMemcpy(ctx->asm_chunks, chunks.data(), chunks.size() * sizeof(int4),
+                                      hipMemcpyHostToDevice));
+            CK(up_i(&ctx->asm_elems, elems.data(), elems.size()));
+            for (int k = 0; k < 3; k++) {
+                ctx->asm_n[k] = (int)lists[k].size();
+                if (!lists[k].empty()) CK(up_i(&ctx->asm_list[k], lists[k].data(), lists[k].size()));
+            }
+        }
         std::vector<int> vcls[2];
         for (int c = 0; c < nc; c++) vcls[cells->voff[c + 1] - cells->voff[c] > VOL_SMALL].push_back(c);
         for (int k = 0; k < 2; k++) {
@@ -580,9 +617,6 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         CK(dev_alloc(ctx, (void **)&ctx->tile_off, (size_t)ctx->ntiles * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->tile_cur, (size_t)ctx->ntiles * sizeof(int)));
         CK(dev_alloc(ctx, (void **)&ctx->tile_cand, (size_t)ctx->ntiles * sizeof(int)));
-        ctx->nchunk_max = ctx->ntiles + (nv + INTERP_CHUNK - 1) / INTERP_CHUNK;
-        CK(dev_alloc(ctx, (void **)&ctx->chunks, (size_t)ctx->nchunk_max * sizeof(int2)));
-        CK(dev_alloc(ctx, (void **)&ctx->tile_chunk, ((size_t)ctx->ntiles + 1) * sizeof(int)));
         {
             // tile orders start as the identity (any permutation is valid)
             std::vector<int> ident(ctx->ntiles);
@@ -593,7 +627,6 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         {
             const int zero = 0;
             CK(up_i(&ctx->order_calls, &zero, 1));
-            CK(up_i(&ctx->chunk_total, &zero, 1));
         }
         CK(dev_alloc(ctx, (void **)&ctx->blk_fmax, (size_t)ctx->nblk_bin * sizeof(float)));
         CK(dev_alloc(ctx, (void **)&ctx->absmax, sizeof(unsigned)));

@@ -36,7 +36,7 @@ struct DevStatus {
 // Lattice tiles of the immersed-boundary kernels.
 constexpr int VOL_SMALL = 3072;
 constexpr int TILE_X = 16, TILE_Y = 8, TILE_Z = 8;
-constexpr int INTERP_CHUNK = 512;  // home vertices per work group of the interpolation and spreading form 1
+constexpr int ASM_CH = 256;  // vertices per chunk of the fused membrane assembly (k_assemble)
 
 }  // namespace lbmibm
 
@@ -110,6 +110,13 @@ struct lbmibm_ctx {
     void *tt = nullptr, *rg = nullptr, *tp = nullptr;
     void *qv = nullptr, *qi = nullptr;                 // [3Nv]
     void *qp = nullptr;                                // [3Nv] the loads before qv (bicg_warm_start 2)
+    // fused assembly: chunks (first vertex, vertices, first entry of
+    // asm_elems, elements) and the chunk lists of all / large / on-chip cells
+    int4 *asm_chunks = nullptr;
+    int *asm_elems = nullptr;
+    int *asm_list[3] = {nullptr, nullptr, nullptr};
+    int asm_n[3] = {0, 0, 0};
+    unsigned char *eslot = nullptr;                    // [3Nt] slot of each corner in its vertex's list
     void *coef = nullptr;                              // [6][Nv] off-diagonal Jacobi-scaled coefficients
     unsigned short *nbr = nullptr;                     // [6][Nv] cell-local neighbour ids (self for unused)
     // BiCGSTAB cell classes: 0 on chip with one vertex per thread, 1 on chip
@@ -150,10 +157,6 @@ struct lbmibm_ctx {
     int *tile_cand = nullptr;                          // [ntiles] spreading candidates
     int *order_spread = nullptr, *order_interp = nullptr;  // [ntiles] heavy-first tile orders
     int *order_calls = nullptr;              // device counter of binnings (tile-order refresh)
-    int2 *chunks = nullptr;                  // [nchunk_max] (tile, chunk) work list of the interpolation
-    int *chunk_total = nullptr;              // its length (device)
-    int *tile_chunk = nullptr;               // [ntiles + 1] first chunk of each tile, then the total
-    int nchunk_max = 0;                      // ntiles + ceil(Nv / INTERP_CHUNK)
     float *blk_fmax = nullptr;               // [nblk_bin] max |f| per binning block
     unsigned *absmax = nullptr;
     bool bins_valid = false;                 // bins match the current positions

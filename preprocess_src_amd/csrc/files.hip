@@ -321,6 +321,54 @@ lbmibm_status lbmibm_write_cell(lbmibm_ctx *ctx, const char *path) {
     return close_out(ctx, o, path);
 }
 
+lbmibm_status lbmibm_write_adhesion(lbmibm_ctx *ctx, const char *dir, int iter, int *fnum_ad) {
+    // Adhesion() file output (adhesion.cu:785-822): dir/adhesion/adhe%05d.bin
+    // with fnum_ad, iter, f_adh [3 Nv] (AoS), attachment_point [Nv NS],
+    // is_knob [Nv] (one byte each), the receptors' is_attached [num_wall]
+    // and r_ij [Nv NS], then dir/adhesion/restart_adhe.dat ("%d\n%d\n",
+    // fnum_ad, iter); fnum_ad is incremented.
+    if (!ctx || !dir || !fnum_ad) return LBMIBM_ERR_ARG;
+    if (!ctx->adh_on) {
+        ctx->err = "adhesion: call lbmibm_adhesion_setup first";
+        return LBMIBM_ERR_STATE;
+    }
+    const size_t nv = ctx->nv, ns = nv * ctx->adh.springs_per_knob;
+    const size_t nrec = (size_t)ctx->adh_nxw * ctx->adh_nzw;
+    std::vector<double> fadh(3 * nv), rij(ns);
+    std::vector<int> attach(ns), isatt(nrec);
+    std::vector<unsigned char> knob(nv);
+    lbmibm_status s = lbmibm_adhesion_state(ctx, knob.data(), nullptr, nullptr, attach.data(), rij.data(),
+                                            isatt.data(), fadh.data());
+    if (s != LBMIBM_OK) return s;
+    char name[4096];
+    snprintf(name, sizeof name, "%s/adhesion/adhe%05d.bin", dir, *fnum_ad);
+    Out o;
+    s = open_out(ctx, o, name);
+    if (s != LBMIBM_OK) return s;
+    o.put(fnum_ad, 1);
+    o.put(&iter, 1);
+    o.put(fadh);
+    o.put(attach);
+    o.put(knob);  // sizeof(bool) = 1
+    o.put(isatt);
+    o.put(rij);
+    s = close_out(ctx, o, name);
+    if (s != LBMIBM_OK) return s;
+    snprintf(name, sizeof name, "%s/adhesion/restart_adhe.dat", dir);
+    FILE *f = fopen(name, "w");
+    if (!f) {
+        ctx->err = std::string("cannot open ") + name + " for writing";
+        return LBMIBM_ERR_ARG;
+    }
+    const bool ok = fprintf(f, "%d\n%d\n", *fnum_ad, iter) > 0;
+    if (fclose(f) != 0 || !ok) {
+        ctx->err = std::string("write to ") + name + " failed";
+        return LBMIBM_ERR_ARG;
+    }
+    (*fnum_ad)++;
+    return LBMIBM_OK;
+}
+
 lbmibm_status lbmibm_output_data(lbmibm_ctx *ctx, const char *dir, int iter, int *fnum) {
     if (!ctx || !dir || !fnum) return LBMIBM_ERR_ARG;
     char name[4096];

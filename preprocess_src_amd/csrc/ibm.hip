@@ -263,21 +263,15 @@ __device__ void heavy_first_order(const int *load, int n, int *hist, int *order)
     __syncthreads();
 }
 
-// The work list of the interpolation and of spreading form 1: each tile's
-// home vertices in chunks of INTERP_CHUNK, one work group per chunk, so that
-// no work group holds more than a chunk however the cells crowd the tiles.
-// chunks[c] = (tile, chunk of the tile); the first *chunk_total entries are
-// valid; the chunks of tile t are tile_chunk[t] .. tile_chunk[t + 1] - 1.
 __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile_count, int *__restrict__ tile_off,
                                                     int *__restrict__ tile_cur, int ntiles,
                                                     const float *__restrict__ blk_fmax, int nblk,
-                                                    unsigned *__restrict__ absmax, int2 *__restrict__ chunks,
-                                                    int *__restrict__ chunk_total, int *__restrict__ tile_chunk) {
-    // exclusive scans over tiles (vertices, chunks) in one work group, 1024
-    // entries per round, and the maximum of the per-block force maxima
-    __shared__ int sh[1024], shc[1024];
+                                                    unsigned *__restrict__ absmax) {
+    // exclusive scan over tiles in one work group, 1024 entries per round,
+    // and the maximum of the per-block force maxima
+    __shared__ int sh[1024];
     __shared__ float shm[1024];
-    __shared__ int carry, carryc;
+    __shared__ int carry;
     float fm = 0.0f;
     for (int i = threadIdx.x; i < nblk; i += 1024) fm = fmaxf(fm, blk_fmax[i]);
     shm[threadIdx.x] = fm;
@@ -287,42 +281,25 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int *__restrict__ tile
         if (threadIdx.x < o) shm[threadIdx.x] = fmaxf(shm[threadIdx.x], shm[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        *absmax = __float_as_uint(shm[0]);
-        carryc = 0;
-    }
+    if (threadIdx.x == 0) *absmax = __float_as_uint(shm[0]);
     for (int base = 0; base < ntiles; base += 1024) {
         const int i = base + threadIdx.x;
         const int v = i < ntiles ? tile_count[i] : 0;
-        const int nch = (v + INTERP_CHUNK - 1) / INTERP_CHUNK;
         sh[threadIdx.x] = v;
-        shc[threadIdx.x] = nch;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
             const int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
-            const int tc = threadIdx.x >= o ? shc[threadIdx.x - o] : 0;
             __syncthreads();
             sh[threadIdx.x] += t;
-            shc[threadIdx.x] += tc;
             __syncthreads();
         }
         if (i < ntiles) {
             tile_off[i] = carry + sh[threadIdx.x] - v;
             tile_cur[i] = 0;
-            const int c0 = carryc + shc[threadIdx.x] - nch;
-            tile_chunk[i] = c0;
-            for (int k = 0; k < nch; k++) chunks[c0 + k] = make_int2(i, k);
         }
         __syncthreads();
-        if (threadIdx.x == 1023) {
-            carry += sh[1023];
-            carryc += shc[1023];
-        }
+        if (threadIdx.x == 1023) carry += sh[1023];
         __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        *chunk_total = carryc;
-        tile_chunk[ntiles] = carryc;
     }
 }
 
@@ -611,19 +588,15 @@ __global__ __launch_bounds__(256) void k_spread_tiles(const SpreadRec<T> *__rest
     }
 }
 
-// Spreading, form 1.  One work group per chunk of a tile's home vertices
-// (the interpolation's work list, at most INTERP_CHUNK vertices) adds each
-// one's whole 4x4x4 stencil into an LDS box of the tile plus the 3 nodes
-// above it on each axis (its stencils cannot leave that box), in the fixed
-// point of form 0, with no per-node masks; the box is then written to the
-// chunk's staging slot, and k_spread_gather sums for each lattice node the
-// box entries that map to it, over the chunks of each tile in order.  Form 0
-// visits a vertex once per tile its stencil touches (2.2 tiles on average
-// with 16x8x8 tiles), each visit walking the 64 nodes under masks.  Chunks
-// keep a crowded tile from holding one CU for the whole launch: its LDS
-// atomics spread over as many CUs as it has chunks.  Lane l of wave w takes
-// column NW l + w of the chunk read as WG columns, so that the lanes of a
-// wave hold vertices far apart.
+// Spreading, form 1.  One work group per tile takes its home vertices only
+// and adds each one's whole 4x4x4 stencil into an LDS box of the tile plus
+// the 3 nodes above it on each axis (its stencils cannot leave that box), in
+// the fixed point of form 0, with no per-node masks; the box is then written
+// to a staging array, and k_spread_gather sums for each lattice node the box
+// entries that map to it.  Form 0 visits a vertex once per tile its stencil
+// touches (2.2 tiles on average with 16x8x8 tiles), each visit walking the 64
+// nodes under masks.  Lane l of wave w takes column NW l + w of the bin read
+// as WG columns, so that the lanes of a wave hold vertices far apart.
 constexpr int SB_X = TILE_X + 3, SB_Y = TILE_Y + 3, SB_Z = TILE_Z + 3, SB_N = SB_X * SB_Y * SB_Z;
 constexpr int SPREAD_BOX_WG = 512;
 constexpr int SB_MAXC = 4;  // box entries per lattice coordinate and axis (k_spread_gather)
@@ -635,14 +608,11 @@ __global__ __launch_bounds__(SPREAD_BOX_WG) void k_spread_box(const SpreadRec<T>
                                                               const int *__restrict__ tile_off, T *__restrict__ S,
                                                               IbmGeom g, TileGeom tg,
                                                               const unsigned *__restrict__ absmax,
-                                                              const int2 *__restrict__ chunks,
-                                                              const int *__restrict__ chunk_total) {
+                                                              const int *__restrict__ order) {
     __shared__ unsigned long long acc[3 * SB_N];
-    if ((int)blockIdx.x >= *chunk_total) return;  // whole work group leaves together
-    const int2 ch = chunks[blockIdx.x];
-    const int tile = ch.x;
-    const int off = tile_off[tile] + ch.y * INTERP_CHUNK;
-    const int cnt = min(INTERP_CHUNK, tile_count[tile] - ch.y * INTERP_CHUNK);
+    const int tile = order[blockIdx.x];
+    const int cnt = tile_count[tile], off = tile_off[tile];
+    if (cnt == 0) return;  // whole work group leaves together; the gather skips empty tiles
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
     const int ox = tx * TILE_X, oy = ty * TILE_Y, oz = tz * TILE_Z;
     for (int k = threadIdx.x; k < 3 * SB_N; k += blockDim.x) acc[k] = 0ull;
@@ -690,7 +660,7 @@ __global__ __launch_bounds__(SPREAD_BOX_WG) void k_spread_box(const SpreadRec<T>
     }
     __syncthreads();
     const double inv_scale = 1.0 / scale;
-    T *St = S + (size_t)blockIdx.x * 3 * SB_N;
+    T *St = S + (size_t)tile * 3 * SB_N;
     for (int k = threadIdx.x; k < SB_N; k += blockDim.x)
 #pragma unroll
         for (int d = 0; d < 3; d++)
@@ -698,13 +668,11 @@ __global__ __launch_bounds__(SPREAD_BOX_WG) void k_spread_box(const SpreadRec<T>
 }
 
 // Sums, per lattice node, the box entries of k_spread_box that map to it, in
-// a fixed order (deterministic): per (z, y, x) entry the chunk boxes of its
-// tile in chunk order.  cx/cy/cz list for each coordinate of an axis up to
-// SB_MAXC entries (tile index along the axis) * 32 + box coordinate, padded
-// with -1 (built once on the host); tile_chunk[t] is the first chunk of tile
-// t, tile_chunk[t + 1] the end.
+// a fixed order (deterministic).  cx/cy/cz list for each coordinate of an axis
+// up to SB_MAXC entries (tile index along the axis) * 32 + box coordinate,
+// padded with -1 (built once on the host).
 template <typename T>
-__global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, const int *__restrict__ tile_chunk,
+__global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, const int *__restrict__ tile_count,
                                                        const int *__restrict__ cx, const int *__restrict__ cy,
                                                        const int *__restrict__ cz, T *__restrict__ F, IbmGeom g,
                                                        TileGeom tg) {
@@ -712,9 +680,9 @@ __global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, 
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((size_t)g.nx * g.ny));
-    // all table entries first, then per (z, y) entry the chunk ranges and the
-    // first chunk's box values of every x entry, so that independent loads
-    // are in flight together; the sums keep the order (z, y, x entry, chunk)
+    // all table entries first, then per (z, y) entry the tile counts and box
+    // values of every x entry, so that independent loads are in flight
+    // together; the sums keep the order (z, y, x entry)
     int ex[SB_MAXC], ey[SB_MAXC], ez[SB_MAXC];
 #pragma unroll
     for (int i = 0; i < SB_MAXC; i++) {
@@ -731,36 +699,23 @@ __global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, 
             if (ey[iy] < 0) break;
             const int tyz = tg.ntx * ((ey[iy] >> 5) + tg.nty * (ez[iz] >> 5));
             const int byz = SB_X * ((ey[iy] & 31) + SB_Y * (ez[iz] & 31));
-            int c0[SB_MAXC], c1[SB_MAXC];
+            int cnt[SB_MAXC];
 #pragma unroll
-            for (int ix = 0; ix < SB_MAXC; ix++) {
-                c0[ix] = c1[ix] = 0;
-                if (ex[ix] >= 0) {
-                    const int t = (ex[ix] >> 5) + tyz;
-                    c0[ix] = tile_chunk[t];
-                    c1[ix] = tile_chunk[t + 1];
-                }
-            }
+            for (int ix = 0; ix < SB_MAXC; ix++) cnt[ix] = ex[ix] >= 0 ? tile_count[(ex[ix] >> 5) + tyz] : 0;
             T v0[SB_MAXC], v1[SB_MAXC], v2[SB_MAXC];
 #pragma unroll
             for (int ix = 0; ix < SB_MAXC; ix++) {
                 v0[ix] = v1[ix] = v2[ix] = T(0);
-                if (c0[ix] < c1[ix]) {
-                    const size_t b = (size_t)c0[ix] * 3 * SB_N + (ex[ix] & 31) + byz;
+                if (cnt[ix] > 0) {
+                    const size_t b = (size_t)((ex[ix] >> 5) + tyz) * 3 * SB_N + (ex[ix] & 31) + byz;
                     v0[ix] = S[b];
                     v1[ix] = S[b + SB_N];
                     v2[ix] = S[b + 2 * SB_N];
                 }
             }
 #pragma unroll
-            for (int ix = 0; ix < SB_MAXC; ix++) {
-                if (c0[ix] >= c1[ix]) continue;
-                a0 += v0[ix]; a1 += v1[ix]; a2 += v2[ix];
-                for (int c = c0[ix] + 1; c < c1[ix]; c++) {  // the tile's further chunks
-                    const size_t b = (size_t)c * 3 * SB_N + (ex[ix] & 31) + byz;
-                    a0 += S[b]; a1 += S[b + SB_N]; a2 += S[b + 2 * SB_N];
-                }
-            }
+            for (int ix = 0; ix < SB_MAXC; ix++)
+                if (cnt[ix] > 0) { a0 += v0[ix]; a1 += v1[ix]; a2 += v2[ix]; }
         }
     }
     F[n] = a0;
@@ -776,6 +731,7 @@ __global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, 
 // the stencil spans 4 nodes; stages that leave the box read global memory.
 constexpr int RX = TILE_X + 5, RY = TILE_Y + 5, RZ = TILE_Z + 5, RN = RX * RY * RZ;
 constexpr int RLO = 1;  // nodes of the box below the tile
+constexpr int INTERP_WG = 512;  // work-group size of k_interp_tiles
 
 // Out-of-line and not unrolled: it is the rare path, and inlining it into
 // every RK stage would multiply the kernel's code size.
@@ -962,26 +918,20 @@ __device__ __forceinline__ void advect_vertex(int j, const double *xin, double *
     xd[3 * (size_t)j] = xr; xd[3 * (size_t)j + 1] = yr; xd[3 * (size_t)j + 2] = zr;
 }
 
-// Interpolation form 0: one work group per chunk of a tile's home vertices
-// (at most INTERP_CHUNK) stages the tile's velocity plus halo in LDS and runs
-// the chunk's vertices from there.  The chunks of one tile stage the same
-// box, which the L2 serves after the first; a crowded tile then spreads
-// over several CUs instead of holding one for the whole launch.
+// Interpolation form 0: one work group per tile stages the tile's velocity
+// plus halo in LDS once and runs its home vertices from there.
 template <typename T>
-__global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, const double *xin, double *xo,
+__global__ __launch_bounds__(INTERP_WG) void k_interp_tiles(const T *__restrict__ U, const double *xin, double *xo,
                                                       T *__restrict__ uvw, double *__restrict__ xd,
                                                       const int *__restrict__ tile_count,
                                                       const int *__restrict__ tile_off,
                                                       const int *__restrict__ home_list, int nv, IbmGeom g,
-                                                      TileGeom tg, AdvectArgs a, const int2 *__restrict__ chunks,
-                                                      const int *__restrict__ chunk_total) {
+                                                      TileGeom tg, AdvectArgs a, const int *__restrict__ order) {
     extern __shared__ double dyn[];
     T *Us = (T *)dyn;  // [3][RZ][RY][RX]
-    if ((int)blockIdx.x >= *chunk_total) return;  // whole work group leaves together
-    const int2 ch = chunks[blockIdx.x];
-    const int tile = ch.x;
-    const int off = tile_off[tile] + ch.y * INTERP_CHUNK;
-    const int cnt = min(INTERP_CHUNK, tile_count[tile] - ch.y * INTERP_CHUNK);
+    const int tile = order[blockIdx.x];
+    const int cnt = tile_count[tile], off = tile_off[tile];
+    if (cnt == 0) return;  // whole work group leaves together
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
     const int bx = tx * TILE_X - RLO, by = ty * TILE_Y - RLO, bz = tz * TILE_Z - RLO;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
@@ -1007,14 +957,6 @@ __global__ __launch_bounds__(256) void k_interp_tiles(const T *__restrict__ U, c
                              interp_tile<T>(U, N, Us, g, bx, by, bz, x, y, z, unit, u, v, w);
                          });
     }
-}
-
-// Sum over the 4 lanes of an aligned quad; every lane gets the same bits
-// ((a + b) + (c + d) in each, addition being commutative).
-template <typename T> __device__ __forceinline__ T quad_sum(T v) {
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
-    return v;
 }
 
 // Interpolation form 1: one thread per vertex in bin order (neighbouring
@@ -1128,6 +1070,14 @@ template <typename W> struct StencilD {
         }
     }
 };
+
+// Sum over the 4 lanes of an aligned quad; every lane gets the same bits
+// ((a + b) + (c + d) in each, addition being commutative).
+template <typename T> __device__ __forceinline__ T quad_sum(T v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    return v;
+}
 
 // Adjoint of ibm_st1 at the current positions and nodal forces: given Fa (the
 // adjoint of the lattice force), cfa = sum D Fa is the adjoint of the nodal
@@ -1439,8 +1389,7 @@ template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
                        (const T *)spread_force(ctx), nv, g, tg, ctx->home, ctx->tile_count, ctx->blk_fmax);
     LBMIBM_LAUNCHED(ctx, "k_bin_count");
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->tile_off, ctx->tile_cur,
-                       ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax, ctx->chunks, ctx->chunk_total,
-                       ctx->tile_chunk);
+                       ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax);
     LBMIBM_LAUNCHED(ctx, "k_tile_scan");
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->ntiles, g, tg,
                        ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->order_calls);
@@ -1460,13 +1409,13 @@ template <typename T> lbmibm_status spread_launch(lbmibm_ctx *ctx, int form) {
     const IbmGeom g = geom_of(ctx);
     if (form == 1) {
         LBMIBM_KT(ctx, ctx->stream, "k_spread_box",
-                  hipLaunchKernelGGL(k_spread_box<T>, dim3(ctx->nchunk_max), dim3(SPREAD_BOX_WG), 0, ctx->stream,
+                  hipLaunchKernelGGL(k_spread_box<T>, dim3(ctx->ntiles), dim3(SPREAD_BOX_WG), 0, ctx->stream,
                                      (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off,
-                                     (T *)ctx->sbox, g, tg, ctx->absmax, ctx->chunks, ctx->chunk_total));
+                                     (T *)ctx->sbox, g, tg, ctx->absmax, ctx->order_interp));
         LBMIBM_LAUNCHED(ctx, "k_spread_box");
         LBMIBM_KT(ctx, ctx->stream, "k_spread_gather",
                   hipLaunchKernelGGL(k_spread_gather<T>, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0,
-                                     ctx->stream, (const T *)ctx->sbox, ctx->tile_chunk, ctx->sb_cand,
+                                     ctx->stream, (const T *)ctx->sbox, ctx->tile_count, ctx->sb_cand,
                                      ctx->sb_cand + SB_MAXC * ctx->nx, ctx->sb_cand + SB_MAXC * (ctx->nx + ctx->ny),
                                      (T *)ctx->F, g, tg));
         LBMIBM_LAUNCHED(ctx, "k_spread_gather");
@@ -1514,9 +1463,9 @@ lbmibm_status interp_launch(lbmibm_ctx *ctx, int form, const double *xin, double
     TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const size_t lds = 3 * (size_t)RN * sizeof(T);
     LBMIBM_KT(ctx, ctx->stream, "k_interp_tiles",
-              hipLaunchKernelGGL(k_interp_tiles<T>, dim3(ctx->nchunk_max), dim3(256), lds, ctx->stream,
+              hipLaunchKernelGGL(k_interp_tiles<T>, dim3(ctx->ntiles), dim3(INTERP_WG), lds, ctx->stream,
                                  (const T *)ctx->U, xin, xo, uvw, xd, ctx->tile_count, ctx->tile_off, ctx->home_list,
-                                 ctx->nv, geom_of(ctx), tg, a, ctx->chunks, ctx->chunk_total));
+                                 ctx->nv, geom_of(ctx), tg, a, ctx->order_interp));
     LBMIBM_LAUNCHED(ctx, "k_interp_tiles");
     return LBMIBM_OK;
 }
@@ -1689,7 +1638,7 @@ static lbmibm_status spread_box_setup(lbmibm_ctx *ctx, bool *ok) {
         LBMIBM_HIP(ctx, hipMemcpy(ctx->sb_cand, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     if (!ctx->sbox) {
-        lbmibm_status s = dev_alloc(ctx, &ctx->sbox, (size_t)ctx->nchunk_max * 3 * SB_N * ctx->szT);
+        lbmibm_status s = dev_alloc(ctx, &ctx->sbox, (size_t)ctx->ntiles * 3 * SB_N * ctx->szT);
         if (s != LBMIBM_OK) return s;
     }
     *ok = true;
@@ -1734,11 +1683,11 @@ template <typename T> lbmibm_status spread_autotune_impl(lbmibm_ctx *ctx) {
 }
 
 // a form fixed in the environment: "0" or "1" into *form, -1 when unset
-static lbmibm_status env_form(const char *name, int *form, char max = '1') {
+static lbmibm_status env_form(const char *name, int *form) {
     *form = -1;
     const char *v = getenv(name);
     if (!v || !v[0]) return LBMIBM_OK;
-    if (v[0] < '0' || v[0] > max || v[1]) return LBMIBM_ERR_ARG;
+    if ((v[0] != '0' && v[0] != '1') || v[1]) return LBMIBM_ERR_ARG;
     *form = v[0] - '0';
     return LBMIBM_OK;
 }

@@ -60,12 +60,13 @@ template <> struct Corner<float> { using type = float4; };
 template <> struct Corner<double> { using type = double4; };
 
 // One element's calc_tension quantities in FP64 (stress.cu:63-99): edges with
-// minimum image, the stored tension tk (rounded to T, as the reference's
-// calc_load reads td), gab = rg rounded to T, and the invariants for tp.
+// minimum image, the tension t, gab = rg rounded to T, and the invariants
+// for tp.  The corner terms use t unrounded: near the reference state they
+// are small sums of O(1) products, so a tension rounded to FP32 would cost
+// up to 1e-3 of the nodal force.
 template <typename T> struct ElemTension {
     double v1[3], v2[3];
-    double t[4];   // tension before rounding
-    double tk[4];  // (double)(T) tension
+    double t[4];
     double rg, gab, i1, i2;
     __device__ __forceinline__ ElemTension(const double *__restrict__ xd, int ea, int eb, int ec,
                                            const double *__restrict__ nr, const PeriodicBox &box,
@@ -94,7 +95,6 @@ template <typename T> struct ElemTension {
         for (int k = 0; k < 4; k++) {
             if (!a.neo) t[k] = a.rgs * (i1 + 1.0) / js * nn[k] + (a.c * i2 - 1.0) * js * nd[k];
             else t[k] = a.rgs * (nn[k] / js - nd[k] * js / (i2 + 1.0) / (i2 + 1.0));
-            tk[k] = (double)(T)t[k];
         }
         gab = (double)(T)rg;
     }
@@ -110,7 +110,7 @@ template <typename T> struct ElemTension {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int al = k >> 1, be = k & 1;
-                sum += gab * tk[k] * (vd[al] * F[en][be] + vd[be] * F[en][al]) / 4.0;
+                sum += gab * t[k] * (vd[al] * F[en][be] + vd[be] * F[en][al]) / 4.0;
             }
             tb[d] = sum;
         }
@@ -218,6 +218,86 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
 #pragma unroll
     for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
     dsv[jj] = (T)(ds * (double)gscale[vcell[jj]]);
+}
+
+// Fused assembly (forward stress): one work group per chunk of up to ASM_CH
+// consecutive vertices of a cell computes, for every element incident to
+// them (a static list built at set-up; elements on a chunk boundary are
+// computed by each chunk they touch), the tension and the corner terms of
+// the chunk's vertices into LDS, in FP64, and then writes the rows of
+// k_vert from LDS.  The corner records never reach memory, and the
+// right-hand sides are summed from unrounded corner terms: k_elem + k_vert
+// round each term to T first, which in FP32 loses up to 1e-4 of a small
+// nodal force to cancellation.  The tension and rg outputs and the tp
+// accumulator are written by the chunk holding the element's first vertex.
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble(const int *__restrict__ list, const int4 *__restrict__ chunks,
+                                                  const int *__restrict__ elems, const double *__restrict__ xd,
+                                                  const int *__restrict__ ele, const unsigned char *__restrict__ eslot,
+                                                  const double *__restrict__ nref, T *__restrict__ tt,
+                                                  T *__restrict__ rg, T *__restrict__ tp,
+                                                  const int *__restrict__ nln, const unsigned char *__restrict__ epair,
+                                                  const int *__restrict__ vcell, const T *__restrict__ gscale,
+                                                  T *__restrict__ coef, T *__restrict__ bv, T *__restrict__ dsv,
+                                                  int nv, PeriodicBox box, TensionArgs a,
+                                                  const DevStep *__restrict__ st) {
+    __shared__ double4 slot[ASM_CH * 6];  // [local vertex][incident element]
+    const int4 ch = chunks[list[blockIdx.x]];
+    const int cv0 = ch.x, cnv = ch.y, e0 = ch.z, ne = ch.w;
+    for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+        const int e = elems[e0 + i];
+        const int ev[3] = {ele[3 * (size_t)e], ele[3 * (size_t)e + 1], ele[3 * (size_t)e + 2]};
+        const ElemTension<T> el(xd, ev[0], ev[1], ev[2], nref + 4 * (size_t)e, box, a);
+        if ((unsigned)(ev[0] - cv0) < (unsigned)cnv) {  // the element's owner chunk
+            if (tt) {
+                rg[e] = (T)el.rg;
+#pragma unroll
+                for (int k = 0; k < 4; k++) tt[4 * (size_t)e + k] = (T)el.t[k];
+            }
+            if (tp) el.accumulate_tp(tp, e, st->iter, a);
+        }
+#pragma unroll
+        for (int en = 0; en < 3; en++) {
+            const int jl = ev[en] - cv0;
+            if ((unsigned)jl >= (unsigned)cnv) continue;
+            double tb[3];
+            el.corner(en, tb);
+            slot[6 * jl + eslot[3 * (size_t)e + en]] = make_double4(tb[0], tb[1], tb[2], el.gab);
+        }
+    }
+    __syncthreads();
+    for (int jl = threadIdx.x; jl < cnv; jl += blockDim.x) {
+        const int jj = cv0 + jl;
+        const int n = nln[jj];
+        double r24[6], tb[3] = {0.0, 0.0, 0.0}, jac = 0.0, ds = 0.0;
+#pragma unroll
+        for (int s = 0; s < 6; s++) {  // as k_vert
+            r24[s] = 0.0;
+            if (s < n) {
+                const double4 cv = slot[6 * jl + s];
+                r24[s] = cv.w / 24.0;
+                jac += 2.0 * r24[s];
+                ds += 4.0 * r24[s];
+                tb[0] += cv.x;
+                tb[1] += cv.y;
+                tb[2] += cv.z;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            const unsigned ep = epair[(size_t)m * nv + jj];
+            double ra = 0.0, rb = 0.0;
+#pragma unroll
+            for (int s = 0; s < 6; s++) {
+                if ((int)(ep & 15u) == s) ra = r24[s];
+                if ((int)(ep >> 4) == s) rb = r24[s];
+            }
+            coef[(size_t)m * nv + jj] = ep == 0xffu ? T(0) : (T)((ra + rb) / jac);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
+        dsv[jj] = (T)(ds * (double)gscale[vcell[jj]]);
+    }
 }
 
 // Solver epilogue per vertex: the load q (fem->q, substitute_load
@@ -1306,6 +1386,24 @@ lbmibm_status assemble_impl(lbmibm_ctx *ctx, const PeriodicBox &box, const Tensi
     return LBMIBM_OK;
 }
 
+// Forward assembly of the chunks in list `which` (0 all cells, 1 the large
+// cells, 2 the on-chip cells) with k_assemble.
+template <typename T>
+lbmibm_status assemble_fused(lbmibm_ctx *ctx, const PeriodicBox &box, const TensionArgs &ta, int which) {
+    const lbmibm_params &p = ctx->p;
+    const int n = ctx->asm_n[which];
+    if (n == 0) return LBMIBM_OK;
+    LBMIBM_KT(ctx, ctx->stream, "k_assemble",
+              hipLaunchKernelGGL(k_assemble<T>, dim3(n), dim3(256), 0, ctx->stream, ctx->asm_list[which],
+                                 ctx->asm_chunks, ctx->asm_elems, (const double *)ctx->xd, ctx->ele, ctx->eslot,
+                                 (const double *)ctx->nref, p.store_macros ? (T *)ctx->tt : (T *)nullptr,
+                                 (T *)ctx->rg, p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ctx->nln,
+                                 ctx->epair, ctx->vcell, (const T *)ctx->gscale, (T *)ctx->coef, (T *)ctx->bv,
+                                 (T *)ctx->dsv, ctx->nv, box, ta, ctx->step));
+    LBMIBM_LAUNCHED(ctx, "k_assemble");
+    return LBMIBM_OK;
+}
+
 PeriodicBox box_of(const lbmibm_ctx *ctx) {
     return PeriodicBox{ctx->nx * ctx->p.dx, ctx->ny * ctx->p.dx, ctx->nz * ctx->p.dx, ctx->p.periodic_y};
 }
@@ -1393,10 +1491,8 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx, int warm) {
     const int ne = ctx->nt, nv = ctx->nv;
     // with large cells, their systems are assembled first so that their
     // solve (the longest) starts while the others are assembled and solved
-    const bool split = ctx->bicg_n[2] > 0 && ctx->bicg_n[2] <= 65535 && ctx->n_onchip <= 65535;  // grid y
-    lbmibm_status st = split ? assemble_impl<T>(ctx, box, ta, ctx->bicg_cells[2], ctx->bicg_n[2], ctx->cls_maxe[1],
-                                                ctx->bicg_maxv[2])
-                             : assemble_impl<T>(ctx, box, ta);
+    const bool split = ctx->bicg_n[2] > 0;
+    lbmibm_status st = assemble_fused<T>(ctx, box, ta, split ? 1 : 0);
     if (st != LBMIBM_OK) return st;
     const SolveOut<T> out = solve_out<T>(ctx);
     if (ctx->bicg_n[2] > 0) {
@@ -1408,7 +1504,7 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx, int warm) {
         lbmibm_status sb = launch_big<T>(ctx, ctx->bicg_var[2], out, ctx->side, warm);
         if (sb != LBMIBM_OK) return sb;
         if (split && ctx->n_onchip > 0) {
-            sb = assemble_impl<T>(ctx, box, ta, ctx->onchip_cells, ctx->n_onchip, ctx->cls_maxe[0], ctx->cls_maxv[0]);
+            sb = assemble_fused<T>(ctx, box, ta, 2);
             if (sb != LBMIBM_OK) return sb;
         }
     }
@@ -1680,9 +1776,11 @@ __global__ __launch_bounds__(256) void k_vert_adj(const int *__restrict__ nln, c
 template <typename T> lbmibm_status forces_adj_impl(lbmibm_ctx *ctx, bool have_q) {
     const lbmibm_params &p = ctx->p;
     const int ne = ctx->nt, nv = ctx->nv;
-    // forward at the current positions: rows, ring areas, corner records, and
-    // q unless the caller restored it (qv) from a recording of this step
-    lbmibm_status s = have_q ? assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx)) : forces_impl<T>(ctx, false);
+    // forward at the current positions: q unless the caller restored it (qv)
+    // from a recording of this step, then the rows, ring areas and the corner
+    // records (k_elem + k_vert: k_adj_load reads the records)
+    lbmibm_status s = have_q ? LBMIBM_OK : forces_impl<T>(ctx, false);
+    if (s == LBMIBM_OK) s = assemble_impl<T>(ctx, box_of(ctx), tension_args(ctx));
     if (s != LBMIBM_OK) return s;
     hipLaunchKernelGGL(k_adj_load<T>, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, ctx->nln,
                        (const typename Corner<T>::type *)ctx->corner, (const T *)ctx->dsv, (const T *)ctx->qv,

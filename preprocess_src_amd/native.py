@@ -111,6 +111,7 @@ SIGNATURES = {
     "lbmibm_upload_vf": (C.c_int, [C.c_void_p, c_double_p]),
     "lbmibm_adhesion_setup": (C.c_int, [C.c_void_p, C.POINTER(AdhesionParams), C.POINTER(C.c_ubyte)]),
     "lbmibm_adhesion": (C.c_int, [C.c_void_p]),
+    "lbmibm_write_adhesion": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, c_int_p]),
     "lbmibm_adhesion_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_ubyte), c_int_p, c_double_p, c_int_p, c_double_p,
                                         c_int_p, c_double_p]),
     "lbmibm_set_body_force_field": (C.c_int, [C.c_void_p] + [c_double_p] * 3),

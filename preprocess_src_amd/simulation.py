@@ -281,6 +281,13 @@ class Simulation:
     def adhesion(self):
         self._ck(self.L.lbmibm_adhesion(self.ctx))
 
+    def write_adhesion(self, directory, it, fnum_ad):
+        """adhe%05d.bin and restart_adhe.dat under directory/adhesion; returns
+        the next file number."""
+        f = C.c_int(int(fnum_ad))
+        self._ck(self.L.lbmibm_write_adhesion(self.ctx, str(directory).encode(), int(it), C.byref(f)))
+        return f.value
+
     def adhesion_state(self, nrec):
         A = self._adh
         nv, mn, ns = self.nv, A.max_neighwall, A.springs_per_knob

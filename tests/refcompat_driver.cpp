@@ -3,14 +3,15 @@
 // loop body is main.cu:73-110 with the M gating.  Built and run by
 // tests/test_refcompat.py: without arguments it only checks that the
 // adaptor compiles and links and that default parameters come back; with
-// "run" it drives a small channel on the GPU.
+// "run" it drives a small channel on the GPU, with "vof" the VOF stages,
+// and with "cells <in> <out>" a set of cells read from a case file.
 #include <cstdio>
 #include <cstring>
 #include <vector>
 
 #include "lbmibm_refcompat.hpp"
 
-struct process { int iter, fnum; };
+struct process { int iter, fnum, fnum_ad; };
 struct domain { int restart; };
 struct lattice {};
 struct cell {};
@@ -53,7 +54,7 @@ static int run_cells(const char *in, const char *out) {
     p.store_macros = 1;
     lbmibm_cells cs{nc, voff.data(), eoff.data(), x.data(), y.data(), z.data(), ele.data()};
     lbmibm_ref::attach(p, cs);
-    process prc{0, 0};
+    process prc{0, 0, 0};
     domain cdo{0};
     lattice ltc;
     cell cel;
@@ -121,7 +122,7 @@ int main(int argc, char **argv) {
         p.sor_residual = 1e-8;
         p.volume_correction = 0;
         lbmibm_ref::attach(p, one);
-        process prc{0, 0};
+        process prc{0, 0, 0};
         domain cdo{0};
         lattice ltc;
         cell cel;
@@ -158,7 +159,7 @@ int main(int argc, char **argv) {
     p.p = 5e-6;
     lbmibm_cells none{};
     lbmibm_ref::attach(p, none);
-    process prc{0, 0};
+    process prc{0, 0, 0};
     domain cdo{0};
     lattice ltc;
     cell cel;
@@ -176,7 +177,7 @@ int main(int argc, char **argv) {
         if (prc.iter % 10 == 0) output_data(&prc, &cdo, &ltc, &cel, &fm, &wall, &wall);
     }
     // restart.bin was last written at iter 20 with file number 2
-    process back{0, 0};
+    process back{0, 0, 0};
     cdo.restart = 1;
     restart_file(&back, &cdo, &ltc, &cel, &fm, &wall, &wall);
     if (back.iter != 20 || back.fnum != 2) return 4;

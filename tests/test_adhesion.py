@@ -9,6 +9,8 @@ that attach or detach in one call are p = 1 - exp(-k dt M).  GPU: the
 library's knobs, receptor lists, attachments (integers, exactly) and forces
 against the oracle over whole steps.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -232,6 +234,32 @@ def test_adhesion_steps_match_the_oracle(prec, tol, law):
     sg = g.cells_state()
     for k in ("x", "y", "z"):
         assert np.max(np.abs(sg[k] - o.cells[k])) < tol * 10, k
+
+
+@pytest.mark.gpu
+def test_adhesion_file_has_the_reference_layout(tmp_path):
+    """adhe%05d.bin of Adhesion() (adhesion.cu:785-822) after four steps: the
+    bytes equal the layout restated in oracle/reffiles.py from the library's
+    own adhesion arrays, whose values the other tests hold to the oracle;
+    restart_adhe.dat holds the file number and iteration, and the file
+    number advances."""
+    from oracle import reffiles
+    print("[adhesion file]", flush=True)
+    o, g, st, nrec = _pair("f64", tau=0.9, p=5e-6, b=5e-3)
+    g.volume_init()
+    for it in range(1, 5):
+        g.step(it)
+    os.makedirs(tmp_path / "adhesion")
+    nxt = g.write_adhesion(tmp_path, 4, 7)
+    assert nxt == 8
+    gs = g.adhesion_state(nrec)
+    buf = (tmp_path / "adhesion" / "adhe00007.bin").read_bytes()
+    ns = ADH["springs_per_knob"]
+    assert buf == reffiles.adhesion_bytes(7, 4, gs["f_adh"], gs["attach"], gs["is_knob"], gs["is_attached"],
+                                          gs["r_ij"])
+    parsed = reffiles.parse_adhesion(buf, g.nv, ns, nrec)
+    assert parsed["iter"] == 4 and np.array_equal(parsed["attach"], gs["attach"])
+    assert (tmp_path / "adhesion" / "restart_adhe.dat").read_text() == "7\n4\n"
 
 
 @pytest.mark.gpu

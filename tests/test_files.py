@@ -216,3 +216,20 @@ def test_output_data_names_files_as_the_reference(tmp_path):
     assert names == ["cell", "prof0000.bin", "restart.bin", "restart0000.bin",
                      "cell0000.bin", "cell0001.bin", "cell0002.bin"], names
     assert fnum == 3
+
+
+def test_adhesion_file_layout_round_trip():
+    """The restated adhe%05d.bin layout (adhesion.cu:785-810): size formula and
+    a parse of the written bytes give the arrays back."""
+    from oracle import reffiles
+    rng = np.random.default_rng(5)
+    nv, ns, nrec = 37, 2, 53
+    a = dict(f_adh=rng.standard_normal(3 * nv), attach=rng.integers(-1, nrec, nv * ns).astype(np.int32),
+             is_knob=(rng.random(nv) < 0.3).astype(np.uint8), is_attached=rng.integers(0, 2, nrec).astype(np.int32),
+             r_ij=rng.random(nv * ns))
+    buf = reffiles.adhesion_bytes(3, 41, **a)
+    assert len(buf) == 4 + 4 + 8 * 3 * nv + 4 * nv * ns + nv + 4 * nrec + 8 * nv * ns
+    p = reffiles.parse_adhesion(buf, nv, ns, nrec)
+    assert p["fnum_ad"] == 3 and p["iter"] == 41
+    for k, v in a.items():
+        assert np.array_equal(p[k], v), k
