@@ -412,14 +412,27 @@ def test_vof_arguments_and_states_are_checked(tmp_path):
     assert np.array_equal(h.vf(), g.vf())
 
 
+TUBE_VOF_CENTRES = {
+    "axis": (0.4, -0.3, 0.2),
+    # stencils and indicator reach the +z halo: the z-only fold, the z BUFF
+    # copies of THINC and the periodic SOR rows act on vf > 0
+    "z end": (0.4, -0.3, 7.0 - 3.3),
+    # 0.4 lu from the wall: the THINC face closure and the WALL rows of the
+    # SOR meet the interface
+    "wall": (4.1, 0.3, 0.2),
+}
+
+
 @pytest.mark.gpu
+@pytest.mark.parametrize("where", list(TUBE_VOF_CENTRES))
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-7), ("f32", 2e-5)])
-def test_vof_in_the_tube_matches_the_oracle(prec, tol):
+def test_vof_in_the_tube_matches_the_oracle(prec, tol, where):
     """CapsuleCHANNELFLOW: the indicator of a cell in the tube (walls outside
     the disc, z periodic; the fold, the BUFF rows and the THINC copies in z
-    only) and steps with a viscosity contrast."""
-    print(f"[vof tube] prec={prec}", flush=True)
-    cells = _sphere((0.4, -0.3, 0.2), 3.0, level=3)
+    only) and steps with a viscosity contrast, for a cell on the axis, one
+    whose indicator crosses the z period and one next to the wall."""
+    print(f"[vof tube] prec={prec} cell={where}", flush=True)
+    cells = _sphere(TUBE_VOF_CENTRES[where], 3.0, level=3)
     kw = dict(tube_radius=7.5, tau=0.9, p=2e-5, e=(0.0, 0.0, 1.0), force_modulated=2, flowstart=0, b=5e-3,
               lambda1=5.0, vof_period=2, **VOF_KW)
     o = oracle_of(18, 18, 14, cells=cells, **kw)
@@ -432,7 +445,14 @@ def test_vof_in_the_tube_matches_the_oracle(prec, tol):
     assert rel(g.vf()[keep], o.vf[keep]) < 1e-8
     assert not g.vf()[o.bc == 2].any()
     comp = o.bc == 0
-    assert abs(o.vf[comp].sum() / (4 / 3 * np.pi * 27) - 1) < 0.03
+    # next to the wall the WALL rows hold vf = 0 and take about 13% of the volume
+    assert abs(o.vf[comp].sum() / (4 / 3 * np.pi * 27) - 1) < (0.15 if where == "wall" else 0.03)
+    if where == "z end":
+        nz_f = o.vf.reshape(o.nz, o.ny, o.nx)
+        assert nz_f[2].max() > 0.05 and nz_f[-3].max() > 0.3  # the indicator wraps through z
+    if where == "wall":
+        r = np.hypot(*o.node_coords()[:2])
+        assert o.vf[comp & (r > 6.4)].max() > 0.05  # the interface reaches the wall's last fluid ring
     for it in range(1, 5):
         assert o.step(it)[0] == 0
         g.step(it)
