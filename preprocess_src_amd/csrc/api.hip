@@ -263,6 +263,8 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->step, ctx->status, ctx->node0, ctx->home, ctx->home_list, ctx->tile_count,
                     ctx->tile_off, ctx->tile_cur, ctx->order_calls, ctx->blk_fmax, ctx->absmax, ctx->nbr, ctx->bicg_cells[0],
                     ctx->bicg_cells[1], ctx->bicg_cells[2], ctx->vol_cells[0], ctx->vol_cells[1], ctx->dsv,
+                    ctx->vol_echunks, ctx->vol_vchunks, ctx->vol_crange, ctx->vol_epart, ctx->vol_vpart,
+                    ctx->vol_need,
                     ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->corner, ctx->cslot, ctx->epair,
                     ctx->asm_chunks, ctx->asm_elems, ctx->asm_list[0], ctx->asm_list[1], ctx->asm_list[2], ctx->eslot,
                     ctx->srec, ctx->onchip_cells, ctx->Hb, ctx->adj_f[0], ctx->adj_f[1], ctx->adj_u, ctx->adj_F,
@@ -598,6 +600,30 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         for (int k = 0; k < 2; k++) {
             ctx->vol_n[k] = (int)vcls[k].size();
             if (!vcls[k].empty()) CK(up_i(&ctx->vol_cells[k], vcls[k].data(), vcls[k].size()));
+        }
+        {
+            // volume-correction chunks of VOL_CH elements / vertices, each
+            // within one cell
+            std::vector<int4> ech, vch, range(nc);
+            for (int c = 0; c < nc; c++) {
+                range[c] = make_int4((int)ech.size(), 0, (int)vch.size(), 0);
+                for (int e = cells->eoff[c]; e < cells->eoff[c + 1]; e += VOL_CH)
+                    ech.push_back(make_int4(c, e, std::min(VOL_CH, cells->eoff[c + 1] - e), 0));
+                for (int v = cells->voff[c]; v < cells->voff[c + 1]; v += VOL_CH)
+                    vch.push_back(make_int4(c, v, std::min(VOL_CH, cells->voff[c + 1] - v), 0));
+                range[c].y = (int)ech.size() - range[c].x;
+                range[c].w = (int)vch.size() - range[c].z;
+            }
+            ctx->vol_ne = (int)ech.size();
+            ctx->vol_nvc = (int)vch.size();
+            for (auto *t : {&ech, &vch, &range}) {
+                int4 **dst = t == &ech ? &ctx->vol_echunks : t == &vch ? &ctx->vol_vchunks : &ctx->vol_crange;
+                CK(dev_alloc(ctx, (void **)dst, t->size() * sizeof(int4)));
+                LBMIBM_HIP(ctx, hipMemcpy(*dst, t->data(), t->size() * sizeof(int4), hipMemcpyHostToDevice));
+            }
+            CK(dev_alloc(ctx, (void **)&ctx->vol_epart, ech.size() * sizeof(double2)));
+            CK(dev_alloc(ctx, (void **)&ctx->vol_vpart, vch.size() * sizeof(double4)));
+            CK(dev_alloc(ctx, (void **)&ctx->vol_need, nc * sizeof(int)));
         }
         CK(membrane_setup(ctx));
         std::vector<double> ones(nc, 1.0);

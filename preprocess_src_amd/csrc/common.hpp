@@ -35,6 +35,7 @@ struct DevStatus {
 
 // Lattice tiles of the immersed-boundary kernels.
 constexpr int VOL_SMALL = 3072;
+constexpr int VOL_CH = 1024;  // elements / vertices per volume-correction chunk
 constexpr int TILE_X = 16, TILE_Y = 8, TILE_Z = 8;
 constexpr int ASM_CH = 256;  // vertices per chunk of the fused membrane assembly (k_assemble)
 
@@ -136,6 +137,14 @@ struct lbmibm_ctx {
     // volume-correction work-group classes: up to VOL_SMALL vertices, larger
     int *vol_cells[2] = {nullptr, nullptr};
     int vol_n[2] = {0, 0};
+    // volume-correction chunks: elements (first pass sums) and vertices
+    // (centroid, rewrap) of one cell each, VOL_CH per chunk; per cell the
+    // first element chunk, their count, the first vertex chunk, their count
+    int4 *vol_echunks = nullptr, *vol_vchunks = nullptr, *vol_crange = nullptr;
+    int vol_ne = 0, vol_nvc = 0;
+    double2 *vol_epart = nullptr;  // [vol_ne] (V, A) of each element chunk
+    double4 *vol_vpart = nullptr;  // [vol_nvc] centroid sums of each vertex chunk
+    int *vol_need = nullptr;       // [nc] 1: the cell ran correction passes
     void *bv = nullptr, *rv = nullptr, *rh = nullptr, *pv = nullptr, *vv = nullptr, *tv = nullptr;
     void *gscale = nullptr;                            // [nc] (T)
     void *dsv = nullptr;                               // [Nv] ring area x force scale (T)
