@@ -589,7 +589,16 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
             CK(dev_alloc(ctx, (void **)&ctx->asm_chunks, chunks.size() * sizeof(int4)));
             LBMIBM_HIP(ctx, hipMemcpy(ctx->asm_chunks, chunks.data(), chunks.size() * sizeof(int4),
                                       hipMemcpyHostToDevice));
-            CK(up_i(&ctx->asm_elems, elems.data(), elems.size()));
+            {
+                std::vector<int4> er(elems.size());
+                for (size_t i = 0; i < elems.size(); i++) {
+                    const int e = elems[i];
+                    er[i] = make_int4(e, cells->ele[3 * (size_t)e], cells->ele[3 * (size_t)e + 1],
+                                      cells->ele[3 * (size_t)e + 2]);
+                }
+                CK(dev_alloc(ctx, (void **)&ctx->asm_elems, er.size() * sizeof(int4)));
+                LBMIBM_HIP(ctx, hipMemcpy(ctx->asm_elems, er.data(), er.size() * sizeof(int4), hipMemcpyHostToDevice));
+            }
             for (int k = 0; k < 3; k++) {
                 ctx->asm_n[k] = (int)lists[k].size();
                 if (!lists[k].empty()) CK(up_i(&ctx->asm_list[k], lists[k].data(), lists[k].size()));

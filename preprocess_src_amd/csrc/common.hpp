@@ -114,7 +114,7 @@ struct lbmibm_ctx {
     // fused assembly: chunks (first vertex, vertices, first entry of
     // asm_elems, elements) and the chunk lists of all / large / on-chip cells
     int4 *asm_chunks = nullptr;
-    int *asm_elems = nullptr;
+    int4 *asm_elems = nullptr;  // (element, its three vertices)
     int *asm_list[3] = {nullptr, nullptr, nullptr};
     int asm_n[3] = {0, 0, 0};
     unsigned char *eslot = nullptr;                    // [3Nt] slot of each corner in its vertex's list

@@ -79,22 +79,25 @@ template <typename T> struct ElemTension {
         min_img3<double>(v1, box);
         min_img3<double>(v2, box);
         const double n0 = nr[0], n1 = nr[1], n2 = nr[2], n3 = nr[3];
-        const double l1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
-        const double l2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
-        const double i12 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
-        const double dv = (l1 * l1) * (l2 * l2) - i12 * i12;
+        // the metric g = (|v1|^2, v1.v2, |v2|^2) and its inverse; one
+        // division each for 1/det g and 1/sqrt(det G / det g)
+        const double g0 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+        const double g3 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
+        const double g1 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+        const double dv = g0 * g3 - g1 * g1;
         const double dn = n0 * n3 - n1 * n2;
         rg = sqrt(dv);
-        const double g0 = l1 * l1, g1 = i12, g3 = l2 * l2;
-        const double nd0 = g3 / dv, nd1 = -g1 / dv, nd3 = g0 / dv;
+        const double idv = 1.0 / dv;
+        const double nd0 = g3 * idv, nd1 = -g1 * idv, nd3 = g0 * idv;
         i1 = n0 * g0 + n1 * g1 + n2 * g1 + n3 * g3 - 2.0;
         i2 = dv * dn - 1.0;
-        const double js = sqrt(i2 + 1.0);
+        const double js = sqrt(i2 + 1.0), ijs = 1.0 / js;
         const double nn[4] = {n0, n1, n2, n3}, nd[4] = {nd0, nd1, nd1, nd3};
+        const double ta = a.rgs * (i1 + 1.0) * ijs, tb = (a.c * i2 - 1.0) * js;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            if (!a.neo) t[k] = a.rgs * (i1 + 1.0) / js * nn[k] + (a.c * i2 - 1.0) * js * nd[k];
-            else t[k] = a.rgs * (nn[k] / js - nd[k] * js / (i2 + 1.0) / (i2 + 1.0));
+            if (!a.neo) t[k] = ta * nn[k] + tb * nd[k];
+            else t[k] = a.rgs * (nn[k] * ijs - nd[k] * js * (ijs * ijs) * (ijs * ijs));
         }
         gab = (double)(T)rg;
     }
@@ -204,6 +207,7 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
             tb[2] += (double)cv.z;
         }
     }
+    const double ijac = 1.0 / jac;
 #pragma unroll
     for (int m = 0; m < 6; m++) {
         const unsigned ep = epair[(size_t)m * nv + jj];
@@ -213,10 +217,10 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
             if ((int)(ep & 15u) == s) ra = r24[s];
             if ((int)(ep >> 4) == s) rb = r24[s];
         }
-        coef[(size_t)m * nv + jj] = ep == 0xffu ? T(0) : (T)((ra + rb) / jac);
+        coef[(size_t)m * nv + jj] = ep == 0xffu ? T(0) : (T)((ra + rb) * ijac);
     }
 #pragma unroll
-    for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
+    for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] * ijac);
     dsv[jj] = (T)(ds * (double)gscale[vcell[jj]]);
 }
 
@@ -232,8 +236,8 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
 // accumulator are written by the chunk holding the element's first vertex.
 template <typename T>
 __global__ __launch_bounds__(256) void k_assemble(const int *__restrict__ list, const int4 *__restrict__ chunks,
-                                                  const int *__restrict__ elems, const double *__restrict__ xd,
-                                                  const int *__restrict__ ele, const unsigned char *__restrict__ eslot,
+                                                  const int4 *__restrict__ elems, const double *__restrict__ xd,
+                                                  const unsigned char *__restrict__ eslot,
                                                   const double *__restrict__ nref, T *__restrict__ tt,
                                                   T *__restrict__ rg, T *__restrict__ tp,
                                                   const int *__restrict__ nln, const unsigned char *__restrict__ epair,
@@ -245,8 +249,8 @@ __global__ __launch_bounds__(256) void k_assemble(const int *__restrict__ list, 
     const int4 ch = chunks[list[blockIdx.x]];
     const int cv0 = ch.x, cnv = ch.y, e0 = ch.z, ne = ch.w;
     for (int i = threadIdx.x; i < ne; i += blockDim.x) {
-        const int e = elems[e0 + i];
-        const int ev[3] = {ele[3 * (size_t)e], ele[3 * (size_t)e + 1], ele[3 * (size_t)e + 2]};
+        const int4 er = elems[e0 + i];
+        const int e = er.x, ev[3] = {er.y, er.z, er.w};
         const ElemTension<T> el(xd, ev[0], ev[1], ev[2], nref + 4 * (size_t)e, box, a);
         if ((unsigned)(ev[0] - cv0) < (unsigned)cnv) {  // the element's owner chunk
             if (tt) {
@@ -283,6 +287,7 @@ __global__ __launch_bounds__(256) void k_assemble(const int *__restrict__ list, 
                 tb[2] += cv.z;
             }
         }
+        const double ijac = 1.0 / jac;
 #pragma unroll
         for (int m = 0; m < 6; m++) {
             const unsigned ep = epair[(size_t)m * nv + jj];
@@ -292,10 +297,10 @@ __global__ __launch_bounds__(256) void k_assemble(const int *__restrict__ list, 
                 if ((int)(ep & 15u) == s) ra = r24[s];
                 if ((int)(ep >> 4) == s) rb = r24[s];
             }
-            coef[(size_t)m * nv + jj] = ep == 0xffu ? T(0) : (T)((ra + rb) / jac);
+            coef[(size_t)m * nv + jj] = ep == 0xffu ? T(0) : (T)((ra + rb) * ijac);
         }
 #pragma unroll
-        for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] / jac);
+        for (int d = 0; d < 3; d++) bv[3 * (size_t)jj + d] = (T)(tb[d] * ijac);
         dsv[jj] = (T)(ds * (double)gscale[vcell[jj]]);
     }
 }
@@ -1395,7 +1400,7 @@ lbmibm_status assemble_fused(lbmibm_ctx *ctx, const PeriodicBox &box, const Tens
     if (n == 0) return LBMIBM_OK;
     LBMIBM_KT(ctx, ctx->stream, "k_assemble",
               hipLaunchKernelGGL(k_assemble<T>, dim3(n), dim3(256), 0, ctx->stream, ctx->asm_list[which],
-                                 ctx->asm_chunks, ctx->asm_elems, (const double *)ctx->xd, ctx->ele, ctx->eslot,
+                                 ctx->asm_chunks, ctx->asm_elems, (const double *)ctx->xd, ctx->eslot,
                                  (const double *)ctx->nref, p.store_macros ? (T *)ctx->tt : (T *)nullptr,
                                  (T *)ctx->rg, p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ctx->nln,
                                  ctx->epair, ctx->vcell, (const T *)ctx->gscale, (T *)ctx->coef, (T *)ctx->bv,
