@@ -240,6 +240,33 @@ def traffic_from_profile(precision):
     return out
 
 
+# kernel-table names -> kernel names in the rocprof summaries (FP32 workload)
+PROF_NAMES = {"k_lbm": "k_lbm<float, float, double, true, false, 1>", "k_spread_box": "k_spread_box<float>",
+              "k_spread_gather": "k_spread_gather<float>", "k_interp_tiles": "k_interp_tiles<float>",
+              "k_bicg_res": "k_bicg_res<11>", "k_assemble": "k_assemble<float>",
+              "k_bicg_onchip class 0": "k_bicg_onchip<float, 2, 512>",
+              "k_bicg_onchip class 1": "k_bicg_onchip<float, 6, 512>", "k_vol_sums": "k_vol_sums",
+              "k_vol_center": "k_vol_center", "k_volume<256>": "k_volume<256>", "k_volume<1024>": "k_volume<1024>",
+              # the tile counts come with the volume correction: the step's binning is the scatter
+              "binning": "k_bin_scatter<float>"}
+
+
+def step_traffic(kernels):
+    """HBM bytes of one step from the rocprof bytes per launch of the timed
+    kernels (launches per step x bytes per launch); None unless every kernel
+    of the table above 1% of the step's kernel time has a measurement."""
+    total = sum(k["launches_per_step"] * k["avg_ms"] for k in kernels) or 1.0
+    out = 0.0
+    for k in kernels:
+        b = k.get("rocprof_bytes")
+        if b is None:
+            if k["launches_per_step"] * k["avg_ms"] > 0.01 * total:
+                return None
+            continue
+        out += k["launches_per_step"] * b
+    return round(out)
+
+
 def kernel_table(kt, n, nv, nt, precision, prof):
     """Per-kernel entries: launches per step and average duration (HIP events
     on the launch's stream, eager steps), algorithmic bytes per launch where
@@ -254,9 +281,7 @@ def kernel_table(kt, n, nv, nt, precision, prof):
         # out (48 B), the velocity field in
         "k_interp_tiles": 72 * nv + 3 * t * n, "k_interp_vert": 72 * nv + 3 * t * n,
     }
-    prof_name = {"k_lbm": "k_lbm<float, float, true, false, 1>", "k_spread_box": "k_spread_box<float>",
-                 "k_spread_gather": "k_spread_gather<float>", "k_interp_tiles": "k_interp_tiles<float>",
-                 "k_bicg_res": "k_bicg_res<11>", "k_elem": "k_elem<float>", "k_vert": "k_vert<float>"}
+    prof_name = PROF_NAMES
     rows = []
     for name, (per_step, ms) in sorted(kt.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
         b = algo.get(name)
@@ -583,14 +608,14 @@ def main():
         "roofline": {"bound": "hbm", "scope": "whole step (D3Q19 + IBM + membrane)",
                      "achieved": round(sbytes / (ms_per_step * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(sbytes / (ms_per_step * 1e-3) / HBM_PEAK, 4),
-                     "traffic": None, "algorithmic_bytes_per_step": sbytes,
+                     "traffic": step_traffic(kernels), "algorithmic_bytes_per_step": sbytes,
                      "bytes_per_node": bpn + (3 * ARITH_BYTES[a.precision] if coupled else 0),
                      "membrane_bytes": (64 * nv + 28 * nt) if coupled else 0},
         "lattice_kernel_roofline": {"kernel": "k_lbm", "achieved": round(n * bpn / (lbm * 1e-3) / 1e9, 1),
                                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                                     "frac": round(n * bpn / (lbm * 1e-3) / HBM_PEAK, 4), "avg_ms": round(lbm, 5),
                                     "bytes_per_node": bpn,
-                                    "traffic": prof.get("k_lbm<float, float, true, false, 1>")},
+                                    "traffic": prof.get(PROF_NAMES["k_lbm"])},
         "dominant_kernel": dominant,
         "kernels": kernels,
         "stages_ms": {k: round(v[0], 5) for k, v in stages.items()},

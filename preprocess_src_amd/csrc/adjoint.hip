@@ -177,7 +177,7 @@ lbmibm_status positions_from_soa(lbmibm_ctx *ctx) {
     hipLaunchKernelGGL(k_soa_to_aos, dim3((ctx->nv + 255) / 256), dim3(256), 0, ctx->stream,
                        (const double *)ctx->xs, (double *)ctx->xd, ctx->nv);
     LBMIBM_LAUNCHED(ctx, "k_soa_to_aos");
-    ctx->bins_valid = false;
+    invalidate_bins(ctx);
     return LBMIBM_OK;
 }
 
@@ -608,7 +608,7 @@ lbmibm_status lbmibm_adjoint_wall_shear(lbmibm_ctx *ctx, int iter0, int nsteps, 
             if (!active) return;
             c->f[0] = f0; c->f[1] = f1; c->U = U; c->F = F; c->cf = cf; c->qv = qv; c->xs = xs;
             c->xs_out = nullptr;
-            c->bins_valid = false;
+            invalidate_bins(c);
             active = false;
         }
         ~Owned() { restore(); }
@@ -627,7 +627,7 @@ lbmibm_status lbmibm_adjoint_wall_shear(lbmibm_ctx *ctx, int iter0, int nsteps, 
             ctx->f[1 - ctx->cur] = k + 1 < len ? popk(k + 1) : owned.f0;  // the last output is not needed
             ctx->xs = xk(k);
             ctx->xs_out = k + 1 < len ? xk(k + 1) : (double *)owned.xs;
-            ctx->bins_valid = false;
+            invalidate_bins(ctx);
             ctx->cf = tp(k, T_CF); ctx->qv = tp(k, T_Q); ctx->F = tp(k, T_F); ctx->U = tp(k, T_U);
             s = set_step(ctx, iter, false);
             if (s == LBMIBM_OK && ctx->nc > 0) s = membrane_forces(ctx);

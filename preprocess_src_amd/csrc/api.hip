@@ -820,7 +820,7 @@ lbmibm_status lbmibm_run(lbmibm_ctx *ctx, int iter0, int nsteps, int use_graph) 
             ctx->graph_force_zero = ctx->force_zero;
             // the block's first spreading must bin: on replay the positions
             // are the previous block's, whatever the host flag says now
-            ctx->bins_valid = false;
+            invalidate_bins(ctx);
             hipGraph_t g = nullptr;
             LBMIBM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             lbmibm_status cs = LBMIBM_OK;
@@ -1266,7 +1266,7 @@ lbmibm_status lbmibm_upload_cells(lbmibm_ctx *ctx, const double *x, const double
         std::copy(y, y + nv, a.begin() + nv);
         std::copy(z, z + nv, a.begin() + 2 * nv);
         LBMIBM_HIP(ctx, hipMemcpy(ctx->xs, a.data(), 3 * nv * sizeof(double), hipMemcpyHostToDevice));
-        ctx->bins_valid = false;
+        invalidate_bins(ctx);
     }
     if (xd) LBMIBM_HIP(ctx, hipMemcpy(ctx->xd, xd, 3 * nv * sizeof(double), hipMemcpyHostToDevice));
     if (s == LBMIBM_OK && fx && fy && fz) {

@@ -39,6 +39,55 @@ constexpr int VOL_CH = 1024;  // elements / vertices per volume-correction chunk
 constexpr int TILE_X = 16, TILE_Y = 8, TILE_Z = 8;
 constexpr int ASM_CH = 256;  // vertices per chunk of the fused membrane assembly (k_assemble)
 
+// Lattice geometry of the vertex binning: unpadded and padded sizes, walls
+// beyond the y range or periodic y, spacing and tiles per axis.
+struct BinGeom {
+    int nx, ny, nz, nxp, nyp, nzp, walls, ntx, nty;
+    double dx;
+};
+
+// First stencil node of one coordinate (ibm.cu:103-121: floor(xtr/dx) - 1 in
+// the padded frame, xtr = X + (n_p - 1) dx / 2), unpadded.
+__device__ __forceinline__ int bin_first(double x, int np, double dx) {
+    const double xtr = x + (double)(np - 1) * dx / 2.0;
+    const double q = dx == 1.0 ? xtr : xtr / dx;
+    return (int)floor(q) - 1 - 2;
+}
+
+// The tile of a vertex's first stencil node: x and z wrapped once, y wrapped
+// (periodic) or clamped (walls); positions far outside the box (or NaN) land
+// in a valid tile, where their stencils meet no node and are dropped.
+__device__ __forceinline__ int home_tile(const BinGeom &g, double x, double y, double z) {
+    int xw = bin_first(x, g.nxp, g.dx), yh = bin_first(y, g.nyp, g.dx), zw = bin_first(z, g.nzp, g.dx);
+    if (xw < 0) xw += g.nx; else if (xw >= g.nx) xw -= g.nx;
+    if (zw < 0) zw += g.nz; else if (zw >= g.nz) zw -= g.nz;
+    if (g.walls) yh = min(max(yh, 0), g.ny - 1);
+    else if (yh < 0) yh += g.ny; else if (yh >= g.ny) yh -= g.ny;
+    xw = min(max(xw, 0), g.nx - 1);
+    zw = min(max(zw, 0), g.nz - 1);
+    yh = min(max(yh, 0), g.ny - 1);
+    return xw / TILE_X + g.ntx * (yh / TILE_Y + g.nty * (zw / TILE_Z));
+}
+
+// Slot for a lane in counters[key], lanes with equal keys aggregated into one
+// atomic per wave.  All lanes of the wave must call it.
+__device__ __forceinline__ int wave_aggregated_add(int *counters, int key, bool active) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(active);
+    int result = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int lkey = __shfl(key, leader);
+        const unsigned long long grp = __ballot(active && key == lkey) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counters[lkey], __popcll(grp));
+        base = __shfl(base, leader);
+        if ((grp >> lane) & 1ull) result = base + __popcll(grp & ((1ull << lane) - 1ull));
+        pending &= ~grp;
+    }
+    return result;
+}
+
 }  // namespace lbmibm
 
 struct lbmibm_ctx {
@@ -169,6 +218,10 @@ struct lbmibm_ctx {
     float *blk_fmax = nullptr;               // [nblk_bin] max |f| per binning block
     unsigned *absmax = nullptr;
     bool bins_valid = false;                 // bins match the current positions
+    // tile counts, offsets, orders and home tiles match the current positions
+    // (counted by the volume correction's centroid pass); the spreading then
+    // only scatters the records
+    bool counts_ready = false;
 
     // VOF indicator (vof_period > 0, allocated on first use): vf [N] FP64 in
     // vof[vof_cur] and its partner for the THINC/WLIC sweeps; front-tracking
@@ -251,6 +304,13 @@ lbmibm_status adhesion_step(lbmibm_ctx *ctx);                // adhesion.hip
 inline const void *spread_force(const lbmibm_ctx *ctx) { return ctx->adh_on ? ctx->adh_fsum : ctx->cf; }
 // both classes on the given stream (not the context's main stream)
 lbmibm_status volume_run_on(lbmibm_ctx *ctx, int mode, hipStream_t stream);  // volume.hip
+lbmibm_status ibm_count_begin(lbmibm_ctx *ctx, hipStream_t stream);  // ibm.hip: zero the tile counts
+lbmibm_status ibm_count_end(lbmibm_ctx *ctx, hipStream_t stream);    // ibm.hip: tile offsets and orders
+// the positions changed: the bins and the tile counts are stale
+inline void invalidate_bins(lbmibm_ctx *ctx) {
+    ctx->bins_valid = false;
+    ctx->counts_ready = false;
+}
 lbmibm_status set_step(lbmibm_ctx *ctx, int iter, bool advance);  // api.hip
 // api.hip helpers shared with adjoint.hip
 lbmibm_status dev_alloc(lbmibm_ctx *ctx, void **p, size_t bytes);

@@ -176,28 +176,9 @@ __device__ __forceinline__ int source_tiles(int t, int o, int n, int tile, bool 
 
 // ---- binning ---------------------------------------------------------------
 
-// Slot for a lane in counters[key], lanes with equal keys aggregated into one
-// atomic per wave.  All lanes of the wave must call it.
-__device__ __forceinline__ int wave_aggregated_add(int *counters, int key, bool active) {
-    const int lane = threadIdx.x & 63;
-    unsigned long long pending = __ballot(active);
-    int result = 0;
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const int lkey = __shfl(key, leader);
-        const unsigned long long grp = __ballot(active && key == lkey) & pending;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&counters[lkey], __popcll(grp));
-        base = __shfl(base, leader);
-        if ((grp >> lane) & 1ull) result = base + __popcll(grp & ((1ull << lane) - 1ull));
-        pending &= ~grp;
-    }
-    return result;
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs, const T *__restrict__ cf, int nv,
-                                                   IbmGeom g, TileGeom tg, int *__restrict__ home,
+                                                   BinGeom bg, int *__restrict__ home,
                                                    int *__restrict__ tile_count, float *__restrict__ blk_fmax) {
     __shared__ float shm[4];
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,20 +186,7 @@ __global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ xs
     int t = 0;
     float fm = 0.0f;
     if (on) {
-        int x1, y1, z1;
-        T wx[4], wy[4], wz[4];
-        stencil3<T>(g, xs[v], xs[nv + v], xs[2 * (size_t)nv + v], false, x1, y1, z1, wx, wy, wz);
-        int xw = x1, zw = z1, yh = y1;
-        wrap_index(xw, g.nx);
-        wrap_index(zw, g.nz);
-        if (g.walls) yh = min(max(yh, 0), g.ny - 1);
-        else wrap_index(yh, g.ny);
-        // invalid positions (NaN or far outside) land in a valid bin; their
-        // stencils meet no tile and are dropped later
-        xw = min(max(xw, 0), g.nx - 1);
-        zw = min(max(zw, 0), g.nz - 1);
-        yh = min(max(yh, 0), g.ny - 1);
-        t = xw / TILE_X + tg.ntx * (yh / TILE_Y + tg.nty * (zw / TILE_Z));
+        t = home_tile(bg, xs[v], xs[nv + v], xs[2 * (size_t)nv + v]);
         home[v] = t;
         fm = fmaxf(fabsf((float)cf[v]), fmaxf(fabsf((float)cf[nv + v]), fabsf((float)cf[2 * (size_t)nv + v])));
         fm *= 1.0001f;  // margin for the float rounding of |f|
@@ -607,28 +575,44 @@ __global__ __launch_bounds__(SPREAD_BOX_WG) void k_spread_box(const SpreadRec<T>
                                                               const int *__restrict__ tile_count,
                                                               const int *__restrict__ tile_off, T *__restrict__ S,
                                                               IbmGeom g, TileGeom tg,
-                                                              const unsigned *__restrict__ absmax,
                                                               const int *__restrict__ order) {
     __shared__ unsigned long long acc[3 * SB_N];
+    __shared__ float wmax[SPREAD_BOX_WG / 64];
     const int tile = order[blockIdx.x];
     const int cnt = tile_count[tile], off = tile_off[tile];
     if (cnt == 0) return;  // whole work group leaves together; the gather skips empty tiles
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
     const int ox = tx * TILE_X, oy = ty * TILE_Y, oz = tz * TILE_Z;
     for (int k = threadIdx.x; k < 3 * SB_N; k += blockDim.x) acc[k] = 0ull;
+    constexpr int NW = SPREAD_BOX_WG / 64;
+    const int R = (cnt + SPREAD_BOX_WG - 1) / SPREAD_BOX_WG;
+    const int col = NW * (threadIdx.x & 63) + (threadIdx.x >> 6);
     // a box node takes at most one term per home vertex, each at most
-    // |f|max / (8 dx^3): the power of two keeps the sums below 2^kBits
-    const double fmax = (double)__uint_as_float(*absmax);
+    // |f|max / (8 dx^3) with |f|max over the tile's own vertices (a maximum,
+    // so independent of the order of the records): the power of two keeps
+    // the sums below 2^kBits
+    float fm = 0.0f;
+    for (int r = 0; r < R; r++) {
+        const int i = col * R + r;
+        if (i >= cnt) break;
+        const SpreadRec<T> sr = rec[off + i];
+        fm = fmaxf(fm, fmaxf(fabsf((float)sr.f[0]), fmaxf(fabsf((float)sr.f[1]), fabsf((float)sr.f[2]))));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) fm = fmaxf(fm, __shfl_xor(fm, o));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = fm;
+    __syncthreads();
+    fm = wmax[0];
+#pragma unroll
+    for (int w = 1; w < NW; w++) fm = fmaxf(fm, wmax[w]);
+    // the float rounding of a double |f| may round down
+    const double fmax = (double)fm * (1.0 + 0x1p-20);
     double scale = 1.0;
     if (fmax > 0.0) {
         const double bound = (double)cnt * fmax / (8.0 * g.dx * g.dx * g.dx);
         scale = ldexp(1.0, FixedPoint<T>::kBits - 1 - ilogb(bound));
     }
     const T w64 = (T)(1.0 / (64.0 * g.dx * g.dx * g.dx));
-    __syncthreads();
-    constexpr int NW = SPREAD_BOX_WG / 64;
-    const int R = (cnt + SPREAD_BOX_WG - 1) / SPREAD_BOX_WG;
-    const int col = NW * (threadIdx.x & 63) + (threadIdx.x >> 6);
     for (int r = 0; r < R; r++) {
         const int i = col * R + r;
         if (i >= cnt) break;
@@ -1374,32 +1358,58 @@ IbmGeom geom_of(const lbmibm_ctx *ctx) {
     return g;
 }
 
+BinGeom bin_geom(const lbmibm_ctx *ctx) {
+    BinGeom b;
+    b.nx = ctx->nx; b.ny = ctx->ny; b.nz = ctx->nz;
+    b.nxp = ctx->nxp; b.nyp = ctx->nyp; b.nzp = ctx->nzp;
+    b.walls = ctx->p.periodic_y == 0;
+    b.ntx = ctx->ntx; b.nty = ctx->nty;
+    b.dx = ctx->p.dx;
+    return b;
+}
+
+// The offsets and orders of the tiles from their counts (k_tile_scan with no
+// force maxima, k_tile_order).
+lbmibm_status tile_offsets(lbmibm_ctx *ctx, hipStream_t stream, int nblk_fmax) {
+    const IbmGeom g = geom_of(ctx);
+    TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->tile_off, ctx->tile_cur,
+                       ctx->ntiles, ctx->blk_fmax, nblk_fmax, ctx->absmax);
+    LBMIBM_LAUNCHED(ctx, "k_tile_scan");
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->ntiles, g, tg,
+                       ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->order_calls);
+    LBMIBM_LAUNCHED(ctx, "k_tile_order");
+    return LBMIBM_OK;
+}
+
+// The scatter of the binning: home list, first stencil nodes and spreading
+// records (fractions, current forces) in bin order.
+template <typename T> lbmibm_status bin_scatter(lbmibm_ctx *ctx, hipStream_t stream) {
+    hipLaunchKernelGGL(k_bin_scatter<T>, dim3(ctx->nblk_bin), dim3(256), 0, stream, (const double *)ctx->xs,
+                       (const T *)spread_force(ctx), ctx->home, ctx->nv, geom_of(ctx), ctx->tile_off, ctx->tile_cur,
+                       ctx->home_list, ctx->node0, (SpreadRec<T> *)ctx->srec);
+    LBMIBM_LAUNCHED(ctx, "k_bin_scatter");
+    return LBMIBM_OK;
+}
+
 // Binning of the vertices by the tile of their first stencil node, with the
 // spreading records of the current nodal forces and max |f|.
 template <typename T> lbmibm_status bin_impl(lbmibm_ctx *ctx) {
     hipStream_t stream = ctx->stream;
-    const IbmGeom g = geom_of(ctx);
-    TileGeom tg{ctx->ntx, ctx->nty, ctx->ntz, ctx->ntiles};
     const int nv = ctx->nv;
     const hipEvent_t kt = kt_begin(ctx, stream);
     hipLaunchKernelGGL(k_zero_int, dim3((ctx->ntiles + 255) / 256), dim3(256), 0, stream, ctx->tile_count,
                        ctx->ntiles);
     LBMIBM_LAUNCHED(ctx, "k_zero_int");
     hipLaunchKernelGGL(k_bin_count<T>, dim3(ctx->nblk_bin), dim3(256), 0, stream, (const double *)ctx->xs,
-                       (const T *)spread_force(ctx), nv, g, tg, ctx->home, ctx->tile_count, ctx->blk_fmax);
+                       (const T *)spread_force(ctx), nv, bin_geom(ctx), ctx->home, ctx->tile_count, ctx->blk_fmax);
     LBMIBM_LAUNCHED(ctx, "k_bin_count");
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->tile_off, ctx->tile_cur,
-                       ctx->ntiles, ctx->blk_fmax, ctx->nblk_bin, ctx->absmax);
-    LBMIBM_LAUNCHED(ctx, "k_tile_scan");
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, ctx->tile_count, ctx->ntiles, g, tg,
-                       ctx->tile_cand, ctx->order_spread, ctx->order_interp, ctx->order_calls);
-    LBMIBM_LAUNCHED(ctx, "k_tile_order");
-    hipLaunchKernelGGL(k_bin_scatter<T>, dim3(ctx->nblk_bin), dim3(256), 0, stream, (const double *)ctx->xs,
-                       (const T *)spread_force(ctx), ctx->home, nv, g, ctx->tile_off, ctx->tile_cur, ctx->home_list,
-                       ctx->node0, (SpreadRec<T> *)ctx->srec);
-    LBMIBM_LAUNCHED(ctx, "k_bin_scatter");
+    lbmibm_status s = tile_offsets(ctx, stream, ctx->nblk_bin);
+    if (s == LBMIBM_OK) s = bin_scatter<T>(ctx, stream);
+    if (s != LBMIBM_OK) return s;
     kt_end(ctx, stream, "binning", kt);
     ctx->bins_valid = true;
+    ctx->counts_ready = false;
     return LBMIBM_OK;
 }
 
@@ -1411,7 +1421,7 @@ template <typename T> lbmibm_status spread_launch(lbmibm_ctx *ctx, int form) {
         LBMIBM_KT(ctx, ctx->stream, "k_spread_box",
                   hipLaunchKernelGGL(k_spread_box<T>, dim3(ctx->ntiles), dim3(SPREAD_BOX_WG), 0, ctx->stream,
                                      (const SpreadRec<T> *)ctx->srec, ctx->node0, ctx->tile_count, ctx->tile_off,
-                                     (T *)ctx->sbox, g, tg, ctx->absmax, ctx->order_interp));
+                                     (T *)ctx->sbox, g, tg, ctx->order_interp));
         LBMIBM_LAUNCHED(ctx, "k_spread_box");
         LBMIBM_KT(ctx, ctx->stream, "k_spread_gather",
                   hipLaunchKernelGGL(k_spread_gather<T>, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0,
@@ -1431,7 +1441,16 @@ template <typename T> lbmibm_status spread_launch(lbmibm_ctx *ctx, int form) {
 
 template <typename T> lbmibm_status spread_impl(lbmibm_ctx *ctx) {
     if (ctx->nv == 0) return lattice_fill(ctx, ctx->F, 3 * (size_t)ctx->N, 0.0, false);
-    if (!ctx->bins_valid) {
+    if (!ctx->bins_valid && ctx->counts_ready && ctx->spread_form == 1) {
+        // counted with the volume correction: only the scatter is left (form 1
+        // scales each tile by its own max |f| and needs no global maximum)
+        const hipEvent_t kt = kt_begin(ctx, ctx->stream);
+        lbmibm_status s = bin_scatter<T>(ctx, ctx->stream);
+        if (s != LBMIBM_OK) return s;
+        kt_end(ctx, ctx->stream, "binning", kt);
+        ctx->bins_valid = true;
+        ctx->counts_ready = false;
+    } else if (!ctx->bins_valid) {
         lbmibm_status s = bin_impl<T>(ctx);
         if (s != LBMIBM_OK) return s;
     } else {  // same positions, new forces: refresh the force part of the records
@@ -1479,7 +1498,7 @@ template <typename T> lbmibm_status interp_impl(lbmibm_ctx *ctx) {
     double *xo = ctx->xs_out ? ctx->xs_out : (double *)ctx->xs;
     lbmibm_status s = interp_launch<T>(ctx, ctx->interp_form, (const double *)ctx->xs, xo,
                                        ctx->p.store_macros ? (T *)ctx->uvw : (T *)nullptr, (double *)ctx->xd);
-    ctx->bins_valid = false;  // the positions moved
+    invalidate_bins(ctx);  // the positions moved
     return s;
 }
 
@@ -1589,6 +1608,15 @@ lbmibm_status ibm_setup(lbmibm_ctx *ctx) {
                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds64));
     return LBMIBM_OK;
 }
+
+lbmibm_status ibm_count_begin(lbmibm_ctx *ctx, hipStream_t stream) {
+    hipLaunchKernelGGL(k_zero_int, dim3((ctx->ntiles + 255) / 256), dim3(256), 0, stream, ctx->tile_count,
+                       ctx->ntiles);
+    LBMIBM_LAUNCHED(ctx, "k_zero_int");
+    return LBMIBM_OK;
+}
+
+lbmibm_status ibm_count_end(lbmibm_ctx *ctx, hipStream_t stream) { return tile_offsets(ctx, stream, 0); }
 
 lbmibm_status ibm_spread(lbmibm_ctx *ctx) {
     lbmibm_status s = ctx->prec == LBMIBM_F64 ? spread_impl<double>(ctx) : spread_impl<float>(ctx);

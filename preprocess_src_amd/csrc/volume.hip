@@ -233,9 +233,12 @@ __global__ __launch_bounds__(NT) void k_volume(const int *__restrict__ cells, in
 // CalcCenter's sums over one chunk of a cell's vertices, of the positions as
 // CapsulePeriodicBefore left them (unwrapped here for the cells that ran no
 // pass), and CapsulePeriodicAfter: each position wrapped back into the box.
+// With `home` the final positions are also counted into the spreading tiles
+// (home tile of each vertex, per-tile counts), as the binning would.
 __global__ __launch_bounds__(256) void k_vol_center(const int4 *__restrict__ chunks, double *__restrict__ xs, int nv,
                                                     const double *__restrict__ gx, const int *__restrict__ need,
-                                                    double4 *__restrict__ part, VolArgs a) {
+                                                    double4 *__restrict__ part, VolArgs a, BinGeom bg,
+                                                    int *__restrict__ home, int *__restrict__ tile_count) {
     __shared__ double sh[128];
     WgReduce red{sh, 0};
     const int4 ch = chunks[blockIdx.x];
@@ -243,15 +246,27 @@ __global__ __launch_bounds__(256) void k_vol_center(const int4 *__restrict__ chu
     const double g[3] = {gx[3 * ch.x], gx[3 * ch.x + 1], gx[3 * ch.x + 2]};
     const double len[3] = {a.lx, a.ly, a.lz};
     double s[3] = {0.0, 0.0, 0.0};
-    for (int j = ch.y + threadIdx.x; j < ch.y + ch.z; j += 256) {
-        for (int d = 0; d < 3; d++) {
-            double *P = xs + d * (size_t)nv + j;
-            const double x0 = *P;
-            const bool per = d != 1 || a.py;
-            const double x = unwrap && per ? unwrap1(x0, g[d], len[d]) : x0;
-            s[d] += x;
-            const double w = per ? rewrap1(x, len[d]) : x;
-            if (w != x0) *P = w;
+    // a uniform trip count: the tile counters aggregate over whole waves
+    for (int base = 0; base < ch.z; base += 256) {
+        const int j = ch.y + base + (int)threadIdx.x;
+        const bool on = base + (int)threadIdx.x < ch.z;
+        double fin[3] = {0.0, 0.0, 0.0};
+        if (on) {
+            for (int d = 0; d < 3; d++) {
+                double *P = xs + d * (size_t)nv + j;
+                const double x0 = *P;
+                const bool per = d != 1 || a.py;
+                const double x = unwrap && per ? unwrap1(x0, g[d], len[d]) : x0;
+                s[d] += x;
+                const double w = per ? rewrap1(x, len[d]) : x;
+                if (w != x0) *P = w;
+                fin[d] = w;
+            }
+        }
+        if (home) {
+            const int t = on ? home_tile(bg, fin[0], fin[1], fin[2]) : 0;
+            if (on) home[j] = t;
+            (void)wave_aggregated_add(tile_count, t, on);
         }
     }
     red.sum<3>(s);
@@ -317,10 +332,26 @@ lbmibm_status run_impl(lbmibm_ctx *ctx, int mode, hipStream_t stream) {
         LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_vjoin, 0));
     }
     if (mode == 1) {
+        // the corrected positions are final until the next interpolation:
+        // count them into the spreading tiles here (spread form 1, whose
+        // scatter then needs no count of its own)
+        const bool count = ctx->spread_form == 1 && ctx->nv > 0 && ctx->tile_count;
+        if (count) {
+            lbmibm_status s = ibm_count_begin(ctx, stream);
+            if (s != LBMIBM_OK) return s;
+        }
+        const BinGeom bg{ctx->nx, ctx->ny, ctx->nz, ctx->nxp, ctx->nyp, ctx->nzp, ctx->p.periodic_y == 0,
+                         ctx->ntx, ctx->nty, ctx->p.dx};
         LBMIBM_KT(ctx, stream, "k_vol_center",
                   hipLaunchKernelGGL(k_vol_center, dim3(ctx->vol_nvc), dim3(256), 0, stream, ctx->vol_vchunks,
-                                     (double *)ctx->xs, ctx->nv, ctx->gx, ctx->vol_need, ctx->vol_vpart, a));
+                                     (double *)ctx->xs, ctx->nv, ctx->gx, ctx->vol_need, ctx->vol_vpart, a, bg,
+                                     count ? ctx->home : nullptr, ctx->tile_count));
         LBMIBM_LAUNCHED(ctx, "k_vol_center");
+        if (count) {
+            lbmibm_status s = ibm_count_end(ctx, stream);
+            if (s != LBMIBM_OK) return s;
+            ctx->counts_ready = true;
+        }
         hipLaunchKernelGGL(k_vol_fin, dim3((ctx->nc + 255) / 256), dim3(256), 0, stream, ctx->vol_crange, ctx->voff,
                            ctx->nc, ctx->vol_vpart, ctx->gx, a);
         LBMIBM_LAUNCHED(ctx, "k_vol_fin");
@@ -334,7 +365,7 @@ lbmibm_status volume_run(lbmibm_ctx *ctx, int mode) { return volume_run_on(ctx, 
 
 lbmibm_status volume_run_on(lbmibm_ctx *ctx, int mode, hipStream_t stream) {
     if (ctx->nc == 0) return LBMIBM_OK;
-    if (mode == 1) ctx->bins_valid = false;  // the correction moves the vertices
+    if (mode == 1) invalidate_bins(ctx);  // the correction moves the vertices
     return run_impl(ctx, mode, stream);
 }
 
