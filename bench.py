@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -267,7 +268,7 @@ def step_traffic(kernels):
     return round(out)
 
 
-def kernel_table(kt, n, nv, nt, precision, prof):
+def kernel_table(kt, n, nv, nt, precision, prof, autotune=None):
     """Per-kernel entries: launches per step and average duration (HIP events
     on the launch's stream, eager steps), algorithmic bytes per launch where
     the kernel has its own share of the step's dataflow, achieved bandwidth
@@ -281,7 +282,12 @@ def kernel_table(kt, n, nv, nt, precision, prof):
         # out (48 B), the velocity field in
         "k_interp_tiles": 72 * nv + 3 * t * n, "k_interp_vert": 72 * nv + 3 * t * n,
     }
-    prof_name = PROF_NAMES
+    prof_name = dict(PROF_NAMES)
+    # the on-chip classes run the shape the autotuner chose for this run
+    for line in autotune or []:
+        m = re.match(r"bicg class ([01]): using (\d+)x(\d+)", line)
+        if m:
+            prof_name[f"k_bicg_onchip class {m.group(1)}"] = f"k_bicg_onchip<float, {m.group(2)}, {m.group(3)}>"
     rows = []
     for name, (per_step, ms) in sorted(kt.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
         b = algo.get(name)
@@ -582,7 +588,7 @@ def main():
         diag["bicg_iters_mean_by_vertices"] = {int(k): round(float(its[nvc == k].mean()), 2) for k in np.unique(nvc)}
         diag["autotune"] = [l for l in sim.autotune_report().splitlines() if l]
     prof = traffic_from_profile(a.precision) if a.workload == "suspension" else {}
-    kernels = kernel_table(kt, n, nv, nt, a.precision, prof)
+    kernels = kernel_table(kt, n, nv, nt, a.precision, prof, diag.get("autotune"))
     dominant = kernels[0]["name"] if kernels else None
     lbm = kt.get("k_lbm", (1, stages["lbm_d3q19"][0]))[1]
     bpn = lattice_bytes_per_node(a.precision, coupled)
