@@ -271,7 +271,7 @@ void free_all(lbmibm_ctx *ctx) {
                     ctx->adj_x, ctx->adj_cf, ctx->grad_H, ctx->grad_gs, ctx->adj_lam, ctx->adj_dsv,
                     ctx->adj_corner, ctx->adj_scratch, ctx->adj_cscale, ctx->adj_gpart, ctx->adj_xmax, ctx->adj_rec, ctx->sbox, ctx->sb_cand,
                     ctx->vof[0], ctx->vof[1], ctx->ft_g, ctx->ft_x, ctx->ft_b, ctx->ft_part, ctx->ft_ctl,
-                    ctx->tube_col, ctx->tube_r, ctx->tube_fc, ctx->adh_knob, ctx->adh_idw, ctx->adh_attach, ctx->adh_pos,
+                    ctx->tube_col, ctx->tube_r, ctx->adh_knob, ctx->adh_idw, ctx->adh_attach, ctx->adh_pos,
                     ctx->adh_active, ctx->adh_prop, ctx->adh_isatt, ctx->adh_claim, ctx->adh_rw, ctx->adh_rij,
                     ctx->adh_fadh, ctx->adh_fsum, ctx->adh_nprop, ctx->adh_klist, ctx->adh_calls, ctx->adh_ring, ctx->mw};
     for (void *p : ptrs)
@@ -709,14 +709,6 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
                     }
                 col[c] = w;
             }
-        std::vector<int> fc;
-        for (size_t c = 0; c < ncol; c++)
-            if (col[c] & 1u) fc.push_back((int)c);
-        ctx->tube_nfc = (int)fc.size();
-        if (!fc.empty()) {
-            CK(dev_alloc(ctx, (void **)&ctx->tube_fc, fc.size() * sizeof(int)));
-            LBMIBM_HIP(ctx, hipMemcpy(ctx->tube_fc, fc.data(), fc.size() * sizeof(int), hipMemcpyHostToDevice));
-        }
         CK(dev_alloc(ctx, &ctx->tube_col, ncol * sizeof(unsigned)));
         LBMIBM_HIP(ctx, hipMemcpy(ctx->tube_col, col.data(), ncol * sizeof(unsigned), hipMemcpyHostToDevice));
         if (!p->tube_staircase) {  // the staircase bounces every cut link half-way

@@ -236,8 +236,6 @@ struct lbmibm_ctx {
     // tube geometry (tube_radius > 0): per-column words (bit 0 inside the
     // disc, bit q the pull source of q is a wall) and wall fractions [col][19]
     void *tube_col = nullptr, *tube_r = nullptr;  // tube_r stays null for the staircase
-    int *tube_fc = nullptr;  // the columns inside the disc (ix + nx iy), row by row
-    int tube_nfc = 0;
     std::vector<double> tube_rh;  // host copy of the fractions (-1 everywhere for the staircase)
     // sliding walls (wall_motion > 0): per wall-row node, 6 rho Cr u_w(iter) of
     // the step that last collided it, [2][nx*nz] (T), bottom row then top row;

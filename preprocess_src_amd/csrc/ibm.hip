@@ -1710,12 +1710,12 @@ template <typename T> lbmibm_status spread_autotune_impl(lbmibm_ctx *ctx) {
     return s;
 }
 
-// a form fixed in the environment: "0" or "1" into *form, -1 when unset
-static lbmibm_status env_form(const char *name, int *form) {
+// a form fixed in the environment: "0" .. last into *form, -1 when unset
+static lbmibm_status env_form(const char *name, int *form, char last = '1') {
     *form = -1;
     const char *v = getenv(name);
     if (!v || !v[0]) return LBMIBM_OK;
-    if ((v[0] != '0' && v[0] != '1') || v[1]) return LBMIBM_ERR_ARG;
+    if (v[0] < '0' || v[0] > last || v[1]) return LBMIBM_ERR_ARG;
     *form = v[0] - '0';
     return LBMIBM_OK;
 }

@@ -49,10 +49,6 @@ struct LbmDims {
     unsigned N;
     int nx, ny, nz;
     int nblocks;
-    // tube: the columns inside the disc; k_lbm then runs one thread per node
-    // of those columns (waves along the disc's chords, none on wall columns)
-    const int *cols = nullptr;
-    int ncols = 0;
 };
 
 // Geometry of k_lbm: periodic box, slab (half-way bounce-back walls at the
@@ -92,13 +88,7 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
     constexpr bool WALLS = GEOM == GEOM_SLAB || GEOM == GEOM_SHEAR, MOVING = GEOM == GEOM_SHEAR;
     const int lb = xcd_block(d.nblocks);
     if (lb >= d.nblocks) return;
-    unsigned i = (unsigned)lb * WG + threadIdx.x;
-    if (GEOM == GEOM_TUBE && d.cols) {
-        const unsigned nc = (unsigned)d.ncols;
-        if (i >= nc * (unsigned)d.nz) return;
-        const unsigned z = i / nc;
-        i = (unsigned)d.cols[i - z * nc] + (unsigned)d.nx * (unsigned)d.ny * z;
-    }
+    const unsigned i = (unsigned)lb * WG + threadIdx.x;
     if (i >= d.N) return;
     const unsigned nx = d.nx, ny = d.ny, nz = d.nz;
     const size_t N = d.N;
@@ -395,11 +385,6 @@ template <typename S, typename T, typename A> lbmibm_status step_impl(lbmibm_ctx
     d.N = (unsigned)ctx->N;
     d.nx = ctx->nx; d.ny = ctx->ny; d.nz = ctx->nz;
     d.nblocks = (int)((ctx->N + WG - 1) / WG);
-    if (p.tube_radius > 0.0 && ctx->tube_fc) {
-        d.cols = ctx->tube_fc;
-        d.ncols = ctx->tube_nfc;
-        d.nblocks = (int)(((long long)ctx->tube_nfc * ctx->nz + WG - 1) / WG);
-    }
     const int grid = xcd_grid(d.nblocks);
     const double om = 1.0 / (3.0 * p.df + 0.5);  // lbm.cu:1131 with vf = 0
     if (p.vof_period > 0 && !ctx->vof[0]) {

@@ -26,6 +26,7 @@
 // images, which is what the reference's BUFF copies hold, and the wall
 // layers hold vf = 0.
 #include <algorithm>
+#include <climits>
 
 #include "common.hpp"
 
@@ -98,13 +99,30 @@ template <typename V> __global__ void k_vfill(V *__restrict__ p, size_t n, V v) 
 
 // front_tracking_source2 (fronttrack.cu:336-441): 64 lanes per element, one
 // per node of the 4x4x4 box starting at floor(.)-1 (the box is 3 wide where
-// the centroid sits on a node: ceil = floor).
-__global__ __launch_bounds__(VWG) void k_ft_spread(const double *__restrict__ xs, int nv, const int *__restrict__ ele,
-                                                   int ne, FtGeom g, unsigned long long *__restrict__ G) {
-    const long long t = (long long)blockIdx.x * VWG + threadIdx.x;
-    const long long e = t >> 6;
-    if (e >= ne) return;
-    const int k = (int)(t & 63), kx = k & 3, ky = (k >> 2) & 3, kz = k >> 4;
+// the centroid sits on a node: ceil = floor).  A work group takes FT_CHUNK
+// consecutive elements (FT_CHUNK / 4 rounds of 4 elements) and sums their
+// terms into an LDS box around the chunk, then adds the box's nonzero nodes
+// to G: one global atomic per touched node and component instead of one per
+// term.  The sums are integers, so G does not depend on that grouping.  A
+// chunk whose stencils span more than the box (elements of two cells, or a
+// cell across a periodic face) adds its terms to G directly.
+constexpr int FT_CHUNK = 64;
+constexpr int FTB = 12, FTB_N = FTB * FTB * FTB;  // LDS box edge (nodes)
+
+struct FtTerm {
+    int ix, iy, iz;  // padded node; ix < 0: no term
+    int fx, fy, fz;  // the first node floor(.)-1 of the element's 4x4x4 box
+    long long v[3];
+};
+
+// The term of lane k of element e (as front_tracking_source2); VALUES false:
+// the node only.
+template <bool VALUES>
+__device__ __forceinline__ FtTerm ft_term(const double *__restrict__ xs, int nv, const int *__restrict__ ele,
+                                          long long e, int k, const FtGeom &g) {
+    FtTerm out;
+    out.ix = -1;
+    const int kx = k & 3, ky = (k >> 2) & 3, kz = k >> 4;
     const double *X = xs, *Y = xs + nv, *Z = xs + 2 * (size_t)nv;
     const int j1 = ele[3 * e], j2 = ele[3 * e + 1], j3 = ele[3 * e + 2];
     const double LX = g.lx, LZ = g.lz, dx = g.dx;
@@ -124,9 +142,14 @@ __global__ __launch_bounds__(VWG) void k_ft_spread(const double *__restrict__ xs
     const double xtr = (tx1 + tx2 + tx3) / 3.0 + (double)(g.nx - 1) * dx / 2.0;
     const double ytr = (ty1 + ty2 + ty3) / 3.0 + (double)(g.ny - 1) * dx / 2.0;
     const double ztr = (tz1 + tz2 + tz3) / 3.0 + (double)(g.nz - 1) * dx / 2.0;
-    const int ix = (int)floor(xtr / dx) - 1 + kx, iy = (int)floor(ytr / dx) - 1 + ky, iz = (int)floor(ztr / dx) - 1 + kz;
-    if (ix > (int)ceil(xtr / dx) + 1 || iy > (int)ceil(ytr / dx) + 1 || iz > (int)ceil(ztr / dx) + 1) return;
-    if (ix < 0 || ix >= g.nx || iy < 0 || iy >= g.ny || iz < 0 || iz >= g.nz) return;
+    out.fx = (int)floor(xtr / dx) - 1;
+    out.fy = (int)floor(ytr / dx) - 1;
+    out.fz = (int)floor(ztr / dx) - 1;
+    const int ix = out.fx + kx, iy = out.fy + ky, iz = out.fz + kz;
+    if (ix > (int)ceil(xtr / dx) + 1 || iy > (int)ceil(ytr / dx) + 1 || iz > (int)ceil(ztr / dx) + 1) return out;
+    if (ix < 0 || ix >= g.nx || iy < 0 || iy >= g.ny || iz < 0 || iz >= g.nz) return out;
+    out.ix = ix; out.iy = iy; out.iz = iz;
+    if (!VALUES) return out;
     double nvx = -(yy1 * zz2 - zz1 * yy2);
     double nvy = -(zz1 * xx2 - xx1 * zz2);
     double nvz = -(xx1 * yy2 - yy1 * xx2);
@@ -138,10 +161,73 @@ __global__ __launch_bounds__(VWG) void k_ft_spread(const double *__restrict__ xs
     const double zcos = 1.0 + cos(M_PI * ((double)iz * dx - ztr) / (2.0 * dx));
     const double dlt = xcos * ycos * zcos / (64.0 * dx * dx * dx);
     const double s = dx * FT_UNIT;
-    const long long i = ix + (long long)g.nx * (iy + (long long)g.ny * iz);
-    atomicAdd(G + i, (unsigned long long)__double2ll_rn(dlt * nvx * ds * s));
-    atomicAdd(G + g.np + i, (unsigned long long)__double2ll_rn(dlt * nvy * ds * s));
-    atomicAdd(G + 2 * g.np + i, (unsigned long long)__double2ll_rn(dlt * nvz * ds * s));
+    out.v[0] = __double2ll_rn(dlt * nvx * ds * s);
+    out.v[1] = __double2ll_rn(dlt * nvy * ds * s);
+    out.v[2] = __double2ll_rn(dlt * nvz * ds * s);
+    return out;
+}
+
+__global__ __launch_bounds__(VWG) void k_ft_spread(const double *__restrict__ xs, int nv, const int *__restrict__ ele,
+                                                   int ne, FtGeom g, unsigned long long *__restrict__ G) {
+    __shared__ unsigned long long box[3 * FTB_N];
+    __shared__ int lo[3], hi[3];
+    const long long e0 = (long long)blockIdx.x * FT_CHUNK;
+    const int k = threadIdx.x & 63, sub = threadIdx.x >> 6;  // 4 elements per round
+    if (threadIdx.x < 3) { lo[threadIdx.x] = INT_MAX; hi[threadIdx.x] = INT_MIN; }
+    for (int i = threadIdx.x; i < 3 * FTB_N; i += VWG) box[i] = 0ull;
+    __syncthreads();
+    // the extent of the chunk's terms: the 4x4x4 box from floor(.)-1 of each
+    // element's centroid, one element per lane of the first wave
+    if (threadIdx.x < 64) {
+        int l[3] = {INT_MAX, INT_MAX, INT_MAX}, h[3] = {INT_MIN, INT_MIN, INT_MIN};
+        const long long e = e0 + threadIdx.x;
+        if (threadIdx.x < FT_CHUNK && e < ne) {
+            const FtTerm t = ft_term<false>(xs, nv, ele, e, 0, g);  // lane 0: the box's first node
+            l[0] = t.fx; l[1] = t.fy; l[2] = t.fz;
+            h[0] = t.fx + 3; h[1] = t.fy + 3; h[2] = t.fz + 3;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                l[d] = min(l[d], __shfl_xor(l[d], o));
+                h[d] = max(h[d], __shfl_xor(h[d], o));
+            }
+        if (threadIdx.x == 0) {
+            // only nodes inside the padded lattice take terms
+            lo[0] = max(l[0], 0); lo[1] = max(l[1], 0); lo[2] = max(l[2], 0);
+            hi[0] = min(h[0], g.nx - 1); hi[1] = min(h[1], g.ny - 1); hi[2] = min(h[2], g.nz - 1);
+        }
+    }
+    __syncthreads();
+    if (lo[0] > hi[0]) return;  // no term in the chunk (uniform)
+    const int ox = lo[0], oy = lo[1], oz = lo[2];
+    const int ex = hi[0] - ox + 1, ey = hi[1] - oy + 1, ez = hi[2] - oz + 1;
+    const bool local = ex <= FTB && ey <= FTB && ez <= FTB;
+    for (int r = 0; r < FT_CHUNK / 4; r++) {
+        const long long e = e0 + 4 * r + sub;
+        if (e >= ne) break;
+        const FtTerm t = ft_term<true>(xs, nv, ele, e, k, g);
+        if (t.ix < 0) continue;
+        if (local) {
+            const int b = (t.ix - ox) + FTB * ((t.iy - oy) + FTB * (t.iz - oz));
+            for (int d = 0; d < 3; d++) atomicAdd(&box[d * FTB_N + b], (unsigned long long)t.v[d]);
+        } else {
+            const long long i = t.ix + (long long)g.nx * (t.iy + (long long)g.ny * t.iz);
+            for (int d = 0; d < 3; d++) atomicAdd(G + d * g.np + i, (unsigned long long)t.v[d]);
+        }
+    }
+    if (!local) return;  // uniform
+    __syncthreads();
+    for (int b = threadIdx.x; b < ex * ey * ez; b += VWG) {
+        const int bx = b % ex, by = (b / ex) % ey, bz = b / (ex * ey);
+        const int lb = bx + FTB * (by + FTB * bz);
+        const long long i = (ox + bx) + (long long)g.nx * ((oy + by) + (long long)g.ny * (oz + bz));
+        for (int d = 0; d < 3; d++) {
+            const unsigned long long v = box[d * FTB_N + lb];
+            if (v) atomicAdd(G + d * g.np + i, v);
+        }
+    }
 }
 
 __device__ __forceinline__ bool ft_comp(const FtGeom &g, int ix, int iy, int iz) {
@@ -548,7 +634,8 @@ lbmibm_status vof_front_tracking(lbmibm_ctx *ctx, int *iterations, int *attempts
                        (size_t)(3 * g.np), 0ull);
     LBMIBM_LAUNCHED(ctx, "k_vfill");
     if (ctx->nt > 0) {
-        hipLaunchKernelGGL(k_ft_spread, dim3(blocks_of(64LL * ctx->nt)), dim3(VWG), 0, st, (const double *)ctx->xs,
+        hipLaunchKernelGGL(k_ft_spread, dim3((unsigned)((ctx->nt + FT_CHUNK - 1) / FT_CHUNK)), dim3(VWG), 0, st,
+                           (const double *)ctx->xs,
                            ctx->nv, ctx->ele, ctx->nt, g, G);
         LBMIBM_LAUNCHED(ctx, "k_ft_spread");
     }
