@@ -224,6 +224,12 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
     dsv[jj] = (T)(ds * (double)gscale[vcell[jj]]);
 }
 
+// Threads of a k_assemble work group.  Its LDS (ASM_CH x 6 corner slots,
+// 48 KB) admits three work groups per CU; with twice as many threads as chunk
+// vertices those hold 24 waves where 256 threads held 12, and a chunk's 550-
+// 630 elements take two passes of the element loop instead of three.
+constexpr int ASM_NT = 512;
+
 // Fused assembly (forward stress): one work group per chunk of up to ASM_CH
 // consecutive vertices of a cell computes, for every element incident to
 // them (a static list built at set-up; elements on a chunk boundary are
@@ -235,7 +241,7 @@ __global__ __launch_bounds__(256) void k_vert(const int *__restrict__ nln, const
 // nodal force to cancellation.  The tension and rg outputs and the tp
 // accumulator are written by the chunk holding the element's first vertex.
 template <typename T>
-__global__ __launch_bounds__(256) void k_assemble(const int *__restrict__ list, const int4 *__restrict__ chunks,
+__global__ __launch_bounds__(ASM_NT) void k_assemble(const int *__restrict__ list, const int4 *__restrict__ chunks,
                                                   const int4 *__restrict__ elems, const double *__restrict__ xd,
                                                   const unsigned char *__restrict__ eslot,
                                                   const double *__restrict__ nref, T *__restrict__ tt,
@@ -1399,7 +1405,7 @@ lbmibm_status assemble_fused(lbmibm_ctx *ctx, const PeriodicBox &box, const Tens
     const int n = ctx->asm_n[which];
     if (n == 0) return LBMIBM_OK;
     LBMIBM_KT(ctx, ctx->stream, "k_assemble",
-              hipLaunchKernelGGL(k_assemble<T>, dim3(n), dim3(256), 0, ctx->stream, ctx->asm_list[which],
+              hipLaunchKernelGGL(k_assemble<T>, dim3(n), dim3(ASM_NT), 0, ctx->stream, ctx->asm_list[which],
                                  ctx->asm_chunks, ctx->asm_elems, (const double *)ctx->xd, ctx->eslot,
                                  (const double *)ctx->nref, p.store_macros ? (T *)ctx->tt : (T *)nullptr,
                                  (T *)ctx->rg, p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ctx->nln,
