@@ -279,6 +279,7 @@ void free_all(lbmibm_ctx *ctx) {
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_afork) (void)hipEventDestroy(ctx->ev_afork);
     if (ctx->ev_vfork) (void)hipEventDestroy(ctx->ev_vfork);
     if (ctx->ev_vjoin) (void)hipEventDestroy(ctx->ev_vjoin);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -395,6 +396,7 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
     if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) return fail(LBMIBM_ERR_HIP);
     if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
     if (hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
+    if (hipEventCreateWithFlags(&ctx->ev_afork, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
     if (hipEventCreateWithFlags(&ctx->ev_vfork, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
     if (hipEventCreateWithFlags(&ctx->ev_vjoin, hipEventDisableTiming) != hipSuccess) return fail(LBMIBM_ERR_HIP);
     if (hipStreamCreateWithFlags(&ctx->vstream, hipStreamNonBlocking) != hipSuccess) return fail(LBMIBM_ERR_HIP);

@@ -104,6 +104,7 @@ struct lbmibm_ctx {
     // and joined back into `stream` with events; captured graphs keep the fork
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;    // membrane solve
+    hipEvent_t ev_afork = nullptr;                       // on-chip assembly done (class 0 on `side`)
     hipEvent_t ev_vfork = nullptr, ev_vjoin = nullptr;  // volume correction classes
     // third stream (default priority): inside lbmibm_run, the volume
     // correction of one step and the binning of its positions overlap the

@@ -1519,11 +1519,27 @@ template <typename T> lbmibm_status forces_impl(lbmibm_ctx *ctx, int warm) {
             if (sb != LBMIBM_OK) return sb;
         }
     }
-    for (int cls = 0; cls < 2; cls++)
+    // With all three classes, class 0 follows the large-cell solve on the
+    // side stream and class 1 runs alone here: the large-cell solve holds its
+    // CUs for about as long as the on-chip assembly and class 1 take on the
+    // others, so the two streams end close together.  Otherwise class 1 runs
+    // first (its work groups take a whole CU's LDS, one cell per CU), and
+    // class 0's short work groups fill the CUs its last round leaves free.
+    // Class 1 is enqueued before class 0's fork, so that in a captured graph
+    // it is the assembly's first dependent and keeps the assembly's queue
+    // (a dependent on another queue starts about 9 us after its parent).
+    const bool c0_side = split && ctx->bicg_n[0] > 0 && ctx->bicg_n[1] > 0;
+    if (c0_side) LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_afork, ctx->stream));
+    for (int cls = 1; cls >= (c0_side ? 1 : 0); cls--)
         if (ctx->bicg_n[cls] > 0) {
             lbmibm_status s = launch_onchip<T>(ctx, cls, ctx->bicg_var[cls], out, ctx->stream, warm);
             if (s != LBMIBM_OK) return s;
         }
+    if (c0_side) {
+        LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_afork, 0));
+        lbmibm_status s = launch_onchip<T>(ctx, 0, ctx->bicg_var[0], out, ctx->side, warm);
+        if (s != LBMIBM_OK) return s;
+    }
     if (ctx->bicg_n[2] > 0) {
         LBMIBM_HIP(ctx, hipEventRecord(ctx->ev_join, ctx->side));
         LBMIBM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
