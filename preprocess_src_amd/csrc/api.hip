@@ -714,8 +714,15 @@ lbmibm_status lbmibm_create(const lbmibm_params *p, const lbmibm_cells *cells, l
         CK(dev_alloc(ctx, &ctx->tube_col, ncol * sizeof(unsigned)));
         LBMIBM_HIP(ctx, hipMemcpy(ctx->tube_col, col.data(), ncol * sizeof(unsigned), hipMemcpyHostToDevice));
         if (!p->tube_staircase) {  // the staircase bounces every cut link half-way
+            // the lattice kernel's link coefficients: 2r (r < 1/2), -1/(2r)
+            // (1/2 <= r <= 1), -2 (no interpolation)
+            std::vector<double> sv(ncol * Q);
+            for (size_t k = 0; k < sv.size(); k++) {
+                const double r = ctx->tube_rh[k];
+                sv[k] = r < 0.0 || r > 1.0 ? -2.0 : (r < 0.5 ? 2.0 * r : -(1.0 / (2.0 * r)));
+            }
             CK(dev_alloc(ctx, &ctx->tube_r, ncol * Q * sizeof(double)));
-            LBMIBM_HIP(ctx, hipMemcpy(ctx->tube_r, ctx->tube_rh.data(), ncol * Q * sizeof(double), hipMemcpyHostToDevice));
+            LBMIBM_HIP(ctx, hipMemcpy(ctx->tube_r, sv.data(), ncol * Q * sizeof(double), hipMemcpyHostToDevice));
         }
     }
     CK(reset_status(ctx));

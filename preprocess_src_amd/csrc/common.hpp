@@ -235,7 +235,9 @@ struct lbmibm_ctx {
     void *ft_ctl = nullptr;
     int ft_nparts = 0;
     // tube geometry (tube_radius > 0): per-column words (bit 0 inside the
-    // disc, bit q the pull source of q is a wall) and wall fractions [col][19]
+    // disc, bit q the pull source of q is a wall) and the interpolation
+    // coefficients of the cut links [col][19] (2r, -1/(2r) or -2 from the
+    // wall fractions r)
     void *tube_col = nullptr, *tube_r = nullptr;  // tube_r stays null for the staircase
     std::vector<double> tube_rh;  // host copy of the fractions (-1 everywhere for the staircase)
     // sliding walls (wall_motion > 0): per wall-row node, 6 rho Cr u_w(iter) of

@@ -58,16 +58,18 @@ struct LbmDims {
 // GEOM_SHEAR is the slab with walls sliding along z (lbm.cu:951-970).
 enum { GEOM_PERIODIC = 0, GEOM_SLAB = 1, GEOM_TUBE = 2, GEOM_SHEAR = 3 };
 
-// BounceBackLinearInterpolation_DEV (lbm_cylinder.cu:61-85): r is the
-// wall's fraction of the link from the node, f1 the half-way bounced value,
-// f2 the value streamed in along the link, f3 the node's own value going
-// the other way; r < 0 marks a link without an interpolation.
-template <typename T> __device__ __forceinline__ T tube_interp(double r, T f1, T f2, T f3) {
-    if (r < 0.0) return f1;
-    const T rr = (T)r;
-    if (r < 0.5) return T(2) * rr * f1 + (T(1) - T(2) * rr) * f2;
-    if (r <= 1.0) return T(1) / (T(2) * rr) * f1 + (T(2) * rr - T(1)) / (T(2) * rr) * f3;
-    return f1;
+// BounceBackLinearInterpolation_DEV (lbm_cylinder.cu:61-85) with f1 the
+// half-way bounced value, f2 the value streamed in along the link and f3 the
+// node's own value going the other way: 2r f1 + (1 - 2r) f2 for r < 1/2,
+// f1/(2r) + (1 - 1/(2r)) f3 for 1/2 <= r <= 1, f1 otherwise, r being the
+// wall's fraction of the link from the node.  The table holds s = 2r for
+// r < 1/2, s = -1/(2r) for 1/2 <= r <= 1 and s = -2 for a link without an
+// interpolation, so that no lane divides: every wave of a tube row holds an
+// edge node and runs this for the union of its lanes' cut links.
+template <typename T> __device__ __forceinline__ T tube_interp(double s, T f1, T f2, T f3) {
+    if (s < -1.0) return f1;
+    const T a = (T)fabs(s);
+    return a * f1 + (T(1) - a) * (s >= 0.0 ? f2 : f3);
 }
 
 // S: population storage, T: the lattice fields (F, H, U, macros), A: the
@@ -147,23 +149,25 @@ __global__ __launch_bounds__(WG) void k_lbm(const S *__restrict__ fin, S *__rest
         // of uncut directions are read as f2, so the in-place update is safe.
         // all loads first (each under its lane mask), then the arithmetic,
         // so that a wave waits once rather than once per cut direction
-        const double *rt = tube_r + (size_t)(ix + nx * iy) * Q;
-        A own[Q];
-        double rq[Q];
+        // (the node's own values are held as stored and converted at use)
+        const unsigned rt = (ix + nx * iy) * (unsigned)Q;  // 32-bit offset into the column table
+        S own[Q];
+        double sq[Q];
 #pragma unroll
         for (int q = 1; q < Q; q++) {
-            own[q] = A(0);
-            rq[q] = -1.0;
+            own[q] = S(0);
+            sq[q] = -2.0;
             if ((cut >> q) & 1u) {
-                own[q] = Pop<S, A>::ld(fin + (size_t)q * N + i, q);
-                rq[q] = rt[q];
+                own[q] = fin[(size_t)q * N + i];
+                sq[q] = tube_r[rt + q];
             }
         }
 #pragma unroll
         for (int q = 1; q < Q; q++) {
             if (!((cut >> q) & 1u)) continue;
-            const A f2 = ((cut >> opp(q)) & 1u) ? own[q] : g[opp(q)];
-            g[q] = tube_interp<A>(rq[q], g[q], f2, own[q]);
+            const A f3 = Pop<S, A>::ld(&own[q], q);
+            const A f2 = ((cut >> opp(q)) & 1u) ? f3 : g[opp(q)];
+            g[q] = tube_interp<A>(sq[q], g[q], f2, f3);
         }
     }
 
