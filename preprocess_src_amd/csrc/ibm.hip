@@ -661,7 +661,13 @@ __global__ __launch_bounds__(256) void k_spread_gather(const T *__restrict__ S, 
                                                        const int *__restrict__ cz, T *__restrict__ F, IbmGeom g,
                                                        TileGeom tg) {
     const size_t N = (size_t)g.nx * g.ny * g.nz;
-    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // each XCD gathers one contiguous eighth of the lattice: the box rows
+    // of neighbouring y rows share cache lines, which are then fetched into
+    // one L2 instead of into the L2s of the XCDs the rows' work groups land on
+    const int nblocks = (int)((N + 255) / 256);
+    const int lb = xcd_block(nblocks);
+    if (lb >= nblocks) return;
+    const size_t n = (size_t)lb * 256 + threadIdx.x;
     if (n >= N) return;
     const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((size_t)g.nx * g.ny));
     // all table entries first, then per (z, y) entry the tile counts and box
@@ -1424,7 +1430,7 @@ template <typename T> lbmibm_status spread_launch(lbmibm_ctx *ctx, int form) {
                                      (T *)ctx->sbox, g, tg, ctx->order_interp));
         LBMIBM_LAUNCHED(ctx, "k_spread_box");
         LBMIBM_KT(ctx, ctx->stream, "k_spread_gather",
-                  hipLaunchKernelGGL(k_spread_gather<T>, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0,
+                  hipLaunchKernelGGL(k_spread_gather<T>, dim3((unsigned)xcd_grid((int)((ctx->N + 255) / 256))), dim3(256), 0,
                                      ctx->stream, (const T *)ctx->sbox, ctx->tile_count, ctx->sb_cand,
                                      ctx->sb_cand + SB_MAXC * ctx->nx, ctx->sb_cand + SB_MAXC * (ctx->nx + ctx->ny),
                                      (T *)ctx->F, g, tg));
