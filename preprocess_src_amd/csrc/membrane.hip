@@ -250,9 +250,14 @@ __global__ __launch_bounds__(ASM_NT) void k_assemble(const int *__restrict__ lis
                                                   const int *__restrict__ vcell, const T *__restrict__ gscale,
                                                   T *__restrict__ coef, T *__restrict__ bv, T *__restrict__ dsv,
                                                   int nv, PeriodicBox box, TensionArgs a,
-                                                  const DevStep *__restrict__ st) {
+                                                  const DevStep *__restrict__ st, int nchunks) {
     __shared__ double4 slot[ASM_CH * 6];  // [local vertex][incident element]
-    const int4 ch = chunks[list[blockIdx.x]];
+    // each XCD takes one contiguous eighth of the chunk list: neighbouring
+    // chunks of a cell share vertices and boundary elements, whose lines
+    // then go into one L2
+    const int lb = xcd_block(nchunks);
+    if (lb >= nchunks) return;
+    const int4 ch = chunks[list[lb]];
     const int cv0 = ch.x, cnv = ch.y, e0 = ch.z, ne = ch.w;
     for (int i = threadIdx.x; i < ne; i += blockDim.x) {
         const int4 er = elems[e0 + i];
@@ -1405,12 +1410,12 @@ lbmibm_status assemble_fused(lbmibm_ctx *ctx, const PeriodicBox &box, const Tens
     const int n = ctx->asm_n[which];
     if (n == 0) return LBMIBM_OK;
     LBMIBM_KT(ctx, ctx->stream, "k_assemble",
-              hipLaunchKernelGGL(k_assemble<T>, dim3(n), dim3(ASM_NT), 0, ctx->stream, ctx->asm_list[which],
+              hipLaunchKernelGGL(k_assemble<T>, dim3(xcd_grid(n)), dim3(ASM_NT), 0, ctx->stream, ctx->asm_list[which],
                                  ctx->asm_chunks, ctx->asm_elems, (const double *)ctx->xd, ctx->eslot,
                                  (const double *)ctx->nref, p.store_macros ? (T *)ctx->tt : (T *)nullptr,
                                  (T *)ctx->rg, p.fileoutput > 0 ? (T *)ctx->tp : (T *)nullptr, ctx->nln,
                                  ctx->epair, ctx->vcell, (const T *)ctx->gscale, (T *)ctx->coef, (T *)ctx->bv,
-                                 (T *)ctx->dsv, ctx->nv, box, ta, ctx->step));
+                                 (T *)ctx->dsv, ctx->nv, box, ta, ctx->step, n));
     LBMIBM_LAUNCHED(ctx, "k_assemble");
     return LBMIBM_OK;
 }
