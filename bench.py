@@ -558,9 +558,15 @@ def main():
         print(f"[bench] set-up {time.time() - t_setup:.1f} s: {desc}, precision {precision}", file=sys.stderr,
               flush=True)
         if spinup:
-            sim.run(0, spinup, graph=graph)
-            sim.synchronize()
-            print(f"[bench] {precision}: spin-up of {spinup} steps done", file=sys.stderr, flush=True)
+            # in blocks of 200 steps (whole graph blocks), each reported, so a
+            # slow run (under a profiler's counters) shows its progress
+            done = 0
+            while done < spinup:
+                k = min(200, spinup - done)
+                sim.run(done, k, graph=graph)
+                sim.synchronize()
+                done += k
+                print(f"[bench] {precision}: spin-up {done}/{spinup} steps", file=sys.stderr, flush=True)
         sim.run(spinup, a.warmup, graph=graph)
         sim.synchronize()
         t0 = time.perf_counter()
